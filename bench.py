@@ -1,0 +1,137 @@
+"""Benchmark: single network RHS nw(du, u, p, t) of the N = 1e6 swing network.
+
+Workload: second-order Kuramoto (swing) vertices on a Barabasi-Albert graph
+N = 1,000,000, m = 2 (E = 1,999,996), Kuramoto edges K ~ U(5,10); FP64.
+A step is one RHS evaluation with u, p, du resident in HBM.
+
+Prints one JSON line: RHS evals/s, the roofline of the fused vertex kernel
+(algorithmic bytes 56 N + 16 E per launch over its HIP-event-timed average
+duration, against 8 TB/s), and the CPU baseline (OpenMP restatement of the
+reference's ThreadedExecution + ThreadedAggregator path, timed on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "swing_ba_1e6_traffic.txt")
+
+
+def _traffic_from_profile():
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    try:
+        for line in open(PROFILE_TRAFFIC):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(nw, p, u0, seconds=10.0):
+    """ThreadedExecution{true} + ThreadedAggregator(+) restated in C/OpenMP, all host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    on = oracle.OracleNet(nw.nv(), nw.g.src, nw.g.dst, nw.kinds, nw.vertex_kind, nw.edge_kind)
+    du = np.empty(nw.dim)
+    on.rhs_threaded(u0, p, nthreads=cores, du=du)     # warm-up
+    n, t0 = 0, time.perf_counter()
+    best = float("inf")
+    while True:
+        t1 = time.perf_counter()
+        on.rhs_threaded(u0, p, nthreads=cores, du=du)
+        best = min(best, time.perf_counter() - t1)
+        n += 1
+        if time.perf_counter() - t0 > seconds and n >= 3:
+            break
+    el = time.perf_counter() - t0
+    return dict(value=n / el, unit="RHS evals/s", cores=cores, kind="port",
+                best_ms=best * 1e3,
+                sample=f"full N=1e6 swing network, {n} calls in {el:.1f} s, {cores} OpenMP threads "
+                       "(C restatement of ThreadedExecution{true}+ThreadedAggregator(+))")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the single-GPU "
+              "kernel is the per-GPU share of a sharded network", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import __graft_entry__ as entry
+
+    ND = entry.load()
+    dev = torch.device("cuda:0")
+    nw, p, u0 = ND.workloads.swing_ba(args.n, 2, seed=1)
+    N, E = nw.nv(), nw.ne()
+    u = torch.from_numpy(u0).to(dev)
+    pd = torch.from_numpy(p).to(dev)
+    du = torch.empty(nw.dim, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    for _ in range(args.warmup):
+        nw(du, u, pd, 0.0)
+    torch.cuda.synchronize()
+
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        nw(du, u, pd, 0.0)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = start.elapsed_time(stop)
+    per_launch_s = ev_ms * 1e-3 / args.steps        # one kernel launch per RHS on this network
+    value = args.steps / wall
+
+    algo_bytes = ND.workloads.algorithmic_bytes_swing(N, E)
+    achieved = algo_bytes / per_launch_s / 1e9
+    traffic = _traffic_from_profile()
+
+    cpu = None if args.no_cpu else cpu_baseline(nw, p, u0, args.cpu_seconds)
+    out = {
+        "metric": "RHS evals/s + % HBM peak at N=1e6 swing",
+        "value": value,
+        "unit": "RHS evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Barabasi-Albert graph and parameters)",
+        "config": {"workload": f"swing (2nd-order Kuramoto) RHS, Barabasi-Albert N={N}, m=2, E={E}, FP64",
+                   "n": N, "e": E, "hub_degree": nw.tuning()["hub_degree"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": algo_bytes, "kernel_us": per_launch_s * 1e6},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,240 @@
+/*
+ * nd_hip.h -- C ABI of libnd_hip.so, the MI355X (gfx950) implementation of
+ * NetworkDynamics.jl's network right-hand side `nw(du, u, p, t)`.
+ *
+ * Plain C: pointers, sizes, status codes.  Device pointers are HIP device
+ * addresses owned by the caller; `stream` is a hipStream_t passed as void*
+ * (NULL = the null stream).  Every entry point returns ND_OK or an ND_ERR_*
+ * code; nd_last_error() returns the message of the last failure on the
+ * calling thread.
+ *
+ * Reference interfaces replaced (NetworkDynamics.jl v1.1.0):
+ *   nd_network_create        Network(g, vertexm, edgem; ...)        src/construction.jl:31-236
+ *                            IndexManager + register_vertices!/edges! src/network_structure.jl:1-55,224-289
+ *   nd_network_export_*      AggregationMap / EagerGBufProvider map   src/aggregators.jl:54-86, src/gbufs.jl:11-22
+ *                            _inv_aggregation_map                     src/aggregators.jl:203-235
+ *   nd_plan_create           Adapt.adapt_structure(to, nw)            ext/NetworkDynamicsCUDAExt.jl:16-45
+ *   nd_rhs                   (nw::Network)(du, u, p, t)               src/coreloop.jl:1-102
+ *   nd_get_buffers           get_buffers(nw, u, p, t)                 src/coreloop.jl:103-109
+ *   nd_gather                gather!(::EagerGBufProvider, gbuf, o)    src/gbufs.jl:25
+ *   nd_aggregate             aggregate!(::Aggregator, aggbuf, data)   src/aggregators.jl:140-151 (additive)
+ */
+#ifndef ND_HIP_H
+#define ND_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+enum {
+    ND_OK = 0,
+    ND_ERR_ARGUMENT = 1,      /* the reference's ArgumentError / DimensionMismatch cases  */
+    ND_ERR_UNSUPPORTED = 2,   /* component model or option without a HIP implementation    */
+    ND_ERR_DEVICE = 3,        /* HIP runtime failure                                        */
+    ND_ERR_ALLOC = 4,         /* host or device allocation failure                          */
+    ND_ERR_NOT_CONVERGED = 5  /* iterative solver stopped before its tolerance              */
+};
+
+/* ---- component description (src/component_functions.jl) ----------------- */
+
+/* Output wrappers of edge models, component_functions.jl:81-203. */
+enum {
+    ND_WRAP_NONE = 0,          /* vertex models; or a two-sided edge g(osrc, odst, ...)     */
+    ND_WRAP_ANTISYMMETRIC = 1, /* y_src = -y_dst                                            */
+    ND_WRAP_SYMMETRIC = 2,     /* y_src =  y_dst                                            */
+    ND_WRAP_DIRECTED = 3,      /* no src output                                             */
+    ND_WRAP_FIDUCIAL = 4       /* independent src/dst outputs (StateMask pair)              */
+};
+
+/* Feed-forward traits, component_functions.jl:6-64. */
+enum {
+    ND_FF_PURESTATEMAP = 0,
+    ND_FF_PUREFEEDFORWARD = 1,
+    ND_FF_FEEDFORWARD = 2,
+    ND_FF_NOFEEDFORWARD = 3
+};
+
+/* Vertex models (the equations a fused kernel implements).  Reference twins:
+ *   DIFFUSION          test/ComponentLibrary.jl:42-46       dv = acc
+ *   KURAMOTO           test/ComponentLibrary.jl:69-72       dtheta = omega + acc,        p = [omega]
+ *   SWING              test/ComponentLibrary.jl:59-67       ddelta = w; dw = 1/M*(Pm - D*w + acc), p = [M, D, Pm]
+ *   SWING_UNIT_DAMPING benchmark/benchmark_models.jl:37-42  dw = P - 1.0*w; dw += acc,   p = [P]
+ *   LOAD               (no reference twin) first-order load  dtheta = 1/tau*(P + acc),    p = [tau, P]
+ *   NOOP               f does nothing (aggregator tests, test/aggregators_test.jl:15-18)  */
+enum {
+    ND_VM_NOOP = 0,
+    ND_VM_DIFFUSION = 1,
+    ND_VM_KURAMOTO = 2,
+    ND_VM_SWING = 3,
+    ND_VM_SWING_UNIT_DAMPING = 4,
+    ND_VM_LOAD = 5
+};
+
+/* Edge models.  Reference twins:
+ *   DIFFUSION            test/diffusion_test.jl:57-60          y = v_s - v_d
+ *   DIFFUSION_WEIGHTED   test/ComponentLibrary.jl:8-13         y = p*(v_s - v_d),  p = [w]
+ *   KURAMOTO             test/ComponentLibrary.jl:51-57        y = K*sin(v_s - v_d), p = [K]
+ *   STATIC_LINE          (no reference twin)                   y = B*sin(v_s - v_d - phi), p = [B, phi]
+ *   DIFFUSION_FIDUCIAL   test/ComponentLibrary.jl:22-28        y_dst = p*(v_s - v_d); y_src = -y_dst
+ *   ODE_SIN_DIFFUSION    test/ComponentLibrary.jl:30-40        de = 1/tau*(sin(+-(v_s - v_d)) - e), p = [tau]
+ *   ODE_SIN_DIFFUSION_FAST benchmark/benchmark_models.jl:11-16 de = 100.0*(sin(+-(v_s - v_d)) - e)
+ *   ODE_LINEAR           test/diffusion_test.jl:102-107        de = +-(v_s - v_d) - e
+ *   NOOP                 g does nothing (test/aggregators_test.jl:20-30)                    */
+enum {
+    ND_EM_NOOP = 0,
+    ND_EM_DIFFUSION = 1,
+    ND_EM_DIFFUSION_WEIGHTED = 2,
+    ND_EM_KURAMOTO = 3,
+    ND_EM_STATIC_LINE = 4,
+    ND_EM_DIFFUSION_FIDUCIAL = 5,
+    ND_EM_ODE_SIN_DIFFUSION = 6,
+    ND_EM_ODE_SIN_DIFFUSION_FAST = 7,
+    ND_EM_ODE_LINEAR = 8
+};
+
+#define ND_MAX_OUT 4
+
+/* One component kind: what the reference's ComponentBatch type parameters
+ * carry (network_structure.jl:176-198): dims, ff trait, wrapper, and the
+ * StateMask indices of a PureStateMap output (1-based, like StateMask).
+ * Vertices use outdim_dst as their outdim and outdim_src = 0. */
+typedef struct {
+    int32_t is_edge;
+    int32_t model;
+    int32_t wrapper;
+    int32_t ff;
+    int32_t dim;
+    int32_t pdim;
+    int32_t outdim_src;
+    int32_t outdim_dst;
+    int32_t mask_src[ND_MAX_OUT];
+    int32_t mask_dst[ND_MAX_OUT];
+} nd_kind;
+
+/* ---- host index layer ------------------------------------------------------ */
+
+typedef struct nd_network nd_network;
+
+/* Network(g, vertexm, edgem) -- construction.jl:31-236.
+ * src/dst: the graph's edge list in Graphs.jl `edges(g)` order, 1-based (Int64),
+ * as IndexManager.edgevec (network_structure.jl:37).
+ * vertex_kind[nv], edge_kind[ne]: index into kinds[] per component; components
+ * with equal kind index are batched together in first-appearance order
+ * (_find_identical_components, construction.jl:238-256; utils.jl:197-217).
+ * Errors (ND_ERR_ARGUMENT): unequal vertex outdims or edge dst outdims
+ * (construction.jl:99-106), out-of-range vertex ids, bad kind indices. */
+int nd_network_create(int64_t nv, int64_t ne, const int64_t* src, const int64_t* dst,
+                      int32_t nkinds, const nd_kind* kinds,
+                      const int32_t* vertex_kind, const int32_t* edge_kind,
+                      nd_network** out);
+void nd_network_destroy(nd_network* nw);
+
+typedef struct {
+    int64_t nv, ne;
+    int64_t dim;            /* lastidx_dynamic */
+    int64_t pdim;           /* lastidx_p       */
+    int64_t lastidx_out;
+    int64_t lastidx_aggr;
+    int64_t lastidx_gbuf;
+    int64_t agg_first;      /* AggregationMap.range (1-based, inclusive); empty: agg_last < agg_first */
+    int64_t agg_last;
+    int32_t vdepth, edepth;
+    int32_t nvbatches, nebatches;
+} nd_network_info;
+
+int nd_network_get_info(const nd_network* nw, nd_network_info* info);
+
+/* ComponentBatch + its BatchStride firsts (network_structure.jl:176-222,
+ * utils.jl:1-42).  `indices` points into the network (valid while it lives).
+ * in_first is the v_aggr first for vertex batches and the e_gbufr first for
+ * edge batches. */
+typedef struct {
+    int32_t kind;
+    int32_t pad_;
+    int64_t count;
+    const int64_t* indices;
+    int64_t state_first, param_first, out_first, in_first;
+} nd_batch;
+
+int nd_network_get_batch(const nd_network* nw, int32_t is_edge, int32_t b, nd_batch* out);
+
+/* Per-component ranges of the IndexManager, 1-based (first, last) pairs:
+ * v_ranges[nv][4][2]: v_data, v_out, v_para, v_aggr;
+ * e_ranges[ne][6][2]: e_data, e_out.src, e_out.dst, e_para, e_gbufr.src, e_gbufr.dst. */
+int nd_network_export_ranges(const nd_network* nw, int64_t* v_ranges, int64_t* e_ranges);
+
+/* AggregationMap.map over AggregationMap.range (agg_last-agg_first+1 entries)
+ * and the EagerGBufProvider map (lastidx_gbuf entries), bit-exact, 1-based. */
+int nd_network_export_maps(const nd_network* nw, int64_t* aggmap, int64_t* gbufmap);
+
+/* The destination-ordered reduction lists (CSR): for aggregation index a
+ * (1-based), the output-buffer indices summed into aggbuf[a], in summation
+ * order (increasing output index, as SequentialAggregator, aggregators.jl:140-151).
+ * rowptr[lastidx_aggr+1] (0-based offsets), cols[rowptr[lastidx_aggr]] (1-based). */
+int nd_network_csr_size(const nd_network* nw, int64_t* nnz);
+int nd_network_export_csr(const nd_network* nw, int64_t* rowptr, int64_t* cols);
+
+/* ---- device plan ------------------------------------------------------------ */
+
+typedef struct nd_plan nd_plan;
+
+/* Uploads the network's index structures to the current HIP device and selects
+ * the fused kernels.  ND_ERR_UNSUPPORTED if a model has no HIP kernel (external
+ * inputs, loopback, feed-forward vertices, output depth > 1 with a functional model). */
+int nd_plan_create(const nd_network* nw, nd_plan** out);
+void nd_plan_destroy(nd_plan* plan);
+
+/* Tuning knobs of a plan (0 = keep current). */
+typedef struct {
+    int32_t hub_degree;      /* segments longer than this go to the wave-per-vertex path */
+    int32_t block_size;      /* threads per block of the vertex kernel                   */
+} nd_plan_tuning;
+int nd_plan_tune(nd_plan* plan, const nd_plan_tuning* t);
+int nd_plan_get_tuning(const nd_plan* plan, nd_plan_tuning* t);
+
+/* nw(du, u, p, t): coreloop.jl:1-102.  du fully overwritten.  Lengths are
+ * checked like coreloop.jl:2-7 (len_du == len_u == dim, len_p == pdim when
+ * pdim > 0) -> ND_ERR_ARGUMENT.  Asynchronous on `stream`. */
+int nd_rhs(nd_plan* plan, double* du, const double* u, const double* p, double t,
+           int64_t len_du, int64_t len_u, int64_t len_p, void* stream);
+
+/* get_buffers(nw, u, p, t; initbufs=true), coreloop.jl:103-109: fills the
+ * output buffer o[lastidx_out] and the aggregation buffer aggbuf[lastidx_aggr]. */
+int nd_get_buffers(nd_plan* plan, double* o, double* aggbuf, const double* u,
+                   const double* p, double t, void* stream);
+
+/* gather!(EagerGBufProvider, gbuf, o): gbuf[k] = o[map[k]], gbufs.jl:25. */
+int nd_gather(nd_plan* plan, double* gbuf, const double* o, void* stream);
+
+/* aggregate!(aggregator, aggbuf, o): aggbuf[a] += sum over its CSR segment of o,
+ * additive like every reference aggregator (aggregators_test.jl:69-79),
+ * deterministic (no atomics). */
+int nd_aggregate(nd_plan* plan, double* aggbuf, const double* o, void* stream);
+
+/* ---- synthetic inputs ----------------------------------------------------------
+ * Seeded graph generators for the workloads.  No counterpart in the reference's
+ * RHS path (its tests use Graphs.jl generators, whose random streams need Julia);
+ * edge lists come back simple and sorted as Graphs.jl `edges(g)` orders a
+ * SimpleGraph: src < dst, lexicographic, 1-based. */
+typedef struct nd_edgelist nd_edgelist;
+int nd_graph_erdos_renyi(int64_t n, int64_t m, uint64_t seed, nd_edgelist** out);
+int nd_graph_barabasi_albert(int64_t n, int64_t m, uint64_t seed, nd_edgelist** out);
+int nd_graph_random_geometric(int64_t n, double radius, uint64_t seed, double* xy, nd_edgelist** out);
+int nd_graph_watts_strogatz(int64_t n, int64_t k, double beta, uint64_t seed, nd_edgelist** out);
+int64_t nd_edgelist_size(const nd_edgelist* el);
+int nd_edgelist_copy(const nd_edgelist* el, int64_t* src, int64_t* dst);
+void nd_edgelist_free(nd_edgelist* el);
+
+/* ---- misc -------------------------------------------------------------------- */
+
+const char* nd_last_error(void);
+int nd_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ND_HIP_H */

@@ -1,0 +1,32 @@
+"""MI355X-native network right-hand side of NetworkDynamics.jl.
+
+    import networkdynamics_jl_amd as ND
+    g  = ND.barabasi_albert(1_000_000, 2, seed=1)
+    nw = ND.Network(g, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge(),
+                    execution=ND.HIPExecution(True), aggregator=ND.SegmentedAggregator)
+    nw(du, u, p, t)          # du, u, p: float64 torch tensors on the HIP device
+
+The directory is named `networkdynamics.jl_amd`; it is importable under the
+module name `networkdynamics_jl_amd` through `load()` in __graft_entry__.py.
+"""
+from . import _abi
+from . import library as Lib
+from . import workloads
+from ._abi import ArgumentError, DeviceError, NotConvergedError, UnsupportedError
+from .components import (AntiSymmetric, DeviceFunction, Directed, EdgeModel, FeedForward, Fiducial,
+                         NoFeedForward, PureFeedForward, PureStateMap, StateMask, Symmetric, VertexModel,
+                         fftype, hasff)
+from .graphs import (SimpleDiGraph, SimpleGraph, barabasi_albert, complete_graph, cycle_graph, erdos_renyi,
+                     path_graph, random_geometric, watts_strogatz)
+from .network import (Aggregator, ExecutionStyle, HIPExecution, Network, SegmentedAggregator, usebuffer)
+
+
+def dim(nw):
+    return nw.dim
+
+
+def pdim(nw):
+    return nw.pdim
+
+
+__all__ = [n for n in dir() if not n.startswith("_")]

@@ -1,0 +1,758 @@
+// Device plan and kernels of the network right-hand side on gfx950.
+//
+// The reference's coreloop (coreloop.jl:1-102) runs ~9-11 phases per call:
+// fills, vertex g, gather (NNlib), edge g, scatter-add aggregation, vertex f.
+// Here one fused kernel per vertex batch does all of it for the component
+// kinds of the library:
+//
+//   * vertex outputs are StateMasks, so the output buffer o is never formed:
+//     neighbour outputs are read straight from u through precomputed u-indices;
+//   * each vertex walks its destination list (the CSR of the aggregation map,
+//     rows in vertex order, entries in increasing output index -- the order of
+//     SequentialAggregator, aggregators.jl:140-151), evaluates the edge model
+//     for its own side on the fly (AntiSymmetric src = -dst exactly) and sums
+//     in that order; no float atomics, no aggbuf, no gbuf;
+//   * the vertex model f is applied in the epilogue and du is written once.
+//
+// Segments longer than hub_degree ("hubs") are reduced by one 64-lane wave:
+// lane-strided partial sums, then a fixed xor-butterfly, so results are
+// reproducible run to run (a deterministic tree order instead of the
+// sequential one).  Edges with their own state (ODE edges) get a separate
+// per-edge kernel for du_e.
+//
+// Generic device primitives mirror the reference's internal dispatch points
+// for any output depth: gather! (gbufs.jl:25), aggregate! (aggregators.jl:140)
+// and get_buffers (coreloop.jl:103-109).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <vector>
+
+#include "nd_internal.h"
+
+namespace {
+
+constexpr int kWave = 64;
+
+struct VDesc {            // one vertex batch as the kernels see it (0-based offsets)
+    int32_t model, dim, pdim, depth;
+    int32_t state0, param0, row0, count;
+    int32_t out0;         // o index of the first vertex output (get_buffers)
+    int32_t nhubs;
+    int32_t mask[ND_MAX_OUT];
+    const int32_t* hubs;  // positions q of hub vertices
+};
+
+struct EDesc {            // one edge batch
+    int32_t model, wrap, ode, dim, pdim;
+    int32_t state0, param0, count;
+    int32_t out0;         // o index of the first edge output
+    int32_t outdim_src, outdim_dst;
+    int32_t mask_src[ND_MAX_OUT];
+    int32_t mask_dst[ND_MAX_OUT];
+    const int2* ends;     // per edge (batch order): u-index of the src / dst vertex output
+};
+
+__host__ __device__ constexpr bool is_ode_model(int em) {
+    return em == ND_EM_ODE_SIN_DIFFUSION || em == ND_EM_ODE_SIN_DIFFUSION_FAST || em == ND_EM_ODE_LINEAR;
+}
+
+// dst-side output of a pure feed-forward edge model g(v_s, v_d, p)
+template <int EM>
+__device__ __forceinline__ double g_dst(double vs, double vd, const double* __restrict__ pe) {
+    if constexpr (EM == ND_EM_KURAMOTO) return pe[0] * sin(vs - vd);             // ComponentLibrary.jl:51-53
+    else if constexpr (EM == ND_EM_STATIC_LINE) return pe[0] * sin(vs - vd - pe[1]);
+    else if constexpr (EM == ND_EM_DIFFUSION) return vs - vd;                     // diffusion_test.jl:57-60
+    else if constexpr (EM == ND_EM_DIFFUSION_WEIGHTED) return pe[0] * (vs - vd);  // ComponentLibrary.jl:8-10
+    else if constexpr (EM == ND_EM_DIFFUSION_FIDUCIAL) return pe[0] * (vs - vd);  // ComponentLibrary.jl:22-25
+    else return __builtin_nan("");
+}
+
+__device__ __forceinline__ double g_dst_rt(int em, double vs, double vd, const double* __restrict__ pe) {
+    switch (em) {
+    case ND_EM_KURAMOTO: return g_dst<ND_EM_KURAMOTO>(vs, vd, pe);
+    case ND_EM_STATIC_LINE: return g_dst<ND_EM_STATIC_LINE>(vs, vd, pe);
+    case ND_EM_DIFFUSION: return g_dst<ND_EM_DIFFUSION>(vs, vd, pe);
+    case ND_EM_DIFFUSION_WEIGHTED: return g_dst<ND_EM_DIFFUSION_WEIGHTED>(vs, vd, pe);
+    case ND_EM_DIFFUSION_FIDUCIAL: return g_dst<ND_EM_DIFFUSION_FIDUCIAL>(vs, vd, pe);
+    default: return __builtin_nan("");
+    }
+}
+
+// Value of one destination-list entry for the vertex whose own output is `own`.
+// s.x: u-index of the other endpoint's output (feed-forward edges) or of the edge
+//      state element this side outputs (ODE edges);
+// s.y: (edge parameter offset << 1) | (1 if this vertex is the edge's src).
+template <int EM, int WRAP>
+__device__ __forceinline__ double slot_value(int2 s, double own, const double* __restrict__ u,
+                                             const double* __restrict__ p) {
+    if constexpr (EM == ND_EM_NOOP) {
+        return __builtin_nan("");          // g writes nothing: o keeps its NaN fill
+    } else if constexpr (is_ode_model(EM)) {
+        return u[s.x];                     // Fiducial(StateMask, StateMask)
+    } else {
+        const double nbr = u[s.x];
+        const double* pe = p + (s.y >> 1);
+        if (s.y & 1) {
+            const double y = g_dst<EM>(own, nbr, pe);
+            return WRAP == ND_WRAP_SYMMETRIC ? y : -y;
+        }
+        return g_dst<EM>(nbr, own, pe);
+    }
+}
+
+__device__ __forceinline__ double slot_value_rt(const EDesc& e, int2 s, double own, const double* __restrict__ u,
+                                                const double* __restrict__ p) {
+    if (e.model == ND_EM_NOOP) return __builtin_nan("");
+    if (e.ode) return u[s.x];
+    const double nbr = u[s.x];
+    const double* pe = p + (s.y >> 1);
+    if (s.y & 1) {
+        const double y = g_dst_rt(e.model, own, nbr, pe);
+        return e.wrap == ND_WRAP_SYMMETRIC ? y : -y;
+    }
+    return g_dst_rt(e.model, nbr, own, pe);
+}
+
+template <int VM>
+struct VModel { static constexpr int dim = -1; };
+template <> struct VModel<ND_VM_DIFFUSION> { static constexpr int dim = 1; };
+template <> struct VModel<ND_VM_KURAMOTO> { static constexpr int dim = 1; };
+template <> struct VModel<ND_VM_SWING> { static constexpr int dim = 2; };
+template <> struct VModel<ND_VM_SWING_UNIT_DAMPING> { static constexpr int dim = 2; };
+template <> struct VModel<ND_VM_LOAD> { static constexpr int dim = 1; };
+
+// vertex f (apply_compf, coreloop.jl:213-216); operation order as the reference writes it
+template <int VM>
+__device__ __forceinline__ void vertex_f(double* __restrict__ dx, const double* __restrict__ x, double acc,
+                                         const double* __restrict__ pv, int dim) {
+    if constexpr (VM == ND_VM_DIFFUSION) {                 // ComponentLibrary.jl:42-45
+        dx[0] = acc;
+    } else if constexpr (VM == ND_VM_KURAMOTO) {           // ComponentLibrary.jl:69-71
+        dx[0] = pv[0] + acc;
+    } else if constexpr (VM == ND_VM_SWING) {              // ComponentLibrary.jl:59-63
+        const double w = x[1];
+        dx[0] = w;
+        dx[1] = 1.0 / pv[0] * (pv[2] - pv[1] * w + acc);
+    } else if constexpr (VM == ND_VM_SWING_UNIT_DAMPING) { // benchmark_models.jl:37-41
+        const double w = x[1];
+        dx[0] = w;
+        double d = pv[0] - 1.0 * w;
+        d += acc;
+        dx[1] = d;
+    } else if constexpr (VM == ND_VM_LOAD) {
+        dx[0] = 1.0 / pv[0] * (pv[1] + acc);
+    } else {                                               // f does nothing: du keeps its zero fill
+        for (int j = 0; j < dim; ++j) dx[j] = 0.0;
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Fused RHS for one vertex batch.  Blocks [0, nlow) take one vertex per thread
+// (segments up to hub_degree, summed in reference order); blocks [nlow, ...) take
+// one hub vertex per wave.
+template <int VM, int EM, int WRAP, bool MULTI>
+__global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __restrict__ rowptr,
+                                                    const int2* __restrict__ slots,
+                                                    const uint8_t* __restrict__ slot_eb,
+                                                    const EDesc* __restrict__ ed,
+                                                    const double* __restrict__ u, const double* __restrict__ p,
+                                                    double* __restrict__ du, int32_t hub_degree, int32_t nlow) {
+    constexpr int DIMC = VModel<VM>::dim;
+    const int dim = DIMC > 0 ? DIMC : vd.dim;
+    if (static_cast<int32_t>(blockIdx.x) < nlow) {
+        const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+        if (q >= vd.count) return;
+        const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * dim;
+        if constexpr (VM == ND_VM_NOOP) {
+            vertex_f<VM>(du + x0, u + x0, 0.0, p, dim);
+        } else {
+            const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
+            if (end - beg > hub_degree) return;
+            const double own = u[x0 + vd.mask[0]];
+            double acc = 0.0;
+            for (int32_t s = beg; s < end; ++s) {
+                const int2 sl = slots[s];
+                if constexpr (MULTI) acc += slot_value_rt(ed[slot_eb[s]], sl, own, u, p);
+                else acc += slot_value<EM, WRAP>(sl, own, u, p);
+            }
+            vertex_f<VM>(du + x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim);
+        }
+    } else {
+        if constexpr (VM != ND_VM_NOOP) {
+            const int32_t w = static_cast<int32_t>((blockIdx.x - nlow) * (blockDim.x / kWave) + threadIdx.x / kWave);
+            const int lane = static_cast<int>(threadIdx.x & (kWave - 1));
+            if (w >= vd.nhubs) return;
+            const int32_t q = vd.hubs[w];
+            const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * dim;
+            const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
+            const double own = u[x0 + vd.mask[0]];
+            double part = 0.0;
+            for (int32_t s = beg + lane; s < end; s += kWave) {
+                const int2 sl = slots[s];
+                if constexpr (MULTI) part += slot_value_rt(ed[slot_eb[s]], sl, own, u, p);
+                else part += slot_value<EM, WRAP>(sl, own, u, p);
+            }
+            const double acc = wave_sum(part);
+            if (lane == 0) vertex_f<VM>(du + x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim);
+        }
+    }
+}
+
+// du of ODE edges (coreloop.jl:76, edge f without feed-forward)
+template <int EM>
+__global__ void __launch_bounds__(256) k_edge_ode(EDesc e, const double* __restrict__ u,
+                                                  const double* __restrict__ p, double* __restrict__ du) {
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= e.count) return;
+    const int2 en = e.ends[q];
+    const double vs = u[en.x], vd = u[en.y];
+    const int64_t x0 = static_cast<int64_t>(e.state0) + static_cast<int64_t>(q) * 2;
+    const double* x = u + x0;
+    double* dx = du + x0;
+    if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION) {            // ComponentLibrary.jl:30-34
+        const double* pe = p + e.param0 + static_cast<int64_t>(q) * e.pdim;
+        dx[0] = 1.0 / pe[0] * (sin(vs - vd) - x[0]);
+        dx[1] = 1.0 / pe[0] * (sin(vd - vs) - x[1]);
+    } else if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION_FAST) { // benchmark_models.jl:11-15
+        dx[0] = 100.0 * (sin(vs - vd) - x[0]);
+        dx[1] = 100.0 * (sin(vd - vs) - x[1]);
+    } else {                                                   // diffusion_test.jl:102-106
+        dx[0] = vs - vd - x[0];
+        dx[1] = vd - vs - x[1];
+    }
+}
+
+__global__ void k_fill(double* __restrict__ x, int64_t n, double v) {
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        x[i] = v;
+}
+
+// o[v_out] = u[StateMask] for one vertex batch (coreloop.jl:39)
+__global__ void k_vertex_out(VDesc vd, const double* __restrict__ u, double* __restrict__ o) {
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= vd.count) return;
+    const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * vd.dim;
+    const int64_t o0 = static_cast<int64_t>(vd.out0) + static_cast<int64_t>(q) * vd.depth;
+    for (int j = 0; j < vd.depth; ++j) o[o0 + j] = u[x0 + vd.mask[j]];
+}
+
+// both outputs of one edge batch (coreloop.jl:41,78 with the wrappers of
+// component_functions.jl:81-203)
+__global__ void k_edge_out(EDesc e, const double* __restrict__ u, const double* __restrict__ p,
+                           double* __restrict__ o) {
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= e.count) return;
+    if (e.model == ND_EM_NOOP) return;
+    const int64_t o0 = static_cast<int64_t>(e.out0) + static_cast<int64_t>(q) * (e.outdim_src + e.outdim_dst);
+    if (e.ode) {
+        const int64_t x0 = static_cast<int64_t>(e.state0) + static_cast<int64_t>(q) * e.dim;
+        for (int j = 0; j < e.outdim_src; ++j) o[o0 + j] = u[x0 + e.mask_src[j]];
+        for (int j = 0; j < e.outdim_dst; ++j) o[o0 + e.outdim_src + j] = u[x0 + e.mask_dst[j]];
+        return;
+    }
+    const int2 en = e.ends[q];
+    const double* pe = p + e.param0 + static_cast<int64_t>(q) * e.pdim;
+    const double y = g_dst_rt(e.model, u[en.x], u[en.y], pe);
+    o[o0 + e.outdim_src] = y;
+    if (e.outdim_src > 0) {
+        if (e.wrap == ND_WRAP_SYMMETRIC) o[o0] = y;
+        else if (e.wrap == ND_WRAP_DIRECTED) o[o0] = 0.0;
+        else o[o0] = -y;
+    }
+}
+
+__global__ void k_gather(const int32_t* __restrict__ gmap, const double* __restrict__ o, double* __restrict__ gbuf,
+                         int64_t n) {
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        gbuf[i] = o[gmap[i]];
+}
+
+// aggbuf[a] += sum of o over row a.  Rows up to hub_degree: one thread, reference
+// order; longer rows: one wave, lane-strided + butterfly.
+__global__ void __launch_bounds__(256) k_aggregate(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+                                                   const double* __restrict__ o, double* __restrict__ aggbuf,
+                                                   int32_t nrows, const int32_t* __restrict__ hub_rows, int32_t nhub,
+                                                   int32_t hub_degree, int32_t nlow) {
+    if (static_cast<int32_t>(blockIdx.x) < nlow) {
+        const int32_t r = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+        if (r >= nrows) return;
+        const int32_t beg = rowptr[r], end = rowptr[r + 1];
+        if (end - beg > hub_degree) return;
+        double acc = aggbuf[r];
+        for (int32_t s = beg; s < end; ++s) acc = acc + o[cols[s]];
+        aggbuf[r] = acc;
+    } else {
+        const int32_t w = static_cast<int32_t>((blockIdx.x - nlow) * (blockDim.x / kWave) + threadIdx.x / kWave);
+        const int lane = static_cast<int>(threadIdx.x & (kWave - 1));
+        if (w >= nhub) return;
+        const int32_t r = hub_rows[w];
+        const int32_t beg = rowptr[r], end = rowptr[r + 1];
+        double part = 0.0;
+        for (int32_t s = beg + lane; s < end; s += kWave) part += o[cols[s]];
+        const double sum = wave_sum(part);
+        if (lane == 0) aggbuf[r] = aggbuf[r] + sum;
+    }
+}
+
+// ---- kernel selection ----------------------------------------------------------
+
+using VertexKernel = void (*)(VDesc, const int32_t*, const int2*, const uint8_t*, const EDesc*, const double*,
+                              const double*, double*, int32_t, int32_t);
+
+template <int VM, int EM, int WRAP>
+VertexKernel pick3(bool multi) {
+    return multi ? k_vertex_rhs<VM, ND_EM_NOOP, ND_WRAP_NONE, true> : k_vertex_rhs<VM, EM, WRAP, false>;
+}
+
+template <int VM>
+VertexKernel pick_edge(int em, int wrap, bool multi) {
+    if (multi) return k_vertex_rhs<VM, ND_EM_NOOP, ND_WRAP_NONE, true>;
+    const bool sym = wrap == ND_WRAP_SYMMETRIC;
+    switch (em) {
+    case ND_EM_KURAMOTO:
+        return sym ? pick3<VM, ND_EM_KURAMOTO, ND_WRAP_SYMMETRIC>(false) : pick3<VM, ND_EM_KURAMOTO, ND_WRAP_ANTISYMMETRIC>(false);
+    case ND_EM_STATIC_LINE:
+        return sym ? pick3<VM, ND_EM_STATIC_LINE, ND_WRAP_SYMMETRIC>(false) : pick3<VM, ND_EM_STATIC_LINE, ND_WRAP_ANTISYMMETRIC>(false);
+    case ND_EM_DIFFUSION:
+        return sym ? pick3<VM, ND_EM_DIFFUSION, ND_WRAP_SYMMETRIC>(false) : pick3<VM, ND_EM_DIFFUSION, ND_WRAP_ANTISYMMETRIC>(false);
+    case ND_EM_DIFFUSION_WEIGHTED:
+        return sym ? pick3<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_SYMMETRIC>(false) : pick3<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_ANTISYMMETRIC>(false);
+    case ND_EM_DIFFUSION_FIDUCIAL:
+        return pick3<VM, ND_EM_DIFFUSION_FIDUCIAL, ND_WRAP_ANTISYMMETRIC>(false);
+    case ND_EM_ODE_SIN_DIFFUSION:
+    case ND_EM_ODE_SIN_DIFFUSION_FAST:
+    case ND_EM_ODE_LINEAR:
+        return pick3<VM, ND_EM_ODE_LINEAR, ND_WRAP_FIDUCIAL>(false);
+    default:
+        return pick3<VM, ND_EM_NOOP, ND_WRAP_NONE>(false);
+    }
+}
+
+VertexKernel pick_vertex_kernel(int vm, int em, int wrap, bool multi) {
+    switch (vm) {
+    case ND_VM_DIFFUSION: return pick_edge<ND_VM_DIFFUSION>(em, wrap, multi);
+    case ND_VM_KURAMOTO: return pick_edge<ND_VM_KURAMOTO>(em, wrap, multi);
+    case ND_VM_SWING: return pick_edge<ND_VM_SWING>(em, wrap, multi);
+    case ND_VM_SWING_UNIT_DAMPING: return pick_edge<ND_VM_SWING_UNIT_DAMPING>(em, wrap, multi);
+    case ND_VM_LOAD: return pick_edge<ND_VM_LOAD>(em, wrap, multi);
+    default: return k_vertex_rhs<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false>;
+    }
+}
+
+using EdgeOdeKernel = void (*)(EDesc, const double*, const double*, double*);
+EdgeOdeKernel pick_edge_ode(int em) {
+    switch (em) {
+    case ND_EM_ODE_SIN_DIFFUSION: return k_edge_ode<ND_EM_ODE_SIN_DIFFUSION>;
+    case ND_EM_ODE_SIN_DIFFUSION_FAST: return k_edge_ode<ND_EM_ODE_SIN_DIFFUSION_FAST>;
+    default: return k_edge_ode<ND_EM_ODE_LINEAR>;
+    }
+}
+
+bool vertex_model_known(int vm) { return vm >= ND_VM_NOOP && vm <= ND_VM_LOAD; }
+int VModelDim(int vm) {
+    switch (vm) {
+    case ND_VM_SWING:
+    case ND_VM_SWING_UNIT_DAMPING: return 2;
+    case ND_VM_DIFFUSION:
+    case ND_VM_KURAMOTO:
+    case ND_VM_LOAD: return 1;
+    default: return -1;
+    }
+}
+bool edge_model_known(int em) { return em >= ND_EM_NOOP && em <= ND_EM_ODE_LINEAR; }
+
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct nd_plan {
+    int device = 0;
+    int64_t dim = 0, pdim = 0, lastidx_out = 0, lastidx_aggr = 0, lastidx_gbuf = 0;
+    int32_t vdepth = 0, edepth = 0;
+    bool multi = false;
+    int32_t hub_degree = 32;
+    int32_t block = 256;
+    std::vector<VDesc> vdesc;
+    std::vector<EDesc> edesc;
+    std::vector<VertexKernel> vkern;
+    std::vector<std::vector<int32_t>> hubs_host;   // per vertex batch: positions sorted
+    std::vector<int32_t> rowptr_host;
+    std::vector<int32_t> agg_hub_rows_host;
+    std::vector<void*> allocations;
+    int32_t* d_rowptr = nullptr;
+    int2* d_slots = nullptr;
+    uint8_t* d_slot_eb = nullptr;
+    EDesc* d_edesc = nullptr;
+    int32_t* d_cols = nullptr;
+    int32_t* d_gmap = nullptr;
+    int32_t* d_hubs_all = nullptr;
+    int32_t* d_agg_hub_rows = nullptr;
+    int32_t n_agg_hub = 0;
+    std::vector<size_t> hub_offset;
+
+    ~nd_plan() {
+        for (void* p : allocations) (void)hipFree(p);
+    }
+};
+
+namespace {
+
+int hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return ND_OK;
+    return nd::set_error(ND_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+template <typename T>
+int upload(nd_plan* pl, const std::vector<T>& host, T** dev) {
+    *dev = nullptr;
+    if (host.empty()) return ND_OK;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, host.size() * sizeof(T));
+    if (e != hipSuccess) return nd::set_error(ND_ERR_ALLOC, "hipMalloc(%zu): %s", host.size() * sizeof(T), hipGetErrorString(e));
+    pl->allocations.push_back(p);
+    e = hipMemcpy(p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_check(e, "hipMemcpy");
+    *dev = static_cast<T*>(p);
+    return ND_OK;
+}
+
+// hub lists follow hub_degree; rebuilt when it changes
+int rebuild_hubs(nd_plan* pl) {
+    const int32_t H = pl->hub_degree;
+    std::vector<int32_t> all;
+    pl->hub_offset.assign(pl->vdesc.size() + 1, 0);
+    for (size_t b = 0; b < pl->vdesc.size(); ++b) {
+        VDesc& vd = pl->vdesc[b];
+        pl->hub_offset[b] = all.size();
+        if (vd.model != ND_VM_NOOP && vd.depth == 1)
+            for (int32_t q = 0; q < vd.count; ++q) {
+                const int32_t r = vd.row0 + q;
+                if (pl->rowptr_host[r + 1] - pl->rowptr_host[r] > H) all.push_back(q);
+            }
+        vd.nhubs = static_cast<int32_t>(all.size() - pl->hub_offset[b]);
+    }
+    pl->hub_offset[pl->vdesc.size()] = all.size();
+    std::vector<int32_t> rows;
+    for (int64_t r = 0; r < pl->lastidx_aggr; ++r)
+        if (pl->rowptr_host[r + 1] - pl->rowptr_host[r] > H) rows.push_back(static_cast<int32_t>(r));
+    pl->n_agg_hub = static_cast<int32_t>(rows.size());
+    // replace previous device lists
+    for (int32_t* old : {pl->d_hubs_all, pl->d_agg_hub_rows}) {
+        if (!old) continue;
+        auto it = std::find(pl->allocations.begin(), pl->allocations.end(), static_cast<void*>(old));
+        if (it != pl->allocations.end()) { (void)hipFree(*it); pl->allocations.erase(it); }
+    }
+    pl->d_hubs_all = nullptr;
+    pl->d_agg_hub_rows = nullptr;
+    int rc = upload(pl, all, &pl->d_hubs_all);
+    if (rc) return rc;
+    rc = upload(pl, rows, &pl->d_agg_hub_rows);
+    if (rc) return rc;
+    for (size_t b = 0; b < pl->vdesc.size(); ++b)
+        pl->vdesc[b].hubs = pl->d_hubs_all ? pl->d_hubs_all + pl->hub_offset[b] : nullptr;
+    return ND_OK;
+}
+
+int check_device(const nd_plan* pl) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_check(e, "hipGetDevice");
+    if (dev != pl->device) return nd::set_error(ND_ERR_ARGUMENT, "plan lives on device %d, current device is %d", pl->device, dev);
+    return ND_OK;
+}
+
+inline unsigned blocks_for(int64_t n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
+
+}  // namespace
+
+extern "C" {
+
+int nd_plan_create(const nd_network* h, nd_plan** out) {
+    nd::clear_error();
+    if (!h || !out) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    const nd::Network& n = h->net;
+    const int64_t lim = std::numeric_limits<int32_t>::max() / 2;
+    if (n.dim > lim || n.pdim > lim || n.lastidx_out > lim || static_cast<int64_t>(n.csr_cols.size()) > lim ||
+        n.lastidx_gbuf > lim)
+        return nd::set_error(ND_ERR_UNSUPPORTED, "network too large for 32-bit device indices");
+
+    bool functional_vertex = false, functional_edge = false;
+    for (const nd::Batch& b : n.vb) {
+        const nd_kind& k = n.kinds[b.kind];
+        if (!vertex_model_known(k.model)) return nd::set_error(ND_ERR_UNSUPPORTED, "vertex model %d has no HIP kernel", k.model);
+        if (k.ff != ND_FF_PURESTATEMAP)
+            return nd::set_error(ND_ERR_UNSUPPORTED, "feed-forward vertex outputs (loopback networks) have no HIP kernel");
+        if (k.model != ND_VM_NOOP) {
+            functional_vertex = true;
+            if (k.dim != VModelDim(k.model)) return nd::set_error(ND_ERR_ARGUMENT, "vertex model %d needs dim %d", k.model, VModelDim(k.model));
+        }
+    }
+    for (const nd::Batch& b : n.eb) {
+        const nd_kind& k = n.kinds[b.kind];
+        if (!edge_model_known(k.model)) return nd::set_error(ND_ERR_UNSUPPORTED, "edge model %d has no HIP kernel", k.model);
+        if (k.model != ND_EM_NOOP) functional_edge = true;
+        if (is_ode_model(k.model)) {
+            if (k.ff != ND_FF_PURESTATEMAP || k.dim != 2 || k.wrapper != ND_WRAP_FIDUCIAL)
+                return nd::set_error(ND_ERR_UNSUPPORTED, "ODE edge model %d needs dim 2 and a Fiducial StateMask output", k.model);
+        } else if (k.model != ND_EM_NOOP) {
+            if (k.ff != ND_FF_PUREFEEDFORWARD)
+                return nd::set_error(ND_ERR_UNSUPPORTED, "edge model %d needs a pure feed-forward output", k.model);
+            if (k.dim != 0) return nd::set_error(ND_ERR_UNSUPPORTED, "edge model %d has no state", k.model);
+        }
+    }
+    if ((functional_vertex || functional_edge) && (n.vdepth != 1 || (n.ne > 0 && n.edepth != 1)))
+        return nd::set_error(ND_ERR_UNSUPPORTED, "models with output depth other than 1 have no HIP kernel (vdepth=%d, edepth=%d)",
+                             n.vdepth, n.edepth);
+
+    nd_plan* pl = new (std::nothrow) nd_plan;
+    if (!pl) return nd::set_error(ND_ERR_ALLOC, "host allocation failed");
+    hipError_t he = hipGetDevice(&pl->device);
+    if (he != hipSuccess) { delete pl; return hip_check(he, "hipGetDevice"); }
+    pl->dim = n.dim; pl->pdim = n.pdim; pl->lastidx_out = n.lastidx_out;
+    pl->lastidx_aggr = n.lastidx_aggr; pl->lastidx_gbuf = n.lastidx_gbuf;
+    pl->vdepth = n.vdepth; pl->edepth = n.edepth;
+    pl->multi = n.eb.size() > 1;
+
+    // edge batch descriptors + per-edge endpoint output u-indices (batch order)
+    std::vector<std::vector<int2>> ends(n.eb.size());
+    for (size_t b = 0; b < n.eb.size(); ++b) {
+        const nd::Batch& B = n.eb[b];
+        const nd_kind& k = n.kinds[B.kind];
+        EDesc e{};
+        e.model = k.model;
+        e.wrap = k.model == ND_EM_DIFFUSION_FIDUCIAL ? ND_WRAP_ANTISYMMETRIC : k.wrapper;
+        e.ode = is_ode_model(k.model) ? 1 : 0;
+        e.dim = k.dim; e.pdim = k.pdim;
+        e.state0 = static_cast<int32_t>(B.state_first - 1);
+        e.param0 = static_cast<int32_t>(B.param_first - 1);
+        e.count = static_cast<int32_t>(B.indices.size());
+        e.out0 = static_cast<int32_t>(B.out_first - 1);
+        e.outdim_src = k.outdim_src; e.outdim_dst = k.outdim_dst;
+        for (int j = 0; j < ND_MAX_OUT; ++j) { e.mask_src[j] = k.mask_src[j] - 1; e.mask_dst[j] = k.mask_dst[j] - 1; }
+        if (n.vdepth == 1) {
+            ends[b].resize(B.indices.size());
+            for (size_t q = 0; q < B.indices.size(); ++q) {
+                const int64_t ei = B.indices[q] - 1;
+                ends[b][q] = make_int2(static_cast<int32_t>(nd::vertex_out_u_index(n, n.src[ei] - 1, 0)),
+                                       static_cast<int32_t>(nd::vertex_out_u_index(n, n.dst[ei] - 1, 0)));
+            }
+        }
+        pl->edesc.push_back(e);
+    }
+
+    // CSR rows (aggregation indices) and the per-entry slots of the fused kernel
+    const size_t nnz = n.csr_cols.size();
+    pl->rowptr_host.resize(static_cast<size_t>(n.lastidx_aggr) + 1);
+    for (size_t r = 0; r < pl->rowptr_host.size(); ++r) pl->rowptr_host[r] = static_cast<int32_t>(n.csr_rowptr[r]);
+    // Without edges (edepth 0) there are no aggregation rows; vertex kernels then read
+    // empty segments from a zero row table spanning the largest vertex batch.
+    bool no_rows = n.edepth == 0;
+    if (no_rows) {
+        size_t mx = 0;
+        for (const nd::Batch& B : n.vb) mx = std::max(mx, B.indices.size());
+        pl->rowptr_host.assign(mx + 1, 0);
+    }
+    std::vector<int32_t> cols(nnz);
+    std::vector<int2> slots(nnz);
+    std::vector<uint8_t> slot_eb(pl->multi ? nnz : 0);
+    if (pl->multi && n.eb.size() > 255) { delete pl; return nd::set_error(ND_ERR_UNSUPPORTED, "more than 255 edge batches"); }
+    {
+        // output index -> (edge batch, position, side): batches hold ascending output blocks
+        std::vector<int64_t> bfirst(n.eb.size());
+        for (size_t b = 0; b < n.eb.size(); ++b) bfirst[b] = n.eb[b].out_first;
+        for (int64_t r = 0; r < n.lastidx_aggr; ++r) {
+            for (int64_t s = n.csr_rowptr[r]; s < n.csr_rowptr[r + 1]; ++s) {
+                const int64_t k = n.csr_cols[s];
+                cols[s] = static_cast<int32_t>(k - 1);
+                const size_t b = static_cast<size_t>(std::upper_bound(bfirst.begin(), bfirst.end(), k) - bfirst.begin()) - 1;
+                const nd::Batch& B = n.eb[b];
+                const nd_kind& kk = n.kinds[B.kind];
+                const int64_t stride = kk.outdim_src + kk.outdim_dst;
+                const int64_t q = (k - B.out_first) / stride, rem = (k - B.out_first) % stride;
+                const bool is_src = rem < kk.outdim_src;
+                const int j = static_cast<int>(is_src ? rem : rem - kk.outdim_src);
+                const int64_t e = B.indices[q] - 1;
+                int2 sl = make_int2(0, 0);
+                if (is_ode_model(kk.model)) {
+                    sl.x = static_cast<int32_t>(n.e_state_first(e) - 1 + (is_src ? kk.mask_src[j] : kk.mask_dst[j]) - 1);
+                } else if (kk.model != ND_EM_NOOP) {
+                    const int64_t other = is_src ? n.dst[e] - 1 : n.src[e] - 1;
+                    sl.x = static_cast<int32_t>(nd::vertex_out_u_index(n, other, 0));
+                    sl.y = static_cast<int32_t>(((n.e_param_first(e) - 1) << 1) | (is_src ? 1 : 0));
+                }
+                slots[s] = sl;
+                if (pl->multi) slot_eb[s] = static_cast<uint8_t>(b);
+            }
+        }
+    }
+    std::vector<int32_t> gmap(static_cast<size_t>(n.lastidx_gbuf));
+    for (int64_t e = 0; e < n.ne; ++e) {
+        const int64_t g = n.e_gbuf_first(e) - 1;
+        const int64_t os = n.v_out_first(n.src[e] - 1) - 1, od = n.v_out_first(n.dst[e] - 1) - 1;
+        for (int j = 0; j < n.vdepth; ++j) {
+            gmap[static_cast<size_t>(g + j)] = static_cast<int32_t>(os + j);
+            gmap[static_cast<size_t>(g + n.vdepth + j)] = static_cast<int32_t>(od + j);
+        }
+    }
+
+    int rc = ND_OK;
+    rc = rc ? rc : upload(pl, pl->rowptr_host, &pl->d_rowptr);
+    rc = rc ? rc : upload(pl, slots, &pl->d_slots);
+    rc = rc ? rc : upload(pl, slot_eb, &pl->d_slot_eb);
+    rc = rc ? rc : upload(pl, cols, &pl->d_cols);
+    rc = rc ? rc : upload(pl, gmap, &pl->d_gmap);
+    for (size_t b = 0; b < n.eb.size() && !rc; ++b) {
+        int2* d = nullptr;
+        rc = upload(pl, ends[b], &d);
+        pl->edesc[b].ends = d;
+    }
+    rc = rc ? rc : upload(pl, pl->edesc, &pl->d_edesc);
+    if (rc) { delete pl; return rc; }
+
+    // vertex batch descriptors + kernels
+    const int em0 = n.eb.empty() ? ND_EM_NOOP : n.kinds[n.eb[0].kind].model;
+    const int wrap0 = n.eb.empty() ? ND_WRAP_NONE : pl->edesc[0].wrap;
+    for (const nd::Batch& B : n.vb) {
+        const nd_kind& k = n.kinds[B.kind];
+        VDesc vd{};
+        vd.model = k.model; vd.dim = k.dim; vd.pdim = k.pdim; vd.depth = k.outdim_dst;
+        vd.state0 = static_cast<int32_t>(B.state_first - 1);
+        vd.param0 = static_cast<int32_t>(B.param_first - 1);
+        vd.row0 = no_rows ? 0 : static_cast<int32_t>(B.in_first - 1);
+        vd.count = static_cast<int32_t>(B.indices.size());
+        vd.out0 = static_cast<int32_t>(B.out_first - 1);
+        for (int j = 0; j < ND_MAX_OUT; ++j) vd.mask[j] = k.mask_dst[j] - 1;
+        pl->vdesc.push_back(vd);
+        pl->vkern.push_back(pick_vertex_kernel(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi));
+    }
+    rc = rebuild_hubs(pl);
+    if (rc) { delete pl; return rc; }
+    *out = pl;
+    return ND_OK;
+}
+
+void nd_plan_destroy(nd_plan* plan) { delete plan; }
+
+int nd_plan_tune(nd_plan* pl, const nd_plan_tuning* t) {
+    if (!pl || !t) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
+    if (t->block_size) {
+        if (t->block_size < kWave || t->block_size > 256 || t->block_size % kWave)
+            return nd::set_error(ND_ERR_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
+        pl->block = t->block_size;
+    }
+    if (t->hub_degree) {
+        if (t->hub_degree < 1) return nd::set_error(ND_ERR_ARGUMENT, "hub_degree must be positive");
+        pl->hub_degree = t->hub_degree;
+        return rebuild_hubs(pl);
+    }
+    return ND_OK;
+}
+
+int nd_plan_get_tuning(const nd_plan* pl, nd_plan_tuning* t) {
+    if (!pl || !t) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
+    t->hub_degree = pl->hub_degree;
+    t->block_size = pl->block;
+    return ND_OK;
+}
+
+int nd_rhs(nd_plan* pl, double* du, const double* u, const double* p, double t, int64_t len_du, int64_t len_u,
+           int64_t len_p, void* stream) {
+    (void)t;
+    if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
+    if (!(len_du == len_u && len_u == pl->dim))   // coreloop.jl:2-4
+        return nd::set_error(ND_ERR_ARGUMENT, "du or u does not have expected size %lld", (long long)pl->dim);
+    if (pl->pdim > 0 && len_p != pl->pdim)        // coreloop.jl:5-7
+        return nd::set_error(ND_ERR_ARGUMENT, "p does not has expecte size %lld", (long long)pl->pdim);
+    if (pl->dim > 0 && (!du || !u)) return nd::set_error(ND_ERR_ARGUMENT, "NULL state pointer");
+    if (pl->pdim > 0 && !p) return nd::set_error(ND_ERR_ARGUMENT, "NULL parameter pointer");
+    int rc = check_device(pl);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int B = pl->block;
+    for (size_t b = 0; b < pl->vdesc.size(); ++b) {
+        const VDesc& vd = pl->vdesc[b];
+        if (vd.count == 0) continue;
+        const int32_t nlow = static_cast<int32_t>(blocks_for(vd.count, B));
+        const int32_t nhub = static_cast<int32_t>(blocks_for(static_cast<int64_t>(vd.nhubs) * kWave, B));
+        hipLaunchKernelGGL(pl->vkern[b], dim3(static_cast<unsigned>(nlow + nhub)), dim3(static_cast<unsigned>(B)), 0, s,
+                           vd, pl->d_rowptr, pl->d_slots, pl->d_slot_eb, pl->d_edesc, u, p, du, pl->hub_degree, nlow);
+    }
+    for (const EDesc& e : pl->edesc) {
+        if (!e.ode || e.count == 0) continue;
+        hipLaunchKernelGGL(pick_edge_ode(e.model), dim3(blocks_for(e.count, B)), dim3(static_cast<unsigned>(B)), 0, s,
+                           e, u, p, du);
+    }
+    return hip_check(hipGetLastError(), "nd_rhs launch");
+}
+
+int nd_gather(nd_plan* pl, double* gbuf, const double* o, void* stream) {
+    if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
+    if (pl->lastidx_gbuf == 0) return ND_OK;
+    if (!gbuf || !o) return nd::set_error(ND_ERR_ARGUMENT, "NULL buffer");
+    int rc = check_device(pl);
+    if (rc) return rc;
+    const unsigned nb = std::min<unsigned>(blocks_for(pl->lastidx_gbuf, 256), 65535u);
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, static_cast<hipStream_t>(stream), pl->d_gmap, o, gbuf,
+                       pl->lastidx_gbuf);
+    return hip_check(hipGetLastError(), "nd_gather launch");
+}
+
+int nd_aggregate(nd_plan* pl, double* aggbuf, const double* o, void* stream) {
+    if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
+    if (pl->lastidx_aggr == 0) return ND_OK;
+    if (!aggbuf || !o) return nd::set_error(ND_ERR_ARGUMENT, "NULL buffer");
+    int rc = check_device(pl);
+    if (rc) return rc;
+    const int32_t nrows = static_cast<int32_t>(pl->lastidx_aggr);
+    const int32_t nlow = static_cast<int32_t>(blocks_for(nrows, 256));
+    const int32_t nhub = static_cast<int32_t>(blocks_for(static_cast<int64_t>(pl->n_agg_hub) * kWave, 256));
+    hipLaunchKernelGGL(k_aggregate, dim3(static_cast<unsigned>(nlow + nhub)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), pl->d_rowptr, pl->d_cols, o, aggbuf, nrows, pl->d_agg_hub_rows,
+                       pl->n_agg_hub, pl->hub_degree, nlow);
+    return hip_check(hipGetLastError(), "nd_aggregate launch");
+}
+
+int nd_get_buffers(nd_plan* pl, double* o, double* aggbuf, const double* u, const double* p, double t,
+                   void* stream) {
+    (void)t;
+    if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
+    if (!o || (pl->lastidx_aggr > 0 && !aggbuf) || (pl->dim > 0 && !u))
+        return nd::set_error(ND_ERR_ARGUMENT, "NULL buffer");
+    int rc = check_device(pl);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const unsigned nbo = std::max(1u, std::min<unsigned>(blocks_for(pl->lastidx_out, 256), 65535u));
+    hipLaunchKernelGGL(k_fill, dim3(nbo), dim3(256), 0, s, o, pl->lastidx_out, std::numeric_limits<double>::quiet_NaN());
+    if (pl->lastidx_aggr > 0) {
+        const unsigned nba = std::min<unsigned>(blocks_for(pl->lastidx_aggr, 256), 65535u);
+        hipLaunchKernelGGL(k_fill, dim3(nba), dim3(256), 0, s, aggbuf, pl->lastidx_aggr, 0.0);
+    }
+    for (const VDesc& vd : pl->vdesc)
+        if (vd.count) hipLaunchKernelGGL(k_vertex_out, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, u, o);
+    for (const EDesc& e : pl->edesc)
+        if (e.count) hipLaunchKernelGGL(k_edge_out, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, u, p, o);
+    rc = hip_check(hipGetLastError(), "nd_get_buffers launch");
+    if (rc) return rc;
+    return nd_aggregate(pl, aggbuf, o, stream);
+}
+
+int nd_device_synchronize(void) { return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+}  // extern "C"

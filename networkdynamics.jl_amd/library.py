@@ -1,0 +1,100 @@
+"""Component library: the reference's test/ComponentLibrary.jl and
+benchmark/benchmark_models.jl constructors, plus two build-defined kinds
+(first-order load vertex, static line with phase shift).
+
+Every model here is one of the equations the fused HIP kernels implement.
+"""
+from . import components as c
+
+
+# test/ComponentLibrary.jl:11-13
+def diffusion_edge():
+    return c.EdgeModel(g=c.AntiSymmetric(c.diffusionedge_weighted), outdim=1, psym=[("p", 1.0)],
+                       name="diff_edge")
+
+
+# test/ComponentLibrary.jl:26-28
+def diffusion_edge_fid():
+    return c.EdgeModel(g=c.diffusionedge_fid, outdim=1, psym=[("p", 1.0)], name="diff_edge_fid")
+
+
+# test/ComponentLibrary.jl:35-40
+def diffusion_odeedge():
+    return c.EdgeModel(f=c.diffusion_dedge, sym=["e_dst", "e_src"], psym=[("tau", 100.0)],
+                       g=c.Fiducial(dst=1, src=2), name="diff_edge_ode")
+
+
+# test/ComponentLibrary.jl:46
+def diffusion_vertex():
+    return c.VertexModel(f=c.diffusionvertex, dim=1, g=1, name="diffusion_vertex")
+
+
+# test/ComponentLibrary.jl:54-57
+def kuramoto_edge(name="kuramoto_edge"):
+    return c.EdgeModel(g=c.AntiSymmetric(c.kuramoto_edge_g), outsym=["P"], psym=["K"], name=name)
+
+
+# test/ComponentLibrary.jl:64-67
+def kuramoto_second(name="kuramoto_second"):
+    return c.VertexModel(f=c.kuramoto_inertia, sym=[("delta", 0.0), ("omega", 0.0)],
+                         psym=[("M", 1.0), ("D", 0.1), ("Pm", 1.0)], g=c.StateMask(1), name=name)
+
+
+# test/ComponentLibrary.jl:72
+def kuramoto_first():
+    return c.VertexModel(f=c.kuramoto_vertex, sym=["theta"], psym=["omega"], g=1, name="kuramoto_first")
+
+
+# build-defined: tau * dtheta = P + sum of edge inputs; p = [tau, P]
+def load_vertex(name="load"):
+    return c.VertexModel(f=c.load_vertex_f, sym=["theta"], psym=["tau", "P"], g=1, name=name)
+
+
+# build-defined: y_dst = B * sin(theta_src - theta_dst - phi), AntiSymmetric; p = [B, phi]
+def static_line_edge(name="static_line"):
+    return c.EdgeModel(g=c.AntiSymmetric(c.static_line_g), outsym=["P"], psym=["B", ("phi", 0.0)], name=name)
+
+
+class bench:
+    """benchmark/benchmark_models.jl:5-42."""
+
+    @staticmethod
+    def diffusion_edge():            # :9
+        return c.EdgeModel(g=c.AntiSymmetric(c.diffusionedge), outdim=1, pdim=0)
+
+    @staticmethod
+    def diffusion_dedge():           # :16
+        return c.EdgeModel(f=c.diffusion_dedge_fast, dim=2, pdim=0, g=c.Fiducial(dst=1, src=2))
+
+    @staticmethod
+    def diffusion_vertex():          # :22
+        return c.VertexModel(f=c.diffusionvertex, dim=1, pdim=0, g=c.StateMask(1))
+
+    @staticmethod
+    def static_kuramoto_edge():      # :30
+        return c.EdgeModel(g=c.AntiSymmetric(c.kuramoto_edge_g), outdim=1, pdim=1)
+
+    @staticmethod
+    def kuramoto_vertex_1d():        # :35
+        return c.VertexModel(f=c.kuramoto_vertex, pdim=1, sym=["theta"], g=c.StateMask(1))
+
+    @staticmethod
+    def kuramoto_vertex_2d():        # :42
+        return c.VertexModel(f=c.kuramoto_inertia_unit_damping, dim=2, pdim=1, sym=["theta", "omega"],
+                             g=c.StateMask(1))
+
+
+class diffusion_test:
+    """The models defined inline in test/diffusion_test.jl."""
+
+    @staticmethod
+    def odevertex():                 # :66
+        return c.VertexModel(f=c.diffusionvertex, dim=1, g=1)
+
+    @staticmethod
+    def staticedge():                # :67
+        return c.EdgeModel(g=c.AntiSymmetric(c.diffusionedge), outdim=1)
+
+    @staticmethod
+    def odeedge():                   # :108
+        return c.EdgeModel(f=c.real_ode_edge, dim=2, g=c.Fiducial(2, 1), mass_matrix=0)

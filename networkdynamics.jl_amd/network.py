@@ -1,0 +1,288 @@
+"""Network: the reference's `Network(g, vertexm, edgem; execution, aggregator)`
+constructor and its `nw(du, u, p, t)` call, executed by libnd_hip.so.
+
+Reference: src/construction.jl:31-236 (constructor), src/coreloop.jl:1-109
+(call, get_buffers), src/executionstyles.jl (execution styles),
+src/aggregators.jl (aggregators).  Device arrays are torch CUDA (HIP) tensors;
+work is queued on torch's current stream.
+"""
+import ctypes as C
+import operator
+
+import numpy as np
+
+from . import _abi as A
+from . import components as comp
+from . import graphs
+
+
+class ExecutionStyle:
+    """ExecutionStyle{buffered} (executionstyles.jl:13)."""
+
+    def __init__(self, buffered=True):
+        self.buffered = bool(buffered)
+
+    def __repr__(self):
+        return f"{type(self).__name__}{{{str(self.buffered).lower()}}}()"
+
+
+class HIPExecution(ExecutionStyle):
+    """Fused HIP kernels on the current device.  `buffered` is kept for API
+    parity with the reference's styles; both values run the same fused kernel
+    (the gather is fused into the edge evaluation)."""
+
+
+def usebuffer(ex):
+    return ex.buffered
+
+
+class Aggregator:
+    def __init__(self, f=operator.add):
+        if f not in (operator.add, "+", sum):
+            raise A.ArgumentError(f"{type(self).__name__} only supports + as reducer")
+        self.f = operator.add
+
+
+class SegmentedAggregator(Aggregator):
+    """Deterministic destination-segmented reduction (no float atomics).  Segments
+    up to `hub_degree` entries are summed in SequentialAggregator's order
+    (aggregators.jl:140-151); longer ones by a fixed wave-wide tree."""
+
+
+def _as_list(models, n, name):
+    if isinstance(models, comp.ComponentModel):
+        return [models] * n, True
+    models = list(models)
+    if len(models) != n:
+        raise A.ArgumentError(f"expected {n} {name} models, got {len(models)}")
+    return models, False
+
+
+def _tensor_ptr(x, n, what, torch):
+    if not isinstance(x, torch.Tensor):
+        raise A.ArgumentError(f"{what} must be a torch tensor on the HIP device")
+    if x.dtype != torch.float64:
+        # network_structure.jl:146-151: caches are Float64
+        raise A.ArgumentError(f"Network caches are initialized with Float64 but is used for {x.dtype} data!")
+    if x.device.type != "cuda":
+        raise A.ArgumentError(f"{what} must live on the HIP device, got {x.device}")
+    if x.dim() != 1 or not x.is_contiguous():
+        raise A.ArgumentError(f"{what} must be a contiguous vector")
+    return x.data_ptr(), x.numel()
+
+
+class Network:
+    """Network(g, vertexm, edgem; execution=HIPExecution(True), aggregator=SegmentedAggregator(+))."""
+
+    def __init__(self, g, vertexm, edgem, execution=None, aggregator=None, check_graphelement=True,
+                 dealias=False, verbose=False):
+        if execution is None:
+            execution = HIPExecution(True)
+        if aggregator is None:
+            aggregator = SegmentedAggregator(operator.add)
+        if isinstance(aggregator, type):
+            aggregator = aggregator(operator.add)
+        if not isinstance(execution, HIPExecution):
+            raise A.ArgumentError(f"Execution type {execution} not supported (choose HIPExecution)")
+        if not isinstance(aggregator, SegmentedAggregator):
+            raise A.ArgumentError(f"Aggregator {aggregator} not supported (choose SegmentedAggregator)")
+        if not isinstance(g, graphs.SimpleGraph):
+            raise A.ArgumentError("g must be a SimpleGraph / SimpleDiGraph")
+        self.g = g
+        self.execution = execution
+        self.aggregator = aggregator
+        nv, ne = g.nv(), g.ne()
+        vm, _ = _as_list(vertexm, nv, "vertex")
+        em, _ = _as_list(edgem, ne, "edge")
+        for m in vm:
+            if not isinstance(m, comp.VertexModel):
+                raise A.ArgumentError(f"Expected VertexModels, got {type(m)}")
+        for m in em:
+            if not isinstance(m, comp.EdgeModel):
+                raise A.ArgumentError(f"Expected EdgeModels, got {type(m)}")
+        if dealias:
+            vm = [m.copy() for m in vm]
+            em = [m.copy() for m in em]
+        self.vertexm, self.edgem = vm, em
+        # kind table: one entry per distinct component hash, in first-appearance order
+        kinds, key_to_kind = [], {}
+        vkind = np.empty(nv, np.int32)
+        ekind = np.empty(max(ne, 1), np.int32)
+        cache = {}
+        for arr, models in ((vkind, vm), (ekind, em)):
+            for i, m in enumerate(models):
+                k = cache.get(id(m))
+                if k is None:
+                    h = m.component_hash()
+                    k = key_to_kind.get(h)
+                    if k is None:
+                        k = len(kinds)
+                        key_to_kind[h] = k
+                        kinds.append(m.kind())
+                    cache[id(m)] = k
+                arr[i] = k
+        self.kinds = kinds
+        self.vertex_kind, self.edge_kind = vkind, ekind[:ne]
+        karr = (A.NDKind * len(kinds))()
+        for a, k in zip(karr, kinds):
+            for f in ("is_edge", "model", "wrapper", "ff", "dim", "pdim", "outdim_src", "outdim_dst"):
+                setattr(a, f, int(k[f]))
+            for j, x in enumerate(k["mask_src"]):
+                a.mask_src[j] = x
+            for j, x in enumerate(k["mask_dst"]):
+                a.mask_dst[j] = x
+        L = A.lib()
+        h = C.c_void_p()
+        A.check(L.nd_network_create(nv, ne, g.src.ctypes.data, g.dst.ctypes.data, len(kinds),
+                                    C.cast(karr, C.c_void_p), vkind.ctypes.data, ekind.ctypes.data, C.byref(h)))
+        self._net = h
+        info = A.NDNetworkInfo()
+        A.check(L.nd_network_get_info(h, C.byref(info)))
+        self.info = info
+        self._plan = None
+        if verbose:
+            print(f"Create dynamic network with {nv} vertices and {ne} edges:")
+            for b in self.batches(False):
+                print(f" - VertexBatch: dim={kinds[b['kind']]['dim']}, pdim={kinds[b['kind']]['pdim']}, length={b['count']}")
+            for b in self.batches(True):
+                print(f" - EdgeBatch: dim={kinds[b['kind']]['dim']}, pdim={kinds[b['kind']]['pdim']}, length={b['count']}")
+
+    def __del__(self):
+        L = A._lib
+        if L is None:
+            return
+        if getattr(self, "_plan", None):
+            L.nd_plan_destroy(self._plan)
+            self._plan = None
+        if getattr(self, "_net", None):
+            L.nd_network_destroy(self._net)
+            self._net = None
+
+    # ---- sizes (network_structure.jl:119-135) --------------------------------------------
+    @property
+    def dim(self):
+        return int(self.info.dim)
+
+    @property
+    def pdim(self):
+        return int(self.info.pdim)
+
+    def nv(self):
+        return int(self.info.nv)
+
+    def ne(self):
+        return int(self.info.ne)
+
+    # ---- layout exports (integer, bit-exact with the reference's IndexManager) ------------
+    def batches(self, is_edge):
+        out = []
+        n = self.info.nebatches if is_edge else self.info.nvbatches
+        for b in range(n):
+            x = A.NDBatch()
+            A.check(A.lib().nd_network_get_batch(self._net, int(is_edge), b, C.byref(x)))
+            idx = np.ctypeslib.as_array(x.indices, shape=(x.count,)).copy() if x.count else np.zeros(0, np.int64)
+            out.append(dict(kind=x.kind, count=x.count, indices=idx, state_first=x.state_first,
+                            param_first=x.param_first, out_first=x.out_first, in_first=x.in_first))
+        return out
+
+    def ranges(self):
+        """(v_ranges[nv,4,2], e_ranges[ne,6,2]) 1-based (first, last): v_data, v_out, v_para, v_aggr /
+        e_data, e_out.src, e_out.dst, e_para, e_gbufr.src, e_gbufr.dst."""
+        v = np.zeros((self.nv(), 4, 2), np.int64)
+        e = np.zeros((max(self.ne(), 1), 6, 2), np.int64)
+        A.check(A.lib().nd_network_export_ranges(self._net, v.ctypes.data, e.ctypes.data))
+        return v, e[: self.ne()]
+
+    def maps(self):
+        """(AggregationMap.range, AggregationMap.map, EagerGBufProvider.map), 1-based."""
+        i = self.info
+        n = max(i.agg_last - i.agg_first + 1, 0)
+        a = np.zeros(max(n, 1), np.int64)
+        gm = np.zeros(max(i.lastidx_gbuf, 1), np.int64)
+        A.check(A.lib().nd_network_export_maps(self._net, a.ctypes.data, gm.ctypes.data))
+        return (int(i.agg_first), int(i.agg_last)), a[:n], gm[: i.lastidx_gbuf]
+
+    def csr(self):
+        nnz = C.c_int64()
+        A.check(A.lib().nd_network_csr_size(self._net, C.byref(nnz)))
+        rp = np.zeros(self.info.lastidx_aggr + 1, np.int64)
+        cols = np.zeros(max(nnz.value, 1), np.int64)
+        A.check(A.lib().nd_network_export_csr(self._net, rp.ctypes.data, cols.ctypes.data))
+        return rp, cols[: nnz.value]
+
+    # ---- parameters / states ----------------------------------------------------------------
+    def default_p(self):
+        """Flat parameter vector from the models' defaults (NaN where none)."""
+        p = np.full(self.pdim, np.nan)
+        v, e = self.ranges()
+        for i, m in enumerate(self.vertexm):
+            for j, d in enumerate(m.psym_default):
+                if d is not None:
+                    p[v[i, 2, 0] - 1 + j] = d
+        for i, m in enumerate(self.edgem):
+            for j, d in enumerate(m.psym_default):
+                if d is not None:
+                    p[e[i, 3, 0] - 1 + j] = d
+        return p
+
+    # ---- device plan ---------------------------------------------------------------------
+    @property
+    def plan(self):
+        if self._plan is None:
+            import torch  # noqa: F401  (initialises the HIP runtime on torch's device)
+            h = C.c_void_p()
+            A.check(A.lib().nd_plan_create(self._net, C.byref(h)))
+            self._plan = h
+        return self._plan
+
+    def tune(self, hub_degree=0, block_size=0):
+        t = A.NDPlanTuning(int(hub_degree), int(block_size))
+        A.check(A.lib().nd_plan_tune(self.plan, C.byref(t)))
+
+    def tuning(self):
+        t = A.NDPlanTuning()
+        A.check(A.lib().nd_plan_get_tuning(self.plan, C.byref(t)))
+        return dict(hub_degree=t.hub_degree, block_size=t.block_size)
+
+    def __call__(self, du, u, p, t=0.0):
+        """nw(du, u, p, t): coreloop.jl:1-102.  du is overwritten; stream-ordered on
+        torch's current stream."""
+        import torch
+        dptr, ndu = _tensor_ptr(du, self.dim, "du", torch)
+        uptr, nu = _tensor_ptr(u, self.dim, "u", torch)
+        if self.pdim > 0:
+            pptr, np_ = _tensor_ptr(p, self.pdim, "p", torch)
+        else:
+            pptr, np_ = (p.data_ptr(), p.numel()) if isinstance(p, torch.Tensor) else (None, 0)
+        stream = torch.cuda.current_stream(du.device).cuda_stream
+        A.check(A.lib().nd_rhs(self.plan, dptr, uptr, pptr, float(t), ndu, nu, np_, stream))
+        return None
+
+    def get_buffers(self, u, p, t=0.0):
+        """get_buffers(nw, u, p, t; initbufs=true): (o, aggbuf) device tensors, coreloop.jl:103-109."""
+        import torch
+        uptr, _ = _tensor_ptr(u, self.dim, "u", torch)
+        pptr = p.data_ptr() if isinstance(p, torch.Tensor) and p.numel() else None
+        o = torch.empty(self.info.lastidx_out, dtype=torch.float64, device=u.device)
+        agg = torch.empty(self.info.lastidx_aggr, dtype=torch.float64, device=u.device)
+        stream = torch.cuda.current_stream(u.device).cuda_stream
+        A.check(A.lib().nd_get_buffers(self.plan, o.data_ptr(), agg.data_ptr() if agg.numel() else None,
+                                       uptr, pptr, float(t), stream))
+        return o, agg
+
+    def gather(self, gbuf, o):
+        """gather!(EagerGBufProvider, gbuf, o): gbuf[k] = o[map[k]] (gbufs.jl:25)."""
+        import torch
+        stream = torch.cuda.current_stream(o.device).cuda_stream
+        A.check(A.lib().nd_gather(self.plan, gbuf.data_ptr(), o.data_ptr(), stream))
+
+    def aggregate(self, aggbuf, o):
+        """aggregate!(aggregator, aggbuf, o): additive (aggregators_test.jl:69-79)."""
+        import torch
+        stream = torch.cuda.current_stream(o.device).cuda_stream
+        A.check(A.lib().nd_aggregate(self.plan, aggbuf.data_ptr(), o.data_ptr(), stream))
+
+    def __repr__(self):
+        return (f"Network with {self.dim} states and {self.pdim} parameters\n"
+                f" ├ {self.nv()} vertices ({self.info.nvbatches} batches)\n"
+                f" └ {self.ne()} edges ({self.info.nebatches} batches)")

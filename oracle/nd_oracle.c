@@ -1,0 +1,716 @@
+/*
+ * nd_oracle.c -- CPU ORACLE.  Test infrastructure only: a plain-C restatement
+ * of NetworkDynamics.jl v1.1.0's network right-hand side, used by tests/ to
+ * check the HIP library and by bench.py's cpu_baseline leg.  Nothing in the
+ * product links or calls this file.
+ *
+ * Built with -O2 -ffp-contract=off (no FMA contraction) so each model's
+ * arithmetic happens in the order the reference writes it.
+ *
+ * What each part follows (paths relative to the reference repository):
+ *   construction / batching    src/construction.jl:31-256, 367-401; src/utils.jl:197-217
+ *   IndexManager, registration src/network_structure.jl:1-55, 224-289
+ *   BatchStride                src/utils.jl:1-42
+ *   AggregationMap             src/aggregators.jl:54-86
+ *   inverse aggregation map    src/aggregators.jl:203-235
+ *   Eager gather map           src/gbufs.jl:11-31; Lazy provider src/gbufs.jl:44-66
+ *   coreloop (phase order)     src/coreloop.jl:1-102, 176-248
+ *   SequentialAggregator       src/aggregators.jl:140-151
+ *   Threaded{Execution,Aggregator} src/coreloop.jl:121-129, src/aggregators.jl:192-201
+ *   output wrappers            src/component_functions.jl:81-203
+ *   model equations            test/ComponentLibrary.jl:8-72, benchmark/benchmark_models.jl:5-42,
+ *                              test/diffusion_test.jl:57-107
+ *
+ * Pinning (tests/test_oracle.py): BatchStride known answers (test/utils_test.jl:147-178);
+ * the complete_graph(4) maps derived by hand from the source (fixture
+ * tests/golden/complete4_layout.json, cross-checked with
+ * test/symbolicindexing_test.jl:84-113); Eager==Lazy and Sequential==Threaded==Naive
+ * agreement (test/testutils.jl:10-64, test/aggregators_test.jl:58-79);
+ * closed forms du = -L x (test/diffusion_test.jl:80-90) and the Kuramoto
+ * incidence form dtheta = omega - K B sin(B^T theta) (oldexamples/paper.jl:10-11);
+ * dense O(N^2) brute force; conservation invariants.
+ * Parity unpinned: the last ulp of sin (Julia's own sin vs glibc's).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/nd_hip.h"
+
+typedef struct { int64_t first, last; } rng_t;   /* Julia UnitRange, 1-based, empty: last = first-1 */
+
+static int64_t rlen(rng_t r) { return r.last - r.first + 1; }
+
+/* BatchStride (utils.jl:1-42): element i (1-based) range = first + (i-1)*sum(strides) + offset(j). */
+typedef struct { int64_t first; int32_t n; int32_t s[2]; } bstride_t;
+static int64_t bs_full(const bstride_t* b) { int64_t t = 0; for (int j = 0; j < b->n; ++j) t += b->s[j]; return t; }
+static rng_t bs_range(const bstride_t* b, int64_t i) {
+    int64_t st = b->first + (i - 1) * bs_full(b);
+    rng_t r = {st, st + bs_full(b) - 1};
+    return r;
+}
+static rng_t bs_subrange(const bstride_t* b, int64_t i, int j /* 1-based */) {
+    int64_t off = 0;
+    for (int q = 0; q < j - 1; ++q) off += b->s[q];
+    int64_t st = b->first + (i - 1) * bs_full(b) + off;
+    rng_t r = {st, st + b->s[j - 1] - 1};
+    return r;
+}
+
+/* exported for the BatchStride known-answer test (test/utils_test.jl:147-178) */
+void ndo_batchstride_range(int64_t first, int32_t n, const int32_t* strides, int64_t i, int32_t j,
+                           int64_t* out_first, int64_t* out_last) {
+    bstride_t b; b.first = first; b.n = n; b.s[0] = strides[0]; b.s[1] = n > 1 ? strides[1] : 0;
+    rng_t r = (j == 0) ? bs_range(&b, i) : bs_subrange(&b, i, j);
+    *out_first = r.first; *out_last = r.last;
+}
+int64_t ndo_batchstride_fullstride(int32_t n, const int32_t* strides) {
+    bstride_t b; b.first = 1; b.n = n; b.s[0] = strides[0]; b.s[1] = n > 1 ? strides[1] : 0;
+    return bs_full(&b);
+}
+
+typedef struct {
+    int32_t kind;
+    int64_t n;
+    int64_t* idx;                         /* 1-based component ids */
+    bstride_t state, p, in, out;
+} batch_t;
+
+typedef struct {
+    int64_t nv, ne;
+    int64_t *src, *dst;
+    int32_t nkinds;
+    nd_kind* kinds;
+    int32_t *vkind, *ekind;
+    /* IndexManager, network_structure.jl:1-31 */
+    rng_t *v_data, *v_out, *v_para, *v_aggr;
+    rng_t *e_data, *e_out_src, *e_out_dst, *e_para, *e_gb_src, *e_gb_dst;
+    int32_t vdepth, edepth;
+    int64_t lastidx_dynamic, lastidx_out, lastidx_p, lastidx_aggr, lastidx_gbuf;
+    int32_t nvb, neb;
+    batch_t *vb, *eb;
+    /* AggregationMap, aggregators.jl:54-86 (tightened) */
+    int64_t agg_first, agg_last;
+    int64_t* aggmap;
+    /* EagerGBufProvider map, gbufs.jl:11-22 */
+    int64_t* gmap;
+    /* _inv_aggregation_map, aggregators.jl:203-235: one (dst, srcs) entry per aggr index, vertex order */
+    int64_t n_inv;
+    int64_t *inv_dst, *inv_ptr, *inv_src;
+    /* caches (construction.jl:205-208, gbufs.jl:20) */
+    double *o, *aggbuf, *gbuf;
+    char err[256];
+} ndo_net;
+
+static void* xcalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
+
+void ndo_net_destroy(ndo_net* nw) {
+    if (!nw) return;
+    free(nw->src); free(nw->dst); free(nw->kinds); free(nw->vkind); free(nw->ekind);
+    free(nw->v_data); free(nw->v_out); free(nw->v_para); free(nw->v_aggr);
+    free(nw->e_data); free(nw->e_out_src); free(nw->e_out_dst); free(nw->e_para);
+    free(nw->e_gb_src); free(nw->e_gb_dst);
+    for (int b = 0; b < nw->nvb; ++b) free(nw->vb[b].idx);
+    for (int b = 0; b < nw->neb; ++b) free(nw->eb[b].idx);
+    free(nw->vb); free(nw->eb);
+    free(nw->aggmap); free(nw->gmap);
+    free(nw->inv_dst); free(nw->inv_ptr); free(nw->inv_src);
+    free(nw->o); free(nw->aggbuf); free(nw->gbuf);
+    free(nw);
+}
+
+/* find_identical (utils.jl:197-217) over the kind ids: groups in first-appearance order */
+static int find_identical(const int32_t* key, int64_t n, batch_t** out) {
+    int32_t ng = 0;
+    int32_t* gk = (int32_t*)xcalloc((size_t)n, sizeof(int32_t));
+    int64_t* cnt = (int64_t*)xcalloc((size_t)n, sizeof(int64_t));
+    int32_t* gid = (int32_t*)xcalloc((size_t)n, sizeof(int32_t));
+    for (int64_t i = 0; i < n; ++i) {
+        int found = -1;
+        for (int32_t j = 0; j < ng; ++j) if (gk[j] == key[i]) { found = j; break; }
+        if (found < 0) { found = ng; gk[ng++] = key[i]; }
+        gid[i] = found; cnt[found]++;
+    }
+    batch_t* bs = (batch_t*)xcalloc((size_t)ng, sizeof(batch_t));
+    for (int32_t j = 0; j < ng; ++j) { bs[j].kind = gk[j]; bs[j].idx = (int64_t*)xcalloc((size_t)cnt[j], sizeof(int64_t)); }
+    for (int64_t i = 0; i < n; ++i) { batch_t* b = &bs[gid[i]]; b->idx[b->n++] = i + 1; }
+    free(gk); free(cnt); free(gid);
+    *out = bs;
+    return ng;
+}
+
+/* _nextrange (network_structure.jl:289) */
+static rng_t nextrange(int64_t* last, int64_t N) { rng_t r = {*last + 1, *last + N}; *last += N; return r; }
+
+ndo_net* ndo_net_create(int64_t nv, int64_t ne, const int64_t* src, const int64_t* dst,
+                        int32_t nkinds, const nd_kind* kinds, const int32_t* vkind, const int32_t* ekind,
+                        char* errbuf, int64_t errlen) {
+    ndo_net* nw = (ndo_net*)xcalloc(1, sizeof(ndo_net));
+#define FAIL(msg) do { if (errbuf && errlen > 0) { strncpy(errbuf, msg, (size_t)errlen - 1); errbuf[errlen - 1] = 0; } ndo_net_destroy(nw); return NULL; } while (0)
+    nw->nv = nv; nw->ne = ne; nw->nkinds = nkinds;
+    nw->src = (int64_t*)xcalloc((size_t)ne, 8); nw->dst = (int64_t*)xcalloc((size_t)ne, 8);
+    memcpy(nw->src, src, (size_t)ne * 8); memcpy(nw->dst, dst, (size_t)ne * 8);
+    nw->kinds = (nd_kind*)xcalloc((size_t)nkinds, sizeof(nd_kind)); memcpy(nw->kinds, kinds, (size_t)nkinds * sizeof(nd_kind));
+    nw->vkind = (int32_t*)xcalloc((size_t)nv, 4); nw->ekind = (int32_t*)xcalloc((size_t)ne, 4);
+    memcpy(nw->vkind, vkind, (size_t)nv * 4); memcpy(nw->ekind, ekind, (size_t)ne * 4);
+    for (int64_t i = 0; i < nv; ++i) if (vkind[i] < 0 || vkind[i] >= nkinds || kinds[vkind[i]].is_edge) FAIL("bad vertex kind");
+    for (int64_t i = 0; i < ne; ++i) if (ekind[i] < 0 || ekind[i] >= nkinds || !kinds[ekind[i]].is_edge) FAIL("bad edge kind");
+    for (int64_t i = 0; i < ne; ++i) if (src[i] < 1 || src[i] > nv || dst[i] < 1 || dst[i] > nv) FAIL("edge endpoint out of range");
+    if (nv < 1) FAIL("empty graph");
+
+    /* construction.jl:99-106: equal vertex outdims / edge dst outdims */
+    nw->vdepth = kinds[vkind[0]].outdim_dst;
+    for (int64_t i = 0; i < nv; ++i) if (kinds[vkind[i]].outdim_dst != nw->vdepth) FAIL("All vertex models must have the same output dimension!");
+    nw->edepth = ne ? kinds[ekind[0]].outdim_dst : 0;
+    for (int64_t i = 0; i < ne; ++i) if (kinds[ekind[i]].outdim_dst != nw->edepth) FAIL("All edge models must have the same output dimension!");
+
+    nw->v_data = (rng_t*)xcalloc((size_t)nv, sizeof(rng_t)); nw->v_out = (rng_t*)xcalloc((size_t)nv, sizeof(rng_t));
+    nw->v_para = (rng_t*)xcalloc((size_t)nv, sizeof(rng_t)); nw->v_aggr = (rng_t*)xcalloc((size_t)nv, sizeof(rng_t));
+    nw->e_data = (rng_t*)xcalloc((size_t)ne, sizeof(rng_t)); nw->e_out_src = (rng_t*)xcalloc((size_t)ne, sizeof(rng_t));
+    nw->e_out_dst = (rng_t*)xcalloc((size_t)ne, sizeof(rng_t)); nw->e_para = (rng_t*)xcalloc((size_t)ne, sizeof(rng_t));
+    nw->e_gb_src = (rng_t*)xcalloc((size_t)ne, sizeof(rng_t)); nw->e_gb_dst = (rng_t*)xcalloc((size_t)ne, sizeof(rng_t));
+
+    nw->nvb = find_identical(vkind, nv, &nw->vb);
+    nw->neb = find_identical(ekind, ne, &nw->eb);
+
+    /* VertexBatch -> register_vertices! (construction.jl:367-383, network_structure.jl:224-239) */
+    for (int b = 0; b < nw->nvb; ++b) {
+        batch_t* B = &nw->vb[b];
+        const nd_kind* k = &kinds[B->kind];
+        for (int64_t q = 0; q < B->n; ++q) {
+            int64_t i = B->idx[q] - 1;
+            nw->v_data[i] = nextrange(&nw->lastidx_dynamic, k->dim);
+            nw->v_out[i] = nextrange(&nw->lastidx_out, k->outdim_dst);
+            nw->v_para[i] = nextrange(&nw->lastidx_p, k->pdim);
+            nw->v_aggr[i] = nextrange(&nw->lastidx_aggr, nw->edepth);
+        }
+        int64_t f = B->idx[0] - 1;
+        B->state.first = nw->v_data[f].first; B->state.n = 1; B->state.s[0] = k->dim;
+        B->p.first = nw->v_para[f].first; B->p.n = 1; B->p.s[0] = k->pdim;
+        B->in.first = nw->v_aggr[f].first; B->in.n = 1; B->in.s[0] = nw->edepth;
+        B->out.first = nw->v_out[f].first; B->out.n = 1; B->out.s[0] = k->outdim_dst;
+    }
+    /* EdgeBatch -> register_edges! (construction.jl:385-401, network_structure.jl:240-258) */
+    for (int b = 0; b < nw->neb; ++b) {
+        batch_t* B = &nw->eb[b];
+        const nd_kind* k = &kinds[B->kind];
+        for (int64_t q = 0; q < B->n; ++q) {
+            int64_t i = B->idx[q] - 1;
+            nw->e_data[i] = nextrange(&nw->lastidx_dynamic, k->dim);
+            nw->e_out_src[i] = nextrange(&nw->lastidx_out, k->outdim_src);
+            nw->e_out_dst[i] = nextrange(&nw->lastidx_out, k->outdim_dst);
+            nw->e_para[i] = nextrange(&nw->lastidx_p, k->pdim);
+            nw->e_gb_src[i] = nextrange(&nw->lastidx_gbuf, nw->vdepth);
+            nw->e_gb_dst[i] = nextrange(&nw->lastidx_gbuf, nw->vdepth);
+        }
+        int64_t f = B->idx[0] - 1;
+        B->state.first = nw->e_data[f].first; B->state.n = 1; B->state.s[0] = k->dim;
+        B->p.first = nw->e_para[f].first; B->p.n = 1; B->p.s[0] = k->pdim;
+        B->in.first = nw->e_gb_src[f].first; B->in.n = 2; B->in.s[0] = nw->vdepth; B->in.s[1] = nw->vdepth;
+        B->out.first = nw->e_out_src[f].first; B->out.n = 2; B->out.s[0] = k->outdim_src; B->out.s[1] = k->outdim_dst;
+    }
+
+    /* AggregationMap (aggregators.jl:58-79) */
+    int64_t* m = (int64_t*)xcalloc((size_t)nw->lastidx_out, 8);
+    for (int b = 0; b < nw->neb; ++b) {
+        batch_t* B = &nw->eb[b];
+        for (int64_t q = 0; q < B->n; ++q) {
+            int64_t e = B->idx[q] - 1;
+            rng_t s = nw->e_out_dst[e], t = nw->v_aggr[nw->dst[e] - 1];
+            if (rlen(s) != rlen(t)) FAIL("DimensionMismatch in aggregation map");
+            for (int64_t j = 0; j < rlen(s); ++j) m[s.first - 1 + j] = t.first + j;
+            s = nw->e_out_src[e];
+            if (rlen(s) > 0) {
+                t = nw->v_aggr[nw->src[e] - 1];
+                if (rlen(s) != rlen(t)) FAIL("DimensionMismatch in aggregation map");
+                for (int64_t j = 0; j < rlen(s); ++j) m[s.first - 1 + j] = t.first + j;
+            }
+        }
+    }
+    /* _tighten_idxrange (aggregators.jl:80-86) */
+    int64_t fst = -1, lst = -1;
+    for (int64_t k = 0; k < nw->lastidx_out; ++k) if (m[k] != 0) { fst = k; break; }
+    for (int64_t k = nw->lastidx_out - 1; k >= 0; --k) if (m[k] != 0) { lst = k; break; }
+    if (fst < 0) { nw->agg_first = 1; nw->agg_last = 0; nw->aggmap = (int64_t*)xcalloc(1, 8); }
+    else {
+        nw->agg_first = fst + 1; nw->agg_last = lst + 1;
+        nw->aggmap = (int64_t*)xcalloc((size_t)(lst - fst + 1), 8);
+        memcpy(nw->aggmap, m + fst, (size_t)(lst - fst + 1) * 8);
+    }
+    free(m);
+
+    /* EagerGBufProvider map (gbufs.jl:11-22): iterates pairs(edgevec) */
+    nw->gmap = (int64_t*)xcalloc((size_t)nw->lastidx_gbuf, 8);
+    for (int64_t e = 0; e < ne; ++e) {
+        rng_t s = nw->e_gb_src[e], t = nw->v_out[nw->src[e] - 1];
+        for (int64_t j = 0; j < rlen(s); ++j) nw->gmap[s.first - 1 + j] = t.first + j;
+        s = nw->e_gb_dst[e]; t = nw->v_out[nw->dst[e] - 1];
+        for (int64_t j = 0; j < rlen(s); ++j) nw->gmap[s.first - 1 + j] = t.first + j;
+    }
+
+    /* _inv_aggregation_map (aggregators.jl:203-235): per vertex, per aggr element a list;
+       dst outputs pushed before src outputs for each edge, batches and components in order. */
+    {
+        int64_t na = nw->lastidx_aggr;
+        int64_t* cnt = (int64_t*)xcalloc((size_t)na + 1, 8);
+        for (int b = 0; b < nw->neb; ++b) {
+            batch_t* B = &nw->eb[b];
+            for (int64_t q = 0; q < B->n; ++q) {
+                int64_t e = B->idx[q] - 1;
+                rng_t t = nw->v_aggr[nw->dst[e] - 1];
+                for (int64_t j = 0; j < rlen(nw->e_out_dst[e]); ++j) cnt[t.first - 1 + j]++;
+                if (rlen(nw->e_out_src[e]) > 0) {
+                    t = nw->v_aggr[nw->src[e] - 1];
+                    for (int64_t j = 0; j < rlen(nw->e_out_src[e]); ++j) cnt[t.first - 1 + j]++;
+                }
+            }
+        }
+        int64_t* ptr_by_aggr = (int64_t*)xcalloc((size_t)na + 1, 8);
+        for (int64_t a = 0; a < na; ++a) ptr_by_aggr[a + 1] = ptr_by_aggr[a] + cnt[a];
+        int64_t* lists = (int64_t*)xcalloc((size_t)ptr_by_aggr[na], 8);
+        int64_t* fill = (int64_t*)xcalloc((size_t)na + 1, 8);
+        for (int b = 0; b < nw->neb; ++b) {
+            batch_t* B = &nw->eb[b];
+            for (int64_t q = 0; q < B->n; ++q) {
+                int64_t e = B->idx[q] - 1;
+                rng_t t = nw->v_aggr[nw->dst[e] - 1], s = nw->e_out_dst[e];
+                for (int64_t j = 0; j < rlen(s); ++j) { int64_t a = t.first - 1 + j; lists[ptr_by_aggr[a] + fill[a]++] = s.first + j; }
+                s = nw->e_out_src[e];
+                if (rlen(s) > 0) {
+                    t = nw->v_aggr[nw->src[e] - 1];
+                    for (int64_t j = 0; j < rlen(s); ++j) { int64_t a = t.first - 1 + j; lists[ptr_by_aggr[a] + fill[a]++] = s.first + j; }
+                }
+            }
+        }
+        /* ret in vertex order: for (dstr, srcs) in zip(v_aggr, srcidxs) */
+        nw->n_inv = na;
+        nw->inv_dst = (int64_t*)xcalloc((size_t)na, 8);
+        nw->inv_ptr = (int64_t*)xcalloc((size_t)na + 1, 8);
+        nw->inv_src = (int64_t*)xcalloc((size_t)ptr_by_aggr[na], 8);
+        int64_t w = 0, pos = 0;
+        for (int64_t v = 0; v < nv; ++v) {
+            rng_t r = nw->v_aggr[v];
+            for (int64_t a = r.first - 1; a < r.last; ++a) {
+                nw->inv_dst[w] = a + 1;
+                nw->inv_ptr[w] = pos;
+                for (int64_t q = ptr_by_aggr[a]; q < ptr_by_aggr[a + 1]; ++q) nw->inv_src[pos++] = lists[q];
+                ++w;
+            }
+        }
+        nw->inv_ptr[w] = pos;
+        free(cnt); free(ptr_by_aggr); free(lists); free(fill);
+    }
+
+    nw->o = (double*)xcalloc((size_t)nw->lastidx_out, 8);
+    nw->aggbuf = (double*)xcalloc((size_t)nw->lastidx_aggr, 8);
+    nw->gbuf = (double*)xcalloc((size_t)nw->lastidx_gbuf, 8);
+    return nw;
+#undef FAIL
+}
+
+/* ---- accessors for the tests ---------------------------------------------- */
+void ndo_net_info(const ndo_net* nw, int64_t* out /* 11 */) {
+    out[0] = nw->lastidx_dynamic; out[1] = nw->lastidx_p; out[2] = nw->lastidx_out;
+    out[3] = nw->lastidx_aggr; out[4] = nw->lastidx_gbuf; out[5] = nw->agg_first; out[6] = nw->agg_last;
+    out[7] = nw->vdepth; out[8] = nw->edepth; out[9] = nw->nvb; out[10] = nw->neb;
+}
+void ndo_net_ranges(const ndo_net* nw, int64_t* v /* nv*4*2 */, int64_t* e /* ne*6*2 */) {
+    for (int64_t i = 0; i < nw->nv; ++i) {
+        rng_t rs[4] = {nw->v_data[i], nw->v_out[i], nw->v_para[i], nw->v_aggr[i]};
+        for (int j = 0; j < 4; ++j) { v[i * 8 + 2 * j] = rs[j].first; v[i * 8 + 2 * j + 1] = rs[j].last; }
+    }
+    for (int64_t i = 0; i < nw->ne; ++i) {
+        rng_t rs[6] = {nw->e_data[i], nw->e_out_src[i], nw->e_out_dst[i], nw->e_para[i], nw->e_gb_src[i], nw->e_gb_dst[i]};
+        for (int j = 0; j < 6; ++j) { e[i * 12 + 2 * j] = rs[j].first; e[i * 12 + 2 * j + 1] = rs[j].last; }
+    }
+}
+void ndo_net_maps(const ndo_net* nw, int64_t* aggmap, int64_t* gmap) {
+    int64_t n = nw->agg_last - nw->agg_first + 1;
+    if (n > 0) memcpy(aggmap, nw->aggmap, (size_t)n * 8);
+    if (nw->lastidx_gbuf > 0) memcpy(gmap, nw->gmap, (size_t)nw->lastidx_gbuf * 8);
+}
+int64_t ndo_net_inv_nnz(const ndo_net* nw) { return nw->inv_ptr[nw->n_inv]; }
+void ndo_net_inv(const ndo_net* nw, int64_t* dst, int64_t* ptr, int64_t* srcs) {
+    memcpy(dst, nw->inv_dst, (size_t)nw->n_inv * 8);
+    memcpy(ptr, nw->inv_ptr, (size_t)(nw->n_inv + 1) * 8);
+    memcpy(srcs, nw->inv_src, (size_t)nw->inv_ptr[nw->n_inv] * 8);
+}
+void ndo_net_batch(const ndo_net* nw, int32_t is_edge, int32_t b, int64_t* out /* 6 */, int64_t* idx) {
+    const batch_t* B = is_edge ? &nw->eb[b] : &nw->vb[b];
+    out[0] = B->kind; out[1] = B->n; out[2] = B->state.first; out[3] = B->p.first; out[4] = B->out.first; out[5] = B->in.first;
+    if (idx) memcpy(idx, B->idx, (size_t)B->n * 8);
+}
+
+/* ---- models ---------------------------------------------------------------- */
+
+static int hasff(const nd_kind* k) { return k->ff == ND_FF_PUREFEEDFORWARD || k->ff == ND_FF_FEEDFORWARD; }
+
+/* vertex f (apply_compf, coreloop.jl:213-216) */
+static void vertex_f(const nd_kind* k, double* du, const double* u, const double* acc, const double* p, double t) {
+    (void)t;
+    switch (k->model) {
+    case ND_VM_DIFFUSION:          /* ComponentLibrary.jl:42-45 */
+        du[0] = acc[0];
+        break;
+    case ND_VM_KURAMOTO:           /* ComponentLibrary.jl:69-71 */
+        du[0] = p[0] + acc[0];
+        break;
+    case ND_VM_SWING: {            /* ComponentLibrary.jl:59-63 */
+        double M = p[0], D = p[1], Pm = p[2];
+        du[0] = u[1];
+        du[1] = 1.0 / M * (Pm - D * u[1] + acc[0]);
+        break;
+    }
+    case ND_VM_SWING_UNIT_DAMPING: /* benchmark_models.jl:37-41 */
+        du[0] = u[1];
+        du[1] = p[0] - 1.0 * u[1];
+        du[1] += acc[0];
+        break;
+    case ND_VM_LOAD: {             /* build-defined first-order load */
+        double tau = p[0], P = p[1];
+        du[0] = 1.0 / tau * (P + acc[0]);
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+/* single-sided (dst) output of a pure feed-forward edge model */
+static void edge_gdst(const nd_kind* k, double* y, const double* vs, const double* vd, const double* p, int32_t depth) {
+    switch (k->model) {
+    case ND_EM_DIFFUSION:          /* diffusion_test.jl:57-60: e .= v_s .- v_d */
+        for (int j = 0; j < depth; ++j) y[j] = vs[j] - vd[j];
+        break;
+    case ND_EM_DIFFUSION_WEIGHTED: /* ComponentLibrary.jl:8-10 */
+        y[0] = p[0] * (vs[0] - vd[0]);
+        break;
+    case ND_EM_KURAMOTO:           /* ComponentLibrary.jl:51-53 */
+        y[0] = p[0] * sin(vs[0] - vd[0]);
+        break;
+    case ND_EM_STATIC_LINE:        /* build-defined static line with phase shift */
+        y[0] = p[0] * sin(vs[0] - vd[0] - p[1]);
+        break;
+    default:
+        break;
+    }
+}
+
+/* edge g: the wrappers of component_functions.jl:81-203 around the model */
+static void edge_g(const nd_kind* k, double* osrc, double* odst, const double* ue, const double* vs,
+                   const double* vd, const double* p) {
+    int32_t ns = k->outdim_src, nd = k->outdim_dst;
+    if (k->model == ND_EM_NOOP) return;
+    if (k->ff == ND_FF_PURESTATEMAP) {   /* Fiducial(StateMask src, StateMask dst) */
+        for (int j = 0; j < ns; ++j) osrc[j] = ue[k->mask_src[j] - 1];
+        for (int j = 0; j < nd; ++j) odst[j] = ue[k->mask_dst[j] - 1];
+        return;
+    }
+    if (k->model == ND_EM_DIFFUSION_FIDUCIAL) {   /* ComponentLibrary.jl:22-25 */
+        odst[0] = p[0] * (vs[0] - vd[0]);
+        osrc[0] = -odst[0];
+        return;
+    }
+    edge_gdst(k, odst, vs, vd, p, nd);
+    switch (k->wrapper) {
+    case ND_WRAP_ANTISYMMETRIC: for (int j = 0; j < ns; ++j) osrc[j] = -odst[j]; break;
+    case ND_WRAP_SYMMETRIC:     for (int j = 0; j < ns; ++j) osrc[j] = odst[j]; break;
+    case ND_WRAP_DIRECTED:      for (int j = 0; j < ns; ++j) osrc[j] = 0.0; break;
+    default: break;
+    }
+}
+
+/* edge f for ODE edges */
+static void edge_f(const nd_kind* k, double* de, const double* e, const double* vs, const double* vd, const double* p) {
+    switch (k->model) {
+    case ND_EM_ODE_SIN_DIFFUSION:      /* ComponentLibrary.jl:30-34 */
+        de[0] = 1.0 / p[0] * (sin(vs[0] - vd[0]) - e[0]);
+        de[1] = 1.0 / p[0] * (sin(vd[0] - vs[0]) - e[1]);
+        break;
+    case ND_EM_ODE_SIN_DIFFUSION_FAST: /* benchmark_models.jl:11-15 */
+        de[0] = 100.0 * (sin(vs[0] - vd[0]) - e[0]);
+        de[1] = 100.0 * (sin(vd[0] - vs[0]) - e[1]);
+        break;
+    case ND_EM_ODE_LINEAR:             /* diffusion_test.jl:102-106 */
+        de[0] = vs[0] - vd[0] - e[0];
+        de[1] = vd[0] - vs[0] - e[1];
+        break;
+    default:
+        break;
+    }
+}
+
+static int edge_has_f(const nd_kind* k) {
+    return k->model == ND_EM_ODE_SIN_DIFFUSION || k->model == ND_EM_ODE_SIN_DIFFUSION_FAST || k->model == ND_EM_ODE_LINEAR;
+}
+static int vertex_has_f(const nd_kind* k) { (void)k; return 1; }
+
+/* ---- coreloop ----------------------------------------------------------------- */
+
+/* apply_comp! for one vertex (coreloop.jl:176-192), BatchStride-indexed */
+static void apply_vertex(const ndo_net* nw, const batch_t* B, int64_t i, int pass_g, double* du,
+                         const double* u, double* o, const double* aggbuf, const double* p, double t) {
+    const nd_kind* k = &nw->kinds[B->kind];
+    rng_t sr = bs_range(&B->state, i), pr = bs_range(&B->p, i);
+    if (pass_g) {   /* StateMask output, PureStateMap: g(o, u) */
+        rng_t orr = bs_range(&B->out, i);
+        for (int j = 0; j < k->outdim_dst; ++j) o[orr.first - 1 + j] = u[sr.first - 1 + k->mask_dst[j] - 1];
+    } else {
+        rng_t ir = bs_range(&B->in, i);
+        vertex_f(k, du + sr.first - 1, u + sr.first - 1, aggbuf + ir.first - 1, p + pr.first - 1, t);
+    }
+}
+
+/* apply_comp! for one edge (coreloop.jl:194-211) */
+static void apply_edge(const ndo_net* nw, const batch_t* B, int64_t i, int evalf, int evalg, int buffered,
+                       double* du, const double* u, double* o, const double* gbuf, const double* p, double t) {
+    (void)t;
+    const nd_kind* k = &nw->kinds[B->kind];
+    rng_t sr = bs_range(&B->state, i), pr = bs_range(&B->p, i);
+    const double *vs, *vd;
+    if (buffered) {   /* get_src_dst on the gathered buffer (gbufs.jl:27-31) */
+        vs = gbuf + bs_subrange(&B->in, i, 1).first - 1;
+        vd = gbuf + bs_subrange(&B->in, i, 2).first - 1;
+    } else {          /* LazyGBuf: views into o through v_out (gbufs.jl:58-66) */
+        int64_t e = B->idx[i - 1] - 1;
+        vs = o + nw->v_out[nw->src[e] - 1].first - 1;
+        vd = o + nw->v_out[nw->dst[e] - 1].first - 1;
+    }
+    if (evalf && edge_has_f(k)) edge_f(k, du + sr.first - 1, u + sr.first - 1, vs, vd, p + pr.first - 1);
+    if (evalg) {
+        rng_t os = bs_subrange(&B->out, i, 1), od = bs_subrange(&B->out, i, 2);
+        edge_g(k, o + os.first - 1, o + od.first - 1, u + sr.first - 1, vs, vd, p + pr.first - 1);
+    }
+}
+
+static void fill_d(double* x, int64_t n, double v) { for (int64_t i = 0; i < n; ++i) x[i] = v; }
+
+/* SequentialAggregator aggregate! (aggregators.jl:140-151); additive */
+void ndo_aggregate_seq(const ndo_net* nw, double* aggbuf, const double* data) {
+    int64_t n = nw->agg_last - nw->agg_first + 1;
+    for (int64_t k = 0; k < n; ++k) {
+        int64_t d = nw->aggmap[k];
+        if (d != 0) aggbuf[d - 1] = aggbuf[d - 1] + data[nw->agg_first - 1 + k];
+    }
+}
+/* ThreadedAggregator aggregate! over the inverse map (aggregators.jl:192-201); additive */
+void ndo_aggregate_inv(const ndo_net* nw, double* aggbuf, const double* data, int32_t nthreads) {
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+    for (int64_t w = 0; w < nw->n_inv; ++w) {
+        int64_t d = nw->inv_dst[w] - 1;
+        double acc = aggbuf[d];
+        for (int64_t q = nw->inv_ptr[w]; q < nw->inv_ptr[w + 1]; ++q) acc = acc + data[nw->inv_src[q] - 1];
+        aggbuf[d] = acc;
+    }
+    (void)nthreads;
+}
+/* NaiveAggregator (aggregators.jl:29-51); additive */
+void ndo_aggregate_naive(const ndo_net* nw, double* aggbuf, const double* data) {
+    for (int b = 0; b < nw->neb; ++b) {
+        const batch_t* B = &nw->eb[b];
+        for (int64_t q = 0; q < B->n; ++q) {
+            int64_t e = B->idx[q] - 1;
+            rng_t s = nw->e_out_dst[e], t = nw->v_aggr[nw->dst[e] - 1];
+            for (int64_t j = 0; j < rlen(s); ++j) aggbuf[t.first - 1 + j] = aggbuf[t.first - 1 + j] + data[s.first - 1 + j];
+            s = nw->e_out_src[e];
+            if (rlen(s) > 0) {
+                t = nw->v_aggr[nw->src[e] - 1];
+                for (int64_t j = 0; j < rlen(s); ++j) aggbuf[t.first - 1 + j] = aggbuf[t.first - 1 + j] + data[s.first - 1 + j];
+            }
+        }
+    }
+}
+/* gather!(EagerGBufProvider): NNlib.gather!(gbuf, o, map) (gbufs.jl:25) */
+void ndo_gather(const ndo_net* nw, double* gbuf, const double* o) {
+    for (int64_t k = 0; k < nw->lastidx_gbuf; ++k) gbuf[k] = o[nw->gmap[k] - 1];
+}
+
+/* Full RHS, SequentialExecution{buffered} + SequentialAggregator(+). Returns 0, or 1 on size mismatch
+   (the ArgumentError of coreloop.jl:2-7).  If o_out/agg_out are non-NULL they receive the buffers
+   (get_buffers with initbufs=true, coreloop.jl:103-109; then du may be NULL). */
+int ndo_rhs(ndo_net* nw, double* du, const double* u, const double* p, double t, int32_t buffered,
+            double* o_out, double* agg_out) {
+    double* o = nw->o;
+    double* aggbuf = nw->aggbuf;
+    double* gbuf = nw->gbuf;
+    if (du) fill_d(du, nw->lastidx_dynamic, 0.0);                                /* coreloop.jl:24 */
+    fill_d(o, nw->lastidx_out, NAN);                                             /* network_structure.jl:153 */
+    fill_d(aggbuf, nw->lastidx_aggr, 0.0);                                       /* coreloop.jl:30 */
+    for (int b = 0; b < nw->nvb; ++b) {                                          /* coreloop.jl:39 */
+        const batch_t* B = &nw->vb[b];
+        if (hasff(&nw->kinds[B->kind])) continue;
+        for (int64_t i = 1; i <= B->n; ++i) apply_vertex(nw, B, i, 1, du, u, o, aggbuf, p, t);
+    }
+    for (int b = 0; b < nw->neb; ++b) {                                          /* coreloop.jl:41 */
+        const batch_t* B = &nw->eb[b];
+        if (hasff(&nw->kinds[B->kind])) continue;
+        for (int64_t i = 1; i <= B->n; ++i) apply_edge(nw, B, i, 0, 1, buffered, du, u, o, gbuf, p, t);
+    }
+    if (buffered) ndo_gather(nw, gbuf, o);                                       /* coreloop.jl:67 */
+    for (int b = 0; b < nw->neb; ++b) {                                          /* coreloop.jl:76 */
+        const batch_t* B = &nw->eb[b];
+        if (hasff(&nw->kinds[B->kind])) continue;
+        if (!du) continue;
+        for (int64_t i = 1; i <= B->n; ++i) apply_edge(nw, B, i, 1, 0, buffered, du, u, o, gbuf, p, t);
+    }
+    for (int b = 0; b < nw->neb; ++b) {                                          /* coreloop.jl:78 */
+        const batch_t* B = &nw->eb[b];
+        if (!hasff(&nw->kinds[B->kind])) continue;
+        for (int64_t i = 1; i <= B->n; ++i) apply_edge(nw, B, i, du != NULL, 1, buffered, du, u, o, gbuf, p, t);
+    }
+    ndo_aggregate_seq(nw, aggbuf, o);                                            /* coreloop.jl:90 */
+    if (o_out) memcpy(o_out, o, (size_t)nw->lastidx_out * 8);
+    if (agg_out) memcpy(agg_out, aggbuf, (size_t)nw->lastidx_aggr * 8);
+    if (!du) return 0;
+    for (int b = 0; b < nw->nvb; ++b) {                                          /* coreloop.jl:97 */
+        const batch_t* B = &nw->vb[b];
+        if (hasff(&nw->kinds[B->kind]) || !vertex_has_f(&nw->kinds[B->kind])) continue;
+        for (int64_t i = 1; i <= B->n; ++i) apply_vertex(nw, B, i, 0, du, u, o, aggbuf, p, t);
+    }
+    return 0;
+}
+
+/* coreloop.jl:2-7 size checks, for the error-path tests */
+int ndo_check_sizes(const ndo_net* nw, int64_t len_du, int64_t len_u, int64_t len_p) {
+    if (!(len_du == len_u && len_u == nw->lastidx_dynamic)) return 1;
+    if (nw->lastidx_p > 0 && len_p != nw->lastidx_p) return 2;
+    return 0;
+}
+
+/* ThreadedExecution{true} + ThreadedAggregator(+): coreloop.jl:121-129, aggregators.jl:192-201.
+   Per batch and phase, components are split over threads; fills and the NNlib gather stay serial
+   as in the reference's CPU path. */
+int ndo_rhs_threaded(ndo_net* nw, double* du, const double* u, const double* p, double t, int32_t nthreads) {
+    double* o = nw->o;
+    double* aggbuf = nw->aggbuf;
+    double* gbuf = nw->gbuf;
+    if (nthreads < 1) nthreads = 1;
+    fill_d(du, nw->lastidx_dynamic, 0.0);
+    fill_d(o, nw->lastidx_out, NAN);
+    fill_d(aggbuf, nw->lastidx_aggr, 0.0);
+    for (int b = 0; b < nw->nvb; ++b) {
+        const batch_t* B = &nw->vb[b];
+        if (hasff(&nw->kinds[B->kind])) continue;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+        for (int64_t i = 1; i <= B->n; ++i) apply_vertex(nw, B, i, 1, du, u, o, aggbuf, p, t);
+    }
+    for (int b = 0; b < nw->neb; ++b) {
+        const batch_t* B = &nw->eb[b];
+        if (hasff(&nw->kinds[B->kind])) continue;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+        for (int64_t i = 1; i <= B->n; ++i) apply_edge(nw, B, i, 0, 1, 1, du, u, o, gbuf, p, t);
+    }
+    ndo_gather(nw, gbuf, o);
+    for (int b = 0; b < nw->neb; ++b) {
+        const batch_t* B = &nw->eb[b];
+        if (hasff(&nw->kinds[B->kind])) continue;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+        for (int64_t i = 1; i <= B->n; ++i) apply_edge(nw, B, i, 1, 0, 1, du, u, o, gbuf, p, t);
+    }
+    for (int b = 0; b < nw->neb; ++b) {
+        const batch_t* B = &nw->eb[b];
+        if (!hasff(&nw->kinds[B->kind])) continue;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+        for (int64_t i = 1; i <= B->n; ++i) apply_edge(nw, B, i, 1, 1, 1, du, u, o, gbuf, p, t);
+    }
+    ndo_aggregate_inv(nw, aggbuf, o, nthreads);
+    for (int b = 0; b < nw->nvb; ++b) {
+        const batch_t* B = &nw->vb[b];
+        if (hasff(&nw->kinds[B->kind])) continue;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+        for (int64_t i = 1; i <= B->n; ++i) apply_vertex(nw, B, i, 0, du, u, o, aggbuf, p, t);
+    }
+    return 0;
+}
+
+/* Per-element magnitude scale for the parity tolerance: the same RHS with every
+   term replaced by its absolute value (|terms| summed), so |du_gpu - du| <= rtol * S
+   is meaningful where du itself cancels to ~0. */
+void ndo_rhs_scale(ndo_net* nw, double* S, const double* u, const double* p, double t) {
+    (void)t;
+    int64_t nout = nw->lastidx_out;
+    double* o = nw->o;
+    double* agg = nw->aggbuf;
+    fill_d(S, nw->lastidx_dynamic, 0.0);
+    ndo_rhs(nw, NULL, u, p, t, 1, NULL, NULL);    /* fills o */
+    for (int64_t k = 0; k < nout; ++k) o[k] = fabs(o[k]);
+    fill_d(agg, nw->lastidx_aggr, 0.0);
+    ndo_aggregate_seq(nw, agg, o);
+    for (int b = 0; b < nw->nvb; ++b) {
+        const batch_t* B = &nw->vb[b];
+        const nd_kind* k = &nw->kinds[B->kind];
+        for (int64_t i = 1; i <= B->n; ++i) {
+            rng_t sr = bs_range(&B->state, i), pr = bs_range(&B->p, i), ir = bs_range(&B->in, i);
+            double* s = S + sr.first - 1;
+            const double* x = u + sr.first - 1;
+            const double* q = p + pr.first - 1;
+            double a = agg[ir.first - 1];
+            switch (k->model) {
+            case ND_VM_DIFFUSION: s[0] = a; break;
+            case ND_VM_KURAMOTO: s[0] = fabs(q[0]) + a; break;
+            case ND_VM_SWING: s[0] = fabs(x[1]); s[1] = fabs(1.0 / q[0]) * (fabs(q[2]) + fabs(q[1] * x[1]) + a); break;
+            case ND_VM_SWING_UNIT_DAMPING: s[0] = fabs(x[1]); s[1] = fabs(q[0]) + fabs(x[1]) + a; break;
+            case ND_VM_LOAD: s[0] = fabs(1.0 / q[0]) * (fabs(q[1]) + a); break;
+            default: break;
+            }
+        }
+    }
+    for (int b = 0; b < nw->neb; ++b) {
+        const batch_t* B = &nw->eb[b];
+        const nd_kind* k = &nw->kinds[B->kind];
+        if (!edge_has_f(k)) continue;
+        for (int64_t i = 1; i <= B->n; ++i) {
+            int64_t e = B->idx[i - 1] - 1;
+            rng_t sr = bs_range(&B->state, i), pr = bs_range(&B->p, i);
+            double vs = u[nw->v_data[nw->src[e] - 1].first - 1 + nw->kinds[nw->vkind[nw->src[e] - 1]].mask_dst[0] - 1];
+            double vd = u[nw->v_data[nw->dst[e] - 1].first - 1 + nw->kinds[nw->vkind[nw->dst[e] - 1]].mask_dst[0] - 1];
+            double* s = S + sr.first - 1;
+            const double* x = u + sr.first - 1;
+            double c = k->model == ND_EM_ODE_SIN_DIFFUSION ? fabs(1.0 / p[pr.first - 1]) : (k->model == ND_EM_ODE_SIN_DIFFUSION_FAST ? 100.0 : 1.0);
+            double dv = k->model == ND_EM_ODE_LINEAR ? fabs(vs) + fabs(vd) : fabs(sin(vs - vd));
+            s[0] = c * (dv + fabs(x[0]));
+            s[1] = c * (dv + fabs(x[1]));
+        }
+    }
+}
+
+/* Classic fourth-order Runge-Kutta with fixed dt, in place on u (textbook RK4). */
+int ndo_rk4(ndo_net* nw, double* u, const double* p, double t0, double dt, int64_t nsteps) {
+    int64_t n = nw->lastidx_dynamic;
+    double* k1 = (double*)xcalloc((size_t)n, 8);
+    double* k2 = (double*)xcalloc((size_t)n, 8);
+    double* k3 = (double*)xcalloc((size_t)n, 8);
+    double* k4 = (double*)xcalloc((size_t)n, 8);
+    double* y = (double*)xcalloc((size_t)n, 8);
+    double t = t0;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        ndo_rhs(nw, k1, u, p, t, 1, NULL, NULL);
+        for (int64_t i = 0; i < n; ++i) y[i] = u[i] + 0.5 * dt * k1[i];
+        ndo_rhs(nw, k2, y, p, t + 0.5 * dt, 1, NULL, NULL);
+        for (int64_t i = 0; i < n; ++i) y[i] = u[i] + 0.5 * dt * k2[i];
+        ndo_rhs(nw, k3, y, p, t + 0.5 * dt, 1, NULL, NULL);
+        for (int64_t i = 0; i < n; ++i) y[i] = u[i] + dt * k3[i];
+        ndo_rhs(nw, k4, y, p, t + dt, 1, NULL, NULL);
+        for (int64_t i = 0; i < n; ++i) u[i] = u[i] + dt / 6.0 * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
+        t = t0 + (double)(s + 1) * dt;
+    }
+    free(k1); free(k2); free(k3); free(k4); free(y);
+    return 0;
+}

@@ -1,0 +1,217 @@
+"""GPU parity of nw(du, u, p, t) through the C ABI against the CPU oracle.
+
+Tolerance (FP64): |du_gpu - du_ref| <= 1e-12 * max(|du_ref|, S) per element,
+S = sum of the magnitudes of the element's terms (oracle scale).  Segments of
+at most hub_degree entries are summed in the reference order, so their only
+difference from the oracle is the last ulp of sin; longer segments use a fixed
+wave-wide tree (still far inside 1e-12 * S).
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_for, scaled_error
+
+RTOL = 1e-12
+pytestmark = pytest.mark.gpu
+
+
+def _dev_rhs(nw, u, p, dev, t=0.0):
+    import torch
+    du = torch.full((nw.dim,), float("nan"), dtype=torch.float64, device=dev)
+    nw(du, torch.from_numpy(np.ascontiguousarray(u)).to(dev),
+       torch.from_numpy(np.ascontiguousarray(p if len(p) else np.zeros(1))).to(dev), t)
+    torch.cuda.synchronize()
+    return du.cpu().numpy()
+
+
+def _check(nw, u, p, dev, rtol=RTOL):
+    on = oracle_for(nw)
+    ref = on.rhs(u, p)
+    S = on.scale(u, p)
+    got = _dev_rhs(nw, u, p, dev)
+    assert np.all(np.isfinite(got))
+    err = scaled_error(got, ref, S)
+    assert err.max() <= rtol, f"max scaled error {err.max():.3e} at {err.argmax()}"
+    return got, ref
+
+
+def test_kuramoto_er_c1(ND, dev):
+    # first-order Kuramoto, ER N=1000 E=4000, K=3 (config 1)
+    nw, p, u0 = ND.workloads.kuramoto_er(1000, 4000, 3.0, seed=1)
+    _check(nw, u0, p, dev)
+
+
+def test_swing_ba_20k(ND, dev):
+    nw, p, u0 = ND.workloads.swing_ba(20_000, 2, seed=2)
+    _check(nw, u0, p, dev)
+
+
+def test_swing_ba_hub_thresholds(ND, dev):
+    nw, p, u0 = ND.workloads.swing_ba(20_000, 2, seed=3)
+    on = oracle_for(nw)
+    ref, S = on.rhs(u0, p), on.scale(u0, p)
+    for hub in (1, 8, 64, 1 << 30):
+        nw.tune(hub_degree=hub)
+        got = _dev_rhs(nw, u0, p, dev)
+        assert scaled_error(got, ref, S).max() <= RTOL, hub
+    # hub_degree beyond every segment: reference summation order everywhere; only sin ulps differ
+    nw.tune(hub_degree=1 << 30)
+    got = _dev_rhs(nw, u0, p, dev)
+    assert np.array_equal(got[0::2], ref[0::2])
+
+
+def test_heterogeneous_rgg_c3_small(ND, dev):
+    nw, p, u0 = ND.workloads.heterogeneous_rgg(20_000, seed=4)
+    assert nw.info.nvbatches == 2
+    _check(nw, u0, p, dev)
+
+
+def test_complete4_multibatch_with_ode_edges(ND, dev):
+    from test_oracle import _complete4
+    nw = _complete4(ND)
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=nw.dim)
+    p = rng.uniform(0.5, 2, nw.pdim)
+    _check(nw, u, p, dev)
+
+
+@pytest.mark.parametrize("seed,directed", [(1, True), (2, False)])
+def test_random_mixed_batches(ND, dev, seed, directed):
+    from test_oracle import _random_mixed
+    nw, u, p = _random_mixed(ND, n=3000, seed=seed, directed=directed)
+    _check(nw, u, p, dev)
+
+
+def test_diffusion_closed_form_gpu(ND, dev):
+    import scipy.sparse as sp
+    g = ND.barabasi_albert(2000, 5, seed=2)
+    nw = ND.Network(g, ND.Lib.diffusion_test.odevertex(), ND.Lib.diffusion_test.staticedge())
+    n = g.nv()
+    A = sp.coo_matrix((np.ones(g.ne()), (g.src - 1, g.dst - 1)), shape=(n, n))
+    A = (A + A.T).tocsr()
+    Lap = sp.diags(np.asarray(A.sum(axis=1)).ravel()) - A
+    x = np.random.default_rng(0).normal(size=n)
+    got = _dev_rhs(nw, x, np.zeros(0), dev)
+    np.testing.assert_allclose(got, -(Lap @ x), rtol=0, atol=1e-11)
+
+
+def test_ode_edge_networks(ND, dev):
+    g = ND.watts_strogatz(500, 6, 0.3, seed=1)
+    for em in (ND.Lib.bench.diffusion_dedge(), ND.Lib.diffusion_test.odeedge(), ND.Lib.diffusion_odeedge()):
+        nw = ND.Network(g, ND.Lib.bench.diffusion_vertex(), em)
+        rng = np.random.default_rng(1)
+        _check(nw, rng.normal(size=nw.dim), rng.uniform(1, 2, nw.pdim), dev)
+
+
+def test_benchmark_cases_models(ND, dev):
+    # benchmark/benchmark_cases.jl: kuramoto homogeneous / heterogeneous, diffusion static edge
+    g = ND.watts_strogatz(3000, 4, 0.8, seed=1)
+    rng = np.random.default_rng(2)
+    B = ND.Lib.bench
+    v1, v2 = B.kuramoto_vertex_1d(), B.kuramoto_vertex_2d()
+    vm = [v1 if x else v2 for x in rng.random(3000) < 0.5]
+    for nw in (ND.Network(g, B.kuramoto_vertex_2d(), B.static_kuramoto_edge()),
+               ND.Network(g, vm, B.static_kuramoto_edge()),
+               ND.Network(g, B.diffusion_vertex(), B.diffusion_edge())):
+        _check(nw, rng.normal(size=nw.dim), rng.normal(size=nw.pdim), dev)
+
+
+def test_deterministic_and_inputs_untouched(ND, dev):
+    import torch
+    nw, p, u0 = ND.workloads.swing_ba(50_000, 2, seed=6)
+    nw.tune(hub_degree=8)
+    u = torch.from_numpy(u0).to(dev)
+    pd = torch.from_numpy(p).to(dev)
+    a = torch.empty(nw.dim, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    nw(a, u, pd, 0.0)
+    nw(b, u, pd, 0.0)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    assert np.array_equal(u.cpu().numpy(), u0) and np.array_equal(pd.cpu().numpy(), p)   # testutils.jl:43-44
+
+
+def test_size_and_type_errors(ND, dev):
+    import torch
+    nw, p, u0 = ND.workloads.swing_ba(1000, 2, seed=7)
+    u = torch.from_numpy(u0).to(dev)
+    pd = torch.from_numpy(p).to(dev)
+    du = torch.empty(nw.dim, dtype=torch.float64, device=dev)
+    with pytest.raises(ND.ArgumentError):                      # coreloop.jl:2-4
+        nw(du[:-1], u, pd, 0.0)
+    with pytest.raises(ND.ArgumentError):                      # coreloop.jl:5-7
+        nw(du, u, pd[:-1], 0.0)
+    with pytest.raises(ND.ArgumentError):                      # network_structure.jl:146-151
+        nw(du.float(), u.float(), pd.float(), 0.0)
+
+
+def test_get_buffers_and_primitives(ND, dev):
+    import torch
+    from test_oracle import _random_mixed
+    nw, u, p = _random_mixed(ND, n=2000, seed=8)
+    on = oracle_for(nw)
+    ud, pd = torch.from_numpy(u).to(dev), torch.from_numpy(p).to(dev)
+    o, agg = nw.get_buffers(ud, pd, 0.0)
+    oref, aref = on.buffers(u, p)
+    o, agg = o.cpu().numpy(), agg.cpu().numpy()
+    assert np.array_equal(np.isnan(o), np.isnan(oref))
+    m = ~np.isnan(oref)
+    np.testing.assert_allclose(o[m], oref[m], rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(agg, aref, rtol=1e-12, atol=1e-12)
+    # gather! (gbufs.jl:25) bit-exact
+    ob = torch.from_numpy(np.nan_to_num(oref)).to(dev)
+    gb = torch.empty(nw.info.lastidx_gbuf, dtype=torch.float64, device=dev)
+    nw.gather(gb, ob)
+    torch.cuda.synchronize()
+    assert np.array_equal(gb.cpu().numpy(), on.gather(np.nan_to_num(oref)))
+
+
+def test_aggregate_primitive_depth2(ND, dev):
+    # aggregate! on the mixed depth-2 network of test/aggregators_test.jl: additive, and
+    # bit-exact with SequentialAggregator once every segment is summed in order
+    import torch
+    rng = np.random.default_rng(11)
+    n = 5000
+    g0 = ND.watts_strogatz(n, 4, 0.8, seed=1)
+    flip = rng.random(g0.ne()) < 0.5
+    g = ND.SimpleDiGraph(n, np.where(flip, g0.dst, g0.src), np.where(flip, g0.src, g0.dst))
+    C = ND.components
+    vt = [ND.VertexModel(dim=d, g=[1, 2]) for d in (2, 3, 4)]
+    et = [ND.EdgeModel(g=W(C.noop_edge), outdim=2) for W in (ND.Symmetric, ND.AntiSymmetric, ND.Directed)]
+    et.append(ND.EdgeModel(g=C.noop_edge2, outdim=2))
+    nw = ND.Network(g, [vt[i] for i in rng.integers(0, 3, n)], [et[i] for i in rng.integers(0, 4, g.ne())])
+    on = oracle_for(nw)
+    states = rng.random(on.lastidx_out)
+    ref = on.aggregate(np.zeros(on.lastidx_aggr), states, "seq")
+    sd = torch.from_numpy(states).to(dev)
+    nw.tune(hub_degree=1 << 30)
+    agg = torch.zeros(on.lastidx_aggr, dtype=torch.float64, device=dev)
+    nw.aggregate(agg, sd)
+    torch.cuda.synchronize()
+    assert np.array_equal(agg.cpu().numpy(), ref)
+    off = torch.arange(1, on.lastidx_aggr + 1, dtype=torch.float64, device=dev)
+    nw.tune(hub_degree=2)
+    nw.aggregate(off, sd)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(off.cpu().numpy() - ref, np.arange(1, on.lastidx_aggr + 1), rtol=0, atol=1e-12)
+
+
+def test_swing_c2_full_size_properties(ND, dev):
+    # config 2 at full size (N = 1e6): size-independent checks on the GPU result
+    import torch
+    nw, p, u0 = ND.workloads.swing_ba(1_000_000, 2, seed=1)
+    n = nw.nv()
+    got = _dev_rhs(nw, u0, p, dev)
+    assert np.all(np.isfinite(got))
+    assert np.array_equal(got[0::2], u0[1::2])               # dtheta = omega exactly
+    M, D, P = p[0:3 * n:3], p[1:3 * n:3], p[2:3 * n:3]
+    w = u0[1::2]
+    # sum_i M_i dw_i = sum_i P_i - D_i w_i: the coupling cancels pairwise (AntiSymmetric)
+    lhs = np.sum(M * got[1::2])
+    rhs = np.sum(P - D * w)
+    scale = np.sum(np.abs(M * got[1::2])) + np.sum(np.abs(P)) + np.sum(np.abs(D * w))
+    assert abs(lhs - rhs) <= 1e-12 * scale
+    # sampled vertices against the oracle restated on their neighbourhoods: full-size oracle run
+    on = oracle_for(nw)
+    ref, S = on.rhs(u0, p), on.scale(u0, p)
+    assert scaled_error(got, ref, S).max() <= RTOL

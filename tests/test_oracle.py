@@ -1,0 +1,221 @@
+"""Pins the CPU oracle (oracle/nd_oracle.c) before it is trusted as the checker.
+
+Each check is chosen so a plausible restatement mistake (wrong registration
+order, swapped src/dst, dropped sign, wrong summation order) fails it:
+known answers from the reference's own tests, the hand-derived layout of the
+reference's complete_graph(4) network, cross-style / cross-aggregator
+agreement, closed forms, dense brute force and conservation invariants.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import oracle_for, scaled_error
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_batchstride_known_answers(oracle):
+    # test/utils_test.jl:147-178
+    assert oracle.batchstride_fullstride([3]) == 3
+    assert oracle.batchstride_fullstride([3, 1]) == 4
+    assert oracle.batchstride_range(2, [3], 1) == (2, 4)
+    assert oracle.batchstride_range(2, [3], 2) == (5, 7)
+    assert oracle.batchstride_range(2, [3, 1], 1, 1) == (2, 4)
+    assert oracle.batchstride_range(2, [3, 1], 1, 2) == (5, 5)
+    assert oracle.batchstride_range(2, [3, 1], 2, 1) == (6, 8)
+    assert oracle.batchstride_range(2, [3, 1], 2, 2) == (9, 9)
+
+
+def _complete4(ND):
+    g = ND.complete_graph(4)
+    L = ND.Lib
+    vf = [L.kuramoto_second(), L.diffusion_vertex(), L.kuramoto_second(), L.diffusion_vertex()]
+    ef = [L.diffusion_odeedge(), L.kuramoto_edge(), L.kuramoto_edge(), L.diffusion_edge_fid(),
+          L.diffusion_odeedge(), L.diffusion_edge_fid()]
+    return ND.Network(g, vf, ef)
+
+
+def test_complete4_layout_matches_hand_derivation(ND):
+    fx = json.load(open(os.path.join(GOLDEN, "complete4_layout.json")))
+    on = oracle_for(_complete4(ND))
+    assert (on.dim, on.pdim, on.lastidx_out, on.lastidx_aggr, on.lastidx_gbuf) == (
+        fx["dim"], fx["pdim"], fx["lastidx_out"], fx["lastidx_aggr"], fx["lastidx_gbuf"])
+    v, e = on.ranges()
+    for j, name in enumerate(["v_data", "v_out", "v_para", "v_aggr"]):
+        assert v[:, j, :].tolist() == fx[name], name
+    for j, name in enumerate(["e_data", "e_out_src", "e_out_dst", "e_para", "e_gbuf_src", "e_gbuf_dst"]):
+        assert e[:, j, :].tolist() == fx[name], name
+    a, gm = on.maps()
+    assert [on.agg_first, on.agg_last] == fx["aggregation_range"]
+    assert a.tolist() == fx["aggregation_map"]
+    assert gm.tolist() == fx["gather_map"]
+    d, ptr, s = on.inverse_map()
+    inv = [[int(d[w]), s[ptr[w]:ptr[w + 1]].tolist()] for w in range(len(d))]
+    assert inv == fx["inverse_map"]
+    assert [sorted(on.batch(False, b)["indices"].tolist()) for b in range(on.nvb)] == fx["vertex_batches"]
+    assert [on.batch(True, b)["indices"].tolist() for b in range(on.neb)] == fx["edge_batches"]
+    # variable_index(EIndex(1, :e_dst)) == 7  (sym = [:e_dst, :e_src])
+    assert e[0, 0, 0] == fx["e_dst_state_of_edge1"]
+
+
+def _random_mixed(ND, n=300, seed=3, directed=True):
+    """Random mix of wrappers / batches like test/aggregators_test.jl:21-30."""
+    rng = np.random.default_rng(seed)
+    L = ND.Lib
+    g = ND.watts_strogatz(n, 4, 0.8, seed=seed)
+    if directed:
+        flip = rng.random(g.ne()) < 0.5
+        s = np.where(flip, g.dst, g.src)
+        d = np.where(flip, g.src, g.dst)
+        g = ND.SimpleDiGraph(n, s, d)
+    vt = [L.kuramoto_second(), L.kuramoto_first(), L.diffusion_vertex(), L.load_vertex()]
+    et = [L.kuramoto_edge(), L.diffusion_edge(), L.diffusion_edge_fid(), L.diffusion_odeedge(),
+          L.static_line_edge(),
+          ND.EdgeModel(g=ND.Symmetric(ND.components.kuramoto_edge_g), pdim=1),
+          ND.EdgeModel(g=ND.Directed(ND.components.diffusionedge_weighted), pdim=1)]
+    vm = [vt[i] for i in rng.integers(0, len(vt), n)]
+    em = [et[i] for i in rng.integers(0, len(et), g.ne())]
+    nw = ND.Network(g, vm, em)
+    u = rng.normal(size=nw.dim)
+    p = rng.uniform(0.5, 2.0, nw.pdim)
+    return nw, u, p
+
+
+def test_eager_equals_lazy_bitwise(ND):
+    # Eager vs Lazy GBufProvider agree (test/testutils.jl:10-64)
+    nw, u, p = _random_mixed(ND)
+    on = oracle_for(nw)
+    a = on.rhs(u, p, 0.0, buffered=True)
+    b = on.rhs(u, p, 0.0, buffered=False)
+    assert np.array_equal(a, b)
+
+
+def test_aggregators_agree_and_are_additive(ND):
+    # test/aggregators_test.jl:58-79 on a mixed network with output depth 2
+    rng = np.random.default_rng(11)
+    n = 2000
+    g0 = ND.watts_strogatz(n, 4, 0.8, seed=1)
+    flip = rng.random(g0.ne()) < 0.5
+    g = ND.SimpleDiGraph(n, np.where(flip, g0.dst, g0.src), np.where(flip, g0.src, g0.dst))
+    C = ND.components
+    vt = [ND.VertexModel(dim=d, g=[1, 2]) for d in (2, 3, 4)]
+    et = [ND.EdgeModel(g=W(C.noop_edge), outdim=2) for W in (ND.Symmetric, ND.AntiSymmetric, ND.Directed)]
+    et.append(ND.EdgeModel(g=C.noop_edge2, outdim=2))   # Fiducial(gss, gss): two independent outputs
+    nw = ND.Network(g, [vt[i] for i in rng.integers(0, 3, n)], [et[i] for i in rng.integers(0, 4, g.ne())])
+    on = oracle_for(nw)
+    states = rng.random(on.lastidx_out)
+    res = [on.aggregate(np.zeros(on.lastidx_aggr), states, m) for m in ("seq", "inv", "naive")]
+    assert np.array_equal(res[0], res[1]) and np.array_equal(res[0], res[2])
+    off = np.arange(1, on.lastidx_aggr + 1, dtype=float)
+    got = on.aggregate(off.copy(), states, "seq")
+    np.testing.assert_allclose(got - res[0], off, rtol=0, atol=1e-12)
+    # threaded restatement == sequential, bitwise
+    assert np.array_equal(on.aggregate(np.zeros(on.lastidx_aggr), states, "inv", nthreads=4), res[0])
+
+
+def test_threaded_equals_sequential_bitwise(ND):
+    nw, u, p = _random_mixed(ND, n=500, directed=False)
+    on = oracle_for(nw)
+    assert np.array_equal(on.rhs(u, p), on.rhs_threaded(u, p, nthreads=4))
+
+
+def test_diffusion_closed_form(ND):
+    # du = -L x (test/diffusion_test.jl:33, 80-90)
+    g = ND.barabasi_albert(200, 5, seed=2)
+    nw = ND.Network(g, ND.Lib.diffusion_test.odevertex(), ND.Lib.diffusion_test.staticedge())
+    on = oracle_for(nw)
+    n = g.nv()
+    A = sp.coo_matrix((np.ones(g.ne()), (g.src - 1, g.dst - 1)), shape=(n, n))
+    A = (A + A.T).tocsr()
+    Lap = sp.diags(np.asarray(A.sum(axis=1)).ravel()) - A
+    rng = np.random.default_rng(0)
+    for _ in range(30):
+        x = rng.normal(size=n)
+        du = on.rhs(x, np.zeros(0))
+        np.testing.assert_allclose(du, -(Lap @ x), rtol=0, atol=1e-12)
+
+
+def test_kuramoto_incidence_closed_form(ND):
+    # dtheta = omega - B (K o sin(B^T theta)) with oriented incidence B (oldexamples/paper.jl:10-11)
+    g = ND.erdos_renyi(300, 1200, seed=5)
+    nw = ND.Network(g, ND.Lib.kuramoto_first(), ND.Lib.kuramoto_edge())
+    on = oracle_for(nw)
+    n, m = g.nv(), g.ne()
+    rng = np.random.default_rng(1)
+    th = rng.uniform(-np.pi, np.pi, n)
+    om = rng.normal(size=n)
+    K = rng.uniform(1, 3, m)
+    B = sp.coo_matrix((np.r_[np.ones(m), -np.ones(m)], (np.r_[g.src - 1, g.dst - 1], np.r_[np.arange(m), np.arange(m)])),
+                      shape=(n, m)).tocsr()
+    ref = om - B @ (K * np.sin(B.T @ th))
+    du = on.rhs(th, np.r_[om, K])
+    S = on.scale(th, np.r_[om, K])
+    assert scaled_error(du, ref, S).max() < 1e-14
+
+
+def test_swing_dense_bruteforce_and_invariant(ND):
+    g = ND.barabasi_albert(60, 2, seed=9)
+    nw = ND.Network(g, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    on = oracle_for(nw)
+    n = g.nv()
+    rng = np.random.default_rng(2)
+    M, D, P, K = rng.uniform(1, 5, n), rng.uniform(0.1, 1, n), rng.normal(size=n), rng.uniform(5, 10, g.ne())
+    p = np.empty(nw.pdim)
+    p[0:3 * n:3], p[1:3 * n:3], p[2:3 * n:3], p[3 * n:] = M, D, P, K
+    u = np.empty(2 * n)
+    u[0::2], u[1::2] = rng.uniform(-np.pi, np.pi, n), rng.normal(size=n)
+    du = on.rhs(u, p)
+    Kd = np.zeros((n, n))
+    Kd[g.src - 1, g.dst - 1] = K
+    Kd = Kd + Kd.T
+    th, w = u[0::2], u[1::2]
+    acc = np.array([sum(Kd[i, j] * np.sin(th[j] - th[i]) for j in range(n) if Kd[i, j] != 0) for i in range(n)])
+    np.testing.assert_array_equal(du[0::2], w)
+    np.testing.assert_allclose(du[1::2], (P - D * w + acc) / M, rtol=1e-13, atol=1e-13)
+    # sum_i M_i dw_i = sum P - sum D w  (antisymmetric coupling cancels)
+    lhs = np.sum(M * du[1::2])
+    assert abs(lhs - (P.sum() - np.sum(D * w))) < 1e-11 * np.sum(np.abs(M * du[1::2]) + 1)
+
+
+def test_swing_ring_fixpoint(ND):
+    # test/initialization_test.jl:15-38: ring of 4, P = [1,-1,1,-1], K = 6: each line carries 0.5
+    g = ND.cycle_graph(4)
+    nw = ND.Network(g, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    on = oracle_for(nw)
+    d = np.arcsin(0.5 / 6)
+    u = np.array([d, 0, 0, 0, d, 0, 0, 0], float)
+    p = np.r_[np.tile([1.0, 0.1, 0.0], 4), np.full(4, 6.0)]
+    p[2::3][:4] = [1, -1, 1, -1]
+    du = on.rhs(u, p)
+    assert np.abs(du).max() < 1e-14
+    u2 = u.copy()
+    u2[0] += 1e-3
+    assert np.abs(on.rhs(u2, p)).max() > 1e-4
+
+
+def test_size_checks(ND):
+    nw, u, p = _random_mixed(ND, n=50)
+    on = oracle_for(nw)
+    assert on.check_sizes(nw.dim, nw.dim, nw.pdim) == 0
+    assert on.check_sizes(nw.dim - 1, nw.dim, nw.pdim) == 1
+    assert on.check_sizes(nw.dim, nw.dim, nw.pdim + 1) == 2
+    with pytest.raises(ValueError):
+        on.rhs(u[:-1], p)
+
+
+def test_rk4_order_of_convergence(ND):
+    # textbook RK4: global error ~ dt^4
+    g = ND.erdos_renyi(50, 120, seed=1)
+    nw = ND.Network(g, ND.Lib.kuramoto_first(), ND.Lib.kuramoto_edge())
+    on = oracle_for(nw)
+    rng = np.random.default_rng(4)
+    p = np.r_[rng.normal(size=50), np.full(120, 1.0)]
+    u0 = rng.uniform(-np.pi, np.pi, 50)
+    ref = on.rk4(u0, p, 0.0, 1e-3, 1000)
+    e1 = np.abs(on.rk4(u0, p, 0.0, 0.04, 25) - ref).max()
+    e2 = np.abs(on.rk4(u0, p, 0.0, 0.02, 50) - ref).max()
+    assert 12 < e1 / e2 < 20
