@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="also run one device copy of this many MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the single-GPU "
@@ -87,6 +89,14 @@ def main():
     pd = torch.from_numpy(p).to(dev)
     du = torch.empty(nw.dim, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev)
+
+    if args.calibrate_copy_mb:
+        a = torch.ones(args.calibrate_copy_mb * 2**20 // 8, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     for _ in range(args.warmup):
         nw(du, u, pd, 0.0)

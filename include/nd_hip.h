@@ -189,8 +189,9 @@ void nd_plan_destroy(nd_plan* plan);
 
 /* Tuning knobs of a plan (0 = keep current). */
 typedef struct {
-    int32_t hub_degree;      /* segments longer than this go to the wave-per-vertex path */
-    int32_t block_size;      /* threads per block of the vertex kernel                   */
+    int32_t hub_degree;      /* segments longer than this: one 64-lane wave per vertex    */
+    int32_t block_size;      /* threads per block of the vertex kernel (64..256)         */
+    int32_t big_hub_degree;  /* segments longer than this: one block per vertex          */
 } nd_plan_tuning;
 int nd_plan_tune(nd_plan* plan, const nd_plan_tuning* t);
 int nd_plan_get_tuning(const nd_plan* plan, nd_plan_tuning* t);
@@ -200,6 +201,16 @@ int nd_plan_get_tuning(const nd_plan* plan, nd_plan_tuning* t);
  * pdim > 0) -> ND_ERR_ARGUMENT.  Asynchronous on `stream`. */
 int nd_rhs(nd_plan* plan, double* du, const double* u, const double* p, double t,
            int64_t len_du, int64_t len_u, int64_t len_p, void* stream);
+
+/* Fixed-step classic Runge-Kutta 4, in place on u (the "plain method" stepper that
+ * replaces an OrdinaryDiffEq RK4 solve of ODEProblem(nw, u0, tspan, p),
+ * src/post_utils.jl:43-82).  Each stage is one fused RHS launch whose epilogue
+ * forms the stage state and the weighted derivative sum, in the order
+ * u + dt/6*(((k1 + 2k2) + 2k3) + k4).  work: 3*dim doubles of device scratch, or
+ * NULL for a plan-owned buffer.  The library's models are autonomous, so t0 only
+ * documents the time axis. */
+int nd_rk4(nd_plan* plan, double* u, const double* p, double t0, double dt, int64_t nsteps, double* work,
+           int64_t len_u, int64_t len_p, void* stream);
 
 /* get_buffers(nw, u, p, t; initbufs=true), coreloop.jl:103-109: fills the
  * output buffer o[lastidx_out] and the aggregation buffer aggbuf[lastidx_aggr]. */

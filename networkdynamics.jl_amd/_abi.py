@@ -57,7 +57,7 @@ class NDBatch(C.Structure):
 
 
 class NDPlanTuning(C.Structure):
-    _fields_ = [("hub_degree", C.c_int32), ("block_size", C.c_int32)]
+    _fields_ = [("hub_degree", C.c_int32), ("block_size", C.c_int32), ("big_hub_degree", C.c_int32)]
 
 
 P = C.c_void_p
@@ -81,6 +81,7 @@ PROTOTYPES = {
     "nd_plan_get_tuning": (C.c_int, [P, C.POINTER(NDPlanTuning)]),
     "nd_rhs": (C.c_int, [P, P, P, P, D, I64, I64, I64, P]),
     "nd_get_buffers": (C.c_int, [P, P, P, P, P, D, P]),
+    "nd_rk4": (C.c_int, [P, P, P, D, D, I64, P, I64, I64, P]),
     "nd_gather": (C.c_int, [P, P, P, P]),
     "nd_aggregate": (C.c_int, [P, P, P, P]),
     "nd_graph_erdos_renyi": (C.c_int, [I64, I64, C.c_uint64, C.POINTER(P)]),

@@ -42,9 +42,12 @@ struct VDesc {            // one vertex batch as the kernels see it (0-based off
     int32_t model, dim, pdim, depth;
     int32_t state0, param0, row0, count;
     int32_t out0;         // o index of the first vertex output (get_buffers)
-    int32_t nhubs;
+    int32_t nbig, nmid;   // hub vertices: block-per-vertex / wave-per-vertex classes
     int32_t mask[ND_MAX_OUT];
-    const int32_t* hubs;  // positions q of hub vertices
+    const int32_t* hubs;  // positions q of hub vertices: nbig big ones, then nmid mid ones
+    const int32_t* lrp;   // hub-free destination lists: count+1 offsets into lslots
+    const int2* lslots;
+    const uint8_t* lseb;  // edge batch of each lslots entry (several edge batches only)
 };
 
 struct EDesc {            // one edge batch
@@ -151,74 +154,258 @@ __device__ __forceinline__ void vertex_f(double* __restrict__ dx, const double* 
     }
 }
 
+// Explicit Runge-Kutta stage epilogue fused into the RHS kernels.  kind 0: plain
+// du = f(u).  For a stage derivative k at index i:
+//   kind 1 (first stage):  acc[i] = k;             yout[i] = base[i] + h*k
+//   kind 2 (middle stage): acc[i] = acc[i] + w*k;  yout[i] = base[i] + h*k
+//   kind 3 (final stage):  yout[i] = base[i] + h*(acc[i] + k)
+// With h, w chosen on the host this is the classic RK4 in the operation order
+// u + dt/6*(((k1 + 2k2) + 2k3) + k4).
+struct StageArgs {
+    const double* base;
+    double* acc;
+    double* yout;
+    double h, w;
+    int32_t kind;
+};
+
+template <bool STAGE>
+__device__ __forceinline__ void emit1(double* __restrict__ du, const StageArgs& sa, int64_t i, double k) {
+    if constexpr (!STAGE) {
+        du[i] = k;
+    } else {
+        if (sa.kind == 3) {
+            sa.yout[i] = sa.base[i] + sa.h * (sa.acc[i] + k);
+        } else {
+            sa.acc[i] = sa.kind == 1 ? k : sa.acc[i] + sa.w * k;
+            sa.yout[i] = sa.base[i] + sa.h * k;
+        }
+    }
+}
+
+template <int VM, bool STAGE>
+__device__ __forceinline__ void vertex_emit(double* __restrict__ du, const StageArgs& sa, int64_t x0,
+                                            const double* __restrict__ x, double acc,
+                                            const double* __restrict__ pv, int dim) {
+    constexpr int DIMC = VModel<VM>::dim;
+    if constexpr (DIMC > 0) {
+        double dx[DIMC];
+        vertex_f<VM>(dx, x, acc, pv, DIMC);
+#pragma unroll
+        for (int j = 0; j < DIMC; ++j) emit1<STAGE>(du, sa, x0 + j, dx[j]);
+    } else {
+        for (int j = 0; j < dim; ++j) emit1<STAGE>(du, sa, x0 + j, 0.0);
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
     return v;
 }
 
-// Fused RHS for one vertex batch.  Blocks [0, nlow) take one vertex per thread
-// (segments up to hub_degree, summed in reference order); blocks [nlow, ...) take
-// one hub vertex per wave.
-template <int VM, int EM, int WRAP, bool MULTI>
+// Parameters each edge model reads (p entries per edge).
+template <int EM> struct EModel { static constexpr int np = 0; };
+template <> struct EModel<ND_EM_KURAMOTO> { static constexpr int np = 1; };
+template <> struct EModel<ND_EM_STATIC_LINE> { static constexpr int np = 2; };
+template <> struct EModel<ND_EM_DIFFUSION_WEIGHTED> { static constexpr int np = 1; };
+template <> struct EModel<ND_EM_DIFFUSION_FIDUCIAL> { static constexpr int np = 1; };
+
+// Entry value from operands already in registers (nbr: u[s.x]; k: the edge parameters).
+template <int EM, int WRAP>
+__device__ __forceinline__ double slot_eval(int2 s, double nbr, const double* k, double own) {
+    if constexpr (EM == ND_EM_NOOP) {
+        return __builtin_nan("");
+    } else if constexpr (is_ode_model(EM)) {
+        return nbr;
+    } else {
+        if (s.y & 1) {
+            const double y = g_dst<EM>(own, nbr, k);
+            return WRAP == ND_WRAP_SYMMETRIC ? y : -y;
+        }
+        return g_dst<EM>(nbr, own, k);
+    }
+}
+
+constexpr int kUnroll = 4;   // entries whose loads are issued together per thread
+
+// Partial sum of the entries beg+t, beg+t+G, beg+t+2G, ... of one segment, in that
+// order.  Loads of kUnroll entries are issued before any of them is used, so a
+// thread keeps several gathers in flight instead of one dependent chain.
+template <int EM, int WRAP, bool MULTI>
+__device__ __forceinline__ double segment_sum(int32_t beg, int32_t end, int32_t t, int32_t G, double own,
+                                              const int2* __restrict__ slots, const uint8_t* __restrict__ slot_eb,
+                                              const EDesc* __restrict__ ed, const double* __restrict__ u,
+                                              const double* __restrict__ p) {
+    double acc = 0.0;
+    if constexpr (MULTI) {
+        for (int32_t s = beg + t; s < end; s += G) acc += slot_value_rt(ed[slot_eb[s]], slots[s], own, u, p);
+    } else {
+        constexpr int NP = EModel<EM>::np;
+        for (int32_t s0 = beg + t; s0 < end; s0 += kUnroll * G) {
+            int2 sl[kUnroll];
+            double nb[kUnroll];
+            double kk[kUnroll][NP > 0 ? NP : 1];
+#pragma unroll
+            for (int j = 0; j < kUnroll; ++j)
+                if (s0 + j * G < end) sl[j] = slots[s0 + j * G];
+#pragma unroll
+            for (int j = 0; j < kUnroll; ++j)
+                if (s0 + j * G < end) {
+                    if constexpr (EM != ND_EM_NOOP) nb[j] = u[sl[j].x];
+#pragma unroll
+                    for (int c = 0; c < NP; ++c) kk[j][c] = p[(sl[j].y >> 1) + c];
+                }
+#pragma unroll
+            for (int j = 0; j < kUnroll; ++j)
+                if (s0 + j * G < end) acc += slot_eval<EM, WRAP>(sl[j], nb[j], kk[j], own);
+        }
+    }
+    return acc;
+}
+
+// Fused RHS for one vertex batch.  Block layout (hubs first, so the longest
+// segments start earliest):
+//   [0, nbig)                 one block per big hub (segment > big_hub_degree),
+//   [nbig, nbig + nmidb)      one wave per mid hub (segment > hub_degree),
+//   [nbig + nmidb, ...)       one vertex per thread, segment summed in reference order.
+// Hub segments: per-thread partial sums in a fixed order, xor-butterfly per wave,
+// wave partials combined in wave order -- deterministic run to run.
+template <int VM, int EM, int WRAP, bool MULTI, bool STAGE>
 __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __restrict__ rowptr,
                                                     const int2* __restrict__ slots,
                                                     const uint8_t* __restrict__ slot_eb,
                                                     const EDesc* __restrict__ ed,
                                                     const double* __restrict__ u, const double* __restrict__ p,
-                                                    double* __restrict__ du, int32_t hub_degree, int32_t nlow) {
+                                                    double* __restrict__ du, StageArgs sa) {
     constexpr int DIMC = VModel<VM>::dim;
     const int dim = DIMC > 0 ? DIMC : vd.dim;
-    if (static_cast<int32_t>(blockIdx.x) < nlow) {
-        const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
-        if (q >= vd.count) return;
-        const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * dim;
-        if constexpr (VM == ND_VM_NOOP) {
-            vertex_f<VM>(du + x0, u + x0, 0.0, p, dim);
-        } else {
-            const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
-            if (end - beg > hub_degree) return;
-            const double own = u[x0 + vd.mask[0]];
-            double acc = 0.0;
-            for (int32_t s = beg; s < end; ++s) {
-                const int2 sl = slots[s];
-                if constexpr (MULTI) acc += slot_value_rt(ed[slot_eb[s]], sl, own, u, p);
-                else acc += slot_value<EM, WRAP>(sl, own, u, p);
-            }
-            vertex_f<VM>(du + x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim);
-        }
-    } else {
-        if constexpr (VM != ND_VM_NOOP) {
-            const int32_t w = static_cast<int32_t>((blockIdx.x - nlow) * (blockDim.x / kWave) + threadIdx.x / kWave);
-            const int lane = static_cast<int>(threadIdx.x & (kWave - 1));
-            if (w >= vd.nhubs) return;
-            const int32_t q = vd.hubs[w];
+    const int32_t wpb = static_cast<int32_t>(blockDim.x / kWave);
+    const int32_t nmidb = (vd.nmid + wpb - 1) / wpb;
+    int32_t b = static_cast<int32_t>(blockIdx.x);
+    if constexpr (VM != ND_VM_NOOP) {
+        if (b < vd.nbig) {
+            __shared__ double wpart[256 / kWave];
+            const int32_t q = vd.hubs[b];
             const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * dim;
             const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
             const double own = u[x0 + vd.mask[0]];
-            double part = 0.0;
-            for (int32_t s = beg + lane; s < end; s += kWave) {
-                const int2 sl = slots[s];
-                if constexpr (MULTI) part += slot_value_rt(ed[slot_eb[s]], sl, own, u, p);
-                else part += slot_value<EM, WRAP>(sl, own, u, p);
+            const int32_t t = static_cast<int32_t>(threadIdx.x);
+            const double part = wave_sum(segment_sum<EM, WRAP, MULTI>(beg, end, t, static_cast<int32_t>(blockDim.x),
+                                                                      own, slots, slot_eb, ed, u, p));
+            if ((t & (kWave - 1)) == 0) wpart[t / kWave] = part;
+            __syncthreads();
+            if (t == 0) {
+                double acc = wpart[0];
+                for (int32_t w = 1; w < wpb; ++w) acc += wpart[w];
+                vertex_emit<VM, STAGE>(du, sa, x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim);
             }
-            const double acc = wave_sum(part);
-            if (lane == 0) vertex_f<VM>(du + x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim);
+            return;
         }
+        b -= vd.nbig;
+        if (b < nmidb) {
+            const int32_t w = b * wpb + static_cast<int32_t>(threadIdx.x / kWave);
+            const int32_t lane = static_cast<int32_t>(threadIdx.x & (kWave - 1));
+            if (w >= vd.nmid) return;
+            const int32_t q = vd.hubs[vd.nbig + w];
+            const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * dim;
+            const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
+            const double own = u[x0 + vd.mask[0]];
+            const double acc = wave_sum(segment_sum<EM, WRAP, MULTI>(beg, end, lane, kWave, own, slots, slot_eb,
+                                                                     ed, u, p));
+            if (lane == 0) vertex_emit<VM, STAGE>(du, sa, x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim);
+            return;
+        }
+        b -= nmidb;
+    }
+    // Low-degree vertices: block b takes the T consecutive vertices q0..q0+T-1 and their
+    // hub-free destination lists lrp/lslots.  Entries are evaluated lane-densely in LDS
+    // windows of 4*T (loads issued together, no lane idles on a neighbour's long
+    // segment); then each thread sums its own segment from LDS in order, so the
+    // summation order is the reference's.
+    const int32_t T = static_cast<int32_t>(blockDim.x);
+    const int32_t t = static_cast<int32_t>(threadIdx.x);
+    const int32_t q0 = b * T;
+    const int32_t q = q0 + t;
+    if (q0 >= vd.count) return;
+    const bool valid = q < vd.count;
+    const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(valid ? q : q0) * dim;
+    if constexpr (VM == ND_VM_NOOP) {
+        if (valid) vertex_emit<VM, STAGE>(du, sa, x0, u + x0, 0.0, p, dim);
+    } else {
+        __shared__ double s_val[4 * 256];
+        __shared__ double s_own[256];
+        __shared__ uint8_t s_owner[4 * 256];
+        const int32_t* __restrict__ lrp = vd.lrp;
+        // lrp[q] = offset | (hub ? 1 << 31 : 0): hub vertices keep an empty segment here and
+        // their du is written by the hub paths
+        constexpr int32_t kOff = 0x7fffffff;
+        const int32_t qc = valid ? q : vd.count;
+        const int32_t raw = lrp[qc];
+        const bool hub = raw < 0;
+        const int32_t sb = raw & kOff;
+        const int32_t se = valid ? (lrp[qc + 1] & kOff) : sb;
+        const int32_t S0 = lrp[q0] & kOff;
+        const int32_t S1 = lrp[min(q0 + T, vd.count)] & kOff;
+        const double own = valid ? u[x0 + vd.mask[0]] : 0.0;
+        s_own[t] = own;
+        double acc = 0.0;
+        const int32_t W = 4 * T;
+        for (int32_t base = S0; base < S1; base += W) {
+            const int32_t wend = min(base + W, S1);
+            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) s_owner[s2 - base] = static_cast<uint8_t>(t);
+            __syncthreads();
+            constexpr int NP = EModel<EM>::np;
+            int2 sl[4];
+            double nb[4];
+            double kk[4][NP > 0 ? NP : 1];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int32_t s2 = base + j * T + t;
+                if (s2 < wend) sl[j] = vd.lslots[s2];
+            }
+            if constexpr (!MULTI) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int32_t s2 = base + j * T + t;
+                    if (s2 < wend) {
+                        if constexpr (EM != ND_EM_NOOP) nb[j] = u[sl[j].x];
+#pragma unroll
+                        for (int c = 0; c < NP; ++c) kk[j][c] = p[(sl[j].y >> 1) + c];
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int32_t s2 = base + j * T + t;
+                if (s2 < wend) {
+                    const double ow = s_own[s_owner[s2 - base]];
+                    double v;
+                    if constexpr (MULTI) v = slot_value_rt(ed[vd.lseb[s2]], sl[j], ow, u, p);
+                    else v = slot_eval<EM, WRAP>(sl[j], nb[j], kk[j], ow);
+                    s_val[s2 - base] = v;
+                }
+            }
+            __syncthreads();
+            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) acc += s_val[s2 - base];
+            __syncthreads();
+        }
+        if (valid && !hub) vertex_emit<VM, STAGE>(du, sa, x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim);
     }
 }
 
 // du of ODE edges (coreloop.jl:76, edge f without feed-forward)
-template <int EM>
+template <int EM, bool STAGE>
 __global__ void __launch_bounds__(256) k_edge_ode(EDesc e, const double* __restrict__ u,
-                                                  const double* __restrict__ p, double* __restrict__ du) {
+                                                  const double* __restrict__ p, double* __restrict__ du,
+                                                  StageArgs sa) {
     const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= e.count) return;
     const int2 en = e.ends[q];
     const double vs = u[en.x], vd = u[en.y];
     const int64_t x0 = static_cast<int64_t>(e.state0) + static_cast<int64_t>(q) * 2;
     const double* x = u + x0;
-    double* dx = du + x0;
+    double dx[2];
     if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION) {            // ComponentLibrary.jl:30-34
         const double* pe = p + e.param0 + static_cast<int64_t>(q) * e.pdim;
         dx[0] = 1.0 / pe[0] * (sin(vs - vd) - x[0]);
@@ -230,6 +417,8 @@ __global__ void __launch_bounds__(256) k_edge_ode(EDesc e, const double* __restr
         dx[0] = vs - vd - x[0];
         dx[1] = vd - vs - x[1];
     }
+    emit1<STAGE>(du, sa, x0, dx[0]);
+    emit1<STAGE>(du, sa, x0 + 1, dx[1]);
 }
 
 __global__ void k_fill(double* __restrict__ x, int64_t n, double v) {
@@ -309,54 +498,60 @@ __global__ void __launch_bounds__(256) k_aggregate(const int32_t* __restrict__ r
 // ---- kernel selection ----------------------------------------------------------
 
 using VertexKernel = void (*)(VDesc, const int32_t*, const int2*, const uint8_t*, const EDesc*, const double*,
-                              const double*, double*, int32_t, int32_t);
+                              const double*, double*, StageArgs);
 
-template <int VM, int EM, int WRAP>
+template <int VM, int EM, int WRAP, bool STAGE>
 VertexKernel pick3(bool multi) {
-    return multi ? k_vertex_rhs<VM, ND_EM_NOOP, ND_WRAP_NONE, true> : k_vertex_rhs<VM, EM, WRAP, false>;
+    return multi ? k_vertex_rhs<VM, ND_EM_NOOP, ND_WRAP_NONE, true, STAGE> : k_vertex_rhs<VM, EM, WRAP, false, STAGE>;
 }
 
-template <int VM>
+template <int VM, bool STAGE>
 VertexKernel pick_edge(int em, int wrap, bool multi) {
-    if (multi) return k_vertex_rhs<VM, ND_EM_NOOP, ND_WRAP_NONE, true>;
+    if (multi) return k_vertex_rhs<VM, ND_EM_NOOP, ND_WRAP_NONE, true, STAGE>;
     const bool sym = wrap == ND_WRAP_SYMMETRIC;
     switch (em) {
     case ND_EM_KURAMOTO:
-        return sym ? pick3<VM, ND_EM_KURAMOTO, ND_WRAP_SYMMETRIC>(false) : pick3<VM, ND_EM_KURAMOTO, ND_WRAP_ANTISYMMETRIC>(false);
+        return sym ? pick3<VM, ND_EM_KURAMOTO, ND_WRAP_SYMMETRIC, STAGE>(false)
+                   : pick3<VM, ND_EM_KURAMOTO, ND_WRAP_ANTISYMMETRIC, STAGE>(false);
     case ND_EM_STATIC_LINE:
-        return sym ? pick3<VM, ND_EM_STATIC_LINE, ND_WRAP_SYMMETRIC>(false) : pick3<VM, ND_EM_STATIC_LINE, ND_WRAP_ANTISYMMETRIC>(false);
+        return sym ? pick3<VM, ND_EM_STATIC_LINE, ND_WRAP_SYMMETRIC, STAGE>(false)
+                   : pick3<VM, ND_EM_STATIC_LINE, ND_WRAP_ANTISYMMETRIC, STAGE>(false);
     case ND_EM_DIFFUSION:
-        return sym ? pick3<VM, ND_EM_DIFFUSION, ND_WRAP_SYMMETRIC>(false) : pick3<VM, ND_EM_DIFFUSION, ND_WRAP_ANTISYMMETRIC>(false);
+        return sym ? pick3<VM, ND_EM_DIFFUSION, ND_WRAP_SYMMETRIC, STAGE>(false)
+                   : pick3<VM, ND_EM_DIFFUSION, ND_WRAP_ANTISYMMETRIC, STAGE>(false);
     case ND_EM_DIFFUSION_WEIGHTED:
-        return sym ? pick3<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_SYMMETRIC>(false) : pick3<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_ANTISYMMETRIC>(false);
+        return sym ? pick3<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_SYMMETRIC, STAGE>(false)
+                   : pick3<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_ANTISYMMETRIC, STAGE>(false);
     case ND_EM_DIFFUSION_FIDUCIAL:
-        return pick3<VM, ND_EM_DIFFUSION_FIDUCIAL, ND_WRAP_ANTISYMMETRIC>(false);
+        return pick3<VM, ND_EM_DIFFUSION_FIDUCIAL, ND_WRAP_ANTISYMMETRIC, STAGE>(false);
     case ND_EM_ODE_SIN_DIFFUSION:
     case ND_EM_ODE_SIN_DIFFUSION_FAST:
     case ND_EM_ODE_LINEAR:
-        return pick3<VM, ND_EM_ODE_LINEAR, ND_WRAP_FIDUCIAL>(false);
+        return pick3<VM, ND_EM_ODE_LINEAR, ND_WRAP_FIDUCIAL, STAGE>(false);
     default:
-        return pick3<VM, ND_EM_NOOP, ND_WRAP_NONE>(false);
+        return pick3<VM, ND_EM_NOOP, ND_WRAP_NONE, STAGE>(false);
     }
 }
 
+template <bool STAGE>
 VertexKernel pick_vertex_kernel(int vm, int em, int wrap, bool multi) {
     switch (vm) {
-    case ND_VM_DIFFUSION: return pick_edge<ND_VM_DIFFUSION>(em, wrap, multi);
-    case ND_VM_KURAMOTO: return pick_edge<ND_VM_KURAMOTO>(em, wrap, multi);
-    case ND_VM_SWING: return pick_edge<ND_VM_SWING>(em, wrap, multi);
-    case ND_VM_SWING_UNIT_DAMPING: return pick_edge<ND_VM_SWING_UNIT_DAMPING>(em, wrap, multi);
-    case ND_VM_LOAD: return pick_edge<ND_VM_LOAD>(em, wrap, multi);
-    default: return k_vertex_rhs<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false>;
+    case ND_VM_DIFFUSION: return pick_edge<ND_VM_DIFFUSION, STAGE>(em, wrap, multi);
+    case ND_VM_KURAMOTO: return pick_edge<ND_VM_KURAMOTO, STAGE>(em, wrap, multi);
+    case ND_VM_SWING: return pick_edge<ND_VM_SWING, STAGE>(em, wrap, multi);
+    case ND_VM_SWING_UNIT_DAMPING: return pick_edge<ND_VM_SWING_UNIT_DAMPING, STAGE>(em, wrap, multi);
+    case ND_VM_LOAD: return pick_edge<ND_VM_LOAD, STAGE>(em, wrap, multi);
+    default: return k_vertex_rhs<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false, STAGE>;
     }
 }
 
-using EdgeOdeKernel = void (*)(EDesc, const double*, const double*, double*);
+using EdgeOdeKernel = void (*)(EDesc, const double*, const double*, double*, StageArgs);
+template <bool STAGE>
 EdgeOdeKernel pick_edge_ode(int em) {
     switch (em) {
-    case ND_EM_ODE_SIN_DIFFUSION: return k_edge_ode<ND_EM_ODE_SIN_DIFFUSION>;
-    case ND_EM_ODE_SIN_DIFFUSION_FAST: return k_edge_ode<ND_EM_ODE_SIN_DIFFUSION_FAST>;
-    default: return k_edge_ode<ND_EM_ODE_LINEAR>;
+    case ND_EM_ODE_SIN_DIFFUSION: return k_edge_ode<ND_EM_ODE_SIN_DIFFUSION, STAGE>;
+    case ND_EM_ODE_SIN_DIFFUSION_FAST: return k_edge_ode<ND_EM_ODE_SIN_DIFFUSION_FAST, STAGE>;
+    default: return k_edge_ode<ND_EM_ODE_LINEAR, STAGE>;
     }
 }
 
@@ -386,14 +581,16 @@ struct nd_plan {
     int64_t dim = 0, pdim = 0, lastidx_out = 0, lastidx_aggr = 0, lastidx_gbuf = 0;
     int32_t vdepth = 0, edepth = 0;
     bool multi = false;
-    int32_t hub_degree = 32;
+    int32_t hub_degree = 16;        // longer segments: one wave per vertex
+    int32_t big_hub_degree = 256;   // longer segments: one block per vertex
     int32_t block = 256;
     std::vector<VDesc> vdesc;
     std::vector<EDesc> edesc;
-    std::vector<VertexKernel> vkern;
-    std::vector<std::vector<int32_t>> hubs_host;   // per vertex batch: positions sorted
+    std::vector<VertexKernel> vkern, vkern_stage;
+    double* rk_work = nullptr;
     std::vector<int32_t> rowptr_host;
-    std::vector<int32_t> agg_hub_rows_host;
+    std::vector<int2> slots_host;
+    std::vector<uint8_t> slot_eb_host;
     std::vector<void*> allocations;
     int32_t* d_rowptr = nullptr;
     int2* d_slots = nullptr;
@@ -403,6 +600,9 @@ struct nd_plan {
     int32_t* d_gmap = nullptr;
     int32_t* d_hubs_all = nullptr;
     int32_t* d_agg_hub_rows = nullptr;
+    int32_t* d_lrp = nullptr;
+    int2* d_lslots = nullptr;
+    uint8_t* d_lseb = nullptr;
     int32_t n_agg_hub = 0;
     std::vector<size_t> hub_offset;
 
@@ -432,40 +632,77 @@ int upload(nd_plan* pl, const std::vector<T>& host, T** dev) {
     return ND_OK;
 }
 
-// hub lists follow hub_degree; rebuilt when it changes
+void release(nd_plan* pl, void* ptr) {
+    if (!ptr) return;
+    auto it = std::find(pl->allocations.begin(), pl->allocations.end(), ptr);
+    if (it != pl->allocations.end()) { (void)hipFree(*it); pl->allocations.erase(it); }
+}
+
+// Hub lists and the hub-free destination lists follow hub_degree / big_hub_degree;
+// rebuilt when they change.  Per vertex batch: big hubs then mid hubs, each by
+// decreasing segment length; every other vertex keeps its segment in lrp/lslots.
 int rebuild_hubs(nd_plan* pl) {
-    const int32_t H = pl->hub_degree;
-    std::vector<int32_t> all;
+    const int32_t H = pl->hub_degree, HB = pl->big_hub_degree;
+    std::vector<int32_t> all, lrp;
+    std::vector<int2> lslots;
+    std::vector<uint8_t> lseb;
+    std::vector<size_t> lrp_offset(pl->vdesc.size(), 0);
     pl->hub_offset.assign(pl->vdesc.size() + 1, 0);
     for (size_t b = 0; b < pl->vdesc.size(); ++b) {
         VDesc& vd = pl->vdesc[b];
         pl->hub_offset[b] = all.size();
-        if (vd.model != ND_VM_NOOP && vd.depth == 1)
-            for (int32_t q = 0; q < vd.count; ++q) {
-                const int32_t r = vd.row0 + q;
-                if (pl->rowptr_host[r + 1] - pl->rowptr_host[r] > H) all.push_back(q);
-            }
-        vd.nhubs = static_cast<int32_t>(all.size() - pl->hub_offset[b]);
+        lrp_offset[b] = lrp.size();
+        std::vector<std::pair<int32_t, int32_t>> big, mid;   // (-length, q)
+        const bool functional = vd.model != ND_VM_NOOP && vd.depth == 1;
+        for (int32_t q = 0; q < vd.count; ++q) {
+            lrp.push_back(static_cast<int32_t>(lslots.size()));
+            if (!functional) continue;
+            const int32_t r = vd.row0 + q;
+            const int32_t beg = pl->rowptr_host[r], len = pl->rowptr_host[r + 1] - beg;
+            if (len > H) lrp.back() = static_cast<int32_t>(static_cast<uint32_t>(lrp.back()) | 0x80000000u);
+            if (len > HB) big.emplace_back(-len, q);
+            else if (len > H) mid.emplace_back(-len, q);
+            else
+                for (int32_t k = beg; k < beg + len; ++k) {
+                    lslots.push_back(pl->slots_host[k]);
+                    if (pl->multi) lseb.push_back(pl->slot_eb_host[k]);
+                }
+        }
+        lrp.push_back(static_cast<int32_t>(lslots.size()));
+        std::sort(big.begin(), big.end());
+        std::sort(mid.begin(), mid.end());
+        for (auto& x : big) all.push_back(x.second);
+        for (auto& x : mid) all.push_back(x.second);
+        vd.nbig = static_cast<int32_t>(big.size());
+        vd.nmid = static_cast<int32_t>(mid.size());
     }
     pl->hub_offset[pl->vdesc.size()] = all.size();
     std::vector<int32_t> rows;
     for (int64_t r = 0; r < pl->lastidx_aggr; ++r)
         if (pl->rowptr_host[r + 1] - pl->rowptr_host[r] > H) rows.push_back(static_cast<int32_t>(r));
     pl->n_agg_hub = static_cast<int32_t>(rows.size());
-    // replace previous device lists
-    for (int32_t* old : {pl->d_hubs_all, pl->d_agg_hub_rows}) {
-        if (!old) continue;
-        auto it = std::find(pl->allocations.begin(), pl->allocations.end(), static_cast<void*>(old));
-        if (it != pl->allocations.end()) { (void)hipFree(*it); pl->allocations.erase(it); }
-    }
+    release(pl, pl->d_hubs_all);
+    release(pl, pl->d_agg_hub_rows);
+    release(pl, pl->d_lrp);
+    release(pl, pl->d_lslots);
+    release(pl, pl->d_lseb);
     pl->d_hubs_all = nullptr;
     pl->d_agg_hub_rows = nullptr;
+    pl->d_lrp = nullptr;
+    pl->d_lslots = nullptr;
+    pl->d_lseb = nullptr;
     int rc = upload(pl, all, &pl->d_hubs_all);
+    rc = rc ? rc : upload(pl, rows, &pl->d_agg_hub_rows);
+    rc = rc ? rc : upload(pl, lrp, &pl->d_lrp);
+    rc = rc ? rc : upload(pl, lslots, &pl->d_lslots);
+    rc = rc ? rc : upload(pl, lseb, &pl->d_lseb);
     if (rc) return rc;
-    rc = upload(pl, rows, &pl->d_agg_hub_rows);
-    if (rc) return rc;
-    for (size_t b = 0; b < pl->vdesc.size(); ++b)
+    for (size_t b = 0; b < pl->vdesc.size(); ++b) {
         pl->vdesc[b].hubs = pl->d_hubs_all ? pl->d_hubs_all + pl->hub_offset[b] : nullptr;
+        pl->vdesc[b].lrp = pl->d_lrp + lrp_offset[b];
+        pl->vdesc[b].lslots = pl->d_lslots;
+        pl->vdesc[b].lseb = pl->d_lseb;
+    }
     return ND_OK;
 }
 
@@ -478,6 +715,25 @@ int check_device(const nd_plan* pl) {
 }
 
 inline unsigned blocks_for(int64_t n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
+
+// One right-hand side (stage = false: du = f(u)) or one fused RK stage (stage = true).
+void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const double* p, const StageArgs& sa,
+                hipStream_t s) {
+    const int B = pl->block;
+    for (size_t b = 0; b < pl->vdesc.size(); ++b) {
+        const VDesc& vd = pl->vdesc[b];
+        if (vd.count == 0) continue;
+        const int64_t nblocks = vd.nbig + blocks_for(static_cast<int64_t>(vd.nmid) * kWave, B) + blocks_for(vd.count, B);
+        hipLaunchKernelGGL(stage ? pl->vkern_stage[b] : pl->vkern[b], dim3(static_cast<unsigned>(nblocks)),
+                           dim3(static_cast<unsigned>(B)), 0, s, vd, pl->d_rowptr, pl->d_slots, pl->d_slot_eb,
+                           pl->d_edesc, u, p, du, sa);
+    }
+    for (const EDesc& e : pl->edesc) {
+        if (!e.ode || e.count == 0) continue;
+        hipLaunchKernelGGL(stage ? pick_edge_ode<true>(e.model) : pick_edge_ode<false>(e.model),
+                           dim3(blocks_for(e.count, B)), dim3(static_cast<unsigned>(B)), 0, s, e, u, p, du, sa);
+    }
+}
 
 }  // namespace
 
@@ -616,6 +872,8 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
     rc = rc ? rc : upload(pl, pl->rowptr_host, &pl->d_rowptr);
     rc = rc ? rc : upload(pl, slots, &pl->d_slots);
     rc = rc ? rc : upload(pl, slot_eb, &pl->d_slot_eb);
+    pl->slots_host = std::move(slots);
+    pl->slot_eb_host = std::move(slot_eb);
     rc = rc ? rc : upload(pl, cols, &pl->d_cols);
     rc = rc ? rc : upload(pl, gmap, &pl->d_gmap);
     for (size_t b = 0; b < n.eb.size() && !rc; ++b) {
@@ -640,7 +898,8 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         vd.out0 = static_cast<int32_t>(B.out_first - 1);
         for (int j = 0; j < ND_MAX_OUT; ++j) vd.mask[j] = k.mask_dst[j] - 1;
         pl->vdesc.push_back(vd);
-        pl->vkern.push_back(pick_vertex_kernel(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi));
+        pl->vkern.push_back(pick_vertex_kernel<false>(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi));
+        pl->vkern_stage.push_back(pick_vertex_kernel<true>(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi));
     }
     rc = rebuild_hubs(pl);
     if (rc) { delete pl; return rc; }
@@ -657,9 +916,12 @@ int nd_plan_tune(nd_plan* pl, const nd_plan_tuning* t) {
             return nd::set_error(ND_ERR_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
         pl->block = t->block_size;
     }
-    if (t->hub_degree) {
-        if (t->hub_degree < 1) return nd::set_error(ND_ERR_ARGUMENT, "hub_degree must be positive");
-        pl->hub_degree = t->hub_degree;
+    if (t->hub_degree < 0 || t->big_hub_degree < 0)
+        return nd::set_error(ND_ERR_ARGUMENT, "hub degrees must be positive");
+    if (t->hub_degree || t->big_hub_degree) {
+        if (t->hub_degree) pl->hub_degree = t->hub_degree;
+        if (t->big_hub_degree) pl->big_hub_degree = t->big_hub_degree;
+        if (pl->big_hub_degree < pl->hub_degree) pl->big_hub_degree = pl->hub_degree;
         return rebuild_hubs(pl);
     }
     return ND_OK;
@@ -669,6 +931,7 @@ int nd_plan_get_tuning(const nd_plan* pl, nd_plan_tuning* t) {
     if (!pl || !t) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
     t->hub_degree = pl->hub_degree;
     t->block_size = pl->block;
+    t->big_hub_degree = pl->big_hub_degree;
     return ND_OK;
 }
 
@@ -684,22 +947,44 @@ int nd_rhs(nd_plan* pl, double* du, const double* u, const double* p, double t, 
     if (pl->pdim > 0 && !p) return nd::set_error(ND_ERR_ARGUMENT, "NULL parameter pointer");
     int rc = check_device(pl);
     if (rc) return rc;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const int B = pl->block;
-    for (size_t b = 0; b < pl->vdesc.size(); ++b) {
-        const VDesc& vd = pl->vdesc[b];
-        if (vd.count == 0) continue;
-        const int32_t nlow = static_cast<int32_t>(blocks_for(vd.count, B));
-        const int32_t nhub = static_cast<int32_t>(blocks_for(static_cast<int64_t>(vd.nhubs) * kWave, B));
-        hipLaunchKernelGGL(pl->vkern[b], dim3(static_cast<unsigned>(nlow + nhub)), dim3(static_cast<unsigned>(B)), 0, s,
-                           vd, pl->d_rowptr, pl->d_slots, pl->d_slot_eb, pl->d_edesc, u, p, du, pl->hub_degree, nlow);
-    }
-    for (const EDesc& e : pl->edesc) {
-        if (!e.ode || e.count == 0) continue;
-        hipLaunchKernelGGL(pick_edge_ode(e.model), dim3(blocks_for(e.count, B)), dim3(static_cast<unsigned>(B)), 0, s,
-                           e, u, p, du);
-    }
+    launch_rhs(pl, false, du, u, p, StageArgs{}, static_cast<hipStream_t>(stream));
     return hip_check(hipGetLastError(), "nd_rhs launch");
+}
+
+int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_t nsteps, double* work,
+           int64_t len_u, int64_t len_p, void* stream) {
+    (void)t0;
+    if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
+    if (len_u != pl->dim) return nd::set_error(ND_ERR_ARGUMENT, "u does not have expected size %lld", (long long)pl->dim);
+    if (pl->pdim > 0 && len_p != pl->pdim)
+        return nd::set_error(ND_ERR_ARGUMENT, "p does not has expecte size %lld", (long long)pl->pdim);
+    if (nsteps < 0 || !(dt == dt)) return nd::set_error(ND_ERR_ARGUMENT, "bad step count or dt");
+    if (pl->dim > 0 && !u) return nd::set_error(ND_ERR_ARGUMENT, "NULL state pointer");
+    if (pl->pdim > 0 && !p) return nd::set_error(ND_ERR_ARGUMENT, "NULL parameter pointer");
+    int rc = check_device(pl);
+    if (rc) return rc;
+    if (!work) {
+        if (!pl->rk_work) {
+            void* w = nullptr;
+            hipError_t e = hipMalloc(&w, static_cast<size_t>(3 * std::max<int64_t>(pl->dim, 1)) * sizeof(double));
+            if (e != hipSuccess) return nd::set_error(ND_ERR_ALLOC, "hipMalloc(rk work): %s", hipGetErrorString(e));
+            pl->allocations.push_back(w);
+            pl->rk_work = static_cast<double*>(w);
+        }
+        work = pl->rk_work;
+    }
+    double* acc = work;
+    double* ya = work + pl->dim;
+    double* yb = work + 2 * pl->dim;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const double half = 0.5 * dt, sixth = dt / 6.0;
+    for (int64_t k = 0; k < nsteps; ++k) {
+        launch_rhs(pl, true, nullptr, u, p, StageArgs{u, acc, ya, half, 0.0, 1}, s);
+        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, yb, half, 2.0, 2}, s);
+        launch_rhs(pl, true, nullptr, yb, p, StageArgs{u, acc, ya, dt, 2.0, 2}, s);
+        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, u, sixth, 0.0, 3}, s);
+    }
+    return hip_check(hipGetLastError(), "nd_rk4 launch");
 }
 
 int nd_gather(nd_plan* pl, double* gbuf, const double* o, void* stream) {

@@ -1,0 +1,143 @@
+# NDHIP.jl -- binds NetworkDynamics.jl's `Network` to libnd_hip.so through `ccall`.
+#
+#     using NetworkDynamics, AMDGPU, NDHIP
+#     nw = Network(g, vertexm, edgem; execution=HIPExecution{true}(), aggregator=SegmentedAggregator(+))
+#     nw(du, u, p, t)            # du, u, p :: ROCVector{Float64}
+#
+# The reference constructor (src/construction.jl:31-236) runs unchanged: it builds
+# its IndexManager and batches and calls the aggregator constructor
+# (src/construction.jl:198).  `SegmentedAggregator(im, batches, +)` hands the graph and
+# component kinds to `nd_network_create`, checks that the library's layout equals
+# `im` bit for bit, and creates the device plan.  The specialised call method
+# replaces src/coreloop.jl:1-102 for `Network{<:HIPExecution}`.
+#
+# Only the component functions of the library's table below run on the GPU; any
+# other `compf`/`compg` is an ArgumentError at construction.
+module NDHIP
+
+using NetworkDynamics
+using NetworkDynamics: ExecutionStyle, Aggregator, IndexManager, ComponentBatch, compf, compg, fftype,
+                       dim, pdim, outdim, PureFeedForward, PureStateMap, StateMask, AntiSymmetric,
+                       Symmetric, Directed, Fiducial
+using AMDGPU: ROCVector, AMDGPU
+
+export HIPExecution, SegmentedAggregator, register_model!
+
+const libnd = normpath(joinpath(@__DIR__, "..", "libnd_hip.so"))
+
+struct HIPExecution{buffered} <: ExecutionStyle{buffered} end
+NetworkDynamics.iscudacompatible(::Type{<:HIPExecution}) = false
+
+# mirrors nd_kind in include/nd_hip.h
+struct NDKind
+    is_edge::Int32
+    model::Int32
+    wrapper::Int32
+    ff::Int32
+    dim::Int32
+    pdim::Int32
+    outdim_src::Int32
+    outdim_dst::Int32
+    mask_src::NTuple{4,Int32}
+    mask_dst::NTuple{4,Int32}
+end
+
+# component function -> ND_VM_* / ND_EM_* id (fill from the user's component library,
+# e.g. register_model!(Lib.kuramoto_inertia!, 3) for ND_VM_SWING)
+const MODEL_IDS = IdDict{Any,Int32}()
+register_model!(f, id::Integer) = (MODEL_IDS[f] = Int32(id); nothing)
+
+_check(rc) = rc == 0 || throw(ArgumentError(unsafe_string(ccall((:nd_last_error, libnd), Cstring, ()))))
+
+_mask(s::StateMask) = ntuple(i -> i <= length(s.idxs) ? Int32(s.idxs[i]) : Int32(0), 4)
+_nomask() = ntuple(_ -> Int32(0), 4)
+
+function _kind(c::VertexModel)
+    g = compg(c)
+    g isa StateMask || throw(ArgumentError("HIPExecution needs StateMask vertex outputs"))
+    NDKind(0, get(MODEL_IDS, compf(c), -1), 0, 0, dim(c), pdim(c), 0, outdim(c), _nomask(), _mask(g))
+end
+
+function _kind(c::EdgeModel)
+    g = compg(c)
+    od = outdim(c)
+    if g isa Fiducial && g.src isa StateMask
+        return NDKind(1, get(MODEL_IDS, compf(c), -1), 4, 0, dim(c), pdim(c), od.src, od.dst,
+                      _mask(g.src), _mask(g.dst))
+    end
+    wrap = g isa AntiSymmetric ? 1 : g isa Symmetric ? 2 : g isa Directed ? 3 : 0
+    inner = wrap == 0 ? g : g.g
+    NDKind(1, get(MODEL_IDS, inner, -1), wrap, 1, dim(c), pdim(c), od.src, od.dst, _nomask(), _nomask())
+end
+
+mutable struct SegmentedAggregator{F} <: Aggregator
+    f::F
+    net::Ptr{Cvoid}
+    plan::Ptr{Cvoid}
+end
+SegmentedAggregator(f) = (im, batches) -> SegmentedAggregator(im, batches, f)
+
+function SegmentedAggregator(im::IndexManager, batches, f)
+    f === (+) || throw(ArgumentError("SegmentedAggregator only supports +"))
+    src = Int64[e.src for e in im.edgevec]
+    dst = Int64[e.dst for e in im.edgevec]
+    kinds = NDKind[]
+    keys = Dict{NDKind,Int32}()
+    kid(k) = get!(() -> (push!(kinds, k); Int32(length(kinds) - 1)), keys, k)
+    vk = Int32[kid(_kind(m)) for m in im.vertexm]
+    ek = Int32[kid(_kind(m)) for m in im.edgem]
+    net = Ref{Ptr{Cvoid}}(C_NULL)
+    _check(ccall((:nd_network_create, libnd), Cint,
+                 (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Int32, Ptr{NDKind}, Ptr{Int32}, Ptr{Int32}, Ptr{Ptr{Cvoid}}),
+                 length(im.vertexm), length(im.edgem), src, dst, length(kinds), kinds, vk, ek, net))
+    _check_layout(net[], im)
+    plan = Ref{Ptr{Cvoid}}(C_NULL)
+    _check(ccall((:nd_plan_create, libnd), Cint, (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}), net[], plan))
+    a = SegmentedAggregator(f, net[], plan[])
+    finalizer(a) do x
+        ccall((:nd_plan_destroy, libnd), Cvoid, (Ptr{Cvoid},), x.plan)
+        ccall((:nd_network_destroy, libnd), Cvoid, (Ptr{Cvoid},), x.net)
+    end
+    a
+end
+
+# bit-exact check of the library's flat layout against the reference IndexManager
+function _check_layout(net, im)
+    nv, ne = length(im.v_data), length(im.e_data)
+    vr = zeros(Int64, 8, nv)
+    er = zeros(Int64, 12, max(ne, 1))
+    _check(ccall((:nd_network_export_ranges, libnd), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}), net, vr, er))
+    for i in 1:nv
+        for (j, r) in enumerate((im.v_data[i], im.v_out[i], im.v_para[i], im.v_aggr[i]))
+            (vr[2j-1, i], vr[2j, i]) == (first(r), last(r)) || error("layout mismatch at vertex $i")
+        end
+    end
+    for i in 1:ne
+        rs = (im.e_data[i], im.e_out[i].src, im.e_out[i].dst, im.e_para[i], im.e_gbufr[i].src, im.e_gbufr[i].dst)
+        for (j, r) in enumerate(rs)
+            (er[2j-1, i], er[2j, i]) == (first(r), last(r)) || error("layout mismatch at edge $i")
+        end
+    end
+    nothing
+end
+
+# aggregate!(::SegmentedAggregator, aggbuf, o): additive, deterministic (src/aggregators.jl:140)
+function NetworkDynamics.aggregate!(a::SegmentedAggregator, aggbuf::ROCVector{Float64}, o::ROCVector{Float64})
+    _check(ccall((:nd_aggregate, libnd), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
+                 a.plan, pointer(aggbuf), pointer(o), AMDGPU.stream().stream))
+    AMDGPU.synchronize()
+    nothing
+end
+
+# nw(du, u, p, t) for HIPExecution (replaces src/coreloop.jl:1-102)
+function (nw::Network{<:HIPExecution})(du::ROCVector{Float64}, u::ROCVector{Float64}, p::ROCVector{Float64}, t)
+    a = nw.layer.aggregator::SegmentedAggregator
+    _check(ccall((:nd_rhs, libnd), Cint,
+                 (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Int64, Int64, Int64, Ptr{Cvoid}),
+                 a.plan, pointer(du), pointer(u), pointer(p), Float64(t), length(du), length(u), length(p),
+                 AMDGPU.stream().stream))
+    AMDGPU.synchronize()   # the reference call returns after its work is done (coreloop.jl:100)
+    nothing
+end
+
+end # module

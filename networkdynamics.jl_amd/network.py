@@ -148,7 +148,7 @@ class Network:
                 print(f" - EdgeBatch: dim={kinds[b['kind']]['dim']}, pdim={kinds[b['kind']]['pdim']}, length={b['count']}")
 
     def __del__(self):
-        L = A._lib
+        L = getattr(A, "_lib", None) if A is not None else None
         if L is None:
             return
         if getattr(self, "_plan", None):
@@ -235,14 +235,14 @@ class Network:
             self._plan = h
         return self._plan
 
-    def tune(self, hub_degree=0, block_size=0):
-        t = A.NDPlanTuning(int(hub_degree), int(block_size))
+    def tune(self, hub_degree=0, block_size=0, big_hub_degree=0):
+        t = A.NDPlanTuning(int(hub_degree), int(block_size), int(big_hub_degree))
         A.check(A.lib().nd_plan_tune(self.plan, C.byref(t)))
 
     def tuning(self):
         t = A.NDPlanTuning()
         A.check(A.lib().nd_plan_get_tuning(self.plan, C.byref(t)))
-        return dict(hub_degree=t.hub_degree, block_size=t.block_size)
+        return dict(hub_degree=t.hub_degree, block_size=t.block_size, big_hub_degree=t.big_hub_degree)
 
     def __call__(self, du, u, p, t=0.0):
         """nw(du, u, p, t): coreloop.jl:1-102.  du is overwritten; stream-ordered on
@@ -257,6 +257,22 @@ class Network:
         stream = torch.cuda.current_stream(du.device).cuda_stream
         A.check(A.lib().nd_rhs(self.plan, dptr, uptr, pptr, float(t), ndu, nu, np_, stream))
         return None
+
+    def rk4(self, u, p, t0, dt, nsteps, work=None):
+        """Fixed-step classic RK4 in place on the device vector u (nsteps steps of dt)."""
+        import torch
+        uptr, nu = _tensor_ptr(u, self.dim, "u", torch)
+        if self.pdim > 0:
+            pptr, np_ = _tensor_ptr(p, self.pdim, "p", torch)
+        else:
+            pptr, np_ = None, 0
+        wptr = None
+        if work is not None:
+            wptr, nw_ = _tensor_ptr(work, 3 * self.dim, "work", torch)
+            if nw_ < 3 * self.dim:
+                raise A.ArgumentError("work needs 3*dim entries")
+        stream = torch.cuda.current_stream(u.device).cuda_stream
+        A.check(A.lib().nd_rk4(self.plan, uptr, pptr, float(t0), float(dt), int(nsteps), wptr, nu, np_, stream))
 
     def get_buffers(self, u, p, t=0.0):
         """get_buffers(nw, u, p, t; initbufs=true): (o, aggbuf) device tensors, coreloop.jl:103-109."""

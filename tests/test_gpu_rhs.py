@@ -193,7 +193,8 @@ def test_aggregate_primitive_depth2(ND, dev):
     nw.tune(hub_degree=2)
     nw.aggregate(off, sd)
     torch.cuda.synchronize()
-    np.testing.assert_allclose(off.cpu().numpy() - ref, np.arange(1, on.lastidx_aggr + 1), rtol=0, atol=1e-12)
+    # additivity (aggregators_test.jl:69-79, there with isapprox): exact up to the ulp of the offset
+    np.testing.assert_allclose(off.cpu().numpy() - ref, np.arange(1, on.lastidx_aggr + 1), rtol=4e-16, atol=1e-12)
 
 
 def test_swing_c2_full_size_properties(ND, dev):
@@ -215,3 +216,34 @@ def test_swing_c2_full_size_properties(ND, dev):
     on = oracle_for(nw)
     ref, S = on.rhs(u0, p), on.scale(u0, p)
     assert scaled_error(got, ref, S).max() <= RTOL
+
+
+def _dev_rk4(nw, u0, p, dev, dt, nsteps):
+    import torch
+    u = torch.from_numpy(np.array(u0)).to(dev)
+    nw.rk4(u, torch.from_numpy(np.ascontiguousarray(p if len(p) else np.zeros(1))).to(dev), 0.0, dt, nsteps)
+    torch.cuda.synchronize()
+    return u.cpu().numpy()
+
+
+def test_rk4_kuramoto_c1(ND, dev):
+    # config 1: RK4 dt = 0.01 to t = 10 against the oracle's textbook RK4.  The dynamics
+    # contract (a 1e-15 relative perturbation of u0 stays below 1e-15 at t = 10), so the
+    # only difference, sin's last ulp per stage, stays at round-off level.
+    nw, p, u0 = ND.workloads.kuramoto_er(1000, 4000, 3.0, seed=1)
+    ref = oracle_for(nw).rk4(u0, p, 0.0, 0.01, 1000)
+    got = _dev_rk4(nw, u0, p, dev, 0.01, 1000)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
+
+
+def test_rk4_swing_and_ode_edges(ND, dev):
+    nw, p, u0 = ND.workloads.swing_ba(20_000, 2, seed=2)
+    ref = oracle_for(nw).rk4(u0, p, 0.0, 0.001, 200)
+    np.testing.assert_allclose(_dev_rk4(nw, u0, p, dev, 0.001, 200), ref, rtol=0, atol=1e-10)
+    from test_oracle import _complete4
+    nw = _complete4(ND)
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=nw.dim)
+    p = rng.uniform(0.5, 2, nw.pdim)
+    ref = oracle_for(nw).rk4(u, p, 0.0, 0.002, 100)
+    np.testing.assert_allclose(_dev_rk4(nw, u, p, dev, 0.002, 100), ref, rtol=0, atol=1e-12)
