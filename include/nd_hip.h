@@ -225,6 +225,50 @@ int nd_gather(nd_plan* plan, double* gbuf, const double* o, void* stream);
  * deterministic (no atomics). */
 int nd_aggregate(nd_plan* plan, double* aggbuf, const double* o, void* stream);
 
+/* ---- batched ensemble integration ------------------------------------------------
+ * Many independent networks (instances) of one vertex kind and one edge kind each,
+ * integrated in one launch per size class: one workgroup per instance, the whole
+ * trajectory inside the kernel (register-resident Runge-Kutta state, LDS-resident
+ * topology).  The reference has no ensemble code; the caller it replaces is a SciML
+ * EnsembleProblem over ODEProblem(nw, u0, tspan, p) (src/post_utils.jl:43-82)
+ * solved with OrdinaryDiffEqTsit5.  Instance i: nv[i] vertices, ne[i] edges whose
+ * 1-based endpoints follow in src/dst (concatenated, Graphs.jl edge order), state
+ * and parameters in the reference's flat layout for a one-batch network,
+ * concatenated in instance order (offsets from nd_ensemble_sizes).  Vertex kinds:
+ * KURAMOTO, SWING; edge kinds: AntiSymmetric KURAMOTO, STATIC_LINE; up to 2048
+ * vertices per instance with the default classes. */
+#define ND_ENSEMBLE_CLASSES 4
+enum { ND_METHOD_TSIT5 = 0, ND_METHOD_RK4 = 1 };
+
+typedef struct {
+    int32_t method;       /* ND_METHOD_TSIT5 (adaptive) or ND_METHOD_RK4 (fixed dt0)          */
+    int32_t pad_;
+    double reltol, abstol;
+    double dt0;           /* RK4: the step; Tsit5: initial step, 0 = automatic                  */
+    double dtmax, dtmin;  /* 0 = t1-t0 / 1e-14*max(1,|t1|)                                       */
+    int64_t maxiters;     /* steps per instance, 0 = 1e6                                         */
+} nd_ensemble_opts;
+
+typedef struct {
+    double t, dt;         /* final time and next step                                           */
+    int64_t naccept, nreject, nf;
+    int32_t retcode;      /* 0 success, 1 maxiters, 2 dt below dtmin, 3 NaN step                 */
+    int32_t pad_;
+} nd_ensemble_stat;
+
+typedef struct nd_ensemble nd_ensemble;
+int nd_ensemble_create(int32_t ninst, const int64_t* nv, const int64_t* ne, const int64_t* src, const int64_t* dst,
+                       const nd_kind* vertex_kind, const nd_kind* edge_kind, nd_ensemble** out);
+void nd_ensemble_destroy(nd_ensemble* en);
+int nd_ensemble_sizes(const nd_ensemble* en, int64_t* total_states, int64_t* total_params,
+                      int64_t* state_offsets, int64_t* param_offsets);
+/* size class c holds instances with nv <= threads*vper (first class that fits) */
+int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t vper);
+/* integrates every instance from t0 to t1 in place on u (device); stats: device array
+ * of ninst entries */
+int nd_ensemble_solve(nd_ensemble* en, double* u, const double* p, double t0, double t1,
+                      const nd_ensemble_opts* opts, nd_ensemble_stat* stats, void* stream);
+
 /* ---- synthetic inputs ----------------------------------------------------------
  * Seeded graph generators for the workloads.  No counterpart in the reference's
  * RHS path (its tests use Graphs.jl generators, whose random streams need Julia);

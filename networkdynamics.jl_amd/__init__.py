@@ -12,6 +12,7 @@ module name `networkdynamics_jl_amd` through `load()` in __graft_entry__.py.
 from . import _abi
 from . import library as Lib
 from . import workloads
+from .ensemble import Ensemble, tsit5_per_instance
 from ._abi import ArgumentError, DeviceError, NotConvergedError, UnsupportedError
 from .components import (AntiSymmetric, DeviceFunction, Directed, EdgeModel, FeedForward, Fiducial,
                          NoFeedForward, PureFeedForward, PureStateMap, StateMask, Symmetric, VertexModel,

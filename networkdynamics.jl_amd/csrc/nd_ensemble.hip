@@ -1,0 +1,627 @@
+// Batched ensemble integrator: thousands of independent (uneven) networks, one
+// workgroup per instance, whole trajectory in one launch per size class.
+//
+// Per instance everything a stage touches is on-chip:
+//   registers  per-vertex state y and the Runge-Kutta stage derivatives (up to V
+//              vertices per thread);
+//   LDS        stage vertex outputs theta[n], edge parameters, the destination
+//              lists (neighbour | role bit, edge index) and their chunking.
+// A stage: write theta, barrier, evaluate chunks (<= kChunk entries of one vertex's
+// destination list each, so hubs are split over threads), barrier, each vertex sums
+// its chunk partials in order and applies its model.  Vertices whose list fits one
+// chunk are summed in the reference order (increasing output index).
+//
+// Methods: Tsit5 (Tsitouras 2011) with the PI step-size controller of
+// OrdinaryDiffEq (beta1 = 7/50, beta2 = 2/25, qmin = 1/5, qmax = 10, gamma = 9/10,
+// error norm sqrt(mean((err / (abstol + reltol*max(|y|,|ynew|)))^2))), and fixed-step
+// classic RK4.  Step acceptance is block-uniform, so every wave leaves the loop
+// together.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "nd_internal.h"
+#include "nd_models.h"
+
+namespace {
+
+using namespace nd;
+constexpr int kWave = 64;
+constexpr int kChunk = 8;          // destination-list entries per chunk
+
+struct InstDesc {
+    int32_t n, ne, nslot, nchunk;
+    int64_t uoff, poff, soff, csoff, cvoff, cpoff;
+    int32_t lds_bytes, cls;
+};
+
+struct SolveArgs {
+    const InstDesc* inst;
+    const int32_t* order;
+    const uint16_t* slot_nbr;     // local neighbour | 0x8000 if this vertex is the edge's src
+    const uint16_t* slot_e;       // local edge index
+    const uint16_t* cs;           // chunk starts (nchunk + 1 per instance)
+    const uint16_t* cv;           // chunk vertex
+    const uint16_t* cptr;         // vertex -> chunk range (n + 1 per instance)
+    double* u;
+    const double* p;
+    nd_ensemble_stat* stats;
+    double t0, t1, dt0, reltol, abstol, dtmax, dtmin;
+    int64_t maxiters;
+    int32_t first, count;
+};
+
+template <int VM> struct VPar { static constexpr int np = 0; };
+template <> struct VPar<ND_VM_KURAMOTO> { static constexpr int np = 1; };
+template <> struct VPar<ND_VM_SWING> { static constexpr int np = 3; };
+template <> struct VPar<ND_VM_SWING_UNIT_DAMPING> { static constexpr int np = 1; };
+template <> struct VPar<ND_VM_LOAD> { static constexpr int np = 2; };
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    const int t = static_cast<int>(threadIdx.x), nw = static_cast<int>(blockDim.x) / kWave;
+    if ((t & (kWave - 1)) == 0) red[t / kWave] = v;
+    __syncthreads();
+    if (t == 0) {
+        double s = red[0];
+        for (int w = 1; w < nw; ++w) s += red[w];
+        red[16] = s;
+    }
+    __syncthreads();
+    const double r = red[16];
+    __syncthreads();
+    return r;
+}
+
+struct Lds {
+    double* theta;
+    double* epar;
+    double* partial;
+    double* red;
+    uint16_t *nbr, *eidx, *cs, *cv, *cptr;
+};
+
+// out = f(ys) for the thread's vertices (two block barriers)
+template <int VM, int EM, int V>
+__device__ __forceinline__ void rhs(const InstDesc& I, const Lds& L, const double (&ys)[V][VModel<VM>::dim],
+                                    const double (&pv)[V][VPar<VM>::np > 0 ? VPar<VM>::np : 1],
+                                    double (&out)[V][VModel<VM>::dim]) {
+    constexpr int D = VModel<VM>::dim;
+    constexpr int NPE = EModel<EM>::np;
+    const int T = static_cast<int>(blockDim.x), t = static_cast<int>(threadIdx.x);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const int v = t + k * T;
+        if (v < I.n) L.theta[v] = ys[k][0];
+    }
+    __syncthreads();
+    for (int c = t; c < I.nchunk; c += T) {
+        const double own = L.theta[L.cv[c]];
+        double acc = 0.0;
+        const int s1 = L.cs[c + 1];
+        for (int s = L.cs[c]; s < s1; ++s) {
+            const int nb = L.nbr[s];
+            const double* pe = L.epar + static_cast<int>(L.eidx[s]) * (NPE > 0 ? NPE : 1);
+            const double th = L.theta[nb & 0x7fff];
+            acc += (nb & 0x8000) ? -g_dst<EM>(own, th, pe) : g_dst<EM>(th, own, pe);
+        }
+        L.partial[c] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const int v = t + k * T;
+        if (v < I.n) {
+            double acc = 0.0;
+            const int c1 = L.cptr[v + 1];
+            for (int c = L.cptr[v]; c < c1; ++c) acc += L.partial[c];
+            vertex_f<VM>(out[k], ys[k], acc, pv[k], D);
+        }
+    }
+}
+
+// Tsitouras (2011) 5(4) coefficients
+constexpr double c_a21 = 0.161;
+constexpr double c_a31 = -0.008480655492356989, c_a32 = 0.335480655492357;
+constexpr double c_a41 = 2.897153057105493, c_a42 = -6.359448489975075, c_a43 = 4.3622954328695815;
+constexpr double c_a51 = 5.325864828439257, c_a52 = -11.748883564062828, c_a53 = 7.4955393428898365,
+                 c_a54 = -0.09249506636175525;
+constexpr double c_a61 = 5.86145544294642, c_a62 = -12.92096931784711, c_a63 = 8.159367898576159,
+                 c_a64 = -0.071584973281401, c_a65 = -0.028269050394068383;
+constexpr double c_a71 = 0.09646076681806523, c_a72 = 0.01, c_a73 = 0.4798896504144996,
+                 c_a74 = 1.379008574103742, c_a75 = -3.290069515436081, c_a76 = 2.324710524099774;
+constexpr double c_bt1 = -0.00178001105222577714, c_bt2 = -0.0008164344596567469, c_bt3 = 0.007880878010261995,
+                 c_bt4 = -0.1447110071732629, c_bt5 = 0.5823571654525552, c_bt6 = -0.45808210592918697,
+                 c_bt7 = 0.015151515151515152;
+
+template <int VM, int EM, int METHOD, int V>
+__global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
+    constexpr int D = VModel<VM>::dim;
+    constexpr int NPV = VPar<VM>::np > 0 ? VPar<VM>::np : 1;
+    constexpr int NPE = EModel<EM>::np;
+    const int T = static_cast<int>(blockDim.x), t = static_cast<int>(threadIdx.x);
+    const int inst = a.order[a.first + static_cast<int>(blockIdx.x)];
+    const InstDesc I = a.inst[inst];
+    extern __shared__ double smem[];
+    Lds L;
+    L.theta = smem;
+    L.epar = L.theta + I.n;
+    L.partial = L.epar + I.ne * NPE;
+    L.red = L.partial + I.nchunk;
+    L.nbr = reinterpret_cast<uint16_t*>(L.red + 32);
+    L.eidx = L.nbr + I.nslot;
+    L.cs = L.eidx + I.nslot;
+    L.cv = L.cs + I.nchunk + 1;
+    L.cptr = L.cv + I.nchunk;
+    for (int i = t; i < I.nslot; i += T) {
+        L.nbr[i] = a.slot_nbr[I.soff + i];
+        L.eidx[i] = a.slot_e[I.soff + i];
+    }
+    for (int i = t; i <= I.nchunk; i += T) L.cs[i] = a.cs[I.csoff + i];
+    for (int i = t; i < I.nchunk; i += T) L.cv[i] = a.cv[I.cvoff + i];
+    for (int i = t; i <= I.n; i += T) L.cptr[i] = a.cptr[I.cpoff + i];
+    const double* pe_g = a.p + I.poff + static_cast<int64_t>(I.n) * VPar<VM>::np;
+    for (int i = t; i < I.ne * NPE; i += T) L.epar[i] = pe_g[i];
+
+    double y[V][D], pv[V][NPV];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const int v = t + k * T;
+#pragma unroll
+        for (int j = 0; j < D; ++j) y[k][j] = v < I.n ? a.u[I.uoff + static_cast<int64_t>(v) * D + j] : 0.0;
+#pragma unroll
+        for (int j = 0; j < NPV; ++j)
+            pv[k][j] = (v < I.n && VPar<VM>::np > 0) ? a.p[I.poff + static_cast<int64_t>(v) * VPar<VM>::np + j] : 0.0;
+    }
+    __syncthreads();
+
+    double tcur = a.t0;
+    int64_t naccept = 0, nreject = 0, nf = 0;
+    int retcode = 0;
+    double dt = a.dt0;
+    const double inv_len = 1.0 / (static_cast<double>(I.n) * D);
+
+    if constexpr (METHOD == 1) {
+        // fixed-step classic RK4: u + dt/6*(((k1 + 2k2) + 2k3) + k4)
+        double k[V][D], acc[V][D], ys[V][D];
+        const int64_t nsteps = static_cast<int64_t>(llround((a.t1 - a.t0) / dt));
+        const double half = 0.5 * dt, sixth = dt / 6.0;
+        for (int64_t step = 0; step < nsteps; ++step) {
+            rhs<VM, EM, V>(I, L, y, pv, k);
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+#pragma unroll
+                for (int j = 0; j < D; ++j) { acc[q][j] = k[q][j]; ys[q][j] = y[q][j] + half * k[q][j]; }
+            rhs<VM, EM, V>(I, L, ys, pv, k);
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+#pragma unroll
+                for (int j = 0; j < D; ++j) { acc[q][j] = acc[q][j] + 2.0 * k[q][j]; ys[q][j] = y[q][j] + half * k[q][j]; }
+            rhs<VM, EM, V>(I, L, ys, pv, k);
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+#pragma unroll
+                for (int j = 0; j < D; ++j) { acc[q][j] = acc[q][j] + 2.0 * k[q][j]; ys[q][j] = y[q][j] + dt * k[q][j]; }
+            rhs<VM, EM, V>(I, L, ys, pv, k);
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+#pragma unroll
+                for (int j = 0; j < D; ++j) y[q][j] = y[q][j] + sixth * (acc[q][j] + k[q][j]);
+            nf += 4;
+            ++naccept;
+        }
+        tcur = a.t0 + static_cast<double>(nsteps) * dt;
+    } else {
+        double k1[V][D], k2[V][D], k3[V][D], k4[V][D], k5[V][D], k6[V][D], k7[V][D], ys[V][D];
+        rhs<VM, EM, V>(I, L, y, pv, k1);
+        ++nf;
+        auto wnorm2 = [&](const double (&x)[V][D], const double (&ref)[V][D]) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+                if (t + q * T < I.n)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) {
+                        const double sc = a.abstol + fabs(ref[q][j]) * a.reltol;
+                        const double r = x[q][j] / sc;
+                        s += r * r;
+                    }
+            return block_sum(s, L.red);
+        };
+        if (!(dt > 0.0)) {
+            // initial step (Hairer, Norsett & Wanner, Solving ODEs I, II.4)
+            const double d0 = sqrt(wnorm2(y, y) * inv_len), d1 = sqrt(wnorm2(k1, y) * inv_len);
+            double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+            h0 = fmin(h0, a.t1 - a.t0);
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+#pragma unroll
+                for (int j = 0; j < D; ++j) ys[q][j] = y[q][j] + h0 * k1[q][j];
+            rhs<VM, EM, V>(I, L, ys, pv, k2);
+            ++nf;
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+#pragma unroll
+                for (int j = 0; j < D; ++j) k3[q][j] = k2[q][j] - k1[q][j];
+            const double d2 = sqrt(wnorm2(k3, y) * inv_len) / h0;
+            const double m = fmax(d1, d2);
+            const double h1 = m <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / m, 1.0 / 6.0);
+            dt = fmin(100.0 * h0, h1);
+        }
+        dt = fmin(dt, a.dtmax);
+        double qold = 1e-4;
+        int64_t iters = 0;
+        while (tcur < a.t1) {
+            if (iters++ >= a.maxiters) { retcode = 1; break; }
+            if (dt < a.dtmin) { retcode = 2; break; }
+            const bool last = tcur + dt >= a.t1;
+            const double h = last ? a.t1 - tcur : dt;
+#define ND_STAGE(expr)                                                        \
+    _Pragma("unroll") for (int q = 0; q < V; ++q)                             \
+        _Pragma("unroll") for (int j = 0; j < D; ++j) ys[q][j] = y[q][j] + h * (expr);
+            ND_STAGE(c_a21 * k1[q][j]);
+            rhs<VM, EM, V>(I, L, ys, pv, k2);
+            ND_STAGE(c_a31 * k1[q][j] + c_a32 * k2[q][j]);
+            rhs<VM, EM, V>(I, L, ys, pv, k3);
+            ND_STAGE(c_a41 * k1[q][j] + c_a42 * k2[q][j] + c_a43 * k3[q][j]);
+            rhs<VM, EM, V>(I, L, ys, pv, k4);
+            ND_STAGE(c_a51 * k1[q][j] + c_a52 * k2[q][j] + c_a53 * k3[q][j] + c_a54 * k4[q][j]);
+            rhs<VM, EM, V>(I, L, ys, pv, k5);
+            ND_STAGE(c_a61 * k1[q][j] + c_a62 * k2[q][j] + c_a63 * k3[q][j] + c_a64 * k4[q][j] + c_a65 * k5[q][j]);
+            rhs<VM, EM, V>(I, L, ys, pv, k6);
+            ND_STAGE(c_a71 * k1[q][j] + c_a72 * k2[q][j] + c_a73 * k3[q][j] + c_a74 * k4[q][j] + c_a75 * k5[q][j] +
+                     c_a76 * k6[q][j]);
+#undef ND_STAGE
+            // error-estimate combination of k1..k6 before the last evaluation, so only
+            // y, ynew and this sum stay live across it
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+#pragma unroll
+                for (int j = 0; j < D; ++j)
+                    k2[q][j] = c_bt1 * k1[q][j] + c_bt2 * k2[q][j] + c_bt3 * k3[q][j] + c_bt4 * k4[q][j] +
+                               c_bt5 * k5[q][j] + c_bt6 * k6[q][j];
+            rhs<VM, EM, V>(I, L, ys, pv, k7);
+            nf += 6;
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+                if (t + q * T < I.n)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) {
+                        const double e = h * (k2[q][j] + c_bt7 * k7[q][j]);
+                        const double sc = a.abstol + fmax(fabs(y[q][j]), fabs(ys[q][j])) * a.reltol;
+                        const double r = e / sc;
+                        s += r * r;
+                    }
+            const double eest = sqrt(block_sum(s, L.red) * inv_len);
+            const double q11 = pow(eest, 0.14);
+            if (eest <= 1.0) {
+                double qq = eest == 0.0 ? 0.1 : fmax(0.1, fmin(5.0, q11 / pow(qold, 0.08) / 0.9));
+                tcur = last ? a.t1 : tcur + h;
+#pragma unroll
+                for (int q = 0; q < V; ++q)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) { y[q][j] = ys[q][j]; k1[q][j] = k7[q][j]; }
+                qold = fmax(eest, 1e-4);
+                dt = fmin(h / qq, a.dtmax);
+                ++naccept;
+            } else {
+                dt = h / fmin(5.0, q11 / 0.9);
+                ++nreject;
+            }
+            if (!(dt == dt)) { retcode = 3; break; }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const int v = t + k * T;
+        if (v < I.n)
+#pragma unroll
+            for (int j = 0; j < D; ++j) a.u[I.uoff + static_cast<int64_t>(v) * D + j] = y[k][j];
+    }
+    if (t == 0) {
+        nd_ensemble_stat st;
+        st.t = tcur;
+        st.dt = dt;
+        st.naccept = naccept;
+        st.nreject = nreject;
+        st.nf = nf;
+        st.retcode = retcode;
+        st.pad_ = 0;
+        a.stats[inst] = st;
+    }
+}
+
+using EnsKernel = void (*)(SolveArgs);
+
+template <int VM, int EM, int METHOD>
+EnsKernel pick_v(int vper) {
+    switch (vper) {
+    case 1: return k_ensemble<VM, EM, METHOD, 1>;
+    case 4: return k_ensemble<VM, EM, METHOD, 4>;
+    default: return k_ensemble<VM, EM, METHOD, 2>;
+    }
+}
+
+template <int VM, int EM>
+EnsKernel pick_m(int method, int vper) {
+    return method == ND_METHOD_RK4 ? pick_v<VM, EM, 1>(vper) : pick_v<VM, EM, 0>(vper);
+}
+
+EnsKernel pick_ensemble(int vm, int em, int method, int vper) {
+    if (vm == ND_VM_SWING && em == ND_EM_KURAMOTO) return pick_m<ND_VM_SWING, ND_EM_KURAMOTO>(method, vper);
+    if (vm == ND_VM_SWING && em == ND_EM_STATIC_LINE) return pick_m<ND_VM_SWING, ND_EM_STATIC_LINE>(method, vper);
+    if (vm == ND_VM_KURAMOTO && em == ND_EM_KURAMOTO) return pick_m<ND_VM_KURAMOTO, ND_EM_KURAMOTO>(method, vper);
+    if (vm == ND_VM_KURAMOTO && em == ND_EM_STATIC_LINE) return pick_m<ND_VM_KURAMOTO, ND_EM_STATIC_LINE>(method, vper);
+    return nullptr;
+}
+
+int vpar(int vm) {
+    switch (vm) {
+    case ND_VM_KURAMOTO: return 1;
+    case ND_VM_SWING: return 3;
+    default: return -1;
+    }
+}
+int epar(int em) { return em == ND_EM_KURAMOTO ? 1 : em == ND_EM_STATIC_LINE ? 2 : -1; }
+
+int hip_err(hipError_t e, const char* what) {
+    if (e == hipSuccess) return ND_OK;
+    return nd::set_error(ND_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+}  // namespace
+
+struct nd_ensemble {
+    int device = 0;
+    int32_t ninst = 0;
+    nd_kind vk{}, ek{};
+    int32_t dim = 0;
+    int64_t total_states = 0, total_params = 0;
+    std::vector<InstDesc> inst;
+    int32_t cls_threads[ND_ENSEMBLE_CLASSES] = {128, 256, 512, 512};
+    int32_t cls_vper[ND_ENSEMBLE_CLASSES] = {2, 2, 2, 4};
+    std::vector<void*> allocs;
+    InstDesc* d_inst = nullptr;
+    int32_t* d_order = nullptr;
+    uint16_t *d_nbr = nullptr, *d_e = nullptr, *d_cs = nullptr, *d_cv = nullptr, *d_cptr = nullptr;
+    std::vector<int32_t> order;                     // instances grouped by class, largest first
+    int32_t cls_first[ND_ENSEMBLE_CLASSES + 1] = {0, 0, 0, 0, 0};
+    ~nd_ensemble() {
+        for (void* p : allocs) (void)hipFree(p);
+    }
+};
+
+namespace {
+
+template <typename T>
+int up(nd_ensemble* en, const std::vector<T>& h, T** d) {
+    *d = nullptr;
+    if (h.empty()) return ND_OK;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, h.size() * sizeof(T));
+    if (e != hipSuccess) return nd::set_error(ND_ERR_ALLOC, "hipMalloc: %s", hipGetErrorString(e));
+    en->allocs.push_back(p);
+    e = hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_err(e, "hipMemcpy");
+    *d = static_cast<T*>(p);
+    return ND_OK;
+}
+
+int class_of(const nd_ensemble* en, int32_t n) {
+    for (int c = 0; c < ND_ENSEMBLE_CLASSES; ++c)
+        if (n <= en->cls_threads[c] * en->cls_vper[c]) return c;
+    return -1;
+}
+
+int regroup(nd_ensemble* en) {
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> byc(ND_ENSEMBLE_CLASSES);
+    for (int32_t i = 0; i < en->ninst; ++i) {
+        const int c = class_of(en, en->inst[i].n);
+        if (c < 0) return nd::set_error(ND_ERR_UNSUPPORTED, "instance %d: %d vertices exceed the largest class", i, en->inst[i].n);
+        en->inst[i].cls = c;
+        byc[c].emplace_back(-en->inst[i].n, i);
+    }
+    en->order.clear();
+    for (int c = 0; c < ND_ENSEMBLE_CLASSES; ++c) {
+        en->cls_first[c] = static_cast<int32_t>(en->order.size());
+        std::sort(byc[c].begin(), byc[c].end());
+        for (auto& x : byc[c]) en->order.push_back(x.second);
+    }
+    en->cls_first[ND_ENSEMBLE_CLASSES] = static_cast<int32_t>(en->order.size());
+    if (en->d_order) {
+        auto it = std::find(en->allocs.begin(), en->allocs.end(), static_cast<void*>(en->d_order));
+        if (it != en->allocs.end()) { (void)hipFree(*it); en->allocs.erase(it); }
+        en->d_order = nullptr;
+    }
+    int rc = up(en, en->order, &en->d_order);
+    if (rc) return rc;
+    if (en->d_inst) {
+        return hip_err(hipMemcpy(en->d_inst, en->inst.data(), en->inst.size() * sizeof(InstDesc), hipMemcpyHostToDevice),
+                       "hipMemcpy");
+    }
+    return ND_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nd_ensemble_create(int32_t ninst, const int64_t* nv, const int64_t* ne, const int64_t* src, const int64_t* dst,
+                       const nd_kind* vkind, const nd_kind* ekind, nd_ensemble** out) {
+    nd::clear_error();
+    if (!out || ninst < 1 || !nv || !ne || !vkind || !ekind) return nd::set_error(ND_ERR_ARGUMENT, "bad arguments");
+    *out = nullptr;
+    if (vkind->is_edge || !ekind->is_edge) return nd::set_error(ND_ERR_ARGUMENT, "kinds: need a vertex and an edge kind");
+    if (vpar(vkind->model) < 0 || vkind->pdim != vpar(vkind->model) || vkind->ff != ND_FF_PURESTATEMAP ||
+        vkind->outdim_dst != 1 || vkind->mask_dst[0] != 1)
+        return nd::set_error(ND_ERR_UNSUPPORTED, "ensemble vertex model %d (needs Kuramoto or swing with output x[1])",
+                             vkind->model);
+    if (epar(ekind->model) < 0 || ekind->pdim != epar(ekind->model) || ekind->wrapper != ND_WRAP_ANTISYMMETRIC ||
+        ekind->outdim_dst != 1 || ekind->dim != 0)
+        return nd::set_error(ND_ERR_UNSUPPORTED, "ensemble edge model %d (needs an AntiSymmetric Kuramoto or static line)",
+                             ekind->model);
+    nd_ensemble* en = new (std::nothrow) nd_ensemble;
+    if (!en) return nd::set_error(ND_ERR_ALLOC, "allocation failed");
+    (void)hipGetDevice(&en->device);
+    en->ninst = ninst;
+    en->vk = *vkind;
+    en->ek = *ekind;
+    en->dim = vkind->dim;
+    std::vector<uint16_t> nbr, eix, cs, cv, cptr;
+    int64_t eoff = 0, uoff = 0, poff = 0;
+    for (int32_t i = 0; i < ninst; ++i) {
+        const int64_t n = nv[i], m = ne[i];
+        if (n < 1 || n > 32767 || m < 0 || 2 * m > 65535) {
+            delete en;
+            return nd::set_error(ND_ERR_UNSUPPORTED, "instance %d: sizes outside the ensemble kernel limits", i);
+        }
+        // destination lists in the reference aggregation order: increasing edge index,
+        // src output before dst output (aggregators.jl:54-86, 140-151)
+        std::vector<int32_t> deg(static_cast<size_t>(n) + 1, 0);
+        for (int64_t e = 0; e < m; ++e) {
+            const int64_t s = src[eoff + e] - 1, d = dst[eoff + e] - 1;
+            if (s < 0 || s >= n || d < 0 || d >= n) {
+                delete en;
+                return nd::set_error(ND_ERR_ARGUMENT, "instance %d: edge endpoint out of range", i);
+            }
+            deg[s + 1]++;
+            deg[d + 1]++;
+        }
+        for (int64_t v = 0; v < n; ++v) deg[v + 1] += deg[v];
+        std::vector<int32_t> fill(deg.begin(), deg.end() - 1);
+        std::vector<uint16_t> ln(static_cast<size_t>(2 * m)), le(static_cast<size_t>(2 * m));
+        for (int64_t e = 0; e < m; ++e) {
+            const int64_t s = src[eoff + e] - 1, d = dst[eoff + e] - 1;
+            ln[fill[s]] = static_cast<uint16_t>(d | 0x8000);
+            le[fill[s]++] = static_cast<uint16_t>(e);
+            ln[fill[d]] = static_cast<uint16_t>(s);
+            le[fill[d]++] = static_cast<uint16_t>(e);
+        }
+        InstDesc I{};
+        I.n = static_cast<int32_t>(n);
+        I.ne = static_cast<int32_t>(m);
+        I.nslot = static_cast<int32_t>(2 * m);
+        I.uoff = uoff;
+        I.poff = poff;
+        I.soff = static_cast<int64_t>(nbr.size());
+        I.csoff = static_cast<int64_t>(cs.size());
+        I.cvoff = static_cast<int64_t>(cv.size());
+        I.cpoff = static_cast<int64_t>(cptr.size());
+        nbr.insert(nbr.end(), ln.begin(), ln.end());
+        eix.insert(eix.end(), le.begin(), le.end());
+        int32_t nch = 0;
+        for (int64_t v = 0; v < n; ++v) {
+            cptr.push_back(static_cast<uint16_t>(nch));
+            for (int32_t b = deg[v]; b < deg[v + 1]; b += kChunk) {
+                cs.push_back(static_cast<uint16_t>(b));
+                cv.push_back(static_cast<uint16_t>(v));
+                ++nch;
+            }
+        }
+        cptr.push_back(static_cast<uint16_t>(nch));
+        cs.push_back(static_cast<uint16_t>(2 * m));
+        if (nch > 65535) {
+            delete en;
+            return nd::set_error(ND_ERR_UNSUPPORTED, "instance %d: too many chunks", i);
+        }
+        I.nchunk = nch;
+        const int npe = epar(ekind->model);
+        I.lds_bytes = static_cast<int32_t>(8 * (n + m * npe + nch + 32) + 2 * (2 * 2 * m + 2 * nch + 1 + n + 1));
+        en->inst.push_back(I);
+        eoff += m;
+        uoff += n * vkind->dim;
+        poff += n * vkind->pdim + m * ekind->pdim;
+    }
+    en->total_states = uoff;
+    en->total_params = poff;
+    int rc = regroup(en);
+    rc = rc ? rc : up(en, en->inst, &en->d_inst);
+    rc = rc ? rc : up(en, nbr, &en->d_nbr);
+    rc = rc ? rc : up(en, eix, &en->d_e);
+    rc = rc ? rc : up(en, cs, &en->d_cs);
+    rc = rc ? rc : up(en, cv, &en->d_cv);
+    rc = rc ? rc : up(en, cptr, &en->d_cptr);
+    if (rc) { delete en; return rc; }
+    *out = en;
+    return ND_OK;
+}
+
+void nd_ensemble_destroy(nd_ensemble* en) { delete en; }
+
+int nd_ensemble_sizes(const nd_ensemble* en, int64_t* total_states, int64_t* total_params, int64_t* state_offsets,
+                      int64_t* param_offsets) {
+    if (!en) return nd::set_error(ND_ERR_ARGUMENT, "NULL ensemble");
+    if (total_states) *total_states = en->total_states;
+    if (total_params) *total_params = en->total_params;
+    for (int32_t i = 0; i < en->ninst; ++i) {
+        if (state_offsets) state_offsets[i] = en->inst[i].uoff;
+        if (param_offsets) param_offsets[i] = en->inst[i].poff;
+    }
+    return ND_OK;
+}
+
+int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t vper) {
+    if (!en || cls < 0 || cls >= ND_ENSEMBLE_CLASSES) return nd::set_error(ND_ERR_ARGUMENT, "bad class");
+    if (threads < kWave || threads > 512 || threads % kWave || !(vper == 1 || vper == 2 || vper == 4))
+        return nd::set_error(ND_ERR_ARGUMENT, "threads must be a multiple of 64 in [64,512], vper in {1,2,4}");
+    en->cls_threads[cls] = threads;
+    en->cls_vper[cls] = vper;
+    return regroup(en);
+}
+
+int nd_ensemble_solve(nd_ensemble* en, double* u, const double* p, double t0, double t1, const nd_ensemble_opts* o,
+                      nd_ensemble_stat* stats, void* stream) {
+    if (!en || !u || !p || !o || !stats) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
+    if (!(t1 >= t0)) return nd::set_error(ND_ERR_ARGUMENT, "t1 < t0");
+    if (o->method != ND_METHOD_TSIT5 && o->method != ND_METHOD_RK4) return nd::set_error(ND_ERR_ARGUMENT, "unknown method");
+    if (o->method == ND_METHOD_RK4 && !(o->dt0 > 0.0)) return nd::set_error(ND_ERR_ARGUMENT, "RK4 needs dt0 > 0");
+    int dev = -1;
+    (void)hipGetDevice(&dev);
+    if (dev != en->device) return nd::set_error(ND_ERR_ARGUMENT, "ensemble lives on device %d", en->device);
+    SolveArgs a{};
+    a.inst = en->d_inst;
+    a.order = en->d_order;
+    a.slot_nbr = en->d_nbr;
+    a.slot_e = en->d_e;
+    a.cs = en->d_cs;
+    a.cv = en->d_cv;
+    a.cptr = en->d_cptr;
+    a.u = u;
+    a.p = p;
+    a.stats = stats;
+    a.t0 = t0;
+    a.t1 = t1;
+    a.dt0 = o->dt0;
+    a.reltol = o->reltol > 0 ? o->reltol : 1e-3;
+    a.abstol = o->abstol > 0 ? o->abstol : 1e-6;
+    a.dtmax = o->dtmax > 0 ? o->dtmax : (t1 - t0);
+    a.dtmin = o->dtmin > 0 ? o->dtmin : 1e-14 * std::max(1.0, std::fabs(t1));
+    a.maxiters = o->maxiters > 0 ? o->maxiters : 1000000;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (int c = 0; c < ND_ENSEMBLE_CLASSES; ++c) {
+        const int32_t cnt = en->cls_first[c + 1] - en->cls_first[c];
+        if (cnt == 0) continue;
+        EnsKernel k = pick_ensemble(en->vk.model, en->ek.model, o->method, en->cls_vper[c]);
+        if (!k) return nd::set_error(ND_ERR_UNSUPPORTED, "no ensemble kernel for these models");
+        int32_t lds = 0;
+        for (int32_t q = en->cls_first[c]; q < en->cls_first[c + 1]; ++q)
+            lds = std::max(lds, en->inst[en->order[q]].lds_bytes);
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return hip_err(e, "hipFuncSetAttribute(LDS)");
+        a.first = en->cls_first[c];
+        a.count = cnt;
+        hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(cnt)), dim3(static_cast<unsigned>(en->cls_threads[c])),
+                           static_cast<unsigned>(lds), s, a);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_err(e, "ensemble launch");
+    }
+    return ND_OK;
+}
+
+}  // extern "C"

@@ -33,9 +33,11 @@
 #include <vector>
 
 #include "nd_internal.h"
+#include "nd_models.h"
 
 namespace {
 
+using namespace nd;
 constexpr int kWave = 64;
 
 struct VDesc {            // one vertex batch as the kernels see it (0-based offsets)
@@ -60,45 +62,19 @@ struct EDesc {            // one edge batch
     const int2* ends;     // per edge (batch order): u-index of the src / dst vertex output
 };
 
-__host__ __device__ constexpr bool is_ode_model(int em) {
-    return em == ND_EM_ODE_SIN_DIFFUSION || em == ND_EM_ODE_SIN_DIFFUSION_FAST || em == ND_EM_ODE_LINEAR;
-}
-
-// dst-side output of a pure feed-forward edge model g(v_s, v_d, p)
-template <int EM>
-__device__ __forceinline__ double g_dst(double vs, double vd, const double* __restrict__ pe) {
-    if constexpr (EM == ND_EM_KURAMOTO) return pe[0] * sin(vs - vd);             // ComponentLibrary.jl:51-53
-    else if constexpr (EM == ND_EM_STATIC_LINE) return pe[0] * sin(vs - vd - pe[1]);
-    else if constexpr (EM == ND_EM_DIFFUSION) return vs - vd;                     // diffusion_test.jl:57-60
-    else if constexpr (EM == ND_EM_DIFFUSION_WEIGHTED) return pe[0] * (vs - vd);  // ComponentLibrary.jl:8-10
-    else if constexpr (EM == ND_EM_DIFFUSION_FIDUCIAL) return pe[0] * (vs - vd);  // ComponentLibrary.jl:22-25
-    else return __builtin_nan("");
-}
-
-__device__ __forceinline__ double g_dst_rt(int em, double vs, double vd, const double* __restrict__ pe) {
-    switch (em) {
-    case ND_EM_KURAMOTO: return g_dst<ND_EM_KURAMOTO>(vs, vd, pe);
-    case ND_EM_STATIC_LINE: return g_dst<ND_EM_STATIC_LINE>(vs, vd, pe);
-    case ND_EM_DIFFUSION: return g_dst<ND_EM_DIFFUSION>(vs, vd, pe);
-    case ND_EM_DIFFUSION_WEIGHTED: return g_dst<ND_EM_DIFFUSION_WEIGHTED>(vs, vd, pe);
-    case ND_EM_DIFFUSION_FIDUCIAL: return g_dst<ND_EM_DIFFUSION_FIDUCIAL>(vs, vd, pe);
-    default: return __builtin_nan("");
-    }
-}
-
 // Value of one destination-list entry for the vertex whose own output is `own`.
 // s.x: u-index of the other endpoint's output (feed-forward edges) or of the edge
 //      state element this side outputs (ODE edges);
 // s.y: (edge parameter offset << 1) | (1 if this vertex is the edge's src).
 template <int EM, int WRAP>
 __device__ __forceinline__ double slot_value(int2 s, double own, const double* __restrict__ u,
-                                             const double* __restrict__ p) {
+                                             const double* __restrict__ vo, const double* __restrict__ p) {
     if constexpr (EM == ND_EM_NOOP) {
         return __builtin_nan("");          // g writes nothing: o keeps its NaN fill
     } else if constexpr (is_ode_model(EM)) {
         return u[s.x];                     // Fiducial(StateMask, StateMask)
     } else {
-        const double nbr = u[s.x];
+        const double nbr = vo[s.x];
         const double* pe = p + (s.y >> 1);
         if (s.y & 1) {
             const double y = g_dst<EM>(own, nbr, pe);
@@ -109,49 +85,16 @@ __device__ __forceinline__ double slot_value(int2 s, double own, const double* _
 }
 
 __device__ __forceinline__ double slot_value_rt(const EDesc& e, int2 s, double own, const double* __restrict__ u,
-                                                const double* __restrict__ p) {
+                                                const double* __restrict__ vo, const double* __restrict__ p) {
     if (e.model == ND_EM_NOOP) return __builtin_nan("");
     if (e.ode) return u[s.x];
-    const double nbr = u[s.x];
+    const double nbr = vo[s.x];
     const double* pe = p + (s.y >> 1);
     if (s.y & 1) {
         const double y = g_dst_rt(e.model, own, nbr, pe);
         return e.wrap == ND_WRAP_SYMMETRIC ? y : -y;
     }
     return g_dst_rt(e.model, nbr, own, pe);
-}
-
-template <int VM>
-struct VModel { static constexpr int dim = -1; };
-template <> struct VModel<ND_VM_DIFFUSION> { static constexpr int dim = 1; };
-template <> struct VModel<ND_VM_KURAMOTO> { static constexpr int dim = 1; };
-template <> struct VModel<ND_VM_SWING> { static constexpr int dim = 2; };
-template <> struct VModel<ND_VM_SWING_UNIT_DAMPING> { static constexpr int dim = 2; };
-template <> struct VModel<ND_VM_LOAD> { static constexpr int dim = 1; };
-
-// vertex f (apply_compf, coreloop.jl:213-216); operation order as the reference writes it
-template <int VM>
-__device__ __forceinline__ void vertex_f(double* __restrict__ dx, const double* __restrict__ x, double acc,
-                                         const double* __restrict__ pv, int dim) {
-    if constexpr (VM == ND_VM_DIFFUSION) {                 // ComponentLibrary.jl:42-45
-        dx[0] = acc;
-    } else if constexpr (VM == ND_VM_KURAMOTO) {           // ComponentLibrary.jl:69-71
-        dx[0] = pv[0] + acc;
-    } else if constexpr (VM == ND_VM_SWING) {              // ComponentLibrary.jl:59-63
-        const double w = x[1];
-        dx[0] = w;
-        dx[1] = 1.0 / pv[0] * (pv[2] - pv[1] * w + acc);
-    } else if constexpr (VM == ND_VM_SWING_UNIT_DAMPING) { // benchmark_models.jl:37-41
-        const double w = x[1];
-        dx[0] = w;
-        double d = pv[0] - 1.0 * w;
-        d += acc;
-        dx[1] = d;
-    } else if constexpr (VM == ND_VM_LOAD) {
-        dx[0] = 1.0 / pv[0] * (pv[1] + acc);
-    } else {                                               // f does nothing: du keeps its zero fill
-        for (int j = 0; j < dim; ++j) dx[j] = 0.0;
-    }
 }
 
 // Explicit Runge-Kutta stage epilogue fused into the RHS kernels.  kind 0: plain
@@ -204,13 +147,6 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Parameters each edge model reads (p entries per edge).
-template <int EM> struct EModel { static constexpr int np = 0; };
-template <> struct EModel<ND_EM_KURAMOTO> { static constexpr int np = 1; };
-template <> struct EModel<ND_EM_STATIC_LINE> { static constexpr int np = 2; };
-template <> struct EModel<ND_EM_DIFFUSION_WEIGHTED> { static constexpr int np = 1; };
-template <> struct EModel<ND_EM_DIFFUSION_FIDUCIAL> { static constexpr int np = 1; };
-
 // Entry value from operands already in registers (nbr: u[s.x]; k: the edge parameters).
 template <int EM, int WRAP>
 __device__ __forceinline__ double slot_eval(int2 s, double nbr, const double* k, double own) {
@@ -236,10 +172,10 @@ template <int EM, int WRAP, bool MULTI>
 __device__ __forceinline__ double segment_sum(int32_t beg, int32_t end, int32_t t, int32_t G, double own,
                                               const int2* __restrict__ slots, const uint8_t* __restrict__ slot_eb,
                                               const EDesc* __restrict__ ed, const double* __restrict__ u,
-                                              const double* __restrict__ p) {
+                                              const double* __restrict__ vo, const double* __restrict__ p) {
     double acc = 0.0;
     if constexpr (MULTI) {
-        for (int32_t s = beg + t; s < end; s += G) acc += slot_value_rt(ed[slot_eb[s]], slots[s], own, u, p);
+        for (int32_t s = beg + t; s < end; s += G) acc += slot_value_rt(ed[slot_eb[s]], slots[s], own, u, vo, p);
     } else {
         constexpr int NP = EModel<EM>::np;
         for (int32_t s0 = beg + t; s0 < end; s0 += kUnroll * G) {
@@ -252,7 +188,8 @@ __device__ __forceinline__ double segment_sum(int32_t beg, int32_t end, int32_t 
 #pragma unroll
             for (int j = 0; j < kUnroll; ++j)
                 if (s0 + j * G < end) {
-                    if constexpr (EM != ND_EM_NOOP) nb[j] = u[sl[j].x];
+                    if constexpr (is_ode_model(EM)) nb[j] = u[sl[j].x];
+                        else if constexpr (EM != ND_EM_NOOP) nb[j] = vo[sl[j].x];
 #pragma unroll
                     for (int c = 0; c < NP; ++c) kk[j][c] = p[(sl[j].y >> 1) + c];
                 }
@@ -276,7 +213,8 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
                                                     const int2* __restrict__ slots,
                                                     const uint8_t* __restrict__ slot_eb,
                                                     const EDesc* __restrict__ ed,
-                                                    const double* __restrict__ u, const double* __restrict__ p,
+                                                    const double* __restrict__ u, const double* __restrict__ vo,
+                                                    const double* __restrict__ p,
                                                     double* __restrict__ du, StageArgs sa) {
     constexpr int DIMC = VModel<VM>::dim;
     const int dim = DIMC > 0 ? DIMC : vd.dim;
@@ -292,7 +230,7 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
             const double own = u[x0 + vd.mask[0]];
             const int32_t t = static_cast<int32_t>(threadIdx.x);
             const double part = wave_sum(segment_sum<EM, WRAP, MULTI>(beg, end, t, static_cast<int32_t>(blockDim.x),
-                                                                      own, slots, slot_eb, ed, u, p));
+                                                                      own, slots, slot_eb, ed, u, vo, p));
             if ((t & (kWave - 1)) == 0) wpart[t / kWave] = part;
             __syncthreads();
             if (t == 0) {
@@ -312,7 +250,7 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
             const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
             const double own = u[x0 + vd.mask[0]];
             const double acc = wave_sum(segment_sum<EM, WRAP, MULTI>(beg, end, lane, kWave, own, slots, slot_eb,
-                                                                     ed, u, p));
+                                                                     ed, u, vo, p));
             if (lane == 0) vertex_emit<VM, STAGE>(du, sa, x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim);
             return;
         }
@@ -369,7 +307,8 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
                 for (int j = 0; j < 4; ++j) {
                     const int32_t s2 = base + j * T + t;
                     if (s2 < wend) {
-                        if constexpr (EM != ND_EM_NOOP) nb[j] = u[sl[j].x];
+                        if constexpr (is_ode_model(EM)) nb[j] = u[sl[j].x];
+                        else if constexpr (EM != ND_EM_NOOP) nb[j] = vo[sl[j].x];
 #pragma unroll
                         for (int c = 0; c < NP; ++c) kk[j][c] = p[(sl[j].y >> 1) + c];
                     }
@@ -381,7 +320,7 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
                 if (s2 < wend) {
                     const double ow = s_own[s_owner[s2 - base]];
                     double v;
-                    if constexpr (MULTI) v = slot_value_rt(ed[vd.lseb[s2]], sl[j], ow, u, p);
+                    if constexpr (MULTI) v = slot_value_rt(ed[vd.lseb[s2]], sl[j], ow, u, vo, p);
                     else v = slot_eval<EM, WRAP>(sl[j], nb[j], kk[j], ow);
                     s_val[s2 - base] = v;
                 }
@@ -498,7 +437,7 @@ __global__ void __launch_bounds__(256) k_aggregate(const int32_t* __restrict__ r
 // ---- kernel selection ----------------------------------------------------------
 
 using VertexKernel = void (*)(VDesc, const int32_t*, const int2*, const uint8_t*, const EDesc*, const double*,
-                              const double*, double*, StageArgs);
+                              const double*, const double*, double*, StageArgs);
 
 template <int VM, int EM, int WRAP, bool STAGE>
 VertexKernel pick3(bool multi) {
@@ -581,13 +520,14 @@ struct nd_plan {
     int64_t dim = 0, pdim = 0, lastidx_out = 0, lastidx_aggr = 0, lastidx_gbuf = 0;
     int32_t vdepth = 0, edepth = 0;
     bool multi = false;
-    int32_t hub_degree = 16;        // longer segments: one wave per vertex
-    int32_t big_hub_degree = 256;   // longer segments: one block per vertex
+    int32_t hub_degree = 32;        // longer segments: one wave per vertex
+    int32_t big_hub_degree = 128;   // longer segments: one block per vertex
     int32_t block = 256;
     std::vector<VDesc> vdesc;
     std::vector<EDesc> edesc;
     std::vector<VertexKernel> vkern, vkern_stage;
     double* rk_work = nullptr;
+    double* d_vout = nullptr;      // compact vertex outputs (vdepth 1 networks with functional edges)
     std::vector<int32_t> rowptr_host;
     std::vector<int2> slots_host;
     std::vector<uint8_t> slot_eb_host;
@@ -720,13 +660,18 @@ inline unsigned blocks_for(int64_t n, int block) { return static_cast<unsigned>(
 void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const double* p, const StageArgs& sa,
                 hipStream_t s) {
     const int B = pl->block;
+    // vertex outputs o_v = u[StateMask] into a compact array (coreloop.jl:39): the edge
+    // evaluations gather neighbour outputs from it instead of from the interleaved state
+    if (pl->d_vout)
+        for (const VDesc& vd : pl->vdesc)
+            if (vd.count) hipLaunchKernelGGL(k_vertex_out, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, u, pl->d_vout);
     for (size_t b = 0; b < pl->vdesc.size(); ++b) {
         const VDesc& vd = pl->vdesc[b];
         if (vd.count == 0) continue;
         const int64_t nblocks = vd.nbig + blocks_for(static_cast<int64_t>(vd.nmid) * kWave, B) + blocks_for(vd.count, B);
         hipLaunchKernelGGL(stage ? pl->vkern_stage[b] : pl->vkern[b], dim3(static_cast<unsigned>(nblocks)),
                            dim3(static_cast<unsigned>(B)), 0, s, vd, pl->d_rowptr, pl->d_slots, pl->d_slot_eb,
-                           pl->d_edesc, u, p, du, sa);
+                           pl->d_edesc, u, pl->d_vout, p, du, sa);
     }
     for (const EDesc& e : pl->edesc) {
         if (!e.ode || e.count == 0) continue;
@@ -850,7 +795,7 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
                     sl.x = static_cast<int32_t>(n.e_state_first(e) - 1 + (is_src ? kk.mask_src[j] : kk.mask_dst[j]) - 1);
                 } else if (kk.model != ND_EM_NOOP) {
                     const int64_t other = is_src ? n.dst[e] - 1 : n.src[e] - 1;
-                    sl.x = static_cast<int32_t>(nd::vertex_out_u_index(n, other, 0));
+                    sl.x = static_cast<int32_t>(n.v_out_first(other) - 1);
                     sl.y = static_cast<int32_t>(((n.e_param_first(e) - 1) << 1) | (is_src ? 1 : 0));
                 }
                 slots[s] = sl;
@@ -903,6 +848,13 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
     }
     rc = rebuild_hubs(pl);
     if (rc) { delete pl; return rc; }
+    if (n.vdepth == 1 && n.ne > 0) {
+        void* w = nullptr;
+        hipError_t e = hipMalloc(&w, static_cast<size_t>(n.nv) * sizeof(double));
+        if (e != hipSuccess) { delete pl; return nd::set_error(ND_ERR_ALLOC, "hipMalloc(vertex outputs): %s", hipGetErrorString(e)); }
+        pl->allocations.push_back(w);
+        pl->d_vout = static_cast<double*>(w);
+    }
     *out = pl;
     return ND_OK;
 }

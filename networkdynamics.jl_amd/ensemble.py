@@ -1,0 +1,166 @@
+"""Batched ensembles of independent networks (basin-stability style), and the
+per-instance reference path they are compared with.
+
+`Ensemble(graphs, vertexm, edgem)` -> one device integrator for all instances
+(nd_ensemble_* in include/nd_hip.h).  `tsit5_per_instance` integrates one
+network at a time with the network RHS kernel and host-side step control: the
+plain per-instance GPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from . import components as comp
+
+TSIT5, RK4 = 0, 1
+
+# Tsitouras (2011) 5(4) pair: stage matrix, solution weights b (= last stage row,
+# first-same-as-last) and error weights btilde = b - bhat.
+TSIT5_A = [
+    [],
+    [0.161],
+    [-0.008480655492356989, 0.335480655492357],
+    [2.897153057105493, -6.359448489975075, 4.3622954328695815],
+    [5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525],
+    [5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383],
+    [0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774],
+]
+TSIT5_BTILDE = [-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995, -0.1447110071732629,
+                0.5823571654525552, -0.45808210592918697, 0.015151515151515152]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("method", C.c_int32), ("pad_", C.c_int32), ("reltol", C.c_double), ("abstol", C.c_double),
+                ("dt0", C.c_double), ("dtmax", C.c_double), ("dtmin", C.c_double), ("maxiters", C.c_int64)]
+
+
+STAT_DTYPE = np.dtype([("t", "f8"), ("dt", "f8"), ("naccept", "i8"), ("nreject", "i8"), ("nf", "i8"),
+                       ("retcode", "i4"), ("pad", "i4")])
+
+
+def _kind_struct(m):
+    k = A.NDKind()
+    d = m.kind()
+    for f in ("is_edge", "model", "wrapper", "ff", "dim", "pdim", "outdim_src", "outdim_dst"):
+        setattr(k, f, int(d[f]))
+    for j, x in enumerate(d["mask_src"]):
+        k.mask_src[j] = x
+    for j, x in enumerate(d["mask_dst"]):
+        k.mask_dst[j] = x
+    return k
+
+
+class Ensemble:
+    """Independent networks sharing one vertex model and one edge model."""
+
+    def __init__(self, graphs, vertexm, edgem):
+        if not isinstance(vertexm, comp.VertexModel) or not isinstance(edgem, comp.EdgeModel):
+            raise A.ArgumentError("Ensemble needs one VertexModel and one EdgeModel")
+        self.graphs = list(graphs)
+        self.vertexm, self.edgem = vertexm, edgem
+        self.ninst = len(self.graphs)
+        nv = np.array([g.nv() for g in self.graphs], np.int64)
+        ne = np.array([g.ne() for g in self.graphs], np.int64)
+        src = np.ascontiguousarray(np.concatenate([g.src for g in self.graphs]) if ne.sum() else np.zeros(1, np.int64))
+        dst = np.ascontiguousarray(np.concatenate([g.dst for g in self.graphs]) if ne.sum() else np.zeros(1, np.int64))
+        self.nv, self.ne = nv, ne
+        self._vk, self._ek = _kind_struct(vertexm), _kind_struct(edgem)
+        h = C.c_void_p()
+        L = A.lib()
+        A.check(L.nd_ensemble_create(self.ninst, nv.ctypes.data, ne.ctypes.data, src.ctypes.data, dst.ctypes.data,
+                                     C.byref(self._vk), C.byref(self._ek), C.byref(h)))
+        self._h = h
+        ts, tp = C.c_int64(), C.c_int64()
+        self.state_offsets = np.zeros(self.ninst, np.int64)
+        self.param_offsets = np.zeros(self.ninst, np.int64)
+        A.check(L.nd_ensemble_sizes(h, C.byref(ts), C.byref(tp), self.state_offsets.ctypes.data,
+                                    self.param_offsets.ctypes.data))
+        self.total_states, self.total_params = ts.value, tp.value
+
+    def __del__(self):
+        L = getattr(A, "_lib", None) if A is not None else None
+        if L is not None and getattr(self, "_h", None):
+            L.nd_ensemble_destroy(self._h)
+            self._h = None
+
+    def set_class(self, cls, threads, vper):
+        A.check(A.lib().nd_ensemble_set_class(self._h, int(cls), int(threads), int(vper)))
+
+    def state_slice(self, i):
+        return slice(int(self.state_offsets[i]), int(self.state_offsets[i] + self.nv[i] * self.vertexm.dim))
+
+    def param_slice(self, i):
+        n = self.nv[i] * self.vertexm.pdim + self.ne[i] * self.edgem.pdim
+        return slice(int(self.param_offsets[i]), int(self.param_offsets[i] + n))
+
+    def solve(self, u, p, tspan, method=TSIT5, reltol=1e-8, abstol=1e-8, dt=0.0, dtmax=0.0, maxiters=0):
+        """Integrates all instances in place on the device vector u; returns per-instance stats."""
+        import torch
+        if u.dtype != torch.float64 or p.dtype != torch.float64 or u.device.type != "cuda":
+            raise A.ArgumentError("u and p must be float64 device tensors")
+        if u.numel() != self.total_states or p.numel() != self.total_params:
+            raise A.ArgumentError("u / p do not match the ensemble's sizes")
+        stats = torch.zeros(self.ninst * STAT_DTYPE.itemsize // 8, dtype=torch.float64, device=u.device)
+        o = _Opts(int(method), 0, float(reltol), float(abstol), float(dt), float(dtmax), 0.0, int(maxiters))
+        stream = torch.cuda.current_stream(u.device).cuda_stream
+        A.check(A.lib().nd_ensemble_solve(self._h, u.data_ptr(), p.data_ptr(), float(tspan[0]), float(tspan[1]),
+                                          C.byref(o), stats.data_ptr(), stream))
+        return stats
+
+    @staticmethod
+    def stats_to_numpy(stats):
+        return stats.cpu().numpy().view(STAT_DTYPE)
+
+
+def tsit5_per_instance(nw, u, p, tspan, reltol=1e-8, abstol=1e-8, dt0=0.0, maxiters=10 ** 7):
+    """Adaptive Tsit5 for one network: device RHS per stage, host-side step control
+    (same tableau and PI controller as the ensemble kernel).  u: device vector, in place."""
+    import torch
+    t0, t1 = float(tspan[0]), float(tspan[1])
+    ks = [torch.empty_like(u) for _ in range(7)]
+    ys = torch.empty_like(u)
+    nw(ks[0], u, p, t0)
+    n = u.numel()
+
+    def norm(x, ref):
+        return torch.sqrt(torch.sum((x / (abstol + ref.abs() * reltol)) ** 2) / n).item()
+
+    dt = dt0
+    if not dt > 0:
+        d0, d1 = norm(u, u), norm(ks[0], u)
+        h0 = 1e-6 if (d0 < 1e-5 or d1 < 1e-5) else 0.01 * d0 / d1
+        h0 = min(h0, t1 - t0)
+        torch.add(u, ks[0], alpha=h0, out=ys)
+        nw(ks[1], ys, p, t0 + h0)
+        d2 = norm(ks[1] - ks[0], u) / h0
+        m = max(d1, d2)
+        h1 = max(1e-6, h0 * 1e-3) if m <= 1e-15 else (0.01 / m) ** (1.0 / 6.0)
+        dt = min(100 * h0, h1)
+    dt = min(dt, t1 - t0)
+    t, qold, naccept, nreject, it = t0, 1e-4, 0, 0, 0
+    while t < t1 and it < maxiters:
+        it += 1
+        last = t + dt >= t1
+        h = t1 - t if last else dt
+        for s in range(1, 7):
+            ys.copy_(u)
+            for j, a in enumerate(TSIT5_A[s]):
+                ys.add_(ks[j], alpha=h * a)
+            nw(ks[s], ys, p, t + h)
+        err = sum(bt * k for bt, k in zip(TSIT5_BTILDE, ks)) * h
+        sc = abstol + torch.maximum(u.abs(), ys.abs()) * reltol
+        eest = torch.sqrt(torch.sum((err / sc) ** 2) / n).item()
+        q11 = eest ** 0.14
+        if eest <= 1.0:
+            qq = 0.1 if eest == 0 else max(0.1, min(5.0, q11 / qold ** 0.08 / 0.9))
+            t = t1 if last else t + h
+            u.copy_(ys)
+            ks[0].copy_(ks[6])
+            qold = max(eest, 1e-4)
+            dt = h / qq
+            naccept += 1
+        else:
+            dt = h / min(5.0, q11 / 0.9)
+            nreject += 1
+    return dict(t=t, naccept=naccept, nreject=nreject)

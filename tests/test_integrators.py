@@ -1,0 +1,109 @@
+"""Time integration: Tsit5 tableau pinned by its order conditions (CPU), the batched
+ensemble integrator against per-instance references (GPU)."""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import oracle_for
+
+
+def _tableau(ND):
+    A_ = ND.ensemble.TSIT5_A
+    s = 7
+    A = np.zeros((s, s))
+    for i, row in enumerate(A_):
+        A[i, :len(row)] = row
+    b = A[6].copy()
+    bt = np.array(ND.ensemble.TSIT5_BTILDE)
+    c = A.sum(axis=1)
+    return A, b, b - bt, c
+
+
+def _order_conditions(A, b, c, order):
+    """Butcher order conditions up to `order` (trees of order <= 5), as residuals."""
+    r = []
+    e = np.ones(len(b))
+    Ac, Ac2, Ac3 = A @ c, A @ c ** 2, A @ c ** 3
+    AAc = A @ Ac
+    if order >= 1:
+        r += [b @ e - 1]
+    if order >= 2:
+        r += [b @ c - 1 / 2]
+    if order >= 3:
+        r += [b @ c ** 2 - 1 / 3, b @ Ac - 1 / 6]
+    if order >= 4:
+        r += [b @ c ** 3 - 1 / 4, b @ (c * Ac) - 1 / 8, b @ Ac2 - 1 / 12, b @ AAc - 1 / 24]
+    if order >= 5:
+        r += [b @ c ** 4 - 1 / 5, b @ (c ** 2 * Ac) - 1 / 10, b @ (c * Ac2) - 1 / 15, b @ (c * AAc) - 1 / 30,
+              b @ (Ac * Ac) - 1 / 20, b @ Ac3 - 1 / 20, b @ (A @ (c * Ac)) - 1 / 40, b @ (A @ Ac2) - 1 / 60,
+              b @ (A @ AAc) - 1 / 120]
+    return np.abs(r)
+
+
+def test_tsit5_tableau_order_conditions(ND):
+    A, b, bhat, c = _tableau(ND)
+    assert np.allclose(np.tril(A, -1), A)                       # explicit
+    assert np.allclose(A[:, 0] + A[:, 1:].sum(axis=1), c)
+    assert _order_conditions(A, b, c, 5).max() < 1e-12          # 5th-order solution
+    assert _order_conditions(A, bhat, c, 4).max() < 1e-12       # 4th-order embedded pair
+    assert _order_conditions(A, bhat, c, 5).max() > 1e-6        # ... which is not 5th order
+    assert abs(b[6]) == 0.0 and np.allclose(A[6, :6], b[:6])    # first same as last
+
+
+def _instances(ND, n_inst, nmin, nmax, seed, m=2):
+    rng = np.random.default_rng(seed)
+    gs = [ND.barabasi_albert(int(n), m, seed=seed * 1000 + i) for i, n in enumerate(rng.integers(nmin, nmax + 1, n_inst))]
+    return gs, rng
+
+
+def _swing_params(ND, gs, rng):
+    ps, us = [], []
+    for g in gs:
+        M, D, P, K = ND.workloads.swing_params(g.nv(), g.ne(), rng)
+        p = np.empty(3 * g.nv() + g.ne())
+        p[0:3 * g.nv():3], p[1:3 * g.nv():3], p[2:3 * g.nv():3], p[3 * g.nv():] = M, D, P, K
+        u = np.empty(2 * g.nv())
+        u[0::2], u[1::2] = rng.uniform(-0.5, 0.5, g.nv()), rng.normal(scale=0.5, size=g.nv())
+        ps.append(p)
+        us.append(u)
+    return ps, us
+
+
+@pytest.mark.gpu
+def test_ensemble_rk4_matches_oracle(ND, dev):
+    import torch
+    gs, rng = _instances(ND, 24, 60, 2000, seed=3)
+    ps, us = _swing_params(ND, gs, rng)
+    ens = ND.Ensemble(gs, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    u = torch.from_numpy(np.concatenate(us)).to(dev)
+    p = torch.from_numpy(np.concatenate(ps)).to(dev)
+    st = ND.Ensemble.stats_to_numpy(ens.solve(u, p, (0.0, 0.5), method=ND.ensemble.RK4, dt=0.005))
+    torch.cuda.synchronize()
+    got = u.cpu().numpy()
+    assert (st["retcode"] == 0).all() and (st["naccept"] == 100).all()
+    for i, g in enumerate(gs):
+        nw = ND.Network(g, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+        ref = oracle_for(nw).rk4(us[i], ps[i], 0.0, 0.005, 100)
+        np.testing.assert_allclose(got[ens.state_slice(i)], ref, rtol=0, atol=1e-11, err_msg=f"instance {i}")
+
+
+@pytest.mark.gpu
+def test_ensemble_tsit5_matches_per_instance(ND, dev):
+    import torch
+    gs, rng = _instances(ND, 12, 50, 600, seed=4)
+    ps, us = _swing_params(ND, gs, rng)
+    ens = ND.Ensemble(gs, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    u = torch.from_numpy(np.concatenate(us)).to(dev)
+    p = torch.from_numpy(np.concatenate(ps)).to(dev)
+    st = ND.Ensemble.stats_to_numpy(ens.solve(u, p, (0.0, 2.0), reltol=1e-8, abstol=1e-8))
+    torch.cuda.synchronize()
+    got = u.cpu().numpy()
+    assert (st["retcode"] == 0).all() and np.allclose(st["t"], 2.0)
+    for i, g in enumerate(gs):
+        nw = ND.Network(g, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+        ui = torch.from_numpy(us[i]).to(dev)
+        info = ND.tsit5_per_instance(nw, ui, torch.from_numpy(ps[i]).to(dev), (0.0, 2.0), reltol=1e-8, abstol=1e-8)
+        # same method and controller: step counts agree closely, solutions to tolerance level
+        assert abs(info["naccept"] - st["naccept"][i]) <= 2 + 0.02 * info["naccept"]
+        np.testing.assert_allclose(got[ens.state_slice(i)], ui.cpu().numpy(), rtol=0, atol=1e-6)
