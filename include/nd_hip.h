@@ -269,6 +269,37 @@ int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t
 int nd_ensemble_solve(nd_ensemble* en, double* u, const double* p, double t0, double t1,
                       const nd_ensemble_opts* opts, nd_ensemble_stat* stats, void* stream);
 
+/* ---- batched fixpoint (Newton) ---------------------------------------------------
+ * Synchronous fixpoints of many Kuramoto-type grids at once (one workgroup per grid),
+ * replacing find_fixpoint's NonlinearSolve Newton (src/initialization.jl:73-144) for
+ * networks of swing / first-order load vertices with AntiSymmetric K*sin lines: with
+ * omega = 0 a state is a fixpoint iff F_i = P_i + sum_j K_ij sin(theta_j - theta_i) = 0.
+ * Grid g: nv[g] vertices, ne[g] edges (1-based endpoints concatenated in src/dst).
+ * theta (in/out), P and scale (residual divisor per vertex: M for swing, tau for load,
+ * so F_i/scale_i is the vertex's du) are concatenated per vertex, K per edge.
+ * Converged when max_i |F_i / scale_i| <= tol; the acceptance of a fixpoint is the
+ * network RHS itself (isfixpoint, src/linear_analysis.jl:8-15).  Up to 5120 vertices
+ * per grid.  stats: device array of ngrid entries. */
+typedef struct nd_fixpoint nd_fixpoint;
+typedef struct {
+    double tol;            /* max |F_i/scale_i| to stop at, 0 = 1e-10                    */
+    double cg_rtol;        /* relative residual of each linear solve, 0 = 1e-12           */
+    int32_t max_newton;    /* 0 = 50                                                       */
+    int32_t cg_maxiter;    /* 0 = 20000                                                    */
+} nd_newton_opts;
+typedef struct {
+    double residual;       /* final max |F_i/scale_i|                                      */
+    int32_t newton_iters;  /* linear solves (Newton steps) taken                           */
+    int32_t cg_iters;      /* conjugate-gradient iterations over all steps                 */
+    int32_t retcode;       /* 0 converged, 1 max_newton reached, 2 line search stalled     */
+    int32_t pad_;
+} nd_newton_stat;
+int nd_fixpoint_create(int32_t ngrid, const int64_t* nv, const int64_t* ne, const int64_t* src, const int64_t* dst,
+                       nd_fixpoint** out);
+void nd_fixpoint_destroy(nd_fixpoint* f);
+int nd_fixpoint_solve(nd_fixpoint* f, double* theta, const double* P, const double* scale, const double* K,
+                      const nd_newton_opts* opts, nd_newton_stat* stats, void* stream);
+
 /* ---- synthetic inputs ----------------------------------------------------------
  * Seeded graph generators for the workloads.  No counterpart in the reference's
  * RHS path (its tests use Graphs.jl generators, whose random streams need Julia);

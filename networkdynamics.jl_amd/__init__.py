@@ -13,6 +13,8 @@ from . import _abi
 from . import library as Lib
 from . import workloads
 from .ensemble import Ensemble, tsit5_per_instance
+from .fixpoint import FixpointSolver, swing_load_grid, isfixpoint_residual
+from . import fixpoint
 from ._abi import ArgumentError, DeviceError, NotConvergedError, UnsupportedError
 from .components import (AntiSymmetric, DeviceFunction, Directed, EdgeModel, FeedForward, Fiducial,
                          NoFeedForward, PureFeedForward, PureStateMap, StateMask, Symmetric, VertexModel,

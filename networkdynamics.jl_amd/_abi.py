@@ -89,6 +89,9 @@ PROTOTYPES = {
     "nd_ensemble_sizes": (C.c_int, [P, P, P, P, P]),
     "nd_ensemble_set_class": (C.c_int, [P, I32, I32, I32]),
     "nd_ensemble_solve": (C.c_int, [P, P, P, D, D, P, P, P]),
+    "nd_fixpoint_create": (C.c_int, [I32, P, P, P, P, C.POINTER(P)]),
+    "nd_fixpoint_destroy": (None, [P]),
+    "nd_fixpoint_solve": (C.c_int, [P, P, P, P, P, P, P, P]),
     "nd_graph_erdos_renyi": (C.c_int, [I64, I64, C.c_uint64, C.POINTER(P)]),
     "nd_graph_barabasi_albert": (C.c_int, [I64, I64, C.c_uint64, C.POINTER(P)]),
     "nd_graph_random_geometric": (C.c_int, [I64, D, C.c_uint64, P, C.POINTER(P)]),

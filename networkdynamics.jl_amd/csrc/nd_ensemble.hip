@@ -55,11 +55,6 @@ struct SolveArgs {
     int32_t first, count;
 };
 
-template <int VM> struct VPar { static constexpr int np = 0; };
-template <> struct VPar<ND_VM_KURAMOTO> { static constexpr int np = 1; };
-template <> struct VPar<ND_VM_SWING> { static constexpr int np = 3; };
-template <> struct VPar<ND_VM_SWING_UNIT_DAMPING> { static constexpr int np = 1; };
-template <> struct VPar<ND_VM_LOAD> { static constexpr int np = 2; };
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
 #pragma unroll

@@ -74,4 +74,11 @@ template <> struct EModel<ND_EM_STATIC_LINE> { static constexpr int np = 2; };
 template <> struct EModel<ND_EM_DIFFUSION_WEIGHTED> { static constexpr int np = 1; };
 template <> struct EModel<ND_EM_DIFFUSION_FIDUCIAL> { static constexpr int np = 1; };
 
+// parameters per vertex model
+template <int VM> struct VPar { static constexpr int np = 0; };
+template <> struct VPar<ND_VM_KURAMOTO> { static constexpr int np = 1; };
+template <> struct VPar<ND_VM_SWING> { static constexpr int np = 3; };
+template <> struct VPar<ND_VM_SWING_UNIT_DAMPING> { static constexpr int np = 1; };
+template <> struct VPar<ND_VM_LOAD> { static constexpr int np = 2; };
+
 }  // namespace nd

@@ -285,7 +285,16 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
         const int32_t se = valid ? (lrp[qc + 1] & kOff) : sb;
         const int32_t S0 = lrp[q0] & kOff;
         const int32_t S1 = lrp[min(q0 + T, vd.count)] & kOff;
-        const double own = valid ? u[x0 + vd.mask[0]] : 0.0;
+        // own state and parameters are loaded up front, in parallel with the offsets
+        constexpr int DC = VModel<VM>::dim > 0 ? VModel<VM>::dim : 1;
+        constexpr int NPV = VPar<VM>::np > 0 ? VPar<VM>::np : 1;
+        double xs[DC], pvs[NPV];
+#pragma unroll
+        for (int j = 0; j < DC; ++j) xs[j] = valid ? u[x0 + j] : 0.0;
+#pragma unroll
+        for (int j = 0; j < NPV; ++j)
+            pvs[j] = (valid && VPar<VM>::np > 0) ? p[vd.param0 + static_cast<int64_t>(q) * vd.pdim + j] : 0.0;
+        const double own = (DC == 1 || vd.mask[0] == 0) ? xs[0] : xs[DC - 1];
         s_own[t] = own;
         double acc = 0.0;
         const int32_t W = 4 * T;
@@ -329,7 +338,7 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
             for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) acc += s_val[s2 - base];
             __syncthreads();
         }
-        if (valid && !hub) vertex_emit<VM, STAGE>(du, sa, x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim);
+        if (valid && !hub) vertex_emit<VM, STAGE>(du, sa, x0, xs, acc, pvs, dim);
     }
 }
 
@@ -520,7 +529,7 @@ struct nd_plan {
     int64_t dim = 0, pdim = 0, lastidx_out = 0, lastidx_aggr = 0, lastidx_gbuf = 0;
     int32_t vdepth = 0, edepth = 0;
     bool multi = false;
-    int32_t hub_degree = 32;        // longer segments: one wave per vertex
+    int32_t hub_degree = 48;        // longer segments: one wave per vertex
     int32_t big_hub_degree = 128;   // longer segments: one block per vertex
     int32_t block = 256;
     std::vector<VDesc> vdesc;

@@ -1,0 +1,52 @@
+"""Batched Newton fixpoints, judged by the network RHS (isfixpoint, linear_analysis.jl:8-15)."""
+import numpy as np
+import pytest
+
+
+def test_swing_load_grid_construction(ND):
+    gr = ND.swing_load_grid(800, seed=3)
+    n = gr.g.nv()
+    assert n > 600 and abs(gr.P.sum()) < 1e-12
+    p = gr.flat_p()
+    nw = gr.network()
+    assert len(p) == nw.pdim and np.isfinite(p).all()
+    assert nw.dim == n + gr.is_gen.sum()
+
+
+@pytest.mark.gpu
+def test_ring_fixpoint_known_answer(ND, dev):
+    # test/initialization_test.jl:15-38: ring of 4, P = [1,-1,1,-1], K = 6 -> each line carries 0.5
+    import torch
+    g = ND.cycle_graph(4)
+    fs = ND.FixpointSolver([g])
+    th = torch.zeros(4, dtype=torch.float64, device=dev)
+    P = torch.tensor([1.0, -1.0, 1.0, -1.0], dtype=torch.float64, device=dev)
+    st = ND.FixpointSolver.stats_to_numpy(fs.solve(th, P, torch.ones_like(P), torch.full((4,), 6.0, dtype=torch.float64, device=dev)))
+    t = th.cpu().numpy()
+    d = np.arcsin(0.5 / 6)
+    assert st["retcode"][0] == 0 and st["residual"][0] <= 1e-10
+    np.testing.assert_allclose(np.diff(t), [-d, d, -d], atol=1e-12)
+
+
+@pytest.mark.gpu
+def test_batched_grids_are_fixpoints(ND, dev):
+    import torch
+    grids = [ND.swing_load_grid(int(n), seed=100 + i) for i, n in enumerate(np.linspace(300, 5000, 10).astype(int))]
+    fs = ND.FixpointSolver([gr.g for gr in grids])
+    th = torch.zeros(int(fs.voff[-1]), dtype=torch.float64, device=dev)
+    P = torch.from_numpy(np.concatenate([gr.P for gr in grids])).to(dev)
+    S = torch.from_numpy(np.concatenate([gr.scale for gr in grids])).to(dev)
+    K = torch.from_numpy(np.concatenate([gr.K for gr in grids])).to(dev)
+    st = ND.FixpointSolver.stats_to_numpy(fs.solve(th, P, S, K))
+    ok = st["retcode"] == 0
+    # a grid without a synchronous state (a line would need |flow| > K) must be reported,
+    # never returned as converged
+    assert ok.sum() >= 8, st
+    assert (st["residual"][~ok] > 1e-10).all()
+    t = th.cpu().numpy()
+    for i, gr in enumerate(grids):
+        if not ok[i]:
+            continue
+        ti = t[fs.voff[i]:fs.voff[i + 1]]
+        assert ND.isfixpoint_residual(gr, ti, dev) <= 1e-10
+        assert 1 <= st["newton_iters"][i] <= 20
