@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c2 (default): the N=1e6 swing RHS line; c1/c3/c4/c5: the other workloads")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -83,6 +85,12 @@ def main():
 
     ND = entry.load()
     dev = torch.device("cuda:0")
+    if args.config != "c2":
+        from importlib import import_module
+        ex = import_module(ND.__name__ + ".experiments")
+        fn = dict(c1=ex.c1_rk4, c3=ex.c3_rhs, c4=ex.c4_ensemble, c5=ex.c5_warmstart)[args.config]
+        print(json.dumps(dict(config=args.config, **fn(dev))))
+        return
     nw, p, u0 = ND.workloads.swing_ba(args.n, 2, seed=1)
     N, E = nw.nv(), nw.ne()
     u = torch.from_numpy(u0).to(dev)

@@ -286,6 +286,8 @@ typedef struct {
     double cg_rtol;        /* relative residual of each linear solve, 0 = 1e-12           */
     int32_t max_newton;    /* 0 = 50                                                       */
     int32_t cg_maxiter;    /* 0 = 20000                                                    */
+    int32_t linearized;    /* 1: solve the DC power flow P + sum K (theta_j - theta_i) = 0 */
+    int32_t pad_;
 } nd_newton_opts;
 typedef struct {
     double residual;       /* final max |F_i/scale_i|                                      */
@@ -299,6 +301,25 @@ int nd_fixpoint_create(int32_t ngrid, const int64_t* nv, const int64_t* ne, cons
 void nd_fixpoint_destroy(nd_fixpoint* f);
 int nd_fixpoint_solve(nd_fixpoint* f, double* theta, const double* P, const double* scale, const double* K,
                       const nd_newton_opts* opts, nd_newton_stat* stats, void* stream);
+
+/* ---- learned warm start: message-passing network (FP32) -------------------------
+ * No counterpart in the reference (its find_fixpoint starts from the given state).
+ * A 4-layer message-passing network, hidden width 64, over the stacked nodes of a
+ * batch of grids: H0 = relu(X W_in + b_in); H_l = relu(A (H_{l-1} W_l) + H_{l-1} U_l + b_l);
+ * out = H_4 w_out + b_out, with A the row-normalised K-weighted adjacency (CSR rowptr
+ * / nbr / a_fwd; a_bwd = weights of its transpose on the same pattern).  Parameter
+ * layout: W_in[nfeat*64], b_in[64], {[W_l|U_l][64*128], b_l[64]} x 4, w_out[64], b_out.
+ * work: nd_mp_workspace_floats(n) floats, shared by a forward and its backward. */
+int64_t nd_mp_param_count(int32_t nfeat);
+int64_t nd_mp_workspace_floats(int64_t nnodes);
+int nd_mp_forward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const float* a_fwd, const float* X,
+                  int32_t nfeat, const float* params, float* work, float* out, void* stream);
+/* gradient of mean((out - target)^2) into grads; loss_part: n/4+1 per-block sums of squared errors */
+int nd_mp_backward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const float* a_bwd, const float* X,
+                   int32_t nfeat, const float* params, const float* target, float* work, float* grads,
+                   float* loss_part, void* stream);
+int nd_mp_adam(int64_t np, float* params, const float* grads, float* m, float* v, float lr, float b1, float b2,
+               float eps, int64_t step, void* stream);
 
 /* ---- synthetic inputs ----------------------------------------------------------
  * Seeded graph generators for the workloads.  No counterpart in the reference's

@@ -33,3 +33,5 @@ def pdim(nw):
 
 
 __all__ = [n for n in dir() if not n.startswith("_")]
+from . import warmstart
+from . import experiments

@@ -92,6 +92,11 @@ PROTOTYPES = {
     "nd_fixpoint_create": (C.c_int, [I32, P, P, P, P, C.POINTER(P)]),
     "nd_fixpoint_destroy": (None, [P]),
     "nd_fixpoint_solve": (C.c_int, [P, P, P, P, P, P, P, P]),
+    "nd_mp_param_count": (I64, [I32]),
+    "nd_mp_workspace_floats": (I64, [I64]),
+    "nd_mp_forward": (C.c_int, [I64, P, P, P, P, I32, P, P, P, P]),
+    "nd_mp_backward": (C.c_int, [I64, P, P, P, P, I32, P, P, P, P, P, P]),
+    "nd_mp_adam": (C.c_int, [I64, P, P, P, P, C.c_float, C.c_float, C.c_float, C.c_float, I64, P]),
     "nd_graph_erdos_renyi": (C.c_int, [I64, I64, C.c_uint64, C.POINTER(P)]),
     "nd_graph_barabasi_albert": (C.c_int, [I64, I64, C.c_uint64, C.POINTER(P)]),
     "nd_graph_random_geometric": (C.c_int, [I64, D, C.c_uint64, P, C.POINTER(P)]),

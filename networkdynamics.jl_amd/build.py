@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libnd_hip.so")
 ARCH = os.environ.get("ND_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["nd_network.cpp", "nd_graphgen.cpp"]
-HIP_SOURCES = ["nd_plan.hip", "nd_ensemble.hip", "nd_newton.hip"]
+HIP_SOURCES = ["nd_plan.hip", "nd_ensemble.hip", "nd_newton.hip", "nd_mpnn.hip"]
 
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wextra"]
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

@@ -1,0 +1,355 @@
+// Message-passing network for fixpoint warm starts: FP32 forward, backward and Adam,
+// all in hand-written kernels.
+//
+// Nodes of a batch of grids are stacked (n rows); the graph is a row-normalised
+// weighted adjacency (a_fwd[s] = K_s / sum of the row's K) in CSR form, plus its
+// transpose weights a_bwd for the backward pass.  Layers (hidden width 64):
+//   H0 = relu(X W_in + b_in)                      X: n x F features
+//   Y  = H_{l-1} [W_l | U_l]                       (n x 128, one GEMM)
+//   H_l = relu(A Y[:, :64] + Y[:, 64:] + b_l)      l = 1..4
+//   out = H_4 w_out + b_out                        (n)
+// Loss: mean squared error against the target.  Weight gradients are reduced in
+// two stages (per-block partials, then a fixed-order sum), so training is
+// deterministic.
+//
+// Parameter vector: W_in[F*64], b_in[64], {W_l|U_l [64*128], b_l[64]} x 4, w_out[64], b_out[1].
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "nd_internal.h"
+
+namespace {
+
+constexpr int kH = 64;          // hidden width
+constexpr int kLayers = 4;
+constexpr int kWave = 64;
+
+// ---- dense: C[M x N] = A[M x K] * op(B) (+ bias, relu) ---------------------------------
+// op(B) = B[K x N] (TRANSB = false) or B[N x K]^T (TRANSB = true).  64x64 tiles, 256
+// threads, 4x4 outputs per thread, K in steps of 16 through LDS.
+template <bool TRANSB>
+__global__ void __launch_bounds__(256) k_sgemm(int64_t M, int N, int K, const float* __restrict__ A, int lda,
+                                               const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
+                                               const float* __restrict__ bias, int relu) {
+    __shared__ float As[16][64 + 1];
+    __shared__ float Bs[16][64 + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int64_t row0 = static_cast<int64_t>(blockIdx.x) * 64;
+    const int col0 = blockIdx.y * 64;
+    float acc[4][4] = {};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        for (int i = threadIdx.x; i < 16 * 64; i += 256) {
+            const int kk = i % 16, rr = i / 16;      // A tile: 64 rows x 16 k
+            const int64_t r = row0 + rr;
+            As[kk][rr] = (r < M && k0 + kk < K) ? A[r * lda + k0 + kk] : 0.0f;
+            const int cc = i % 64, kb = i / 64;      // B tile: 16 k x 64 cols
+            float bv = 0.0f;
+            if (col0 + cc < N && k0 + kb < K)
+                bv = TRANSB ? B[static_cast<int64_t>(col0 + cc) * ldb + k0 + kb] : B[static_cast<int64_t>(k0 + kb) * ldb + col0 + cc];
+            Bs[kb][cc] = bv;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t r = row0 + ty * 4 + i;
+        if (r >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int c = col0 + tx * 4 + j;
+            if (c >= N) continue;
+            float v = acc[i][j] + (bias ? bias[c] : 0.0f);
+            if (relu) v = fmaxf(v, 0.0f);
+            C[r * ldc + c] = v;
+        }
+    }
+}
+
+// ---- dense: partial[b] = A[rows_b]^T * B[rows_b]  (KA x NB, KA*NB <= 8192) ---------------
+constexpr int kRowsPerBlock = 1024;
+__global__ void __launch_bounds__(256) k_sgemm_tn_partial(int64_t M, int KA, int NB, const float* __restrict__ A, int lda,
+                                                          const float* __restrict__ B, int ldb, float* __restrict__ part) {
+    __shared__ float As[16][128];
+    __shared__ float Bs[16][128];
+    const int nout = KA * NB;
+    float acc[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) acc[q] = 0.0f;
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kRowsPerBlock;
+    const int64_t r1 = std::min<int64_t>(r0 + kRowsPerBlock, M);
+    for (int64_t rb = r0; rb < r1; rb += 16) {
+        for (int i = threadIdx.x; i < 16 * 128; i += 256) {
+            const int rr = i / 128, c = i % 128;
+            const int64_t r = rb + rr;
+            As[rr][c] = (r < r1 && c < KA) ? A[r * lda + c] : 0.0f;
+            Bs[rr][c] = (r < r1 && c < NB) ? B[r * ldb + c] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const int o = threadIdx.x + q * 256;
+            if (o < nout) {
+                const int i = o / NB, j = o % NB;
+                float s = 0.0f;
+#pragma unroll
+                for (int rr = 0; rr < 16; ++rr) s += As[rr][i] * Bs[rr][j];
+                acc[q] += s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+        const int o = threadIdx.x + q * 256;
+        if (o < nout) part[static_cast<int64_t>(blockIdx.x) * nout + o] = acc[q];
+    }
+}
+
+// out[i] (+)= sum_b part[b][i], in block order
+__global__ void k_reduce_partials(int nb, int nout, const float* __restrict__ part, float* __restrict__ out, int add) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nout) return;
+    float s = 0.0f;
+    for (int b = 0; b < nb; ++b) s += part[static_cast<int64_t>(b) * nout + i];
+    out[i] = add ? out[i] + s : s;
+}
+
+// column sums of G[M x 64] into per-block partials (bias gradients)
+__global__ void __launch_bounds__(256) k_colsum_partial(int64_t M, const float* __restrict__ G, int ldg, float* __restrict__ part) {
+    __shared__ float sh[4][kH];
+    const int c = threadIdx.x % kH, w = threadIdx.x / kH;
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kRowsPerBlock;
+    const int64_t r1 = std::min<int64_t>(r0 + kRowsPerBlock, M);
+    float s = 0.0f;
+    for (int64_t r = r0 + w; r < r1; r += 4) s += G[r * ldg + c];
+    sh[w][c] = s;
+    __syncthreads();
+    if (w == 0) part[static_cast<int64_t>(blockIdx.x) * kH + c] = sh[0][c] + sh[1][c] + sh[2][c] + sh[3][c];
+}
+
+// per-block sums of a vector (fixed tree inside the block)
+__global__ void __launch_bounds__(256) k_vsum_partial(int64_t n, const float* __restrict__ x, float* __restrict__ part) {
+    __shared__ float sh[256];
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kRowsPerBlock;
+    const int64_t r1 = std::min<int64_t>(r0 + kRowsPerBlock, n);
+    float s = 0.0f;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) s += x[r];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// ---- sparse: out[i, :] = sum_s a[s] * Y[nbr[s], 0:64]  (+ Y[i, 64:128] + bias, relu) -------
+// one 64-lane wave per node, one lane per feature
+__global__ void __launch_bounds__(256) k_spmm(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ nbr,
+                                              const float* __restrict__ a, const float* __restrict__ Y, int ldy,
+                                              float* __restrict__ out, int ldo, const float* __restrict__ bias, int fused) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 4 + threadIdx.x / kWave;
+    const int c = threadIdx.x % kWave;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s += a[k] * Y[static_cast<int64_t>(nbr[k]) * ldy + c];
+    if (fused) {
+        s += Y[i * ldy + kH + c] + bias[c];
+        s = fmaxf(s, 0.0f);
+    }
+    out[i * ldo + c] = s;
+}
+
+// G[i, c] *= (H[i, c] > 0)
+__global__ void k_relu_mask(int64_t n, float* __restrict__ G, int ldg, const float* __restrict__ H, int ldh) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n * kH) return;
+    const int64_t i = t / kH;
+    const int c = static_cast<int>(t % kH);
+    if (!(H[i * ldh + c] > 0.0f)) G[i * ldg + c] = 0.0f;
+}
+
+// out[i] = H[i, :] . w + b ; optional MSE gradient against target and loss partials
+__global__ void __launch_bounds__(256) k_head(int64_t n, const float* __restrict__ H, const float* __restrict__ w,
+                                              const float* __restrict__ b, float* __restrict__ out,
+                                              const float* __restrict__ target, float* __restrict__ dout,
+                                              float* __restrict__ dH, float* __restrict__ loss_part) {
+    __shared__ float sl[4];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 4 + threadIdx.x / kWave;
+    const int c = threadIdx.x % kWave;
+    float v = 0.0f, l = 0.0f;
+    if (i < n) {
+        v = H[i * kH + c] * w[c];
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+        v += b[0];
+        if (c == 0 && out) out[i] = v;
+        if (target) {
+            const float e = v - target[i];
+            const float g = 2.0f * e / static_cast<float>(n);
+            if (c == 0) { dout[i] = g; l = e * e; }
+            dH[i * kH + c] = g * w[c];
+        }
+    }
+    if (loss_part) {
+        if (c == 0) sl[threadIdx.x / kWave] = l;
+        __syncthreads();
+        if (threadIdx.x == 0) loss_part[blockIdx.x] = sl[0] + sl[1] + sl[2] + sl[3];
+    }
+}
+
+__global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+unsigned nblk(int64_t n, int b) { return static_cast<unsigned>((n + b - 1) / b); }
+
+struct Offsets {
+    int64_t win, bin, wl[kLayers], bl[kLayers], wout, bout, total;
+};
+Offsets offsets(int F) {
+    Offsets o{};
+    int64_t k = 0;
+    o.win = k; k += static_cast<int64_t>(F) * kH;
+    o.bin = k; k += kH;
+    for (int l = 0; l < kLayers; ++l) { o.wl[l] = k; k += kH * 2 * kH; o.bl[l] = k; k += kH; }
+    o.wout = k; k += kH;
+    o.bout = k; k += 1;
+    o.total = k;
+    return o;
+}
+
+int herr(hipError_t e, const char* w) {
+    return e == hipSuccess ? ND_OK : nd::set_error(ND_ERR_DEVICE, "%s: %s", w, hipGetErrorString(e));
+}
+
+void gemm(hipStream_t s, bool transb, int64_t M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C,
+          int ldc, const float* bias, int relu) {
+    dim3 grid(nblk(M, 64), nblk(N, 64));
+    if (transb) hipLaunchKernelGGL(k_sgemm<true>, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias, relu);
+    else hipLaunchKernelGGL(k_sgemm<false>, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias, relu);
+}
+
+void gemm_tn(hipStream_t s, int64_t M, int KA, int NB, const float* A, int lda, const float* B, int ldb, float* part,
+             float* out, int add) {
+    const unsigned nb = std::max(1u, nblk(M, kRowsPerBlock));
+    hipLaunchKernelGGL(k_sgemm_tn_partial, dim3(nb), dim3(256), 0, s, M, KA, NB, A, lda, B, ldb, part);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nblk(KA * NB, 256)), dim3(256), 0, s, static_cast<int>(nb), KA * NB, part, out, add);
+}
+
+void colsum(hipStream_t s, int64_t M, const float* G, int ldg, float* part, float* out) {
+    const unsigned nb = std::max(1u, nblk(M, kRowsPerBlock));
+    hipLaunchKernelGGL(k_colsum_partial, dim3(nb), dim3(256), 0, s, M, G, ldg, part);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kH), 0, s, static_cast<int>(nb), kH, part, out, 0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t nd_mp_param_count(int32_t nfeat) { return offsets(nfeat).total; }
+
+int64_t nd_mp_workspace_floats(int64_t nnodes) {
+    // activations H0..H4 (5*64) + Y (128) per node, backward: g (64), [dYa|g] (128), dH (64),
+    // head gradient (1) + partial buffers
+    return nnodes * (5 * kH + 2 * kH + kH + 2 * kH + kH + 1) + (nnodes / kRowsPerBlock + 2) * (kH * 2 * kH) + nnodes / 4 + 8;
+}
+
+int nd_mp_forward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const float* a_fwd, const float* X, int32_t nfeat,
+                  const float* params, float* work, float* out, void* stream) {
+    if (n < 1 || !rowptr || !X || !params || !work || !out || nfeat < 1 || nfeat > 128)
+        return nd::set_error(ND_ERR_ARGUMENT, "bad arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Offsets o = offsets(nfeat);
+    float* H = work;                        // H_l at H + l*n*64
+    float* Y = work + 5 * n * kH;           // n x 128
+    gemm(s, false, n, kH, nfeat, X, nfeat, params + o.win, kH, H, kH, params + o.bin, 1);
+    for (int l = 0; l < kLayers; ++l) {
+        gemm(s, false, n, 2 * kH, kH, H + l * n * kH, kH, params + o.wl[l], 2 * kH, Y, 2 * kH, nullptr, 0);
+        hipLaunchKernelGGL(k_spmm, dim3(nblk(n, 4)), dim3(256), 0, s, n, rowptr, nbr, a_fwd, Y, 2 * kH,
+                           H + (l + 1) * n * kH, kH, params + o.bl[l], 1);
+    }
+    hipLaunchKernelGGL(k_head, dim3(nblk(n, 4)), dim3(256), 0, s, n, H + 4 * n * kH, params + o.wout, params + o.bout,
+                       out, nullptr, nullptr, nullptr, nullptr);
+    return herr(hipGetLastError(), "nd_mp_forward");
+}
+
+// Gradient of the mean squared error; the forward pass must have filled `work`
+// with the same inputs.  loss_part: n/4 + 1 floats of per-block loss sums.
+int nd_mp_backward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const float* a_bwd, const float* X,
+                   int32_t nfeat, const float* params, const float* target, float* work, float* grads,
+                   float* loss_part, void* stream) {
+    if (n < 1 || !rowptr || !X || !params || !work || !grads || !target || !loss_part || nfeat < 1 || nfeat > 128)
+        return nd::set_error(ND_ERR_ARGUMENT, "bad arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Offsets o = offsets(nfeat);
+    float* H = work;
+    float* G = work + 5 * n * kH + n * 2 * kH;   // g: n x 64
+    float* DY = G + n * kH;                       // [dYa | g]: n x 128
+    float* DH = DY + n * 2 * kH;                  // dH: n x 64
+    float* dout = DH + n * kH;                    // n
+    float* part = dout + n;                       // partials
+    hipLaunchKernelGGL(k_head, dim3(nblk(n, 4)), dim3(256), 0, s, n, H + 4 * n * kH, params + o.wout, params + o.bout,
+                       nullptr, target, dout, DH, loss_part);
+    gemm_tn(s, n, kH, 1, H + 4 * n * kH, kH, dout, 1, part, grads + o.wout, 0);
+    {
+        const unsigned nb = std::max(1u, nblk(n, kRowsPerBlock));
+        hipLaunchKernelGGL(k_vsum_partial, dim3(nb), dim3(256), 0, s, n, dout, part);
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, static_cast<int>(nb), 1, part, grads + o.bout, 0);
+    }
+    for (int l = kLayers - 1; l >= 0; --l) {
+        // g = dH_{l+1} * relu'(H_{l+1})
+        (void)hipMemcpyAsync(G, DH, sizeof(float) * n * kH, hipMemcpyDeviceToDevice, s);
+        hipLaunchKernelGGL(k_relu_mask, dim3(nblk(n * kH, 256)), dim3(256), 0, s, n, G, kH, H + (l + 1) * n * kH, kH);
+        colsum(s, n, G, kH, part, grads + o.bl[l]);
+        // [dYa | g]: dYa = A^T g
+        hipLaunchKernelGGL(k_spmm, dim3(nblk(n, 4)), dim3(256), 0, s, n, rowptr, nbr, a_bwd, G, kH, DY, 2 * kH,
+                           nullptr, 0);
+        (void)hipMemcpy2DAsync(DY + kH, 2 * kH * sizeof(float), G, kH * sizeof(float), kH * sizeof(float), n,
+                         hipMemcpyDeviceToDevice, s);
+        // d[W|U] = H_l^T [dYa | g];  dH_l = [dYa | g] [W|U]^T
+        gemm_tn(s, n, kH, 2 * kH, H + l * n * kH, kH, DY, 2 * kH, part, grads + o.wl[l], 0);
+        gemm(s, true, n, kH, 2 * kH, DY, 2 * kH, params + o.wl[l], 2 * kH, DH, kH, nullptr, 0);
+    }
+    (void)hipMemcpyAsync(G, DH, sizeof(float) * n * kH, hipMemcpyDeviceToDevice, s);
+    hipLaunchKernelGGL(k_relu_mask, dim3(nblk(n * kH, 256)), dim3(256), 0, s, n, G, kH, H, kH);
+    colsum(s, n, G, kH, part, grads + o.bin);
+    gemm_tn(s, n, nfeat, kH, X, nfeat, G, kH, part, grads + o.win, 0);
+    return herr(hipGetLastError(), "nd_mp_backward");
+}
+
+int nd_mp_adam(int64_t np, float* params, const float* grads, float* m, float* v, float lr, float b1, float b2,
+               float eps, int64_t step, void* stream) {
+    if (np < 1 || !params || !grads || !m || !v || step < 1) return nd::set_error(ND_ERR_ARGUMENT, "bad arguments");
+    const float c1 = 1.0f - std::pow(b1, static_cast<float>(step));
+    const float c2 = 1.0f - std::pow(b2, static_cast<float>(step));
+    hipLaunchKernelGGL(k_adam, dim3(nblk(np, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), np, params, grads,
+                       m, v, lr, b1, b2, eps, c1, c2);
+    return herr(hipGetLastError(), "nd_mp_adam");
+}
+
+}  // extern "C"

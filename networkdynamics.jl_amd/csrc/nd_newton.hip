@@ -49,6 +49,7 @@ struct NewtonArgs {
     const int32_t* order;    // launch position -> grid index
     double tol, cg_rtol;
     int32_t max_newton, cg_maxiter;
+    int32_t linearized;
     int32_t first;
 };
 
@@ -108,7 +109,10 @@ __global__ void __launch_bounds__(kThreads) k_newton(NewtonArgs a) {
             if (v < G.n) {
                 const double own = th[v];
                 double acc = 0.0;
-                for (int s = rp[v]; s < rp[v + 1]; ++s) acc += a.K[se[s]] * sin(th[nb[s]] - own);
+                if (a.linearized)
+                    for (int s = rp[v]; s < rp[v + 1]; ++s) acc += a.K[se[s]] * (th[nb[s]] - own);
+                else
+                    for (int s = rp[v]; s < rp[v + 1]; ++s) acc += a.K[se[s]] * sin(th[nb[s]] - own);
                 out[k] = Pg[v] + acc;
                 m = fmax(m, fabs(out[k] / scg[v]));
             }
@@ -130,7 +134,7 @@ __global__ void __launch_bounds__(kThreads) k_newton(NewtonArgs a) {
                 const double own = th[v];
                 double dsum = 0.0;
                 for (int s = rp[v]; s < rp[v + 1]; ++s) {
-                    const double ws = a.K[se[s]] * cos(th[nb[s]] - own);
+                    const double ws = a.linearized ? a.K[se[s]] : a.K[se[s]] * cos(th[nb[s]] - own);
                     w[s] = ws;
                     dsum += ws;
                 }
@@ -377,6 +381,7 @@ int nd_fixpoint_solve(nd_fixpoint* f, double* theta, const double* P, const doub
     a.cg_rtol = o->cg_rtol > 0 ? o->cg_rtol : 1e-12;
     a.max_newton = o->max_newton > 0 ? o->max_newton : 50;
     a.cg_maxiter = o->cg_maxiter > 0 ? o->cg_maxiter : 20000;
+    a.linearized = o->linearized ? 1 : 0;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const size_t ns = f->small.size(), nl = f->large.size();
     if (ns) {

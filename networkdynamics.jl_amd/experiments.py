@@ -1,0 +1,216 @@
+"""Workload runs beyond the single-network RHS: each returns a dict of measurements.
+
+c1_rk4        first-order Kuramoto on G(1000, 4000): RK4 dt = 0.01 to t = 10 through the
+              fused per-stage RHS launches and through the one-launch ensemble kernel
+c3_rhs        heterogeneous swing/load network on a random geometric graph, N = 2e5
+c4_ensemble   4096 uneven Barabasi-Albert swing networks (N ~ U{100..2000}) from their
+              fixpoints plus a one-node perturbation, Tsit5 to t = 50 (reltol 1e-8):
+              the fused ensemble kernel against the per-instance GPU path
+c5_warmstart  swing/load grids: Newton from flat starts vs from the learned warm start
+"""
+import time
+
+import numpy as np
+
+from . import graphs as G
+from . import library as Lib
+from . import workloads
+from .ensemble import RK4, TSIT5, Ensemble, tsit5_per_instance
+from .fixpoint import FixpointSolver, isfixpoint_residual, swing_load_grid
+from .network import Network
+
+
+def _timeit(fn, reps, warm=2):
+    import torch
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps * 1e-3
+
+
+def c1_rk4(dev):
+    import torch
+    nw, p, u0 = workloads.kuramoto_er(1000, 4000, 3.0, seed=1)
+    pd = torch.from_numpy(p).to(dev)
+    u = torch.from_numpy(u0).to(dev)
+    du = torch.empty_like(u)
+    t_rhs = _timeit(lambda: nw(du, u, pd, 0.0), 200)
+    uu = u.clone()
+    t_plan = _timeit(lambda: (uu.copy_(u), nw.rk4(uu, pd, 0.0, 0.01, 1000)), 3, warm=1)
+    ens = Ensemble([nw.g], Lib.kuramoto_first(), Lib.kuramoto_edge())
+    ue = u.clone()
+    t_ens = _timeit(lambda: (ue.copy_(u), ens.solve(ue, pd, (0.0, 10.0), method=RK4, dt=0.01)), 3, warm=1)
+    return dict(workload="Kuramoto ER N=1000 E=4000, RK4 dt=0.01 to t=10", rhs_us=t_rhs * 1e6,
+                rk4_fused_stage_launches_ms=t_plan * 1e3, rk4_one_launch_ms=t_ens * 1e3,
+                max_state_difference=float((uu - ue).abs().max().item()))
+
+
+def c3_rhs(dev):
+    import torch
+    nw, p, u0 = workloads.heterogeneous_rgg(200_000, seed=1)
+    u = torch.from_numpy(u0).to(dev)
+    pd = torch.from_numpy(p).to(dev)
+    du = torch.empty_like(u)
+    t = _timeit(lambda: nw(du, u, pd, 0.0), 200)
+    algo = 8 * nw.dim * 2 + 8 * nw.pdim + 8 * nw.ne()
+    return dict(workload=f"heterogeneous RGG N={nw.nv()} E={nw.ne()} (2 vertex batches)", rhs_us=t * 1e6,
+                rhs_per_s=1.0 / t, algorithmic_bytes=algo, achieved_GBs=algo / t / 1e9)
+
+
+def c4_instances(ninst, seed=1, nmin=100, nmax=2000):
+    """Uneven BA swing networks with C2 parameters whose synchronous fixpoint exists."""
+    rng = np.random.default_rng(seed)
+    out = []
+    k = 0
+    while len(out) < ninst:
+        cand = []
+        for _ in range(ninst - len(out)):
+            n = int(rng.integers(nmin, nmax + 1))
+            g = G.barabasi_albert(n, 2, seed=seed * 100000 + k)
+            k += 1
+            M, D, P, K = workloads.swing_params(n, g.ne(), rng)
+            cand.append((g, M, D, P, K))
+        out += cand
+    return out, rng
+
+
+def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_seconds=30.0, seed=1):
+    import torch
+    t0 = time.perf_counter()
+    inst, rng = c4_instances(ninst, seed)
+    # synchronous fixpoints (batched Newton); drop instances without one
+    fs = FixpointSolver([x[0] for x in inst])
+    th = torch.zeros(int(fs.voff[-1]), dtype=torch.float64, device=dev)
+    P = torch.from_numpy(np.concatenate([x[3] for x in inst])).to(dev)
+    S = torch.from_numpy(np.concatenate([x[1] for x in inst])).to(dev)
+    K = torch.from_numpy(np.concatenate([x[4] for x in inst])).to(dev)
+    torch.cuda.synchronize()
+    tn = time.perf_counter()
+    st = FixpointSolver.stats_to_numpy(fs.solve(th, P, S, K))
+    torch.cuda.synchronize()
+    t_newton = time.perf_counter() - tn
+    ok = st["retcode"] == 0
+    thh = th.cpu().numpy()
+    keep = [i for i in range(len(inst)) if ok[i]]
+    graphs, us, ps = [], [], []
+    for i in keep:
+        g, M, D, Pi, Ki = inst[i]
+        n = g.nv()
+        u = np.zeros(2 * n)
+        u[0::2] = thh[fs.voff[i]:fs.voff[i + 1]]
+        j = int(rng.integers(n))
+        u[2 * j] += rng.uniform(-np.pi, np.pi)
+        u[2 * j + 1] += rng.uniform(-10.0, 10.0)
+        p = np.empty(3 * n + g.ne())
+        p[0:3 * n:3], p[1:3 * n:3], p[2:3 * n:3], p[3 * n:] = M, D, Pi, Ki
+        graphs.append(g)
+        us.append(u)
+        ps.append(p)
+    t_setup = time.perf_counter() - t0
+    ens = Ensemble(graphs, Lib.kuramoto_second(), Lib.kuramoto_edge())
+    u_d = torch.from_numpy(np.concatenate(us)).to(dev)
+    p_d = torch.from_numpy(np.concatenate(ps)).to(dev)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    est = Ensemble.stats_to_numpy(ens.solve(u_d, p_d, (0.0, t1), method=TSIT5, reltol=reltol, abstol=abstol))
+    torch.cuda.synchronize()
+    t_ens = time.perf_counter() - ts
+    uf = u_d.cpu().numpy()
+    sync = np.array([np.abs(uf[ens.state_slice(i)][1::2]).max() < 0.5 for i in range(len(graphs))])
+    # plain per-instance GPU path on a time-bounded sample (largest-first order not used: random sample)
+    order = rng.permutation(len(graphs))
+    done, tb0 = 0, time.perf_counter()
+    steps = []
+    for i in order:
+        nw = Network(graphs[i], Lib.kuramoto_second(), Lib.kuramoto_edge())
+        ui = torch.from_numpy(us[i]).to(dev)
+        info = tsit5_per_instance(nw, ui, torch.from_numpy(ps[i]).to(dev), (0.0, t1), reltol=reltol, abstol=abstol)
+        steps.append(info["naccept"])
+        done += 1
+        if time.perf_counter() - tb0 > baseline_seconds:
+            break
+    t_base = time.perf_counter() - tb0
+    traj_ens = len(graphs) / t_ens
+    traj_base = done / t_base
+    return dict(workload=f"{len(graphs)} BA(m=2) swing networks N~U{{100..2000}}, Tsit5 t=0..{t1}, "
+                         f"reltol={reltol}, abstol={abstol}",
+                instances=len(graphs), dropped_without_fixpoint=int((~ok).sum()), newton_s=t_newton,
+                setup_s=t_setup, ensemble_s=t_ens, ensemble_traj_per_s=traj_ens,
+                ensemble_steps_mean=float(est["naccept"].mean()), ensemble_rejects_mean=float(est["nreject"].mean()),
+                ensemble_retcodes=np.bincount(est["retcode"], minlength=4).tolist(),
+                returned_to_sync=int(sync.sum()), baseline_instances=done, baseline_s=t_base,
+                baseline_traj_per_s=traj_base, baseline_steps_mean=float(np.mean(steps)),
+                speedup=traj_ens / traj_base)
+
+
+def c5_grids(count, seed, nmin=500, nmax=5000):
+    rng = np.random.default_rng(seed)
+    return [swing_load_grid(int(n), seed=seed * 1000003 + i) for i, n in enumerate(rng.integers(nmin, nmax + 1, count))]
+
+
+def c5_warmstart(dev, ntrain=2000, neval=200, epochs=30, batch_grids=100, lr=2e-3, seed=7):
+    import torch
+    from . import warmstart as W
+    out = {}
+    t0 = time.perf_counter()
+    train = c5_grids(ntrain, seed)
+    evalg = c5_grids(neval, seed + 1)
+    out["generate_s"] = time.perf_counter() - t0
+    # training data: fixpoints from flat starts (the solver's own output)
+    th, st, tsol = W.solve_grids(train, dev)
+    ok = st["retcode"] == 0
+    out.update(train_grids=int(ok.sum()), train_dropped=int((~ok).sum()), train_newton_s=tsol)
+    train = [g for g, o in zip(train, ok) if o]
+    th = [W.center(t) for t, o in zip(th, ok) if o]
+    dc, _, _ = W.solve_grids(train, dev, linearized=True)
+    dc = [W.center(t) for t in dc]
+    net = W.WarmStartNet(dev, seed=0)
+    X = np.concatenate([W.features(g, t) for g, t in zip(train, dc)])
+    net.xmean, net.xstd = X.mean(axis=0), X.std(axis=0) + 1e-12
+    tgt = [s - d for s, d in zip(th, dc)]
+    net.tscale = 1.0 / np.concatenate(tgt).std()
+    batches = []
+    for b in range(0, len(train), batch_grids):
+        gs = train[b:b + batch_grids]
+        Xb = np.concatenate([W.features(g, t) for g, t in zip(gs, dc[b:b + batch_grids])])
+        yb = np.concatenate(tgt[b:b + batch_grids]) * net.tscale
+        batches.append((W.GraphBatch(gs, dev), net.normalise(Xb), torch.from_numpy(yb.astype(np.float32)).to(dev)))
+    torch.cuda.synchronize()
+    tt = time.perf_counter()
+    hist = []
+    for ep in range(epochs):
+        tot = 0.0
+        for bt, Xd, yd in batches:
+            tot += net.train_step(bt, Xd, yd, lr=lr).item()
+        hist.append(tot / len(batches))
+    torch.cuda.synchronize()
+    out.update(train_s=time.perf_counter() - tt, loss_first=hist[0], loss_last=hist[-1], epochs=epochs)
+    # held-out evaluation
+    fth, fst, ft = W.solve_grids(evalg, dev)
+    keep = fst["retcode"] == 0
+    evalg = [g for g, o in zip(evalg, keep) if o]
+    torch.cuda.synchronize()
+    tw = time.perf_counter()
+    dce, dst, dct = W.solve_grids(evalg, dev, linearized=True)
+    warm = W.predict(net, evalg, [W.center(t) for t in dce], dev)
+    torch.cuda.synchronize()
+    t_pred = time.perf_counter() - tw
+    wth, wst, wt = W.solve_grids(evalg, dev, theta0=np.concatenate(warm))
+    dth, dst2, dt2 = W.solve_grids(evalg, dev, theta0=np.concatenate([W.center(t) for t in dce]))
+    acc_flat = sum(isfixpoint_residual(g, t, dev) <= 1e-10 for g, t in zip(evalg, [x for x, o in zip(fth, keep) if o]))
+    acc_warm = sum(isfixpoint_residual(g, t, dev) <= 1e-10 for g, t in zip(evalg, wth))
+    fl = fst["newton_iters"][keep]
+    out.update(eval_grids=len(evalg), flat_newton_mean=float(fl.mean()), warm_newton_mean=float(wst["newton_iters"].mean()),
+               dc_only_newton_mean=float(dst2["newton_iters"].mean()),
+               newton_ratio=float(fl.mean() / max(wst["newton_iters"].mean(), 1e-9)),
+               flat_linear_solves_mean=float(fl.mean()), warm_linear_solves_mean=float(1 + wst["newton_iters"].mean()),
+               flat_s=ft, warm_total_s=t_pred + wt, warm_newton_s=wt, warm_dc_and_predict_s=t_pred,
+               accepted_flat=int(acc_flat), accepted_warm=int(acc_warm),
+               warm_retcodes=np.bincount(wst["retcode"], minlength=3).tolist())
+    return out

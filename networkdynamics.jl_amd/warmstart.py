@@ -1,0 +1,187 @@
+"""Learned warm start for fixpoint solves.
+
+Pipeline for a batch of swing/load grids:
+  1. DC power flow theta_dc (one linear solve, `FixpointSolver(linearized=True)`);
+  2. features per vertex from the grid and the DC flows;
+  3. a 4-layer message-passing network (hidden 64, FP32, nd_mp_* kernels)
+     predicts the nonlinear correction theta* - theta_dc;
+  4. Newton (`FixpointSolver`) starts from theta_dc + prediction.
+Training data are fixpoints theta* that the batched Newton solver itself produced
+from flat starts.  Without the network the same Newton starts from theta = 0.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _abi as A
+from .fixpoint import FixpointSolver
+
+NFEAT = 8
+
+
+class GraphBatch:
+    """Stacked vertices of several grids with the row-normalised K-weighted adjacency."""
+
+    def __init__(self, grids, dev):
+        import torch
+        self.grids = list(grids)
+        nv = np.array([gr.g.nv() for gr in self.grids], np.int64)
+        self.voff = np.r_[0, np.cumsum(nv)]
+        n = int(self.voff[-1])
+        rows, cols, w = [], [], []
+        for i, gr in enumerate(self.grids):
+            s, d, o = gr.g.src - 1 + self.voff[i], gr.g.dst - 1 + self.voff[i], self.voff[i]
+            rows += [s, d]
+            cols += [d, s]
+            w += [gr.K, gr.K]
+        rows, cols, w = np.concatenate(rows), np.concatenate(cols), np.concatenate(w)
+        order = np.lexsort((cols, rows))
+        rows, cols, w = rows[order], cols[order], w[order]
+        deg_w = np.bincount(rows, weights=w, minlength=n)
+        rowptr = np.r_[0, np.cumsum(np.bincount(rows, minlength=n))].astype(np.int32)
+        a_fwd = (w / deg_w[rows]).astype(np.float32)
+        a_bwd = (w / deg_w[cols]).astype(np.float32)      # weight of the transposed entry (col -> row)
+        self.n = n
+        self.rowptr = torch.from_numpy(rowptr).to(dev)
+        self.nbr = torch.from_numpy(cols.astype(np.int32)).to(dev)
+        self.a_fwd = torch.from_numpy(a_fwd).to(dev)
+        self.a_bwd = torch.from_numpy(a_bwd).to(dev)
+        self.host = dict(rowptr=rowptr, nbr=cols, a_fwd=a_fwd, a_bwd=a_bwd)
+
+
+def features(grid, theta_dc):
+    """Per-vertex inputs: P, generator flag, log weighted degree, degree, and DC-flow terms."""
+    g = grid.g
+    s, d = g.src - 1, g.dst - 1
+    n = g.nv()
+    x = theta_dc[d] - theta_dc[s]
+    K = grid.K
+    kdeg = np.bincount(s, weights=K, minlength=n) + np.bincount(d, weights=K, minlength=n)
+    deg = np.bincount(s, minlength=n) + np.bincount(d, minlength=n)
+    cub = K * x ** 3 / 6.0               # sin(x) - x ~ -x^3/6: the DC solution's flow error per line
+    src3 = np.bincount(s, weights=cub, minlength=n) - np.bincount(d, weights=cub, minlength=n)
+    sq = np.bincount(s, weights=K * x * x, minlength=n) + np.bincount(d, weights=K * x * x, minlength=n)
+    mx = np.zeros(n)
+    np.maximum.at(mx, s, np.abs(x))
+    np.maximum.at(mx, d, np.abs(x))
+    return np.stack([grid.P, grid.is_gen.astype(float), np.log(kdeg), deg / 6.0, src3 * 1e3, sq, mx,
+                     np.ones(n)], axis=1)
+
+
+class WarmStartNet:
+    def __init__(self, dev, seed=0):
+        import torch
+        self.dev = dev
+        npar = A.lib().nd_mp_param_count(NFEAT)
+        rng = np.random.default_rng(seed)
+        p = np.zeros(npar, np.float32)
+        k = 0
+
+        def glorot(fan_in, fan_out):
+            return rng.normal(0, np.sqrt(2.0 / (fan_in + fan_out)), (fan_in, fan_out)).astype(np.float32)
+
+        p[k:k + NFEAT * 64] = glorot(NFEAT, 64).ravel(); k += NFEAT * 64 + 64
+        for _ in range(4):
+            p[k:k + 64 * 128] = np.concatenate([glorot(64, 64), glorot(64, 64)], axis=1).ravel(); k += 64 * 128 + 64
+        p[k:k + 64] = glorot(64, 1).ravel(); k += 65
+        assert k == npar
+        self.params = torch.from_numpy(p).to(dev)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.grads = torch.zeros_like(self.params)
+        self.step = 0
+        self.xmean = np.zeros(NFEAT)
+        self.xstd = np.ones(NFEAT)
+        self.tscale = 1.0
+        self._work = None
+
+    def _workspace(self, n):
+        import torch
+        need = int(A.lib().nd_mp_workspace_floats(n))
+        if self._work is None or self._work.numel() < need:
+            self._work = torch.empty(need, dtype=torch.float32, device=self.dev)
+        return self._work
+
+    def normalise(self, X):
+        import torch
+        Xn = (X - self.xmean) / self.xstd
+        Xn[:, -1] = 1.0
+        return torch.from_numpy(np.ascontiguousarray(Xn, np.float32)).to(self.dev)
+
+    def forward(self, batch, Xd):
+        import torch
+        out = torch.empty(batch.n, dtype=torch.float32, device=self.dev)
+        work = self._workspace(batch.n)
+        stream = torch.cuda.current_stream(self.dev).cuda_stream
+        A.check(A.lib().nd_mp_forward(batch.n, batch.rowptr.data_ptr(), batch.nbr.data_ptr(), batch.a_fwd.data_ptr(),
+                                      Xd.data_ptr(), NFEAT, self.params.data_ptr(), work.data_ptr(), out.data_ptr(),
+                                      stream))
+        return out
+
+    def gradient(self, batch, Xd, target):
+        """Forward + backward; returns the loss (device scalar) and fills self.grads."""
+        import torch
+        self.forward(batch, Xd)
+        lp = torch.zeros(batch.n // 4 + 2, dtype=torch.float32, device=self.dev)
+        stream = torch.cuda.current_stream(self.dev).cuda_stream
+        A.check(A.lib().nd_mp_backward(batch.n, batch.rowptr.data_ptr(), batch.nbr.data_ptr(), batch.a_bwd.data_ptr(),
+                                       Xd.data_ptr(), NFEAT, self.params.data_ptr(), target.data_ptr(),
+                                       self._workspace(batch.n).data_ptr(), self.grads.data_ptr(), lp.data_ptr(),
+                                       stream))
+        return lp.sum() / batch.n
+
+    def train_step(self, batch, Xd, target, lr=1e-3):
+        import torch
+        loss = self.gradient(batch, Xd, target)
+        self.step += 1
+        stream = torch.cuda.current_stream(self.dev).cuda_stream
+        A.check(A.lib().nd_mp_adam(self.params.numel(), self.params.data_ptr(), self.grads.data_ptr(),
+                                   self.m.data_ptr(), self.v.data_ptr(), float(lr), 0.9, 0.999, 1e-8, self.step,
+                                   stream))
+        return loss
+
+    def save(self, path):
+        np.savez(path, params=self.params.cpu().numpy(), xmean=self.xmean, xstd=self.xstd, tscale=self.tscale)
+
+    def load(self, path):
+        import torch
+        z = np.load(path)
+        self.params = torch.from_numpy(z["params"]).to(self.dev)
+        self.xmean, self.xstd, self.tscale = z["xmean"], z["xstd"], float(z["tscale"])
+
+
+def _concat(grids, attr):
+    return np.concatenate([getattr(gr, attr) if not callable(getattr(gr, attr)) else getattr(gr, attr)()
+                           for gr in grids])
+
+
+def solve_grids(grids, dev, theta0=None, linearized=False, **kw):
+    """Batched Newton (or DC) solve of several grids; returns (theta per grid, stats, seconds)."""
+    import torch
+    fs = FixpointSolver([gr.g for gr in grids])
+    n = int(fs.voff[-1])
+    th = torch.zeros(n, dtype=torch.float64, device=dev) if theta0 is None else \
+        torch.from_numpy(np.ascontiguousarray(theta0, np.float64)).to(dev)
+    P = torch.from_numpy(np.concatenate([gr.P for gr in grids])).to(dev)
+    S = torch.from_numpy(np.concatenate([gr.scale for gr in grids])).to(dev)
+    K = torch.from_numpy(np.concatenate([gr.K for gr in grids])).to(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = fs.solve(th, P, S, K, linearized=linearized, **kw)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    t = th.cpu().numpy()
+    return [t[fs.voff[i]:fs.voff[i + 1]] for i in range(len(grids))], FixpointSolver.stats_to_numpy(st), el
+
+
+def center(x):
+    return x - x.mean()
+
+
+def predict(net, grids, theta_dc, dev):
+    """Warm-start states theta_dc + predicted correction, per grid."""
+    batch = GraphBatch(grids, dev)
+    X = np.concatenate([features(gr, td) for gr, td in zip(grids, theta_dc)])
+    out = net.forward(batch, net.normalise(X)).double().cpu().numpy() / net.tscale
+    return [center(td + center(out[batch.voff[i]:batch.voff[i + 1]])) for i, td in enumerate(theta_dc)]
