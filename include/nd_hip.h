@@ -268,6 +268,16 @@ int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t
  * of ninst entries */
 int nd_ensemble_solve(nd_ensemble* en, double* u, const double* p, double t0, double t1,
                       const nd_ensemble_opts* opts, nd_ensemble_stat* stats, void* stream);
+/* The same solve captured once into a hipGraph (class launches on parallel branches)
+ * and replayed with nd_ensemble_graph_launch on any stream; u, p, stats, tspan and
+ * opts are fixed at capture.  A graph refuses to launch after nd_ensemble_set_class
+ * changed the classes; destroy graphs before their ensemble.  No reference
+ * counterpart (a solve repeated over many perturbation rounds is its caller). */
+typedef struct nd_ensemble_graph nd_ensemble_graph;
+int nd_ensemble_graph_create(nd_ensemble* en, double* u, const double* p, double t0, double t1,
+                             const nd_ensemble_opts* opts, nd_ensemble_stat* stats, nd_ensemble_graph** out);
+int nd_ensemble_graph_launch(nd_ensemble_graph* graph, void* stream);
+void nd_ensemble_graph_destroy(nd_ensemble_graph* graph);
 
 /* ---- batched fixpoint (Newton) ---------------------------------------------------
  * Synchronous fixpoints of many Kuramoto-type grids at once (one workgroup per grid),

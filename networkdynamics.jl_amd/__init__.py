@@ -35,3 +35,4 @@ def pdim(nw):
 __all__ = [n for n in dir() if not n.startswith("_")]
 from . import warmstart
 from . import experiments
+from . import ensemble_tune

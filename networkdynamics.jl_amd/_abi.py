@@ -89,6 +89,9 @@ PROTOTYPES = {
     "nd_ensemble_sizes": (C.c_int, [P, P, P, P, P]),
     "nd_ensemble_set_class": (C.c_int, [P, I32, I32, I32]),
     "nd_ensemble_solve": (C.c_int, [P, P, P, D, D, P, P, P]),
+    "nd_ensemble_graph_create": (C.c_int, [P, P, P, D, D, P, P, P]),
+    "nd_ensemble_graph_launch": (C.c_int, [P, P]),
+    "nd_ensemble_graph_destroy": (None, [P]),
     "nd_fixpoint_create": (C.c_int, [I32, P, P, P, P, C.POINTER(P)]),
     "nd_fixpoint_destroy": (None, [P]),
     "nd_fixpoint_solve": (C.c_int, [P, P, P, P, P, P, P, P]),

@@ -387,8 +387,30 @@ struct nd_ensemble {
     uint16_t *d_nbr = nullptr, *d_e = nullptr, *d_cs = nullptr, *d_cv = nullptr, *d_cptr = nullptr;
     std::vector<int32_t> order;                     // instances grouped by class, largest first
     int32_t cls_first[ND_ENSEMBLE_CLASSES + 1] = {0, 0, 0, 0, 0};
+    int64_t generation = 0;                         // bumped whenever the class layout changes
+    // one stream per class so the class launches overlap; forked from and joined back
+    // to the caller's stream with events (default priority, non-blocking)
+    hipStream_t cls_stream[ND_ENSEMBLE_CLASSES] = {};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[ND_ENSEMBLE_CLASSES] = {};
+    hipStream_t cap_stream = nullptr;               // origin stream of graph captures
     ~nd_ensemble() {
         for (void* p : allocs) (void)hipFree(p);
+        for (auto& x : cls_stream) if (x) (void)hipStreamDestroy(x);
+        for (auto& x : ev_join) if (x) (void)hipEventDestroy(x);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    }
+};
+
+struct nd_ensemble_graph {
+    nd_ensemble* en = nullptr;
+    int64_t generation = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~nd_ensemble_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
     }
 };
 
@@ -415,6 +437,7 @@ int class_of(const nd_ensemble* en, int32_t n) {
 }
 
 int regroup(nd_ensemble* en) {
+    ++en->generation;
     std::vector<std::vector<std::pair<int32_t, int32_t>>> byc(ND_ENSEMBLE_CLASSES);
     for (int32_t i = 0; i < en->ninst; ++i) {
         const int c = class_of(en, en->inst[i].n);
@@ -441,6 +464,99 @@ int regroup(nd_ensemble* en) {
                        "hipMemcpy");
     }
     return ND_OK;
+}
+
+int make_args(nd_ensemble* en, double* u, const double* p, double t0, double t1, const nd_ensemble_opts* o,
+              nd_ensemble_stat* stats, SolveArgs* a) {
+    if (!en || !u || !p || !o || !stats) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
+    if (!(t1 >= t0)) return nd::set_error(ND_ERR_ARGUMENT, "t1 < t0");
+    if (o->method != ND_METHOD_TSIT5 && o->method != ND_METHOD_RK4) return nd::set_error(ND_ERR_ARGUMENT, "unknown method");
+    if (o->method == ND_METHOD_RK4 && !(o->dt0 > 0.0)) return nd::set_error(ND_ERR_ARGUMENT, "RK4 needs dt0 > 0");
+    int dev = -1;
+    (void)hipGetDevice(&dev);
+    if (dev != en->device) return nd::set_error(ND_ERR_ARGUMENT, "ensemble lives on device %d", en->device);
+    *a = SolveArgs{};
+    a->inst = en->d_inst;
+    a->order = en->d_order;
+    a->slot_nbr = en->d_nbr;
+    a->slot_e = en->d_e;
+    a->cs = en->d_cs;
+    a->cv = en->d_cv;
+    a->cptr = en->d_cptr;
+    a->u = u;
+    a->p = p;
+    a->stats = stats;
+    a->t0 = t0;
+    a->t1 = t1;
+    a->dt0 = o->dt0;
+    a->reltol = o->reltol > 0 ? o->reltol : 1e-3;
+    a->abstol = o->abstol > 0 ? o->abstol : 1e-6;
+    a->dtmax = o->dtmax > 0 ? o->dtmax : (t1 - t0);
+    a->dtmin = o->dtmin > 0 ? o->dtmin : 1e-14 * std::max(1.0, std::fabs(t1));
+    a->maxiters = o->maxiters > 0 ? o->maxiters : 1000000;
+    return ND_OK;
+}
+
+int32_t class_lds(const nd_ensemble* en, int c) {
+    int32_t lds = 0;
+    for (int32_t q = en->cls_first[c]; q < en->cls_first[c + 1]; ++q) lds = std::max(lds, en->inst[en->order[q]].lds_bytes);
+    return lds;
+}
+
+// Streams, events and the dynamic-LDS limit of every class kernel; host-side only,
+// so a capture that follows records nothing but the launches and the event edges.
+int prepare(nd_ensemble* en, int method) {
+    int rc = ND_OK;
+    if (!en->ev_fork) rc = hip_err(hipEventCreateWithFlags(&en->ev_fork, hipEventDisableTiming), "hipEventCreate");
+    for (int c = 0; c < ND_ENSEMBLE_CLASSES && !rc; ++c) {
+        if (!en->cls_stream[c])
+            rc = hip_err(hipStreamCreateWithFlags(&en->cls_stream[c], hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        if (!rc && !en->ev_join[c])
+            rc = hip_err(hipEventCreateWithFlags(&en->ev_join[c], hipEventDisableTiming), "hipEventCreate");
+    }
+    if (rc) return rc;
+    EnsKernel ks[ND_ENSEMBLE_CLASSES] = {};
+    int32_t lds[ND_ENSEMBLE_CLASSES] = {};
+    for (int c = 0; c < ND_ENSEMBLE_CLASSES; ++c) {
+        if (en->cls_first[c + 1] == en->cls_first[c]) continue;
+        ks[c] = pick_ensemble(en->vk.model, en->ek.model, method, en->cls_vper[c]);
+        if (!ks[c]) return nd::set_error(ND_ERR_UNSUPPORTED, "no ensemble kernel for these models");
+        lds[c] = class_lds(en, c);
+    }
+    // one template instance may serve several classes: give it the largest need
+    for (int c = 0; c < ND_ENSEMBLE_CLASSES; ++c) {
+        if (!ks[c]) continue;
+        int32_t need = 0;
+        for (int d = 0; d < ND_ENSEMBLE_CLASSES; ++d)
+            if (ks[d] == ks[c]) need = std::max(need, lds[d]);
+        rc = hip_err(hipFuncSetAttribute(reinterpret_cast<const void*>(ks[c]), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         need),
+                     "hipFuncSetAttribute(LDS)");
+        if (rc) return rc;
+    }
+    return ND_OK;
+}
+
+// Fork: every non-empty class runs on its own stream after the caller's prior work;
+// join: the caller's stream waits for all of them.  Largest class first.
+int launch_classes(nd_ensemble* en, SolveArgs a, int method, hipStream_t s) {
+    int rc = hip_err(hipEventRecord(en->ev_fork, s), "hipEventRecord");
+    for (int c = ND_ENSEMBLE_CLASSES - 1; c >= 0 && !rc; --c) {
+        const int32_t cnt = en->cls_first[c + 1] - en->cls_first[c];
+        if (cnt == 0) continue;
+        EnsKernel k = pick_ensemble(en->vk.model, en->ek.model, method, en->cls_vper[c]);
+        hipStream_t cs = en->cls_stream[c];
+        rc = hip_err(hipStreamWaitEvent(cs, en->ev_fork, 0), "hipStreamWaitEvent");
+        if (rc) break;
+        a.first = en->cls_first[c];
+        a.count = cnt;
+        hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(cnt)), dim3(static_cast<unsigned>(en->cls_threads[c])),
+                           static_cast<unsigned>(class_lds(en, c)), cs, a);
+        rc = hip_err(hipGetLastError(), "ensemble launch");
+        rc = rc ? rc : hip_err(hipEventRecord(en->ev_join[c], cs), "hipEventRecord");
+        rc = rc ? rc : hip_err(hipStreamWaitEvent(s, en->ev_join[c], 0), "hipStreamWaitEvent");
+    }
+    return rc;
 }
 
 }  // namespace
@@ -572,51 +688,51 @@ int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t
 
 int nd_ensemble_solve(nd_ensemble* en, double* u, const double* p, double t0, double t1, const nd_ensemble_opts* o,
                       nd_ensemble_stat* stats, void* stream) {
-    if (!en || !u || !p || !o || !stats) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
-    if (!(t1 >= t0)) return nd::set_error(ND_ERR_ARGUMENT, "t1 < t0");
-    if (o->method != ND_METHOD_TSIT5 && o->method != ND_METHOD_RK4) return nd::set_error(ND_ERR_ARGUMENT, "unknown method");
-    if (o->method == ND_METHOD_RK4 && !(o->dt0 > 0.0)) return nd::set_error(ND_ERR_ARGUMENT, "RK4 needs dt0 > 0");
-    int dev = -1;
-    (void)hipGetDevice(&dev);
-    if (dev != en->device) return nd::set_error(ND_ERR_ARGUMENT, "ensemble lives on device %d", en->device);
     SolveArgs a{};
-    a.inst = en->d_inst;
-    a.order = en->d_order;
-    a.slot_nbr = en->d_nbr;
-    a.slot_e = en->d_e;
-    a.cs = en->d_cs;
-    a.cv = en->d_cv;
-    a.cptr = en->d_cptr;
-    a.u = u;
-    a.p = p;
-    a.stats = stats;
-    a.t0 = t0;
-    a.t1 = t1;
-    a.dt0 = o->dt0;
-    a.reltol = o->reltol > 0 ? o->reltol : 1e-3;
-    a.abstol = o->abstol > 0 ? o->abstol : 1e-6;
-    a.dtmax = o->dtmax > 0 ? o->dtmax : (t1 - t0);
-    a.dtmin = o->dtmin > 0 ? o->dtmin : 1e-14 * std::max(1.0, std::fabs(t1));
-    a.maxiters = o->maxiters > 0 ? o->maxiters : 1000000;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    for (int c = 0; c < ND_ENSEMBLE_CLASSES; ++c) {
-        const int32_t cnt = en->cls_first[c + 1] - en->cls_first[c];
-        if (cnt == 0) continue;
-        EnsKernel k = pick_ensemble(en->vk.model, en->ek.model, o->method, en->cls_vper[c]);
-        if (!k) return nd::set_error(ND_ERR_UNSUPPORTED, "no ensemble kernel for these models");
-        int32_t lds = 0;
-        for (int32_t q = en->cls_first[c]; q < en->cls_first[c + 1]; ++q)
-            lds = std::max(lds, en->inst[en->order[q]].lds_bytes);
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-        if (e != hipSuccess) return hip_err(e, "hipFuncSetAttribute(LDS)");
-        a.first = en->cls_first[c];
-        a.count = cnt;
-        hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(cnt)), dim3(static_cast<unsigned>(en->cls_threads[c])),
-                           static_cast<unsigned>(lds), s, a);
-        e = hipGetLastError();
-        if (e != hipSuccess) return hip_err(e, "ensemble launch");
+    int rc = make_args(en, u, p, t0, t1, o, stats, &a);
+    rc = rc ? rc : prepare(en, o->method);
+    return rc ? rc : launch_classes(en, a, o->method, static_cast<hipStream_t>(stream));
+}
+
+int nd_ensemble_graph_create(nd_ensemble* en, double* u, const double* p, double t0, double t1,
+                             const nd_ensemble_opts* o, nd_ensemble_stat* stats, nd_ensemble_graph** out) {
+    if (!out) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    SolveArgs a{};
+    int rc = make_args(en, u, p, t0, t1, o, stats, &a);
+    rc = rc ? rc : prepare(en, o->method);
+    if (rc) return rc;
+    if (!en->cap_stream) {
+        rc = hip_err(hipStreamCreateWithFlags(&en->cap_stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        if (rc) return rc;
     }
+    nd_ensemble_graph* gr = new (std::nothrow) nd_ensemble_graph;
+    if (!gr) return nd::set_error(ND_ERR_ALLOC, "allocation failed");
+    gr->en = en;
+    gr->generation = en->generation;
+    rc = hip_err(hipStreamBeginCapture(en->cap_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (rc) { delete gr; return rc; }
+    const int lrc = launch_classes(en, a, o->method, en->cap_stream);
+    hipGraph_t g = nullptr;
+    rc = hip_err(hipStreamEndCapture(en->cap_stream, &g), "hipStreamEndCapture");
+    gr->graph = g;
+    if (lrc || rc) { delete gr; return lrc ? lrc : rc; }
+    rc = hip_err(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (rc) { delete gr; return rc; }
+    *out = gr;
     return ND_OK;
 }
+
+int nd_ensemble_graph_launch(nd_ensemble_graph* gr, void* stream) {
+    if (!gr || !gr->exec) return nd::set_error(ND_ERR_ARGUMENT, "NULL graph");
+    if (gr->generation != gr->en->generation)
+        return nd::set_error(ND_ERR_ARGUMENT, "the ensemble's size classes changed after this graph was captured");
+    int dev = -1;
+    (void)hipGetDevice(&dev);
+    if (dev != gr->en->device) return nd::set_error(ND_ERR_ARGUMENT, "graph lives on device %d", gr->en->device);
+    return hip_err(hipGraphLaunch(gr->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+}
+
+void nd_ensemble_graph_destroy(nd_ensemble_graph* gr) { delete gr; }
 
 }  // extern "C"

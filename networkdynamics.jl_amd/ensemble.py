@@ -14,6 +14,8 @@ from . import _abi as A
 from . import components as comp
 
 TSIT5, RK4 = 0, 1
+NCLASS = 4
+DEFAULT_CLASSES = [(128, 2), (256, 2), (512, 2), (512, 4)]     # (threads, vertices per thread)
 
 # Tsitouras (2011) 5(4) pair: stage matrix, solution weights b (= last stage row,
 # first-same-as-last) and error weights btilde = b - bhat.
@@ -77,6 +79,7 @@ class Ensemble:
         A.check(L.nd_ensemble_sizes(h, C.byref(ts), C.byref(tp), self.state_offsets.ctypes.data,
                                     self.param_offsets.ctypes.data))
         self.total_states, self.total_params = ts.value, tp.value
+        self.classes = list(DEFAULT_CLASSES)
 
     def __del__(self):
         L = getattr(A, "_lib", None) if A is not None else None
@@ -86,6 +89,16 @@ class Ensemble:
 
     def set_class(self, cls, threads, vper):
         A.check(A.lib().nd_ensemble_set_class(self._h, int(cls), int(threads), int(vper)))
+        self.classes[int(cls)] = (int(threads), int(vper))
+
+    def set_classes(self, classes):
+        """classes: up to four (threads, vertices per thread) pairs; missing ones repeat the last."""
+        cl = [tuple(map(int, c)) for c in classes]
+        if not 1 <= len(cl) <= NCLASS:
+            raise A.ArgumentError(f"need 1..{NCLASS} classes")
+        cl += [cl[-1]] * (NCLASS - len(cl))
+        for c, (t, v) in enumerate(cl):
+            self.set_class(c, t, v)
 
     def state_slice(self, i):
         return slice(int(self.state_offsets[i]), int(self.state_offsets[i] + self.nv[i] * self.vertexm.dim))
@@ -97,20 +110,57 @@ class Ensemble:
     def solve(self, u, p, tspan, method=TSIT5, reltol=1e-8, abstol=1e-8, dt=0.0, dtmax=0.0, maxiters=0):
         """Integrates all instances in place on the device vector u; returns per-instance stats."""
         import torch
-        if u.dtype != torch.float64 or p.dtype != torch.float64 or u.device.type != "cuda":
-            raise A.ArgumentError("u and p must be float64 device tensors")
-        if u.numel() != self.total_states or p.numel() != self.total_params:
-            raise A.ArgumentError("u / p do not match the ensemble's sizes")
+        self._check(u, p)
         stats = torch.zeros(self.ninst * STAT_DTYPE.itemsize // 8, dtype=torch.float64, device=u.device)
-        o = _Opts(int(method), 0, float(reltol), float(abstol), float(dt), float(dtmax), 0.0, int(maxiters))
+        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters)
         stream = torch.cuda.current_stream(u.device).cuda_stream
         A.check(A.lib().nd_ensemble_solve(self._h, u.data_ptr(), p.data_ptr(), float(tspan[0]), float(tspan[1]),
                                           C.byref(o), stats.data_ptr(), stream))
         return stats
 
+    def _opts(self, method, reltol, abstol, dt, dtmax, maxiters):
+        return _Opts(int(method), 0, float(reltol), float(abstol), float(dt), float(dtmax), 0.0, int(maxiters))
+
+    def _check(self, u, p):
+        import torch
+        if u.dtype != torch.float64 or p.dtype != torch.float64 or u.device.type != "cuda":
+            raise A.ArgumentError("u and p must be float64 device tensors")
+        if u.numel() != self.total_states or p.numel() != self.total_params:
+            raise A.ArgumentError("u / p do not match the ensemble's sizes")
+
+    def capture(self, u, p, tspan, method=TSIT5, reltol=1e-8, abstol=1e-8, dt=0.0, dtmax=0.0, maxiters=0):
+        """Captures the solve of (u, p, tspan) into a hipGraph; `.launch()` replays it in place on u."""
+        import torch
+        self._check(u, p)
+        stats = torch.zeros(self.ninst * STAT_DTYPE.itemsize // 8, dtype=torch.float64, device=u.device)
+        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters)
+        h = C.c_void_p()
+        A.check(A.lib().nd_ensemble_graph_create(self._h, u.data_ptr(), p.data_ptr(), float(tspan[0]),
+                                                 float(tspan[1]), C.byref(o), stats.data_ptr(), C.byref(h)))
+        return EnsembleGraph(self, h, u, p, stats)
+
     @staticmethod
     def stats_to_numpy(stats):
         return stats.cpu().numpy().view(STAT_DTYPE)
+
+
+class EnsembleGraph:
+    """A captured ensemble solve.  Holds its ensemble and buffers so they outlive it."""
+
+    def __init__(self, ens, h, u, p, stats):
+        self.ens, self._h, self.u, self.p, self.stats = ens, h, u, p, stats
+
+    def launch(self):
+        import torch
+        stream = torch.cuda.current_stream(self.u.device).cuda_stream
+        A.check(A.lib().nd_ensemble_graph_launch(self._h, stream))
+        return self.stats
+
+    def __del__(self):
+        L = getattr(A, "_lib", None) if A is not None else None
+        if L is not None and getattr(self, "_h", None):
+            L.nd_ensemble_graph_destroy(self._h)
+            self._h = None
 
 
 def tsit5_per_instance(nw, u, p, tspan, reltol=1e-8, abstol=1e-8, dt0=0.0, maxiters=10 ** 7):

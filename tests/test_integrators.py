@@ -107,3 +107,74 @@ def test_ensemble_tsit5_matches_per_instance(ND, dev):
         # same method and controller: step counts agree closely, solutions to tolerance level
         assert abs(info["naccept"] - st["naccept"][i]) <= 2 + 0.02 * info["naccept"]
         np.testing.assert_allclose(got[ens.state_slice(i)], ui.cpu().numpy(), rtol=0, atol=1e-6)
+
+
+def test_autotune_choice_logic(ND):
+    from networkdynamics_jl_amd.ensemble_tune import choose
+    # three bins; the small bins are cheapest on narrow shapes, the big bin fits only 512x4
+    bin_max = [200, 900, 2000]
+    weight = [10.0, 5.0, 1.0]
+    cost = [{(128, 2): 1.0, (256, 1): 2.0, (256, 4): 3.0, (512, 4): 9.0},
+            {(256, 4): 2.0, (512, 2): 1.5, (512, 4): 4.0},
+            {(512, 4): 1.0}]
+    shapes, pred = choose(bin_max, weight, cost)
+    assert shapes == [(128, 2), (512, 2), (512, 4)]
+    assert pred == 10.0 * 1.0 + 5.0 * 1.5 + 1.0
+    # a single class must hold the largest bin
+    shapes, _ = choose([2000], [1.0], [{(512, 4): 1.0}])
+    assert shapes == [(512, 4)]
+
+
+@pytest.mark.gpu
+def test_ensemble_graph_replays_match_eager(ND, dev):
+    import torch
+    gs, rng = _instances(ND, 40, 30, 1500, seed=5)
+    ps, us = _swing_params(ND, gs, rng)
+    ens = ND.Ensemble(gs, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    u0 = torch.from_numpy(np.concatenate(us)).to(dev)
+    p = torch.from_numpy(np.concatenate(ps)).to(dev)
+    ue = u0.clone()
+    ste = ND.Ensemble.stats_to_numpy(ens.solve(ue, p, (0.0, 1.0), reltol=1e-8, abstol=1e-8))
+    torch.cuda.synchronize()
+    ug = u0.clone()
+    graph = ens.capture(ug, p, (0.0, 1.0), reltol=1e-8, abstol=1e-8)
+    for rep in range(3):
+        ug.copy_(u0)
+        junk = torch.randn(1000, device=dev).sum()          # eager work between replays
+        stg = ND.Ensemble.stats_to_numpy(graph.launch())
+        torch.cuda.synchronize()
+        assert float(junk) == float(junk)
+        assert torch.equal(ug, ue), f"replay {rep}"
+        assert (stg == ste).all(), f"replay {rep}"
+    ens.set_class(0, 64, 2)
+    with pytest.raises(ND.ArgumentError):
+        graph.launch()
+
+
+@pytest.mark.gpu
+def test_ensemble_autotune_and_cache(ND, dev, tmp_path):
+    import torch
+    from networkdynamics_jl_amd.ensemble_tune import autotune
+    gs, rng = _instances(ND, 48, 40, 1200, seed=6)
+    ps, us = _swing_params(ND, gs, rng)
+    ens = ND.Ensemble(gs, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    u0 = torch.from_numpy(np.concatenate(us)).to(dev)
+    p = torch.from_numpy(np.concatenate(ps)).to(dev)
+    cache = str(tmp_path / "tune.json")
+    res = autotune(ens, u0, p, probe_t=0.2, nbins=3, fill=256, cache=cache)
+    assert not res["cached"]
+    caps = [t * v for t, v in res["classes"]]
+    assert caps == sorted(caps) and caps[-1] >= max(g.nv() for g in gs)
+    assert [tuple(c) for c in ens.classes[:len(res["classes"])]] == [tuple(c) for c in res["classes"]]
+    again = autotune(ens, u0, p, probe_t=0.2, nbins=3, fill=256, cache=cache)
+    assert again["cached"] and again["classes"] == res["classes"]
+    # tuned launch shapes integrate to the same answer as the per-instance path
+    u = u0.clone()
+    st = ND.Ensemble.stats_to_numpy(ens.solve(u, p, (0.0, 1.0), reltol=1e-8, abstol=1e-8))
+    torch.cuda.synchronize()
+    assert (st["retcode"] == 0).all()
+    for i in (0, 17, 47):
+        nw = ND.Network(gs[i], ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+        ui = torch.from_numpy(us[i]).to(dev)
+        ND.tsit5_per_instance(nw, ui, torch.from_numpy(ps[i]).to(dev), (0.0, 1.0), reltol=1e-8, abstol=1e-8)
+        np.testing.assert_allclose(u[ens.state_slice(i)].cpu().numpy(), ui.cpu().numpy(), rtol=0, atol=1e-6)
