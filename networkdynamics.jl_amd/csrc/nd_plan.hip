@@ -50,6 +50,23 @@ struct VDesc {            // one vertex batch as the kernels see it (0-based off
     const int32_t* lrp;   // hub-free destination lists: count+1 offsets into lslots
     const int2* lslots;
     const uint8_t* lseb;  // edge batch of each lslots entry (several edge batches only)
+    const double* kfull;  // edge parameters per destination-list entry (rowptr order), or null
+    const double* klow;   // the same for the hub-free lists (lrp order), or null
+};
+
+// Slot-ordered edge parameters: a copy of the edge batch's parameters laid out like the
+// destination lists, so the vertex kernels stream them instead of fetching each entry's
+// parameters from p at a random position.  Each call compares p's parameters with the
+// cached copy kc bit for bit (streamed) and rewrites the slots of the edges that differ,
+// so the copy always equals p when the vertex kernels run.
+struct KSync {
+    const double* pe;     // p + first parameter of the edge batch (count * np, contiguous)
+    double* kc;           // cached copy, same layout
+    double* kfull;
+    double* klow;
+    const int4* kpos;     // per edge: full-list slots of its src / dst entries, then hub-free-list
+                          // slots; -1 where the edge has no such entry
+    int32_t count, np, vblocks;
 };
 
 struct EDesc {            // one edge batch
@@ -168,11 +185,12 @@ constexpr int kUnroll = 4;   // entries whose loads are issued together per thre
 // Partial sum of the entries beg+t, beg+t+G, beg+t+2G, ... of one segment, in that
 // order.  Loads of kUnroll entries are issued before any of them is used, so a
 // thread keeps several gathers in flight instead of one dependent chain.
-template <int EM, int WRAP, bool MULTI>
+template <int EM, int WRAP, bool MULTI, bool KC>
 __device__ __forceinline__ double segment_sum(int32_t beg, int32_t end, int32_t t, int32_t G, double own,
                                               const int2* __restrict__ slots, const uint8_t* __restrict__ slot_eb,
                                               const EDesc* __restrict__ ed, const double* __restrict__ u,
-                                              const double* __restrict__ vo, const double* __restrict__ p) {
+                                              const double* __restrict__ vo, const double* __restrict__ p,
+                                              const double* __restrict__ kslot) {
     double acc = 0.0;
     if constexpr (MULTI) {
         for (int32_t s = beg + t; s < end; s += G) acc += slot_value_rt(ed[slot_eb[s]], slots[s], own, u, vo, p);
@@ -191,7 +209,8 @@ __device__ __forceinline__ double segment_sum(int32_t beg, int32_t end, int32_t 
                     if constexpr (is_ode_model(EM)) nb[j] = u[sl[j].x];
                         else if constexpr (EM != ND_EM_NOOP) nb[j] = vo[sl[j].x];
 #pragma unroll
-                    for (int c = 0; c < NP; ++c) kk[j][c] = p[(sl[j].y >> 1) + c];
+                    for (int c = 0; c < NP; ++c)
+                        kk[j][c] = KC ? kslot[static_cast<int64_t>(s0 + j * G) * NP + c] : p[(sl[j].y >> 1) + c];
                 }
 #pragma unroll
             for (int j = 0; j < kUnroll; ++j)
@@ -208,7 +227,7 @@ __device__ __forceinline__ double segment_sum(int32_t beg, int32_t end, int32_t 
 //   [nbig + nmidb, ...)       one vertex per thread, segment summed in reference order.
 // Hub segments: per-thread partial sums in a fixed order, xor-butterfly per wave,
 // wave partials combined in wave order -- deterministic run to run.
-template <int VM, int EM, int WRAP, bool MULTI, bool STAGE>
+template <int VM, int EM, int WRAP, bool MULTI, bool STAGE, bool KC>
 __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __restrict__ rowptr,
                                                     const int2* __restrict__ slots,
                                                     const uint8_t* __restrict__ slot_eb,
@@ -229,8 +248,8 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
             const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
             const double own = u[x0 + vd.mask[0]];
             const int32_t t = static_cast<int32_t>(threadIdx.x);
-            const double part = wave_sum(segment_sum<EM, WRAP, MULTI>(beg, end, t, static_cast<int32_t>(blockDim.x),
-                                                                      own, slots, slot_eb, ed, u, vo, p));
+            const double part = wave_sum(segment_sum<EM, WRAP, MULTI, KC>(beg, end, t, static_cast<int32_t>(blockDim.x),
+                                                                          own, slots, slot_eb, ed, u, vo, p, vd.kfull));
             if ((t & (kWave - 1)) == 0) wpart[t / kWave] = part;
             __syncthreads();
             if (t == 0) {
@@ -249,8 +268,8 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
             const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * dim;
             const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
             const double own = u[x0 + vd.mask[0]];
-            const double acc = wave_sum(segment_sum<EM, WRAP, MULTI>(beg, end, lane, kWave, own, slots, slot_eb,
-                                                                     ed, u, vo, p));
+            const double acc = wave_sum(segment_sum<EM, WRAP, MULTI, KC>(beg, end, lane, kWave, own, slots, slot_eb,
+                                                                         ed, u, vo, p, vd.kfull));
             if (lane == 0) vertex_emit<VM, STAGE>(du, sa, x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim);
             return;
         }
@@ -319,7 +338,8 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
                         if constexpr (is_ode_model(EM)) nb[j] = u[sl[j].x];
                         else if constexpr (EM != ND_EM_NOOP) nb[j] = vo[sl[j].x];
 #pragma unroll
-                        for (int c = 0; c < NP; ++c) kk[j][c] = p[(sl[j].y >> 1) + c];
+                        for (int c = 0; c < NP; ++c)
+                            kk[j][c] = KC ? vd.klow[static_cast<int64_t>(s2) * NP + c] : p[(sl[j].y >> 1) + c];
                     }
                 }
             }
@@ -375,8 +395,36 @@ __global__ void k_fill(double* __restrict__ x, int64_t n, double v) {
         x[i] = v;
 }
 
-// o[v_out] = u[StateMask] for one vertex batch (coreloop.jl:39)
-__global__ void k_vertex_out(VDesc vd, const double* __restrict__ u, double* __restrict__ o) {
+__device__ __forceinline__ bool same_bits(double a, double b) {
+    return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
+// Blocks [0, vblocks): o[v_out] = u[StateMask] for one vertex batch (coreloop.jl:39).
+// Blocks [vblocks, ...): one thread per edge of the parameter-cached edge batch; if any
+// of its parameters differs (bitwise) from the cached copy, the copy and the edge's
+// slot entries are rewritten.
+__global__ void k_vertex_out(VDesc vd, const double* __restrict__ u, double* __restrict__ o, KSync ks) {
+    if (static_cast<int32_t>(blockIdx.x) >= ks.vblocks) {
+        const int32_t e = static_cast<int32_t>((blockIdx.x - ks.vblocks) * blockDim.x + threadIdx.x);
+        if (e >= ks.count) return;
+        double nv[2];
+        bool changed = false;
+        for (int c = 0; c < ks.np; ++c) {
+            nv[c] = ks.pe[static_cast<int64_t>(e) * ks.np + c];
+            changed |= !same_bits(nv[c], ks.kc[static_cast<int64_t>(e) * ks.np + c]);
+        }
+        if (changed) {
+            const int4 ps = ks.kpos[e];
+            for (int c = 0; c < ks.np; ++c) {
+                ks.kc[static_cast<int64_t>(e) * ks.np + c] = nv[c];
+                if (ps.x >= 0) ks.kfull[static_cast<int64_t>(ps.x) * ks.np + c] = nv[c];
+                if (ps.y >= 0) ks.kfull[static_cast<int64_t>(ps.y) * ks.np + c] = nv[c];
+                if (ps.z >= 0) ks.klow[static_cast<int64_t>(ps.z) * ks.np + c] = nv[c];
+                if (ps.w >= 0) ks.klow[static_cast<int64_t>(ps.w) * ks.np + c] = nv[c];
+            }
+        }
+        return;
+    }
     const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= vd.count) return;
     const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * vd.dim;
@@ -449,29 +497,31 @@ using VertexKernel = void (*)(VDesc, const int32_t*, const int2*, const uint8_t*
                               const double*, const double*, double*, StageArgs);
 
 template <int VM, int EM, int WRAP, bool STAGE>
-VertexKernel pick3(bool multi) {
-    return multi ? k_vertex_rhs<VM, ND_EM_NOOP, ND_WRAP_NONE, true, STAGE> : k_vertex_rhs<VM, EM, WRAP, false, STAGE>;
+VertexKernel pick3(bool kc) {
+    if constexpr (EModel<EM>::np > 0 && !is_ode_model(EM))
+        if (kc) return k_vertex_rhs<VM, EM, WRAP, false, STAGE, true>;
+    return k_vertex_rhs<VM, EM, WRAP, false, STAGE, false>;
 }
 
 template <int VM, bool STAGE>
-VertexKernel pick_edge(int em, int wrap, bool multi) {
-    if (multi) return k_vertex_rhs<VM, ND_EM_NOOP, ND_WRAP_NONE, true, STAGE>;
+VertexKernel pick_edge(int em, int wrap, bool multi, bool kc) {
+    if (multi) return k_vertex_rhs<VM, ND_EM_NOOP, ND_WRAP_NONE, true, STAGE, false>;
     const bool sym = wrap == ND_WRAP_SYMMETRIC;
     switch (em) {
     case ND_EM_KURAMOTO:
-        return sym ? pick3<VM, ND_EM_KURAMOTO, ND_WRAP_SYMMETRIC, STAGE>(false)
-                   : pick3<VM, ND_EM_KURAMOTO, ND_WRAP_ANTISYMMETRIC, STAGE>(false);
+        return sym ? pick3<VM, ND_EM_KURAMOTO, ND_WRAP_SYMMETRIC, STAGE>(kc)
+                   : pick3<VM, ND_EM_KURAMOTO, ND_WRAP_ANTISYMMETRIC, STAGE>(kc);
     case ND_EM_STATIC_LINE:
-        return sym ? pick3<VM, ND_EM_STATIC_LINE, ND_WRAP_SYMMETRIC, STAGE>(false)
-                   : pick3<VM, ND_EM_STATIC_LINE, ND_WRAP_ANTISYMMETRIC, STAGE>(false);
+        return sym ? pick3<VM, ND_EM_STATIC_LINE, ND_WRAP_SYMMETRIC, STAGE>(kc)
+                   : pick3<VM, ND_EM_STATIC_LINE, ND_WRAP_ANTISYMMETRIC, STAGE>(kc);
     case ND_EM_DIFFUSION:
         return sym ? pick3<VM, ND_EM_DIFFUSION, ND_WRAP_SYMMETRIC, STAGE>(false)
                    : pick3<VM, ND_EM_DIFFUSION, ND_WRAP_ANTISYMMETRIC, STAGE>(false);
     case ND_EM_DIFFUSION_WEIGHTED:
-        return sym ? pick3<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_SYMMETRIC, STAGE>(false)
-                   : pick3<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_ANTISYMMETRIC, STAGE>(false);
+        return sym ? pick3<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_SYMMETRIC, STAGE>(kc)
+                   : pick3<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_ANTISYMMETRIC, STAGE>(kc);
     case ND_EM_DIFFUSION_FIDUCIAL:
-        return pick3<VM, ND_EM_DIFFUSION_FIDUCIAL, ND_WRAP_ANTISYMMETRIC, STAGE>(false);
+        return pick3<VM, ND_EM_DIFFUSION_FIDUCIAL, ND_WRAP_ANTISYMMETRIC, STAGE>(kc);
     case ND_EM_ODE_SIN_DIFFUSION:
     case ND_EM_ODE_SIN_DIFFUSION_FAST:
     case ND_EM_ODE_LINEAR:
@@ -482,14 +532,14 @@ VertexKernel pick_edge(int em, int wrap, bool multi) {
 }
 
 template <bool STAGE>
-VertexKernel pick_vertex_kernel(int vm, int em, int wrap, bool multi) {
+VertexKernel pick_vertex_kernel(int vm, int em, int wrap, bool multi, bool kc) {
     switch (vm) {
-    case ND_VM_DIFFUSION: return pick_edge<ND_VM_DIFFUSION, STAGE>(em, wrap, multi);
-    case ND_VM_KURAMOTO: return pick_edge<ND_VM_KURAMOTO, STAGE>(em, wrap, multi);
-    case ND_VM_SWING: return pick_edge<ND_VM_SWING, STAGE>(em, wrap, multi);
-    case ND_VM_SWING_UNIT_DAMPING: return pick_edge<ND_VM_SWING_UNIT_DAMPING, STAGE>(em, wrap, multi);
-    case ND_VM_LOAD: return pick_edge<ND_VM_LOAD, STAGE>(em, wrap, multi);
-    default: return k_vertex_rhs<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false, STAGE>;
+    case ND_VM_DIFFUSION: return pick_edge<ND_VM_DIFFUSION, STAGE>(em, wrap, multi, kc);
+    case ND_VM_KURAMOTO: return pick_edge<ND_VM_KURAMOTO, STAGE>(em, wrap, multi, kc);
+    case ND_VM_SWING: return pick_edge<ND_VM_SWING, STAGE>(em, wrap, multi, kc);
+    case ND_VM_SWING_UNIT_DAMPING: return pick_edge<ND_VM_SWING_UNIT_DAMPING, STAGE>(em, wrap, multi, kc);
+    case ND_VM_LOAD: return pick_edge<ND_VM_LOAD, STAGE>(em, wrap, multi, kc);
+    default: return k_vertex_rhs<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false, STAGE, false>;
     }
 }
 
@@ -515,6 +565,15 @@ int VModelDim(int vm) {
     }
 }
 bool edge_model_known(int em) { return em >= ND_EM_NOOP && em <= ND_EM_ODE_LINEAR; }
+int edge_params(int em) {
+    switch (em) {
+    case ND_EM_KURAMOTO: return EModel<ND_EM_KURAMOTO>::np;
+    case ND_EM_STATIC_LINE: return EModel<ND_EM_STATIC_LINE>::np;
+    case ND_EM_DIFFUSION_WEIGHTED: return EModel<ND_EM_DIFFUSION_WEIGHTED>::np;
+    case ND_EM_DIFFUSION_FIDUCIAL: return EModel<ND_EM_DIFFUSION_FIDUCIAL>::np;
+    default: return 0;
+    }
+}
 
 template <typename T>
 struct DevBuf {
@@ -554,6 +613,14 @@ struct nd_plan {
     uint8_t* d_lseb = nullptr;
     int32_t n_agg_hub = 0;
     std::vector<size_t> hub_offset;
+    // slot-ordered edge parameters (one feed-forward edge batch with parameters)
+    bool kcache = false;
+    int32_t knp = 0, kcount = 0, kparam0 = 0;
+    std::vector<int32_t> slot_eq_host;   // per full-list entry: (edge position << 1) | is_src
+    double* d_kc = nullptr;
+    double* d_kfull = nullptr;
+    double* d_klow = nullptr;
+    int4* d_kpos = nullptr;
 
     ~nd_plan() {
         for (void* p : allocations) (void)hipFree(p);
@@ -597,6 +664,12 @@ int rebuild_hubs(nd_plan* pl) {
     std::vector<uint8_t> lseb;
     std::vector<size_t> lrp_offset(pl->vdesc.size(), 0);
     pl->hub_offset.assign(pl->vdesc.size() + 1, 0);
+    std::vector<int4> kpos(pl->kcache ? pl->kcount : 0, make_int4(-1, -1, -1, -1));
+    if (pl->kcache)
+        for (size_t k = 0; k < pl->slot_eq_host.size(); ++k) {
+            const int32_t eq = pl->slot_eq_host[k];
+            (eq & 1 ? kpos[eq >> 1].x : kpos[eq >> 1].y) = static_cast<int32_t>(k);
+        }
     for (size_t b = 0; b < pl->vdesc.size(); ++b) {
         VDesc& vd = pl->vdesc[b];
         pl->hub_offset[b] = all.size();
@@ -613,6 +686,10 @@ int rebuild_hubs(nd_plan* pl) {
             else if (len > H) mid.emplace_back(-len, q);
             else
                 for (int32_t k = beg; k < beg + len; ++k) {
+                    if (pl->kcache) {
+                        const int32_t eq = pl->slot_eq_host[k];
+                        (eq & 1 ? kpos[eq >> 1].z : kpos[eq >> 1].w) = static_cast<int32_t>(lslots.size());
+                    }
                     lslots.push_back(pl->slots_host[k]);
                     if (pl->multi) lseb.push_back(pl->slot_eb_host[k]);
                 }
@@ -646,11 +723,29 @@ int rebuild_hubs(nd_plan* pl) {
     rc = rc ? rc : upload(pl, lslots, &pl->d_lslots);
     rc = rc ? rc : upload(pl, lseb, &pl->d_lseb);
     if (rc) return rc;
+    if (pl->kcache) {
+        // the cached copy and both slot layouts restart from one NaN pattern, so they agree
+        // entry for entry and the next call rewrites every edge's slots
+        for (void* x : {static_cast<void*>(pl->d_kc), static_cast<void*>(pl->d_kfull), static_cast<void*>(pl->d_klow),
+                        static_cast<void*>(pl->d_kpos)})
+            release(pl, x);
+        pl->d_kc = pl->d_kfull = pl->d_klow = nullptr;
+        pl->d_kpos = nullptr;
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        const size_t np = static_cast<size_t>(pl->knp);
+        rc = upload(pl, std::vector<double>(static_cast<size_t>(pl->kcount) * np, nan), &pl->d_kc);
+        rc = rc ? rc : upload(pl, std::vector<double>(pl->slot_eq_host.size() * np, nan), &pl->d_kfull);
+        rc = rc ? rc : upload(pl, std::vector<double>(std::max<size_t>(lslots.size(), 1) * np, nan), &pl->d_klow);
+        rc = rc ? rc : upload(pl, kpos, &pl->d_kpos);
+        if (rc) return rc;
+    }
     for (size_t b = 0; b < pl->vdesc.size(); ++b) {
         pl->vdesc[b].hubs = pl->d_hubs_all ? pl->d_hubs_all + pl->hub_offset[b] : nullptr;
         pl->vdesc[b].lrp = pl->d_lrp + lrp_offset[b];
         pl->vdesc[b].lslots = pl->d_lslots;
         pl->vdesc[b].lseb = pl->d_lseb;
+        pl->vdesc[b].kfull = pl->d_kfull;
+        pl->vdesc[b].klow = pl->d_klow;
     }
     return ND_OK;
 }
@@ -665,15 +760,41 @@ int check_device(const nd_plan* pl) {
 
 inline unsigned blocks_for(int64_t n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
 
+KSync no_sync(unsigned vblocks) {
+    KSync k{};
+    k.vblocks = static_cast<int32_t>(vblocks);
+    return k;
+}
+
 // One right-hand side (stage = false: du = f(u)) or one fused RK stage (stage = true).
+// sync: compare p's edge parameters with the slot-ordered copy first (skipped only
+// where p is known to be unchanged since the previous launch of this plan).
 void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const double* p, const StageArgs& sa,
-                hipStream_t s) {
+                hipStream_t s, bool sync = true) {
     const int B = pl->block;
     // vertex outputs o_v = u[StateMask] into a compact array (coreloop.jl:39): the edge
     // evaluations gather neighbour outputs from it instead of from the interleaved state
-    if (pl->d_vout)
-        for (const VDesc& vd : pl->vdesc)
-            if (vd.count) hipLaunchKernelGGL(k_vertex_out, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, u, pl->d_vout);
+    if (pl->d_vout) {
+        bool pending = sync && pl->kcache;
+        for (const VDesc& vd : pl->vdesc) {
+            if (!vd.count) continue;
+            const unsigned vb = blocks_for(vd.count, 256);
+            KSync ks = no_sync(vb);
+            unsigned nb = vb;
+            if (pending) {
+                ks.pe = p + pl->kparam0;
+                ks.kc = pl->d_kc;
+                ks.kfull = pl->d_kfull;
+                ks.klow = pl->d_klow;
+                ks.kpos = pl->d_kpos;
+                ks.count = pl->kcount;
+                ks.np = pl->knp;
+                nb += blocks_for(pl->kcount, 256);
+                pending = false;
+            }
+            hipLaunchKernelGGL(k_vertex_out, dim3(nb), dim3(256), 0, s, vd, u, pl->d_vout, ks);
+        }
+    }
     for (size_t b = 0; b < pl->vdesc.size(); ++b) {
         const VDesc& vd = pl->vdesc[b];
         if (vd.count == 0) continue;
@@ -782,6 +903,7 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
     std::vector<int32_t> cols(nnz);
     std::vector<int2> slots(nnz);
     std::vector<uint8_t> slot_eb(pl->multi ? nnz : 0);
+    std::vector<int32_t> slot_eq(nnz);
     if (pl->multi && n.eb.size() > 255) { delete pl; return nd::set_error(ND_ERR_UNSUPPORTED, "more than 255 edge batches"); }
     {
         // output index -> (edge batch, position, side): batches hold ascending output blocks
@@ -809,6 +931,7 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
                 }
                 slots[s] = sl;
                 if (pl->multi) slot_eb[s] = static_cast<uint8_t>(b);
+                slot_eq[s] = static_cast<int32_t>((q << 1) | (is_src ? 1 : 0));
             }
         }
     }
@@ -828,6 +951,17 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
     rc = rc ? rc : upload(pl, slot_eb, &pl->d_slot_eb);
     pl->slots_host = std::move(slots);
     pl->slot_eb_host = std::move(slot_eb);
+    if (n.eb.size() == 1 && n.vdepth == 1 && n.ne > 0) {
+        const nd_kind& k = n.kinds[n.eb[0].kind];
+        const int np = edge_params(k.model);
+        if (np > 0 && !is_ode_model(k.model) && k.pdim == np && !no_rows) {
+            pl->kcache = true;
+            pl->knp = np;
+            pl->kcount = pl->edesc[0].count;
+            pl->kparam0 = pl->edesc[0].param0;
+            pl->slot_eq_host = std::move(slot_eq);
+        }
+    }
     rc = rc ? rc : upload(pl, cols, &pl->d_cols);
     rc = rc ? rc : upload(pl, gmap, &pl->d_gmap);
     for (size_t b = 0; b < n.eb.size() && !rc; ++b) {
@@ -852,8 +986,10 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         vd.out0 = static_cast<int32_t>(B.out_first - 1);
         for (int j = 0; j < ND_MAX_OUT; ++j) vd.mask[j] = k.mask_dst[j] - 1;
         pl->vdesc.push_back(vd);
-        pl->vkern.push_back(pick_vertex_kernel<false>(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi));
-        pl->vkern_stage.push_back(pick_vertex_kernel<true>(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi));
+        pl->vkern.push_back(pick_vertex_kernel<false>(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi,
+                                                      pl->kcache));
+        pl->vkern_stage.push_back(pick_vertex_kernel<true>(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi,
+                                                           pl->kcache));
     }
     rc = rebuild_hubs(pl);
     if (rc) { delete pl; return rc; }
@@ -939,11 +1075,12 @@ int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_
     double* yb = work + 2 * pl->dim;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const double half = 0.5 * dt, sixth = dt / 6.0;
+    // p is const for the whole call: its parameters are compared with the cached copy once
     for (int64_t k = 0; k < nsteps; ++k) {
-        launch_rhs(pl, true, nullptr, u, p, StageArgs{u, acc, ya, half, 0.0, 1}, s);
-        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, yb, half, 2.0, 2}, s);
-        launch_rhs(pl, true, nullptr, yb, p, StageArgs{u, acc, ya, dt, 2.0, 2}, s);
-        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, u, sixth, 0.0, 3}, s);
+        launch_rhs(pl, true, nullptr, u, p, StageArgs{u, acc, ya, half, 0.0, 1}, s, k == 0);
+        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, yb, half, 2.0, 2}, s, false);
+        launch_rhs(pl, true, nullptr, yb, p, StageArgs{u, acc, ya, dt, 2.0, 2}, s, false);
+        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, u, sixth, 0.0, 3}, s, false);
     }
     return hip_check(hipGetLastError(), "nd_rk4 launch");
 }
@@ -991,7 +1128,9 @@ int nd_get_buffers(nd_plan* pl, double* o, double* aggbuf, const double* u, cons
         hipLaunchKernelGGL(k_fill, dim3(nba), dim3(256), 0, s, aggbuf, pl->lastidx_aggr, 0.0);
     }
     for (const VDesc& vd : pl->vdesc)
-        if (vd.count) hipLaunchKernelGGL(k_vertex_out, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, u, o);
+        if (vd.count)
+            hipLaunchKernelGGL(k_vertex_out, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, u, o,
+                               no_sync(blocks_for(vd.count, 256)));
     for (const EDesc& e : pl->edesc)
         if (e.count) hipLaunchKernelGGL(k_edge_out, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, u, p, o);
     rc = hip_check(hipGetLastError(), "nd_get_buffers launch");

@@ -247,3 +247,65 @@ def test_rk4_swing_and_ode_edges(ND, dev):
     p = rng.uniform(0.5, 2, nw.pdim)
     ref = oracle_for(nw).rk4(u, p, 0.0, 0.002, 100)
     np.testing.assert_allclose(_dev_rk4(nw, u, p, dev, 0.002, 100), ref, rtol=0, atol=1e-12)
+
+
+def test_edge_parameter_changes_between_calls(ND, dev):
+    # the slot-ordered copy of the edge parameters must follow every change of p: a few
+    # edges, all edges, alternating parameter vectors, -0.0 vs 0.0, and a hub re-tune
+    import torch
+    nw, p, u0 = ND.workloads.swing_ba(20_000, 2, seed=8)
+    on = oracle_for(nw)
+    n, E = nw.nv(), nw.ne()
+    rng = np.random.default_rng(8)
+    u = torch.from_numpy(u0).to(dev)
+    du = torch.empty_like(u)
+
+    def check(ph):
+        nw(du, u, torch.from_numpy(ph).to(dev), 0.0)
+        torch.cuda.synchronize()
+        err = scaled_error(du.cpu().numpy(), on.rhs(u0, ph), on.scale(u0, ph))
+        assert err.max() <= RTOL
+
+    p1 = p.copy()
+    check(p1)
+    p2 = p1.copy()
+    idx = rng.choice(E, 50, replace=False)
+    p2[3 * n + idx] = rng.uniform(5, 10, 50)
+    check(p2)
+    p3 = p1.copy()
+    p3[3 * n:] = rng.uniform(5, 10, E)
+    for ph in (p3, p1, p3, p2):
+        check(ph)
+    p4 = p1.copy()
+    p4[3 * n + idx[:5]] = 0.0
+    check(p4)
+    p4[3 * n + idx[:5]] = -0.0
+    check(p4)
+    nw.tune(hub_degree=8)
+    check(p2)
+    nw.tune(hub_degree=48)
+    check(p1)
+    # same tensor modified in place between calls
+    pd = torch.from_numpy(p1.copy()).to(dev)
+    nw(du, u, pd, 0.0)
+    pd[3 * n + int(idx[0])] = 123.0
+    nw(du, u, pd, 0.0)
+    torch.cuda.synchronize()
+    p5 = p1.copy()
+    p5[3 * n + int(idx[0])] = 123.0
+    assert scaled_error(du.cpu().numpy(), on.rhs(u0, p5), on.scale(u0, p5)).max() <= RTOL
+    # RK4 after a parameter change
+    ref = on.rk4(u0, p3, 0.0, 0.001, 20)
+    np.testing.assert_allclose(_dev_rk4(nw, u0, p3, dev, 0.001, 20), ref, rtol=0, atol=1e-10)
+
+
+def test_static_line_parameter_cache(ND, dev):
+    # two parameters per edge (admittance, phase shift), several vertex batches
+    nw, p, u0 = ND.workloads.heterogeneous_rgg(20_000, seed=9)
+    assert nw.info.nebatches == 1
+    _check(nw, u0, p, dev)
+    lo, hi = int(nw.ranges()[1][0, 3, 0]) - 1, nw.pdim     # one static-line batch: params contiguous
+    p2 = p.copy()
+    p2[lo + 1:hi:2] = 0.05                      # phase shifts only
+    _check(nw, u0, p2, dev)
+    _check(nw, u0, p, dev)
