@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
                     help="c2 (default): the N=1e6 swing RHS line; c1/c3/c4/c5: the other workloads")
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="keyword arguments of the c1/c3/c4/c5 workload runs (numbers)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -89,7 +91,8 @@ def main():
         from importlib import import_module
         ex = import_module(ND.__name__ + ".experiments")
         fn = dict(c1=ex.c1_rk4, c3=ex.c3_rhs, c4=ex.c4_ensemble, c5=ex.c5_warmstart)[args.config]
-        print(json.dumps(dict(config=args.config, **fn(dev))))
+        kw = {k: float(v) if "." in v or "e" in v else int(v) for k, v in (x.split("=", 1) for x in args.set)}
+        print(json.dumps(dict(config=args.config, **fn(dev, **kw))))
         return
     nw, p, u0 = ND.workloads.swing_ba(args.n, 2, seed=1)
     N, E = nw.nv(), nw.ne()

@@ -149,12 +149,32 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
                 speedup=traj_ens / traj_base)
 
 
-def c5_grids(count, seed, nmin=500, nmax=5000):
+def _grid_job(args):
+    n, s = args
+    return swing_load_grid(int(n), seed=int(s))
+
+
+def c5_grids(count, seed, nmin=500, nmax=5000, workers=16):
+    """Seeded swing/load grids; generated in a process pool while the GPU is untouched."""
     rng = np.random.default_rng(seed)
-    return [swing_load_grid(int(n), seed=seed * 1000003 + i) for i, n in enumerate(rng.integers(nmin, nmax + 1, count))]
+    jobs = [(int(n), seed * 1000003 + i) for i, n in enumerate(rng.integers(nmin, nmax + 1, count))]
+    try:
+        import torch
+        gpu_started = torch.cuda.is_initialized()
+    except ImportError:
+        gpu_started = False
+    if count < 64 or workers < 2 or gpu_started:
+        return [_grid_job(j) for j in jobs]
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_grid_job, jobs, chunksize=32)
 
 
-def c5_warmstart(dev, ntrain=2000, neval=200, epochs=30, batch_grids=100, lr=2e-3, seed=7):
+def _log(msg):
+    print(msg, flush=True)
+
+
+def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=500, lr=2e-3, seed=7):
     import torch
     from . import warmstart as W
     out = {}
@@ -162,10 +182,12 @@ def c5_warmstart(dev, ntrain=2000, neval=200, epochs=30, batch_grids=100, lr=2e-
     train = c5_grids(ntrain, seed)
     evalg = c5_grids(neval, seed + 1)
     out["generate_s"] = time.perf_counter() - t0
+    _log(f"generated {len(train)} + {len(evalg)} grids in {out['generate_s']:.1f} s")
     # training data: fixpoints from flat starts (the solver's own output)
     th, st, tsol = W.solve_grids(train, dev)
     ok = st["retcode"] == 0
     out.update(train_grids=int(ok.sum()), train_dropped=int((~ok).sum()), train_newton_s=tsol)
+    _log(f"training fixpoints: {int(ok.sum())} converged, {tsol:.1f} s")
     train = [g for g, o in zip(train, ok) if o]
     th = [W.center(t) for t, o in zip(th, ok) if o]
     dc, _, _ = W.solve_grids(train, dev, linearized=True)
@@ -189,6 +211,7 @@ def c5_warmstart(dev, ntrain=2000, neval=200, epochs=30, batch_grids=100, lr=2e-
         for bt, Xd, yd in batches:
             tot += net.train_step(bt, Xd, yd, lr=lr).item()
         hist.append(tot / len(batches))
+        _log(f"epoch {ep}: loss {hist[-1]:.4e}")
     torch.cuda.synchronize()
     out.update(train_s=time.perf_counter() - tt, loss_first=hist[0], loss_last=hist[-1], epochs=epochs)
     # held-out evaluation
