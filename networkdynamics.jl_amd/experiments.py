@@ -8,6 +8,7 @@ c4_ensemble   4096 uneven Barabasi-Albert swing networks (N ~ U{100..2000}) from
               the fused ensemble kernel against the per-instance GPU path
 c5_warmstart  swing/load grids: Newton from flat starts vs from the learned warm start
 """
+import sys
 import time
 
 import numpy as np
@@ -18,6 +19,10 @@ from . import workloads
 from .ensemble import RK4, TSIT5, Ensemble, tsit5_per_instance
 from .fixpoint import FixpointSolver, isfixpoint_residual, swing_load_grid
 from .network import Network
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def _timeit(fn, reps, warm=2):
@@ -96,6 +101,7 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
     torch.cuda.synchronize()
     t_newton = time.perf_counter() - tn
     ok = st["retcode"] == 0
+    _log(f"fixpoints: {int(ok.sum())} of {len(inst)} in {t_newton:.2f} s")
     thh = th.cpu().numpy()
     keep = [i for i in range(len(inst)) if ok[i]]
     graphs, us, ps = [], [], []
@@ -124,18 +130,24 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
     uf = u_d.cpu().numpy()
     sync = np.array([np.abs(uf[ens.state_slice(i)][1::2]).max() < 0.5 for i in range(len(graphs))])
     # plain per-instance GPU path on a time-bounded sample (largest-first order not used: random sample)
+    _log(f"ensemble: {len(graphs)} trajectories in {t_ens:.2f} s")
     order = rng.permutation(len(graphs))
-    done, tb0 = 0, time.perf_counter()
+    done, t_base = 0, 0.0
     steps = []
     for i in order:
-        nw = Network(graphs[i], Lib.kuramoto_second(), Lib.kuramoto_edge())
+        nw = Network(graphs[i], Lib.kuramoto_second(), Lib.kuramoto_edge())      # construction not timed
         ui = torch.from_numpy(us[i]).to(dev)
-        info = tsit5_per_instance(nw, ui, torch.from_numpy(ps[i]).to(dev), (0.0, t1), reltol=reltol, abstol=abstol)
+        pi = torch.from_numpy(ps[i]).to(dev)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        info = tsit5_per_instance(nw, ui, pi, (0.0, t1), reltol=reltol, abstol=abstol)
+        torch.cuda.synchronize()
+        t_base += time.perf_counter() - tb
         steps.append(info["naccept"])
         done += 1
-        if time.perf_counter() - tb0 > baseline_seconds:
+        if t_base > baseline_seconds:
             break
-    t_base = time.perf_counter() - tb0
+    _log(f"per-instance path: {done} trajectories in {t_base:.2f} s")
     traj_ens = len(graphs) / t_ens
     traj_base = done / t_base
     return dict(workload=f"{len(graphs)} BA(m=2) swing networks N~U{{100..2000}}, Tsit5 t=0..{t1}, "
@@ -168,10 +180,6 @@ def c5_grids(count, seed, nmin=500, nmax=5000, workers=16):
     import multiprocessing as mp
     with mp.get_context("fork").Pool(workers) as pool:
         return pool.map(_grid_job, jobs, chunksize=32)
-
-
-def _log(msg):
-    print(msg, flush=True)
 
 
 def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=500, lr=2e-3, seed=7):
