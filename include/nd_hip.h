@@ -297,13 +297,16 @@ typedef struct {
     int32_t max_newton;    /* 0 = 50                                                       */
     int32_t cg_maxiter;    /* 0 = 20000                                                    */
     int32_t linearized;    /* 1: solve the DC power flow P + sum K (theta_j - theta_i) = 0 */
-    int32_t pad_;
+    int32_t inexact;       /* 1: each Newton step's CG stops at the relative residual
+                              max(cg_rtol, min(1e-2, current residual)) (inexact Newton
+                              with a residual-based forcing term); 0: always cg_rtol       */
 } nd_newton_opts;
 typedef struct {
     double residual;       /* final max |F_i/scale_i|                                      */
     int32_t newton_iters;  /* linear solves (Newton steps) taken                           */
     int32_t cg_iters;      /* conjugate-gradient iterations over all steps                 */
-    int32_t retcode;       /* 0 converged, 1 max_newton reached, 2 line search stalled     */
+    int32_t retcode;       /* 0 converged, 1 max_newton reached, 2 line search stalled,
+                              3 non-finite residual                                      */
     int32_t pad_;
 } nd_newton_stat;
 int nd_fixpoint_create(int32_t ngrid, const int64_t* nv, const int64_t* ne, const int64_t* src, const int64_t* dst,

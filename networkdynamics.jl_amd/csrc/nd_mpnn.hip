@@ -176,13 +176,16 @@ __global__ void __launch_bounds__(256) k_spmm(int64_t n, const int32_t* __restri
     out[i * ldo + c] = s;
 }
 
-// G[i, c] *= (H[i, c] > 0)
-__global__ void k_relu_mask(int64_t n, float* __restrict__ G, int ldg, const float* __restrict__ H, int ldh) {
+// g = dH * relu'(H): G[i, c] = H[i, c] > 0 ? DH[i, c] : 0, also copied to DY[i, 64 + c] when DY is given
+__global__ void k_relu_grad(int64_t n, const float* __restrict__ DH, const float* __restrict__ H, float* __restrict__ G,
+                            float* __restrict__ DY) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n * kH) return;
     const int64_t i = t / kH;
     const int c = static_cast<int>(t % kH);
-    if (!(H[i * ldh + c] > 0.0f)) G[i * ldg + c] = 0.0f;
+    const float g = H[t] > 0.0f ? DH[t] : 0.0f;
+    G[t] = g;
+    if (DY) DY[i * 2 * kH + kH + c] = g;
 }
 
 // out[i] = H[i, :] . w + b ; optional MSE gradient against target and loss partials
@@ -322,21 +325,17 @@ int nd_mp_backward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const f
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, static_cast<int>(nb), 1, part, grads + o.bout, 0);
     }
     for (int l = kLayers - 1; l >= 0; --l) {
-        // g = dH_{l+1} * relu'(H_{l+1})
-        (void)hipMemcpyAsync(G, DH, sizeof(float) * n * kH, hipMemcpyDeviceToDevice, s);
-        hipLaunchKernelGGL(k_relu_mask, dim3(nblk(n * kH, 256)), dim3(256), 0, s, n, G, kH, H + (l + 1) * n * kH, kH);
+        // g = dH_{l+1} * relu'(H_{l+1}), into G and the right half of [dYa | g]
+        hipLaunchKernelGGL(k_relu_grad, dim3(nblk(n * kH, 256)), dim3(256), 0, s, n, DH, H + (l + 1) * n * kH, G, DY);
         colsum(s, n, G, kH, part, grads + o.bl[l]);
-        // [dYa | g]: dYa = A^T g
+        // dYa = A^T g into the left half
         hipLaunchKernelGGL(k_spmm, dim3(nblk(n, 4)), dim3(256), 0, s, n, rowptr, nbr, a_bwd, G, kH, DY, 2 * kH,
                            nullptr, 0);
-        (void)hipMemcpy2DAsync(DY + kH, 2 * kH * sizeof(float), G, kH * sizeof(float), kH * sizeof(float), n,
-                         hipMemcpyDeviceToDevice, s);
         // d[W|U] = H_l^T [dYa | g];  dH_l = [dYa | g] [W|U]^T
         gemm_tn(s, n, kH, 2 * kH, H + l * n * kH, kH, DY, 2 * kH, part, grads + o.wl[l], 0);
         gemm(s, true, n, kH, 2 * kH, DY, 2 * kH, params + o.wl[l], 2 * kH, DH, kH, nullptr, 0);
     }
-    (void)hipMemcpyAsync(G, DH, sizeof(float) * n * kH, hipMemcpyDeviceToDevice, s);
-    hipLaunchKernelGGL(k_relu_mask, dim3(nblk(n * kH, 256)), dim3(256), 0, s, n, G, kH, H, kH);
+    hipLaunchKernelGGL(k_relu_grad, dim3(nblk(n * kH, 256)), dim3(256), 0, s, n, DH, H, G, nullptr);
     colsum(s, n, G, kH, part, grads + o.bin);
     gemm_tn(s, n, nfeat, kH, X, nfeat, G, kH, part, grads + o.win, 0);
     return herr(hipGetLastError(), "nd_mp_backward");

@@ -49,7 +49,7 @@ struct NewtonArgs {
     const int32_t* order;    // launch position -> grid index
     double tol, cg_rtol;
     int32_t max_newton, cg_maxiter;
-    int32_t linearized;
+    int32_t linearized, inexact;
     int32_t first;
 };
 
@@ -156,7 +156,8 @@ __global__ void __launch_bounds__(kThreads) k_newton(NewtonArgs a) {
         }
         rz = block_reduce(rz, red, false);
         rr0 = block_reduce(rr0, red, false);
-        const double stop2 = a.cg_rtol * a.cg_rtol * rr0;
+        const double eta = a.inexact ? fmax(a.cg_rtol, fmin(1e-2, res)) : a.cg_rtol;
+        const double stop2 = eta * eta * rr0;
         double rr = rr0;
 #pragma unroll
         for (int k = 0; k < V; ++k) {
@@ -225,6 +226,7 @@ __global__ void __launch_bounds__(kThreads) k_newton(NewtonArgs a) {
         if (!(nres < res)) { res = nres; retcode = 2; break; }
         res = nres;
     }
+    if (!(res == res)) retcode = 3;         // a NaN residual never counts as converged
 #pragma unroll
     for (int k = 0; k < V; ++k) {
         const int v = t + k * T;
@@ -382,6 +384,7 @@ int nd_fixpoint_solve(nd_fixpoint* f, double* theta, const double* P, const doub
     a.max_newton = o->max_newton > 0 ? o->max_newton : 50;
     a.cg_maxiter = o->cg_maxiter > 0 ? o->cg_maxiter : 20000;
     a.linearized = o->linearized ? 1 : 0;
+    a.inexact = o->inexact ? 1 : 0;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const size_t ns = f->small.size(), nl = f->large.size();
     if (ns) {

@@ -46,9 +46,12 @@ def test_reference_backward_matches_finite_differences():
 
 
 @pytest.mark.gpu
-def test_gpu_forward_backward_match_reference(ND, dev):
+@pytest.mark.parametrize("n", [300, 100_000])
+def test_gpu_forward_backward_match_reference(ND, dev, n):
+    # n = 100000 rows: past every 16-bit grid and copy-height limit; the workspace starts
+    # as NaN so any entry the kernels fail to write shows up
     import torch
-    A, params, X, y = _tiny_problem(n=300, seed=2)
+    A, params, X, y = _tiny_problem(n=n, seed=2)
     At = A.T.tocsr()
     At.sort_indices()
     # the kernels take A's pattern with transposed weights on the same pattern
@@ -61,9 +64,9 @@ def test_gpu_forward_backward_match_reference(ND, dev):
     n = A.shape[0]
     d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
     rp_d, nb_d, af_d, ab_d, X_d, p_d, y_d = d(rp), d(nbr), d(a_f), d(a_b), d(X.astype(np.float32)), d(p32), d(y.astype(np.float32))
-    work = torch.empty(int(L.nd_mp_workspace_floats(n)), dtype=torch.float32, device=dev)
-    out = torch.empty(n, dtype=torch.float32, device=dev)
-    grads = torch.empty_like(p_d)
+    work = torch.full((int(L.nd_mp_workspace_floats(n)),), float("nan"), dtype=torch.float32, device=dev)
+    out = torch.full((n,), float("nan"), dtype=torch.float32, device=dev)
+    grads = torch.full_like(p_d, float("nan"))
     lp = torch.zeros(n // 4 + 2, dtype=torch.float32, device=dev)
     s = torch.cuda.current_stream(dev).cuda_stream
     ND._abi.check(L.nd_mp_forward(n, rp_d.data_ptr(), nb_d.data_ptr(), af_d.data_ptr(), X_d.data_ptr(), 8,
