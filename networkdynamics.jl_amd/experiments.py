@@ -182,7 +182,9 @@ def c5_grids(count, seed, nmin=500, nmax=5000, workers=16):
         return pool.map(_grid_job, jobs, chunksize=32)
 
 
-def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=500, lr=2e-3, seed=7):
+def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=50, lr=2e-3, seed=7):
+    """Train on fixpoints of ntrain grids, evaluate Newton from flat / DC / physics / learned
+    starts on neval held-out grids (grids without a flat-start fixpoint are dropped)."""
     import torch
     from . import warmstart as W
     out = {}
@@ -191,57 +193,73 @@ def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=500, lr=2
     evalg = c5_grids(neval, seed + 1)
     out["generate_s"] = time.perf_counter() - t0
     _log(f"generated {len(train)} + {len(evalg)} grids in {out['generate_s']:.1f} s")
-    # training data: fixpoints from flat starts (the solver's own output)
     th, st, tsol = W.solve_grids(train, dev)
     ok = st["retcode"] == 0
     out.update(train_grids=int(ok.sum()), train_dropped=int((~ok).sum()), train_newton_s=tsol)
     _log(f"training fixpoints: {int(ok.sum())} converged, {tsol:.1f} s")
     train = [g for g, o in zip(train, ok) if o]
     th = [W.center(t) for t, o in zip(th, ok) if o]
-    dc, _, _ = W.solve_grids(train, dev, linearized=True)
-    dc = [W.center(t) for t in dc]
+    dc, s0, X, _ = W.physics_inputs(train, dev)
+    tgt = [W.source(g, t) - a for g, t, a in zip(train, th, s0)]
     net = W.WarmStartNet(dev, seed=0)
-    X = np.concatenate([W.features(g, t) for g, t in zip(train, dc)])
-    net.xmean, net.xstd = X.mean(axis=0), X.std(axis=0) + 1e-12
-    tgt = [s - d for s, d in zip(th, dc)]
+    Xall = np.concatenate(X)
+    net.xmean, net.xstd = Xall.mean(axis=0), Xall.std(axis=0) + 1e-12
     net.tscale = 1.0 / np.concatenate(tgt).std()
     batches = []
     for b in range(0, len(train), batch_grids):
         gs = train[b:b + batch_grids]
-        Xb = np.concatenate([W.features(g, t) for g, t in zip(gs, dc[b:b + batch_grids])])
         yb = np.concatenate(tgt[b:b + batch_grids]) * net.tscale
-        batches.append((W.GraphBatch(gs, dev), net.normalise(Xb), torch.from_numpy(yb.astype(np.float32)).to(dev)))
+        batches.append((W.GraphBatch(gs, dev), net.normalise(np.concatenate(X[b:b + batch_grids])),
+                        torch.from_numpy(yb.astype(np.float32)).to(dev)))
     torch.cuda.synchronize()
     tt = time.perf_counter()
     hist = []
     for ep in range(epochs):
         tot = 0.0
         for bt, Xd, yd in batches:
-            tot += net.train_step(bt, Xd, yd, lr=lr).item()
+            tot += net.train_step(bt, Xd, yd, lr=lr * (0.1 if ep >= int(0.8 * epochs) else 1.0)).item()
         hist.append(tot / len(batches))
         _log(f"epoch {ep}: loss {hist[-1]:.4e}")
     torch.cuda.synchronize()
-    out.update(train_s=time.perf_counter() - tt, loss_first=hist[0], loss_last=hist[-1], epochs=epochs)
-    # held-out evaluation
+    out.update(train_s=time.perf_counter() - tt, loss_first=hist[0], loss_last=hist[-1], epochs=epochs,
+               adam_steps=epochs * len(batches))
+    # held-out grids
     fth, fst, ft = W.solve_grids(evalg, dev)
     keep = fst["retcode"] == 0
     evalg = [g for g, o in zip(evalg, keep) if o]
+    fth = [t for t, o in zip(fth, keep) if o]
+    fl = fst["newton_iters"][keep]
     torch.cuda.synchronize()
     tw = time.perf_counter()
-    dce, dst, dct = W.solve_grids(evalg, dev, linearized=True)
-    warm = W.predict(net, evalg, [W.center(t) for t in dce], dev)
+    dce, s0e, Xe, t_dc = W.physics_inputs(evalg, dev)
+    tp = time.perf_counter()
+    that = W.predict(net, evalg, Xe, dev)
     torch.cuda.synchronize()
-    t_pred = time.perf_counter() - tw
-    wth, wst, wt = W.solve_grids(evalg, dev, theta0=np.concatenate(warm))
-    dth, dst2, dt2 = W.solve_grids(evalg, dev, theta0=np.concatenate([W.center(t) for t in dce]))
-    acc_flat = sum(isfixpoint_residual(g, t, dev) <= 1e-10 for g, t in zip(evalg, [x for x, o in zip(fth, keep) if o]))
-    acc_warm = sum(isfixpoint_residual(g, t, dev) <= 1e-10 for g, t in zip(evalg, wth))
-    fl = fst["newton_iters"][keep]
-    out.update(eval_grids=len(evalg), flat_newton_mean=float(fl.mean()), warm_newton_mean=float(wst["newton_iters"].mean()),
-               dc_only_newton_mean=float(dst2["newton_iters"].mean()),
-               newton_ratio=float(fl.mean() / max(wst["newton_iters"].mean(), 1e-9)),
-               flat_linear_solves_mean=float(fl.mean()), warm_linear_solves_mean=float(1 + wst["newton_iters"].mean()),
-               flat_s=ft, warm_total_s=t_pred + wt, warm_newton_s=wt, warm_dc_and_predict_s=t_pred,
-               accepted_flat=int(acc_flat), accepted_warm=int(acc_warm),
-               warm_retcodes=np.bincount(wst["retcode"], minlength=3).tolist())
+    t_pred = time.perf_counter() - tp
+    warm, t_ws = W.warm_start(evalg, s0e, that, dev)
+    t_prep = time.perf_counter() - tw
+    phys, _ = W.warm_start(evalg, s0e, None, dev)
+    starts = dict(dc=dce, physics=phys, learned=warm)
+    res = {}
+    for name, th0 in starts.items():
+        sth, sst, stt = W.solve_grids(evalg, dev, theta0=np.concatenate(th0))
+        acc = sum(isfixpoint_residual(g, t, dev) <= 1e-10 for g, t in zip(evalg, sth))
+        res[name] = (sst, stt, acc)
+    tgt_e = [W.source(g, W.center(t)) - a for g, t, a in zip(evalg, fth, s0e)]
+    num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(that, tgt_e))
+    den = sum(float(np.sum((b - b.mean()) ** 2)) for b in tgt_e)
+    acc_flat = sum(isfixpoint_residual(g, t, dev) <= 1e-10 for g, t in zip(evalg, fth))
+    out.update(eval_grids=len(evalg), eval_dropped=int((~keep).sum()), target_r2=1.0 - num / den,
+               flat_newton_mean=float(fl.mean()), flat_s=ft, accepted_flat=int(acc_flat))
+    for name, (sst, stt, acc) in res.items():
+        out[f"{name}_newton_mean"] = float(sst["newton_iters"].mean())
+        out[f"{name}_newton_s"] = stt
+        out[f"{name}_accepted"] = int(acc)
+        out[f"{name}_retcodes"] = np.bincount(sst["retcode"], minlength=4).tolist()
+    out.update(newton_ratio_learned=float(fl.mean() / max(res["learned"][0]["newton_iters"].mean(), 1e-9)),
+               newton_ratio_physics=float(fl.mean() / max(res["physics"][0]["newton_iters"].mean(), 1e-9)),
+               warm_prep_s=t_prep, warm_dc_s=t_dc, warm_predict_s=t_pred, warm_solve_s=t_ws,
+               warm_total_s=t_prep + res["learned"][1],
+               linear_solves_flat=float(fl.mean()),
+               linear_solves_learned=float(2 + res["learned"][0]["newton_iters"].mean()))
     return out

@@ -1,13 +1,22 @@
 """Learned warm start for fixpoint solves.
 
+At a fixpoint theta* of  P_i + sum_j K_ij sin(theta_j - theta_i) = 0  the weighted
+Laplacian L_K satisfies  L_K theta* = P + s*,  with the per-vertex source
+    s_i = sum_j K_ij (sin(D_ij) - D_ij),   D_ij = theta_j - theta_i.
+So theta* is one linear solve away once s* is known.  The source is a sum over the
+vertex's own lines, small (cubic in the angle differences) and much more local than
+theta* itself, whose correction from the DC solution spreads over the whole grid.
+
 Pipeline for a batch of swing/load grids:
-  1. DC power flow theta_dc (one linear solve, `FixpointSolver(linearized=True)`);
-  2. features per vertex from the grid and the DC flows;
+  1. DC power flow theta_dc: L_K theta_dc = P (`FixpointSolver(linearized=True)`);
+  2. physics source s0 = s(theta_dc) and per-vertex features;
   3. a 4-layer message-passing network (hidden 64, FP32, nd_mp_* kernels)
-     predicts the nonlinear correction theta* - theta_dc;
-  4. Newton (`FixpointSolver`) starts from theta_dc + prediction.
-Training data are fixpoints theta* that the batched Newton solver itself produced
-from flat starts.  Without the network the same Newton starts from theta = 0.
+     predicts the remaining source correction t = s* - s0;
+  4. warm start: L_K theta_w = P + s0 + t_hat (a second linear solve);
+  5. Newton (`FixpointSolver`) from theta_w; its residual <= 1e-10 is the judge.
+Training targets come from fixpoints the batched Newton solver produced from flat
+starts.  The method without the network is Newton from theta = 0; steps 1, 2, 4
+with t_hat = 0 are the physics-only ablation.
 """
 import ctypes as C
 import time
@@ -50,23 +59,34 @@ class GraphBatch:
         self.host = dict(rowptr=rowptr, nbr=cols, a_fwd=a_fwd, a_bwd=a_bwd)
 
 
-def features(grid, theta_dc):
-    """Per-vertex inputs: P, generator flag, log weighted degree, degree, and DC-flow terms."""
+def source(grid, theta):
+    """s_i = sum_j K_ij (sin(D_ij) - D_ij), D_ij = theta_j - theta_i (per vertex)."""
+    g = grid.g
+    s, d = g.src - 1, g.dst - 1
+    D = theta[d] - theta[s]
+    q = grid.K * (np.sin(D) - D)
+    n = g.nv()
+    return np.bincount(s, weights=q, minlength=n) - np.bincount(d, weights=q, minlength=n)
+
+
+def features(grid, theta_dc, s0=None):
+    """Per-vertex inputs: P, generator flag, log weighted degree, degree, the physics
+    source s0 and its K-weighted neighbour mean, sum K D^2 and max |D| of the DC flows."""
     g = grid.g
     s, d = g.src - 1, g.dst - 1
     n = g.nv()
     x = theta_dc[d] - theta_dc[s]
     K = grid.K
+    if s0 is None:
+        s0 = source(grid, theta_dc)
     kdeg = np.bincount(s, weights=K, minlength=n) + np.bincount(d, weights=K, minlength=n)
     deg = np.bincount(s, minlength=n) + np.bincount(d, minlength=n)
-    cub = K * x ** 3 / 6.0               # sin(x) - x ~ -x^3/6: the DC solution's flow error per line
-    src3 = np.bincount(s, weights=cub, minlength=n) - np.bincount(d, weights=cub, minlength=n)
+    ns0 = (np.bincount(s, weights=K * s0[d], minlength=n) + np.bincount(d, weights=K * s0[s], minlength=n)) / kdeg
     sq = np.bincount(s, weights=K * x * x, minlength=n) + np.bincount(d, weights=K * x * x, minlength=n)
     mx = np.zeros(n)
     np.maximum.at(mx, s, np.abs(x))
     np.maximum.at(mx, d, np.abs(x))
-    return np.stack([grid.P, grid.is_gen.astype(float), np.log(kdeg), deg / 6.0, src3 * 1e3, sq, mx,
-                     np.ones(n)], axis=1)
+    return np.stack([grid.P, grid.is_gen.astype(float), np.log(kdeg), deg / 6.0, s0, ns0, sq, mx], axis=1)
 
 
 class WarmStartNet:
@@ -106,7 +126,6 @@ class WarmStartNet:
     def normalise(self, X):
         import torch
         Xn = (X - self.xmean) / self.xstd
-        Xn[:, -1] = 1.0
         return torch.from_numpy(np.ascontiguousarray(Xn, np.float32)).to(self.dev)
 
     def forward(self, batch, Xd):
@@ -156,19 +175,20 @@ def _concat(grids, attr):
                            for gr in grids])
 
 
-def solve_grids(grids, dev, theta0=None, linearized=False, **kw):
-    """Batched Newton (or DC) solve of several grids; returns (theta per grid, stats, seconds)."""
+def solve_grids(grids, dev, theta0=None, linearized=False, P=None, **kw):
+    """Batched Newton (or, linearized, L_K theta = P) solve of several grids; returns
+    (theta per grid, stats, seconds).  P: per-grid source override (linearized solves)."""
     import torch
     fs = FixpointSolver([gr.g for gr in grids])
     n = int(fs.voff[-1])
     th = torch.zeros(n, dtype=torch.float64, device=dev) if theta0 is None else \
         torch.from_numpy(np.ascontiguousarray(theta0, np.float64)).to(dev)
-    P = torch.from_numpy(np.concatenate([gr.P for gr in grids])).to(dev)
+    Pd = torch.from_numpy(np.concatenate([gr.P for gr in grids] if P is None else P)).to(dev)
     S = torch.from_numpy(np.concatenate([gr.scale for gr in grids])).to(dev)
     K = torch.from_numpy(np.concatenate([gr.K for gr in grids])).to(dev)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    st = fs.solve(th, P, S, K, linearized=linearized, **kw)
+    st = fs.solve(th, Pd, S, K, linearized=linearized, **kw)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     t = th.cpu().numpy()
@@ -179,9 +199,25 @@ def center(x):
     return x - x.mean()
 
 
-def predict(net, grids, theta_dc, dev):
-    """Warm-start states theta_dc + predicted correction, per grid."""
+def physics_inputs(grids, dev):
+    """DC solutions, physics sources s0 and features of a batch of grids."""
+    dc, _, t_dc = solve_grids(grids, dev, linearized=True)
+    dc = [center(t) for t in dc]
+    s0 = [source(gr, t) for gr, t in zip(grids, dc)]
+    X = [features(gr, t, s) for gr, t, s in zip(grids, dc, s0)]
+    return dc, s0, X, t_dc
+
+
+def predict(net, grids, X, dev):
+    """Network source corrections t_hat per grid (host float64)."""
     batch = GraphBatch(grids, dev)
-    X = np.concatenate([features(gr, td) for gr, td in zip(grids, theta_dc)])
-    out = net.forward(batch, net.normalise(X)).double().cpu().numpy() / net.tscale
-    return [center(td + center(out[batch.voff[i]:batch.voff[i + 1]])) for i, td in enumerate(theta_dc)]
+    out = net.forward(batch, net.normalise(np.concatenate(X))).double().cpu().numpy() / net.tscale
+    return [out[batch.voff[i]:batch.voff[i + 1]] for i in range(len(grids))]
+
+
+def warm_start(grids, s0, t_hat, dev):
+    """theta_w from L_K theta_w = P + s0 + t_hat (t_hat None: physics only); (theta list, seconds)."""
+    src = [gr.P + a + (0.0 if t_hat is None else b) for gr, a, b in
+           zip(grids, s0, t_hat if t_hat is not None else [None] * len(grids))]
+    th, _, el = solve_grids(grids, dev, linearized=True, P=src)
+    return [center(t) for t in th], el

@@ -83,6 +83,38 @@ def test_gpu_forward_backward_match_reference(ND, dev, n):
     assert abs(lp.sum().item() / n - loss) <= 1e-4 * loss
 
 
+def test_source_term_identity(ND):
+    # L_K theta* = P + s(theta*) at any fixpoint: checked on the 4-node ring of
+    # test/initialization_test.jl:15-38 style (balanced P, known fixpoint by Newton on the CPU)
+    W = ND.warmstart
+    g = ND.swing_load_grid(300, seed=11)
+    n = g.g.nv()
+    s, d = g.g.src - 1, g.g.dst - 1
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+
+    def lap(w):
+        A = sp.coo_matrix((np.r_[w, w], (np.r_[s, d], np.r_[d, s])), shape=(n, n)).tocsr()
+        return (sp.diags(np.asarray(A.sum(1)).ravel()) - A).tocsc()
+
+    def F(th):
+        D = th[d] - th[s]
+        f = g.K * np.sin(D)
+        return g.P + np.bincount(s, f, n) - np.bincount(d, f, n)
+
+    th = np.zeros(n)
+    for _ in range(30):
+        r = F(th)
+        if np.abs(r).max() < 1e-13:
+            break
+        dth = np.zeros(n)
+        dth[1:] = spl.spsolve(lap(g.K * np.cos(th[d] - th[s]))[1:, 1:], r[1:])
+        th = th + dth
+    assert np.abs(F(th)).max() < 1e-12
+    lhs = lap(g.K) @ th
+    np.testing.assert_allclose(lhs, g.P + W.source(g, th), rtol=0, atol=1e-10)
+
+
 @pytest.mark.gpu
 def test_trained_network_reduces_newton_iterations(ND, dev):
     import torch
@@ -93,28 +125,27 @@ def test_trained_network_reduces_newton_iterations(ND, dev):
     ok = st["retcode"] == 0
     train = [g for g, o in zip(train, ok) if o]
     th_star = [W.center(t) for t, o in zip(th_star, ok) if o]
-    th_dc, _, _ = W.solve_grids(train, dev, linearized=True)
-    th_dc = [W.center(t) for t in th_dc]
+    dc, s0, X, _ = W.physics_inputs(train, dev)
+    tgt = np.concatenate([W.source(g, t) - a for g, t, a in zip(train, th_star, s0)])
     net = W.WarmStartNet(dev, seed=0)
-    X = np.concatenate([W.features(g, t) for g, t in zip(train, th_dc)])
-    net.xmean, net.xstd = X.mean(axis=0), X.std(axis=0) + 1e-12
-    tgt = np.concatenate([s - d for s, d in zip(th_star, th_dc)])
+    Xa = np.concatenate(X)
+    net.xmean, net.xstd = Xa.mean(axis=0), Xa.std(axis=0) + 1e-12
     net.tscale = 1.0 / tgt.std()
     batch = W.GraphBatch(train, dev)
-    Xd = net.normalise(X)
+    Xd = net.normalise(Xa)
     yd = torch.from_numpy((tgt * net.tscale).astype(np.float32)).to(dev)
     first = net.train_step(batch, Xd, yd, lr=2e-3).item()
     for _ in range(300):
         last = net.train_step(batch, Xd, yd, lr=2e-3)
-    assert last.item() < 0.5 * first
-    # held-out grids: flat start vs DC + network
+    assert last.item() < 0.8 * first
+    # held-out grids: flat start vs the learned warm start; every result must pass the judge
     flat_th, flat_st, _ = W.solve_grids(test, dev)
     keep = flat_st["retcode"] == 0
     test = [g for g, o in zip(test, keep) if o]
-    dc, _, _ = W.solve_grids(test, dev, linearized=True)
-    warm0 = W.predict(net, test, [W.center(t) for t in dc], dev)
+    dce, s0e, Xe, _ = W.physics_inputs(test, dev)
+    warm0, _ = W.warm_start(test, s0e, W.predict(net, test, Xe, dev), dev)
     warm_th, warm_st, _ = W.solve_grids(test, dev, theta0=np.concatenate(warm0))
     assert (warm_st["retcode"] == 0).all()
     for g, t in zip(test, warm_th):
         assert ND.isfixpoint_residual(g, t, dev) <= 1e-10
-    assert warm_st["newton_iters"].mean() < flat_st["newton_iters"][keep].mean()
+    assert warm_st["newton_iters"].mean() <= 0.75 * flat_st["newton_iters"][keep].mean()
