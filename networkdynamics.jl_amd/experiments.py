@@ -245,6 +245,10 @@ def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=50, lr=2e
         sth, sst, stt = W.solve_grids(evalg, dev, theta0=np.concatenate(th0))
         acc = sum(isfixpoint_residual(g, t, dev) <= 1e-10 for g, t in zip(evalg, sth))
         res[name] = (sst, stt, acc)
+        _, xst, _ = W.solve_grids(evalg, dev, theta0=np.concatenate(th0), inexact=False)
+        out[f"{name}_newton_mean_exact_cg"] = float(xst["newton_iters"].mean())
+    _, xst, _ = W.solve_grids(evalg, dev, inexact=False)
+    out["flat_newton_mean_exact_cg"] = float(xst["newton_iters"][xst["retcode"] == 0].mean())
     tgt_e = [W.source(g, W.center(t)) - a for g, t, a in zip(evalg, fth, s0e)]
     num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(that, tgt_e))
     den = sum(float(np.sum((b - b.mean()) ** 2)) for b in tgt_e)

@@ -209,10 +209,11 @@ def physics_inputs(grids, dev):
 
 
 def predict(net, grids, X, dev):
-    """Network source corrections t_hat per grid (host float64)."""
+    """Network source corrections t_hat per grid (host float64), each centred: a source
+    must sum to zero over a grid for L_K theta = P + s0 + t_hat to have a solution."""
     batch = GraphBatch(grids, dev)
     out = net.forward(batch, net.normalise(np.concatenate(X))).double().cpu().numpy() / net.tscale
-    return [out[batch.voff[i]:batch.voff[i + 1]] for i in range(len(grids))]
+    return [center(out[batch.voff[i]:batch.voff[i + 1]]) for i in range(len(grids))]
 
 
 def warm_start(grids, s0, t_hat, dev):
