@@ -26,7 +26,7 @@ def _traffic_from_profile():
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
     try:
         for line in open(PROFILE_TRAFFIC):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith("traffic_bytes_per_rhs:"):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass

@@ -33,22 +33,24 @@ class GraphBatch:
     """Stacked vertices of several grids with the row-normalised K-weighted adjacency."""
 
     def __init__(self, grids, dev):
+        import scipy.sparse as sp
         import torch
         self.grids = list(grids)
         nv = np.array([gr.g.nv() for gr in self.grids], np.int64)
         self.voff = np.r_[0, np.cumsum(nv)]
         n = int(self.voff[-1])
-        rows, cols, w = [], [], []
-        for i, gr in enumerate(self.grids):
-            s, d, o = gr.g.src - 1 + self.voff[i], gr.g.dst - 1 + self.voff[i], self.voff[i]
-            rows += [s, d]
-            cols += [d, s]
-            w += [gr.K, gr.K]
-        rows, cols, w = np.concatenate(rows), np.concatenate(cols), np.concatenate(w)
-        order = np.lexsort((cols, rows))
-        rows, cols, w = rows[order], cols[order], w[order]
-        deg_w = np.bincount(rows, weights=w, minlength=n)
-        rowptr = np.r_[0, np.cumsum(np.bincount(rows, minlength=n))].astype(np.int32)
+        ne = np.array([gr.g.ne() for gr in self.grids], np.int64)
+        shift = np.repeat(self.voff[:-1], ne)
+        s = np.concatenate([gr.g.src for gr in self.grids]) - 1 + shift
+        d = np.concatenate([gr.g.dst for gr in self.grids]) - 1 + shift
+        K = np.concatenate([gr.K for gr in self.grids])
+        # CSR of the symmetric K-weighted adjacency (scipy's O(nnz) COO -> CSR conversion)
+        A_ = sp.csr_matrix((np.r_[K, K], (np.r_[s, d], np.r_[d, s])), shape=(n, n))
+        rowptr = A_.indptr.astype(np.int32)
+        cols = A_.indices
+        w = A_.data
+        deg_w = np.add.reduceat(w, rowptr[:-1])            # every vertex of a grid's largest component has a line
+        rows = np.repeat(np.arange(n), np.diff(rowptr))
         a_fwd = (w / deg_w[rows]).astype(np.float32)
         a_bwd = (w / deg_w[cols]).astype(np.float32)      # weight of the transposed entry (col -> row)
         self.n = n
