@@ -50,6 +50,7 @@ struct VDesc {            // one vertex batch as the kernels see it (0-based off
     const int32_t* lrp;   // hub-free destination lists: count+1 offsets into lslots
     const int2* lslots;
     const uint8_t* lseb;  // edge batch of each lslots entry (several edge batches only)
+    const int32_t* lnbr;  // with cached parameters instead of lslots: neighbour output index | (is_src << 31)
     const double* kfull;  // edge parameters per destination-list entry (rowptr order), or null
     const double* klow;   // the same for the hub-free lists (lrp order), or null
 };
@@ -328,7 +329,14 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int32_t s2 = base + j * T + t;
-                if (s2 < wend) sl[j] = vd.lslots[s2];
+                if (s2 < wend) {
+                    if constexpr (KC) {
+                        const uint32_t w = static_cast<uint32_t>(vd.lnbr[s2]);
+                        sl[j] = make_int2(static_cast<int32_t>(w & 0x7fffffffu), static_cast<int32_t>(w >> 31));
+                    } else {
+                        sl[j] = vd.lslots[s2];
+                    }
+                }
             }
             if constexpr (!MULTI) {
 #pragma unroll
@@ -610,6 +618,7 @@ struct nd_plan {
     int32_t* d_agg_hub_rows = nullptr;
     int32_t* d_lrp = nullptr;
     int2* d_lslots = nullptr;
+    int32_t* d_lnbr = nullptr;
     uint8_t* d_lseb = nullptr;
     int32_t n_agg_hub = 0;
     std::vector<size_t> hub_offset;
@@ -661,6 +670,8 @@ int rebuild_hubs(nd_plan* pl) {
     const int32_t H = pl->hub_degree, HB = pl->big_hub_degree;
     std::vector<int32_t> all, lrp;
     std::vector<int2> lslots;
+    std::vector<int32_t> lnbr;       // compact form of lslots when the parameters are cached
+    size_t nlow = 0;
     std::vector<uint8_t> lseb;
     std::vector<size_t> lrp_offset(pl->vdesc.size(), 0);
     pl->hub_offset.assign(pl->vdesc.size() + 1, 0);
@@ -677,7 +688,7 @@ int rebuild_hubs(nd_plan* pl) {
         std::vector<std::pair<int32_t, int32_t>> big, mid;   // (-length, q)
         const bool functional = vd.model != ND_VM_NOOP && vd.depth == 1;
         for (int32_t q = 0; q < vd.count; ++q) {
-            lrp.push_back(static_cast<int32_t>(lslots.size()));
+            lrp.push_back(static_cast<int32_t>(nlow));
             if (!functional) continue;
             const int32_t r = vd.row0 + q;
             const int32_t beg = pl->rowptr_host[r], len = pl->rowptr_host[r + 1] - beg;
@@ -688,13 +699,19 @@ int rebuild_hubs(nd_plan* pl) {
                 for (int32_t k = beg; k < beg + len; ++k) {
                     if (pl->kcache) {
                         const int32_t eq = pl->slot_eq_host[k];
-                        (eq & 1 ? kpos[eq >> 1].z : kpos[eq >> 1].w) = static_cast<int32_t>(lslots.size());
+                        (eq & 1 ? kpos[eq >> 1].z : kpos[eq >> 1].w) = static_cast<int32_t>(nlow);
                     }
-                    lslots.push_back(pl->slots_host[k]);
+                    if (pl->kcache) {
+                        const int2 sl = pl->slots_host[k];
+                        lnbr.push_back(static_cast<int32_t>(static_cast<uint32_t>(sl.x) | (static_cast<uint32_t>(sl.y & 1) << 31)));
+                    } else {
+                        lslots.push_back(pl->slots_host[k]);
+                    }
+                    ++nlow;
                     if (pl->multi) lseb.push_back(pl->slot_eb_host[k]);
                 }
         }
-        lrp.push_back(static_cast<int32_t>(lslots.size()));
+        lrp.push_back(static_cast<int32_t>(nlow));
         std::sort(big.begin(), big.end());
         std::sort(mid.begin(), mid.end());
         for (auto& x : big) all.push_back(x.second);
@@ -711,6 +728,8 @@ int rebuild_hubs(nd_plan* pl) {
     release(pl, pl->d_agg_hub_rows);
     release(pl, pl->d_lrp);
     release(pl, pl->d_lslots);
+    release(pl, pl->d_lnbr);
+    pl->d_lnbr = nullptr;
     release(pl, pl->d_lseb);
     pl->d_hubs_all = nullptr;
     pl->d_agg_hub_rows = nullptr;
@@ -721,6 +740,7 @@ int rebuild_hubs(nd_plan* pl) {
     rc = rc ? rc : upload(pl, rows, &pl->d_agg_hub_rows);
     rc = rc ? rc : upload(pl, lrp, &pl->d_lrp);
     rc = rc ? rc : upload(pl, lslots, &pl->d_lslots);
+    rc = rc ? rc : upload(pl, lnbr, &pl->d_lnbr);
     rc = rc ? rc : upload(pl, lseb, &pl->d_lseb);
     if (rc) return rc;
     if (pl->kcache) {
@@ -735,7 +755,7 @@ int rebuild_hubs(nd_plan* pl) {
         const size_t np = static_cast<size_t>(pl->knp);
         rc = upload(pl, std::vector<double>(static_cast<size_t>(pl->kcount) * np, nan), &pl->d_kc);
         rc = rc ? rc : upload(pl, std::vector<double>(pl->slot_eq_host.size() * np, nan), &pl->d_kfull);
-        rc = rc ? rc : upload(pl, std::vector<double>(std::max<size_t>(lslots.size(), 1) * np, nan), &pl->d_klow);
+        rc = rc ? rc : upload(pl, std::vector<double>(std::max<size_t>(nlow, 1) * np, nan), &pl->d_klow);
         rc = rc ? rc : upload(pl, kpos, &pl->d_kpos);
         if (rc) return rc;
     }
@@ -743,6 +763,7 @@ int rebuild_hubs(nd_plan* pl) {
         pl->vdesc[b].hubs = pl->d_hubs_all ? pl->d_hubs_all + pl->hub_offset[b] : nullptr;
         pl->vdesc[b].lrp = pl->d_lrp + lrp_offset[b];
         pl->vdesc[b].lslots = pl->d_lslots;
+        pl->vdesc[b].lnbr = pl->d_lnbr;
         pl->vdesc[b].lseb = pl->d_lseb;
         pl->vdesc[b].kfull = pl->d_kfull;
         pl->vdesc[b].klow = pl->d_klow;
