@@ -42,8 +42,8 @@ struct InstDesc {
 struct SolveArgs {
     const InstDesc* inst;
     const int32_t* order;
-    const uint16_t* slot_nbr;     // local neighbour | 0x8000 if this vertex is the edge's src
-    const uint16_t* slot_e;       // local edge index
+    const uint16_t* ends;         // per edge: local src, local dst (2 per edge, offset soff)
+    const uint16_t* slot_e;       // local edge index | 0x8000 if this vertex is the edge's src
     const uint16_t* cs;           // chunk starts (nchunk + 1 per instance)
     const uint16_t* cv;           // chunk vertex
     const uint16_t* cptr;         // vertex -> chunk range (n + 1 per instance)
@@ -76,12 +76,15 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 struct Lds {
     double* theta;
     double* epar;
+    double* fval;                 // dst-side output of each edge at the current stage
     double* partial;
     double* red;
-    uint16_t *nbr, *eidx, *cs, *cv, *cptr;
+    uint16_t *ends, *eidx, *cs, *cv, *cptr;
 };
 
-// out = f(ys) for the thread's vertices (two block barriers)
+// out = f(ys) for the thread's vertices (three block barriers): each edge's output is
+// evaluated once (the src side takes it negated, AntiSymmetric), then every chunk sums
+// its entries in list order
 template <int VM, int EM, int V>
 __device__ __forceinline__ void rhs(const InstDesc& I, const Lds& L, const double (&ys)[V][VModel<VM>::dim],
                                     const double (&pv)[V][VPar<VM>::np > 0 ? VPar<VM>::np : 1],
@@ -95,15 +98,16 @@ __device__ __forceinline__ void rhs(const InstDesc& I, const Lds& L, const doubl
         if (v < I.n) L.theta[v] = ys[k][0];
     }
     __syncthreads();
+    for (int e = t; e < I.ne; e += T)
+        L.fval[e] = g_dst<EM>(L.theta[L.ends[2 * e]], L.theta[L.ends[2 * e + 1]], L.epar + e * (NPE > 0 ? NPE : 1));
+    __syncthreads();
     for (int c = t; c < I.nchunk; c += T) {
-        const double own = L.theta[L.cv[c]];
         double acc = 0.0;
         const int s1 = L.cs[c + 1];
         for (int s = L.cs[c]; s < s1; ++s) {
-            const int nb = L.nbr[s];
-            const double* pe = L.epar + static_cast<int>(L.eidx[s]) * (NPE > 0 ? NPE : 1);
-            const double th = L.theta[nb & 0x7fff];
-            acc += (nb & 0x8000) ? -g_dst<EM>(own, th, pe) : g_dst<EM>(th, own, pe);
+            const int se = L.eidx[s];
+            const double f = L.fval[se & 0x7fff];
+            acc += (se & 0x8000) ? -f : f;
         }
         L.partial[c] = acc;
     }
@@ -146,15 +150,16 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
     Lds L;
     L.theta = smem;
     L.epar = L.theta + I.n;
-    L.partial = L.epar + I.ne * NPE;
+    L.fval = L.epar + I.ne * NPE;
+    L.partial = L.fval + I.ne;
     L.red = L.partial + I.nchunk;
-    L.nbr = reinterpret_cast<uint16_t*>(L.red + 32);
-    L.eidx = L.nbr + I.nslot;
+    L.ends = reinterpret_cast<uint16_t*>(L.red + 32);
+    L.eidx = L.ends + I.nslot;
     L.cs = L.eidx + I.nslot;
     L.cv = L.cs + I.nchunk + 1;
     L.cptr = L.cv + I.nchunk;
     for (int i = t; i < I.nslot; i += T) {
-        L.nbr[i] = a.slot_nbr[I.soff + i];
+        L.ends[i] = a.ends[I.soff + i];
         L.eidx[i] = a.slot_e[I.soff + i];
     }
     for (int i = t; i <= I.nchunk; i += T) L.cs[i] = a.cs[I.csoff + i];
@@ -384,7 +389,7 @@ struct nd_ensemble {
     std::vector<void*> allocs;
     InstDesc* d_inst = nullptr;
     int32_t* d_order = nullptr;
-    uint16_t *d_nbr = nullptr, *d_e = nullptr, *d_cs = nullptr, *d_cv = nullptr, *d_cptr = nullptr;
+    uint16_t *d_ends = nullptr, *d_e = nullptr, *d_cs = nullptr, *d_cv = nullptr, *d_cptr = nullptr;
     std::vector<int32_t> order;                     // instances grouped by class, largest first
     int32_t cls_first[ND_ENSEMBLE_CLASSES + 1] = {0, 0, 0, 0, 0};
     int64_t generation = 0;                         // bumped whenever the class layout changes
@@ -478,7 +483,7 @@ int make_args(nd_ensemble* en, double* u, const double* p, double t0, double t1,
     *a = SolveArgs{};
     a->inst = en->d_inst;
     a->order = en->d_order;
-    a->slot_nbr = en->d_nbr;
+    a->ends = en->d_ends;
     a->slot_e = en->d_e;
     a->cs = en->d_cs;
     a->cv = en->d_cv;
@@ -580,11 +585,14 @@ int nd_ensemble_create(int32_t ninst, const int64_t* nv, const int64_t* ne, cons
     nd_ensemble* en = new (std::nothrow) nd_ensemble;
     if (!en) return nd::set_error(ND_ERR_ALLOC, "allocation failed");
     (void)hipGetDevice(&en->device);
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, en->device) != hipSuccess || lds_max <= 0)
+        lds_max = 64 * 1024;
     en->ninst = ninst;
     en->vk = *vkind;
     en->ek = *ekind;
     en->dim = vkind->dim;
-    std::vector<uint16_t> nbr, eix, cs, cv, cptr;
+    std::vector<uint16_t> ends, eix, cs, cv, cptr;
     int64_t eoff = 0, uoff = 0, poff = 0;
     for (int32_t i = 0; i < ninst; ++i) {
         const int64_t n = nv[i], m = ne[i];
@@ -606,13 +614,13 @@ int nd_ensemble_create(int32_t ninst, const int64_t* nv, const int64_t* ne, cons
         }
         for (int64_t v = 0; v < n; ++v) deg[v + 1] += deg[v];
         std::vector<int32_t> fill(deg.begin(), deg.end() - 1);
-        std::vector<uint16_t> ln(static_cast<size_t>(2 * m)), le(static_cast<size_t>(2 * m));
+        std::vector<uint16_t> le(static_cast<size_t>(2 * m));
         for (int64_t e = 0; e < m; ++e) {
             const int64_t s = src[eoff + e] - 1, d = dst[eoff + e] - 1;
-            ln[fill[s]] = static_cast<uint16_t>(d | 0x8000);
-            le[fill[s]++] = static_cast<uint16_t>(e);
-            ln[fill[d]] = static_cast<uint16_t>(s);
+            le[fill[s]++] = static_cast<uint16_t>(e | 0x8000);
             le[fill[d]++] = static_cast<uint16_t>(e);
+            ends.push_back(static_cast<uint16_t>(s));
+            ends.push_back(static_cast<uint16_t>(d));
         }
         InstDesc I{};
         I.n = static_cast<int32_t>(n);
@@ -620,11 +628,11 @@ int nd_ensemble_create(int32_t ninst, const int64_t* nv, const int64_t* ne, cons
         I.nslot = static_cast<int32_t>(2 * m);
         I.uoff = uoff;
         I.poff = poff;
-        I.soff = static_cast<int64_t>(nbr.size());
+        I.soff = static_cast<int64_t>(eix.size());      // offset of both eix and ends (2 m entries each)
         I.csoff = static_cast<int64_t>(cs.size());
         I.cvoff = static_cast<int64_t>(cv.size());
         I.cpoff = static_cast<int64_t>(cptr.size());
-        nbr.insert(nbr.end(), ln.begin(), ln.end());
+
         eix.insert(eix.end(), le.begin(), le.end());
         int32_t nch = 0;
         for (int64_t v = 0; v < n; ++v) {
@@ -643,7 +651,12 @@ int nd_ensemble_create(int32_t ninst, const int64_t* nv, const int64_t* ne, cons
         }
         I.nchunk = nch;
         const int npe = epar(ekind->model);
-        I.lds_bytes = static_cast<int32_t>(8 * (n + m * npe + nch + 32) + 2 * (2 * 2 * m + 2 * nch + 1 + n + 1));
+        I.lds_bytes = static_cast<int32_t>(8 * (n + m * npe + m + nch + 32) + 2 * (2 * 2 * m + 2 * nch + 1 + n + 1));
+        if (I.lds_bytes > lds_max) {
+            delete en;
+            return nd::set_error(ND_ERR_UNSUPPORTED, "instance %d needs %d bytes of LDS (device limit %d)", i, I.lds_bytes,
+                                 lds_max);
+        }
         en->inst.push_back(I);
         eoff += m;
         uoff += n * vkind->dim;
@@ -653,7 +666,7 @@ int nd_ensemble_create(int32_t ninst, const int64_t* nv, const int64_t* ne, cons
     en->total_params = poff;
     int rc = regroup(en);
     rc = rc ? rc : up(en, en->inst, &en->d_inst);
-    rc = rc ? rc : up(en, nbr, &en->d_nbr);
+    rc = rc ? rc : up(en, ends, &en->d_ends);
     rc = rc ? rc : up(en, eix, &en->d_e);
     rc = rc ? rc : up(en, cs, &en->d_cs);
     rc = rc ? rc : up(en, cv, &en->d_cv);
