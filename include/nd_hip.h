@@ -236,8 +236,9 @@ int nd_aggregate(nd_plan* plan, double* aggbuf, const double* o, void* stream);
  * and parameters in the reference's flat layout for a one-batch network,
  * concatenated in instance order (offsets from nd_ensemble_sizes).  Vertex kinds:
  * KURAMOTO, SWING; edge kinds: AntiSymmetric KURAMOTO, STATIC_LINE; up to 2048
- * vertices per instance with the default classes. */
-#define ND_ENSEMBLE_CLASSES 4
+ * vertices per instance with the default classes (threads x vertices per thread:
+ * 128x2, 256x2, 512x2, 512x3, 512x4, 512x4). */
+#define ND_ENSEMBLE_CLASSES 6
 enum { ND_METHOD_TSIT5 = 0, ND_METHOD_RK4 = 1 };
 
 typedef struct {

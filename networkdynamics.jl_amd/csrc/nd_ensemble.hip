@@ -343,6 +343,7 @@ template <int VM, int EM, int METHOD>
 EnsKernel pick_v(int vper) {
     switch (vper) {
     case 1: return k_ensemble<VM, EM, METHOD, 1>;
+    case 3: return k_ensemble<VM, EM, METHOD, 3>;
     case 4: return k_ensemble<VM, EM, METHOD, 4>;
     default: return k_ensemble<VM, EM, METHOD, 2>;
     }
@@ -384,14 +385,14 @@ struct nd_ensemble {
     int32_t dim = 0;
     int64_t total_states = 0, total_params = 0;
     std::vector<InstDesc> inst;
-    int32_t cls_threads[ND_ENSEMBLE_CLASSES] = {128, 256, 512, 512};
-    int32_t cls_vper[ND_ENSEMBLE_CLASSES] = {2, 2, 2, 4};
+    int32_t cls_threads[ND_ENSEMBLE_CLASSES] = {128, 256, 512, 512, 512, 512};
+    int32_t cls_vper[ND_ENSEMBLE_CLASSES] = {2, 2, 2, 3, 4, 4};
     std::vector<void*> allocs;
     InstDesc* d_inst = nullptr;
     int32_t* d_order = nullptr;
     uint16_t *d_ends = nullptr, *d_e = nullptr, *d_cs = nullptr, *d_cv = nullptr, *d_cptr = nullptr;
     std::vector<int32_t> order;                     // instances grouped by class, largest first
-    int32_t cls_first[ND_ENSEMBLE_CLASSES + 1] = {0, 0, 0, 0, 0};
+    int32_t cls_first[ND_ENSEMBLE_CLASSES + 1] = {};
     int64_t generation = 0;                         // bumped whenever the class layout changes
     // one stream per class so the class launches overlap; forked from and joined back
     // to the caller's stream with events (default priority, non-blocking)
@@ -692,8 +693,8 @@ int nd_ensemble_sizes(const nd_ensemble* en, int64_t* total_states, int64_t* tot
 
 int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t vper) {
     if (!en || cls < 0 || cls >= ND_ENSEMBLE_CLASSES) return nd::set_error(ND_ERR_ARGUMENT, "bad class");
-    if (threads < kWave || threads > 512 || threads % kWave || !(vper == 1 || vper == 2 || vper == 4))
-        return nd::set_error(ND_ERR_ARGUMENT, "threads must be a multiple of 64 in [64,512], vper in {1,2,4}");
+    if (threads < kWave || threads > 512 || threads % kWave || vper < 1 || vper > 4)
+        return nd::set_error(ND_ERR_ARGUMENT, "threads must be a multiple of 64 in [64,512], vper in {1,2,3,4}");
     en->cls_threads[cls] = threads;
     en->cls_vper[cls] = vper;
     return regroup(en);

@@ -14,8 +14,8 @@ from . import _abi as A
 from . import components as comp
 
 TSIT5, RK4 = 0, 1
-NCLASS = 4
-DEFAULT_CLASSES = [(128, 2), (256, 2), (512, 2), (512, 4)]     # (threads, vertices per thread)
+NCLASS = 6
+DEFAULT_CLASSES = [(128, 2), (256, 2), (512, 2), (512, 3), (512, 4), (512, 4)]   # (threads, vertices per thread)
 
 # Tsitouras (2011) 5(4) pair: stage matrix, solution weights b (= last stage row,
 # first-same-as-last) and error weights btilde = b - bhat.

@@ -7,7 +7,7 @@ nv <= threads * vper that no earlier class holds.  The tuner
   2. for every bin and every launch shape that fits the bin's largest instance, runs
      a short solve of a sample of that bin (replicated to fill the GPU) and measures
      the time per instance and right-hand-side evaluation;
-  3. picks the list of at most four shapes (increasing capacity) that minimises the
+  3. picks the list of at most six shapes (increasing capacity) that minimises the
      predicted total time, each bin going to the first shape that holds it.
 Results are cached in a JSON file keyed by device, models, method and bins.
 """
@@ -19,7 +19,7 @@ import numpy as np
 
 from .ensemble import NCLASS, TSIT5, Ensemble
 
-SHAPES = [(t, v) for t in (64, 128, 192, 256, 320, 384, 448, 512) for v in (1, 2, 4)]
+SHAPES = [(t, v) for t in (64, 128, 192, 256, 320, 384, 448, 512) for v in (1, 2, 3, 4)]
 
 
 def _bins(nv, nbins):
@@ -29,26 +29,51 @@ def _bins(nv, nbins):
 
 
 def choose(bin_max, bin_weight, cost):
-    """Best list of <= NCLASS shapes.  cost[b][shape] = seconds per instance-evaluation
-    (missing = does not fit).  Returns (shapes, predicted cost)."""
+    """Best list of <= NCLASS shapes with increasing capacity.  bin_max: increasing bin
+    upper sizes; cost[b][shape] = seconds per instance-evaluation (missing = does not fit).
+    A bin goes to the first listed shape whose capacity holds it, so a list splits the
+    bins into consecutive groups; dynamic programming over (shapes used, last shape).
+    Returns (shapes, predicted cost)."""
     shapes = sorted({s for c in cost for s in c}, key=lambda s: (s[0] * s[1], s))
-    best, best_cost = None, float("inf")
-    for k in range(1, NCLASS + 1):
-        for combo in itertools.combinations(shapes, k):
-            caps = [t * v for t, v in combo]
-            if any(caps[i] >= caps[i + 1] for i in range(k - 1)) or caps[-1] < bin_max[-1]:
-                continue
-            tot = 0.0
-            for b, m in enumerate(bin_max):
-                s = next(x for x, cp in zip(combo, caps) if cp >= m)
-                c = cost[b].get(s)
+    caps = [t * v for t, v in shapes]
+    nb = len(bin_max)
+
+    def group_cost(lo_cap, j):
+        tot = 0.0
+        for b in range(nb):
+            if lo_cap < bin_max[b] <= caps[j]:
+                c = cost[b].get(shapes[j])
                 if c is None:
-                    tot = float("inf")
-                    break
+                    return float("inf")
                 tot += bin_weight[b] * c
-            if tot < best_cost:
-                best, best_cost = list(combo), tot
-    return best, best_cost
+        return tot
+
+    inf = float("inf")
+    f = [[(inf, None)] * len(shapes) for _ in range(NCLASS + 1)]
+    for j in range(len(shapes)):
+        f[1][j] = (group_cost(-1, j), None)
+    for k in range(2, NCLASS + 1):
+        for j in range(len(shapes)):
+            best = (inf, None)
+            for i in range(len(shapes)):
+                if caps[i] < caps[j] and f[k - 1][i][0] < inf:
+                    c = f[k - 1][i][0] + group_cost(caps[i], j)
+                    if c < best[0]:
+                        best = (c, i)
+            f[k][j] = best
+    best = (inf, None, None)
+    for k in range(1, NCLASS + 1):
+        for j in range(len(shapes)):
+            if caps[j] >= bin_max[-1] and f[k][j][0] < best[0]:
+                best = (f[k][j][0], k, j)
+    if best[1] is None:
+        return None, inf
+    out, k, j = [], best[1], best[2]
+    while j is not None:
+        out.append(shapes[j])
+        j = f[k][j][1]
+        k -= 1
+    return out[::-1], best[0]
 
 
 def _signature(ens, method, reltol, qs):

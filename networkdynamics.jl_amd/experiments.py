@@ -85,7 +85,7 @@ def c4_instances(ninst, seed=1, nmin=100, nmax=2000):
     return out, rng
 
 
-def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_seconds=30.0, seed=1):
+def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_seconds=30.0, seed=1, tune=0):
     import torch
     t0 = time.perf_counter()
     inst, rng = c4_instances(ninst, seed)
@@ -122,6 +122,12 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
     ens = Ensemble(graphs, Lib.kuramoto_second(), Lib.kuramoto_edge())
     u_d = torch.from_numpy(np.concatenate(us)).to(dev)
     p_d = torch.from_numpy(np.concatenate(ps)).to(dev)
+    tuning = None
+    if tune:
+        from .ensemble_tune import autotune
+        tt = time.perf_counter()
+        tuning = autotune(ens, u_d, p_d, probe_t=0.5, reltol=reltol, abstol=abstol)
+        _log(f"autotune: {tuning['classes']} in {time.perf_counter() - tt:.1f} s")
     torch.cuda.synchronize()
     ts = time.perf_counter()
     est = Ensemble.stats_to_numpy(ens.solve(u_d, p_d, (0.0, t1), method=TSIT5, reltol=reltol, abstol=abstol))
@@ -158,7 +164,8 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
                 ensemble_retcodes=np.bincount(est["retcode"], minlength=4).tolist(),
                 returned_to_sync=int(sync.sum()), baseline_instances=done, baseline_s=t_base,
                 baseline_traj_per_s=traj_base, baseline_steps_mean=float(np.mean(steps)),
-                speedup=traj_ens / traj_base)
+                speedup=traj_ens / traj_base, classes=[list(c) for c in ens.classes],
+                tuning=None if tuning is None else dict(predicted_s=tuning["predicted_s"], bins=tuning["bins"]))
 
 
 def _grid_job(args):

@@ -123,6 +123,27 @@ def test_autotune_choice_logic(ND):
     # a single class must hold the largest bin
     shapes, _ = choose([2000], [1.0], [{(512, 4): 1.0}])
     assert shapes == [(512, 4)]
+    # the dynamic programme agrees with brute force over every admissible list
+    import itertools
+    from networkdynamics_jl_amd.ensemble import NCLASS
+    rng = np.random.default_rng(3)
+    pool = [(64, 2), (128, 2), (256, 1), (128, 4), (256, 2), (512, 1), (256, 4), (512, 2), (512, 3), (512, 4)]
+    for trial in range(20):
+        bins = sorted(rng.choice(np.arange(50, 2049), size=4, replace=False).tolist())
+        weight = rng.uniform(0.5, 3, 4).tolist()
+        cost = [{sh: float(rng.uniform(0.5, 2)) for sh in pool if sh[0] * sh[1] >= m} for m in bins]
+        shapes = sorted(pool, key=lambda x: (x[0] * x[1], x))
+        best = float("inf")
+        for k in range(1, NCLASS + 1):
+            for combo in itertools.combinations(shapes, k):
+                caps = [t * v for t, v in combo]
+                if any(caps[i] >= caps[i + 1] for i in range(k - 1)) or caps[-1] < bins[-1]:
+                    continue
+                tot = sum(w * cost[b].get(next(x for x, cp in zip(combo, caps) if cp >= m), float("inf"))
+                          for b, (m, w) in enumerate(zip(bins, weight)))
+                best = min(best, tot)
+        got, pred = choose(bins, weight, cost)
+        assert abs(pred - best) <= 1e-12 * best
 
 
 @pytest.mark.gpu
