@@ -309,3 +309,25 @@ def test_static_line_parameter_cache(ND, dev):
     p2[lo + 1:hi:2] = 0.05                      # phase shifts only
     _check(nw, u0, p2, dev)
     _check(nw, u0, p, dev)
+
+
+def test_edge_cases_no_edges_isolated_star(ND, dev):
+    # no edges at all; isolated vertices next to a component; a star whose centre is a
+    # big hub (block-per-vertex path) with cached parameters
+    import torch
+    rng = np.random.default_rng(12)
+    vm, em = ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge()
+    for g in (ND.SimpleGraph(5), ND.SimpleGraph(1),
+              ND.SimpleGraph(40, [1, 2, 3, 10], [2, 3, 4, 11]),
+              ND.SimpleGraph(3001, np.ones(3000, np.int64), np.arange(2, 3002))):
+        nw = ND.Network(g, vm, em)
+        p = np.r_[np.stack([rng.uniform(1, 5, g.nv()), rng.uniform(0.1, 1, g.nv()), rng.normal(size=g.nv())],
+                           1).ravel(), rng.uniform(5, 10, g.ne())]
+        u = rng.normal(size=nw.dim)
+        _check(nw, u, p, dev)
+
+
+def test_heterogeneous_rgg_c3_full_size(ND, dev):
+    # the heterogeneous network at its benchmark size (N = 2e5, two vertex batches)
+    nw, p, u0 = ND.workloads.heterogeneous_rgg(200_000, seed=1)
+    _check(nw, u0, p, dev)

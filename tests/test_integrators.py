@@ -199,3 +199,39 @@ def test_ensemble_autotune_and_cache(ND, dev, tmp_path):
         ui = torch.from_numpy(us[i]).to(dev)
         ND.tsit5_per_instance(nw, ui, torch.from_numpy(ps[i]).to(dev), (0.0, 1.0), reltol=1e-8, abstol=1e-8)
         np.testing.assert_allclose(u[ens.state_slice(i)].cpu().numpy(), ui.cpu().numpy(), rtol=0, atol=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["swing_static_line", "kuramoto_first"])
+def test_ensemble_rk4_other_models_match_oracle(ND, dev, kind):
+    # the per-edge stage values are shared by both endpoints; check two more model pairs,
+    # including the two-parameter static line (B, phi), against the oracle's RK4
+    import torch
+    gs, rng = _instances(ND, 10, 30, 1500, seed=7)
+    if kind == "swing_static_line":
+        vm, em = ND.Lib.kuramoto_second(), ND.Lib.static_line_edge()
+    else:
+        vm, em = ND.Lib.kuramoto_first(), ND.Lib.kuramoto_edge()
+    us, ps = [], []
+    for g in gs:
+        n, m = g.nv(), g.ne()
+        if kind == "swing_static_line":
+            pv = np.stack([rng.uniform(1, 5, n), rng.uniform(0.1, 1, n), rng.normal(size=n)], 1).ravel()
+            pe = np.stack([rng.uniform(1, 20, m), rng.uniform(0, 0.1, m)], 1).ravel()
+            u = np.empty(2 * n)
+            u[0::2], u[1::2] = rng.uniform(-1, 1, n), rng.normal(scale=0.5, size=n)
+        else:
+            pv, pe = rng.normal(size=n), np.full(m, 3.0)
+            u = rng.uniform(-np.pi, np.pi, n)
+        us.append(u)
+        ps.append(np.r_[pv, pe])
+    ens = ND.Ensemble(gs, vm, em)
+    u_d = torch.from_numpy(np.concatenate(us)).to(dev)
+    p_d = torch.from_numpy(np.concatenate(ps)).to(dev)
+    st = ND.Ensemble.stats_to_numpy(ens.solve(u_d, p_d, (0.0, 0.2), method=ND.ensemble.RK4, dt=0.002))
+    torch.cuda.synchronize()
+    assert (st["retcode"] == 0).all() and (st["naccept"] == 100).all()
+    got = u_d.cpu().numpy()
+    for i, g in enumerate(gs):
+        ref = oracle_for(ND.Network(g, vm, em)).rk4(us[i], ps[i], 0.0, 0.002, 100)
+        np.testing.assert_allclose(got[ens.state_slice(i)], ref, rtol=0, atol=1e-11, err_msg=f"instance {i}")
