@@ -217,6 +217,15 @@ int nd_rk4(nd_plan* plan, double* u, const double* p, double t0, double dt, int6
 int nd_get_buffers(nd_plan* plan, double* o, double* aggbuf, const double* u,
                    const double* p, double t, void* stream);
 
+/* nw(du, u + eps*v, p, t) on first-order dual numbers: du = f(u) and jv = J(u) v, the
+ * Jacobian-vector product ForwardDiff forms when linearize_network
+ * (src/linear_analysis.jl:74-85) or an implicit solver calls nw with Dual inputs through
+ * the DiffCache buffers (src/network_structure.jl:152-155, src/gbufs.jl:20-24).  Every
+ * aggregation segment is summed in the reference order.  v, jv: device vectors of
+ * length dim. */
+int nd_rhs_jvp(nd_plan* plan, double* du, double* jv, const double* u, const double* v, const double* p, double t,
+               int64_t len_du, int64_t len_u, int64_t len_p, void* stream);
+
 /* gather!(EagerGBufProvider, gbuf, o): gbuf[k] = o[map[k]], gbufs.jl:25. */
 int nd_gather(nd_plan* plan, double* gbuf, const double* o, void* stream);
 

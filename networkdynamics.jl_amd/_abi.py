@@ -82,6 +82,7 @@ PROTOTYPES = {
     "nd_rhs": (C.c_int, [P, P, P, P, D, I64, I64, I64, P]),
     "nd_get_buffers": (C.c_int, [P, P, P, P, P, D, P]),
     "nd_rk4": (C.c_int, [P, P, P, D, D, I64, P, I64, I64, P]),
+    "nd_rhs_jvp": (C.c_int, [P, P, P, P, P, P, D, I64, I64, I64, P]),
     "nd_gather": (C.c_int, [P, P, P, P]),
     "nd_aggregate": (C.c_int, [P, P, P, P]),
     "nd_ensemble_create": (C.c_int, [I32, P, P, P, P, P, P, C.POINTER(P)]),

@@ -8,29 +8,57 @@
 
 namespace nd {
 
+using ::sin;   // the double overloads stay visible next to sin(Dual)
+
+// First-order dual number v + d*eps (one tangent direction), the forward-mode scalar
+// the Jacobian-vector product evaluates the models with.  Parameters stay plain doubles.
+struct Dual {
+    double v, d;
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator+(double c, Dual a) { return {c + a.v, a.d}; }
+__device__ __forceinline__ Dual operator+(Dual a, double c) { return {a.v + c, a.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, double c) { return {a.v - c, a.d}; }
+__device__ __forceinline__ Dual operator-(double c, Dual a) { return {c - a.v, -a.d}; }
+__device__ __forceinline__ Dual operator*(double c, Dual a) { return {c * a.v, c * a.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, double c) { return {a.v * c, a.d * c}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual sin(Dual a) { return {::sin(a.v), ::cos(a.v) * a.d}; }
+__device__ __forceinline__ Dual dual_nan() { return {__builtin_nan(""), __builtin_nan("")}; }
+
+template <typename S> __device__ __forceinline__ S nan_of();
+template <> __device__ __forceinline__ double nan_of<double>() { return __builtin_nan(""); }
+template <> __device__ __forceinline__ Dual nan_of<Dual>() { return dual_nan(); }
+template <typename S> __device__ __forceinline__ S zero_of();
+template <> __device__ __forceinline__ double zero_of<double>() { return 0.0; }
+template <> __device__ __forceinline__ Dual zero_of<Dual>() { return {0.0, 0.0}; }
+
 __host__ __device__ constexpr bool is_ode_model(int em) {
     return em == ND_EM_ODE_SIN_DIFFUSION || em == ND_EM_ODE_SIN_DIFFUSION_FAST || em == ND_EM_ODE_LINEAR;
 }
 
-// dst-side output of a pure feed-forward edge model g(v_s, v_d, p)
-template <int EM>
-__device__ __forceinline__ double g_dst(double vs, double vd, const double* __restrict__ pe) {
+// dst-side output of a pure feed-forward edge model g(v_s, v_d, p); S = double or Dual
+template <int EM, typename S = double>
+__device__ __forceinline__ S g_dst(S vs, S vd, const double* __restrict__ pe) {
     if constexpr (EM == ND_EM_KURAMOTO) return pe[0] * sin(vs - vd);             // ComponentLibrary.jl:51-53
     else if constexpr (EM == ND_EM_STATIC_LINE) return pe[0] * sin(vs - vd - pe[1]);
     else if constexpr (EM == ND_EM_DIFFUSION) return vs - vd;                     // diffusion_test.jl:57-60
     else if constexpr (EM == ND_EM_DIFFUSION_WEIGHTED) return pe[0] * (vs - vd);  // ComponentLibrary.jl:8-10
     else if constexpr (EM == ND_EM_DIFFUSION_FIDUCIAL) return pe[0] * (vs - vd);  // ComponentLibrary.jl:22-25
-    else return __builtin_nan("");
+    else return nan_of<S>();
 }
 
-__device__ __forceinline__ double g_dst_rt(int em, double vs, double vd, const double* __restrict__ pe) {
+template <typename S = double>
+__device__ __forceinline__ S g_dst_rt(int em, S vs, S vd, const double* __restrict__ pe) {
     switch (em) {
-    case ND_EM_KURAMOTO: return g_dst<ND_EM_KURAMOTO>(vs, vd, pe);
-    case ND_EM_STATIC_LINE: return g_dst<ND_EM_STATIC_LINE>(vs, vd, pe);
-    case ND_EM_DIFFUSION: return g_dst<ND_EM_DIFFUSION>(vs, vd, pe);
-    case ND_EM_DIFFUSION_WEIGHTED: return g_dst<ND_EM_DIFFUSION_WEIGHTED>(vs, vd, pe);
-    case ND_EM_DIFFUSION_FIDUCIAL: return g_dst<ND_EM_DIFFUSION_FIDUCIAL>(vs, vd, pe);
-    default: return __builtin_nan("");
+    case ND_EM_KURAMOTO: return g_dst<ND_EM_KURAMOTO, S>(vs, vd, pe);
+    case ND_EM_STATIC_LINE: return g_dst<ND_EM_STATIC_LINE, S>(vs, vd, pe);
+    case ND_EM_DIFFUSION: return g_dst<ND_EM_DIFFUSION, S>(vs, vd, pe);
+    case ND_EM_DIFFUSION_WEIGHTED: return g_dst<ND_EM_DIFFUSION_WEIGHTED, S>(vs, vd, pe);
+    case ND_EM_DIFFUSION_FIDUCIAL: return g_dst<ND_EM_DIFFUSION_FIDUCIAL, S>(vs, vd, pe);
+    default: return nan_of<S>();
     }
 }
 
@@ -43,27 +71,27 @@ template <> struct VModel<ND_VM_SWING_UNIT_DAMPING> { static constexpr int dim =
 template <> struct VModel<ND_VM_LOAD> { static constexpr int dim = 1; };
 
 // vertex f (apply_compf, coreloop.jl:213-216); operation order as the reference writes it
-template <int VM>
-__device__ __forceinline__ void vertex_f(double* __restrict__ dx, const double* __restrict__ x, double acc,
+template <int VM, typename S = double>
+__device__ __forceinline__ void vertex_f(S* __restrict__ dx, const S* __restrict__ x, S acc,
                                          const double* __restrict__ pv, int dim) {
     if constexpr (VM == ND_VM_DIFFUSION) {                 // ComponentLibrary.jl:42-45
         dx[0] = acc;
     } else if constexpr (VM == ND_VM_KURAMOTO) {           // ComponentLibrary.jl:69-71
         dx[0] = pv[0] + acc;
     } else if constexpr (VM == ND_VM_SWING) {              // ComponentLibrary.jl:59-63
-        const double w = x[1];
+        const S w = x[1];
         dx[0] = w;
         dx[1] = 1.0 / pv[0] * (pv[2] - pv[1] * w + acc);
     } else if constexpr (VM == ND_VM_SWING_UNIT_DAMPING) { // benchmark_models.jl:37-41
-        const double w = x[1];
+        const S w = x[1];
         dx[0] = w;
-        double d = pv[0] - 1.0 * w;
-        d += acc;
+        S d = pv[0] - 1.0 * w;
+        d = d + acc;
         dx[1] = d;
     } else if constexpr (VM == ND_VM_LOAD) {
         dx[0] = 1.0 / pv[0] * (pv[1] + acc);
     } else {                                               // f does nothing: du keeps its zero fill
-        for (int j = 0; j < dim; ++j) dx[j] = 0.0;
+        for (int j = 0; j < dim; ++j) dx[j] = zero_of<S>();
     }
 }
 

@@ -397,6 +397,104 @@ __global__ void __launch_bounds__(256) k_edge_ode(EDesc e, const double* __restr
     emit1<STAGE>(du, sa, x0 + 1, dx[1]);
 }
 
+// ---- Jacobian-vector product: the RHS evaluated on dual numbers u + eps*v -------------
+// du = f(u) and jv = J(u) v in one pass; the value part is the plain RHS with every
+// segment summed in the reference order (no hub trees), the tangent part follows the
+// forward-mode rules of each operation.  One thread per vertex.
+template <int EM, int WRAP>
+__device__ __forceinline__ Dual slot_value_dual(int2 s, Dual own, const double* __restrict__ u,
+                                                const double* __restrict__ v, const double* __restrict__ vo,
+                                                const double* __restrict__ dvo, const double* __restrict__ p) {
+    if constexpr (EM == ND_EM_NOOP) {
+        return dual_nan();
+    } else if constexpr (is_ode_model(EM)) {
+        return Dual{u[s.x], v[s.x]};
+    } else {
+        const Dual nbr{vo[s.x], dvo[s.x]};
+        const double* pe = p + (s.y >> 1);
+        if (s.y & 1) {
+            const Dual y = g_dst<EM, Dual>(own, nbr, pe);
+            return WRAP == ND_WRAP_SYMMETRIC ? y : -y;
+        }
+        return g_dst<EM, Dual>(nbr, own, pe);
+    }
+}
+
+__device__ __forceinline__ Dual slot_value_dual_rt(const EDesc& e, int2 s, Dual own, const double* __restrict__ u,
+                                                   const double* __restrict__ v, const double* __restrict__ vo,
+                                                   const double* __restrict__ dvo, const double* __restrict__ p) {
+    if (e.model == ND_EM_NOOP) return dual_nan();
+    if (e.ode) return Dual{u[s.x], v[s.x]};
+    const Dual nbr{vo[s.x], dvo[s.x]};
+    const double* pe = p + (s.y >> 1);
+    if (s.y & 1) {
+        const Dual y = g_dst_rt<Dual>(e.model, own, nbr, pe);
+        return e.wrap == ND_WRAP_SYMMETRIC ? y : -y;
+    }
+    return g_dst_rt<Dual>(e.model, nbr, own, pe);
+}
+
+template <int VM, int EM, int WRAP, bool MULTI>
+__global__ void __launch_bounds__(256) k_vertex_jvp(VDesc vd, const int32_t* __restrict__ rowptr,
+                                                    const int2* __restrict__ slots, const uint8_t* __restrict__ slot_eb,
+                                                    const EDesc* __restrict__ ed, const double* __restrict__ u,
+                                                    const double* __restrict__ v, const double* __restrict__ vo,
+                                                    const double* __restrict__ dvo, const double* __restrict__ p,
+                                                    double* __restrict__ du, double* __restrict__ jv) {
+    constexpr int DIMC = VModel<VM>::dim;
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= vd.count) return;
+    const int dim = DIMC > 0 ? DIMC : vd.dim;
+    const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * dim;
+    if constexpr (VM == ND_VM_NOOP) {
+        for (int j = 0; j < dim; ++j) { du[x0 + j] = 0.0; jv[x0 + j] = 0.0; }
+    } else {
+        Dual x[DIMC];
+#pragma unroll
+        for (int j = 0; j < DIMC; ++j) x[j] = Dual{u[x0 + j], v[x0 + j]};
+        const Dual own = x[vd.mask[0]];
+        Dual acc{0.0, 0.0};
+        const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
+        for (int32_t s = beg; s < end; ++s) {
+            if constexpr (MULTI) acc = acc + slot_value_dual_rt(ed[slot_eb[s]], slots[s], own, u, v, vo, dvo, p);
+            else acc = acc + slot_value_dual<EM, WRAP>(slots[s], own, u, v, vo, dvo, p);
+        }
+        Dual dx[DIMC];
+        vertex_f<VM, Dual>(dx, x, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, DIMC);
+#pragma unroll
+        for (int j = 0; j < DIMC; ++j) { du[x0 + j] = dx[j].v; jv[x0 + j] = dx[j].d; }
+    }
+}
+
+// du, jv of ODE edges (the dual form of k_edge_ode)
+template <int EM>
+__global__ void __launch_bounds__(256) k_edge_ode_jvp(EDesc e, const double* __restrict__ u, const double* __restrict__ v,
+                                                      const double* __restrict__ p, double* __restrict__ du,
+                                                      double* __restrict__ jv) {
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= e.count) return;
+    const int2 en = e.ends[q];
+    const Dual vs{u[en.x], v[en.x]}, vdn{u[en.y], v[en.y]};
+    const int64_t x0 = static_cast<int64_t>(e.state0) + static_cast<int64_t>(q) * 2;
+    const Dual x[2] = {Dual{u[x0], v[x0]}, Dual{u[x0 + 1], v[x0 + 1]}};
+    Dual dx[2];
+    if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION) {            // ComponentLibrary.jl:30-34
+        const double* pe = p + e.param0 + static_cast<int64_t>(q) * e.pdim;
+        dx[0] = 1.0 / pe[0] * (sin(vs - vdn) - x[0]);
+        dx[1] = 1.0 / pe[0] * (sin(vdn - vs) - x[1]);
+    } else if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION_FAST) { // benchmark_models.jl:11-15
+        dx[0] = 100.0 * (sin(vs - vdn) - x[0]);
+        dx[1] = 100.0 * (sin(vdn - vs) - x[1]);
+    } else {                                                   // diffusion_test.jl:102-106
+        dx[0] = vs - vdn - x[0];
+        dx[1] = vdn - vs - x[1];
+    }
+    du[x0] = dx[0].v;
+    du[x0 + 1] = dx[1].v;
+    jv[x0] = dx[0].d;
+    jv[x0 + 1] = dx[1].d;
+}
+
 __global__ void k_fill(double* __restrict__ x, int64_t n, double v) {
     for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x)
@@ -551,6 +649,57 @@ VertexKernel pick_vertex_kernel(int vm, int em, int wrap, bool multi, bool kc) {
     }
 }
 
+using JvpKernel = void (*)(VDesc, const int32_t*, const int2*, const uint8_t*, const EDesc*, const double*,
+                           const double*, const double*, const double*, const double*, double*, double*);
+
+template <int VM>
+JvpKernel pick_jvp_edge(int em, int wrap, bool multi) {
+    if (multi) return k_vertex_jvp<VM, ND_EM_NOOP, ND_WRAP_NONE, true>;
+    const bool sym = wrap == ND_WRAP_SYMMETRIC;
+    switch (em) {
+    case ND_EM_KURAMOTO:
+        return sym ? k_vertex_jvp<VM, ND_EM_KURAMOTO, ND_WRAP_SYMMETRIC, false>
+                   : k_vertex_jvp<VM, ND_EM_KURAMOTO, ND_WRAP_ANTISYMMETRIC, false>;
+    case ND_EM_STATIC_LINE:
+        return sym ? k_vertex_jvp<VM, ND_EM_STATIC_LINE, ND_WRAP_SYMMETRIC, false>
+                   : k_vertex_jvp<VM, ND_EM_STATIC_LINE, ND_WRAP_ANTISYMMETRIC, false>;
+    case ND_EM_DIFFUSION:
+        return sym ? k_vertex_jvp<VM, ND_EM_DIFFUSION, ND_WRAP_SYMMETRIC, false>
+                   : k_vertex_jvp<VM, ND_EM_DIFFUSION, ND_WRAP_ANTISYMMETRIC, false>;
+    case ND_EM_DIFFUSION_WEIGHTED:
+        return sym ? k_vertex_jvp<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_SYMMETRIC, false>
+                   : k_vertex_jvp<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_ANTISYMMETRIC, false>;
+    case ND_EM_DIFFUSION_FIDUCIAL:
+        return k_vertex_jvp<VM, ND_EM_DIFFUSION_FIDUCIAL, ND_WRAP_ANTISYMMETRIC, false>;
+    case ND_EM_ODE_SIN_DIFFUSION:
+    case ND_EM_ODE_SIN_DIFFUSION_FAST:
+    case ND_EM_ODE_LINEAR:
+        return k_vertex_jvp<VM, ND_EM_ODE_LINEAR, ND_WRAP_FIDUCIAL, false>;
+    default:
+        return k_vertex_jvp<VM, ND_EM_NOOP, ND_WRAP_NONE, false>;
+    }
+}
+
+JvpKernel pick_jvp_kernel(int vm, int em, int wrap, bool multi) {
+    switch (vm) {
+    case ND_VM_DIFFUSION: return pick_jvp_edge<ND_VM_DIFFUSION>(em, wrap, multi);
+    case ND_VM_KURAMOTO: return pick_jvp_edge<ND_VM_KURAMOTO>(em, wrap, multi);
+    case ND_VM_SWING: return pick_jvp_edge<ND_VM_SWING>(em, wrap, multi);
+    case ND_VM_SWING_UNIT_DAMPING: return pick_jvp_edge<ND_VM_SWING_UNIT_DAMPING>(em, wrap, multi);
+    case ND_VM_LOAD: return pick_jvp_edge<ND_VM_LOAD>(em, wrap, multi);
+    default: return k_vertex_jvp<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false>;
+    }
+}
+
+using EdgeOdeJvpKernel = void (*)(EDesc, const double*, const double*, const double*, double*, double*);
+EdgeOdeJvpKernel pick_edge_ode_jvp(int em) {
+    switch (em) {
+    case ND_EM_ODE_SIN_DIFFUSION: return k_edge_ode_jvp<ND_EM_ODE_SIN_DIFFUSION>;
+    case ND_EM_ODE_SIN_DIFFUSION_FAST: return k_edge_ode_jvp<ND_EM_ODE_SIN_DIFFUSION_FAST>;
+    default: return k_edge_ode_jvp<ND_EM_ODE_LINEAR>;
+    }
+}
+
 using EdgeOdeKernel = void (*)(EDesc, const double*, const double*, double*, StageArgs);
 template <bool STAGE>
 EdgeOdeKernel pick_edge_ode(int em) {
@@ -602,8 +751,11 @@ struct nd_plan {
     std::vector<VDesc> vdesc;
     std::vector<EDesc> edesc;
     std::vector<VertexKernel> vkern, vkern_stage;
+    std::vector<JvpKernel> vkern_jvp;
+    double* d_dvout = nullptr;     // compact tangent outputs (Jacobian-vector products)
     double* rk_work = nullptr;
     double* d_vout = nullptr;      // compact vertex outputs (vdepth 1 networks with functional edges)
+    size_t vout_len = 0;
     std::vector<int32_t> rowptr_host;
     std::vector<int2> slots_host;
     std::vector<uint8_t> slot_eb_host;
@@ -1011,6 +1163,7 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
                                                       pl->kcache));
         pl->vkern_stage.push_back(pick_vertex_kernel<true>(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi,
                                                            pl->kcache));
+        pl->vkern_jvp.push_back(pick_jvp_kernel(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi));
     }
     rc = rebuild_hubs(pl);
     if (rc) { delete pl; return rc; }
@@ -1020,6 +1173,7 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         if (e != hipSuccess) { delete pl; return nd::set_error(ND_ERR_ALLOC, "hipMalloc(vertex outputs): %s", hipGetErrorString(e)); }
         pl->allocations.push_back(w);
         pl->d_vout = static_cast<double*>(w);
+        pl->vout_len = static_cast<size_t>(n.nv);
     }
     *out = pl;
     return ND_OK;
@@ -1067,6 +1221,47 @@ int nd_rhs(nd_plan* pl, double* du, const double* u, const double* p, double t, 
     if (rc) return rc;
     launch_rhs(pl, false, du, u, p, StageArgs{}, static_cast<hipStream_t>(stream));
     return hip_check(hipGetLastError(), "nd_rhs launch");
+}
+
+int nd_rhs_jvp(nd_plan* pl, double* du, double* jv, const double* u, const double* v, const double* p, double t,
+               int64_t len_du, int64_t len_u, int64_t len_p, void* stream) {
+    (void)t;
+    if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
+    if (!(len_du == len_u && len_u == pl->dim))   // coreloop.jl:2-4
+        return nd::set_error(ND_ERR_ARGUMENT, "du or u does not have expected size %lld", (long long)pl->dim);
+    if (pl->pdim > 0 && len_p != pl->pdim)        // coreloop.jl:5-7
+        return nd::set_error(ND_ERR_ARGUMENT, "p does not has expecte size %lld", (long long)pl->pdim);
+    if (pl->dim > 0 && (!du || !u || !jv || !v)) return nd::set_error(ND_ERR_ARGUMENT, "NULL state pointer");
+    if (pl->pdim > 0 && !p) return nd::set_error(ND_ERR_ARGUMENT, "NULL parameter pointer");
+    int rc = check_device(pl);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (pl->d_vout && !pl->d_dvout) {
+        void* w = nullptr;
+        hipError_t e = hipMalloc(&w, pl->vout_len * sizeof(double));
+        if (e != hipSuccess) return nd::set_error(ND_ERR_ALLOC, "hipMalloc(tangent outputs): %s", hipGetErrorString(e));
+        pl->allocations.push_back(w);
+        pl->d_dvout = static_cast<double*>(w);
+    }
+    if (pl->d_vout)
+        for (const VDesc& vd : pl->vdesc) {
+            if (!vd.count) continue;
+            const unsigned vb = blocks_for(vd.count, 256);
+            hipLaunchKernelGGL(k_vertex_out, dim3(vb), dim3(256), 0, s, vd, u, pl->d_vout, no_sync(vb));
+            hipLaunchKernelGGL(k_vertex_out, dim3(vb), dim3(256), 0, s, vd, v, pl->d_dvout, no_sync(vb));
+        }
+    for (size_t b = 0; b < pl->vdesc.size(); ++b) {
+        const VDesc& vd = pl->vdesc[b];
+        if (vd.count == 0) continue;
+        hipLaunchKernelGGL(pl->vkern_jvp[b], dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, pl->d_rowptr,
+                           pl->d_slots, pl->d_slot_eb, pl->d_edesc, u, v, pl->d_vout, pl->d_dvout, p, du, jv);
+    }
+    for (const EDesc& e : pl->edesc) {
+        if (!e.ode || e.count == 0) continue;
+        hipLaunchKernelGGL(pick_edge_ode_jvp(e.model), dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, u, v, p,
+                           du, jv);
+    }
+    return hip_check(hipGetLastError(), "nd_rhs_jvp launch");
 }
 
 int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_t nsteps, double* work,

@@ -258,6 +258,39 @@ class Network:
         A.check(A.lib().nd_rhs(self.plan, dptr, uptr, pptr, float(t), ndu, nu, np_, stream))
         return None
 
+    def jvp(self, du, jv, u, v, p, t=0.0):
+        """du = f(u) and jv = J(u) v: nw evaluated on dual numbers u + eps*v, the
+        Jacobian-vector product ForwardDiff forms in linearize_network
+        (linear_analysis.jl:74-85).  Stream-ordered on torch's current stream."""
+        import torch
+        dptr, ndu = _tensor_ptr(du, self.dim, "du", torch)
+        jptr, _ = _tensor_ptr(jv, self.dim, "jv", torch)
+        uptr, nu = _tensor_ptr(u, self.dim, "u", torch)
+        vptr, _ = _tensor_ptr(v, self.dim, "v", torch)
+        if self.pdim > 0:
+            pptr, np_ = _tensor_ptr(p, self.pdim, "p", torch)
+        else:
+            pptr, np_ = (p.data_ptr(), p.numel()) if isinstance(p, torch.Tensor) else (None, 0)
+        stream = torch.cuda.current_stream(du.device).cuda_stream
+        A.check(A.lib().nd_rhs_jvp(self.plan, dptr, jptr, uptr, vptr, pptr, float(t), ndu, nu, np_, stream))
+
+    def jacobian(self, u, p, t=0.0):
+        """Dense Jacobian d(du)/du at u (dim x dim, float64 device tensor), one JVP per
+        unit vector; the matrix ForwardDiff.jacobian builds in linearize_network
+        (linear_analysis.jl:84)."""
+        import torch
+        n = self.dim
+        J = torch.empty((n, n), dtype=torch.float64, device=u.device)
+        du = torch.empty(n, dtype=torch.float64, device=u.device)
+        e = torch.zeros(n, dtype=torch.float64, device=u.device)
+        col = torch.empty(n, dtype=torch.float64, device=u.device)
+        for j in range(n):
+            e.zero_()
+            e[j] = 1.0
+            self.jvp(du, col, u, e, p, t)
+            J[:, j] = col
+        return J
+
     def rk4(self, u, p, t0, dt, nsteps, work=None):
         """Fixed-step classic RK4 in place on the device vector u (nsteps steps of dt)."""
         import torch

@@ -31,6 +31,14 @@ def oracle_for(nw):
     return _oracle.OracleNet(nw.nv(), nw.g.src, nw.g.dst, nw.kinds, nw.vertex_kind, nw.edge_kind)
 
 
+def fd_jvp(on, u, v, p, h=1e-3):
+    """Richardson-extrapolated central difference of the oracle RHS along v: the
+    directional derivative J(u) v with an O(h^4) truncation error."""
+    def d(hh):
+        return (on.rhs(u + hh * v, p) - on.rhs(u - hh * v, p)) / (2 * hh)
+    return (4 * d(h / 2) - d(h)) / 3
+
+
 def scaled_error(du, du_ref, S):
     """|du - du_ref| / max(|du_ref|, S): S is the sum of the magnitudes of each element's terms."""
     den = np.maximum(np.abs(du_ref), S)

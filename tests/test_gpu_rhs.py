@@ -331,3 +331,73 @@ def test_heterogeneous_rgg_c3_full_size(ND, dev):
     # the heterogeneous network at its benchmark size (N = 2e5, two vertex batches)
     nw, p, u0 = ND.workloads.heterogeneous_rgg(200_000, seed=1)
     _check(nw, u0, p, dev)
+
+
+def _dev_jvp(nw, u, v, p, dev):
+    import torch
+    d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    du = torch.full((nw.dim,), float("nan"), dtype=torch.float64, device=dev)
+    jv = torch.full((nw.dim,), float("nan"), dtype=torch.float64, device=dev)
+    nw.jvp(du, jv, d(u), d(v), d(p if len(p) else np.zeros(1)), 0.0)
+    torch.cuda.synchronize()
+    return du.cpu().numpy(), jv.cpu().numpy()
+
+
+def test_jvp_closed_forms(ND, dev):
+    # dual-number evaluation against the closed-form Jacobians (Kuramoto incidence form, swing)
+    from test_oracle import kuramoto_jvp_closed_form, swing_jvp_closed_form
+    rng = np.random.default_rng(21)
+    nw, p, u0 = ND.workloads.kuramoto_er(1000, 4000, 3.0, seed=1)
+    v = rng.normal(size=nw.dim)
+    du, jv = _dev_jvp(nw, u0, v, p, dev)
+    ref, S = kuramoto_jvp_closed_form(nw.g, u0, p[nw.nv():], v)
+    assert scaled_error(jv, ref, S).max() <= RTOL
+    on = oracle_for(nw)
+    assert scaled_error(du, on.rhs(u0, p), on.scale(u0, p)).max() <= RTOL
+    nw, p, u0 = ND.workloads.swing_ba(20_000, 2, seed=22)
+    n = nw.nv()
+    v = rng.normal(size=nw.dim)
+    du, jv = _dev_jvp(nw, u0, v, p, dev)
+    ref, S = swing_jvp_closed_form(nw.g, u0, p[0:3 * n:3], p[1:3 * n:3], p[3 * n:], v)
+    assert scaled_error(jv, ref, S).max() <= RTOL
+    # the value part is the RHS, every segment in the reference order
+    on = oracle_for(nw)
+    assert scaled_error(du, on.rhs(u0, p), on.scale(u0, p)).max() <= RTOL
+    nw.tune(hub_degree=1 << 30)
+    assert np.array_equal(du[0::2], _dev_rhs(nw, u0, p, dev)[0::2])
+
+
+@pytest.mark.parametrize("seed,directed", [(31, False), (32, True)])
+def test_jvp_mixed_networks_against_finite_differences(ND, dev, seed, directed):
+    # several vertex and edge batches, every wrapper, ODE edges: against the Richardson
+    # finite difference of the oracle (truncation and rounding ~1e-9 of the term scale)
+    from conftest import fd_jvp
+    from test_oracle import _random_mixed
+    nw, u, p = _random_mixed(ND, n=400, directed=directed, seed=seed)
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=nw.dim)
+    du, jv = _dev_jvp(nw, u, v, p, dev)
+    on = oracle_for(nw)
+    ref = fd_jvp(on, u, v, p)
+    S = np.abs(fd_jvp(on, u, np.abs(v), p)) + np.abs(ref) + 1.0
+    assert np.all(np.isfinite(jv))
+    assert (np.abs(jv - ref) / S).max() <= 1e-7
+    assert scaled_error(du, on.rhs(u, p), on.scale(u, p)).max() <= RTOL
+
+
+def test_dense_jacobian_kuramoto(ND, dev):
+    # the dense Jacobian of first-order Kuramoto: symmetric, rows sum to zero (global phase
+    # shift), negative semidefinite near synchrony, equal to the closed form
+    import torch
+    from test_oracle import _incidence
+    nw, p, u0 = ND.workloads.kuramoto_er(200, 600, 3.0, seed=3)
+    th = np.random.default_rng(3).normal(scale=0.2, size=nw.nv())
+    J = nw.jacobian(torch.from_numpy(th).to(dev), torch.from_numpy(p).to(dev)).cpu().numpy()
+    B = _incidence(nw.g).toarray()
+    K = p[nw.nv():]
+    ref = -(B * (K * np.cos(B.T @ th))) @ B.T
+    np.testing.assert_allclose(J, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    np.testing.assert_allclose(J, J.T, rtol=0, atol=1e-12 * np.abs(ref).max())
+    assert np.abs(J.sum(axis=1)).max() <= 1e-12 * np.abs(ref).max()
+    ev = np.linalg.eigvalsh(0.5 * (J + J.T))
+    assert ev.max() <= 1e-10 * np.abs(ev).max()

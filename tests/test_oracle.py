@@ -219,3 +219,50 @@ def test_rk4_order_of_convergence(ND):
     e1 = np.abs(on.rk4(u0, p, 0.0, 0.04, 25) - ref).max()
     e2 = np.abs(on.rk4(u0, p, 0.0, 0.02, 50) - ref).max()
     assert 12 < e1 / e2 < 20
+
+
+def _incidence(g):
+    n, m = g.nv(), g.ne()
+    return sp.coo_matrix((np.r_[np.ones(m), -np.ones(m)], (np.r_[g.src - 1, g.dst - 1], np.r_[np.arange(m), np.arange(m)])),
+                         shape=(n, m)).tocsr()
+
+
+def kuramoto_jvp_closed_form(g, th, K, v):
+    """J v of dtheta = omega - B (K o sin(B^T theta)): -B (K cos(B^T theta) o B^T v), with the
+    per-element magnitude of its terms."""
+    B = _incidence(g)
+    c = K * np.cos(B.T @ th)
+    Bv = B.T @ v
+    return -(B @ (c * Bv)), abs(B) @ np.abs(c * Bv)
+
+
+def swing_jvp_closed_form(g, u, M, D, K, v):
+    """J v of the swing network (state theta, omega interleaved)."""
+    th, vth, vom = u[0::2], v[0::2], v[1::2]
+    a, sa = kuramoto_jvp_closed_form(g, th, K, vth)
+    out, sc = np.empty_like(v), np.empty_like(v)
+    out[0::2], sc[0::2] = vom, np.abs(vom)
+    out[1::2] = 1.0 / M * (-D * vom + a)
+    sc[1::2] = (np.abs(D * vom) + sa) / M
+    return out, sc
+
+
+def test_fd_jvp_checker_on_closed_forms(ND):
+    # the finite-difference checker used for the GPU Jacobian-vector products agrees with
+    # the closed-form Jacobians of the Kuramoto and swing networks
+    from conftest import fd_jvp
+    rng = np.random.default_rng(4)
+    g = ND.erdos_renyi(200, 800, seed=6)
+    nw = ND.Network(g, ND.Lib.kuramoto_first(), ND.Lib.kuramoto_edge())
+    th, om, K = rng.uniform(-np.pi, np.pi, g.nv()), rng.normal(size=g.nv()), rng.uniform(1, 3, g.ne())
+    v = rng.normal(size=g.nv())
+    ref, S = kuramoto_jvp_closed_form(g, th, K, v)
+    assert scaled_error(fd_jvp(oracle_for(nw), th, v, np.r_[om, K]), ref, S).max() < 1e-8
+    g = ND.barabasi_albert(300, 2, seed=7)
+    nw = ND.Network(g, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    n = g.nv()
+    M, D, P, K = rng.uniform(1, 5, n), rng.uniform(0.1, 1, n), rng.normal(size=n), rng.uniform(5, 10, g.ne())
+    u, v = rng.normal(size=2 * n), rng.normal(size=2 * n)
+    p = np.r_[np.stack([M, D, P], 1).ravel(), K]
+    ref, S = swing_jvp_closed_form(g, u, M, D, K, v)
+    assert scaled_error(fd_jvp(oracle_for(nw), u, v, p), ref, S).max() < 1e-8
