@@ -257,13 +257,18 @@ typedef struct {
     double dt0;           /* RK4: the step; Tsit5: initial step, 0 = automatic                  */
     double dtmax, dtmin;  /* 0 = t1-t0 / 1e-14*max(1,|t1|)                                       */
     int64_t maxiters;     /* steps per instance, 0 = 1e6                                         */
+    const double* line_limit; /* per edge, concatenated in instance order, or NULL.  Line
+                             tripping (the cascading-failure callback of
+                             docs/examples/cascading_failure.jl, checked at the end of each
+                             accepted step): when |edge output| > limit the edge's first
+                             parameter (K) is set to 0, in p as well                        */
 } nd_ensemble_opts;
 
 typedef struct {
     double t, dt;         /* final time and next step                                           */
     int64_t naccept, nreject, nf;
     int32_t retcode;      /* 0 success, 1 maxiters, 2 dt below dtmin, 3 NaN step                 */
-    int32_t pad_;
+    int32_t ntrip;        /* lines tripped (line_limit set)                                     */
 } nd_ensemble_stat;
 
 typedef struct nd_ensemble nd_ensemble;
@@ -275,8 +280,9 @@ int nd_ensemble_sizes(const nd_ensemble* en, int64_t* total_states, int64_t* tot
 /* size class c holds instances with nv <= threads*vper (first class that fits) */
 int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t vper);
 /* integrates every instance from t0 to t1 in place on u (device); stats: device array
- * of ninst entries */
-int nd_ensemble_solve(nd_ensemble* en, double* u, const double* p, double t0, double t1,
+ * of ninst entries.  p is read only, except that with opts->line_limit the first
+ * parameter of each tripped edge is set to 0 (the callback's affect on the parameters) */
+int nd_ensemble_solve(nd_ensemble* en, double* u, double* p, double t0, double t1,
                       const nd_ensemble_opts* opts, nd_ensemble_stat* stats, void* stream);
 /* The same solve captured once into a hipGraph (class launches on parallel branches)
  * and replayed with nd_ensemble_graph_launch on any stream; u, p, stats, tspan and
@@ -284,7 +290,7 @@ int nd_ensemble_solve(nd_ensemble* en, double* u, const double* p, double t0, do
  * changed the classes; destroy graphs before their ensemble.  No reference
  * counterpart (a solve repeated over many perturbation rounds is its caller). */
 typedef struct nd_ensemble_graph nd_ensemble_graph;
-int nd_ensemble_graph_create(nd_ensemble* en, double* u, const double* p, double t0, double t1,
+int nd_ensemble_graph_create(nd_ensemble* en, double* u, double* p, double t0, double t1,
                              const nd_ensemble_opts* opts, nd_ensemble_stat* stats, nd_ensemble_graph** out);
 int nd_ensemble_graph_launch(nd_ensemble_graph* graph, void* stream);
 void nd_ensemble_graph_destroy(nd_ensemble_graph* graph);

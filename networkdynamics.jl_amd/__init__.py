@@ -36,3 +36,4 @@ __all__ = [n for n in dir() if not n.startswith("_")]
 from . import warmstart
 from . import experiments
 from . import ensemble_tune
+from . import linear_analysis

@@ -48,7 +48,8 @@ struct SolveArgs {
     const uint16_t* cv;           // chunk vertex
     const uint16_t* cptr;         // vertex -> chunk range (n + 1 per instance)
     double* u;
-    const double* p;
+    double* p;                    // written only where a line trips (line_limit set)
+    const double* limit;          // per edge (instance order), or null: no line tripping
     nd_ensemble_stat* stats;
     double t0, t1, dt0, reltol, abstol, dtmax, dtmin;
     int64_t maxiters;
@@ -124,6 +125,44 @@ __device__ __forceinline__ void rhs(const InstDesc& I, const Lds& L, const doubl
     }
 }
 
+// fval[e] = dst-side output of every edge at the state ys (one block barrier pair)
+template <int VM, int EM, int V>
+__device__ __forceinline__ void edge_flows(const InstDesc& I, const Lds& L, const double (&ys)[V][VModel<VM>::dim]) {
+    constexpr int NPE = EModel<EM>::np;
+    const int T = static_cast<int>(blockDim.x), t = static_cast<int>(threadIdx.x);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const int v = t + k * T;
+        if (v < I.n) L.theta[v] = ys[k][0];
+    }
+    __syncthreads();
+    for (int e = t; e < I.ne; e += T)
+        L.fval[e] = g_dst<EM>(L.theta[L.ends[2 * e]], L.theta[L.ends[2 * e + 1]], L.epar + e * (NPE > 0 ? NPE : 1));
+    __syncthreads();
+}
+
+// Line tripping (the component callback of the reference's cascading-failure example,
+// docs/examples/cascading_failure.jl: condition |P_line| - limit, affect K = 0), checked
+// at the end of every accepted step on the flows in fval: a line whose |flow| exceeds
+// its limit gets its first parameter set to 0, in LDS and in p.  Returns the number of
+// lines tripped in this check (block-uniform).
+template <int VM, int EM>
+__device__ __forceinline__ int trip_lines(const SolveArgs& a, const InstDesc& I, const Lds& L) {
+    constexpr int NPE = EModel<EM>::np > 0 ? EModel<EM>::np : 1;
+    const int T = static_cast<int>(blockDim.x), t = static_cast<int>(threadIdx.x);
+    const double* lim = a.limit + I.soff / 2;
+    double* pe_g = a.p + I.poff + static_cast<int64_t>(I.n) * VPar<VM>::np;
+    double cnt = 0.0;
+    for (int e = t; e < I.ne; e += T) {
+        if (L.epar[e * NPE] != 0.0 && fabs(L.fval[e]) > lim[e]) {
+            L.epar[e * NPE] = 0.0;
+            pe_g[static_cast<int64_t>(e) * NPE] = 0.0;
+            cnt += 1.0;
+        }
+    }
+    return static_cast<int>(block_sum(cnt, L.red));
+}
+
 // Tsitouras (2011) 5(4) coefficients
 constexpr double c_a21 = 0.161;
 constexpr double c_a31 = -0.008480655492356989, c_a32 = 0.335480655492357;
@@ -182,7 +221,7 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
 
     double tcur = a.t0;
     int64_t naccept = 0, nreject = 0, nf = 0;
-    int retcode = 0;
+    int retcode = 0, ntrip = 0;
     double dt = a.dt0;
     const double inv_len = 1.0 / (static_cast<double>(I.n) * D);
 
@@ -214,6 +253,10 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
                 for (int j = 0; j < D; ++j) y[q][j] = y[q][j] + sixth * (acc[q][j] + k[q][j]);
             nf += 4;
             ++naccept;
+            if (a.limit) {
+                edge_flows<VM, EM, V>(I, L, y);
+                ntrip += trip_lines<VM, EM>(a, I, L);
+            }
         }
         tcur = a.t0 + static_cast<double>(nsteps) * dt;
     } else {
@@ -310,6 +353,16 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
                 qold = fmax(eest, 1e-4);
                 dt = fmin(h / qq, a.dtmax);
                 ++naccept;
+                // fval holds the flows at the accepted state (the last evaluation was at ys);
+                // after a trip the first-same-as-last derivative is stale: re-evaluate it
+                if (a.limit) {
+                    const int tr = trip_lines<VM, EM>(a, I, L);
+                    if (tr > 0) {
+                        ntrip += tr;
+                        rhs<VM, EM, V>(I, L, y, pv, k1);
+                        ++nf;
+                    }
+                }
             } else {
                 dt = h / fmin(5.0, q11 / 0.9);
                 ++nreject;
@@ -332,7 +385,7 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
         st.nreject = nreject;
         st.nf = nf;
         st.retcode = retcode;
-        st.pad_ = 0;
+        st.ntrip = ntrip;
         a.stats[inst] = st;
     }
 }
@@ -472,7 +525,7 @@ int regroup(nd_ensemble* en) {
     return ND_OK;
 }
 
-int make_args(nd_ensemble* en, double* u, const double* p, double t0, double t1, const nd_ensemble_opts* o,
+int make_args(nd_ensemble* en, double* u, double* p, double t0, double t1, const nd_ensemble_opts* o,
               nd_ensemble_stat* stats, SolveArgs* a) {
     if (!en || !u || !p || !o || !stats) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
     if (!(t1 >= t0)) return nd::set_error(ND_ERR_ARGUMENT, "t1 < t0");
@@ -491,6 +544,7 @@ int make_args(nd_ensemble* en, double* u, const double* p, double t0, double t1,
     a->cptr = en->d_cptr;
     a->u = u;
     a->p = p;
+    a->limit = o->line_limit;
     a->stats = stats;
     a->t0 = t0;
     a->t1 = t1;
@@ -700,7 +754,7 @@ int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t
     return regroup(en);
 }
 
-int nd_ensemble_solve(nd_ensemble* en, double* u, const double* p, double t0, double t1, const nd_ensemble_opts* o,
+int nd_ensemble_solve(nd_ensemble* en, double* u, double* p, double t0, double t1, const nd_ensemble_opts* o,
                       nd_ensemble_stat* stats, void* stream) {
     SolveArgs a{};
     int rc = make_args(en, u, p, t0, t1, o, stats, &a);
@@ -708,7 +762,7 @@ int nd_ensemble_solve(nd_ensemble* en, double* u, const double* p, double t0, do
     return rc ? rc : launch_classes(en, a, o->method, static_cast<hipStream_t>(stream));
 }
 
-int nd_ensemble_graph_create(nd_ensemble* en, double* u, const double* p, double t0, double t1,
+int nd_ensemble_graph_create(nd_ensemble* en, double* u, double* p, double t0, double t1,
                              const nd_ensemble_opts* o, nd_ensemble_stat* stats, nd_ensemble_graph** out) {
     if (!out) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
     *out = nullptr;

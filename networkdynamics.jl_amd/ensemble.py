@@ -34,11 +34,12 @@ TSIT5_BTILDE = [-0.00178001105222577714, -0.0008164344596567469, 0.0078808780102
 
 class _Opts(C.Structure):
     _fields_ = [("method", C.c_int32), ("pad_", C.c_int32), ("reltol", C.c_double), ("abstol", C.c_double),
-                ("dt0", C.c_double), ("dtmax", C.c_double), ("dtmin", C.c_double), ("maxiters", C.c_int64)]
+                ("dt0", C.c_double), ("dtmax", C.c_double), ("dtmin", C.c_double), ("maxiters", C.c_int64),
+                ("line_limit", C.c_void_p)]
 
 
 STAT_DTYPE = np.dtype([("t", "f8"), ("dt", "f8"), ("naccept", "i8"), ("nreject", "i8"), ("nf", "i8"),
-                       ("retcode", "i4"), ("pad", "i4")])
+                       ("retcode", "i4"), ("ntrip", "i4")])
 
 
 def _kind_struct(m):
@@ -107,19 +108,29 @@ class Ensemble:
         n = self.nv[i] * self.vertexm.pdim + self.ne[i] * self.edgem.pdim
         return slice(int(self.param_offsets[i]), int(self.param_offsets[i] + n))
 
-    def solve(self, u, p, tspan, method=TSIT5, reltol=1e-8, abstol=1e-8, dt=0.0, dtmax=0.0, maxiters=0):
-        """Integrates all instances in place on the device vector u; returns per-instance stats."""
+    def solve(self, u, p, tspan, method=TSIT5, reltol=1e-8, abstol=1e-8, dt=0.0, dtmax=0.0, maxiters=0,
+              line_limit=None):
+        """Integrates all instances in place on the device vector u; returns per-instance stats.
+        line_limit: per-edge limits (device, instance order) to trip lines whose |flow|
+        exceeds them at the end of an accepted step; tripped lines get K = 0 in p."""
         import torch
         self._check(u, p)
         stats = torch.zeros(self.ninst * STAT_DTYPE.itemsize // 8, dtype=torch.float64, device=u.device)
-        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters)
+        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters, line_limit)
         stream = torch.cuda.current_stream(u.device).cuda_stream
         A.check(A.lib().nd_ensemble_solve(self._h, u.data_ptr(), p.data_ptr(), float(tspan[0]), float(tspan[1]),
                                           C.byref(o), stats.data_ptr(), stream))
         return stats
 
-    def _opts(self, method, reltol, abstol, dt, dtmax, maxiters):
-        return _Opts(int(method), 0, float(reltol), float(abstol), float(dt), float(dtmax), 0.0, int(maxiters))
+    def _opts(self, method, reltol, abstol, dt, dtmax, maxiters, line_limit=None):
+        lim = None
+        if line_limit is not None:
+            import torch
+            if line_limit.dtype != torch.float64 or line_limit.device.type != "cuda" or \
+                    line_limit.numel() != int(self.ne.sum()):
+                raise A.ArgumentError("line_limit must be a float64 device vector with one entry per edge")
+            lim = line_limit.data_ptr()
+        return _Opts(int(method), 0, float(reltol), float(abstol), float(dt), float(dtmax), 0.0, int(maxiters), lim)
 
     def _check(self, u, p):
         import torch
@@ -128,12 +139,13 @@ class Ensemble:
         if u.numel() != self.total_states or p.numel() != self.total_params:
             raise A.ArgumentError("u / p do not match the ensemble's sizes")
 
-    def capture(self, u, p, tspan, method=TSIT5, reltol=1e-8, abstol=1e-8, dt=0.0, dtmax=0.0, maxiters=0):
+    def capture(self, u, p, tspan, method=TSIT5, reltol=1e-8, abstol=1e-8, dt=0.0, dtmax=0.0, maxiters=0,
+                line_limit=None):
         """Captures the solve of (u, p, tspan) into a hipGraph; `.launch()` replays it in place on u."""
         import torch
         self._check(u, p)
         stats = torch.zeros(self.ninst * STAT_DTYPE.itemsize // 8, dtype=torch.float64, device=u.device)
-        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters)
+        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters, line_limit)
         h = C.c_void_p()
         A.check(A.lib().nd_ensemble_graph_create(self._h, u.data_ptr(), p.data_ptr(), float(tspan[0]),
                                                  float(tspan[1]), C.byref(o), stats.data_ptr(), C.byref(h)))

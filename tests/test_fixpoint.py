@@ -50,3 +50,30 @@ def test_batched_grids_are_fixpoints(ND, dev):
         ti = t[fs.voff[i]:fs.voff[i + 1]]
         assert ND.isfixpoint_residual(gr, ti, dev) <= 1e-10
         assert 1 <= st["newton_iters"][i] <= 20
+
+
+@pytest.mark.gpu
+def test_linearization_at_a_fixpoint(ND, dev):
+    # linearize_network / jacobian_eigenvals at a batched-Newton fixpoint of a swing/load
+    # grid: the state is a fixpoint, one zero eigenvalue (global phase), the rest stable
+    import torch
+    LA = ND.linear_analysis
+    grid = ND.swing_load_grid(150, seed=41)
+    th = torch.zeros(grid.g.nv(), dtype=torch.float64, device=dev)
+    fs = ND.FixpointSolver([grid.g])
+    st = ND.FixpointSolver.stats_to_numpy(fs.solve(th, torch.from_numpy(grid.P).to(dev),
+                                                   torch.from_numpy(grid.scale).to(dev),
+                                                   torch.from_numpy(grid.K).to(dev)))
+    assert st["retcode"][0] == 0
+    nw = grid.network()
+    u = torch.from_numpy(grid.flat_u(th.cpu().numpy())).to(dev)
+    p = torch.from_numpy(grid.flat_p()).to(dev)
+    assert LA.isfixpoint(nw, u, p, tol=1e-9)
+    ev = LA.jacobian_eigenvals(nw, u, p)
+    scale = np.abs(ev).max()
+    assert abs(ev[-1]) <= 1e-9 * scale                # the phase-shift mode
+    assert ev[:-1].real.max() < 0                     # every other mode decays
+    # a perturbed angle breaks the fixpoint
+    u2 = u.clone()
+    u2[0] += 0.1
+    assert not LA.isfixpoint(nw, u2, p, tol=1e-9)

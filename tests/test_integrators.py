@@ -235,3 +235,69 @@ def test_ensemble_rk4_other_models_match_oracle(ND, dev, kind):
     for i, g in enumerate(gs):
         ref = oracle_for(ND.Network(g, vm, em)).rk4(us[i], ps[i], 0.0, 0.002, 100)
         np.testing.assert_allclose(got[ens.state_slice(i)], ref, rtol=0, atol=1e-11, err_msg=f"instance {i}")
+
+
+@pytest.mark.gpu
+def test_ensemble_line_tripping_matches_oracle_stepping(ND, dev):
+    # cascading-failure callback (docs/examples/cascading_failure.jl): after every RK4 step
+    # a line whose |K sin(theta_s - theta_d)| exceeds its limit gets K = 0.  Reference:
+    # the oracle's RK4 one step at a time with the same check on the host.
+    import torch
+    gs, rng = _instances(ND, 8, 30, 400, seed=8)
+    ps, us = _swing_params(ND, gs, rng)
+    for u in us:
+        u[0::2] = rng.uniform(-1.5, 1.5, len(u) // 2)       # far from synchrony: lines overload
+    lims = [rng.uniform(0.85, 1.0, g.ne()) * p[3 * g.nv():] for g, p in zip(gs, ps)]
+    ens = ND.Ensemble(gs, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    u_d = torch.from_numpy(np.concatenate(us)).to(dev)
+    p_d = torch.from_numpy(np.concatenate(ps)).to(dev)
+    lim_d = torch.from_numpy(np.concatenate(lims)).to(dev)
+    dt, nsteps = 0.005, 60
+    st = ND.Ensemble.stats_to_numpy(ens.solve(u_d, p_d, (0.0, dt * nsteps), method=ND.ensemble.RK4, dt=dt,
+                                              line_limit=lim_d))
+    torch.cuda.synchronize()
+    got_u, got_p = u_d.cpu().numpy(), p_d.cpu().numpy()
+    assert st["ntrip"].sum() > 0
+    for i, g in enumerate(gs):
+        nw = ND.Network(g, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+        on = oracle_for(nw)
+        u, p = us[i].copy(), ps[i].copy()
+        n = g.nv()
+        s, d = g.src - 1, g.dst - 1
+        trips = 0
+        for _ in range(nsteps):
+            u = on.rk4(u, p, 0.0, dt, 1)
+            K = p[3 * n:]
+            flow = K * np.sin(u[2 * s] - u[2 * d])
+            hit = (K != 0) & (np.abs(flow) > lims[i])
+            K[hit] = 0.0
+            trips += int(hit.sum())
+        assert st["ntrip"][i] == trips, i
+        np.testing.assert_array_equal(got_p[ens.param_slice(i)], p)
+        np.testing.assert_allclose(got_u[ens.state_slice(i)], u, rtol=0, atol=1e-10, err_msg=f"instance {i}")
+
+
+@pytest.mark.gpu
+def test_ensemble_tsit5_line_tripping_consistent(ND, dev):
+    # adaptive Tsit5 with tripping: every tripped line had K > 0 before and ends at K = 0,
+    # no other parameter changes, and the counts add up
+    import torch
+    gs, rng = _instances(ND, 16, 50, 800, seed=9)
+    ps, us = _swing_params(ND, gs, rng)
+    for u in us:
+        u[0::2] = rng.uniform(-1.5, 1.5, len(u) // 2)
+    lims = np.concatenate([rng.uniform(0.85, 1.0, g.ne()) * p[3 * g.nv():] for g, p in zip(gs, ps)])
+    ens = ND.Ensemble(gs, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    p0 = np.concatenate(ps)
+    u_d = torch.from_numpy(np.concatenate(us)).to(dev)
+    p_d = torch.from_numpy(p0.copy()).to(dev)
+    st = ND.Ensemble.stats_to_numpy(ens.solve(u_d, p_d, (0.0, 2.0), line_limit=torch.from_numpy(lims).to(dev)))
+    torch.cuda.synchronize()
+    p1 = p_d.cpu().numpy()
+    assert (st["retcode"] == 0).all() and st["ntrip"].sum() > 0
+    changed = p1 != p0
+    assert (p1[changed] == 0).all()
+    for i, g in enumerate(gs):
+        sl = ens.param_slice(i)
+        assert changed[sl][:3 * g.nv()].sum() == 0
+        assert changed[sl].sum() == st["ntrip"][i]
