@@ -126,36 +126,51 @@ struct StageArgs {
     const double* base;
     double* acc;
     double* yout;
+    double* vnext;        // compact vertex outputs of yout for the next stage, or null
     double h, w;
     int32_t kind;
 };
 
+// returns the value written to yout (stage) or du (plain)
 template <bool STAGE>
-__device__ __forceinline__ void emit1(double* __restrict__ du, const StageArgs& sa, int64_t i, double k) {
+__device__ __forceinline__ double emit1(double* __restrict__ du, const StageArgs& sa, int64_t i, double k) {
     if constexpr (!STAGE) {
         du[i] = k;
+        return k;
     } else {
+        double y;
         if (sa.kind == 3) {
-            sa.yout[i] = sa.base[i] + sa.h * (sa.acc[i] + k);
+            y = sa.base[i] + sa.h * (sa.acc[i] + k);
         } else {
             sa.acc[i] = sa.kind == 1 ? k : sa.acc[i] + sa.w * k;
-            sa.yout[i] = sa.base[i] + sa.h * k;
+            y = sa.base[i] + sa.h * k;
         }
+        sa.yout[i] = y;
+        return y;
     }
 }
 
+// Vertex f and its epilogue.  In a stage with sa.vnext, the vertex's output component
+// (state mask0) of the new stage state also goes to vnext[oidx], the compact vertex
+// outputs the next stage gathers from.
 template <int VM, bool STAGE>
 __device__ __forceinline__ void vertex_emit(double* __restrict__ du, const StageArgs& sa, int64_t x0,
                                             const double* __restrict__ x, double acc,
-                                            const double* __restrict__ pv, int dim) {
+                                            const double* __restrict__ pv, int dim, int mask0, int64_t oidx) {
     constexpr int DIMC = VModel<VM>::dim;
     if constexpr (DIMC > 0) {
         double dx[DIMC];
         vertex_f<VM>(dx, x, acc, pv, DIMC);
 #pragma unroll
-        for (int j = 0; j < DIMC; ++j) emit1<STAGE>(du, sa, x0 + j, dx[j]);
+        for (int j = 0; j < DIMC; ++j) {
+            const double y = emit1<STAGE>(du, sa, x0 + j, dx[j]);
+            if (STAGE && sa.vnext && j == mask0) sa.vnext[oidx] = y;
+        }
     } else {
-        for (int j = 0; j < dim; ++j) emit1<STAGE>(du, sa, x0 + j, 0.0);
+        for (int j = 0; j < dim; ++j) {
+            const double y = emit1<STAGE>(du, sa, x0 + j, 0.0);
+            if (STAGE && sa.vnext && j == mask0) sa.vnext[oidx] = y;
+        }
     }
 }
 
@@ -256,7 +271,8 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
             if (t == 0) {
                 double acc = wpart[0];
                 for (int32_t w = 1; w < wpb; ++w) acc += wpart[w];
-                vertex_emit<VM, STAGE>(du, sa, x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim);
+                vertex_emit<VM, STAGE>(du, sa, x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim,
+                                       vd.mask[0], static_cast<int64_t>(vd.out0) + q);
             }
             return;
         }
@@ -271,7 +287,9 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
             const double own = u[x0 + vd.mask[0]];
             const double acc = wave_sum(segment_sum<EM, WRAP, MULTI, KC>(beg, end, lane, kWave, own, slots, slot_eb,
                                                                          ed, u, vo, p, vd.kfull));
-            if (lane == 0) vertex_emit<VM, STAGE>(du, sa, x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim);
+            if (lane == 0)
+                vertex_emit<VM, STAGE>(du, sa, x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim,
+                                       vd.mask[0], static_cast<int64_t>(vd.out0) + q);
             return;
         }
         b -= nmidb;
@@ -289,7 +307,7 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
     const bool valid = q < vd.count;
     const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(valid ? q : q0) * dim;
     if constexpr (VM == ND_VM_NOOP) {
-        if (valid) vertex_emit<VM, STAGE>(du, sa, x0, u + x0, 0.0, p, dim);
+        if (valid) vertex_emit<VM, STAGE>(du, sa, x0, u + x0, 0.0, p, dim, vd.mask[0], static_cast<int64_t>(vd.out0) + q);
     } else {
         __shared__ double s_val[4 * 256];
         __shared__ double s_own[256];
@@ -366,7 +384,7 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
             for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) acc += s_val[s2 - base];
             __syncthreads();
         }
-        if (valid && !hub) vertex_emit<VM, STAGE>(du, sa, x0, xs, acc, pvs, dim);
+        if (valid && !hub) vertex_emit<VM, STAGE>(du, sa, x0, xs, acc, pvs, dim, vd.mask[0], static_cast<int64_t>(vd.out0) + q);
     }
 }
 
@@ -756,6 +774,7 @@ struct nd_plan {
     double* rk_work = nullptr;
     double* d_vout = nullptr;      // compact vertex outputs (vdepth 1 networks with functional edges)
     size_t vout_len = 0;
+    double* d_vout2 = nullptr;     // second compact-output buffer (RK stages ping-pong)
     std::vector<int32_t> rowptr_host;
     std::vector<int2> slots_host;
     std::vector<uint8_t> slot_eb_host;
@@ -942,12 +961,15 @@ KSync no_sync(unsigned vblocks) {
 // One right-hand side (stage = false: du = f(u)) or one fused RK stage (stage = true).
 // sync: compare p's edge parameters with the slot-ordered copy first (skipped only
 // where p is known to be unchanged since the previous launch of this plan).
+// vin: the compact vertex outputs of u (default: the plan's buffer, built here first);
+// build = false when a previous stage epilogue already wrote them.
 void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const double* p, const StageArgs& sa,
-                hipStream_t s, bool sync = true) {
+                hipStream_t s, bool sync = true, double* vin = nullptr, bool build = true) {
     const int B = pl->block;
+    if (!vin) vin = pl->d_vout;
     // vertex outputs o_v = u[StateMask] into a compact array (coreloop.jl:39): the edge
     // evaluations gather neighbour outputs from it instead of from the interleaved state
-    if (pl->d_vout) {
+    if (vin && build) {
         bool pending = sync && pl->kcache;
         for (const VDesc& vd : pl->vdesc) {
             if (!vd.count) continue;
@@ -965,7 +987,7 @@ void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const doub
                 nb += blocks_for(pl->kcount, 256);
                 pending = false;
             }
-            hipLaunchKernelGGL(k_vertex_out, dim3(nb), dim3(256), 0, s, vd, u, pl->d_vout, ks);
+            hipLaunchKernelGGL(k_vertex_out, dim3(nb), dim3(256), 0, s, vd, u, vin, ks);
         }
     }
     for (size_t b = 0; b < pl->vdesc.size(); ++b) {
@@ -974,7 +996,7 @@ void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const doub
         const int64_t nblocks = vd.nbig + blocks_for(static_cast<int64_t>(vd.nmid) * kWave, B) + blocks_for(vd.count, B);
         hipLaunchKernelGGL(stage ? pl->vkern_stage[b] : pl->vkern[b], dim3(static_cast<unsigned>(nblocks)),
                            dim3(static_cast<unsigned>(B)), 0, s, vd, pl->d_rowptr, pl->d_slots, pl->d_slot_eb,
-                           pl->d_edesc, u, pl->d_vout, p, du, sa);
+                           pl->d_edesc, u, vin, p, du, sa);
     }
     for (const EDesc& e : pl->edesc) {
         if (!e.ode || e.count == 0) continue;
@@ -1291,12 +1313,27 @@ int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_
     double* yb = work + 2 * pl->dim;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const double half = 0.5 * dt, sixth = dt / 6.0;
-    // p is const for the whole call: its parameters are compared with the cached copy once
+    // Compact vertex outputs ping-pong between two buffers: every stage epilogue writes the
+    // outputs of the state it produces into the buffer the next stage reads, so only the
+    // first stage of the call builds them.  p is const for the whole call: its parameters
+    // are compared with the cached copy once.
+    double* va = pl->d_vout;
+    double* vb = nullptr;
+    if (va) {
+        if (!pl->d_vout2) {
+            void* w = nullptr;
+            hipError_t e = hipMalloc(&w, pl->vout_len * sizeof(double));
+            if (e != hipSuccess) return nd::set_error(ND_ERR_ALLOC, "hipMalloc(vertex outputs): %s", hipGetErrorString(e));
+            pl->allocations.push_back(w);
+            pl->d_vout2 = static_cast<double*>(w);
+        }
+        vb = pl->d_vout2;
+    }
     for (int64_t k = 0; k < nsteps; ++k) {
-        launch_rhs(pl, true, nullptr, u, p, StageArgs{u, acc, ya, half, 0.0, 1}, s, k == 0);
-        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, yb, half, 2.0, 2}, s, false);
-        launch_rhs(pl, true, nullptr, yb, p, StageArgs{u, acc, ya, dt, 2.0, 2}, s, false);
-        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, u, sixth, 0.0, 3}, s, false);
+        launch_rhs(pl, true, nullptr, u, p, StageArgs{u, acc, ya, vb, half, 0.0, 1}, s, k == 0, va, k == 0);
+        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, yb, va, half, 2.0, 2}, s, false, vb, false);
+        launch_rhs(pl, true, nullptr, yb, p, StageArgs{u, acc, ya, vb, dt, 2.0, 2}, s, false, va, false);
+        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, u, va, sixth, 0.0, 3}, s, false, vb, false);
     }
     return hip_check(hipGetLastError(), "nd_rk4 launch");
 }
