@@ -192,6 +192,8 @@ typedef struct {
     int32_t hub_degree;      /* segments longer than this: one 64-lane wave per vertex    */
     int32_t block_size;      /* threads per block of the vertex kernel (64..256)         */
     int32_t big_hub_degree;  /* segments longer than this: one block per vertex          */
+    int32_t rk4_path;        /* nd_rk4: 0 keep, 1 automatic (one workgroup for networks that
+                                fit it), 2 always four launches per step                  */
 } nd_plan_tuning;
 int nd_plan_tune(nd_plan* plan, const nd_plan_tuning* t);
 int nd_plan_get_tuning(const nd_plan* plan, nd_plan_tuning* t);

@@ -775,6 +775,14 @@ struct nd_plan {
     double* d_vout = nullptr;      // compact vertex outputs (vdepth 1 networks with functional edges)
     size_t vout_len = 0;
     double* d_vout2 = nullptr;     // second compact-output buffer (RK stages ping-pong)
+    // networks small enough for one workgroup: RK4 runs in the single-instance ensemble
+    // kernel (one launch for all steps) instead of four launches per step
+    bool rk4_single = false;       // the network fits the single-workgroup kernel
+    int32_t rk4_path = 1;          // 1 automatic, 2 per-stage launches
+    std::vector<int64_t> small_src, small_dst;
+    nd_kind small_vk{}, small_ek{};
+    nd_ensemble* small_en = nullptr;
+    nd_ensemble_stat* d_small_stat = nullptr;
     std::vector<int32_t> rowptr_host;
     std::vector<int2> slots_host;
     std::vector<uint8_t> slot_eb_host;
@@ -803,6 +811,7 @@ struct nd_plan {
     int4* d_kpos = nullptr;
 
     ~nd_plan() {
+        if (small_en) nd_ensemble_destroy(small_en);
         for (void* p : allocations) (void)hipFree(p);
     }
 };
@@ -1140,6 +1149,21 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         }
     }
 
+    if (n.vb.size() == 1 && n.eb.size() == 1 && n.vdepth == 1 && n.nv <= 2048 && 2 * n.ne <= 65535 && n.ne > 0) {
+        const nd_kind& vk = n.kinds[n.vb[0].kind];
+        const nd_kind& ek = n.kinds[n.eb[0].kind];
+        const bool vok = (vk.model == ND_VM_KURAMOTO || vk.model == ND_VM_SWING) && vk.ff == ND_FF_PURESTATEMAP &&
+                         vk.outdim_dst == 1 && vk.mask_dst[0] == 1;
+        const bool eok = (ek.model == ND_EM_KURAMOTO || ek.model == ND_EM_STATIC_LINE) &&
+                         ek.wrapper == ND_WRAP_ANTISYMMETRIC && ek.dim == 0 && ek.outdim_dst == 1;
+        if (vok && eok) {
+            pl->rk4_single = true;
+            pl->small_src.assign(n.src.begin(), n.src.end());
+            pl->small_dst.assign(n.dst.begin(), n.dst.end());
+            pl->small_vk = vk;
+            pl->small_ek = ek;
+        }
+    }
     int rc = ND_OK;
     rc = rc ? rc : upload(pl, pl->rowptr_host, &pl->d_rowptr);
     rc = rc ? rc : upload(pl, slots, &pl->d_slots);
@@ -1212,6 +1236,8 @@ int nd_plan_tune(nd_plan* pl, const nd_plan_tuning* t) {
     }
     if (t->hub_degree < 0 || t->big_hub_degree < 0)
         return nd::set_error(ND_ERR_ARGUMENT, "hub degrees must be positive");
+    if (t->rk4_path < 0 || t->rk4_path > 2) return nd::set_error(ND_ERR_ARGUMENT, "rk4_path must be 0, 1 or 2");
+    if (t->rk4_path) pl->rk4_path = t->rk4_path;
     if (t->hub_degree || t->big_hub_degree) {
         if (t->hub_degree) pl->hub_degree = t->hub_degree;
         if (t->big_hub_degree) pl->big_hub_degree = t->big_hub_degree;
@@ -1226,6 +1252,7 @@ int nd_plan_get_tuning(const nd_plan* pl, nd_plan_tuning* t) {
     t->hub_degree = pl->hub_degree;
     t->block_size = pl->block;
     t->big_hub_degree = pl->big_hub_degree;
+    t->rk4_path = pl->rk4_path;
     return ND_OK;
 }
 
@@ -1298,6 +1325,28 @@ int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_
     if (pl->pdim > 0 && !p) return nd::set_error(ND_ERR_ARGUMENT, "NULL parameter pointer");
     int rc = check_device(pl);
     if (rc) return rc;
+    if (pl->rk4_single && pl->rk4_path == 1 && nsteps > 0 && dt > 0.0) {
+        // one vertex batch (vertex order) and one edge batch (edge order): the plan's flat
+        // layout is the ensemble's layout for a single instance
+        if (!pl->small_en) {
+            const int64_t nv = static_cast<int64_t>(pl->vdesc[0].count), ne = static_cast<int64_t>(pl->small_src.size());
+            rc = nd_ensemble_create(1, &nv, &ne, pl->small_src.data(), pl->small_dst.data(), &pl->small_vk,
+                                    &pl->small_ek, &pl->small_en);
+            if (rc) return rc;
+            void* w = nullptr;
+            hipError_t e = hipMalloc(&w, sizeof(nd_ensemble_stat));
+            if (e != hipSuccess) return nd::set_error(ND_ERR_ALLOC, "hipMalloc(stats): %s", hipGetErrorString(e));
+            pl->allocations.push_back(w);
+            pl->d_small_stat = static_cast<nd_ensemble_stat*>(w);
+        }
+        nd_ensemble_opts o{};
+        o.method = ND_METHOD_RK4;
+        o.dt0 = dt;
+        o.maxiters = nsteps + 1;
+        // without line limits the ensemble kernel never writes p
+        return nd_ensemble_solve(pl->small_en, u, const_cast<double*>(p), 0.0, static_cast<double>(nsteps) * dt, &o,
+                                 pl->d_small_stat, stream);
+    }
     if (!work) {
         if (!pl->rk_work) {
             void* w = nullptr;

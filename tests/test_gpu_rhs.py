@@ -401,3 +401,26 @@ def test_dense_jacobian_kuramoto(ND, dev):
     assert np.abs(J.sum(axis=1)).max() <= 1e-12 * np.abs(ref).max()
     ev = np.linalg.eigvalsh(0.5 * (J + J.T))
     assert ev.max() <= 1e-10 * np.abs(ev).max()
+
+
+@pytest.mark.parametrize("em", ["kuramoto", "static_line"])
+def test_rk4_single_workgroup_path(ND, dev, em):
+    # small single-batch networks: nd_rk4 runs all steps in one workgroup; same answer as
+    # the oracle and as the four-launches-per-step path
+    rng = np.random.default_rng(51)
+    g = ND.barabasi_albert(1500, 2, seed=51)
+    n, m = g.nv(), g.ne()
+    edge = ND.Lib.kuramoto_edge() if em == "kuramoto" else ND.Lib.static_line_edge()
+    nw = ND.Network(g, ND.Lib.kuramoto_second(), edge)
+    pe = rng.uniform(5, 10, m) if em == "kuramoto" else np.stack([rng.uniform(5, 10, m), rng.uniform(0, 0.1, m)], 1).ravel()
+    P = rng.normal(size=n)
+    p = np.r_[np.stack([rng.uniform(1, 5, n), rng.uniform(0.1, 1, n), P - P.mean()], 1).ravel(), pe]
+    u0 = np.empty(2 * n)
+    u0[0::2], u0[1::2] = rng.uniform(-np.pi, np.pi, n), rng.normal(size=n)
+    ref = oracle_for(nw).rk4(u0, p, 0.0, 0.002, 250)
+    assert nw.tuning()["rk4_path"] == 1
+    one = _dev_rk4(nw, u0, p, dev, 0.002, 250)
+    nw.tune(rk4_path=2)
+    four = _dev_rk4(nw, u0, p, dev, 0.002, 250)
+    np.testing.assert_allclose(one, ref, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(four, ref, rtol=0, atol=1e-10)
