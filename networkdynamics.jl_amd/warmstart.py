@@ -59,6 +59,7 @@ class GraphBatch:
         self.a_fwd = torch.from_numpy(a_fwd).to(dev)
         self.a_bwd = torch.from_numpy(a_bwd).to(dev)
         self.host = dict(rowptr=rowptr, nbr=cols, a_fwd=a_fwd, a_bwd=a_bwd)
+        self.w = np.ascontiguousarray(w, np.float64)
 
 
 def source(grid, theta):
@@ -224,3 +225,90 @@ def warm_start(grids, s0, t_hat, dev):
            zip(grids, s0, t_hat if t_hat is not None else [None] * len(grids))]
     th, _, el = solve_grids(grids, dev, linearized=True, P=src)
     return [center(t) for t in th], el
+
+
+class GridBatch:
+    """Device-resident data of a set of grids for repeated solves: the Newton solver's
+    topology, the network's CSR (GraphBatch), and P, scale, K.  Built once per grid set;
+    every per-solve step below runs on the GPU."""
+
+    def __init__(self, grids, dev):
+        import torch
+        self.grids = list(grids)
+        self.dev = dev
+        self.fs = FixpointSolver([gr.g for gr in self.grids])
+        self.graph = GraphBatch(self.grids, dev)
+        h = self.graph.host
+        self.n = self.graph.n
+        self.voff = self.graph.voff
+        self.lengths = torch.from_numpy(np.diff(h["rowptr"]).astype(np.int64)).to(dev)
+        rows = np.repeat(np.arange(self.n), np.diff(h["rowptr"]))
+        self.rows = torch.from_numpy(rows).to(dev)
+        self.cols = torch.from_numpy(np.asarray(h["nbr"], np.int64)).to(dev)
+        self.w = torch.from_numpy(self.graph.w).to(dev)                     # K of each CSR entry
+        self.glen = torch.from_numpy(np.diff(self.voff).astype(np.int64)).to(dev)
+        cat = lambda a: torch.from_numpy(np.ascontiguousarray(np.concatenate(a))).to(dev)
+        self.P = cat([gr.P for gr in self.grids])
+        self.S = cat([gr.scale for gr in self.grids])
+        self.K = cat([gr.K for gr in self.grids])
+        self.is_gen = cat([gr.is_gen.astype(float) for gr in self.grids])
+        self.kdeg = torch.segment_reduce(self.w, "sum", lengths=self.lengths)
+        self.deg = self.lengths.to(torch.float64)
+
+    def solve(self, theta0=None, P=None, linearized=False, **kw):
+        """Batched Newton / linear solve; theta0, P: device tensors (defaults 0, the grids' P).
+        Returns (theta device tensor, stats, seconds)."""
+        import torch
+        th = torch.zeros(self.n, dtype=torch.float64, device=self.dev) if theta0 is None else theta0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = self.fs.solve(th, self.P if P is None else P, self.S, self.K, linearized=linearized, **kw)
+        torch.cuda.synchronize()
+        return th, FixpointSolver.stats_to_numpy(st), time.perf_counter() - t0
+
+    def center(self, x):
+        """x minus its mean over each grid."""
+        import torch
+        mean = torch.segment_reduce(x, "sum", lengths=self.glen) / self.glen.to(x.dtype)
+        return x - torch.repeat_interleave(mean, self.glen)
+
+    def source(self, theta):
+        """s_i = sum_j K_ij (sin(D_ij) - D_ij) per vertex (CSR segment sums, deterministic)."""
+        import torch
+        D = theta[self.cols] - theta[self.rows]
+        return torch.segment_reduce(self.w * (torch.sin(D) - D), "sum", lengths=self.lengths)
+
+    def features(self, theta_dc):
+        """(s0, X): the physics source and the per-vertex inputs of `features`, on the GPU."""
+        import torch
+        D = theta_dc[self.cols] - theta_dc[self.rows]
+        s0 = torch.segment_reduce(self.w * (torch.sin(D) - D), "sum", lengths=self.lengths)
+        ns0 = torch.segment_reduce(self.w * s0[self.cols], "sum", lengths=self.lengths) / self.kdeg
+        sq = torch.segment_reduce(self.w * D * D, "sum", lengths=self.lengths)
+        mx = torch.segment_reduce(D.abs(), "max", lengths=self.lengths)
+        X = torch.stack([self.P, self.is_gen, torch.log(self.kdeg), self.deg / 6.0, s0, ns0, sq, mx], dim=1)
+        return s0, X
+
+
+def warm_start_device(gb, net=None):
+    """Warm start of every grid in a GridBatch, entirely on the GPU: DC solve, physics
+    source, network correction (net None: physics only), second linear solve.
+    Returns (theta_w device tensor, dict of seconds)."""
+    import torch
+    times = {}
+    th_dc, _, times["dc_solve"] = gb.solve(linearized=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    th_dc = gb.center(th_dc)
+    s0, X = gb.features(th_dc)
+    src = gb.P + s0
+    if net is not None:
+        xm = torch.from_numpy(net.xmean).to(gb.dev)
+        xs = torch.from_numpy(net.xstd).to(gb.dev)
+        Xn = ((X - xm) / xs).to(torch.float32).contiguous()
+        out = net.forward(gb.graph, Xn).double() / net.tscale
+        src = src + gb.center(out)
+    torch.cuda.synchronize()
+    times["features_and_network"] = time.perf_counter() - t0
+    th_w, _, times["warm_solve"] = gb.solve(P=src, linearized=True)
+    return gb.center(th_w), times

@@ -149,3 +149,22 @@ def test_trained_network_reduces_newton_iterations(ND, dev):
     for g, t in zip(test, warm_th):
         assert ND.isfixpoint_residual(g, t, dev) <= 1e-10
     assert warm_st["newton_iters"].mean() <= 0.75 * flat_st["newton_iters"][keep].mean()
+
+
+@pytest.mark.gpu
+def test_device_features_and_warm_start_match_host(ND, dev):
+    # the GPU pipeline (GridBatch) computes the same source terms, features and warm
+    # start as the host functions the network is trained with
+    W = ND.warmstart
+    grids = [ND.swing_load_grid(int(n), seed=7000 + i) for i, n in enumerate((400, 900, 1600))]
+    gb = W.GridBatch(grids, dev)
+    dc_host, s0_host, X_host, _ = W.physics_inputs(grids, dev)
+    th_dc, _, _ = gb.solve(linearized=True)
+    th_dc = gb.center(th_dc)
+    np.testing.assert_allclose(th_dc.cpu().numpy(), np.concatenate(dc_host), rtol=0, atol=1e-12)
+    s0, X = gb.features(th_dc)
+    np.testing.assert_allclose(s0.cpu().numpy(), np.concatenate(s0_host), rtol=1e-9, atol=1e-13)
+    np.testing.assert_allclose(X.cpu().numpy(), np.concatenate(X_host), rtol=1e-9, atol=1e-12)
+    th_w, _ = W.warm_start_device(gb, None)
+    th_w_host, _ = W.warm_start(grids, s0_host, None, dev)
+    np.testing.assert_allclose(th_w.cpu().numpy(), np.concatenate(th_w_host), rtol=0, atol=1e-9)
