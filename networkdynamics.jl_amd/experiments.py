@@ -253,7 +253,7 @@ def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=50, lr=2e
     s0_e, _ = gb.features(gb.center(gb.solve(linearized=True)[0]))
     t_true = gb.source(gb.center(fth)) - s0_e
     starts = {}
-    th_dc, _, t_dc = gb.solve(linearized=True)
+    th_dc, _, t_dc = gb.solve(linearized=True, tol=1e-6)
     starts["dc"] = (gb.center(th_dc), dict(dc_solve=t_dc))
     starts["physics"] = W.warm_start_device(gb, None)
     starts["learned"] = W.warm_start_device(gb, net)

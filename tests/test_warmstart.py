@@ -165,6 +165,6 @@ def test_device_features_and_warm_start_match_host(ND, dev):
     s0, X = gb.features(th_dc)
     np.testing.assert_allclose(s0.cpu().numpy(), np.concatenate(s0_host), rtol=1e-9, atol=1e-13)
     np.testing.assert_allclose(X.cpu().numpy(), np.concatenate(X_host), rtol=1e-9, atol=1e-12)
-    th_w, _ = W.warm_start_device(gb, None)
+    th_w, _ = W.warm_start_device(gb, None, tol=1e-10)
     th_w_host, _ = W.warm_start(grids, s0_host, None, dev)
     np.testing.assert_allclose(th_w.cpu().numpy(), np.concatenate(th_w_host), rtol=0, atol=1e-9)
