@@ -259,8 +259,15 @@ struct nd_fixpoint {
     int32_t *d_rp = nullptr, *d_nb = nullptr, *d_se = nullptr, *d_order = nullptr;
     double* d_w = nullptr;
     int64_t total_vertices = 0, total_edges = 0;
+    // the two size classes run on their own streams, forked from and joined back to the
+    // caller's stream with events, so the large grids start while small ones still run
+    hipStream_t cls_stream[2] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {};
     ~nd_fixpoint() {
         for (void* p : allocs) (void)hipFree(p);
+        for (auto& x : cls_stream) if (x) (void)hipStreamDestroy(x);
+        for (auto& x : ev_join) if (x) (void)hipEventDestroy(x);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
     }
 };
 
@@ -387,23 +394,49 @@ int nd_fixpoint_solve(nd_fixpoint* f, double* theta, const double* P, const doub
     a.inexact = o->inexact ? 1 : 0;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const size_t ns = f->small.size(), nl = f->large.size();
+    int rc = ND_OK;
+    if (!f->ev_fork) rc = herr(hipEventCreateWithFlags(&f->ev_fork, hipEventDisableTiming), "hipEventCreate");
+    for (int c = 0; c < 2 && !rc; ++c) {
+        if (!f->cls_stream[c]) rc = herr(hipStreamCreateWithFlags(&f->cls_stream[c], hipStreamNonBlocking), "hipStreamCreate");
+        if (!rc && !f->ev_join[c]) rc = herr(hipEventCreateWithFlags(&f->ev_join[c], hipEventDisableTiming), "hipEventCreate");
+    }
+    if (rc) return rc;
+    size_t lds_s = 0, lds_l = 0;
     if (ns) {
-        const size_t lds = (2 * static_cast<size_t>(f->grids[f->small[0]].n) + 32) * sizeof(double);
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_newton<4>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        if (e != hipSuccess) return herr(e, "hipFuncSetAttribute");
-        a.first = 0;
-        hipLaunchKernelGGL(k_newton<4>, dim3(static_cast<unsigned>(ns)), dim3(kThreads), lds, s, a);
+        lds_s = (2 * static_cast<size_t>(f->grids[f->small[0]].n) + 32) * sizeof(double);
+        rc = herr(hipFuncSetAttribute(reinterpret_cast<const void*>(k_newton<4>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(lds_s)), "hipFuncSetAttribute");
     }
+    if (!rc && nl) {
+        lds_l = (2 * static_cast<size_t>(f->grids[f->large[0]].n) + 32) * sizeof(double);
+        rc = herr(hipFuncSetAttribute(reinterpret_cast<const void*>(k_newton<10>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(lds_l)), "hipFuncSetAttribute");
+    }
+    rc = rc ? rc : herr(hipEventRecord(f->ev_fork, s), "hipEventRecord");
+    if (rc) return rc;
+    // large grids first: they take longest
     if (nl) {
-        const size_t lds = (2 * static_cast<size_t>(f->grids[f->large[0]].n) + 32) * sizeof(double);
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_newton<10>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        if (e != hipSuccess) return herr(e, "hipFuncSetAttribute");
+        hipStream_t cs = f->cls_stream[1];
+        rc = herr(hipStreamWaitEvent(cs, f->ev_fork, 0), "hipStreamWaitEvent");
+        if (rc) return rc;
         a.first = static_cast<int32_t>(ns);
-        hipLaunchKernelGGL(k_newton<10>, dim3(static_cast<unsigned>(nl)), dim3(kThreads), lds, s, a);
+        hipLaunchKernelGGL(k_newton<10>, dim3(static_cast<unsigned>(nl)), dim3(kThreads), lds_l, cs, a);
+        rc = herr(hipGetLastError(), "newton launch");
+        rc = rc ? rc : herr(hipEventRecord(f->ev_join[1], cs), "hipEventRecord");
+        rc = rc ? rc : herr(hipStreamWaitEvent(s, f->ev_join[1], 0), "hipStreamWaitEvent");
+        if (rc) return rc;
     }
-    return herr(hipGetLastError(), "newton launch");
+    if (ns) {
+        hipStream_t cs = f->cls_stream[0];
+        rc = herr(hipStreamWaitEvent(cs, f->ev_fork, 0), "hipStreamWaitEvent");
+        if (rc) return rc;
+        a.first = 0;
+        hipLaunchKernelGGL(k_newton<4>, dim3(static_cast<unsigned>(ns)), dim3(kThreads), lds_s, cs, a);
+        rc = herr(hipGetLastError(), "newton launch");
+        rc = rc ? rc : herr(hipEventRecord(f->ev_join[0], cs), "hipEventRecord");
+        rc = rc ? rc : herr(hipStreamWaitEvent(s, f->ev_join[0], 0), "hipStreamWaitEvent");
+    }
+    return rc;
 }
 
 }  // extern "C"

@@ -190,7 +190,7 @@ def c5_grids(count, seed, nmin=500, nmax=5000, workers=16):
         return pool.map(_grid_job, jobs, chunksize=32)
 
 
-def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=50, lr=2e-3, seed=7):
+def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=50, lr=2e-3, seed=7, warm_tol=1e-6):
     """Train on fixpoints of ntrain grids, evaluate Newton from flat / DC / physics / learned
     starts on neval held-out grids (grids without a flat-start fixpoint are dropped)."""
     import torch
@@ -253,10 +253,10 @@ def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=50, lr=2e
     s0_e, _ = gb.features(gb.center(gb.solve(linearized=True)[0]))
     t_true = gb.source(gb.center(fth)) - s0_e
     starts = {}
-    th_dc, _, t_dc = gb.solve(linearized=True, tol=1e-6)
+    th_dc, _, t_dc = gb.solve(linearized=True, tol=warm_tol)
     starts["dc"] = (gb.center(th_dc), dict(dc_solve=t_dc))
-    starts["physics"] = W.warm_start_device(gb, None)
-    starts["learned"] = W.warm_start_device(gb, net)
+    starts["physics"] = W.warm_start_device(gb, None, tol=warm_tol)
+    starts["learned"] = W.warm_start_device(gb, net, tol=warm_tol)
     # the network's prediction of t on the held-out grids
     import torch
     xm, xs = torch.from_numpy(net.xmean).to(dev), torch.from_numpy(net.xstd).to(dev)
