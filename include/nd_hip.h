@@ -194,6 +194,9 @@ typedef struct {
     int32_t big_hub_degree;  /* segments longer than this: one block per vertex          */
     int32_t rk4_path;        /* nd_rk4: 0 keep, 1 automatic (one workgroup for networks that
                                 fit it), 2 always four launches per step                  */
+    int32_t gather_mode;     /* neighbour outputs: 0 keep, 1 automatic (by size), 2 read
+                                from u directly, 3 compact copy per call (with the slot-
+                                ordered parameter copy); get_tuning reports 2 or 3        */
 } nd_plan_tuning;
 int nd_plan_tune(nd_plan* plan, const nd_plan_tuning* t);
 int nd_plan_get_tuning(const nd_plan* plan, nd_plan_tuning* t);

@@ -58,7 +58,7 @@ class NDBatch(C.Structure):
 
 class NDPlanTuning(C.Structure):
     _fields_ = [("hub_degree", C.c_int32), ("block_size", C.c_int32), ("big_hub_degree", C.c_int32),
-                ("rk4_path", C.c_int32)]
+                ("rk4_path", C.c_int32), ("gather_mode", C.c_int32)]
 
 
 P = C.c_void_p

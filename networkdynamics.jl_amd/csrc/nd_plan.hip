@@ -802,7 +802,14 @@ struct nd_plan {
     int32_t n_agg_hub = 0;
     std::vector<size_t> hub_offset;
     // slot-ordered edge parameters (one feed-forward edge batch with parameters)
-    bool kcache = false;
+    bool kcache = false;           // in use: kcache_ok and the compact gather
+    bool kcache_ok = false;        // the edge batch qualifies for the parameter copy
+    // neighbour outputs: gathered from the compact array (rebuilt per call) or straight
+    // from u; slot .x holds an index into one or the other
+    bool use_vout = false;
+    int32_t gather_mode = 1;       // 1 automatic, 2 direct from u, 3 compact
+    std::vector<int32_t> slot_oidx, slot_uidx;
+    int em0 = ND_EM_NOOP, wrap0 = ND_WRAP_NONE;
     int32_t knp = 0, kcount = 0, kparam0 = 0;
     std::vector<int32_t> slot_eq_host;   // per full-list entry: (edge position << 1) | is_src
     double* d_kc = nullptr;
@@ -951,6 +958,34 @@ int rebuild_hubs(nd_plan* pl) {
     return ND_OK;
 }
 
+// Chooses how the edge phase reads neighbour outputs and rebuilds what depends on it:
+// the slot indices, the hub-free lists, the parameter copy and the kernel variants.
+// Automatic: the compact array (one extra launch per call, 8 B per neighbour read)
+// once the network is large enough that its states do not stay in L2; below that the
+// vertex kernel reads u directly and one RHS is one launch per vertex batch.
+constexpr int64_t kCompactMinVertices = 1 << 18;
+int apply_gather_mode(nd_plan* pl) {
+    int64_t nv = 0;
+    for (const VDesc& vd : pl->vdesc) nv += vd.count;
+    const bool possible = pl->d_vout != nullptr;
+    pl->use_vout = possible && (pl->gather_mode == 3 || (pl->gather_mode == 1 && nv >= kCompactMinVertices));
+    pl->kcache = pl->kcache_ok && pl->use_vout;
+    const std::vector<int32_t>& x = pl->use_vout ? pl->slot_oidx : pl->slot_uidx;
+    for (size_t k = 0; k < pl->slots_host.size(); ++k) pl->slots_host[k].x = x[k];
+    release(pl, pl->d_slots);
+    pl->d_slots = nullptr;
+    int rc = upload(pl, pl->slots_host, &pl->d_slots);
+    if (rc) return rc;
+    pl->vkern.clear();
+    pl->vkern_stage.clear();
+    const int em = pl->edesc.empty() ? ND_EM_NOOP : pl->em0;
+    for (const VDesc& vd : pl->vdesc) {
+        pl->vkern.push_back(pick_vertex_kernel<false>(vd.model, em, pl->wrap0, pl->multi, pl->kcache));
+        pl->vkern_stage.push_back(pick_vertex_kernel<true>(vd.model, em, pl->wrap0, pl->multi, pl->kcache));
+    }
+    return rebuild_hubs(pl);
+}
+
 int check_device(const nd_plan* pl) {
     int dev = -1;
     hipError_t e = hipGetDevice(&dev);
@@ -975,10 +1010,15 @@ KSync no_sync(unsigned vblocks) {
 void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const double* p, const StageArgs& sa,
                 hipStream_t s, bool sync = true, double* vin = nullptr, bool build = true) {
     const int B = pl->block;
-    if (!vin) vin = pl->d_vout;
+    if (!pl->use_vout) {
+        vin = const_cast<double*>(u);    // direct gather: slot indices address u itself
+        build = false;
+    } else if (!vin) {
+        vin = pl->d_vout;
+    }
     // vertex outputs o_v = u[StateMask] into a compact array (coreloop.jl:39): the edge
     // evaluations gather neighbour outputs from it instead of from the interleaved state
-    if (vin && build) {
+    if (pl->use_vout && build) {
         bool pending = sync && pl->kcache;
         for (const VDesc& vd : pl->vdesc) {
             if (!vd.count) continue;
@@ -1108,6 +1148,7 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
     std::vector<int2> slots(nnz);
     std::vector<uint8_t> slot_eb(pl->multi ? nnz : 0);
     std::vector<int32_t> slot_eq(nnz);
+    std::vector<int32_t> uidx(nnz, -1);   // u-index of the neighbour output (direct gather)
     if (pl->multi && n.eb.size() > 255) { delete pl; return nd::set_error(ND_ERR_UNSUPPORTED, "more than 255 edge batches"); }
     {
         // output index -> (edge batch, position, side): batches hold ascending output blocks
@@ -1132,7 +1173,9 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
                     const int64_t other = is_src ? n.dst[e] - 1 : n.src[e] - 1;
                     sl.x = static_cast<int32_t>(n.v_out_first(other) - 1);
                     sl.y = static_cast<int32_t>(((n.e_param_first(e) - 1) << 1) | (is_src ? 1 : 0));
+                    uidx[s] = static_cast<int32_t>(nd::vertex_out_u_index(n, other, 0));
                 }
+                if (uidx[s] < 0) uidx[s] = sl.x;
                 slots[s] = sl;
                 if (pl->multi) slot_eb[s] = static_cast<uint8_t>(b);
                 slot_eq[s] = static_cast<int32_t>((q << 1) | (is_src ? 1 : 0));
@@ -1166,15 +1209,17 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
     }
     int rc = ND_OK;
     rc = rc ? rc : upload(pl, pl->rowptr_host, &pl->d_rowptr);
-    rc = rc ? rc : upload(pl, slots, &pl->d_slots);
     rc = rc ? rc : upload(pl, slot_eb, &pl->d_slot_eb);
+    pl->slot_oidx.resize(nnz);
+    for (size_t k = 0; k < nnz; ++k) pl->slot_oidx[k] = slots[k].x;
+    pl->slot_uidx = std::move(uidx);
     pl->slots_host = std::move(slots);
     pl->slot_eb_host = std::move(slot_eb);
     if (n.eb.size() == 1 && n.vdepth == 1 && n.ne > 0) {
         const nd_kind& k = n.kinds[n.eb[0].kind];
         const int np = edge_params(k.model);
         if (np > 0 && !is_ode_model(k.model) && k.pdim == np && !no_rows) {
-            pl->kcache = true;
+            pl->kcache_ok = true;
             pl->knp = np;
             pl->kcount = pl->edesc[0].count;
             pl->kparam0 = pl->edesc[0].param0;
@@ -1194,6 +1239,8 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
     // vertex batch descriptors + kernels
     const int em0 = n.eb.empty() ? ND_EM_NOOP : n.kinds[n.eb[0].kind].model;
     const int wrap0 = n.eb.empty() ? ND_WRAP_NONE : pl->edesc[0].wrap;
+    pl->em0 = em0;
+    pl->wrap0 = wrap0;
     for (const nd::Batch& B : n.vb) {
         const nd_kind& k = n.kinds[B.kind];
         VDesc vd{};
@@ -1205,14 +1252,8 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         vd.out0 = static_cast<int32_t>(B.out_first - 1);
         for (int j = 0; j < ND_MAX_OUT; ++j) vd.mask[j] = k.mask_dst[j] - 1;
         pl->vdesc.push_back(vd);
-        pl->vkern.push_back(pick_vertex_kernel<false>(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi,
-                                                      pl->kcache));
-        pl->vkern_stage.push_back(pick_vertex_kernel<true>(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi,
-                                                           pl->kcache));
         pl->vkern_jvp.push_back(pick_jvp_kernel(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi));
     }
-    rc = rebuild_hubs(pl);
-    if (rc) { delete pl; return rc; }
     if (n.vdepth == 1 && n.ne > 0) {
         void* w = nullptr;
         hipError_t e = hipMalloc(&w, static_cast<size_t>(n.nv) * sizeof(double));
@@ -1221,6 +1262,8 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         pl->d_vout = static_cast<double*>(w);
         pl->vout_len = static_cast<size_t>(n.nv);
     }
+    rc = apply_gather_mode(pl);
+    if (rc) { delete pl; return rc; }
     *out = pl;
     return ND_OK;
 }
@@ -1238,6 +1281,12 @@ int nd_plan_tune(nd_plan* pl, const nd_plan_tuning* t) {
         return nd::set_error(ND_ERR_ARGUMENT, "hub degrees must be positive");
     if (t->rk4_path < 0 || t->rk4_path > 2) return nd::set_error(ND_ERR_ARGUMENT, "rk4_path must be 0, 1 or 2");
     if (t->rk4_path) pl->rk4_path = t->rk4_path;
+    if (t->gather_mode < 0 || t->gather_mode > 3) return nd::set_error(ND_ERR_ARGUMENT, "gather_mode must be 0..3");
+    if (t->gather_mode && t->gather_mode != pl->gather_mode) {
+        pl->gather_mode = t->gather_mode;
+        int rc = apply_gather_mode(pl);
+        if (rc) return rc;
+    }
     if (t->hub_degree || t->big_hub_degree) {
         if (t->hub_degree) pl->hub_degree = t->hub_degree;
         if (t->big_hub_degree) pl->big_hub_degree = t->big_hub_degree;
@@ -1253,6 +1302,7 @@ int nd_plan_get_tuning(const nd_plan* pl, nd_plan_tuning* t) {
     t->block_size = pl->block;
     t->big_hub_degree = pl->big_hub_degree;
     t->rk4_path = pl->rk4_path;
+    t->gather_mode = pl->use_vout ? 3 : 2;
     return ND_OK;
 }
 
@@ -1285,25 +1335,30 @@ int nd_rhs_jvp(nd_plan* pl, double* du, double* jv, const double* u, const doubl
     int rc = check_device(pl);
     if (rc) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (pl->d_vout && !pl->d_dvout) {
+    const double* vo = u;
+    const double* dvo = v;
+    if (pl->use_vout && !pl->d_dvout) {
         void* w = nullptr;
         hipError_t e = hipMalloc(&w, pl->vout_len * sizeof(double));
         if (e != hipSuccess) return nd::set_error(ND_ERR_ALLOC, "hipMalloc(tangent outputs): %s", hipGetErrorString(e));
         pl->allocations.push_back(w);
         pl->d_dvout = static_cast<double*>(w);
     }
-    if (pl->d_vout)
+    if (pl->use_vout) {
         for (const VDesc& vd : pl->vdesc) {
             if (!vd.count) continue;
             const unsigned vb = blocks_for(vd.count, 256);
             hipLaunchKernelGGL(k_vertex_out, dim3(vb), dim3(256), 0, s, vd, u, pl->d_vout, no_sync(vb));
             hipLaunchKernelGGL(k_vertex_out, dim3(vb), dim3(256), 0, s, vd, v, pl->d_dvout, no_sync(vb));
         }
+        vo = pl->d_vout;
+        dvo = pl->d_dvout;
+    }
     for (size_t b = 0; b < pl->vdesc.size(); ++b) {
         const VDesc& vd = pl->vdesc[b];
         if (vd.count == 0) continue;
         hipLaunchKernelGGL(pl->vkern_jvp[b], dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, pl->d_rowptr,
-                           pl->d_slots, pl->d_slot_eb, pl->d_edesc, u, v, pl->d_vout, pl->d_dvout, p, du, jv);
+                           pl->d_slots, pl->d_slot_eb, pl->d_edesc, u, v, vo, dvo, p, du, jv);
     }
     for (const EDesc& e : pl->edesc) {
         if (!e.ode || e.count == 0) continue;
@@ -1366,7 +1421,7 @@ int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_
     // outputs of the state it produces into the buffer the next stage reads, so only the
     // first stage of the call builds them.  p is const for the whole call: its parameters
     // are compared with the cached copy once.
-    double* va = pl->d_vout;
+    double* va = pl->use_vout ? pl->d_vout : nullptr;
     double* vb = nullptr;
     if (va) {
         if (!pl->d_vout2) {

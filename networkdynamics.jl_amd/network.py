@@ -235,17 +235,18 @@ class Network:
             self._plan = h
         return self._plan
 
-    def tune(self, hub_degree=0, block_size=0, big_hub_degree=0, rk4_path=0):
+    def tune(self, hub_degree=0, block_size=0, big_hub_degree=0, rk4_path=0, gather_mode=0):
         """0 keeps a setting.  rk4_path: 1 automatic (one workgroup for small single-batch
-        networks), 2 four launches per RK4 step."""
-        t = A.NDPlanTuning(int(hub_degree), int(block_size), int(big_hub_degree), int(rk4_path))
+        networks), 2 four launches per RK4 step.  gather_mode: 1 automatic, 2 neighbour
+        outputs read from u, 3 compact output copy plus slot-ordered parameter copy."""
+        t = A.NDPlanTuning(int(hub_degree), int(block_size), int(big_hub_degree), int(rk4_path), int(gather_mode))
         A.check(A.lib().nd_plan_tune(self.plan, C.byref(t)))
 
     def tuning(self):
         t = A.NDPlanTuning()
         A.check(A.lib().nd_plan_get_tuning(self.plan, C.byref(t)))
         return dict(hub_degree=t.hub_degree, block_size=t.block_size, big_hub_degree=t.big_hub_degree,
-                    rk4_path=t.rk4_path)
+                    rk4_path=t.rk4_path, gather_mode=t.gather_mode)
 
     def __call__(self, du, u, p, t=0.0):
         """nw(du, u, p, t): coreloop.jl:1-102.  du is overwritten; stream-ordered on

@@ -25,13 +25,19 @@ def _dev_rhs(nw, u, p, dev, t=0.0):
 
 
 def _check(nw, u, p, dev, rtol=RTOL):
+    """Both ways of gathering neighbour outputs (straight from u, and through the compact
+    per-call copy with slot-ordered parameters) against the oracle."""
     on = oracle_for(nw)
     ref = on.rhs(u, p)
     S = on.scale(u, p)
-    got = _dev_rhs(nw, u, p, dev)
-    assert np.all(np.isfinite(got))
-    err = scaled_error(got, ref, S)
-    assert err.max() <= rtol, f"max scaled error {err.max():.3e} at {err.argmax()}"
+    mode = nw.tuning()["gather_mode"]
+    for gm in (2, 3):
+        nw.tune(gather_mode=gm)
+        got = _dev_rhs(nw, u, p, dev)
+        assert np.all(np.isfinite(got))
+        err = scaled_error(got, ref, S)
+        assert err.max() <= rtol, f"gather mode {gm}: max scaled error {err.max():.3e} at {err.argmax()}"
+    nw.tune(gather_mode=mode)
     return got, ref
 
 
@@ -219,11 +225,20 @@ def test_swing_c2_full_size_properties(ND, dev):
 
 
 def _dev_rk4(nw, u0, p, dev, dt, nsteps):
+    """nd_rk4 with the neighbour outputs read from u and through the compact copy; both
+    must agree (the results are the same arithmetic, gathered from different buffers)."""
     import torch
-    u = torch.from_numpy(np.array(u0)).to(dev)
-    nw.rk4(u, torch.from_numpy(np.ascontiguousarray(p if len(p) else np.zeros(1))).to(dev), 0.0, dt, nsteps)
-    torch.cuda.synchronize()
-    return u.cpu().numpy()
+    mode = nw.tuning()["gather_mode"]
+    out = []
+    for gm in (2, 3):
+        nw.tune(gather_mode=gm)
+        u = torch.from_numpy(np.array(u0)).to(dev)
+        nw.rk4(u, torch.from_numpy(np.ascontiguousarray(p if len(p) else np.zeros(1))).to(dev), 0.0, dt, nsteps)
+        torch.cuda.synchronize()
+        out.append(u.cpu().numpy())
+    nw.tune(gather_mode=mode)
+    assert np.array_equal(out[0], out[1])
+    return out[1]
 
 
 def test_rk4_kuramoto_c1(ND, dev):
@@ -254,6 +269,8 @@ def test_edge_parameter_changes_between_calls(ND, dev):
     # edges, all edges, alternating parameter vectors, -0.0 vs 0.0, and a hub re-tune
     import torch
     nw, p, u0 = ND.workloads.swing_ba(20_000, 2, seed=8)
+    nw.tune(gather_mode=3)
+    assert nw.tuning()["gather_mode"] == 3
     on = oracle_for(nw)
     n, E = nw.nv(), nw.ne()
     rng = np.random.default_rng(8)
@@ -303,6 +320,7 @@ def test_static_line_parameter_cache(ND, dev):
     # two parameters per edge (admittance, phase shift), several vertex batches
     nw, p, u0 = ND.workloads.heterogeneous_rgg(20_000, seed=9)
     assert nw.info.nebatches == 1
+    nw.tune(gather_mode=3)
     _check(nw, u0, p, dev)
     lo, hi = int(nw.ranges()[1][0, 3, 0]) - 1, nw.pdim     # one static-line batch: params contiguous
     p2 = p.copy()
@@ -336,11 +354,18 @@ def test_heterogeneous_rgg_c3_full_size(ND, dev):
 def _dev_jvp(nw, u, v, p, dev):
     import torch
     d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
-    du = torch.full((nw.dim,), float("nan"), dtype=torch.float64, device=dev)
-    jv = torch.full((nw.dim,), float("nan"), dtype=torch.float64, device=dev)
-    nw.jvp(du, jv, d(u), d(v), d(p if len(p) else np.zeros(1)), 0.0)
-    torch.cuda.synchronize()
-    return du.cpu().numpy(), jv.cpu().numpy()
+    mode = nw.tuning()["gather_mode"]
+    res = []
+    for gm in (2, 3):
+        nw.tune(gather_mode=gm)
+        du = torch.full((nw.dim,), float("nan"), dtype=torch.float64, device=dev)
+        jv = torch.full((nw.dim,), float("nan"), dtype=torch.float64, device=dev)
+        nw.jvp(du, jv, d(u), d(v), d(p if len(p) else np.zeros(1)), 0.0)
+        torch.cuda.synchronize()
+        res.append((du.cpu().numpy(), jv.cpu().numpy()))
+    nw.tune(gather_mode=mode)
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    return res[1]
 
 
 def test_jvp_closed_forms(ND, dev):
