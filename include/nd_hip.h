@@ -22,6 +22,7 @@
 #ifndef ND_HIP_H
 #define ND_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -373,6 +374,15 @@ void nd_edgelist_free(nd_edgelist* el);
 
 const char* nd_last_error(void);
 int nd_device_synchronize(void);
+
+/* Device memory for a host shim that has no GPU array package of its own: the
+ * reference's adapt(CuArray, nw) path (ext/NetworkDynamicsCUDAExt.jl:16-45) moves u, p
+ * and the caches to the device; these are the plain-pointer equivalents.  Copies are
+ * synchronous. */
+int nd_malloc(size_t bytes, void** out);
+int nd_free(void* ptr);
+int nd_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int nd_memcpy_d2h(void* dst, const void* src, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -1497,4 +1497,28 @@ int nd_get_buffers(nd_plan* pl, double* o, double* aggbuf, const double* u, cons
 
 int nd_device_synchronize(void) { return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
 
+int nd_malloc(size_t bytes, void** out) {
+    if (!out) return nd::set_error(ND_ERR_ARGUMENT, "NULL output pointer");
+    *out = nullptr;
+    if (bytes == 0) return ND_OK;
+    hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return nd::set_error(ND_ERR_ALLOC, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    return ND_OK;
+}
+
+int nd_free(void* ptr) { return ptr ? hip_check(hipFree(ptr), "hipFree") : ND_OK; }
+
+int nd_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    if (bytes && (!dst || !src)) return nd::set_error(ND_ERR_ARGUMENT, "NULL pointer");
+    return bytes ? hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy") : ND_OK;
+}
+
+int nd_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    if (bytes && (!dst || !src)) return nd::set_error(ND_ERR_ARGUMENT, "NULL pointer");
+    return bytes ? hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy") : ND_OK;
+}
+
 }  // extern "C"

@@ -449,3 +449,30 @@ def test_rk4_single_workgroup_path(ND, dev, em):
     four = _dev_rk4(nw, u0, p, dev, 0.002, 250)
     np.testing.assert_allclose(one, ref, rtol=0, atol=1e-10)
     np.testing.assert_allclose(four, ref, rtol=0, atol=1e-10)
+
+
+def test_plain_pointer_round_trip_through_the_c_abi(ND, dev):
+    # what a host shim without a GPU array package does: nd_malloc, nd_memcpy_h2d,
+    # nd_rhs on the default stream, nd_device_synchronize, nd_memcpy_d2h, nd_free
+    import ctypes as C
+    L = ND._abi.lib()
+    chk = ND._abi.check
+    nw, p, u0 = ND.workloads.swing_ba(5000, 2, seed=61)
+    ptrs = {}
+    for name, arr in (("u", u0), ("p", p), ("du", np.zeros(nw.dim))):
+        h = C.c_void_p()
+        chk(L.nd_malloc(arr.nbytes, C.byref(h)))
+        ptrs[name] = h
+        chk(L.nd_memcpy_h2d(h, np.ascontiguousarray(arr).ctypes.data, arr.nbytes))
+    try:
+        chk(L.nd_rhs(nw.plan, ptrs["du"], ptrs["u"], ptrs["p"], 0.0, nw.dim, nw.dim, nw.pdim, None))
+        chk(L.nd_device_synchronize())
+        got = np.empty(nw.dim)
+        chk(L.nd_memcpy_d2h(got.ctypes.data, ptrs["du"], got.nbytes))
+    finally:
+        for h in ptrs.values():
+            chk(L.nd_free(h))
+    on = oracle_for(nw)
+    assert scaled_error(got, on.rhs(u0, p), on.scale(u0, p)).max() <= RTOL
+    with pytest.raises(ND.ArgumentError):
+        chk(L.nd_memcpy_h2d(None, u0.ctypes.data, 8))
