@@ -268,9 +268,11 @@ class Network:
         (linear_analysis.jl:74-85).  Stream-ordered on torch's current stream."""
         import torch
         dptr, ndu = _tensor_ptr(du, self.dim, "du", torch)
-        jptr, _ = _tensor_ptr(jv, self.dim, "jv", torch)
+        jptr, njv = _tensor_ptr(jv, self.dim, "jv", torch)
         uptr, nu = _tensor_ptr(u, self.dim, "u", torch)
-        vptr, _ = _tensor_ptr(v, self.dim, "v", torch)
+        vptr, nv = _tensor_ptr(v, self.dim, "v", torch)
+        if njv != ndu or nv != nu:      # the tangent vectors have the state's layout
+            raise A.ArgumentError(f"v and jv must have the length of u and du ({self.dim})")
         if self.pdim > 0:
             pptr, np_ = _tensor_ptr(p, self.pdim, "p", torch)
         else:

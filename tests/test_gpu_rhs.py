@@ -410,6 +410,21 @@ def test_jvp_mixed_networks_against_finite_differences(ND, dev, seed, directed):
     assert scaled_error(du, on.rhs(u, p), on.scale(u, p)).max() <= RTOL
 
 
+def test_jvp_argument_errors(ND, dev):
+    # the size checks of nw(du, u, p, t) (coreloop.jl:2-7) hold for the dual evaluation too
+    import torch
+    nw, p, u0 = ND.workloads.kuramoto_er(100, 300, 3.0, seed=2)
+    d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    u, pd = d(u0), d(p)
+    good = torch.empty(nw.dim, dtype=torch.float64, device=dev)
+    with pytest.raises(ND.ArgumentError):
+        nw.jvp(good, good.clone(), d(u0[:-1]), good.clone(), pd)
+    with pytest.raises(ND.ArgumentError):
+        nw.jvp(good, good.clone(), u, good.clone(), d(p[:-1]))
+    with pytest.raises(ND.ArgumentError):
+        nw.jvp(good, good.clone(), u, torch.empty(nw.dim + 1, dtype=torch.float64, device=dev), pd)
+
+
 def test_dense_jacobian_kuramoto(ND, dev):
     # the dense Jacobian of first-order Kuramoto: symmetric, rows sum to zero (global phase
     # shift), negative semidefinite near synchrony, equal to the closed form
