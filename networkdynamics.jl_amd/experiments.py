@@ -136,6 +136,16 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
     t_ens = time.perf_counter() - ts
     uf = u_d.cpu().numpy()
     sync = np.array([np.abs(uf[ens.state_slice(i)][1::2]).max() < 0.5 for i in range(len(graphs))])
+    # the same solve captured once as a hipGraph and replayed
+    u_g = torch.from_numpy(np.concatenate(us)).to(dev)
+    graph = ens.capture(u_g, p_d, (0.0, t1), method=TSIT5, reltol=reltol, abstol=abstol)
+    torch.cuda.synchronize()
+    tg = time.perf_counter()
+    graph.launch()
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - tg
+    graph_equal = bool(torch.equal(u_g, u_d))
+    del graph
     # plain per-instance GPU path on a time-bounded sample (largest-first order not used: random sample)
     _log(f"ensemble: {len(graphs)} trajectories in {t_ens:.2f} s")
     order = rng.permutation(len(graphs))
@@ -161,6 +171,8 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
                          f"reltol={reltol}, abstol={abstol}",
                 instances=len(graphs), dropped_without_fixpoint=int((~ok).sum()), newton_s=t_newton,
                 setup_s=t_setup, ensemble_s=t_ens, ensemble_traj_per_s=traj_ens,
+                graph_replay_s=t_graph, graph_replay_traj_per_s=len(graphs) / t_graph,
+                graph_replay_equals_eager=graph_equal,
                 ensemble_steps_mean=float(est["naccept"].mean()), ensemble_rejects_mean=float(est["nreject"].mean()),
                 ensemble_retcodes=np.bincount(est["retcode"], minlength=4).tolist(),
                 returned_to_sync=int(sync.sum()), baseline_instances=done, baseline_s=t_base,
