@@ -749,6 +749,9 @@ int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t
     if (!en || cls < 0 || cls >= ND_ENSEMBLE_CLASSES) return nd::set_error(ND_ERR_ARGUMENT, "bad class");
     if (threads < kWave || threads > 512 || threads % kWave || vper < 1 || vper > 4)
         return nd::set_error(ND_ERR_ARGUMENT, "threads must be a multiple of 64 in [64,512], vper in {1,2,3,4}");
+    // regrouping replaces the instance-order array: queued solves finish with the old one
+    int rc = hip_err(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (rc) return rc;
     en->cls_threads[cls] = threads;
     en->cls_vper[cls] = vper;
     return regroup(en);

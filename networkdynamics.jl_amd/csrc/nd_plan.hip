@@ -1272,6 +1272,12 @@ void nd_plan_destroy(nd_plan* plan) { delete plan; }
 
 int nd_plan_tune(nd_plan* pl, const nd_plan_tuning* t) {
     if (!pl || !t) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
+    // re-tuning frees and re-uploads device arrays: let work already queued on any
+    // stream finish with the old ones first
+    int src = check_device(pl);
+    if (src) return src;
+    src = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (src) return src;
     if (t->block_size) {
         if (t->block_size < kWave || t->block_size > 256 || t->block_size % kWave)
             return nd::set_error(ND_ERR_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
