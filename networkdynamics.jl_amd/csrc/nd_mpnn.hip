@@ -81,6 +81,124 @@ __global__ void __launch_bounds__(256) k_sgemm(int64_t M, int N, int K, const fl
     }
 }
 
+// ---- MFMA (v_mfma_f32_16x16x4f32) dense products ---------------------------------------
+// Operand layout of one 16x16x4 step in a 64-lane wave: lane l holds A[i][k] and B[k][j]
+// with i = j = l % 16, k = l / 16; the 4 accumulators of lane l are D[4*(l/16) + r][l % 16].
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// C[M x N] = A[M x K] * op(B) (+ bias, relu), N = 64 or 128, K <= 128.  op(B) = B[K x N]
+// (TRANSB false) or B[N x K]^T.  256 threads: a workgroup owns 128 rows, each wave 32 rows
+// (2 row tiles) x N columns; A's rows and all of op(B) are staged in LDS with strides
+// padded so the fragment reads are free of bank conflicts.
+// dynamic LDS floats of k_gemm_mfma: A rows (stride Kp + 2) and op(B) (stride N + 16)
+inline size_t gemm_mfma_lds_floats(int K, int N) {
+    const int Kp = (K + 3) & ~3;
+    return static_cast<size_t>(128) * (Kp + 2) + static_cast<size_t>(Kp) * (N + 16);
+}
+
+template <bool TRANSB, int N>
+__global__ void __launch_bounds__(256) k_gemm_mfma(int64_t M, int K, const float* __restrict__ A, int lda,
+                                                   const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
+                                                   const float* __restrict__ bias, int relu) {
+    constexpr int SB = N + 16, CT = N / 16;
+    extern __shared__ float smem_f[];
+    const int t = threadIdx.x, lane = t % kWave, w = t / kWave;
+    const int Kp = (K + 3) & ~3;
+    const int SA = Kp + 2;
+    float* As = smem_f;
+    float* Bs = smem_f + 128 * SA;
+    const int64_t row0 = static_cast<int64_t>(blockIdx.x) * 128;
+    for (int i = t; i < 128 * Kp; i += 256) {
+        const int r = i / Kp, k = i % Kp;
+        As[r * SA + k] = (row0 + r < M && k < K) ? A[(row0 + r) * lda + k] : 0.0f;
+    }
+    for (int i = t; i < Kp * N; i += 256) {
+        const int k = i / N, j = i % N;
+        float v = 0.0f;
+        if (k < K) v = TRANSB ? B[static_cast<int64_t>(j) * ldb + k] : B[static_cast<int64_t>(k) * ldb + j];
+        Bs[k * SB + j] = v;
+    }
+    __syncthreads();
+    floatx4 acc[2][CT];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int li = lane % 16, lk = lane / 16;
+    for (int k0 = 0; k0 < Kp; k0 += 4) {
+        float a[2], b[CT];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) a[rt] = As[(w * 32 + rt * 16 + li) * SA + k0 + lk];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) b[ct] = Bs[(k0 + lk) * SB + ct * 16 + li];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+    }
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = row0 + w * 32 + rt * 16 + 4 * lk + r;
+                const int col = ct * 16 + li;
+                if (row < M) {
+                    float v = acc[rt][ct][r] + (bias ? bias[col] : 0.0f);
+                    if (relu) v = fmaxf(v, 0.0f);
+                    C[row * ldc + col] = v;
+                }
+            }
+}
+
+// partial[b] = A[rows_b]^T * B[rows_b] on MFMA: KA <= 64 output rows (one 16-row tile per
+// wave), NB <= 128 output columns, kRowsPerBlock rows of the product's inner dimension
+// per workgroup, staged through LDS 64 rows at a time.
+constexpr int kTnChunk = 64;      // rows staged per LDS round
+template <int NBT>
+__global__ void __launch_bounds__(256) k_gemm_tn_mfma(int64_t M, int KA, int NB, const float* __restrict__ A, int lda,
+                                                      const float* __restrict__ B, int ldb, float* __restrict__ part,
+                                                      int rows_per_block) {
+    constexpr int SA = 64 + 16, SB = NBT * 16 + 16;
+    __shared__ float As[kTnChunk * SA];
+    __shared__ float Bs[kTnChunk * SB];
+    const int t = threadIdx.x, lane = t % kWave, w = t / kWave;
+    const int li = lane % 16, lk = lane / 16;
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+    const int64_t r1 = r0 + rows_per_block < M ? r0 + rows_per_block : M;
+    floatx4 acc[NBT];
+#pragma unroll
+    for (int ct = 0; ct < NBT; ++ct) acc[ct] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int64_t rb = r0; rb < r1; rb += kTnChunk) {
+        for (int i = t; i < kTnChunk * 64; i += 256) {
+            const int rr = i / 64, c = i % 64;
+            As[rr * SA + c] = (rb + rr < r1 && c < KA) ? A[(rb + rr) * lda + c] : 0.0f;
+        }
+        for (int i = t; i < kTnChunk * NBT * 16; i += 256) {
+            const int rr = i / (NBT * 16), c = i % (NBT * 16);
+            Bs[rr * SB + c] = (rb + rr < r1 && c < NB) ? B[(rb + rr) * ldb + c] : 0.0f;
+        }
+        __syncthreads();
+        // output rows w*16 .. w*16+15 of A^T B: A^T[i][k] = As[k][i]
+        for (int k0 = 0; k0 < kTnChunk; k0 += 4) {
+            const float a = As[(k0 + lk) * SA + w * 16 + li];
+#pragma unroll
+            for (int ct = 0; ct < NBT; ++ct)
+                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bs[(k0 + lk) * SB + ct * 16 + li], acc[ct], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int nout = KA * NB;
+#pragma unroll
+    for (int ct = 0; ct < NBT; ++ct)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = w * 16 + 4 * lk + r, j = ct * 16 + li;
+            if (i < KA && j < NB) part[static_cast<int64_t>(blockIdx.x) * nout + i * NB + j] = acc[ct][r];
+        }
+}
+
 // ---- dense: partial[b] = A[rows_b]^T * B[rows_b]  (KA x NB, KA*NB <= 8192) ---------------
 constexpr int kRowsPerBlock = 1024;
 __global__ void __launch_bounds__(256) k_sgemm_tn_partial(int64_t M, int KA, int NB, const float* __restrict__ A, int lda,
@@ -250,17 +368,49 @@ int herr(hipError_t e, const char* w) {
     return e == hipSuccess ? ND_OK : nd::set_error(ND_ERR_DEVICE, "%s: %s", w, hipGetErrorString(e));
 }
 
+template <bool TRANSB, int N>
+void gemm_mfma(hipStream_t s, int64_t M, int K, const float* A, int lda, const float* B, int ldb, float* C, int ldc,
+               const float* bias, int relu) {
+    const size_t lds = gemm_mfma_lds_floats(K, N) * sizeof(float);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_gemm_mfma<TRANSB, N>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    hipLaunchKernelGGL((k_gemm_mfma<TRANSB, N>), dim3(nblk(M, 128)), dim3(256), lds, s, M, K, A, lda, B, ldb, C, ldc,
+                       bias, relu);
+}
+
+// C = A op(B): MFMA for the network's shapes (N = 64 or 128 columns, K <= 128), the
+// LDS-tiled FP32 kernel otherwise
 void gemm(hipStream_t s, bool transb, int64_t M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C,
           int ldc, const float* bias, int relu) {
+    if (K <= 128 && (N == 64 || N == 128)) {
+        if (N == 64) {
+            if (transb) gemm_mfma<true, 64>(s, M, K, A, lda, B, ldb, C, ldc, bias, relu);
+            else gemm_mfma<false, 64>(s, M, K, A, lda, B, ldb, C, ldc, bias, relu);
+        } else {
+            if (transb) gemm_mfma<true, 128>(s, M, K, A, lda, B, ldb, C, ldc, bias, relu);
+            else gemm_mfma<false, 128>(s, M, K, A, lda, B, ldb, C, ldc, bias, relu);
+        }
+        return;
+    }
     dim3 grid(nblk(M, 64), nblk(N, 64));
     if (transb) hipLaunchKernelGGL(k_sgemm<true>, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias, relu);
     else hipLaunchKernelGGL(k_sgemm<false>, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias, relu);
 }
 
+// out (+)= A^T B over all M rows: per-block partials, then a fixed-order sum (deterministic).
+// MFMA for KA <= 64 and 16 <= NB <= 128; the FP32 VALU kernel for the thin cases.
 void gemm_tn(hipStream_t s, int64_t M, int KA, int NB, const float* A, int lda, const float* B, int ldb, float* part,
              float* out, int add) {
     const unsigned nb = std::max(1u, nblk(M, kRowsPerBlock));
-    hipLaunchKernelGGL(k_sgemm_tn_partial, dim3(nb), dim3(256), 0, s, M, KA, NB, A, lda, B, ldb, part);
+    if (KA <= 64 && NB >= 16 && NB <= 128) {
+        const int nbt = (NB + 15) / 16;
+        if (nbt <= 4)
+            hipLaunchKernelGGL(k_gemm_tn_mfma<4>, dim3(nb), dim3(256), 0, s, M, KA, NB, A, lda, B, ldb, part, kRowsPerBlock);
+        else
+            hipLaunchKernelGGL(k_gemm_tn_mfma<8>, dim3(nb), dim3(256), 0, s, M, KA, NB, A, lda, B, ldb, part, kRowsPerBlock);
+    } else {
+        hipLaunchKernelGGL(k_sgemm_tn_partial, dim3(nb), dim3(256), 0, s, M, KA, NB, A, lda, B, ldb, part);
+    }
     hipLaunchKernelGGL(k_reduce_partials, dim3(nblk(KA * NB, 256)), dim3(256), 0, s, static_cast<int>(nb), KA * NB, part, out, add);
 }
 
