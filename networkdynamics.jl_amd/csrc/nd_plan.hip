@@ -524,36 +524,65 @@ __device__ __forceinline__ bool same_bits(double a, double b) {
 }
 
 // Blocks [0, vblocks): o[v_out] = u[StateMask] for one vertex batch (coreloop.jl:39).
-// Blocks [vblocks, ...): one thread per edge of the parameter-cached edge batch; if any
-// of its parameters differs (bitwise) from the cached copy, the copy and the edge's
-// slot entries are rewritten.
-__global__ void k_vertex_out(VDesc vd, const double* __restrict__ u, double* __restrict__ o, KSync ks) {
+// Blocks [vblocks, ...): the parameter-cached edge batch; an edge whose parameters differ
+// (bitwise) from the cached copy gets the copy and its slot entries rewritten.  Both
+// parts loop grid-stride over two elements per thread (16-byte accesses where the
+// layout allows), so a modest grid covers any size.
+__device__ __forceinline__ void ksync_edge(const KSync& ks, int64_t e, const double* nv) {
+    bool changed = false;
+    for (int c = 0; c < ks.np; ++c) changed |= !same_bits(nv[c], ks.kc[e * ks.np + c]);
+    if (!changed) return;
+    const int4 ps = ks.kpos[e];
+    for (int c = 0; c < ks.np; ++c) {
+        ks.kc[e * ks.np + c] = nv[c];
+        if (ps.x >= 0) ks.kfull[static_cast<int64_t>(ps.x) * ks.np + c] = nv[c];
+        if (ps.y >= 0) ks.kfull[static_cast<int64_t>(ps.y) * ks.np + c] = nv[c];
+        if (ps.z >= 0) ks.klow[static_cast<int64_t>(ps.z) * ks.np + c] = nv[c];
+        if (ps.w >= 0) ks.klow[static_cast<int64_t>(ps.w) * ks.np + c] = nv[c];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_vertex_out(VDesc vd, const double* __restrict__ u, double* __restrict__ o,
+                                                    KSync ks) {
     if (static_cast<int32_t>(blockIdx.x) >= ks.vblocks) {
-        const int32_t e = static_cast<int32_t>((blockIdx.x - ks.vblocks) * blockDim.x + threadIdx.x);
-        if (e >= ks.count) return;
-        double nv[2];
-        bool changed = false;
-        for (int c = 0; c < ks.np; ++c) {
-            nv[c] = ks.pe[static_cast<int64_t>(e) * ks.np + c];
-            changed |= !same_bits(nv[c], ks.kc[static_cast<int64_t>(e) * ks.np + c]);
-        }
-        if (changed) {
-            const int4 ps = ks.kpos[e];
-            for (int c = 0; c < ks.np; ++c) {
-                ks.kc[static_cast<int64_t>(e) * ks.np + c] = nv[c];
-                if (ps.x >= 0) ks.kfull[static_cast<int64_t>(ps.x) * ks.np + c] = nv[c];
-                if (ps.y >= 0) ks.kfull[static_cast<int64_t>(ps.y) * ks.np + c] = nv[c];
-                if (ps.z >= 0) ks.klow[static_cast<int64_t>(ps.z) * ks.np + c] = nv[c];
-                if (ps.w >= 0) ks.klow[static_cast<int64_t>(ps.w) * ks.np + c] = nv[c];
+        const int64_t nthr = static_cast<int64_t>(gridDim.x - ks.vblocks) * blockDim.x;
+        const int64_t tid = static_cast<int64_t>(blockIdx.x - ks.vblocks) * blockDim.x + threadIdx.x;
+        const bool vec = ks.np == 1 && (reinterpret_cast<uintptr_t>(ks.pe) & 15) == 0;
+        for (int64_t e = 2 * tid; e < ks.count; e += 2 * nthr) {
+            if (vec && e + 1 < ks.count) {
+                const double2 k2 = *reinterpret_cast<const double2*>(ks.pe + e);
+                const double a = k2.x, b = k2.y;
+                ksync_edge(ks, e, &a);
+                ksync_edge(ks, e + 1, &b);
+            } else {
+                for (int64_t f = e; f < e + 2 && f < ks.count; ++f) {
+                    double nv[2];
+                    for (int c = 0; c < ks.np; ++c) nv[c] = ks.pe[f * ks.np + c];
+                    ksync_edge(ks, f, nv);
+                }
             }
         }
         return;
     }
-    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (q >= vd.count) return;
-    const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * vd.dim;
-    const int64_t o0 = static_cast<int64_t>(vd.out0) + static_cast<int64_t>(q) * vd.depth;
-    for (int j = 0; j < vd.depth; ++j) o[o0 + j] = u[x0 + vd.mask[j]];
+    const int64_t nthr = static_cast<int64_t>(ks.vblocks) * blockDim.x;
+    const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    // two-state vertices with output state 1 (swing): two vertices = one 32-byte read
+    const bool vec2 = vd.dim == 2 && vd.depth == 1 && vd.mask[0] == 0 && (vd.state0 & 1) == 0 &&
+                      (reinterpret_cast<uintptr_t>(u) & 15) == 0 && ((vd.out0 & 1) == 0) &&
+                      (reinterpret_cast<uintptr_t>(o) & 15) == 0;
+    for (int64_t q = 2 * tid; q < vd.count; q += 2 * nthr) {
+        if (vec2 && q + 1 < vd.count) {
+            const double2* up = reinterpret_cast<const double2*>(u + vd.state0 + 2 * q);
+            const double2 s0 = up[0], s1 = up[1];
+            *reinterpret_cast<double2*>(o + vd.out0 + q) = make_double2(s0.x, s1.x);
+        } else {
+            for (int64_t r = q; r < q + 2 && r < vd.count; ++r) {
+                const int64_t x0 = static_cast<int64_t>(vd.state0) + r * vd.dim;
+                const int64_t o0 = static_cast<int64_t>(vd.out0) + r * vd.depth;
+                for (int j = 0; j < vd.depth; ++j) o[o0 + j] = u[x0 + vd.mask[j]];
+            }
+        }
+    }
 }
 
 // both outputs of one edge batch (coreloop.jl:41,78 with the wrappers of
@@ -996,6 +1025,13 @@ int check_device(const nd_plan* pl) {
 
 inline unsigned blocks_for(int64_t n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
 
+// grid of the grid-stride parts of k_vertex_out: two elements per thread, at most 2048
+// blocks of 256 per part
+unsigned vout_blocks(int64_t count) {
+    const int64_t b = (count + 511) / 512;
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(b, 2048)));
+}
+
 KSync no_sync(unsigned vblocks) {
     KSync k{};
     k.vblocks = static_cast<int32_t>(vblocks);
@@ -1022,7 +1058,7 @@ void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const doub
         bool pending = sync && pl->kcache;
         for (const VDesc& vd : pl->vdesc) {
             if (!vd.count) continue;
-            const unsigned vb = blocks_for(vd.count, 256);
+            const unsigned vb = vout_blocks(vd.count);
             KSync ks = no_sync(vb);
             unsigned nb = vb;
             if (pending) {
@@ -1033,7 +1069,7 @@ void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const doub
                 ks.kpos = pl->d_kpos;
                 ks.count = pl->kcount;
                 ks.np = pl->knp;
-                nb += blocks_for(pl->kcount, 256);
+                nb += vout_blocks(pl->kcount);
                 pending = false;
             }
             hipLaunchKernelGGL(k_vertex_out, dim3(nb), dim3(256), 0, s, vd, u, vin, ks);
