@@ -178,6 +178,27 @@ int nd_network_export_maps(const nd_network* nw, int64_t* aggmap, int64_t* gbufm
 int nd_network_csr_size(const nd_network* nw, int64_t* nnz);
 int nd_network_export_csr(const nd_network* nw, int64_t* rowptr, int64_t* cols);
 
+/* One component's mass matrix, in the forms VertexModel/EdgeModel normalise a user's
+ * value to (component_functions.jl:948-963): I, UniformScaling(m[0]), Diagonal(m[0:dim])
+ * or a dense dim x dim matrix (column-major, m[0:dim*dim]). */
+enum { ND_MASS_IDENTITY = 0, ND_MASS_UNIFORM = 1, ND_MASS_DIAGONAL = 2, ND_MASS_DENSE = 3 };
+typedef struct {
+    int32_t form;
+    int32_t pad_;
+    const double* m;
+} nd_mass;
+
+/* construct_mass_matrix(im), construction.jl:409-453: the network's mass matrix from
+ * its components' (vertex_mass[nv], edge_mass[ne]; a NULL array means I for all).
+ * Components whose matrix equals I are skipped, like the reference's filter; the others
+ * are written into their v_data / e_data blocks.  *shape = ND_MASS_IDENTITY when every
+ * component's matrix equals I (nothing written), ND_MASS_DIAGONAL when every other one
+ * is diagonal (out: the dim diagonal entries), else ND_MASS_DENSE (out: dim*dim,
+ * column-major).  out == NULL only reports the shape.  ND_ERR_ARGUMENT for an unknown
+ * form or a missing m. */
+int nd_network_mass_matrix(const nd_network* nw, const nd_mass* vertex_mass, const nd_mass* edge_mass,
+                           int32_t* shape, double* out);
+
 /* ---- device plan ------------------------------------------------------------ */
 
 typedef struct nd_plan nd_plan;

@@ -17,6 +17,7 @@ ND_FF_PURESTATEMAP, ND_FF_PUREFEEDFORWARD, ND_FF_FEEDFORWARD, ND_FF_NOFEEDFORWAR
 (ND_EM_NOOP, ND_EM_DIFFUSION, ND_EM_DIFFUSION_WEIGHTED, ND_EM_KURAMOTO, ND_EM_STATIC_LINE,
  ND_EM_DIFFUSION_FIDUCIAL, ND_EM_ODE_SIN_DIFFUSION, ND_EM_ODE_SIN_DIFFUSION_FAST, ND_EM_ODE_LINEAR) = range(9)
 ND_MAX_OUT = 4
+ND_MASS_IDENTITY, ND_MASS_UNIFORM, ND_MASS_DIAGONAL, ND_MASS_DENSE = range(4)
 
 
 class ArgumentError(ValueError):
@@ -56,6 +57,10 @@ class NDBatch(C.Structure):
                 ("param_first", C.c_int64), ("out_first", C.c_int64), ("in_first", C.c_int64)]
 
 
+class NDMass(C.Structure):
+    _fields_ = [("form", C.c_int32), ("pad_", C.c_int32), ("m", C.c_void_p)]
+
+
 class NDPlanTuning(C.Structure):
     _fields_ = [("hub_degree", C.c_int32), ("block_size", C.c_int32), ("big_hub_degree", C.c_int32),
                 ("rk4_path", C.c_int32), ("gather_mode", C.c_int32)]
@@ -76,6 +81,7 @@ PROTOTYPES = {
     "nd_network_export_maps": (C.c_int, [P, P, P]),
     "nd_network_csr_size": (C.c_int, [P, C.POINTER(I64)]),
     "nd_network_export_csr": (C.c_int, [P, P, P]),
+    "nd_network_mass_matrix": (C.c_int, [P, P, P, C.POINTER(I32), P]),
     "nd_plan_create": (C.c_int, [P, C.POINTER(P)]),
     "nd_plan_destroy": (None, [P]),
     "nd_plan_tune": (C.c_int, [P, C.POINTER(NDPlanTuning)]),

@@ -10,6 +10,8 @@ fused HIP kernels, so they are chosen from a fixed set of device functions
 (`DeviceFunction` instances defined in `library`).  Anything else raises
 `UnsupportedError` at construction time.
 """
+import numpy as np
+
 from . import _abi as A
 
 
@@ -125,6 +127,104 @@ class Fiducial:
         self.dst = _as_mask_or_fn(dst)
 
 
+class UniformScaling:
+    """LinearAlgebra.UniformScaling: s times the identity of any size."""
+
+    def __init__(self, s=1.0):
+        self.s = float(s)
+
+    def __eq__(self, other):
+        if isinstance(other, UniformScaling):
+            return self.s == other.s
+        return NotImplemented
+
+    def __hash__(self):
+        return hash(("UniformScaling", self.s))
+
+    def __call__(self, n):
+        return Diagonal(np.full(int(n), self.s))
+
+    def __repr__(self):
+        return "I" if self.s == 1.0 else f"UniformScaling({self.s:g})"
+
+
+I = UniformScaling(1.0)
+
+
+class Diagonal:
+    """LinearAlgebra.Diagonal: a square matrix given by its diagonal."""
+
+    def __init__(self, d):
+        self.diag = np.asarray(d, dtype=np.float64).reshape(-1)
+
+    def toarray(self):
+        return np.diag(self.diag)
+
+    def __eq__(self, other):
+        if isinstance(other, Diagonal):
+            return np.array_equal(self.diag, other.diag)
+        if isinstance(other, UniformScaling):
+            return bool(np.all(self.diag == other.s))
+        if isinstance(other, np.ndarray):
+            return other.shape == (len(self.diag),) * 2 and np.array_equal(self.toarray(), other)
+        return NotImplemented
+
+    __hash__ = None
+
+    def __repr__(self):
+        return f"Diagonal({self.diag.tolist()})"
+
+
+def _normalize_mass(mm, dim):
+    """The user's mass matrix as VertexModel/EdgeModel keep it (component_functions.jl:948-963):
+    nothing -> I, a number -> UniformScaling, a vector -> Diagonal (length dim), a matrix
+    (dim x dim) kept; anything else is an ArgumentError."""
+    if mm is None:
+        return I
+    if isinstance(mm, (UniformScaling, Diagonal)):
+        if isinstance(mm, Diagonal) and len(mm.diag) != dim:
+            raise A.ArgumentError("Size of mass matrix must match dimension of component.")
+        return mm
+    if isinstance(mm, (int, float, np.integer, np.floating)) and not isinstance(mm, bool):
+        return UniformScaling(mm)
+    try:
+        a = np.asarray(mm, dtype=np.float64)
+    except (TypeError, ValueError):
+        a = None
+    if a is None or a.ndim not in (1, 2):
+        raise A.ArgumentError(f"Mass matrix must be a vector, square matrix, a uniform scaling, or scalar. Got {mm!r}.")
+    if a.ndim == 1:
+        if len(a) != dim:
+            raise A.ArgumentError("If given as a vector, mass matrix must have length equal to dimension of component.")
+        return Diagonal(a)
+    if a.shape != (dim, dim):
+        raise A.ArgumentError("Size of mass matrix must match dimension of component.")
+    return a.copy()
+
+
+def is_identity_mass(mm):
+    """mm == I, comparing as LinearAlgebra does (a Diagonal or matrix entry by entry)."""
+    if isinstance(mm, UniformScaling):
+        return mm.s == 1.0
+    if isinstance(mm, Diagonal):
+        return bool(np.all(mm.diag == 1.0))
+    return bool(np.array_equal(mm, np.eye(len(mm))))
+
+
+def _mass_struct(mm):
+    """(nd_mass fields, keep-alive buffer) of a normalised mass matrix."""
+    if isinstance(mm, UniformScaling):
+        if mm.s == 1.0:
+            return (A.ND_MASS_IDENTITY, None), None
+        buf = np.array([mm.s])
+        return (A.ND_MASS_UNIFORM, buf), buf
+    if isinstance(mm, Diagonal):
+        buf = np.ascontiguousarray(mm.diag)
+        return (A.ND_MASS_DIAGONAL, buf), buf
+    buf = np.asfortranarray(mm).ravel(order="F").copy()
+    return (A.ND_MASS_DENSE, buf), buf
+
+
 def fftype(x):
     if isinstance(x, (VertexModel, EdgeModel)):
         return x.ff
@@ -156,9 +256,11 @@ class ComponentModel:
                     mask_src=tuple(self.mask_src), mask_dst=tuple(self.mask_dst))
 
     def component_hash(self):
-        """What batches components together (_component_hash, construction.jl:245-256)."""
+        """What batches components together (_component_hash, construction.jl:245-256):
+        the kind plus the identity of the state and output functions, so that two
+        distinct device functions of one equation still form two batches."""
         k = self.kind()
-        return tuple(sorted(k.items()))
+        return tuple(sorted(k.items())) + (("f", id(self.f)), ("g", id(getattr(self, "_gfn", None))))
 
     def copy(self):
         import copy as _c
@@ -223,7 +325,7 @@ class VertexModel(ComponentModel):
         self.outdim_src, self.outdim_dst = 0, len(g)
         self.mask_src, self.mask_dst = (), g.idxs
         self.name = name or f.name
-        self.mass_matrix = mass_matrix
+        self.mass_matrix = _normalize_mass(mass_matrix, self.dim)
 
     def __repr__(self):
         return f"VertexModel :{self.name} (dim={self.dim}, pdim={self.pdim}, f={self.f.name})"
@@ -260,15 +362,28 @@ class EdgeModel(ComponentModel):
             inner = g.g
             self.wrapper = g.wrapper
             if isinstance(inner, StateMask):
-                raise A.UnsupportedError("StateMask edge outputs need Fiducial in the HIP kernels")
-            if not isinstance(inner, DeviceFunction) or inner.role != "edge_g":
+                # host layout only: nd_plan_create rejects it (no HIP kernel)
+                if f is None:
+                    raise A.ArgumentError("a StateMask output needs an edge state (f)")
+                od = len(inner) if od is None else od
+                if od != len(inner):
+                    raise A.ArgumentError("outdim and g disagree")
+                if max(inner.idxs) > self.dim:
+                    raise A.ArgumentError("StateMask index beyond dim")
+                self.outdim_dst = od
+                self.outdim_src = 0 if isinstance(g, Directed) else od
+                self.mask_dst = inner.idxs
+                self.mask_src = () if isinstance(g, Directed) else inner.idxs
+                self.ff = PureStateMap()
+            elif not isinstance(inner, DeviceFunction) or inner.role != "edge_g":
                 raise A.UnsupportedError(f"edge g must be a library device function, got {inner!r}")
-            fn = inner
-            od = 1 if od is None else od
-            self.outdim_dst = od
-            self.outdim_src = 0 if isinstance(g, Directed) else od
-            self.mask_src, self.mask_dst = (), ()
-            self.ff = PureFeedForward()
+            else:
+                fn = inner
+                od = 1 if od is None else od
+                self.outdim_dst = od
+                self.outdim_src = 0 if isinstance(g, Directed) else od
+                self.mask_src, self.mask_dst = (), ()
+                self.ff = PureFeedForward()
         elif isinstance(g, Fiducial):
             self.wrapper = A.ND_WRAP_FIDUCIAL
             if not (isinstance(g.src, StateMask) and isinstance(g.dst, StateMask)):
@@ -306,7 +421,8 @@ class EdgeModel(ComponentModel):
         if self.model != A.ND_EM_NOOP and self.pdim != natural_p:
             raise A.ArgumentError(f"{(fn or f).name} needs pdim={natural_p}")
         self.name = name or (fn or f).name
-        self.mass_matrix = mass_matrix
+        self._gfn = fn
+        self.mass_matrix = _normalize_mass(mass_matrix, self.dim)
 
     def __repr__(self):
         return f"EdgeModel :{self.name} (dim={self.dim}, pdim={self.pdim}, outdim={self.outdim})"
@@ -329,6 +445,7 @@ load_vertex_f = DeviceFunction("load_vertex!", "vertex_f", A.ND_VM_LOAD, 1, 2, "
 
 noop_edge = DeviceFunction("noop_edge", "edge_g", A.ND_EM_NOOP, 0, 0, "test/aggregators_test.jl:20")
 noop_edge2 = DeviceFunction("noop_edge2", "edge_g2", A.ND_EM_NOOP, 0, 0, "test/aggregators_test.jl:27-28")
+noop_edge_f = DeviceFunction("noop_edge_f", "edge_f", A.ND_EM_NOOP, None, 0, "test/massmatrix_test.jl:13-17")
 diffusionedge = DeviceFunction("diffusionedge!", "edge_g", A.ND_EM_DIFFUSION, 0, 0,
                                "test/diffusion_test.jl:57-60")
 diffusionedge_weighted = DeviceFunction("diffusionedge_weighted!", "edge_g", A.ND_EM_DIFFUSION_WEIGHTED, 0, 1,

@@ -289,4 +289,79 @@ int nd_network_export_csr(const nd_network* h, int64_t* rowptr, int64_t* cols) {
     return ND_OK;
 }
 
+// construct_mass_matrix, construction.jl:409-453.  A component takes part when its
+// matrix differs from I (UniformScaling compares its scale only, whatever the size; the
+// matrix forms compare entry by entry); the result is diagonal when every taking part
+// is (isdiag), and each block is written over the identity.
+int nd_network_mass_matrix(const nd_network* h, const nd_mass* vm, const nd_mass* em, int32_t* shape,
+                           double* out) {
+    if (!h || !shape) return set_error(ND_ERR_ARGUMENT, "NULL argument");
+    const Network& n = h->net;
+    struct Part { int64_t first; int32_t dim; const nd_mass* m; };
+    std::vector<Part> parts;
+    bool all_diag = true;
+    auto consider = [&](const nd_mass& m, int64_t first, int32_t dim, const char* what, int64_t id) -> int {
+        if (m.form < ND_MASS_IDENTITY || m.form > ND_MASS_DENSE)
+            return set_error(ND_ERR_ARGUMENT, "%s %lld: unknown mass matrix form %d", what, (long long)id, m.form);
+        if (m.form == ND_MASS_IDENTITY) return ND_OK;
+        if (!m.m) return set_error(ND_ERR_ARGUMENT, "%s %lld: mass matrix values missing", what, (long long)id);
+        bool is_eye = true, diag = true;
+        if (m.form == ND_MASS_UNIFORM) {
+            is_eye = m.m[0] == 1.0;
+        } else if (m.form == ND_MASS_DIAGONAL) {
+            for (int32_t i = 0; i < dim; ++i) is_eye = is_eye && m.m[i] == 1.0;
+        } else {
+            for (int32_t c = 0; c < dim; ++c)
+                for (int32_t r = 0; r < dim; ++r) {
+                    const double x = m.m[static_cast<int64_t>(c) * dim + r];
+                    is_eye = is_eye && x == (r == c ? 1.0 : 0.0);
+                    diag = diag && (r == c || x == 0.0);
+                }
+        }
+        if (is_eye) return ND_OK;
+        all_diag = all_diag && diag;
+        parts.push_back({first, dim, &m});
+        return ND_OK;
+    };
+    if (vm)
+        for (int64_t v = 0; v < n.nv; ++v) {
+            int rc = consider(vm[v], n.v_state_first(v), n.kinds[n.vkind[v]].dim, "vertex", v + 1);
+            if (rc) return rc;
+        }
+    if (em)
+        for (int64_t e = 0; e < n.ne; ++e) {
+            int rc = consider(em[e], n.e_state_first(e), n.kinds[n.ekind[e]].dim, "edge", e + 1);
+            if (rc) return rc;
+        }
+    if (parts.empty()) { *shape = ND_MASS_IDENTITY; return ND_OK; }
+    *shape = all_diag ? ND_MASS_DIAGONAL : ND_MASS_DENSE;
+    if (!out) return ND_OK;
+    const int64_t d = n.dim;
+    if (all_diag) {
+        std::fill(out, out + d, 1.0);
+        for (const Part& q : parts)
+            for (int32_t i = 0; i < q.dim; ++i) {
+                const nd_mass& m = *q.m;
+                out[q.first - 1 + i] = m.form == ND_MASS_UNIFORM ? m.m[0]
+                                     : m.form == ND_MASS_DIAGONAL ? m.m[i]
+                                     : m.m[static_cast<int64_t>(i) * q.dim + i];
+            }
+        return ND_OK;
+    }
+    std::fill(out, out + d * d, 0.0);
+    for (int64_t i = 0; i < d; ++i) out[i * d + i] = 1.0;
+    for (const Part& q : parts) {
+        const nd_mass& m = *q.m;
+        for (int32_t c = 0; c < q.dim; ++c)
+            for (int32_t r = 0; r < q.dim; ++r) {
+                double x;
+                if (m.form == ND_MASS_DENSE) x = m.m[static_cast<int64_t>(c) * q.dim + r];
+                else if (r != c) x = 0.0;
+                else x = m.form == ND_MASS_UNIFORM ? m.m[0] : m.m[r];
+                out[(q.first - 1 + c) * d + (q.first - 1 + r)] = x;
+            }
+    }
+    return ND_OK;
+}
+
 }  // extern "C"

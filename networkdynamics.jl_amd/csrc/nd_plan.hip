@@ -1119,6 +1119,10 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         const nd_kind& k = n.kinds[b.kind];
         if (!edge_model_known(k.model)) return nd::set_error(ND_ERR_UNSUPPORTED, "edge model %d has no HIP kernel", k.model);
         if (k.model != ND_EM_NOOP) functional_edge = true;
+        if (k.ff == ND_FF_PURESTATEMAP && k.wrapper != ND_WRAP_FIDUCIAL)
+            return nd::set_error(ND_ERR_UNSUPPORTED, "StateMask edge outputs inside AntiSymmetric/Symmetric/Directed have no HIP kernel");
+        if (k.model == ND_EM_NOOP && k.dim > 0)
+            return nd::set_error(ND_ERR_UNSUPPORTED, "edge kinds with states but no state function have no HIP kernel");
         if (is_ode_model(k.model)) {
             if (k.ff != ND_FF_PURESTATEMAP || k.dim != 2 || k.wrapper != ND_WRAP_FIDUCIAL)
                 return nd::set_error(ND_ERR_UNSUPPORTED, "ODE edge model %d needs dim 2 and a Fiducial StateMask output", k.model);

@@ -60,6 +60,8 @@ class Ensemble:
     def __init__(self, graphs, vertexm, edgem):
         if not isinstance(vertexm, comp.VertexModel) or not isinstance(edgem, comp.EdgeModel):
             raise A.ArgumentError("Ensemble needs one VertexModel and one EdgeModel")
+        if not (comp.is_identity_mass(vertexm.mass_matrix) and comp.is_identity_mass(edgem.mass_matrix)):
+            raise A.UnsupportedError("the ensemble integrators are explicit methods and cannot use a mass matrix")
         self.graphs = list(graphs)
         self.vertexm, self.edgem = vertexm, edgem
         self.ninst = len(self.graphs)

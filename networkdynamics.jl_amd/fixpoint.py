@@ -79,17 +79,21 @@ class SwingLoadGrid:
         self.g, self.is_gen = g, np.asarray(is_gen, bool)
         self.M, self.D, self.tau, self.P, self.K = M, D, tau, P, K
         self.xy = xy
-        self._nw = None
+        self._nw = {}
 
     @property
     def scale(self):
         return np.where(self.is_gen, self.M, self.tau)
 
-    def network(self):
-        if self._nw is None:
-            swing, load = Lib.kuramoto_second(), Lib.load_vertex()
-            self._nw = Network(self.g, [swing if x else load for x in self.is_gen], Lib.kuramoto_edge())
-        return self._nw
+    def network(self, algebraic_loads=False):
+        """The grid as a Network; algebraic_loads gives the loads mass 0 (a DAE with the
+        same layout and the same fixpoints)."""
+        key = bool(algebraic_loads)
+        if key not in self._nw:
+            swing = Lib.kuramoto_second()
+            load = Lib.load_vertex(mass_matrix=0 if key else None)
+            self._nw[key] = Network(self.g, [swing if x else load for x in self.is_gen], Lib.kuramoto_edge())
+        return self._nw[key]
 
     def flat_p(self):
         """The network's flat parameter vector (vertex batches, then edges)."""

@@ -64,6 +64,11 @@ def complete_graph(n):
     return SimpleGraph._from_sorted(n, s + 1, d + 1)
 
 
+def star_digraph(n):
+    """Edges 1 -> i for i in 2:n (Graphs.jl star_digraph)."""
+    return SimpleDiGraph(n, np.ones(n - 1, np.int64), np.arange(2, n + 1))
+
+
 def path_graph(n):
     return SimpleGraph(n, np.arange(1, n), np.arange(2, n + 1))
 

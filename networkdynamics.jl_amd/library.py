@@ -45,9 +45,11 @@ def kuramoto_first():
     return c.VertexModel(f=c.kuramoto_vertex, sym=["theta"], psym=["omega"], g=1, name="kuramoto_first")
 
 
-# build-defined: tau * dtheta = P + sum of edge inputs; p = [tau, P]
-def load_vertex(name="load"):
-    return c.VertexModel(f=c.load_vertex_f, sym=["theta"], psym=["tau", "P"], g=1, name=name)
+# build-defined: tau * dtheta = P + sum of edge inputs; p = [tau, P].  With
+# mass_matrix=0 the state is algebraic: 0 = (P + sum of edge inputs) / tau.
+def load_vertex(name="load", mass_matrix=None):
+    return c.VertexModel(f=c.load_vertex_f, sym=["theta"], psym=["tau", "P"], g=1, name=name,
+                         mass_matrix=mass_matrix)
 
 
 # build-defined: y_dst = B * sin(theta_src - theta_dst - phi), AntiSymmetric; p = [B, phi]

@@ -1,10 +1,24 @@
-"""Fixpoint check and linearisation on the device (src/linear_analysis.jl).
+"""Fixpoint check and linear analysis on the device Jacobian (src/linear_analysis.jl,
+src/linear_analysis_base.jl).
 
-isfixpoint          every |du_i| < tol at (u, p, t)                  (linear_analysis.jl:8-15)
-linearize_network   A = d(du)/du, one dual-number JVP per column       (linear_analysis.jl:74-85)
-jacobian_eigenvals  eigenvalues of A (identity mass matrix)           (linear_analysis.jl:232-238)
+isfixpoint              every |du_i| < tol at (u, p, t)                       (linear_analysis.jl:8-15)
+linearize_network       descriptor system M x' = A x, A = d(du)/du from one
+                        dual-number JVP per column on the device              (linear_analysis.jl:74-85)
+reduce_dae              eliminate the algebraic states (M_ii = 0) by the Schur
+                        complement A_s = f_x - f_y g_y^-1 g_x                 (linear_analysis.jl:130-214)
+jacobian_eigenvals      eigenvalues of the reduced Jacobian                   (linear_analysis.jl:230-237)
+is_linear_stable        all eigenvalues in the open left half plane           (linear_analysis.jl:252-261)
+participation_factors   |W[i,k] V[k,i]| of the reduced system                 (linear_analysis.jl:286-309)
+
+The Jacobian comes from the device (`Network.jacobian`); the reductions and
+eigen-decompositions of the resulting dense matrices run in numpy on the host.
 """
+import warnings
+
 import numpy as np
+
+from . import _abi
+from . import components as comp
 
 
 def isfixpoint(nw, u, p, t=0.0, tol=1e-10):
@@ -14,13 +28,157 @@ def isfixpoint(nw, u, p, t=0.0, tol=1e-10):
     return bool((du.abs() < tol).all().item())
 
 
+class NetworkDescriptorSystem:
+    """M x' = A x + B u, y = C x + D u (linear_analysis_base.jl:21-60).  M is `I`, a
+    `Diagonal` or a dense matrix.  Called with a complex frequency s it returns the
+    transfer function C (M s - A)^-1 B + D (linear_analysis_base.jl:79-89)."""
+
+    def __init__(self, A, M=comp.I, B=None, C=None, D=None, sym=None, insym=None, outsym=None):
+        A_ = np.asarray(A)
+        dim = A_.shape[0]
+        if A_.ndim != 2 or A_.shape[1] != dim:
+            raise _abi.ArgumentError("A must be square")
+        if sym is not None and len(sym) != dim:
+            raise _abi.ArgumentError("Length of sym must match number of rows in A")
+        io = (B, C, D, insym, outsym)
+        if any(x is None for x in io) and not all(x is None for x in io):
+            raise _abi.ArgumentError("B, C, D, insym, and outsym must all be provided or all be nothing")
+        if B is not None:
+            B, C, D = np.atleast_2d(B), np.atleast_2d(C), np.atleast_2d(D)
+            if B.shape[0] != dim:
+                raise _abi.ArgumentError("Number of rows in B must match number of rows in A")
+            if C.shape[1] != dim:
+                raise _abi.ArgumentError("Number of columns in C must match number of rows in A")
+            if D.shape != (C.shape[0], B.shape[1]):
+                raise _abi.ArgumentError("D must have same number of rows as C and same number of columns as B")
+        self.M, self.A, self.B, self.C, self.D = M, A_, B, C, D
+        self.sym, self.insym, self.outsym = sym, insym, outsym
+
+    @property
+    def dim(self):
+        return self.A.shape[0]
+
+    def dense_M(self):
+        return _dense_mass(self.M, self.dim)
+
+    def constraint_dim(self):
+        """Number of algebraic states (linear_analysis_base.jl:97-98)."""
+        if isinstance(self.M, comp.UniformScaling):
+            return 0
+        return int(self.dim - np.sum(self.dense_M()))
+
+    def __call__(self, s):
+        if self.B is None:
+            raise _abi.ArgumentError("the system has no inputs and outputs (linearize with in/out channels)")
+        val = self.C @ np.linalg.solve(self.dense_M() * s - self.A, self.B) + self.D
+        return val
+
+    def __repr__(self):
+        return (f"NetworkDescriptorSystem (dim {self.dim}, {self.constraint_dim()} algebraic states"
+                + (f", {self.B.shape[1]} inputs, {self.C.shape[0]} outputs)" if self.B is not None else ")"))
+
+
+def _dense_mass(M, n):
+    if isinstance(M, comp.UniformScaling):
+        return M.s * np.eye(n)
+    if isinstance(M, comp.Diagonal):
+        return M.toarray()
+    return np.asarray(M, dtype=np.float64)
+
+
 def linearize_network(nw, u, p, t=0.0):
-    """Dense state matrix A (numpy, dim x dim) of du = A x around u; the networks here
-    have an identity mass matrix."""
-    return nw.jacobian(u, p, t).cpu().numpy()
+    """NetworkDescriptorSystem(M = nw.mass_matrix, A = d(du)/du at (u, p, t), sym):
+    linearize_network(s0) without in/out channels (linear_analysis.jl:74-85)."""
+    Jd = nw.jacobian(u, p, t).cpu().numpy()
+    return NetworkDescriptorSystem(Jd, M=nw.mass_matrix, sym=nw.variable_symbols())
 
 
-def jacobian_eigenvals(nw, u, p, t=0.0):
-    """Eigenvalues of the linearisation, sorted by real part (largest last)."""
-    ev = np.linalg.eigvals(linearize_network(nw, u, p, t))
-    return ev[np.argsort(ev.real, kind="stable")]
+def reduce_dae(sys):
+    """The ODE system of a descriptor system with algebraic constraints
+    (linear_analysis.jl:130-214).  Differential states are those with M_ii = 1,
+    algebraic ones those with M_ii = 0; with x_a solved from 0 = g_x x_d + g_y x_a + B_a u:
+        A_s = f_x - f_y g_y^-1 g_x,   B_s = B_d - f_y g_y^-1 B_a,
+        C_s = C_d - C_a g_y^-1 g_x,   D_s = D - C_a g_y^-1 B_a.
+    A singular g_y falls back to its pseudo-inverse with a warning, as the reference does.
+    M = I returns the system unchanged."""
+    if isinstance(sys.M, comp.UniformScaling):
+        return NetworkDescriptorSystem(sys.A, M=comp.I, B=sys.B, C=sys.C, D=sys.D, sym=sys.sym,
+                                       insym=sys.insym, outsym=sys.outsym)
+    d = np.diag(sys.dense_M())
+    d_idx = np.flatnonzero(d == 1)
+    c_idx = np.flatnonzero(d == 0)
+    Am = sys.A
+    f_x = Am[np.ix_(d_idx, d_idx)]
+    f_y = Am[np.ix_(d_idx, c_idx)]
+    g_x = Am[np.ix_(c_idx, d_idx)]
+    g_y = Am[np.ix_(c_idx, c_idx)]
+    try:
+        gy_gx = np.linalg.solve(g_y, g_x) if g_y.size else np.zeros((0, len(d_idx)))
+
+        def gy_solve(x):
+            return np.linalg.solve(g_y, x) if g_y.size else np.zeros((0,) + x.shape[1:])
+    except np.linalg.LinAlgError as e:
+        warnings.warn(f"Matrix g_y is singular or ill-conditioned, falling back to pseudoinverse ({e})")
+        gy_inv = np.linalg.pinv(g_y)
+        gy_gx = gy_inv @ g_x
+
+        def gy_solve(x):
+            return gy_inv @ x
+    A_s = f_x - f_y @ gy_gx
+    B_s, C_s, D_s = sys.B, sys.C, sys.D
+    if sys.B is not None:
+        B_d, B_a = sys.B[d_idx, :], sys.B[c_idx, :]
+        C_d, C_a = sys.C[:, d_idx], sys.C[:, c_idx]
+        B_s = B_d - f_y @ gy_solve(B_a)
+        C_s = C_d - C_a @ gy_gx
+        D_s = sys.D - C_a @ gy_solve(B_a)
+    sym = None if sys.sym is None else [sys.sym[i] for i in d_idx]
+    return NetworkDescriptorSystem(A_s, M=comp.I, B=B_s, C=C_s, D=D_s, sym=sym, insym=sys.insym,
+                                   outsym=sys.outsym)
+
+
+def _sort_eigs(ev):
+    """Julia's eigvals order for a general matrix: lexicographic by (real, imag)."""
+    return ev[np.lexsort((ev.imag, ev.real))]
+
+
+def _as_system(x, u, p, t):
+    if isinstance(x, NetworkDescriptorSystem):
+        return x
+    if u is None or p is None:
+        raise _abi.ArgumentError("pass a NetworkDescriptorSystem, or a network with u and p")
+    return linearize_network(x, u, p, t)
+
+
+def jacobian_eigenvals(sys_or_nw, u=None, p=None, t=0.0, eigvalf=np.linalg.eigvals):
+    """Eigenvalues of the reduced Jacobian, in (real, imag) order
+    (linear_analysis.jl:230-237).  Takes a NetworkDescriptorSystem or (nw, u, p, t)."""
+    red = reduce_dae(_as_system(sys_or_nw, u, p, t))
+    return _sort_eigs(np.asarray(eigvalf(red.A)))
+
+
+def is_linear_stable(nw, u, p, t=0.0, marginally_stable=False, atol=1e-14, tol=1e-10):
+    """All eigenvalues of the reduced Jacobian have negative real part; with
+    marginally_stable also those within atol of zero (linear_analysis.jl:252-261).
+    The state must be a fixpoint."""
+    if not isfixpoint(nw, u, p, t, tol=tol):
+        raise _abi.ArgumentError("The state s0 is not a fixpoint of the network.")
+    lam = jacobian_eigenvals(nw, u, p, t)
+    if marginally_stable:
+        return bool(np.all((lam.real < atol) | np.isclose(lam.real, 0.0, atol=atol, rtol=0.0)))
+    return bool(np.all(lam.real < 0.0))
+
+
+def participation_factors(sys_or_nw, u=None, p=None, t=0.0, normalize=True):
+    """(eigenvalues, pfactors, sym) of the reduced system, pfactors[k, i] = |W[i,k] V[k,i]|
+    with V the right eigenvectors and W = V^-1; normalize scales each mode's column to
+    sum 1 (linear_analysis.jl:286-309).  Eigenvalues in the decomposition's order."""
+    red = reduce_dae(_as_system(sys_or_nw, u, p, t))
+    ev, V = np.linalg.eig(red.A)
+    W = np.linalg.inv(V)
+    pf = np.abs(W.T * V)
+    if normalize:
+        s = pf.sum(axis=0)
+        nz = s > 0
+        pf[:, nz] /= s[nz]
+    return dict(eigenvalues=ev, pfactors=pf, sym=red.sym)

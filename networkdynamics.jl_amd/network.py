@@ -210,6 +210,60 @@ class Network:
         A.check(A.lib().nd_network_export_csr(self._net, rp.ctypes.data, cols.ctypes.data))
         return rp, cols[: nnz.value]
 
+    # ---- mass matrix (construction.jl:409-453) ----------------------------------------------
+    @property
+    def mass_matrix(self):
+        """construct_mass_matrix: `I` when every component's mass matrix equals the
+        identity, a `Diagonal` when all others are diagonal, else a dense (dim, dim) array."""
+        if getattr(self, "_mass", None) is None:
+            keep = []
+
+            def arr(models):
+                n = len(models)
+                a = (A.NDMass * max(n, 1))()
+                for x, m in zip(a, models):
+                    (form, buf), k = comp._mass_struct(m.mass_matrix)
+                    x.form = form
+                    x.m = buf.ctypes.data if buf is not None else None
+                    keep.append(k)
+                return a
+
+            va, ea = arr(self.vertexm), arr(self.edgem)
+            L, shape = A.lib(), C.c_int32()
+            A.check(L.nd_network_mass_matrix(self._net, C.cast(va, C.c_void_p), C.cast(ea, C.c_void_p),
+                                             C.byref(shape), None))
+            if shape.value == A.ND_MASS_IDENTITY:
+                self._mass = comp.I
+            elif shape.value == A.ND_MASS_DIAGONAL:
+                d = np.empty(self.dim)
+                A.check(L.nd_network_mass_matrix(self._net, C.cast(va, C.c_void_p), C.cast(ea, C.c_void_p),
+                                                 C.byref(shape), d.ctypes.data))
+                self._mass = comp.Diagonal(d)
+            else:
+                m = np.empty((self.dim, self.dim), order="F")
+                A.check(L.nd_network_mass_matrix(self._net, C.cast(va, C.c_void_p), C.cast(ea, C.c_void_p),
+                                                 C.byref(shape), m.ctypes.data))
+                self._mass = np.ascontiguousarray(m)
+        return self._mass
+
+    def _require_identity_mass(self, what):
+        if not comp.is_identity_mass(self.mass_matrix):
+            raise A.UnsupportedError(f"{what} is an explicit method and cannot use a mass matrix "
+                                     "(the network has algebraic or scaled states)")
+
+    def variable_symbols(self):
+        """The state symbols in flat state order: ("V", i, sym) / ("E", i, sym) with 1-based
+        component indices, the VIndex / EIndex entries of SII.variable_symbols(nw)."""
+        out = [None] * self.dim
+        v, e = self.ranges()
+        for i, m in enumerate(self.vertexm):
+            for j, s in enumerate(m.sym):
+                out[v[i, 0, 0] - 1 + j] = ("V", i + 1, s)
+        for i, m in enumerate(self.edgem):
+            for j, s in enumerate(m.sym):
+                out[e[i, 0, 0] - 1 + j] = ("E", i + 1, s)
+        return out
+
     # ---- parameters / states ----------------------------------------------------------------
     def default_p(self):
         """Flat parameter vector from the models' defaults (NaN where none)."""
@@ -298,8 +352,11 @@ class Network:
         return J
 
     def rk4(self, u, p, t0, dt, nsteps, work=None):
-        """Fixed-step classic RK4 in place on the device vector u (nsteps steps of dt)."""
+        """Fixed-step classic RK4 in place on the device vector u (nsteps steps of dt).
+        Networks with a mass matrix other than I raise UnsupportedError, as explicit
+        OrdinaryDiffEq methods do."""
         import torch
+        self._require_identity_mass("rk4")
         uptr, nu = _tensor_ptr(u, self.dim, "u", torch)
         if self.pdim > 0:
             pptr, np_ = _tensor_ptr(p, self.pdim, "p", torch)

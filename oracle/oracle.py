@@ -6,6 +6,9 @@ It restates NetworkDynamics.jl's construction (src/construction.jl:31-256,
 src/network_structure.jl:224-289), maps (src/aggregators.jl:54-86,203-235,
 src/gbufs.jl:11-22) and the SequentialExecution coreloop (src/coreloop.jl:1-102);
 see the header of nd_oracle.c for the per-function citations and pinning.
+`OracleNet.mass_matrix` restates construct_mass_matrix (src/construction.jl:409-453)
+in numpy; it is pinned by the known answers of test/construction_test.jl:57-86 and
+test/massmatrix_test.jl:20-47.
 """
 import ctypes as C
 import os
@@ -217,3 +220,37 @@ class OracleNet:
         g = np.empty(max(self.lastidx_gbuf, 1), np.float64)
         lib().ndo_gather(self._h, g, o)
         return g[: self.lastidx_gbuf]
+
+    def mass_matrix(self, vmass, emass):
+        """construct_mass_matrix(im), src/construction.jl:409-453, over this net's own ranges.
+        vmass[nv], emass[ne]: ("I",), ("uniform", s), ("diag", vector) or ("dense", rows).
+        Returns ("identity", None), ("diagonal", vector) or ("dense", matrix)."""
+        v, e = self.ranges()
+        blocks = []                         # (first, last) state range and its dense block
+        for rng, masses in ((v[:, 0, :], vmass), (e[:, 0, :] if self.ne else np.zeros((0, 2), np.int64), emass)):
+            for (first, last), mm in zip(rng, masses):
+                n = int(last - first + 1)
+                kind = mm[0]
+                if kind == "I":
+                    continue
+                if kind == "uniform":       # UniformScaling != I compares the scale only
+                    if float(mm[1]) == 1.0:
+                        continue
+                    block = float(mm[1]) * np.eye(n)
+                elif kind == "diag":
+                    block = np.diag(np.asarray(mm[1], np.float64).reshape(n))
+                elif kind == "dense":
+                    block = np.asarray(mm[1], np.float64).reshape(n, n)
+                else:
+                    raise ValueError(kind)
+                if kind != "uniform" and np.array_equal(block, np.eye(n)):
+                    continue                # Diagonal / Matrix == I entry by entry
+                blocks.append((int(first), n, block))
+        if not blocks:
+            return "identity", None
+        M = np.eye(self.dim)
+        for first, n, block in blocks:
+            M[first - 1:first - 1 + n, first - 1:first - 1 + n] = block
+        if all(np.count_nonzero(b - np.diag(np.diag(b))) == 0 for _, _, b in blocks):
+            return "diagonal", np.diag(M).copy()
+        return "dense", M
