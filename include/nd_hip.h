@@ -289,12 +289,18 @@ typedef struct {
                              docs/examples/cascading_failure.jl, checked at the end of each
                              accepted step): when |edge output| > limit the edge's first
                              parameter (K) is set to 0, in p as well                        */
+    double ss_abstol, ss_reltol; /* steady-state termination, both 0 = off: after an accepted
+                             step, an instance whose every |du_i| <= ss_abstol or
+                             <= ss_reltol*|u_i| stops there with retcode 4 (the test of the
+                             TerminateSteadyState callback with which basin-stability runs
+                             stop at sync).  RK4 pays one extra RHS per step for du.       */
 } nd_ensemble_opts;
 
 typedef struct {
     double t, dt;         /* final time and next step                                           */
     int64_t naccept, nreject, nf;
-    int32_t retcode;      /* 0 success, 1 maxiters, 2 dt below dtmin, 3 NaN step                 */
+    int32_t retcode;      /* 0 success, 1 maxiters, 2 dt below dtmin, 3 NaN step,
+                             4 terminated at a steady state (ss_abstol / ss_reltol)          */
     int32_t ntrip;        /* lines tripped (line_limit set)                                     */
 } nd_ensemble_stat;
 

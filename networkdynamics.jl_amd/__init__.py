@@ -37,3 +37,4 @@ from . import warmstart
 from . import experiments
 from . import ensemble_tune
 from . import linear_analysis
+from .initialization import NetworkInitError, find_fixpoint

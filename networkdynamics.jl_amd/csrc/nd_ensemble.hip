@@ -52,8 +52,10 @@ struct SolveArgs {
     const double* limit;          // per edge (instance order), or null: no line tripping
     nd_ensemble_stat* stats;
     double t0, t1, dt0, reltol, abstol, dtmax, dtmin;
+    double ss_abstol, ss_reltol;  // steady-state termination (ss != 0)
     int64_t maxiters;
     int32_t first, count;
+    int32_t ss;
 };
 
 
@@ -163,6 +165,24 @@ __device__ __forceinline__ int trip_lines(const SolveArgs& a, const InstDesc& I,
     return static_cast<int>(block_sum(cnt, L.red));
 }
 
+// Steady-state test after an accepted step (the allDerivPass test of DiffEqCallbacks'
+// TerminateSteadyState): every |du_i| <= ss_abstol or <= ss_reltol*|u_i|.  Block-uniform.
+template <int V, int D>
+__device__ __forceinline__ bool at_steady_state(const SolveArgs& a, const InstDesc& I, const Lds& L,
+                                                const double (&y)[V][D], const double (&du)[V][D]) {
+    const int T = static_cast<int>(blockDim.x), t = static_cast<int>(threadIdx.x);
+    double fail = 0.0;
+#pragma unroll
+    for (int q = 0; q < V; ++q)
+        if (t + q * T < I.n)
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                const double d = fabs(du[q][j]);
+                if (!(d <= a.ss_abstol || d <= a.ss_reltol * fabs(y[q][j]))) fail = 1.0;
+            }
+    return block_sum(fail, L.red) == 0.0;
+}
+
 // Tsitouras (2011) 5(4) coefficients
 constexpr double c_a21 = 0.161;
 constexpr double c_a31 = -0.008480655492356989, c_a32 = 0.335480655492357;
@@ -230,6 +250,7 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
         double k[V][D], acc[V][D], ys[V][D];
         const int64_t nsteps = static_cast<int64_t>(llround((a.t1 - a.t0) / dt));
         const double half = 0.5 * dt, sixth = dt / 6.0;
+        int64_t done = nsteps;
         for (int64_t step = 0; step < nsteps; ++step) {
             rhs<VM, EM, V>(I, L, y, pv, k);
 #pragma unroll
@@ -257,8 +278,17 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
                 edge_flows<VM, EM, V>(I, L, y);
                 ntrip += trip_lines<VM, EM>(a, I, L);
             }
+            if (a.ss) {
+                rhs<VM, EM, V>(I, L, y, pv, k);
+                ++nf;
+                if (at_steady_state<V, D>(a, I, L, y, k)) {
+                    done = step + 1;
+                    retcode = 4;
+                    break;
+                }
+            }
         }
-        tcur = a.t0 + static_cast<double>(nsteps) * dt;
+        tcur = a.t0 + static_cast<double>(done) * dt;
     } else {
         double k1[V][D], k2[V][D], k3[V][D], k4[V][D], k5[V][D], k6[V][D], k7[V][D], ys[V][D];
         rhs<VM, EM, V>(I, L, y, pv, k1);
@@ -362,6 +392,10 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
                         rhs<VM, EM, V>(I, L, y, pv, k1);
                         ++nf;
                     }
+                }
+                if (a.ss && at_steady_state<V, D>(a, I, L, y, k1)) {
+                    retcode = 4;
+                    break;
                 }
             } else {
                 dt = h / fmin(5.0, q11 / 0.9);
@@ -554,6 +588,11 @@ int make_args(nd_ensemble* en, double* u, double* p, double t0, double t1, const
     a->dtmax = o->dtmax > 0 ? o->dtmax : (t1 - t0);
     a->dtmin = o->dtmin > 0 ? o->dtmin : 1e-14 * std::max(1.0, std::fabs(t1));
     a->maxiters = o->maxiters > 0 ? o->maxiters : 1000000;
+    if (!(o->ss_abstol >= 0.0) || !(o->ss_reltol >= 0.0))
+        return nd::set_error(ND_ERR_ARGUMENT, "steady-state tolerances must be >= 0");
+    a->ss_abstol = o->ss_abstol;
+    a->ss_reltol = o->ss_reltol;
+    a->ss = (o->ss_abstol > 0.0 || o->ss_reltol > 0.0) ? 1 : 0;
     return ND_OK;
 }
 

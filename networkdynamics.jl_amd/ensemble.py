@@ -35,7 +35,12 @@ TSIT5_BTILDE = [-0.00178001105222577714, -0.0008164344596567469, 0.0078808780102
 class _Opts(C.Structure):
     _fields_ = [("method", C.c_int32), ("pad_", C.c_int32), ("reltol", C.c_double), ("abstol", C.c_double),
                 ("dt0", C.c_double), ("dtmax", C.c_double), ("dtmin", C.c_double), ("maxiters", C.c_int64),
-                ("line_limit", C.c_void_p)]
+                ("line_limit", C.c_void_p), ("ss_abstol", C.c_double), ("ss_reltol", C.c_double)]
+
+# retcode of an instance stopped at a steady state
+TERMINATED = 4
+# DiffEqCallbacks' TerminateSteadyState defaults
+STEADY_STATE_TOLS = (1e-8, 1e-6)
 
 
 STAT_DTYPE = np.dtype([("t", "f8"), ("dt", "f8"), ("naccept", "i8"), ("nreject", "i8"), ("nf", "i8"),
@@ -111,20 +116,30 @@ class Ensemble:
         return slice(int(self.param_offsets[i]), int(self.param_offsets[i] + n))
 
     def solve(self, u, p, tspan, method=TSIT5, reltol=1e-8, abstol=1e-8, dt=0.0, dtmax=0.0, maxiters=0,
-              line_limit=None):
+              line_limit=None, steady_state=None):
         """Integrates all instances in place on the device vector u; returns per-instance stats.
         line_limit: per-edge limits (device, instance order) to trip lines whose |flow|
-        exceeds them at the end of an accepted step; tripped lines get K = 0 in p."""
+        exceeds them at the end of an accepted step; tripped lines get K = 0 in p.
+        steady_state: True or (abstol, reltol) to stop each instance after the first
+        accepted step where every |du_i| <= abstol or <= reltol*|u_i| (retcode TERMINATED,
+        stats t = the stopping time), as a TerminateSteadyState callback does."""
         import torch
         self._check(u, p)
         stats = torch.zeros(self.ninst * STAT_DTYPE.itemsize // 8, dtype=torch.float64, device=u.device)
-        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters, line_limit)
+        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters, line_limit, steady_state)
         stream = torch.cuda.current_stream(u.device).cuda_stream
         A.check(A.lib().nd_ensemble_solve(self._h, u.data_ptr(), p.data_ptr(), float(tspan[0]), float(tspan[1]),
                                           C.byref(o), stats.data_ptr(), stream))
         return stats
 
-    def _opts(self, method, reltol, abstol, dt, dtmax, maxiters, line_limit=None):
+    def _opts(self, method, reltol, abstol, dt, dtmax, maxiters, line_limit=None, steady_state=None):
+        ss = (0.0, 0.0)
+        if steady_state is True:
+            ss = STEADY_STATE_TOLS
+        elif steady_state:
+            ss = tuple(float(x) for x in steady_state)
+            if len(ss) != 2 or min(ss) < 0:
+                raise A.ArgumentError("steady_state must be True or (abstol, reltol) with tolerances >= 0")
         lim = None
         if line_limit is not None:
             import torch
@@ -132,7 +147,8 @@ class Ensemble:
                     line_limit.numel() != int(self.ne.sum()):
                 raise A.ArgumentError("line_limit must be a float64 device vector with one entry per edge")
             lim = line_limit.data_ptr()
-        return _Opts(int(method), 0, float(reltol), float(abstol), float(dt), float(dtmax), 0.0, int(maxiters), lim)
+        return _Opts(int(method), 0, float(reltol), float(abstol), float(dt), float(dtmax), 0.0, int(maxiters), lim,
+                     ss[0], ss[1])
 
     def _check(self, u, p):
         import torch
@@ -142,12 +158,12 @@ class Ensemble:
             raise A.ArgumentError("u / p do not match the ensemble's sizes")
 
     def capture(self, u, p, tspan, method=TSIT5, reltol=1e-8, abstol=1e-8, dt=0.0, dtmax=0.0, maxiters=0,
-                line_limit=None):
+                line_limit=None, steady_state=None):
         """Captures the solve of (u, p, tspan) into a hipGraph; `.launch()` replays it in place on u."""
         import torch
         self._check(u, p)
         stats = torch.zeros(self.ninst * STAT_DTYPE.itemsize // 8, dtype=torch.float64, device=u.device)
-        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters, line_limit)
+        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters, line_limit, steady_state)
         h = C.c_void_p()
         A.check(A.lib().nd_ensemble_graph_create(self._h, u.data_ptr(), p.data_ptr(), float(tspan[0]),
                                                  float(tspan[1]), C.byref(o), stats.data_ptr(), C.byref(h)))

@@ -86,7 +86,8 @@ def c4_instances(ninst, seed=1, nmin=100, nmax=2000):
     return out, rng
 
 
-def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_seconds=30.0, seed=1, tune=0):
+def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_seconds=30.0, seed=1, tune=0,
+                steady_state_tol=1e-5):
     import torch
     t0 = time.perf_counter()
     inst, rng = c4_instances(ninst, seed)
@@ -146,6 +147,22 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
     t_graph = time.perf_counter() - tg
     graph_equal = bool(torch.equal(u_g, u_d))
     del graph
+    # basin-stability mode: each instance stops once settled (every |du_i| <= tol); the
+    # ones that do not return to sync run to t1
+    ss = None
+    if steady_state_tol > 0:
+        u_s = torch.from_numpy(np.concatenate(us)).to(dev)
+        torch.cuda.synchronize()
+        tss = time.perf_counter()
+        sst = Ensemble.stats_to_numpy(ens.solve(u_s, p_d, (0.0, t1), method=TSIT5, reltol=reltol, abstol=abstol,
+                                                steady_state=(steady_state_tol, 0.0)))
+        torch.cuda.synchronize()
+        t_ss = time.perf_counter() - tss
+        term = sst["retcode"] == 4
+        ss = dict(tol=steady_state_tol, seconds=t_ss, traj_per_s=len(graphs) / t_ss, terminated=int(term.sum()),
+                  terminated_and_synced=int((term & sync).sum()), mean_stop_t=float(sst["t"][term].mean())
+                  if term.any() else None, steps_mean=float(sst["naccept"].mean()))
+        _log(f"steady-state termination: {int(term.sum())} stopped early, {t_ss:.2f} s")
     # plain per-instance GPU path on a time-bounded sample (largest-first order not used: random sample)
     _log(f"ensemble: {len(graphs)} trajectories in {t_ens:.2f} s")
     order = rng.permutation(len(graphs))
@@ -174,10 +191,10 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
                 graph_replay_s=t_graph, graph_replay_traj_per_s=len(graphs) / t_graph,
                 graph_replay_equals_eager=graph_equal,
                 ensemble_steps_mean=float(est["naccept"].mean()), ensemble_rejects_mean=float(est["nreject"].mean()),
-                ensemble_retcodes=np.bincount(est["retcode"], minlength=4).tolist(),
+                ensemble_retcodes=np.bincount(est["retcode"], minlength=5).tolist(),
                 returned_to_sync=int(sync.sum()), baseline_instances=done, baseline_s=t_base,
                 baseline_traj_per_s=traj_base, baseline_steps_mean=float(np.mean(steps)),
-                speedup=traj_ens / traj_base, classes=[list(c) for c in ens.classes],
+                speedup=traj_ens / traj_base, classes=[list(c) for c in ens.classes], steady_state=ss,
                 tuning=None if tuning is None else dict(predicted_s=tuning["predicted_s"], bins=tuning["bins"]))
 
 

@@ -264,6 +264,18 @@ class Network:
                 out[e[i, 0, 0] - 1 + j] = ("E", i + 1, s)
         return out
 
+    def parameter_symbols(self):
+        """The parameter symbols in flat parameter order, like variable_symbols."""
+        out = [None] * self.pdim
+        v, e = self.ranges()
+        for i, m in enumerate(self.vertexm):
+            for j, s in enumerate(m.psym):
+                out[v[i, 2, 0] - 1 + j] = ("V", i + 1, s)
+        for i, m in enumerate(self.edgem):
+            for j, s in enumerate(m.psym):
+                out[e[i, 3, 0] - 1 + j] = ("E", i + 1, s)
+        return out
+
     # ---- parameters / states ----------------------------------------------------------------
     def default_p(self):
         """Flat parameter vector from the models' defaults (NaN where none)."""

@@ -140,3 +140,75 @@ def test_dae_linearization_is_the_kron_reduction(ND, dev):
     pf = LA.participation_factors(sys)
     np.testing.assert_allclose(pf["pfactors"].sum(axis=0), 1.0, rtol=1e-9)
     assert pf["sym"] == red.sym
+
+
+def _ring_network(ND, P, K=6.0):
+    # test/initialization_test.jl:15-38: swing vertices (M = 1, D = 0.1) on
+    # watts_strogatz(4, 2, 0.0) -- the 4-cycle -- with K = 6 lines
+    g = ND.cycle_graph(4)
+    nw = ND.Network(g, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    v, e = nw.ranges()
+    p = np.zeros(nw.pdim)
+    for i in range(4):
+        p[v[i, 2, 0] - 1: v[i, 2, 1]] = [1.0, 0.1, P[i]]     # [M, D, Pm]
+    p[e[:, 3, 0] - 1] = K
+    return g, nw, p
+
+
+@pytest.mark.gpu
+def test_find_fixpoint_ring_known_answer(ND, dev):
+    # P = [1, -1, 1, -1]: every line carries 1/2, so K sin(theta_src - theta_dst) = +-1/2
+    g, nw, p = _ring_network(ND, [1.0, -1.0, 1.0, -1.0])
+    x, info = ND.find_fixpoint(nw, np.zeros(nw.dim), p)
+    assert info["residual"] <= 1e-10 and info["iters"] <= 10
+    u = x.cpu().numpy()
+    v, _ = nw.ranges()
+    th, om = u[v[:, 0, 0] - 1], u[v[:, 0, 0]]
+    np.testing.assert_allclose(om, 0.0, atol=1e-12)
+    flows = 6.0 * np.sin(th[g.src - 1] - th[g.dst - 1])
+    np.testing.assert_allclose(np.abs(flows), 0.5, atol=1e-11)
+    np.testing.assert_allclose(np.abs(np.diff(th)), np.arcsin(0.5 / 6.0), atol=1e-11)
+    import torch
+    assert ND.linear_analysis.isfixpoint(nw, x, torch.from_numpy(p).to(dev))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dense_max", [4096, 0])
+def test_find_fixpoint_dae_grid_matches_batched_solver(ND, dev, dense_max):
+    # algebraic loads (mass 0), flat start; the dense least-squares and the JVP-GMRES steps
+    # both reach the batched solver's fixpoint up to the global phase
+    import torch
+    grid = ND.swing_load_grid(300, seed=47)
+    n = grid.g.nv()
+    th = torch.zeros(n, dtype=torch.float64, device=dev)
+    fs = ND.FixpointSolver([grid.g])
+    st = ND.FixpointSolver.stats_to_numpy(fs.solve(th, torch.from_numpy(grid.P).to(dev),
+                                                   torch.from_numpy(grid.scale).to(dev),
+                                                   torch.from_numpy(grid.K).to(dev)))
+    assert st["retcode"][0] == 0
+    nw = grid.network(algebraic_loads=True)
+    x, info = ND.find_fixpoint(nw, np.zeros(nw.dim), grid.flat_p(), dense_max=dense_max)
+    assert info["residual"] <= 1e-10
+    v, _ = nw.ranges()
+    got = x.cpu().numpy()[v[:, 0, 0] - 1]
+    ref = th.cpu().numpy()
+    np.testing.assert_allclose(got - got.mean(), ref - ref.mean(), atol=1e-8)
+
+
+@pytest.mark.gpu
+def test_find_fixpoint_errors(ND, dev):
+    g, nw, p = _ring_network(ND, [1.0, -1.0, 1.0, -1.0])
+    x0 = np.zeros(nw.dim)
+    x0[2] = np.nan
+    with pytest.raises(ND.NetworkInitError, match=r"Variables:  VIndex\(2, :delta\)"):
+        ND.find_fixpoint(nw, x0, p)
+    p2 = p.copy()
+    p2[-1] = np.nan
+    with pytest.raises(ND.NetworkInitError, match=r"Parameters: EIndex\(4, :K\)"):
+        ND.find_fixpoint(nw, np.zeros(nw.dim), p2)
+    # injections beyond what K = 0.1 lines can carry: no synchronous state exists
+    _, nw3, p3 = _ring_network(ND, [1.0, -1.0, 1.0, -1.0], K=0.1)
+    with pytest.raises(ND.NetworkInitError, match="Could not find fixpoint"):
+        ND.find_fixpoint(nw3, np.zeros(nw3.dim), p3, maxiters=30)
+    with pytest.raises(ND.ArgumentError):
+        ND.find_fixpoint(nw, np.zeros(nw.dim + 1), p)

@@ -301,3 +301,99 @@ def test_ensemble_tsit5_line_tripping_consistent(ND, dev):
         sl = ens.param_slice(i)
         assert changed[sl][:3 * g.nv()].sum() == 0
         assert changed[sl].sum() == st["ntrip"][i]
+
+
+def _damped_near_sync(ND, gs, rng):
+    # M = 1, D = 2, small balanced injections, start at theta = 0: every instance settles
+    # to its synchronous state within a few time units
+    ps, us = [], []
+    for g in gs:
+        n = g.nv()
+        P = rng.normal(scale=0.1, size=n)
+        P -= P.mean()
+        p = np.empty(3 * n + g.ne())
+        p[0:3 * n:3], p[1:3 * n:3], p[2:3 * n:3] = 1.0, 2.0, P
+        p[3 * n:] = rng.uniform(5.0, 10.0, g.ne())
+        u = np.zeros(2 * n)
+        u[1::2] = rng.normal(scale=0.05, size=n)
+        ps.append(p)
+        us.append(u)
+    return ps, us
+
+
+def _ss_ratio(du, u, abstol, reltol):
+    """max_i of how far |du_i| is from passing (<= 1 passes): the allDerivPass test."""
+    a = np.abs(du)
+    return float(np.max(np.minimum(a / abstol, np.where(reltol * np.abs(u) > 0, a / (reltol * np.abs(u) + 1e-300),
+                                                           np.inf))))
+
+
+@pytest.mark.gpu
+def test_ensemble_rk4_steady_state_termination_matches_oracle(ND, dev):
+    # TerminateSteadyState semantics: after every RK4 step, stop at the first state where
+    # every |du_i| <= abstol or <= reltol*|u_i|.  Reference: the oracle stepped one RK4
+    # step at a time with the same test on the host.
+    import torch
+    gs, rng = _instances(ND, 12, 40, 400, seed=21)
+    ps, us = _damped_near_sync(ND, gs, rng)
+    ens = ND.Ensemble(gs, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    u_d = torch.from_numpy(np.concatenate(us)).to(dev)
+    p_d = torch.from_numpy(np.concatenate(ps)).to(dev)
+    dt, nsteps, tols = 0.01, 4000, (1e-6, 1e-9)
+    st = ND.Ensemble.stats_to_numpy(ens.solve(u_d, p_d, (0.0, dt * nsteps), method=ND.ensemble.RK4, dt=dt,
+                                              steady_state=tols))
+    torch.cuda.synchronize()
+    got = u_d.cpu().numpy()
+    assert (st["retcode"] == ND.ensemble.TERMINATED).all(), st["retcode"]
+    checked = 0
+    for i, g in enumerate(gs):
+        on = oracle_for(ND.Network(g, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge()))
+        u = us[i].copy()
+        for k in range(1, nsteps + 1):
+            u = on.rk4(u, ps[i], 0.0, dt, 1)
+            r = _ss_ratio(on.rhs(u, ps[i]), u, *tols)
+            if r <= 1.0:
+                break
+        steps = int(round(st["t"][i] / dt))
+        assert st["naccept"][i] == steps
+        if abs(r - 1.0) > 1e-6:                     # a decision this close to the boundary may round either way
+            assert steps == k, (i, steps, k)
+            np.testing.assert_allclose(got[ens.state_slice(i)], u, rtol=0, atol=1e-10)
+            checked += 1
+    assert checked >= 10
+
+
+@pytest.mark.gpu
+def test_ensemble_tsit5_steady_state_termination(ND, dev):
+    # adaptive steps: every instance stops before t1 with retcode TERMINATED at a state that
+    # passes the test by the oracle's right-hand side; off, the same instances run to t1.
+    # Near the equilibrium the step size settles at the stability limit of the fastest
+    # modes, where their residual oscillation is held at the integration tolerance: the
+    # steady-state tolerance sits well above it.
+    import torch
+    gs, rng = _instances(ND, 16, 50, 800, seed=22)
+    ps, us = _damped_near_sync(ND, gs, rng)
+    ens = ND.Ensemble(gs, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    u0 = np.concatenate(us)
+    p_d = torch.from_numpy(np.concatenate(ps)).to(dev)
+    tols = (1e-5, 1e-9)
+    u_d = torch.from_numpy(u0).to(dev)
+    st = ND.Ensemble.stats_to_numpy(ens.solve(u_d, p_d, (0.0, 200.0), steady_state=tols))
+    u_full = torch.from_numpy(u0).to(dev)
+    st_full = ND.Ensemble.stats_to_numpy(ens.solve(u_full, p_d, (0.0, 200.0)))
+    torch.cuda.synchronize()
+    got, full = u_d.cpu().numpy(), u_full.cpu().numpy()
+    final = [_ss_ratio(oracle_for(ND.Network(g, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())).rhs(
+        full[ens.state_slice(i)], ps[i]), full[ens.state_slice(i)], *tols) for i, g in enumerate(gs)]
+    assert (st["retcode"] == ND.ensemble.TERMINATED).all() and (st["t"] < 200.0).all(), (st["retcode"], final)
+    assert (st_full["retcode"] == 0).all() and (st_full["t"] == 200.0).all()
+    assert (st["nf"] < st_full["nf"]).all()
+    for i, g in enumerate(gs):
+        on = oracle_for(ND.Network(g, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge()))
+        u = got[ens.state_slice(i)]
+        assert _ss_ratio(on.rhs(u, ps[i]), u, *tols) <= 1.0 + 1e-9
+        # both runs sit at the same synchronous state (angles up to the common drift)
+        th, thf = u[0::2], full[ens.state_slice(i)][0::2]
+        np.testing.assert_allclose(th - th.mean(), thf - thf.mean(), atol=1e-4)
+    with pytest.raises(ND.ArgumentError):
+        ens.solve(u_d, p_d, (0.0, 1.0), steady_state=(-1.0, 0.0))
