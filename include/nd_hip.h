@@ -64,6 +64,9 @@ enum {
  *   SWING              test/ComponentLibrary.jl:59-67       ddelta = w; dw = 1/M*(Pm - D*w + acc), p = [M, D, Pm]
  *   SWING_UNIT_DAMPING benchmark/benchmark_models.jl:37-42  dw = P - 1.0*w; dw += acc,   p = [P]
  *   LOAD               (no reference twin) first-order load  dtheta = 1/tau*(P + acc),    p = [tau, P]
+ *   CAPACITOR_SINK     docs/src/external_inputs.md:76-82     dV = 1/C*(acc - (1 + 0.1*sin(t))), p = [C]
+ *   PI_SOURCE          docs/src/external_inputs.md:86-99     d = Vref - ext[1]; dxi = d;
+ *                      (external input: a remote voltage)    dV = 1/C*(acc + (Kp*d + Ki*xi)), p = [C, Vref, Ki, Kp]
  *   NOOP               f does nothing (aggregator tests, test/aggregators_test.jl:15-18)  */
 enum {
     ND_VM_NOOP = 0,
@@ -71,7 +74,9 @@ enum {
     ND_VM_KURAMOTO = 2,
     ND_VM_SWING = 3,
     ND_VM_SWING_UNIT_DAMPING = 4,
-    ND_VM_LOAD = 5
+    ND_VM_LOAD = 5,
+    ND_VM_CAPACITOR_SINK = 6,
+    ND_VM_PI_SOURCE = 7
 };
 
 /* Edge models.  Reference twins:
@@ -83,6 +88,7 @@ enum {
  *   ODE_SIN_DIFFUSION    test/ComponentLibrary.jl:30-40        de = 1/tau*(sin(+-(v_s - v_d)) - e), p = [tau]
  *   ODE_SIN_DIFFUSION_FAST benchmark/benchmark_models.jl:11-16 de = 100.0*(sin(+-(v_s - v_d)) - e)
  *   ODE_LINEAR           test/diffusion_test.jl:102-107        de = +-(v_s - v_d) - e
+ *   RESISTOR             docs/src/external_inputs.md:66-72     y = (v_s - v_d)/R,   p = [R]
  *   NOOP                 g does nothing (test/aggregators_test.jl:20-30)                    */
 enum {
     ND_EM_NOOP = 0,
@@ -93,14 +99,17 @@ enum {
     ND_EM_DIFFUSION_FIDUCIAL = 5,
     ND_EM_ODE_SIN_DIFFUSION = 6,
     ND_EM_ODE_SIN_DIFFUSION_FAST = 7,
-    ND_EM_ODE_LINEAR = 8
+    ND_EM_ODE_LINEAR = 8,
+    ND_EM_RESISTOR = 9
 };
 
 #define ND_MAX_OUT 4
+#define ND_MAX_EXT 8
 
 /* One component kind: what the reference's ComponentBatch type parameters
  * carry (network_structure.jl:176-198): dims, ff trait, wrapper, and the
- * StateMask indices of a PureStateMap output (1-based, like StateMask).
+ * StateMask indices of a PureStateMap output (1-based, like StateMask), and the
+ * number of external inputs (extdim, the ComponentBatch's extbufstride).
  * Vertices use outdim_dst as their outdim and outdim_src = 0. */
 typedef struct {
     int32_t is_edge;
@@ -113,6 +122,7 @@ typedef struct {
     int32_t outdim_dst;
     int32_t mask_src[ND_MAX_OUT];
     int32_t mask_dst[ND_MAX_OUT];
+    int32_t extdim;
 } nd_kind;
 
 /* ---- host index layer ------------------------------------------------------ */
@@ -144,6 +154,7 @@ typedef struct {
     int64_t agg_last;
     int32_t vdepth, edepth;
     int32_t nvbatches, nebatches;
+    int64_t lastidx_extbuf; /* external-input buffer length (0: no external inputs) */
 } nd_network_info;
 
 int nd_network_get_info(const nd_network* nw, nd_network_info* info);
@@ -178,6 +189,21 @@ int nd_network_export_maps(const nd_network* nw, int64_t* aggmap, int64_t* gbufm
 int nd_network_csr_size(const nd_network* nw, int64_t* nnz);
 int nd_network_export_csr(const nd_network* nw, int64_t* rowptr, int64_t* cols);
 
+/* ---- external inputs (src/external_inputs.jl) -------------------------------- */
+
+/* v_ext / e_ext ranges of the IndexManager (network_structure.jl:224-258), 1-based
+ * (first, last) pairs: v_ext[nv][2], e_ext[ne][2] (either may be NULL). */
+int nd_network_export_ext_ranges(const nd_network* nw, int64_t* v_ext, int64_t* e_ext);
+
+/* ExtMap(im), external_inputs.jl:10-46: for each external-buffer index k (1-based),
+ * map[k-1] > 0 is StateBufIdx(map[k-1]) (an index into u) and map[k-1] < 0 is
+ * OutBufIdx(-map[k-1]) (an index into the output buffer o).  Required before
+ * nd_plan_create when the network has external inputs.  ND_ERR_ARGUMENT: no
+ * external inputs ("There are no external inputs."), len != lastidx_extbuf, an index
+ * out of range, or an output of a feed-forward component
+ * (external_inputs.jl:38-41). */
+int nd_network_set_extmap(nd_network* nw, int64_t len, const int64_t* map);
+
 /* One component's mass matrix, in the forms VertexModel/EdgeModel normalise a user's
  * value to (component_functions.jl:948-963): I, UniformScaling(m[0]), Diagonal(m[0:dim])
  * or a dense dim x dim matrix (column-major, m[0:dim*dim]). */
@@ -205,7 +231,10 @@ typedef struct nd_plan nd_plan;
 
 /* Uploads the network's index structures to the current HIP device and selects
  * the fused kernels.  ND_ERR_UNSUPPORTED if a model has no HIP kernel (external
- * inputs, loopback, feed-forward vertices, output depth > 1 with a functional model). */
+ * inputs of edges, loopback, feed-forward vertices, output depth > 1 with a functional
+ * model); ND_ERR_ARGUMENT if the network has external inputs and no ExtMap
+ * (nd_network_set_extmap).  Vertex external inputs resolve to state indices: a
+ * vertex output and an ODE edge's output are StateMasks of states. */
 int nd_plan_create(const nd_network* nw, nd_plan** out);
 void nd_plan_destroy(nd_plan* plan);
 
@@ -252,6 +281,11 @@ int nd_get_buffers(nd_plan* plan, double* o, double* aggbuf, const double* u,
  * length dim. */
 int nd_rhs_jvp(nd_plan* plan, double* du, double* jv, const double* u, const double* v, const double* p, double t,
                int64_t len_du, int64_t len_u, int64_t len_p, void* stream);
+
+/* collect_externals!(map, extbuf, u, o), external_inputs.jl:49-77:
+ * extbuf[k] = u[i] for StateBufIdx(i), o[i] for OutBufIdx(i).  extbuf: lastidx_extbuf
+ * doubles on the device; o: the output buffer (nd_get_buffers). */
+int nd_collect_externals(nd_plan* plan, double* extbuf, const double* u, const double* o, void* stream);
 
 /* gather!(EagerGBufProvider, gbuf, o): gbuf[k] = o[map[k]], gbufs.jl:25. */
 int nd_gather(nd_plan* plan, double* gbuf, const double* o, void* stream);

@@ -16,9 +16,9 @@ from .ensemble import Ensemble, tsit5_per_instance
 from .fixpoint import FixpointSolver, swing_load_grid, isfixpoint_residual
 from . import fixpoint
 from ._abi import ArgumentError, DeviceError, NotConvergedError, UnsupportedError
-from .components import (AntiSymmetric, DeviceFunction, Diagonal, Directed, EdgeModel, FeedForward, Fiducial, I,
-                         NoFeedForward, PureFeedForward, PureStateMap, StateMask, Symmetric, UniformScaling,
-                         VertexModel, fftype, hasff)
+from .components import (AntiSymmetric, DeviceFunction, Diagonal, Directed, EdgeModel, EIndex, FeedForward,
+                         Fiducial, I, NoFeedForward, PureFeedForward, PureStateMap, StateMask, Symmetric,
+                         UniformScaling, VertexModel, VIndex, fftype, hasff)
 from .graphs import (SimpleDiGraph, SimpleGraph, barabasi_albert, complete_graph, cycle_graph, erdos_renyi,
                      path_graph, random_geometric, star_digraph, watts_strogatz)
 from .network import (Aggregator, ExecutionStyle, HIPExecution, Network, SegmentedAggregator, usebuffer)

@@ -13,10 +13,13 @@ ND_OK, ND_ERR_ARGUMENT, ND_ERR_UNSUPPORTED, ND_ERR_DEVICE, ND_ERR_ALLOC, ND_ERR_
 
 ND_WRAP_NONE, ND_WRAP_ANTISYMMETRIC, ND_WRAP_SYMMETRIC, ND_WRAP_DIRECTED, ND_WRAP_FIDUCIAL = range(5)
 ND_FF_PURESTATEMAP, ND_FF_PUREFEEDFORWARD, ND_FF_FEEDFORWARD, ND_FF_NOFEEDFORWARD = range(4)
-(ND_VM_NOOP, ND_VM_DIFFUSION, ND_VM_KURAMOTO, ND_VM_SWING, ND_VM_SWING_UNIT_DAMPING, ND_VM_LOAD) = range(6)
+(ND_VM_NOOP, ND_VM_DIFFUSION, ND_VM_KURAMOTO, ND_VM_SWING, ND_VM_SWING_UNIT_DAMPING, ND_VM_LOAD,
+ ND_VM_CAPACITOR_SINK, ND_VM_PI_SOURCE) = range(8)
 (ND_EM_NOOP, ND_EM_DIFFUSION, ND_EM_DIFFUSION_WEIGHTED, ND_EM_KURAMOTO, ND_EM_STATIC_LINE,
- ND_EM_DIFFUSION_FIDUCIAL, ND_EM_ODE_SIN_DIFFUSION, ND_EM_ODE_SIN_DIFFUSION_FAST, ND_EM_ODE_LINEAR) = range(9)
+ ND_EM_DIFFUSION_FIDUCIAL, ND_EM_ODE_SIN_DIFFUSION, ND_EM_ODE_SIN_DIFFUSION_FAST, ND_EM_ODE_LINEAR,
+ ND_EM_RESISTOR) = range(10)
 ND_MAX_OUT = 4
+ND_MAX_EXT = 8
 ND_MASS_IDENTITY, ND_MASS_UNIFORM, ND_MASS_DIAGONAL, ND_MASS_DENSE = range(4)
 
 
@@ -40,7 +43,8 @@ class NDKind(C.Structure):
     _fields_ = [("is_edge", C.c_int32), ("model", C.c_int32), ("wrapper", C.c_int32),
                 ("ff", C.c_int32), ("dim", C.c_int32), ("pdim", C.c_int32),
                 ("outdim_src", C.c_int32), ("outdim_dst", C.c_int32),
-                ("mask_src", C.c_int32 * ND_MAX_OUT), ("mask_dst", C.c_int32 * ND_MAX_OUT)]
+                ("mask_src", C.c_int32 * ND_MAX_OUT), ("mask_dst", C.c_int32 * ND_MAX_OUT),
+                ("extdim", C.c_int32)]
 
 
 class NDNetworkInfo(C.Structure):
@@ -48,7 +52,8 @@ class NDNetworkInfo(C.Structure):
                 ("lastidx_out", C.c_int64), ("lastidx_aggr", C.c_int64), ("lastidx_gbuf", C.c_int64),
                 ("agg_first", C.c_int64), ("agg_last", C.c_int64),
                 ("vdepth", C.c_int32), ("edepth", C.c_int32),
-                ("nvbatches", C.c_int32), ("nebatches", C.c_int32)]
+                ("nvbatches", C.c_int32), ("nebatches", C.c_int32),
+                ("lastidx_extbuf", C.c_int64)]
 
 
 class NDBatch(C.Structure):
@@ -82,6 +87,9 @@ PROTOTYPES = {
     "nd_network_csr_size": (C.c_int, [P, C.POINTER(I64)]),
     "nd_network_export_csr": (C.c_int, [P, P, P]),
     "nd_network_mass_matrix": (C.c_int, [P, P, P, C.POINTER(I32), P]),
+    "nd_network_export_ext_ranges": (C.c_int, [P, P, P]),
+    "nd_network_set_extmap": (C.c_int, [P, I64, P]),
+    "nd_collect_externals": (C.c_int, [P, P, P, P, P]),
     "nd_plan_create": (C.c_int, [P, C.POINTER(P)]),
     "nd_plan_destroy": (None, [P]),
     "nd_plan_tune": (C.c_int, [P, C.POINTER(NDPlanTuning)]),

@@ -60,8 +60,9 @@ class DeviceFunction:
     g(osrc, odst, v_s, v_d, p, t)), or "edge_f" (state function of an ODE edge).
     """
 
-    def __init__(self, name, role, model, dim=None, pdim=0, ref=""):
+    def __init__(self, name, role, model, dim=None, pdim=0, ref="", ext=0):
         self.name, self.role, self.model, self.dim, self.pdim, self.ref = name, role, model, dim, pdim, ref
+        self.ext = ext          # external inputs the function reads
 
     def __repr__(self):
         return f"<device function {self.name}>"
@@ -125,6 +126,41 @@ class Fiducial:
             raise A.ArgumentError("Fiducial needs src and dst")
         self.src = _as_mask_or_fn(src)
         self.dst = _as_mask_or_fn(dst)
+
+
+class VIndex:
+    """VIndex(component, sym): a vertex state or output by vertex index (1-based) or unique
+    name, and symbol (or 1-based state index) -- the references an `extin` list holds."""
+
+    kind = "V"
+
+    def __init__(self, comp, sym):
+        self.comp, self.sym = comp, sym
+
+    def __repr__(self):
+        return f"{type(self).__name__}({self.comp!r}, {self.sym!r})"
+
+
+class EIndex(VIndex):
+    """EIndex(component, sym): the edge counterpart of VIndex."""
+
+    kind = "E"
+
+
+def _extin(extin, f):
+    if extin is None:
+        refs = []
+    else:
+        refs = list(extin)
+        for r in refs:
+            if not isinstance(r, VIndex):
+                raise A.ArgumentError(f"external inputs are VIndex / EIndex references, got {r!r}")
+    need = getattr(f, "ext", 0) if f is not None else 0
+    if f is not None and f.model not in (A.ND_VM_NOOP, A.ND_EM_NOOP) and len(refs) != need:
+        raise A.ArgumentError(f"{f.name} reads {need} external input(s), extin has {len(refs)}")
+    if len(refs) > A.ND_MAX_EXT:
+        raise A.UnsupportedError(f"at most {A.ND_MAX_EXT} external inputs per component")
+    return refs
 
 
 class UniformScaling:
@@ -253,7 +289,7 @@ class ComponentModel:
         """The nd_kind description handed to the C ABI."""
         return dict(is_edge=int(self.is_edge), model=self.model, wrapper=self.wrapper, ff=self.ff._id,
                     dim=self.dim, pdim=self.pdim, outdim_src=self.outdim_src, outdim_dst=self.outdim_dst,
-                    mask_src=tuple(self.mask_src), mask_dst=tuple(self.mask_dst))
+                    mask_src=tuple(self.mask_src), mask_dst=tuple(self.mask_dst), extdim=len(self.extin))
 
     def component_hash(self):
         """What batches components together (_component_hash, construction.jl:245-256):
@@ -277,9 +313,7 @@ class VertexModel(ComponentModel):
     is_edge = False
 
     def __init__(self, f=None, g=None, dim=None, sym=None, pdim=None, psym=None, outdim=None, name=None,
-                 mass_matrix=None, **kwargs):
-        if kwargs.get("extin") is not None:
-            raise A.UnsupportedError("external inputs (extin) have no HIP kernel")
+                 mass_matrix=None, extin=None):
         if f is None:
             f = noop_vertex
         if not isinstance(f, DeviceFunction) or f.role != "vertex_f":
@@ -326,6 +360,7 @@ class VertexModel(ComponentModel):
         self.mask_src, self.mask_dst = (), g.idxs
         self.name = name or f.name
         self.mass_matrix = _normalize_mass(mass_matrix, self.dim)
+        self.extin = _extin(extin, f)
 
     def __repr__(self):
         return f"VertexModel :{self.name} (dim={self.dim}, pdim={self.pdim}, f={self.f.name})"
@@ -337,9 +372,7 @@ class EdgeModel(ComponentModel):
     is_edge = True
 
     def __init__(self, f=None, g=None, dim=None, sym=None, pdim=None, psym=None, outdim=None, outsym=None,
-                 name=None, mass_matrix=None, **kwargs):
-        if kwargs.get("extin") is not None:
-            raise A.UnsupportedError("external inputs (extin) have no HIP kernel")
+                 name=None, mass_matrix=None, extin=None):
         if g is None:
             raise A.ArgumentError("EdgeModel needs g")
         if f is not None and (not isinstance(f, DeviceFunction) or f.role != "edge_f"):
@@ -422,7 +455,9 @@ class EdgeModel(ComponentModel):
             raise A.ArgumentError(f"{(fn or f).name} needs pdim={natural_p}")
         self.name = name or (fn or f).name
         self._gfn = fn
+        self.outsym = [str(x) for x in outsym] if outsym is not None else None
         self.mass_matrix = _normalize_mass(mass_matrix, self.dim)
+        self.extin = _extin(extin, None)
 
     def __repr__(self):
         return f"EdgeModel :{self.name} (dim={self.dim}, pdim={self.pdim}, outdim={self.outdim})"
@@ -461,3 +496,7 @@ diffusion_dedge_fast = DeviceFunction("diffusion_dedge_fast!", "edge_f", A.ND_EM
                                       "benchmark/benchmark_models.jl:11-15")
 real_ode_edge = DeviceFunction("real_ode_edge!", "edge_f", A.ND_EM_ODE_LINEAR, 2, 0,
                                "test/diffusion_test.jl:102-106")
+resistor_g = DeviceFunction("resistor_g", "edge_g", A.ND_EM_RESISTOR, 0, 1, "docs/src/external_inputs.md:66-72")
+f_load = DeviceFunction("f_load", "vertex_f", A.ND_VM_CAPACITOR_SINK, 1, 1, "docs/src/external_inputs.md:76-82")
+f_source = DeviceFunction("f_source", "vertex_f", A.ND_VM_PI_SOURCE, 2, 4, "docs/src/external_inputs.md:86-99",
+                          ext=1)

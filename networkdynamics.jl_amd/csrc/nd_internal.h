@@ -22,6 +22,7 @@ struct Batch {
     int64_t param_first = 1;          // pstride.first
     int64_t out_first = 1;            // outbufstride.first
     int64_t in_first = 1;             // inbufstride.first (v_aggr or e_gbufr)
+    int64_t ext_first = 1;            // extbufstride.first (v_ext or e_ext)
 };
 
 // Host-side index layer: what the reference's IndexManager + batches hold,
@@ -36,7 +37,10 @@ struct Network {
     std::vector<int64_t> v_pos, e_pos;       // 0-based position inside its batch
     int32_t vdepth = 0, edepth = 0;
     int64_t dim = 0, pdim = 0, lastidx_out = 0, lastidx_aggr = 0, lastidx_gbuf = 0;
+    int64_t lastidx_extbuf = 0;
     int64_t agg_first = 1, agg_last = 0;
+    // ExtMap (external_inputs.jl:10-46): > 0 StateBufIdx (u), < 0 OutBufIdx (o), 1-based
+    std::vector<int64_t> extmap;
 
     // Destination-ordered reduction lists over aggregation indices (0-based rows):
     // row a lists the 1-based output indices summed into aggbuf[a+1], increasing.
@@ -70,7 +74,17 @@ struct Network {
     int64_t e_gbuf_first(int64_t e) const {
         const Batch& b = eb[e_batch[e]]; return b.in_first + e_pos[e] * 2 * vdepth;
     }
+    int64_t v_ext_first(int64_t v) const {
+        const Batch& b = vb[v_batch[v]]; return b.ext_first + v_pos[v] * kinds[b.kind].extdim;
+    }
+    int64_t e_ext_first(int64_t e) const {
+        const Batch& b = eb[e_batch[e]]; return b.ext_first + e_pos[e] * kinds[b.kind].extdim;
+    }
 };
+
+// u-index (0-based) of output-buffer index k (1-based) when that output is a StateMask
+// of a state (vertex outputs; ODE-edge Fiducial outputs), else -1.
+int64_t output_state_index(const Network& n, int64_t k);
 
 // Vertex output u-index (0-based) of vertex v, output element j: StateMask outputs only.
 inline int64_t vertex_out_u_index(const Network& n, int64_t v, int j) {

@@ -25,6 +25,7 @@ __device__ __forceinline__ Dual operator-(double c, Dual a) { return {c - a.v, -
 __device__ __forceinline__ Dual operator*(double c, Dual a) { return {c * a.v, c * a.d}; }
 __device__ __forceinline__ Dual operator*(Dual a, double c) { return {a.v * c, a.d * c}; }
 __device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, double c) { return {a.v / c, a.d / c}; }
 __device__ __forceinline__ Dual sin(Dual a) { return {::sin(a.v), ::cos(a.v) * a.d}; }
 __device__ __forceinline__ Dual dual_nan() { return {__builtin_nan(""), __builtin_nan("")}; }
 
@@ -47,6 +48,7 @@ __device__ __forceinline__ S g_dst(S vs, S vd, const double* __restrict__ pe) {
     else if constexpr (EM == ND_EM_DIFFUSION) return vs - vd;                     // diffusion_test.jl:57-60
     else if constexpr (EM == ND_EM_DIFFUSION_WEIGHTED) return pe[0] * (vs - vd);  // ComponentLibrary.jl:8-10
     else if constexpr (EM == ND_EM_DIFFUSION_FIDUCIAL) return pe[0] * (vs - vd);  // ComponentLibrary.jl:22-25
+    else if constexpr (EM == ND_EM_RESISTOR) return (vs - vd) / pe[0];            // external_inputs.md:66-68
     else return nan_of<S>();
 }
 
@@ -58,6 +60,7 @@ __device__ __forceinline__ S g_dst_rt(int em, S vs, S vd, const double* __restri
     case ND_EM_DIFFUSION: return g_dst<ND_EM_DIFFUSION, S>(vs, vd, pe);
     case ND_EM_DIFFUSION_WEIGHTED: return g_dst<ND_EM_DIFFUSION_WEIGHTED, S>(vs, vd, pe);
     case ND_EM_DIFFUSION_FIDUCIAL: return g_dst<ND_EM_DIFFUSION_FIDUCIAL, S>(vs, vd, pe);
+    case ND_EM_RESISTOR: return g_dst<ND_EM_RESISTOR, S>(vs, vd, pe);
     default: return nan_of<S>();
     }
 }
@@ -69,6 +72,8 @@ template <> struct VModel<ND_VM_KURAMOTO> { static constexpr int dim = 1; };
 template <> struct VModel<ND_VM_SWING> { static constexpr int dim = 2; };
 template <> struct VModel<ND_VM_SWING_UNIT_DAMPING> { static constexpr int dim = 2; };
 template <> struct VModel<ND_VM_LOAD> { static constexpr int dim = 1; };
+template <> struct VModel<ND_VM_CAPACITOR_SINK> { static constexpr int dim = 1; };
+template <> struct VModel<ND_VM_PI_SOURCE> { static constexpr int dim = 2; };
 
 // vertex f (apply_compf, coreloop.jl:213-216); operation order as the reference writes it
 template <int VM, typename S = double>
@@ -95,12 +100,32 @@ __device__ __forceinline__ void vertex_f(S* __restrict__ dx, const S* __restrict
     }
 }
 
+// Vertex f with the arguments the fused kernels do not carry: external inputs ext
+// (external_inputs.jl; apply_compf passes them after the aggregated input) and the time
+// t.  The models without either reduce to vertex_f.
+template <int VM, typename S = double>
+__device__ __forceinline__ void vertex_f_ext(S* __restrict__ dx, const S* __restrict__ x, S acc,
+                                             const double* __restrict__ pv, const S* __restrict__ ext, double t,
+                                             int dim) {
+    if constexpr (VM == ND_VM_CAPACITOR_SINK) {            // external_inputs.md:76-79
+        dx[0] = 1.0 / pv[0] * (acc - (1.0 + 0.1 * sin(t)));
+    } else if constexpr (VM == ND_VM_PI_SOURCE) {          // external_inputs.md:87-92
+        const S d = pv[1] - ext[0];                        // tracking error Vref - V_remote
+        dx[1] = d;                                         // integrator state
+        const S inj = pv[3] * d + pv[2] * x[1];            // Kp*d + Ki*xi
+        dx[0] = 1.0 / pv[0] * (acc + inj);
+    } else {
+        vertex_f<VM, S>(dx, x, acc, pv, dim);
+    }
+}
+
 // Parameters each edge model reads (p entries per edge).
 template <int EM> struct EModel { static constexpr int np = 0; };
 template <> struct EModel<ND_EM_KURAMOTO> { static constexpr int np = 1; };
 template <> struct EModel<ND_EM_STATIC_LINE> { static constexpr int np = 2; };
 template <> struct EModel<ND_EM_DIFFUSION_WEIGHTED> { static constexpr int np = 1; };
 template <> struct EModel<ND_EM_DIFFUSION_FIDUCIAL> { static constexpr int np = 1; };
+template <> struct EModel<ND_EM_RESISTOR> { static constexpr int np = 1; };
 
 // parameters per vertex model
 template <int VM> struct VPar { static constexpr int np = 0; };
@@ -108,5 +133,11 @@ template <> struct VPar<ND_VM_KURAMOTO> { static constexpr int np = 1; };
 template <> struct VPar<ND_VM_SWING> { static constexpr int np = 3; };
 template <> struct VPar<ND_VM_SWING_UNIT_DAMPING> { static constexpr int np = 1; };
 template <> struct VPar<ND_VM_LOAD> { static constexpr int np = 2; };
+template <> struct VPar<ND_VM_CAPACITOR_SINK> { static constexpr int np = 1; };
+template <> struct VPar<ND_VM_PI_SOURCE> { static constexpr int np = 4; };
+
+// external inputs each vertex model reads
+template <int VM> struct VExt { static constexpr int n = 0; };
+template <> struct VExt<ND_VM_PI_SOURCE> { static constexpr int n = 1; };
 
 }  // namespace nd

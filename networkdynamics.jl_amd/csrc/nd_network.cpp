@@ -63,6 +63,8 @@ static int validate_kind(const nd_kind& k, int32_t idx) {
         return set_error(ND_ERR_UNSUPPORTED, "kind %d: output dimension above %d", idx, ND_MAX_OUT);
     if (!k.is_edge && k.outdim_src != 0)
         return set_error(ND_ERR_ARGUMENT, "kind %d: vertex kinds have outdim_src = 0", idx);
+    if (k.extdim < 0 || k.extdim > ND_MAX_EXT)
+        return set_error(ND_ERR_UNSUPPORTED, "kind %d: %d external inputs (at most %d)", idx, k.extdim, ND_MAX_EXT);
     if (k.ff == ND_FF_PURESTATEMAP) {
         for (int j = 0; j < k.outdim_dst; ++j)
             if (k.mask_dst[j] < 1 || k.mask_dst[j] > k.dim)
@@ -96,7 +98,7 @@ static int build(Network& n) {
     group_by_kind(n.ekind, static_cast<int32_t>(n.kinds.size()), n.eb, n.e_batch, n.e_pos);
 
     // registration counters (network_structure.jl:259-289)
-    int64_t c_dyn = 0, c_out = 0, c_par = 0, c_aggr = 0, c_gbuf = 0;
+    int64_t c_dyn = 0, c_out = 0, c_par = 0, c_aggr = 0, c_gbuf = 0, c_ext = 0;
     for (Batch& b : n.vb) {
         const nd_kind& k = n.kinds[b.kind];
         const int64_t cnt = static_cast<int64_t>(b.indices.size());
@@ -104,6 +106,7 @@ static int build(Network& n) {
         b.out_first = c_out + 1;   c_out += cnt * k.outdim_dst;
         b.param_first = c_par + 1; c_par += cnt * k.pdim;
         b.in_first = c_aggr + 1;   c_aggr += cnt * n.edepth;
+        b.ext_first = c_ext + 1;   c_ext += cnt * k.extdim;
     }
     for (Batch& b : n.eb) {
         const nd_kind& k = n.kinds[b.kind];
@@ -112,8 +115,10 @@ static int build(Network& n) {
         b.out_first = c_out + 1;   c_out += cnt * (k.outdim_src + k.outdim_dst);
         b.param_first = c_par + 1; c_par += cnt * k.pdim;
         b.in_first = c_gbuf + 1;   c_gbuf += cnt * 2 * n.vdepth;
+        b.ext_first = c_ext + 1;   c_ext += cnt * k.extdim;
     }
     n.dim = c_dyn; n.pdim = c_par; n.lastidx_out = c_out; n.lastidx_aggr = c_aggr; n.lastidx_gbuf = c_gbuf;
+    n.lastidx_extbuf = c_ext;
 
     // Destination lists: emit (row, output index) in increasing output index, then a
     // stable counting sort by row keeps each row's summation order.
@@ -145,6 +150,31 @@ static int build(Network& n) {
     if (lo < 0) { n.agg_first = 1; n.agg_last = 0; }
     else { n.agg_first = lo; n.agg_last = hi; }
     return ND_OK;
+}
+
+int64_t output_state_index(const Network& n, int64_t k) {
+    // output blocks: vertex batches, then edge batches, ascending (registration order)
+    for (const Batch& b : n.vb) {
+        const nd_kind& kd = n.kinds[b.kind];
+        const int64_t len = static_cast<int64_t>(b.indices.size()) * kd.outdim_dst;
+        if (k >= b.out_first && k < b.out_first + len) {
+            if (kd.ff != ND_FF_PURESTATEMAP) return -1;
+            const int64_t q = (k - b.out_first) / kd.outdim_dst, j = (k - b.out_first) % kd.outdim_dst;
+            return b.state_first - 1 + q * kd.dim + kd.mask_dst[j] - 1;
+        }
+    }
+    for (const Batch& b : n.eb) {
+        const nd_kind& kd = n.kinds[b.kind];
+        const int64_t stride = kd.outdim_src + kd.outdim_dst;
+        const int64_t len = static_cast<int64_t>(b.indices.size()) * stride;
+        if (k >= b.out_first && k < b.out_first + len) {
+            if (kd.ff != ND_FF_PURESTATEMAP) return -1;
+            const int64_t q = (k - b.out_first) / stride, j = (k - b.out_first) % stride;
+            const int32_t m = j < kd.outdim_src ? kd.mask_src[j] : kd.mask_dst[j - kd.outdim_src];
+            return b.state_first - 1 + q * kd.dim + m - 1;
+        }
+    }
+    return -1;
 }
 
 }  // namespace nd
@@ -209,6 +239,52 @@ int nd_network_get_info(const nd_network* h, nd_network_info* info) {
     info->vdepth = n.vdepth; info->edepth = n.edepth;
     info->nvbatches = static_cast<int32_t>(n.vb.size());
     info->nebatches = static_cast<int32_t>(n.eb.size());
+    info->lastidx_extbuf = n.lastidx_extbuf;
+    return ND_OK;
+}
+
+int nd_network_export_ext_ranges(const nd_network* h, int64_t* v_ext, int64_t* e_ext) {
+    if (!h) return set_error(ND_ERR_ARGUMENT, "NULL argument");
+    const Network& n = h->net;
+    if (v_ext)
+        for (int64_t v = 0; v < n.nv; ++v) {
+            v_ext[2 * v] = n.v_ext_first(v);
+            v_ext[2 * v + 1] = n.v_ext_first(v) + n.kinds[n.vkind[v]].extdim - 1;
+        }
+    if (e_ext)
+        for (int64_t e = 0; e < n.ne; ++e) {
+            e_ext[2 * e] = n.e_ext_first(e);
+            e_ext[2 * e + 1] = n.e_ext_first(e) + n.kinds[n.ekind[e]].extdim - 1;
+        }
+    return ND_OK;
+}
+
+// ExtMap (external_inputs.jl:10-46) with _symidx_to_extidx's checks (:33-46)
+int nd_network_set_extmap(nd_network* h, int64_t len, const int64_t* map) {
+    if (!h) return set_error(ND_ERR_ARGUMENT, "NULL argument");
+    Network& n = h->net;
+    if (n.lastidx_extbuf == 0) return set_error(ND_ERR_ARGUMENT, "There are no external inputs.");
+    if (len != n.lastidx_extbuf || !map)
+        return set_error(ND_ERR_ARGUMENT, "ExtMap needs %lld entries, got %lld", (long long)n.lastidx_extbuf,
+                         (long long)len);
+    for (int64_t k = 0; k < len; ++k) {
+        const int64_t x = map[k];
+        if (x > 0) {
+            if (x > n.dim) return set_error(ND_ERR_ARGUMENT, "external input %lld: state index %lld outside 1:%lld",
+                                            (long long)(k + 1), (long long)x, (long long)n.dim);
+        } else if (x < 0) {
+            const int64_t o = -x;
+            if (o > n.lastidx_out)
+                return set_error(ND_ERR_ARGUMENT, "external input %lld: output index %lld outside 1:%lld",
+                                 (long long)(k + 1), (long long)o, (long long)n.lastidx_out);
+            if (output_state_index(n, o) < 0)
+                return set_error(ND_ERR_ARGUMENT, "Cannot resolve external input %lld! Outputs of feed-forward "
+                                 "components are not allowed as external inputs.", (long long)(k + 1));
+        } else {
+            return set_error(ND_ERR_ARGUMENT, "external input %lld: index 0", (long long)(k + 1));
+        }
+    }
+    n.extmap.assign(map, map + len);
     return ND_OK;
 }
 

@@ -53,6 +53,12 @@ struct VDesc {            // one vertex batch as the kernels see it (0-based off
     const int32_t* lnbr;  // with cached parameters instead of lslots: neighbour output index | (is_src << 31)
     const double* kfull;  // edge parameters per destination-list entry (rowptr order), or null
     const double* klow;   // the same for the hub-free lists (lrp order), or null
+    // one-thread-per-vertex path (models with external inputs or time dependence, edge
+    // models outside the fused set): external inputs as u-indices, and the time
+    const int32_t* xmap;  // u-index of every external-buffer entry
+    int32_t generic;
+    int32_t ext0, extdim; // first external-buffer entry of the batch (0-based), per vertex
+    double t;
 };
 
 // Slot-ordered edge parameters: a copy of the edge batch's parameters laid out like the
@@ -644,6 +650,90 @@ __global__ void __launch_bounds__(256) k_aggregate(const int32_t* __restrict__ r
     }
 }
 
+// Vertex batches outside the fused kernels' model set -- models that read external
+// inputs (external_inputs.jl: collect_externals! fills them after the :g passes, from
+// states and StateMask outputs, so here they are read from u directly) or the time,
+// and every batch of a network whose edge model the fused kernels do not carry: one
+// thread per vertex, its destination list summed in the reference order.
+template <int VM, bool STAGE>
+__global__ void __launch_bounds__(256) k_vertex_gen(VDesc vd, const int32_t* __restrict__ rowptr,
+                                                    const int2* __restrict__ slots, const uint8_t* __restrict__ slot_eb,
+                                                    const EDesc* __restrict__ ed, const double* __restrict__ u,
+                                                    const double* __restrict__ vo, const double* __restrict__ p,
+                                                    double* __restrict__ du, StageArgs sa) {
+    constexpr int DIMC = VModel<VM>::dim;
+    constexpr int NX = VExt<VM>::n > 0 ? VExt<VM>::n : 1;
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= vd.count) return;
+    const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * DIMC;
+    double x[DIMC];
+#pragma unroll
+    for (int j = 0; j < DIMC; ++j) x[j] = u[x0 + j];
+    const double own = x[vd.mask[0]];
+    double acc = 0.0;
+    const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
+    for (int32_t k = beg; k < end; ++k) acc = acc + slot_value_rt(ed[slot_eb ? slot_eb[k] : 0], slots[k], own, u, vo, p);
+    double ext[NX];
+#pragma unroll
+    for (int j = 0; j < NX; ++j)
+        ext[j] = j < VExt<VM>::n ? u[vd.xmap[vd.ext0 + static_cast<int64_t>(q) * vd.extdim + j]] : 0.0;
+    double dx[DIMC];
+    vertex_f_ext<VM>(dx, x, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, ext, vd.t, DIMC);
+#pragma unroll
+    for (int j = 0; j < DIMC; ++j) {
+        const double y = emit1<STAGE>(du, sa, x0 + j, dx[j]);
+        if (STAGE && sa.vnext && j == vd.mask[0]) sa.vnext[static_cast<int64_t>(vd.out0) + q] = y;
+    }
+}
+
+// the dual-number form of k_vertex_gen (du and J(u) v)
+template <int VM>
+__global__ void __launch_bounds__(256) k_vertex_gen_jvp(VDesc vd, const int32_t* __restrict__ rowptr,
+                                                        const int2* __restrict__ slots,
+                                                        const uint8_t* __restrict__ slot_eb,
+                                                        const EDesc* __restrict__ ed, const double* __restrict__ u,
+                                                        const double* __restrict__ v, const double* __restrict__ vo,
+                                                        const double* __restrict__ dvo, const double* __restrict__ p,
+                                                        double* __restrict__ du, double* __restrict__ jv) {
+    constexpr int DIMC = VModel<VM>::dim;
+    constexpr int NX = VExt<VM>::n > 0 ? VExt<VM>::n : 1;
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= vd.count) return;
+    const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * DIMC;
+    Dual x[DIMC];
+#pragma unroll
+    for (int j = 0; j < DIMC; ++j) x[j] = Dual{u[x0 + j], v[x0 + j]};
+    const Dual own = x[vd.mask[0]];
+    Dual acc{0.0, 0.0};
+    const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
+    for (int32_t k = beg; k < end; ++k)
+        acc = acc + slot_value_dual_rt(ed[slot_eb ? slot_eb[k] : 0], slots[k], own, u, v, vo, dvo, p);
+    Dual ext[NX];
+#pragma unroll
+    for (int j = 0; j < NX; ++j) {
+        ext[j] = Dual{0.0, 0.0};
+        if (j < VExt<VM>::n) {
+            const int32_t xi = vd.xmap[vd.ext0 + static_cast<int64_t>(q) * vd.extdim + j];
+            ext[j] = Dual{u[xi], v[xi]};
+        }
+    }
+    Dual dx[DIMC];
+    vertex_f_ext<VM, Dual>(dx, x, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, ext, vd.t, DIMC);
+#pragma unroll
+    for (int j = 0; j < DIMC; ++j) { du[x0 + j] = dx[j].v; jv[x0 + j] = dx[j].d; }
+}
+
+// collect_externals!(map, extbuf, u, o), external_inputs.jl:49-77: map > 0 reads u,
+// map < 0 reads o (1-based magnitudes)
+__global__ void k_collect_externals(const int32_t* __restrict__ map, int64_t n, double* __restrict__ extbuf,
+                                    const double* __restrict__ u, const double* __restrict__ o) {
+    for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < n;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t m = map[k];
+        extbuf[k] = m > 0 ? u[m - 1] : o[-m - 1];
+    }
+}
+
 // ---- kernel selection ----------------------------------------------------------
 
 using VertexKernel = void (*)(VDesc, const int32_t*, const int2*, const uint8_t*, const EDesc*, const double*,
@@ -696,6 +786,20 @@ VertexKernel pick_vertex_kernel(int vm, int em, int wrap, bool multi, bool kc) {
     }
 }
 
+template <bool STAGE>
+VertexKernel pick_gen_kernel(int vm) {
+    switch (vm) {
+    case ND_VM_DIFFUSION: return k_vertex_gen<ND_VM_DIFFUSION, STAGE>;
+    case ND_VM_KURAMOTO: return k_vertex_gen<ND_VM_KURAMOTO, STAGE>;
+    case ND_VM_SWING: return k_vertex_gen<ND_VM_SWING, STAGE>;
+    case ND_VM_SWING_UNIT_DAMPING: return k_vertex_gen<ND_VM_SWING_UNIT_DAMPING, STAGE>;
+    case ND_VM_LOAD: return k_vertex_gen<ND_VM_LOAD, STAGE>;
+    case ND_VM_CAPACITOR_SINK: return k_vertex_gen<ND_VM_CAPACITOR_SINK, STAGE>;
+    case ND_VM_PI_SOURCE: return k_vertex_gen<ND_VM_PI_SOURCE, STAGE>;
+    default: return k_vertex_rhs<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false, STAGE, false>;
+    }
+}
+
 using JvpKernel = void (*)(VDesc, const int32_t*, const int2*, const uint8_t*, const EDesc*, const double*,
                            const double*, const double*, const double*, const double*, double*, double*);
 
@@ -738,6 +842,19 @@ JvpKernel pick_jvp_kernel(int vm, int em, int wrap, bool multi) {
     }
 }
 
+JvpKernel pick_gen_jvp(int vm) {
+    switch (vm) {
+    case ND_VM_DIFFUSION: return k_vertex_gen_jvp<ND_VM_DIFFUSION>;
+    case ND_VM_KURAMOTO: return k_vertex_gen_jvp<ND_VM_KURAMOTO>;
+    case ND_VM_SWING: return k_vertex_gen_jvp<ND_VM_SWING>;
+    case ND_VM_SWING_UNIT_DAMPING: return k_vertex_gen_jvp<ND_VM_SWING_UNIT_DAMPING>;
+    case ND_VM_LOAD: return k_vertex_gen_jvp<ND_VM_LOAD>;
+    case ND_VM_CAPACITOR_SINK: return k_vertex_gen_jvp<ND_VM_CAPACITOR_SINK>;
+    case ND_VM_PI_SOURCE: return k_vertex_gen_jvp<ND_VM_PI_SOURCE>;
+    default: return k_vertex_jvp<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false>;
+    }
+}
+
 using EdgeOdeJvpKernel = void (*)(EDesc, const double*, const double*, const double*, double*, double*);
 EdgeOdeJvpKernel pick_edge_ode_jvp(int em) {
     switch (em) {
@@ -757,24 +874,31 @@ EdgeOdeKernel pick_edge_ode(int em) {
     }
 }
 
-bool vertex_model_known(int vm) { return vm >= ND_VM_NOOP && vm <= ND_VM_LOAD; }
+bool vertex_model_known(int vm) { return vm >= ND_VM_NOOP && vm <= ND_VM_PI_SOURCE; }
 int VModelDim(int vm) {
     switch (vm) {
     case ND_VM_SWING:
-    case ND_VM_SWING_UNIT_DAMPING: return 2;
+    case ND_VM_SWING_UNIT_DAMPING:
+    case ND_VM_PI_SOURCE: return 2;
     case ND_VM_DIFFUSION:
     case ND_VM_KURAMOTO:
-    case ND_VM_LOAD: return 1;
+    case ND_VM_LOAD:
+    case ND_VM_CAPACITOR_SINK: return 1;
     default: return -1;
     }
 }
-bool edge_model_known(int em) { return em >= ND_EM_NOOP && em <= ND_EM_ODE_LINEAR; }
+int vertex_ext_inputs(int vm) { return vm == ND_VM_PI_SOURCE ? VExt<ND_VM_PI_SOURCE>::n : 0; }
+// vertex models only the one-thread-per-vertex kernels carry
+bool vertex_model_generic(int vm) { return vm == ND_VM_CAPACITOR_SINK || vm == ND_VM_PI_SOURCE; }
+bool edge_model_known(int em) { return em >= ND_EM_NOOP && em <= ND_EM_RESISTOR; }
+bool edge_model_generic(int em) { return em == ND_EM_RESISTOR; }
 int edge_params(int em) {
     switch (em) {
     case ND_EM_KURAMOTO: return EModel<ND_EM_KURAMOTO>::np;
     case ND_EM_STATIC_LINE: return EModel<ND_EM_STATIC_LINE>::np;
     case ND_EM_DIFFUSION_WEIGHTED: return EModel<ND_EM_DIFFUSION_WEIGHTED>::np;
     case ND_EM_DIFFUSION_FIDUCIAL: return EModel<ND_EM_DIFFUSION_FIDUCIAL>::np;
+    case ND_EM_RESISTOR: return EModel<ND_EM_RESISTOR>::np;
     default: return 0;
     }
 }
@@ -845,6 +969,11 @@ struct nd_plan {
     double* d_kfull = nullptr;
     double* d_klow = nullptr;
     int4* d_kpos = nullptr;
+    // external inputs: u-index per external-buffer entry (vertex kernels), and the ExtMap
+    // as +u / -o 1-based indices (collect_externals!)
+    int64_t lastidx_extbuf = 0;
+    int32_t* d_xmap = nullptr;
+    int32_t* d_extmap = nullptr;
 
     ~nd_plan() {
         if (small_en) nd_ensemble_destroy(small_en);
@@ -902,7 +1031,7 @@ int rebuild_hubs(nd_plan* pl) {
         pl->hub_offset[b] = all.size();
         lrp_offset[b] = lrp.size();
         std::vector<std::pair<int32_t, int32_t>> big, mid;   // (-length, q)
-        const bool functional = vd.model != ND_VM_NOOP && vd.depth == 1;
+        const bool functional = vd.model != ND_VM_NOOP && vd.depth == 1 && !vd.generic;
         for (int32_t q = 0; q < vd.count; ++q) {
             lrp.push_back(static_cast<int32_t>(nlow));
             if (!functional) continue;
@@ -1009,8 +1138,10 @@ int apply_gather_mode(nd_plan* pl) {
     pl->vkern_stage.clear();
     const int em = pl->edesc.empty() ? ND_EM_NOOP : pl->em0;
     for (const VDesc& vd : pl->vdesc) {
-        pl->vkern.push_back(pick_vertex_kernel<false>(vd.model, em, pl->wrap0, pl->multi, pl->kcache));
-        pl->vkern_stage.push_back(pick_vertex_kernel<true>(vd.model, em, pl->wrap0, pl->multi, pl->kcache));
+        pl->vkern.push_back(vd.generic ? pick_gen_kernel<false>(vd.model)
+                                       : pick_vertex_kernel<false>(vd.model, em, pl->wrap0, pl->multi, pl->kcache));
+        pl->vkern_stage.push_back(vd.generic ? pick_gen_kernel<true>(vd.model)
+                                             : pick_vertex_kernel<true>(vd.model, em, pl->wrap0, pl->multi, pl->kcache));
     }
     return rebuild_hubs(pl);
 }
@@ -1044,7 +1175,7 @@ KSync no_sync(unsigned vblocks) {
 // vin: the compact vertex outputs of u (default: the plan's buffer, built here first);
 // build = false when a previous stage epilogue already wrote them.
 void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const double* p, const StageArgs& sa,
-                hipStream_t s, bool sync = true, double* vin = nullptr, bool build = true) {
+                hipStream_t s, double t, bool sync = true, double* vin = nullptr, bool build = true) {
     const int B = pl->block;
     if (!pl->use_vout) {
         vin = const_cast<double*>(u);    // direct gather: slot indices address u itself
@@ -1076,8 +1207,9 @@ void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const doub
         }
     }
     for (size_t b = 0; b < pl->vdesc.size(); ++b) {
-        const VDesc& vd = pl->vdesc[b];
+        VDesc vd = pl->vdesc[b];
         if (vd.count == 0) continue;
+        vd.t = t;
         const int64_t nblocks = vd.nbig + blocks_for(static_cast<int64_t>(vd.nmid) * kWave, B) + blocks_for(vd.count, B);
         hipLaunchKernelGGL(stage ? pl->vkern_stage[b] : pl->vkern[b], dim3(static_cast<unsigned>(nblocks)),
                            dim3(static_cast<unsigned>(B)), 0, s, vd, pl->d_rowptr, pl->d_slots, pl->d_slot_eb,
@@ -1113,12 +1245,18 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         if (k.model != ND_VM_NOOP) {
             functional_vertex = true;
             if (k.dim != VModelDim(k.model)) return nd::set_error(ND_ERR_ARGUMENT, "vertex model %d needs dim %d", k.model, VModelDim(k.model));
+            if (k.extdim != vertex_ext_inputs(k.model))
+                return nd::set_error(ND_ERR_ARGUMENT, "vertex model %d reads %d external inputs, its kind declares %d",
+                                     k.model, vertex_ext_inputs(k.model), k.extdim);
         }
     }
+    bool generic_edges = false;
     for (const nd::Batch& b : n.eb) {
         const nd_kind& k = n.kinds[b.kind];
         if (!edge_model_known(k.model)) return nd::set_error(ND_ERR_UNSUPPORTED, "edge model %d has no HIP kernel", k.model);
         if (k.model != ND_EM_NOOP) functional_edge = true;
+        if (edge_model_generic(k.model)) generic_edges = true;
+        if (k.extdim > 0) return nd::set_error(ND_ERR_UNSUPPORTED, "external inputs of edge models have no HIP kernel");
         if (k.ff == ND_FF_PURESTATEMAP && k.wrapper != ND_WRAP_FIDUCIAL)
             return nd::set_error(ND_ERR_UNSUPPORTED, "StateMask edge outputs inside AntiSymmetric/Symmetric/Directed have no HIP kernel");
         if (k.model == ND_EM_NOOP && k.dim > 0)
@@ -1135,6 +1273,8 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
     if ((functional_vertex || functional_edge) && (n.vdepth != 1 || (n.ne > 0 && n.edepth != 1)))
         return nd::set_error(ND_ERR_UNSUPPORTED, "models with output depth other than 1 have no HIP kernel (vdepth=%d, edepth=%d)",
                              n.vdepth, n.edepth);
+    if (n.lastidx_extbuf > 0 && static_cast<int64_t>(n.extmap.size()) != n.lastidx_extbuf)
+        return nd::set_error(ND_ERR_ARGUMENT, "the network has external inputs: set its ExtMap (nd_network_set_extmap) first");
 
     nd_plan* pl = new (std::nothrow) nd_plan;
     if (!pl) return nd::set_error(ND_ERR_ALLOC, "host allocation failed");
@@ -1291,8 +1431,26 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         vd.count = static_cast<int32_t>(B.indices.size());
         vd.out0 = static_cast<int32_t>(B.out_first - 1);
         for (int j = 0; j < ND_MAX_OUT; ++j) vd.mask[j] = k.mask_dst[j] - 1;
+        vd.generic = k.model != ND_VM_NOOP && (vertex_model_generic(k.model) || generic_edges) ? 1 : 0;
+        vd.ext0 = static_cast<int32_t>(B.ext_first - 1);
+        vd.extdim = k.extdim;
         pl->vdesc.push_back(vd);
-        pl->vkern_jvp.push_back(pick_jvp_kernel(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi));
+        pl->vkern_jvp.push_back(vd.generic ? pick_gen_jvp(k.model)
+                                           : pick_jvp_kernel(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi));
+    }
+    // external inputs: every entry resolves to a state (StateBufIdx, or a StateMask output)
+    pl->lastidx_extbuf = n.lastidx_extbuf;
+    if (n.lastidx_extbuf > 0) {
+        std::vector<int32_t> xmap(static_cast<size_t>(n.lastidx_extbuf)), emap(static_cast<size_t>(n.lastidx_extbuf));
+        for (int64_t k = 0; k < n.lastidx_extbuf; ++k) {
+            const int64_t m = n.extmap[k];
+            xmap[k] = static_cast<int32_t>(m > 0 ? m - 1 : nd::output_state_index(n, -m));
+            emap[k] = static_cast<int32_t>(m);
+        }
+        rc = upload(pl, xmap, &pl->d_xmap);
+        rc = rc ? rc : upload(pl, emap, &pl->d_extmap);
+        if (rc) { delete pl; return rc; }
+        for (VDesc& vd : pl->vdesc) vd.xmap = pl->d_xmap;
     }
     if (n.vdepth == 1 && n.ne > 0) {
         void* w = nullptr;
@@ -1354,7 +1512,6 @@ int nd_plan_get_tuning(const nd_plan* pl, nd_plan_tuning* t) {
 
 int nd_rhs(nd_plan* pl, double* du, const double* u, const double* p, double t, int64_t len_du, int64_t len_u,
            int64_t len_p, void* stream) {
-    (void)t;
     if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
     if (!(len_du == len_u && len_u == pl->dim))   // coreloop.jl:2-4
         return nd::set_error(ND_ERR_ARGUMENT, "du or u does not have expected size %lld", (long long)pl->dim);
@@ -1364,13 +1521,12 @@ int nd_rhs(nd_plan* pl, double* du, const double* u, const double* p, double t, 
     if (pl->pdim > 0 && !p) return nd::set_error(ND_ERR_ARGUMENT, "NULL parameter pointer");
     int rc = check_device(pl);
     if (rc) return rc;
-    launch_rhs(pl, false, du, u, p, StageArgs{}, static_cast<hipStream_t>(stream));
+    launch_rhs(pl, false, du, u, p, StageArgs{}, static_cast<hipStream_t>(stream), t);
     return hip_check(hipGetLastError(), "nd_rhs launch");
 }
 
 int nd_rhs_jvp(nd_plan* pl, double* du, double* jv, const double* u, const double* v, const double* p, double t,
                int64_t len_du, int64_t len_u, int64_t len_p, void* stream) {
-    (void)t;
     if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
     if (!(len_du == len_u && len_u == pl->dim))   // coreloop.jl:2-4
         return nd::set_error(ND_ERR_ARGUMENT, "du or u does not have expected size %lld", (long long)pl->dim);
@@ -1401,8 +1557,9 @@ int nd_rhs_jvp(nd_plan* pl, double* du, double* jv, const double* u, const doubl
         dvo = pl->d_dvout;
     }
     for (size_t b = 0; b < pl->vdesc.size(); ++b) {
-        const VDesc& vd = pl->vdesc[b];
+        VDesc vd = pl->vdesc[b];
         if (vd.count == 0) continue;
+        vd.t = t;
         hipLaunchKernelGGL(pl->vkern_jvp[b], dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, pl->d_rowptr,
                            pl->d_slots, pl->d_slot_eb, pl->d_edesc, u, v, vo, dvo, p, du, jv);
     }
@@ -1416,7 +1573,6 @@ int nd_rhs_jvp(nd_plan* pl, double* du, double* jv, const double* u, const doubl
 
 int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_t nsteps, double* work,
            int64_t len_u, int64_t len_p, void* stream) {
-    (void)t0;
     if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
     if (len_u != pl->dim) return nd::set_error(ND_ERR_ARGUMENT, "u does not have expected size %lld", (long long)pl->dim);
     if (pl->pdim > 0 && len_p != pl->pdim)
@@ -1480,12 +1636,26 @@ int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_
         vb = pl->d_vout2;
     }
     for (int64_t k = 0; k < nsteps; ++k) {
-        launch_rhs(pl, true, nullptr, u, p, StageArgs{u, acc, ya, vb, half, 0.0, 1}, s, k == 0, va, k == 0);
-        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, yb, va, half, 2.0, 2}, s, false, vb, false);
-        launch_rhs(pl, true, nullptr, yb, p, StageArgs{u, acc, ya, vb, dt, 2.0, 2}, s, false, va, false);
-        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, u, va, sixth, 0.0, 3}, s, false, vb, false);
+        // stage times t_k, t_k + dt/2, t_k + dt/2, t_k + dt (time-dependent models)
+        const double tk = t0 + static_cast<double>(k) * dt;
+        launch_rhs(pl, true, nullptr, u, p, StageArgs{u, acc, ya, vb, half, 0.0, 1}, s, tk, k == 0, va, k == 0);
+        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, yb, va, half, 2.0, 2}, s, tk + half, false, vb, false);
+        launch_rhs(pl, true, nullptr, yb, p, StageArgs{u, acc, ya, vb, dt, 2.0, 2}, s, tk + half, false, va, false);
+        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, u, va, sixth, 0.0, 3}, s, tk + dt, false, vb, false);
     }
     return hip_check(hipGetLastError(), "nd_rk4 launch");
+}
+
+int nd_collect_externals(nd_plan* pl, double* extbuf, const double* u, const double* o, void* stream) {
+    if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
+    if (pl->lastidx_extbuf == 0) return ND_OK;
+    if (!extbuf || !u || !o) return nd::set_error(ND_ERR_ARGUMENT, "NULL buffer");
+    int rc = check_device(pl);
+    if (rc) return rc;
+    const unsigned nb = static_cast<unsigned>(std::min<int64_t>((pl->lastidx_extbuf + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_collect_externals, dim3(nb), dim3(256), 0, static_cast<hipStream_t>(stream), pl->d_extmap,
+                       pl->lastidx_extbuf, extbuf, u, o);
+    return hip_check(hipGetLastError(), "nd_collect_externals launch");
 }
 
 int nd_gather(nd_plan* pl, double* gbuf, const double* o, void* stream) {

@@ -50,7 +50,7 @@ STAT_DTYPE = np.dtype([("t", "f8"), ("dt", "f8"), ("naccept", "i8"), ("nreject",
 def _kind_struct(m):
     k = A.NDKind()
     d = m.kind()
-    for f in ("is_edge", "model", "wrapper", "ff", "dim", "pdim", "outdim_src", "outdim_dst"):
+    for f in ("is_edge", "model", "wrapper", "ff", "dim", "pdim", "outdim_src", "outdim_dst", "extdim"):
         setattr(k, f, int(d[f]))
     for j, x in enumerate(d["mask_src"]):
         k.mask_src[j] = x

@@ -17,8 +17,8 @@ module NDHIP
 
 using NetworkDynamics
 using NetworkDynamics: ExecutionStyle, Aggregator, IndexManager, ComponentBatch, compf, compg, fftype,
-                       dim, pdim, outdim, PureFeedForward, PureStateMap, StateMask, AntiSymmetric,
-                       Symmetric, Directed, Fiducial
+                       dim, pdim, outdim, extdim, PureFeedForward, PureStateMap, StateMask, AntiSymmetric,
+                       Symmetric, Directed, Fiducial, ExtMap, StateBufIdx, has_external_input
 using AMDGPU: ROCVector, AMDGPU
 
 export HIPExecution, SegmentedAggregator, register_model!
@@ -40,6 +40,7 @@ struct NDKind
     outdim_dst::Int32
     mask_src::NTuple{4,Int32}
     mask_dst::NTuple{4,Int32}
+    extdim::Int32          # external inputs (extdim(c), external_inputs.jl)
 end
 
 # component function -> ND_VM_* / ND_EM_* id (fill from the user's component library,
@@ -55,7 +56,8 @@ _nomask() = ntuple(_ -> Int32(0), 4)
 function _kind(c::VertexModel)
     g = compg(c)
     g isa StateMask || throw(ArgumentError("HIPExecution needs StateMask vertex outputs"))
-    NDKind(0, get(MODEL_IDS, compf(c), -1), 0, 0, dim(c), pdim(c), 0, outdim(c), _nomask(), _mask(g))
+    NDKind(0, get(MODEL_IDS, compf(c), -1), 0, 0, dim(c), pdim(c), 0, outdim(c), _nomask(), _mask(g),
+           extdim(c))
 end
 
 function _kind(c::EdgeModel)
@@ -63,11 +65,12 @@ function _kind(c::EdgeModel)
     od = outdim(c)
     if g isa Fiducial && g.src isa StateMask
         return NDKind(1, get(MODEL_IDS, compf(c), -1), 4, 0, dim(c), pdim(c), od.src, od.dst,
-                      _mask(g.src), _mask(g.dst))
+                      _mask(g.src), _mask(g.dst), extdim(c))
     end
     wrap = g isa AntiSymmetric ? 1 : g isa Symmetric ? 2 : g isa Directed ? 3 : 0
     inner = wrap == 0 ? g : g.g
-    NDKind(1, get(MODEL_IDS, inner, -1), wrap, 1, dim(c), pdim(c), od.src, od.dst, _nomask(), _nomask())
+    NDKind(1, get(MODEL_IDS, inner, -1), wrap, 1, dim(c), pdim(c), od.src, od.dst, _nomask(), _nomask(),
+           extdim(c))
 end
 
 mutable struct SegmentedAggregator{F} <: Aggregator
@@ -91,6 +94,11 @@ function SegmentedAggregator(im::IndexManager, batches, f)
                  (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Int32, Ptr{NDKind}, Ptr{Int32}, Ptr{Int32}, Ptr{Ptr{Cvoid}}),
                  length(im.vertexm), length(im.edgem), src, dst, length(kinds), kinds, vk, ek, net))
     _check_layout(net[], im)
+    if has_external_input(im)
+        # the reference's own ExtMap (external_inputs.jl:10-31), as +u / -o indices
+        em = [x isa StateBufIdx ? Int64(x.idx) : -Int64(x.idx) for x in ExtMap(im).map]
+        _check(ccall((:nd_network_set_extmap, libnd), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}), net[], length(em), em))
+    end
     plan = Ref{Ptr{Cvoid}}(C_NULL)
     _check(ccall((:nd_plan_create, libnd), Cint, (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}), net[], plan))
     a = SegmentedAggregator(f, net[], plan[])

@@ -57,6 +57,22 @@ def static_line_edge(name="static_line"):
     return c.EdgeModel(g=c.AntiSymmetric(c.static_line_g), outsym=["P"], psym=["B", ("phi", 0.0)], name=name)
 
 
+# docs/src/external_inputs.md:62-99: the resistor, the capacitor load with a periodic
+# current sink, and the PI-controlled current source that reads the load's voltage
+# through an external input
+def resistor(name="resistor"):
+    return c.EdgeModel(g=c.AntiSymmetric(c.resistor_g), outsym=["i"], psym=[("R", 0.1)], name=name)
+
+
+def capacitor_load(name="load"):
+    return c.VertexModel(f=c.f_load, g=1, sym=[("V", 0.5)], psym=[("C", 1.0)], name=name)
+
+
+def pi_source(extin, name="source"):
+    return c.VertexModel(f=c.f_source, g=1, sym=[("V", 0.5), ("xi", 0.0)],
+                         psym=[("C", 1.0), ("Vref", 1.0), ("Ki", 0.5), ("Kp", 10.0)], extin=extin, name=name)
+
+
 class bench:
     """benchmark/benchmark_models.jl:5-42."""
 

@@ -125,7 +125,7 @@ class Network:
         self.vertex_kind, self.edge_kind = vkind, ekind[:ne]
         karr = (A.NDKind * len(kinds))()
         for a, k in zip(karr, kinds):
-            for f in ("is_edge", "model", "wrapper", "ff", "dim", "pdim", "outdim_src", "outdim_dst"):
+            for f in ("is_edge", "model", "wrapper", "ff", "dim", "pdim", "outdim_src", "outdim_dst", "extdim"):
                 setattr(a, f, int(k[f]))
             for j, x in enumerate(k["mask_src"]):
                 a.mask_src[j] = x
@@ -140,6 +140,9 @@ class Network:
         A.check(L.nd_network_get_info(h, C.byref(info)))
         self.info = info
         self._plan = None
+        if info.lastidx_extbuf > 0:
+            m = self.extmap()
+            A.check(L.nd_network_set_extmap(h, len(m), m.ctypes.data))
         if verbose:
             print(f"Create dynamic network with {nv} vertices and {ne} edges:")
             for b in self.batches(False):
@@ -209,6 +212,69 @@ class Network:
         cols = np.zeros(max(nnz.value, 1), np.int64)
         A.check(A.lib().nd_network_export_csr(self._net, rp.ctypes.data, cols.ctypes.data))
         return rp, cols[: nnz.value]
+
+    # ---- external inputs (external_inputs.jl) ------------------------------------------------
+    def ext_ranges(self):
+        """(v_ext[nv, 2], e_ext[ne, 2]): 1-based (first, last) external-buffer ranges."""
+        v = np.zeros((self.nv(), 2), np.int64)
+        e = np.zeros((max(self.ne(), 1), 2), np.int64)
+        A.check(A.lib().nd_network_export_ext_ranges(self._net, v.ctypes.data, e.ctypes.data))
+        return v, e[: self.ne()]
+
+    def _component(self, ref):
+        models = self.vertexm if ref.kind == "V" else self.edgem
+        c = ref.comp
+        if isinstance(c, str):
+            hits = [i for i, m in enumerate(models) if m.name == c]
+            if len(hits) != 1:
+                raise A.ArgumentError(f"Cannot resolve external input {ref!r}: {len(hits)} components named {c!r}")
+            return hits[0]
+        i = int(c) - 1
+        if not 0 <= i < len(models):
+            raise A.ArgumentError(f"Cannot resolve external input {ref!r}: no such component")
+        return i
+
+    def _symidx_to_extidx(self, ref):
+        """external_inputs.jl:33-46: +u-index (StateBufIdx) or -output-index (OutBufIdx)."""
+        i = self._component(ref)
+        v, e = self.ranges()
+        if ref.kind == "V":
+            m, first = self.vertexm[i], v[i, 0, 0]
+        else:
+            m, first = self.edgem[i], e[i, 0, 0]
+        if isinstance(ref.sym, (int, np.integer)):
+            if 1 <= int(ref.sym) <= m.dim:
+                return int(first + int(ref.sym) - 1)
+        elif ref.sym in m.sym:
+            return int(first + m.sym.index(ref.sym))
+        elif ref.kind == "E" and m.outsym and ref.sym in m.outsym:
+            return -int(e[i, 2, 0] + m.outsym.index(ref.sym))
+        raise A.ArgumentError(f"Cannot resolve external input {ref!r}! External inputs musst be states or "
+                              "outputs of non-feed-forward components.")
+
+    def extmap(self):
+        """ExtMap (external_inputs.jl:10-31) as +u / -o 1-based indices per external-buffer entry."""
+        n = int(self.info.lastidx_extbuf)
+        if n == 0:
+            raise A.ArgumentError("There are no external inputs.")
+        out = np.zeros(n, np.int64)
+        ve, ee = self.ext_ranges()
+        for rng, models in ((ve, self.vertexm), (ee, self.edgem)):
+            for i, m in enumerate(models):
+                for j, ref in enumerate(m.extin):
+                    out[rng[i, 0] - 1 + j] = self._symidx_to_extidx(ref)
+        return out
+
+    def collect_externals(self, extbuf, u, o):
+        """collect_externals!(map, extbuf, u, o) (external_inputs.jl:49-77) on the device."""
+        import torch
+        eptr, n = _tensor_ptr(extbuf, self.info.lastidx_extbuf, "extbuf", torch)
+        uptr, _ = _tensor_ptr(u, self.dim, "u", torch)
+        optr, _ = _tensor_ptr(o, self.info.lastidx_out, "o", torch)
+        if n != self.info.lastidx_extbuf or u.numel() != self.dim or o.numel() != self.info.lastidx_out:
+            raise A.ArgumentError("extbuf / u / o sizes do not match the network")
+        stream = torch.cuda.current_stream(u.device).cuda_stream
+        A.check(A.lib().nd_collect_externals(self.plan, eptr, uptr, optr, stream))
 
     # ---- mass matrix (construction.jl:409-453) ----------------------------------------------
     @property

@@ -19,7 +19,9 @@
  *   Threaded{Execution,Aggregator} src/coreloop.jl:121-129, src/aggregators.jl:192-201
  *   output wrappers            src/component_functions.jl:81-203
  *   model equations            test/ComponentLibrary.jl:8-72, benchmark/benchmark_models.jl:5-42,
- *                              test/diffusion_test.jl:57-107
+ *                              test/diffusion_test.jl:57-107, docs/src/external_inputs.md:62-99
+ *   external inputs            src/external_inputs.jl:10-77 (ExtMap, collect_externals!),
+ *                              collected after the :g passes as coreloop.jl:56-57 does
  *
  * Pinning (tests/test_oracle.py): BatchStride known answers (test/utils_test.jl:147-178);
  * the complete_graph(4) maps derived by hand from the source (fixture
@@ -89,8 +91,12 @@ typedef struct {
     /* IndexManager, network_structure.jl:1-31 */
     rng_t *v_data, *v_out, *v_para, *v_aggr;
     rng_t *e_data, *e_out_src, *e_out_dst, *e_para, *e_gb_src, *e_gb_dst;
+    rng_t *v_ext, *e_ext;
     int32_t vdepth, edepth;
-    int64_t lastidx_dynamic, lastidx_out, lastidx_p, lastidx_aggr, lastidx_gbuf;
+    int64_t lastidx_dynamic, lastidx_out, lastidx_p, lastidx_aggr, lastidx_gbuf, lastidx_extbuf;
+    /* ExtMap (external_inputs.jl:10-31): > 0 StateBufIdx, < 0 OutBufIdx; and the extbuf cache */
+    int64_t* extmap;
+    double* extbuf;
     int32_t nvb, neb;
     batch_t *vb, *eb;
     /* AggregationMap, aggregators.jl:54-86 (tightened) */
@@ -114,6 +120,7 @@ void ndo_net_destroy(ndo_net* nw) {
     free(nw->v_data); free(nw->v_out); free(nw->v_para); free(nw->v_aggr);
     free(nw->e_data); free(nw->e_out_src); free(nw->e_out_dst); free(nw->e_para);
     free(nw->e_gb_src); free(nw->e_gb_dst);
+    free(nw->v_ext); free(nw->e_ext); free(nw->extmap); free(nw->extbuf);
     for (int b = 0; b < nw->nvb; ++b) free(nw->vb[b].idx);
     for (int b = 0; b < nw->neb; ++b) free(nw->eb[b].idx);
     free(nw->vb); free(nw->eb);
@@ -173,6 +180,7 @@ ndo_net* ndo_net_create(int64_t nv, int64_t ne, const int64_t* src, const int64_
     nw->e_data = (rng_t*)xcalloc((size_t)ne, sizeof(rng_t)); nw->e_out_src = (rng_t*)xcalloc((size_t)ne, sizeof(rng_t));
     nw->e_out_dst = (rng_t*)xcalloc((size_t)ne, sizeof(rng_t)); nw->e_para = (rng_t*)xcalloc((size_t)ne, sizeof(rng_t));
     nw->e_gb_src = (rng_t*)xcalloc((size_t)ne, sizeof(rng_t)); nw->e_gb_dst = (rng_t*)xcalloc((size_t)ne, sizeof(rng_t));
+    nw->v_ext = (rng_t*)xcalloc((size_t)nv, sizeof(rng_t)); nw->e_ext = (rng_t*)xcalloc((size_t)ne, sizeof(rng_t));
 
     nw->nvb = find_identical(vkind, nv, &nw->vb);
     nw->neb = find_identical(ekind, ne, &nw->eb);
@@ -187,6 +195,7 @@ ndo_net* ndo_net_create(int64_t nv, int64_t ne, const int64_t* src, const int64_
             nw->v_out[i] = nextrange(&nw->lastidx_out, k->outdim_dst);
             nw->v_para[i] = nextrange(&nw->lastidx_p, k->pdim);
             nw->v_aggr[i] = nextrange(&nw->lastidx_aggr, nw->edepth);
+            nw->v_ext[i] = nextrange(&nw->lastidx_extbuf, k->extdim);
         }
         int64_t f = B->idx[0] - 1;
         B->state.first = nw->v_data[f].first; B->state.n = 1; B->state.s[0] = k->dim;
@@ -206,6 +215,7 @@ ndo_net* ndo_net_create(int64_t nv, int64_t ne, const int64_t* src, const int64_
             nw->e_para[i] = nextrange(&nw->lastidx_p, k->pdim);
             nw->e_gb_src[i] = nextrange(&nw->lastidx_gbuf, nw->vdepth);
             nw->e_gb_dst[i] = nextrange(&nw->lastidx_gbuf, nw->vdepth);
+            nw->e_ext[i] = nextrange(&nw->lastidx_extbuf, k->extdim);
         }
         int64_t f = B->idx[0] - 1;
         B->state.first = nw->e_data[f].first; B->state.n = 1; B->state.s[0] = k->dim;
@@ -308,6 +318,7 @@ ndo_net* ndo_net_create(int64_t nv, int64_t ne, const int64_t* src, const int64_
     nw->o = (double*)xcalloc((size_t)nw->lastidx_out, 8);
     nw->aggbuf = (double*)xcalloc((size_t)nw->lastidx_aggr, 8);
     nw->gbuf = (double*)xcalloc((size_t)nw->lastidx_gbuf, 8);
+    nw->extbuf = (double*)xcalloc((size_t)nw->lastidx_extbuf, 8);
     return nw;
 #undef FAIL
 }
@@ -333,6 +344,26 @@ void ndo_net_maps(const ndo_net* nw, int64_t* aggmap, int64_t* gmap) {
     if (n > 0) memcpy(aggmap, nw->aggmap, (size_t)n * 8);
     if (nw->lastidx_gbuf > 0) memcpy(gmap, nw->gmap, (size_t)nw->lastidx_gbuf * 8);
 }
+int64_t ndo_net_extbuf_len(const ndo_net* nw) { return nw->lastidx_extbuf; }
+void ndo_net_ext_ranges(const ndo_net* nw, int64_t* v /* nv*2 */, int64_t* e /* ne*2 */) {
+    for (int64_t i = 0; i < nw->nv; ++i) { v[2 * i] = nw->v_ext[i].first; v[2 * i + 1] = nw->v_ext[i].last; }
+    for (int64_t i = 0; i < nw->ne; ++i) { e[2 * i] = nw->e_ext[i].first; e[2 * i + 1] = nw->e_ext[i].last; }
+}
+/* ExtMap entries as the caller resolved them (StateBufIdx > 0, OutBufIdx < 0) */
+int ndo_net_set_extmap(ndo_net* nw, int64_t len, const int64_t* map) {
+    if (len != nw->lastidx_extbuf) return 1;
+    free(nw->extmap);
+    nw->extmap = (int64_t*)xcalloc((size_t)len, 8);
+    memcpy(nw->extmap, map, (size_t)len * 8);
+    return 0;
+}
+/* collect_externals!(map, extbuf, u, o), CPU version (external_inputs.jl:49-61) */
+void ndo_collect_externals(const ndo_net* nw, double* extbuf, const double* u, const double* o) {
+    for (int64_t k = 0; k < nw->lastidx_extbuf; ++k) {
+        int64_t m = nw->extmap[k];
+        extbuf[k] = m > 0 ? u[m - 1] : o[-m - 1];
+    }
+}
 int64_t ndo_net_inv_nnz(const ndo_net* nw) { return nw->inv_ptr[nw->n_inv]; }
 void ndo_net_inv(const ndo_net* nw, int64_t* dst, int64_t* ptr, int64_t* srcs) {
     memcpy(dst, nw->inv_dst, (size_t)nw->n_inv * 8);
@@ -349,9 +380,9 @@ void ndo_net_batch(const ndo_net* nw, int32_t is_edge, int32_t b, int64_t* out /
 
 static int hasff(const nd_kind* k) { return k->ff == ND_FF_PUREFEEDFORWARD || k->ff == ND_FF_FEEDFORWARD; }
 
-/* vertex f (apply_compf, coreloop.jl:213-216) */
-static void vertex_f(const nd_kind* k, double* du, const double* u, const double* acc, const double* p, double t) {
-    (void)t;
+/* vertex f (apply_compf, coreloop.jl:213-216); ext: the component's external inputs */
+static void vertex_f(const nd_kind* k, double* du, const double* u, const double* acc, const double* ext,
+                     const double* p, double t) {
     switch (k->model) {
     case ND_VM_DIFFUSION:          /* ComponentLibrary.jl:42-45 */
         du[0] = acc[0];
@@ -375,6 +406,17 @@ static void vertex_f(const nd_kind* k, double* du, const double* u, const double
         du[0] = 1.0 / tau * (P + acc[0]);
         break;
     }
+    case ND_VM_CAPACITOR_SINK:     /* docs/src/external_inputs.md:76-79: f_load */
+        du[0] = 1.0 / p[0] * (acc[0] - (1.0 + 0.1 * sin(t)));
+        break;
+    case ND_VM_PI_SOURCE: {        /* docs/src/external_inputs.md:87-92: f_source */
+        double C = p[0], Vref = p[1], Ki = p[2], Kp = p[3];
+        double d = Vref - ext[0];
+        du[1] = d;
+        double i_inj = Kp * d + Ki * u[1];
+        du[0] = 1.0 / C * (acc[0] + i_inj);
+        break;
+    }
     default:
         break;
     }
@@ -394,6 +436,9 @@ static void edge_gdst(const nd_kind* k, double* y, const double* vs, const doubl
         break;
     case ND_EM_STATIC_LINE:        /* build-defined static line with phase shift */
         y[0] = p[0] * sin(vs[0] - vd[0] - p[1]);
+        break;
+    case ND_EM_RESISTOR:           /* docs/src/external_inputs.md:66-68: resistor_g */
+        y[0] = (vs[0] - vd[0]) / p[0];
         break;
     default:
         break;
@@ -461,7 +506,8 @@ static void apply_vertex(const ndo_net* nw, const batch_t* B, int64_t i, int pas
         for (int j = 0; j < k->outdim_dst; ++j) o[orr.first - 1 + j] = u[sr.first - 1 + k->mask_dst[j] - 1];
     } else {
         rng_t ir = bs_range(&B->in, i);
-        vertex_f(k, du + sr.first - 1, u + sr.first - 1, aggbuf + ir.first - 1, p + pr.first - 1, t);
+        const double* ext = nw->extbuf + nw->v_ext[B->idx[i - 1] - 1].first - 1;
+        vertex_f(k, du + sr.first - 1, u + sr.first - 1, aggbuf + ir.first - 1, ext, p + pr.first - 1, t);
     }
 }
 
@@ -553,6 +599,7 @@ int ndo_rhs(ndo_net* nw, double* du, const double* u, const double* p, double t,
         if (hasff(&nw->kinds[B->kind])) continue;
         for (int64_t i = 1; i <= B->n; ++i) apply_edge(nw, B, i, 0, 1, buffered, du, u, o, gbuf, p, t);
     }
+    if (nw->lastidx_extbuf > 0) ndo_collect_externals(nw, nw->extbuf, u, o);     /* coreloop.jl:56-57 */
     if (buffered) ndo_gather(nw, gbuf, o);                                       /* coreloop.jl:67 */
     for (int b = 0; b < nw->neb; ++b) {                                          /* coreloop.jl:76 */
         const batch_t* B = &nw->eb[b];
@@ -611,6 +658,7 @@ int ndo_rhs_threaded(ndo_net* nw, double* du, const double* u, const double* p, 
 #endif
         for (int64_t i = 1; i <= B->n; ++i) apply_edge(nw, B, i, 0, 1, 1, du, u, o, gbuf, p, t);
     }
+    if (nw->lastidx_extbuf > 0) ndo_collect_externals(nw, nw->extbuf, u, o);
     ndo_gather(nw, gbuf, o);
     for (int b = 0; b < nw->neb; ++b) {
         const batch_t* B = &nw->eb[b];
@@ -644,7 +692,6 @@ int ndo_rhs_threaded(ndo_net* nw, double* du, const double* u, const double* p, 
    term replaced by its absolute value (|terms| summed), so |du_gpu - du| <= rtol * S
    is meaningful where du itself cancels to ~0. */
 void ndo_rhs_scale(ndo_net* nw, double* S, const double* u, const double* p, double t) {
-    (void)t;
     int64_t nout = nw->lastidx_out;
     double* o = nw->o;
     double* agg = nw->aggbuf;
@@ -668,6 +715,13 @@ void ndo_rhs_scale(ndo_net* nw, double* S, const double* u, const double* p, dou
             case ND_VM_SWING: s[0] = fabs(x[1]); s[1] = fabs(1.0 / q[0]) * (fabs(q[2]) + fabs(q[1] * x[1]) + a); break;
             case ND_VM_SWING_UNIT_DAMPING: s[0] = fabs(x[1]); s[1] = fabs(q[0]) + fabs(x[1]) + a; break;
             case ND_VM_LOAD: s[0] = fabs(1.0 / q[0]) * (fabs(q[1]) + a); break;
+            case ND_VM_CAPACITOR_SINK: s[0] = fabs(1.0 / q[0]) * (a + 1.0 + fabs(0.1 * sin(t))); break;
+            case ND_VM_PI_SOURCE: {
+                const double ex = fabs(nw->extbuf[nw->v_ext[B->idx[i - 1] - 1].first - 1]);
+                s[1] = fabs(q[1]) + ex;
+                s[0] = fabs(1.0 / q[0]) * (a + fabs(q[3]) * s[1] + fabs(q[2] * x[1]));
+                break;
+            }
             default: break;
             }
         }

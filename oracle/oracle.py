@@ -25,7 +25,8 @@ class NDKind(C.Structure):
     _fields_ = [("is_edge", C.c_int32), ("model", C.c_int32), ("wrapper", C.c_int32),
                 ("ff", C.c_int32), ("dim", C.c_int32), ("pdim", C.c_int32),
                 ("outdim_src", C.c_int32), ("outdim_dst", C.c_int32),
-                ("mask_src", C.c_int32 * ND_MAX_OUT), ("mask_dst", C.c_int32 * ND_MAX_OUT)]
+                ("mask_src", C.c_int32 * ND_MAX_OUT), ("mask_dst", C.c_int32 * ND_MAX_OUT),
+                ("extdim", C.c_int32)]
 
 
 def build():
@@ -57,6 +58,12 @@ def lib():
         L.ndo_net_inv_nnz.argtypes = [P]
         L.ndo_net_inv.argtypes = [P, i64p, i64p, i64p]
         L.ndo_net_batch.argtypes = [P, C.c_int32, C.c_int32, i64p, P]
+        L.ndo_net_extbuf_len.restype = C.c_int64
+        L.ndo_net_extbuf_len.argtypes = [P]
+        L.ndo_net_ext_ranges.argtypes = [P, i64p, i64p]
+        L.ndo_net_set_extmap.restype = C.c_int
+        L.ndo_net_set_extmap.argtypes = [P, C.c_int64, i64p]
+        L.ndo_collect_externals.argtypes = [P, f64p, f64p, f64p]
         L.ndo_rhs.argtypes = [P, P, f64p, f64p, C.c_double, C.c_int32, P, P]
         L.ndo_rhs.restype = C.c_int
         L.ndo_rhs_threaded.argtypes = [P, f64p, f64p, f64p, C.c_double, C.c_int32]
@@ -86,6 +93,7 @@ def make_kind_array(kinds):
             a.mask_src[j] = int(m)
         for j, m in enumerate(k.get("mask_dst", ())):
             a.mask_dst[j] = int(m)
+        a.extdim = int(k.get("extdim", 0))
     return arr
 
 
@@ -136,6 +144,26 @@ class OracleNet:
         e = np.zeros((max(self.ne, 1), 6, 2), np.int64)
         lib().ndo_net_ranges(self._h, v.reshape(-1), e.reshape(-1))
         return v, e[: self.ne]
+
+    def ext_ranges(self):
+        v = np.zeros((self.nv, 2), np.int64)
+        e = np.zeros((max(self.ne, 1), 2), np.int64)
+        lib().ndo_net_ext_ranges(self._h, v.reshape(-1), e.reshape(-1))
+        return v, e[: self.ne]
+
+    @property
+    def lastidx_extbuf(self):
+        return int(lib().ndo_net_extbuf_len(self._h))
+
+    def set_extmap(self, m):
+        m = np.ascontiguousarray(m, np.int64)
+        if lib().ndo_net_set_extmap(self._h, len(m), m):
+            raise ValueError("ExtMap length differs from the external buffer")
+
+    def collect_externals(self, u, o):
+        ext = np.empty(max(self.lastidx_extbuf, 1))
+        lib().ndo_collect_externals(self._h, ext, np.ascontiguousarray(u, np.float64), np.ascontiguousarray(o, np.float64))
+        return ext[: self.lastidx_extbuf]
 
     def maps(self):
         n = max(self.agg_last - self.agg_first + 1, 0)

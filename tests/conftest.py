@@ -27,8 +27,12 @@ def oracle():
 
 
 def oracle_for(nw):
-    """The oracle network for a product Network (same graph, kinds and kind assignment)."""
-    return _oracle.OracleNet(nw.nv(), nw.g.src, nw.g.dst, nw.kinds, nw.vertex_kind, nw.edge_kind)
+    """The oracle network for a product Network (same graph, kinds and kind assignment,
+    and the same ExtMap where the network has external inputs)."""
+    on = _oracle.OracleNet(nw.nv(), nw.g.src, nw.g.dst, nw.kinds, nw.vertex_kind, nw.edge_kind)
+    if nw.info.lastidx_extbuf > 0:
+        on.set_extmap(nw.extmap())
+    return on
 
 
 def fd_jvp(on, u, v, p, h=1e-3):
