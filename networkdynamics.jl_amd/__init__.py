@@ -37,4 +37,5 @@ from . import warmstart
 from . import experiments
 from . import ensemble_tune
 from . import linear_analysis
+from . import sparsity
 from .initialization import NetworkInitError, find_fixpoint

@@ -4,12 +4,13 @@ The reference wraps `nw` as a SteadyStateProblem at fixed t and hands it to
 NonlinearSolve (SSRootfind).  Here Newton's method runs on the device network:
 every residual is one `nw(du, x, p, t)` launch and every Jacobian column one
 dual-number JVP launch (`Network.jacobian`).  The Newton step solves J s = -du on the
-host: rank-revealing least squares for dense Jacobians (the global phase of a
-Kuramoto-type network leaves J singular, and least squares takes the minimum-norm
-step), or GMRES whose products are device JVPs for networks above `dense_max`
-states.  A backtracking line search on ||du||_2 keeps flat starts robust.  Grids of
-swing/load vertices in bulk belong to `FixpointSolver`, which runs thousands of them
-in one launch.
+host: rank-revealing least squares on the dense Jacobian for small networks (the
+global phase of a Kuramoto-type network leaves J singular, and least squares takes
+the minimum-norm step), a sparse LU of the colored sparse Jacobian (one JVP launch
+per color) above `dense_max` states, or GMRES whose products are device JVPs.  A
+backtracking line search on ||du||_2 keeps flat starts robust.  Grids of swing/load
+vertices in bulk belong to `FixpointSolver`, which runs thousands of them in one
+launch.
 
 Errors follow the reference: a NaN right-hand side at x0 raises NetworkInitError
 naming the NaN states and the NaN inputs (initialization.jl:78-120); a solve that
@@ -51,15 +52,28 @@ def _nan_report(nw, x, p, du, t):
     return "\n".join(lines)
 
 
-def _newton_step(nw, x, p, t, du, dense_max, gmres_rtol):
-    """s with J(x) s = -du (least-squares / GMRES), as a host float64 vector."""
+def _newton_step(nw, x, p, t, du, linsolve, gmres_rtol, sparse_state):
+    """s with J(x) s = -du, as a host float64 vector.  dense: least squares on the dense
+    device Jacobian; sparse: LU of J + mu*I on the colored sparse Jacobian, mu = 1e-12
+    max|J| so that the null direction of a phase-invariant network gets a bounded
+    component; gmres: matrix-free with device JVPs."""
     import torch
     n = nw.dim
     rhs = -du.cpu().numpy()
-    if n <= dense_max:
+    if linsolve == "dense":
         from scipy.linalg import lstsq
         J = nw.jacobian(x, p, t).cpu().numpy()
         return lstsq(J, rhs, lapack_driver="gelsy")[0]
+    if linsolve == "sparse":
+        from scipy.sparse import identity
+        from scipy.sparse.linalg import splu
+        from .sparsity import color_columns, jac_prototype, sparse_jacobian
+        if "pattern" not in sparse_state:
+            sparse_state["pattern"] = jac_prototype(nw)
+            sparse_state["colors"] = color_columns(sparse_state["pattern"])[0]
+        J = sparse_jacobian(nw, x, p, t, sparse_state["pattern"], sparse_state["colors"])
+        mu = 1e-12 * (np.abs(J.data).max() if J.nnz else 1.0)
+        return splu((J + mu * identity(n, format="csr")).tocsc()).solve(rhs)
     from scipy.sparse.linalg import LinearOperator, gmres
     v_d = torch.empty(n, dtype=torch.float64, device=x.device)
     jv = torch.empty_like(v_d)
@@ -75,11 +89,19 @@ def _newton_step(nw, x, p, t, du, dense_max, gmres_rtol):
     return s
 
 
-def find_fixpoint(nw, x0, p, t=float("nan"), tol=1e-10, maxiters=50, dense_max=4096, gmres_rtol=1e-10):
+def find_fixpoint(nw, x0, p, t=float("nan"), tol=1e-10, maxiters=50, linsolve="auto", dense_max=2048,
+                  gmres_rtol=1e-10):
     """find_fixpoint(nw, x0, p; t) -- src/initialization.jl:73-144.  x0, p: host arrays or
-    float64 device tensors.  Returns (x, info): x the fixpoint as a float64 device tensor
-    (max |du| <= tol, the isfixpoint judge of linear_analysis.jl:8-15), info the Newton
-    iterations, line-search halvings and final residual."""
+    float64 device tensors.  linsolve: "dense", "sparse" (colored Jacobian, sparse LU),
+    "gmres" (matrix-free), or "auto" (dense up to dense_max states, else sparse).
+    Returns (x, info): x the fixpoint as a float64 device tensor (max |du| <= tol, the
+    isfixpoint judge of linear_analysis.jl:8-15), info the Newton iterations, line-search
+    halvings and final residual."""
+    if linsolve == "auto":
+        linsolve = "dense" if nw.dim <= dense_max else "sparse"
+    if linsolve not in ("dense", "sparse", "gmres"):
+        raise A.ArgumentError(f"linsolve must be dense, sparse, gmres or auto, got {linsolve!r}")
+    sparse_state = {}
     import torch
     dev = torch.device("cuda", torch.cuda.current_device())
     x = torch.as_tensor(np.asarray(x0, dtype=np.float64) if not isinstance(x0, torch.Tensor) else x0,
@@ -103,7 +125,7 @@ def find_fixpoint(nw, x0, p, t=float("nan"), tol=1e-10, maxiters=50, dense_max=4
             return x, dict(iters=it, halvings=halvings, residual=res)
         if it == maxiters:
             break
-        s = torch.from_numpy(_newton_step(nw, x, pd, t, du, dense_max, gmres_rtol)).to(dev)
+        s = torch.from_numpy(_newton_step(nw, x, pd, t, du, linsolve, gmres_rtol, sparse_state)).to(dev)
         lam, accepted = 1.0, False
         while lam >= 1.0 / 1024:
             torch.add(x, s, alpha=lam, out=trial)

@@ -173,10 +173,11 @@ def test_find_fixpoint_ring_known_answer(ND, dev):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dense_max", [4096, 0])
-def test_find_fixpoint_dae_grid_matches_batched_solver(ND, dev, dense_max):
-    # algebraic loads (mass 0), flat start; the dense least-squares and the JVP-GMRES steps
-    # both reach the batched solver's fixpoint up to the global phase
+@pytest.mark.parametrize("linsolve", ["dense", "sparse", "gmres"])
+def test_find_fixpoint_dae_grid_matches_batched_solver(ND, dev, linsolve):
+    # algebraic loads (mass 0), flat start; dense least squares, sparse LU of the colored
+    # Jacobian and JVP-GMRES steps all reach the batched solver's fixpoint up to the
+    # global phase
     import torch
     grid = ND.swing_load_grid(300, seed=47)
     n = grid.g.nv()
@@ -187,7 +188,7 @@ def test_find_fixpoint_dae_grid_matches_batched_solver(ND, dev, dense_max):
                                                    torch.from_numpy(grid.K).to(dev)))
     assert st["retcode"][0] == 0
     nw = grid.network(algebraic_loads=True)
-    x, info = ND.find_fixpoint(nw, np.zeros(nw.dim), grid.flat_p(), dense_max=dense_max)
+    x, info = ND.find_fixpoint(nw, np.zeros(nw.dim), grid.flat_p(), linsolve=linsolve)
     assert info["residual"] <= 1e-10
     v, _ = nw.ranges()
     got = x.cpu().numpy()[v[:, 0, 0] - 1]
