@@ -38,4 +38,5 @@ from . import experiments
 from . import ensemble_tune
 from . import linear_analysis
 from . import sparsity
+from . import dae
 from .initialization import NetworkInitError, find_fixpoint
