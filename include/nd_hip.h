@@ -67,6 +67,13 @@ enum {
  *   CAPACITOR_SINK     docs/src/external_inputs.md:76-82     dV = 1/C*(acc - (1 + 0.1*sin(t))), p = [C]
  *   PI_SOURCE          docs/src/external_inputs.md:86-99     d = Vref - ext[1]; dxi = d;
  *                      (external input: a remote voltage)    dV = 1/C*(acc + (Kp*d + Ki*xi)), p = [C, Vref, Ki, Kp]
+ *   CAPACITOR          test/loopback_test.jl:53-63           dv = acc / C,                 p = [C]
+ *   VOLTAGE_SOURCE     test/loopback_test.jl:19-34           no state; output v = V (NoFeedForward), p = [V]
+ *   RESISTOR_INJECTOR  test/loopback_test.jl:66-78           no state; output i = in / R (PureFeedForward:
+ *                                                            an injector fed the hub potential), p = [R]
+ *   INDUCTOR_INJECTOR  test/loopback_test.jl:80-92           di = in / L, output i (StateMask), p = [L]
+ *   CRL                test/loopback_test.jl:113-131         dv = (acc - v/R - iL) / C; diL = v / L,
+ *                                                            output v, p = [C, R, L]
  *   NOOP               f does nothing (aggregator tests, test/aggregators_test.jl:15-18)  */
 enum {
     ND_VM_NOOP = 0,
@@ -76,7 +83,12 @@ enum {
     ND_VM_SWING_UNIT_DAMPING = 4,
     ND_VM_LOAD = 5,
     ND_VM_CAPACITOR_SINK = 6,
-    ND_VM_PI_SOURCE = 7
+    ND_VM_PI_SOURCE = 7,
+    ND_VM_CAPACITOR = 8,
+    ND_VM_VOLTAGE_SOURCE = 9,
+    ND_VM_RESISTOR_INJECTOR = 10,
+    ND_VM_INDUCTOR_INJECTOR = 11,
+    ND_VM_CRL = 12
 };
 
 /* Edge models.  Reference twins:
@@ -89,6 +101,9 @@ enum {
  *   ODE_SIN_DIFFUSION_FAST benchmark/benchmark_models.jl:11-16 de = 100.0*(sin(+-(v_s - v_d)) - e)
  *   ODE_LINEAR           test/diffusion_test.jl:102-107        de = +-(v_s - v_d) - e
  *   RESISTOR             docs/src/external_inputs.md:66-72     y = (v_s - v_d)/R,   p = [R]
+ *   LOOPBACK             src/post_utils.jl:105-108             Directed: y_dst = -1 * v_s, the
+ *                        LoopbackConnection from an injector (src) to its hub (dst); the hub's
+ *                        output is copied to the injector's input (post_utils.jl:221-228)
  *   NOOP                 g does nothing (test/aggregators_test.jl:20-30)                    */
 enum {
     ND_EM_NOOP = 0,
@@ -100,7 +115,8 @@ enum {
     ND_EM_ODE_SIN_DIFFUSION = 6,
     ND_EM_ODE_SIN_DIFFUSION_FAST = 7,
     ND_EM_ODE_LINEAR = 8,
-    ND_EM_RESISTOR = 9
+    ND_EM_RESISTOR = 9,
+    ND_EM_LOOPBACK = 10
 };
 
 #define ND_MAX_OUT 4
@@ -231,8 +247,8 @@ typedef struct nd_plan nd_plan;
 
 /* Uploads the network's index structures to the current HIP device and selects
  * the fused kernels.  ND_ERR_UNSUPPORTED if a model has no HIP kernel (external
- * inputs of edges, loopback, feed-forward vertices, output depth > 1 with a functional
- * model); ND_ERR_ARGUMENT if the network has external inputs and no ExtMap
+ * inputs of edges, ODE edges next to computed vertex outputs, output depth > 1 with a
+ * functional model); ND_ERR_ARGUMENT if the network has external inputs and no ExtMap
  * (nd_network_set_extmap).  Vertex external inputs resolve to state indices: a
  * vertex output and an ODE edge's output are StateMasks of states. */
 int nd_plan_create(const nd_network* nw, nd_plan** out);

@@ -60,9 +60,10 @@ class DeviceFunction:
     g(osrc, odst, v_s, v_d, p, t)), or "edge_f" (state function of an ODE edge).
     """
 
-    def __init__(self, name, role, model, dim=None, pdim=0, ref="", ext=0):
+    def __init__(self, name, role, model, dim=None, pdim=0, ref="", ext=0, ff=None):
         self.name, self.role, self.model, self.dim, self.pdim, self.ref = name, role, model, dim, pdim, ref
         self.ext = ext          # external inputs the function reads
+        self.ff = ff            # role "vertex_g": the output's feed-forward trait
 
     def __repr__(self):
         return f"<device function {self.name}>"
@@ -314,11 +315,15 @@ class VertexModel(ComponentModel):
 
     def __init__(self, f=None, g=None, dim=None, sym=None, pdim=None, psym=None, outdim=None, name=None,
                  mass_matrix=None, extin=None):
+        if isinstance(g, DeviceFunction) and g.role == "vertex_g":
+            self._init_output_model(f, g, dim, sym, pdim, psym, outdim, name, mass_matrix, extin)
+            return
         if f is None:
             f = noop_vertex
         if not isinstance(f, DeviceFunction) or f.role != "vertex_f":
             raise A.UnsupportedError(f"vertex f must be a vertex device function of the library, got {f!r}")
         self.f = f
+        self._gfn = None
         self.model = f.model
         if sym is not None:
             self.sym, self.sym_default = _split_sym(sym)
@@ -362,8 +367,38 @@ class VertexModel(ComponentModel):
         self.mass_matrix = _normalize_mass(mass_matrix, self.dim)
         self.extin = _extin(extin, f)
 
+    def _init_output_model(self, f, g, dim, sym, pdim, psym, outdim, name, mass_matrix, extin):
+        """A vertex whose output is computed by a library g without states: NoFeedForward
+        g(out, p, t) or an injector's PureFeedForward g(out, in, p, t)."""
+        if f is not None:
+            raise A.UnsupportedError(f"{g.name} has no state function in the library")
+        if (dim or 0) != 0 or sym:
+            raise A.ArgumentError(f"{g.name} has no states")
+        if outdim is not None and outdim != 1:
+            raise A.ArgumentError("outdim and g disagree")
+        self.f, self._gfn, self.model = None, g, g.model
+        self.sym, self.sym_default, self.dim = [], [], 0
+        if psym is not None:
+            self.psym, self.psym_default = _split_sym(psym)
+        else:
+            pd = pdim if pdim is not None else g.pdim
+            self.psym = [f"p{i + 1}" for i in range(pd)] if pd != 1 else ["p"]
+            self.psym_default = [None] * pd
+        self.pdim = len(self.psym)
+        if self.pdim != g.pdim:
+            raise A.ArgumentError(f"{g.name} needs pdim={g.pdim}")
+        self.g = g
+        self.ff = g.ff
+        self.wrapper = A.ND_WRAP_NONE
+        self.outdim_src, self.outdim_dst = 0, 1
+        self.mask_src, self.mask_dst = (), ()
+        self.name = name or g.name
+        self.mass_matrix = _normalize_mass(mass_matrix, 0)
+        self.extin = _extin(extin, None)
+
     def __repr__(self):
-        return f"VertexModel :{self.name} (dim={self.dim}, pdim={self.pdim}, f={self.f.name})"
+        fn = self.f.name if self.f is not None else self.g.name
+        return f"VertexModel :{self.name} (dim={self.dim}, pdim={self.pdim}, {'f' if self.f is not None else 'g'}={fn})"
 
 
 class EdgeModel(ComponentModel):
@@ -500,3 +535,13 @@ resistor_g = DeviceFunction("resistor_g", "edge_g", A.ND_EM_RESISTOR, 0, 1, "doc
 f_load = DeviceFunction("f_load", "vertex_f", A.ND_VM_CAPACITOR_SINK, 1, 1, "docs/src/external_inputs.md:76-82")
 f_source = DeviceFunction("f_source", "vertex_f", A.ND_VM_PI_SOURCE, 2, 4, "docs/src/external_inputs.md:86-99",
                           ext=1)
+# test/loopback_test.jl:19-131: the components of the injector network and of its merged twin
+capacitor_f = DeviceFunction("capacitor", "vertex_f", A.ND_VM_CAPACITOR, 1, 1, "test/loopback_test.jl:53-63")
+voltage_source_g = DeviceFunction("voltage_source", "vertex_g", A.ND_VM_VOLTAGE_SOURCE, 0, 1,
+                                  "test/loopback_test.jl:19-34", ff=NoFeedForward())
+resistor_injector_g = DeviceFunction("resistor_injector", "vertex_g", A.ND_VM_RESISTOR_INJECTOR, 0, 1,
+                                     "test/loopback_test.jl:66-78", ff=PureFeedForward())
+inductor_injector_f = DeviceFunction("inductor_injector", "vertex_f", A.ND_VM_INDUCTOR_INJECTOR, 1, 1,
+                                     "test/loopback_test.jl:80-92")
+crl_f = DeviceFunction("crl", "vertex_f", A.ND_VM_CRL, 2, 3, "test/loopback_test.jl:113-131")
+loopback_g = DeviceFunction("LOOPBACK_G", "edge_g", A.ND_EM_LOOPBACK, 0, 0, "src/post_utils.jl:105-108")

@@ -49,6 +49,7 @@ __device__ __forceinline__ S g_dst(S vs, S vd, const double* __restrict__ pe) {
     else if constexpr (EM == ND_EM_DIFFUSION_WEIGHTED) return pe[0] * (vs - vd);  // ComponentLibrary.jl:8-10
     else if constexpr (EM == ND_EM_DIFFUSION_FIDUCIAL) return pe[0] * (vs - vd);  // ComponentLibrary.jl:22-25
     else if constexpr (EM == ND_EM_RESISTOR) return (vs - vd) / pe[0];            // external_inputs.md:66-68
+    else if constexpr (EM == ND_EM_LOOPBACK) return -1.0 * vs;                     // post_utils.jl:105-108
     else return nan_of<S>();
 }
 
@@ -61,6 +62,7 @@ __device__ __forceinline__ S g_dst_rt(int em, S vs, S vd, const double* __restri
     case ND_EM_DIFFUSION_WEIGHTED: return g_dst<ND_EM_DIFFUSION_WEIGHTED, S>(vs, vd, pe);
     case ND_EM_DIFFUSION_FIDUCIAL: return g_dst<ND_EM_DIFFUSION_FIDUCIAL, S>(vs, vd, pe);
     case ND_EM_RESISTOR: return g_dst<ND_EM_RESISTOR, S>(vs, vd, pe);
+    case ND_EM_LOOPBACK: return g_dst<ND_EM_LOOPBACK, S>(vs, vd, pe);
     default: return nan_of<S>();
     }
 }
@@ -74,6 +76,11 @@ template <> struct VModel<ND_VM_SWING_UNIT_DAMPING> { static constexpr int dim =
 template <> struct VModel<ND_VM_LOAD> { static constexpr int dim = 1; };
 template <> struct VModel<ND_VM_CAPACITOR_SINK> { static constexpr int dim = 1; };
 template <> struct VModel<ND_VM_PI_SOURCE> { static constexpr int dim = 2; };
+template <> struct VModel<ND_VM_CAPACITOR> { static constexpr int dim = 1; };
+template <> struct VModel<ND_VM_VOLTAGE_SOURCE> { static constexpr int dim = 0; };
+template <> struct VModel<ND_VM_RESISTOR_INJECTOR> { static constexpr int dim = 0; };
+template <> struct VModel<ND_VM_INDUCTOR_INJECTOR> { static constexpr int dim = 1; };
+template <> struct VModel<ND_VM_CRL> { static constexpr int dim = 2; };
 
 // vertex f (apply_compf, coreloop.jl:213-216); operation order as the reference writes it
 template <int VM, typename S = double>
@@ -114,9 +121,30 @@ __device__ __forceinline__ void vertex_f_ext(S* __restrict__ dx, const S* __rest
         dx[1] = d;                                         // integrator state
         const S inj = pv[3] * d + pv[2] * x[1];            // Kp*d + Ki*xi
         dx[0] = 1.0 / pv[0] * (acc + inj);
+    } else if constexpr (VM == ND_VM_CAPACITOR) {          // loopback_test.jl:61: D(v) ~ i / C
+        dx[0] = acc / pv[0];
+    } else if constexpr (VM == ND_VM_INDUCTOR_INJECTOR) {  // loopback_test.jl:89: D(i) ~ v / L
+        dx[0] = acc / pv[0];
+    } else if constexpr (VM == ND_VM_CRL) {                // loopback_test.jl:113-131
+        const S v = x[0];
+        dx[0] = (acc - v / pv[1] - x[1]) / pv[0];          // cap.p.i = i - v/R - iL
+        dx[1] = v / pv[2];
     } else {
         vertex_f<VM, S>(dx, x, acc, pv, dim);
     }
+}
+
+// Computed vertex outputs: NoFeedForward g(out, x, p, t) and the PureFeedForward g of an
+// injector, g(out, in, p, t), whose input is its hub's output (the loopback copy).
+template <typename S>
+__device__ __forceinline__ S vertex_out_nff(int vm, const double* __restrict__ pv) {
+    if (vm == ND_VM_VOLTAGE_SOURCE) return S{} + pv[0];    // loopback_test.jl:31: p.v ~ V
+    return nan_of<S>();
+}
+template <typename S>
+__device__ __forceinline__ S vertex_out_ff(int vm, S in, const double* __restrict__ pv) {
+    if (vm == ND_VM_RESISTOR_INJECTOR) return in / pv[0];   // loopback_test.jl:75: p.i ~ p.v / R
+    return nan_of<S>();
 }
 
 // Parameters each edge model reads (p entries per edge).
@@ -135,6 +163,11 @@ template <> struct VPar<ND_VM_SWING_UNIT_DAMPING> { static constexpr int np = 1;
 template <> struct VPar<ND_VM_LOAD> { static constexpr int np = 2; };
 template <> struct VPar<ND_VM_CAPACITOR_SINK> { static constexpr int np = 1; };
 template <> struct VPar<ND_VM_PI_SOURCE> { static constexpr int np = 4; };
+template <> struct VPar<ND_VM_CAPACITOR> { static constexpr int np = 1; };
+template <> struct VPar<ND_VM_VOLTAGE_SOURCE> { static constexpr int np = 1; };
+template <> struct VPar<ND_VM_RESISTOR_INJECTOR> { static constexpr int np = 1; };
+template <> struct VPar<ND_VM_INDUCTOR_INJECTOR> { static constexpr int np = 1; };
+template <> struct VPar<ND_VM_CRL> { static constexpr int np = 3; };
 
 // external inputs each vertex model reads
 template <int VM> struct VExt { static constexpr int n = 0; };

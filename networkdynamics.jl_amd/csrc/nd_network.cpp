@@ -94,6 +94,28 @@ static int build(Network& n) {
                              k.outdim_src, n.edepth);
     }
 
+    // feed-forward vertices and LoopbackConnection edges (construction.jl:55-83)
+    {
+        std::vector<int64_t> deg(static_cast<size_t>(nv), 0), only(static_cast<size_t>(nv), -1);
+        for (int64_t e = 0; e < ne; ++e) {
+            ++deg[n.src[e] - 1]; ++deg[n.dst[e] - 1];
+            only[n.src[e] - 1] = e; only[n.dst[e] - 1] = e;
+        }
+        auto loopback = [&](int64_t e) { return n.kinds[n.ekind[e]].model == ND_EM_LOOPBACK; };
+        for (int64_t v = 0; v < nv; ++v) {
+            const int32_t ff = n.kinds[n.vkind[v]].ff;
+            if (ff != ND_FF_PUREFEEDFORWARD && ff != ND_FF_FEEDFORWARD) continue;
+            if (deg[v] != 1 || !loopback(only[v]) || n.src[only[v]] - 1 != v)
+                return set_error(ND_ERR_ARGUMENT, "Feed forward vertex models are only allowed as leave nodes (1 "
+                                 "neighbor) with single `LoopbackConnection` from satelite to cluster node! (vertex %lld)",
+                                 (long long)(v + 1));
+        }
+        for (int64_t e = 0; e < ne; ++e)
+            if (loopback(e) && deg[n.src[e] - 1] != 1)
+                return set_error(ND_ERR_ARGUMENT, "All LoopbackConnection edges must originate from leaf nodes! "
+                                 "Found at least one going from hub to leaf (edge %lld)", (long long)(e + 1));
+    }
+
     group_by_kind(n.vkind, static_cast<int32_t>(n.kinds.size()), n.vb, n.v_batch, n.v_pos);
     group_by_kind(n.ekind, static_cast<int32_t>(n.kinds.size()), n.eb, n.e_batch, n.e_pos);
 

@@ -59,6 +59,11 @@ struct VDesc {            // one vertex batch as the kernels see it (0-based off
     int32_t generic;
     int32_t ext0, extdim; // first external-buffer entry of the batch (0-based), per vertex
     double t;
+    // computed outputs: 0 StateMask, 1 NoFeedForward g(x, p), 2 an injector's
+    // PureFeedForward g(in, p); lbin[lb0 + q]: the o-index of the hub output copied to
+    // an injector's input (post_utils.jl:221-228), -1 for other vertices
+    int32_t outkind, lb0;
+    const int32_t* lbin;
 };
 
 // Slot-ordered edge parameters: a copy of the edge batch's parameters laid out like the
@@ -549,7 +554,7 @@ __device__ __forceinline__ void ksync_edge(const KSync& ks, int64_t e, const dou
 }
 
 __global__ void __launch_bounds__(256) k_vertex_out(VDesc vd, const double* __restrict__ u, double* __restrict__ o,
-                                                    KSync ks) {
+                                                    KSync ks, const double* __restrict__ p, int tangent) {
     if (static_cast<int32_t>(blockIdx.x) >= ks.vblocks) {
         const int64_t nthr = static_cast<int64_t>(gridDim.x - ks.vblocks) * blockDim.x;
         const int64_t tid = static_cast<int64_t>(blockIdx.x - ks.vblocks) * blockDim.x + threadIdx.x;
@@ -573,6 +578,13 @@ __global__ void __launch_bounds__(256) k_vertex_out(VDesc vd, const double* __re
     const int64_t nthr = static_cast<int64_t>(ks.vblocks) * blockDim.x;
     const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     // two-state vertices with output state 1 (swing): two vertices = one 32-byte read
+    if (vd.outkind == 2) return;                 // injector outputs: k_vertex_out_ff, after the hubs'
+    if (vd.outkind == 1) {                       // NoFeedForward outputs from the parameters
+        for (int64_t q = tid; q < vd.count; q += nthr)
+            o[static_cast<int64_t>(vd.out0) + q] =
+                tangent ? 0.0 : vertex_out_nff<double>(vd.model, p + vd.param0 + q * vd.pdim);
+        return;
+    }
     const bool vec2 = vd.dim == 2 && vd.depth == 1 && vd.mask[0] == 0 && (vd.state0 & 1) == 0 &&
                       (reinterpret_cast<uintptr_t>(u) & 15) == 0 && ((vd.out0 & 1) == 0) &&
                       (reinterpret_cast<uintptr_t>(o) & 15) == 0;
@@ -589,6 +601,37 @@ __global__ void __launch_bounds__(256) k_vertex_out(VDesc vd, const double* __re
             }
         }
     }
+}
+
+// Injector outputs (coreloop.jl:44-55): the loopback copy of the hub's output is the
+// injector's input, and its PureFeedForward g turns it into the output.  vo: the
+// output buffer holding the hubs' outputs (written by k_vertex_out before).
+__global__ void __launch_bounds__(256) k_vertex_out_ff(VDesc vd, double* __restrict__ vo,
+                                                       const double* __restrict__ p) {
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= vd.count) return;
+    const double in = vo[vd.lbin[vd.lb0 + q]];
+    vo[static_cast<int64_t>(vd.out0) + q] = vertex_out_ff<double>(vd.model, in, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim);
+}
+// and their tangents, from the hubs' tangent outputs
+__global__ void __launch_bounds__(256) k_vertex_out_ff_jvp(VDesc vd, double* __restrict__ vo, double* __restrict__ dvo,
+                                                           const double* __restrict__ p) {
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= vd.count) return;
+    const int32_t h = vd.lbin[vd.lb0 + q];
+    const Dual y = vertex_out_ff<Dual>(vd.model, Dual{vo[h], dvo[h]}, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim);
+    vo[static_cast<int64_t>(vd.out0) + q] = y.v;
+    dvo[static_cast<int64_t>(vd.out0) + q] = y.d;
+}
+
+// apply_loopback! into the aggregation buffer (post_utils.jl:221-228): an injector's
+// input row holds its hub's output before the edge outputs are added
+__global__ void __launch_bounds__(256) k_loopback_copy(VDesc vd, const double* __restrict__ o,
+                                                       double* __restrict__ aggbuf) {
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= vd.count) return;
+    const int32_t h = vd.lbin[vd.lb0 + q];
+    if (h >= 0) aggbuf[vd.row0 + q] = o[h];
 }
 
 // both outputs of one edge batch (coreloop.jl:41,78 with the wrappers of
@@ -664,15 +707,22 @@ __global__ void __launch_bounds__(256) k_vertex_gen(VDesc vd, const int32_t* __r
     constexpr int DIMC = VModel<VM>::dim;
     constexpr int NX = VExt<VM>::n > 0 ? VExt<VM>::n : 1;
     const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if constexpr (DIMC <= 0) {
+        return;                                  // no states: only an output (k_vertex_out*)
+    } else {
     if (q >= vd.count) return;
     const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * DIMC;
     double x[DIMC];
 #pragma unroll
     for (int j = 0; j < DIMC; ++j) x[j] = u[x0 + j];
-    const double own = x[vd.mask[0]];
+    const double own = vd.outkind == 0 ? x[vd.mask[0]] : vo[static_cast<int64_t>(vd.out0) + q];
     double acc = 0.0;
     const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
     for (int32_t k = beg; k < end; ++k) acc = acc + slot_value_rt(ed[slot_eb ? slot_eb[k] : 0], slots[k], own, u, vo, p);
+    if (vd.lbin) {                               // an injector: its input is the hub's output
+        const int32_t h = vd.lbin[vd.lb0 + q];
+        if (h >= 0) acc = acc + vo[h];
+    }
     double ext[NX];
 #pragma unroll
     for (int j = 0; j < NX; ++j)
@@ -683,6 +733,7 @@ __global__ void __launch_bounds__(256) k_vertex_gen(VDesc vd, const int32_t* __r
     for (int j = 0; j < DIMC; ++j) {
         const double y = emit1<STAGE>(du, sa, x0 + j, dx[j]);
         if (STAGE && sa.vnext && j == vd.mask[0]) sa.vnext[static_cast<int64_t>(vd.out0) + q] = y;
+    }
     }
 }
 
@@ -698,16 +749,24 @@ __global__ void __launch_bounds__(256) k_vertex_gen_jvp(VDesc vd, const int32_t*
     constexpr int DIMC = VModel<VM>::dim;
     constexpr int NX = VExt<VM>::n > 0 ? VExt<VM>::n : 1;
     const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if constexpr (DIMC <= 0) {
+        return;
+    } else {
     if (q >= vd.count) return;
     const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * DIMC;
     Dual x[DIMC];
 #pragma unroll
     for (int j = 0; j < DIMC; ++j) x[j] = Dual{u[x0 + j], v[x0 + j]};
-    const Dual own = x[vd.mask[0]];
+    const Dual own = vd.outkind == 0 ? x[vd.mask[0]]
+                                      : Dual{vo[static_cast<int64_t>(vd.out0) + q], dvo[static_cast<int64_t>(vd.out0) + q]};
     Dual acc{0.0, 0.0};
     const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
     for (int32_t k = beg; k < end; ++k)
         acc = acc + slot_value_dual_rt(ed[slot_eb ? slot_eb[k] : 0], slots[k], own, u, v, vo, dvo, p);
+    if (vd.lbin) {
+        const int32_t h = vd.lbin[vd.lb0 + q];
+        if (h >= 0) acc = acc + Dual{vo[h], dvo[h]};
+    }
     Dual ext[NX];
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
@@ -721,6 +780,7 @@ __global__ void __launch_bounds__(256) k_vertex_gen_jvp(VDesc vd, const int32_t*
     vertex_f_ext<VM, Dual>(dx, x, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, ext, vd.t, DIMC);
 #pragma unroll
     for (int j = 0; j < DIMC; ++j) { du[x0 + j] = dx[j].v; jv[x0 + j] = dx[j].d; }
+    }
 }
 
 // collect_externals!(map, extbuf, u, o), external_inputs.jl:49-77: map > 0 reads u,
@@ -796,6 +856,11 @@ VertexKernel pick_gen_kernel(int vm) {
     case ND_VM_LOAD: return k_vertex_gen<ND_VM_LOAD, STAGE>;
     case ND_VM_CAPACITOR_SINK: return k_vertex_gen<ND_VM_CAPACITOR_SINK, STAGE>;
     case ND_VM_PI_SOURCE: return k_vertex_gen<ND_VM_PI_SOURCE, STAGE>;
+    case ND_VM_CAPACITOR: return k_vertex_gen<ND_VM_CAPACITOR, STAGE>;
+    case ND_VM_VOLTAGE_SOURCE: return k_vertex_gen<ND_VM_VOLTAGE_SOURCE, STAGE>;
+    case ND_VM_RESISTOR_INJECTOR: return k_vertex_gen<ND_VM_RESISTOR_INJECTOR, STAGE>;
+    case ND_VM_INDUCTOR_INJECTOR: return k_vertex_gen<ND_VM_INDUCTOR_INJECTOR, STAGE>;
+    case ND_VM_CRL: return k_vertex_gen<ND_VM_CRL, STAGE>;
     default: return k_vertex_rhs<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false, STAGE, false>;
     }
 }
@@ -851,6 +916,11 @@ JvpKernel pick_gen_jvp(int vm) {
     case ND_VM_LOAD: return k_vertex_gen_jvp<ND_VM_LOAD>;
     case ND_VM_CAPACITOR_SINK: return k_vertex_gen_jvp<ND_VM_CAPACITOR_SINK>;
     case ND_VM_PI_SOURCE: return k_vertex_gen_jvp<ND_VM_PI_SOURCE>;
+    case ND_VM_CAPACITOR: return k_vertex_gen_jvp<ND_VM_CAPACITOR>;
+    case ND_VM_VOLTAGE_SOURCE: return k_vertex_gen_jvp<ND_VM_VOLTAGE_SOURCE>;
+    case ND_VM_RESISTOR_INJECTOR: return k_vertex_gen_jvp<ND_VM_RESISTOR_INJECTOR>;
+    case ND_VM_INDUCTOR_INJECTOR: return k_vertex_gen_jvp<ND_VM_INDUCTOR_INJECTOR>;
+    case ND_VM_CRL: return k_vertex_gen_jvp<ND_VM_CRL>;
     default: return k_vertex_jvp<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false>;
     }
 }
@@ -874,24 +944,35 @@ EdgeOdeKernel pick_edge_ode(int em) {
     }
 }
 
-bool vertex_model_known(int vm) { return vm >= ND_VM_NOOP && vm <= ND_VM_PI_SOURCE; }
+bool vertex_model_known(int vm) { return vm >= ND_VM_NOOP && vm <= ND_VM_CRL; }
 int VModelDim(int vm) {
     switch (vm) {
     case ND_VM_SWING:
     case ND_VM_SWING_UNIT_DAMPING:
-    case ND_VM_PI_SOURCE: return 2;
+    case ND_VM_PI_SOURCE:
+    case ND_VM_CRL: return 2;
     case ND_VM_DIFFUSION:
     case ND_VM_KURAMOTO:
     case ND_VM_LOAD:
-    case ND_VM_CAPACITOR_SINK: return 1;
+    case ND_VM_CAPACITOR_SINK:
+    case ND_VM_CAPACITOR:
+    case ND_VM_INDUCTOR_INJECTOR: return 1;
+    case ND_VM_VOLTAGE_SOURCE:
+    case ND_VM_RESISTOR_INJECTOR: return 0;
     default: return -1;
     }
 }
+// the output kind each vertex model has: StateMask, NoFeedForward or PureFeedForward
+int vertex_model_ff(int vm) {
+    if (vm == ND_VM_VOLTAGE_SOURCE) return ND_FF_NOFEEDFORWARD;
+    if (vm == ND_VM_RESISTOR_INJECTOR) return ND_FF_PUREFEEDFORWARD;
+    return ND_FF_PURESTATEMAP;
+}
 int vertex_ext_inputs(int vm) { return vm == ND_VM_PI_SOURCE ? VExt<ND_VM_PI_SOURCE>::n : 0; }
 // vertex models only the one-thread-per-vertex kernels carry
-bool vertex_model_generic(int vm) { return vm == ND_VM_CAPACITOR_SINK || vm == ND_VM_PI_SOURCE; }
-bool edge_model_known(int em) { return em >= ND_EM_NOOP && em <= ND_EM_RESISTOR; }
-bool edge_model_generic(int em) { return em == ND_EM_RESISTOR; }
+bool vertex_model_generic(int vm) { return vm >= ND_VM_CAPACITOR_SINK && vm <= ND_VM_CRL; }
+bool edge_model_known(int em) { return em >= ND_EM_NOOP && em <= ND_EM_LOOPBACK; }
+bool edge_model_generic(int em) { return em == ND_EM_RESISTOR || em == ND_EM_LOOPBACK; }
 int edge_params(int em) {
     switch (em) {
     case ND_EM_KURAMOTO: return EModel<ND_EM_KURAMOTO>::np;
@@ -974,6 +1055,10 @@ struct nd_plan {
     int64_t lastidx_extbuf = 0;
     int32_t* d_xmap = nullptr;
     int32_t* d_extmap = nullptr;
+    // vertex outputs computed by a g (voltage sources, injectors): every call writes the
+    // compact outputs, the injectors' after their hubs'
+    bool computed_out = false, ff_out = false;
+    int32_t* d_lbin = nullptr;
 
     ~nd_plan() {
         if (small_en) nd_ensemble_destroy(small_en);
@@ -1126,7 +1211,8 @@ int apply_gather_mode(nd_plan* pl) {
     int64_t nv = 0;
     for (const VDesc& vd : pl->vdesc) nv += vd.count;
     const bool possible = pl->d_vout != nullptr;
-    pl->use_vout = possible && (pl->gather_mode == 3 || (pl->gather_mode == 1 && nv >= kCompactMinVertices));
+    pl->use_vout = possible && (pl->gather_mode == 3 || (pl->gather_mode == 1 && nv >= kCompactMinVertices) ||
+                                pl->computed_out);   // computed outputs exist only in the compact array
     pl->kcache = pl->kcache_ok && pl->use_vout;
     const std::vector<int32_t>& x = pl->use_vout ? pl->slot_oidx : pl->slot_uidx;
     for (size_t k = 0; k < pl->slots_host.size(); ++k) pl->slots_host[k].x = x[k];
@@ -1203,8 +1289,12 @@ void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const doub
                 nb += vout_blocks(pl->kcount);
                 pending = false;
             }
-            hipLaunchKernelGGL(k_vertex_out, dim3(nb), dim3(256), 0, s, vd, u, vin, ks);
+            hipLaunchKernelGGL(k_vertex_out, dim3(nb), dim3(256), 0, s, vd, u, vin, ks, p, 0);
         }
+        if (pl->ff_out)
+            for (const VDesc& vd : pl->vdesc)
+                if (vd.outkind == 2 && vd.count)
+                    hipLaunchKernelGGL(k_vertex_out_ff, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, vin, p);
     }
     for (size_t b = 0; b < pl->vdesc.size(); ++b) {
         VDesc vd = pl->vdesc[b];
@@ -1240,8 +1330,8 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
     for (const nd::Batch& b : n.vb) {
         const nd_kind& k = n.kinds[b.kind];
         if (!vertex_model_known(k.model)) return nd::set_error(ND_ERR_UNSUPPORTED, "vertex model %d has no HIP kernel", k.model);
-        if (k.ff != ND_FF_PURESTATEMAP)
-            return nd::set_error(ND_ERR_UNSUPPORTED, "feed-forward vertex outputs (loopback networks) have no HIP kernel");
+        if (k.ff != vertex_model_ff(k.model))
+            return nd::set_error(ND_ERR_UNSUPPORTED, "vertex model %d: this output form has no HIP kernel", k.model);
         if (k.model != ND_VM_NOOP) {
             functional_vertex = true;
             if (k.dim != VModelDim(k.model)) return nd::set_error(ND_ERR_ARGUMENT, "vertex model %d needs dim %d", k.model, VModelDim(k.model));
@@ -1257,6 +1347,8 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         if (k.model != ND_EM_NOOP) functional_edge = true;
         if (edge_model_generic(k.model)) generic_edges = true;
         if (k.extdim > 0) return nd::set_error(ND_ERR_UNSUPPORTED, "external inputs of edge models have no HIP kernel");
+        if (k.model == ND_EM_LOOPBACK && (k.wrapper != ND_WRAP_DIRECTED || k.ff != ND_FF_PUREFEEDFORWARD))
+            return nd::set_error(ND_ERR_ARGUMENT, "a LoopbackConnection is a Directed feed-forward edge");
         if (k.ff == ND_FF_PURESTATEMAP && k.wrapper != ND_WRAP_FIDUCIAL)
             return nd::set_error(ND_ERR_UNSUPPORTED, "StateMask edge outputs inside AntiSymmetric/Symmetric/Directed have no HIP kernel");
         if (k.model == ND_EM_NOOP && k.dim > 0)
@@ -1275,6 +1367,12 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
                              n.vdepth, n.edepth);
     if (n.lastidx_extbuf > 0 && static_cast<int64_t>(n.extmap.size()) != n.lastidx_extbuf)
         return nd::set_error(ND_ERR_ARGUMENT, "the network has external inputs: set its ExtMap (nd_network_set_extmap) first");
+    bool computed_out = false;
+    for (const nd::Batch& b : n.vb) computed_out = computed_out || n.kinds[b.kind].ff != ND_FF_PURESTATEMAP;
+    if (computed_out)
+        for (const nd::Batch& b : n.eb)
+            if (is_ode_model(n.kinds[b.kind].model))
+                return nd::set_error(ND_ERR_UNSUPPORTED, "ODE edges next to computed vertex outputs have no HIP kernel");
 
     nd_plan* pl = new (std::nothrow) nd_plan;
     if (!pl) return nd::set_error(ND_ERR_ALLOC, "host allocation failed");
@@ -1302,11 +1400,16 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         e.outdim_src = k.outdim_src; e.outdim_dst = k.outdim_dst;
         for (int j = 0; j < ND_MAX_OUT; ++j) { e.mask_src[j] = k.mask_src[j] - 1; e.mask_dst[j] = k.mask_dst[j] - 1; }
         if (n.vdepth == 1) {
+            // endpoint outputs as u-indices (StateMask outputs), or as o-indices where
+            // outputs are computed (get_buffers then reads them from o)
             ends[b].resize(B.indices.size());
             for (size_t q = 0; q < B.indices.size(); ++q) {
                 const int64_t ei = B.indices[q] - 1;
-                ends[b][q] = make_int2(static_cast<int32_t>(nd::vertex_out_u_index(n, n.src[ei] - 1, 0)),
-                                       static_cast<int32_t>(nd::vertex_out_u_index(n, n.dst[ei] - 1, 0)));
+                const int64_t s0 = n.src[ei] - 1, d0 = n.dst[ei] - 1;
+                ends[b][q] = computed_out
+                    ? make_int2(static_cast<int32_t>(n.v_out_first(s0) - 1), static_cast<int32_t>(n.v_out_first(d0) - 1))
+                    : make_int2(static_cast<int32_t>(nd::vertex_out_u_index(n, s0, 0)),
+                                static_cast<int32_t>(nd::vertex_out_u_index(n, d0, 0)));
             }
         }
         pl->edesc.push_back(e);
@@ -1434,9 +1537,34 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         vd.generic = k.model != ND_VM_NOOP && (vertex_model_generic(k.model) || generic_edges) ? 1 : 0;
         vd.ext0 = static_cast<int32_t>(B.ext_first - 1);
         vd.extdim = k.extdim;
+        vd.outkind = k.ff == ND_FF_NOFEEDFORWARD ? 1 : k.ff == ND_FF_PUREFEEDFORWARD ? 2 : 0;
         pl->vdesc.push_back(vd);
         pl->vkern_jvp.push_back(vd.generic ? pick_gen_jvp(k.model)
                                            : pick_jvp_kernel(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi));
+    }
+    // loopback: each injector's input is its hub's output (gen_loopback_map, post_utils.jl:205-217)
+    pl->computed_out = computed_out;
+    {
+        std::vector<int32_t> lbin;
+        bool any = false;
+        for (size_t b = 0; b < n.vb.size(); ++b) {
+            pl->vdesc[b].lb0 = static_cast<int32_t>(lbin.size());
+            lbin.insert(lbin.end(), n.vb[b].indices.size(), -1);
+            if (pl->vdesc[b].outkind == 2) pl->ff_out = true;
+        }
+        for (int64_t e = 0; e < n.ne; ++e) {
+            if (n.kinds[n.ekind[e]].model != ND_EM_LOOPBACK) continue;
+            const int64_t inj = n.src[e] - 1, hub = n.dst[e] - 1;
+            lbin[static_cast<size_t>(pl->vdesc[n.v_batch[inj]].lb0 + n.v_pos[inj])] = static_cast<int32_t>(n.v_out_first(hub) - 1);
+            any = true;
+        }
+        if (any) {
+            rc = upload(pl, lbin, &pl->d_lbin);
+            if (rc) { delete pl; return rc; }
+            for (VDesc& vd : pl->vdesc) vd.lbin = pl->d_lbin;
+        }
+        for (const VDesc& vd : pl->vdesc)
+            if (vd.outkind == 2 && !any) { delete pl; return nd::set_error(ND_ERR_ARGUMENT, "injector outputs need a LoopbackConnection"); }
     }
     // external inputs: every entry resolves to a state (StateBufIdx, or a StateMask output)
     pl->lastidx_extbuf = n.lastidx_extbuf;
@@ -1550,9 +1678,14 @@ int nd_rhs_jvp(nd_plan* pl, double* du, double* jv, const double* u, const doubl
         for (const VDesc& vd : pl->vdesc) {
             if (!vd.count) continue;
             const unsigned vb = blocks_for(vd.count, 256);
-            hipLaunchKernelGGL(k_vertex_out, dim3(vb), dim3(256), 0, s, vd, u, pl->d_vout, no_sync(vb));
-            hipLaunchKernelGGL(k_vertex_out, dim3(vb), dim3(256), 0, s, vd, v, pl->d_dvout, no_sync(vb));
+            hipLaunchKernelGGL(k_vertex_out, dim3(vb), dim3(256), 0, s, vd, u, pl->d_vout, no_sync(vb), p, 0);
+            hipLaunchKernelGGL(k_vertex_out, dim3(vb), dim3(256), 0, s, vd, v, pl->d_dvout, no_sync(vb), p, 1);
         }
+        if (pl->ff_out)
+            for (const VDesc& vd : pl->vdesc)
+                if (vd.outkind == 2 && vd.count)
+                    hipLaunchKernelGGL(k_vertex_out_ff_jvp, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd,
+                                       pl->d_vout, pl->d_dvout, p);
         vo = pl->d_vout;
         dvo = pl->d_dvout;
     }
@@ -1623,8 +1756,19 @@ int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_
     // outputs of the state it produces into the buffer the next stage reads, so only the
     // first stage of the call builds them.  p is const for the whole call: its parameters
     // are compared with the cached copy once.
-    double* va = pl->use_vout ? pl->d_vout : nullptr;
+    double* va = pl->use_vout && !pl->computed_out ? pl->d_vout : nullptr;
     double* vb = nullptr;
+    if (pl->computed_out) {
+        // computed outputs (voltage sources, injectors) are rebuilt for every stage
+        for (int64_t k = 0; k < nsteps; ++k) {
+            const double tk = t0 + static_cast<double>(k) * dt;
+            launch_rhs(pl, true, nullptr, u, p, StageArgs{u, acc, ya, nullptr, half, 0.0, 1}, s, tk, k == 0);
+            launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, yb, nullptr, half, 2.0, 2}, s, tk + half, false);
+            launch_rhs(pl, true, nullptr, yb, p, StageArgs{u, acc, ya, nullptr, dt, 2.0, 2}, s, tk + half, false);
+            launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, u, nullptr, sixth, 0.0, 3}, s, tk + dt, false);
+        }
+        return hip_check(hipGetLastError(), "nd_rk4 launch");
+    }
     if (va) {
         if (!pl->d_vout2) {
             void* w = nullptr;
@@ -1703,9 +1847,20 @@ int nd_get_buffers(nd_plan* pl, double* o, double* aggbuf, const double* u, cons
     for (const VDesc& vd : pl->vdesc)
         if (vd.count)
             hipLaunchKernelGGL(k_vertex_out, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, u, o,
-                               no_sync(blocks_for(vd.count, 256)));
+                               no_sync(blocks_for(vd.count, 256)), p, 0);
+    if (pl->ff_out)
+        for (const VDesc& vd : pl->vdesc)
+            if (vd.outkind == 2 && vd.count)
+                hipLaunchKernelGGL(k_vertex_out_ff, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, o, p);
+    // with computed outputs the edges' endpoint indices address o (no ODE edges then)
     for (const EDesc& e : pl->edesc)
-        if (e.count) hipLaunchKernelGGL(k_edge_out, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, u, p, o);
+        if (e.count)
+            hipLaunchKernelGGL(k_edge_out, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e,
+                               pl->computed_out ? static_cast<const double*>(o) : u, p, o);
+    if (pl->d_lbin && pl->lastidx_aggr > 0)
+        for (const VDesc& vd : pl->vdesc)
+            if (vd.count)
+                hipLaunchKernelGGL(k_loopback_copy, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, o, aggbuf);
     rc = hip_check(hipGetLastError(), "nd_get_buffers launch");
     if (rc) return rc;
     return nd_aggregate(pl, aggbuf, o, stream);

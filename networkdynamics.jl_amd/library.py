@@ -73,6 +73,37 @@ def pi_source(extin, name="source"):
                          psym=[("C", 1.0), ("Vref", 1.0), ("Ki", 0.5), ("Kp", 10.0)], extin=extin, name=name)
 
 
+# test/loopback_test.jl:53-131: a capacitor hub with resistor and inductor injectors on
+# LoopbackConnections, the voltage source feeding it through a resistor, and the
+# single-vertex CRL model the injector network must reproduce
+def capacitor(name="hub"):
+    return c.VertexModel(f=c.capacitor_f, g=1, sym=[("v", 0.0)], psym=[("C", 1.0)], name=name)
+
+
+def voltage_source(name="vs"):
+    return c.VertexModel(g=c.voltage_source_g, psym=[("V", 1.0)], name=name)
+
+
+def resistor_injector(name="R_injector"):
+    return c.VertexModel(g=c.resistor_injector_g, psym=[("R", 100.0)], name=name)
+
+
+def inductor_injector(name="L_injector"):
+    return c.VertexModel(f=c.inductor_injector_f, g=1, sym=[("i", 0.0)], psym=[("L", 0.1)], name=name)
+
+
+def crl(name="CRL_vertex"):
+    return c.VertexModel(f=c.crl_f, g=1, sym=[("v", 0.0), ("iL", 0.0)],
+                         psym=[("C", 1.0), ("R", 100.0), ("L", 0.1)], name=name)
+
+
+def loopback_connection(name="loopback"):
+    """LoopbackConnection (src/post_utils.jl:110-190): a Directed edge from an injector to
+    its hub; the hub receives -1 times the injector's output and the injector's input is
+    the hub's output."""
+    return c.EdgeModel(g=c.Directed(c.loopback_g), outdim=1, name=name)
+
+
 class bench:
     """benchmark/benchmark_models.jl:5-42."""
 

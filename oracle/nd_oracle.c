@@ -22,6 +22,9 @@
  *                              test/diffusion_test.jl:57-107, docs/src/external_inputs.md:62-99
  *   external inputs            src/external_inputs.jl:10-77 (ExtMap, collect_externals!),
  *                              collected after the :g passes as coreloop.jl:56-57 does
+ *   loopback / injectors       src/post_utils.jl:105-228 (LOOPBACK_G, apply_loopback!),
+ *                              coreloop.jl:44-55 (copy, then the feed-forward vertex g);
+ *                              models test/loopback_test.jl:19-131
  *
  * Pinning (tests/test_oracle.py): BatchStride known answers (test/utils_test.jl:147-178);
  * the complete_graph(4) maps derived by hand from the source (fixture
@@ -409,6 +412,19 @@ static void vertex_f(const nd_kind* k, double* du, const double* u, const double
     case ND_VM_CAPACITOR_SINK:     /* docs/src/external_inputs.md:76-79: f_load */
         du[0] = 1.0 / p[0] * (acc[0] - (1.0 + 0.1 * sin(t)));
         break;
+    case ND_VM_CAPACITOR:          /* loopback_test.jl:61: D(p.v) ~ p.i / C */
+        du[0] = acc[0] / p[0];
+        break;
+    case ND_VM_INDUCTOR_INJECTOR:  /* loopback_test.jl:89: D(p.i) ~ p.v / L */
+        du[0] = acc[0] / p[0];
+        break;
+    case ND_VM_CRL: {              /* loopback_test.jl:113-131: CRLModel */
+        double C = p[0], R = p[1], L = p[2];
+        double v = u[0], iL = u[1];
+        du[0] = (acc[0] - v / R - iL) / C;
+        du[1] = v / L;
+        break;
+    }
     case ND_VM_PI_SOURCE: {        /* docs/src/external_inputs.md:87-92: f_source */
         double C = p[0], Vref = p[1], Ki = p[2], Kp = p[3];
         double d = Vref - ext[0];
@@ -439,6 +455,9 @@ static void edge_gdst(const nd_kind* k, double* y, const double* vs, const doubl
         break;
     case ND_EM_RESISTOR:           /* docs/src/external_inputs.md:66-68: resistor_g */
         y[0] = (vs[0] - vd[0]) / p[0];
+        break;
+    case ND_EM_LOOPBACK:           /* post_utils.jl:105-108: outdst .= -1 .* insrc */
+        for (int j = 0; j < depth; ++j) y[j] = -1.0 * vs[j];
         break;
     default:
         break;
@@ -501,9 +520,17 @@ static void apply_vertex(const ndo_net* nw, const batch_t* B, int64_t i, int pas
                          const double* u, double* o, const double* aggbuf, const double* p, double t) {
     const nd_kind* k = &nw->kinds[B->kind];
     rng_t sr = bs_range(&B->state, i), pr = bs_range(&B->p, i);
-    if (pass_g) {   /* StateMask output, PureStateMap: g(o, u) */
+    if (pass_g) {
         rng_t orr = bs_range(&B->out, i);
-        for (int j = 0; j < k->outdim_dst; ++j) o[orr.first - 1 + j] = u[sr.first - 1 + k->mask_dst[j] - 1];
+        double* out = o + orr.first - 1;
+        if (k->ff == ND_FF_PURESTATEMAP) {          /* StateMask: g(o, u) */
+            for (int j = 0; j < k->outdim_dst; ++j) out[j] = u[sr.first - 1 + k->mask_dst[j] - 1];
+        } else if (k->model == ND_VM_VOLTAGE_SOURCE) { /* loopback_test.jl:31: p.v ~ V (NoFeedForward) */
+            out[0] = p[pr.first - 1];
+        } else if (k->model == ND_VM_RESISTOR_INJECTOR) { /* loopback_test.jl:75: p.i ~ p.v / R, input = hub */
+            rng_t ir = bs_range(&B->in, i);
+            out[0] = aggbuf[ir.first - 1] / p[pr.first - 1];
+        }
     } else {
         rng_t ir = bs_range(&B->in, i);
         const double* ext = nw->extbuf + nw->v_ext[B->idx[i - 1] - 1].first - 1;
@@ -578,6 +605,15 @@ void ndo_gather(const ndo_net* nw, double* gbuf, const double* o) {
     for (int64_t k = 0; k < nw->lastidx_gbuf; ++k) gbuf[k] = o[nw->gmap[k] - 1];
 }
 
+/* apply_loopback! (post_utils.jl:221-228): each injector's input is its hub's output */
+static void apply_loopback(const ndo_net* nw, double* aggbuf, const double* o) {
+    for (int64_t e = 0; e < nw->ne; ++e) {
+        if (nw->kinds[nw->ekind[e]].model != ND_EM_LOOPBACK) continue;
+        rng_t a = nw->v_aggr[nw->src[e] - 1], b = nw->v_out[nw->dst[e] - 1];
+        for (int64_t j = 0; j < rlen(a); ++j) aggbuf[a.first - 1 + j] = o[b.first - 1 + j];
+    }
+}
+
 /* Full RHS, SequentialExecution{buffered} + SequentialAggregator(+). Returns 0, or 1 on size mismatch
    (the ArgumentError of coreloop.jl:2-7).  If o_out/agg_out are non-NULL they receive the buffers
    (get_buffers with initbufs=true, coreloop.jl:103-109; then du may be NULL). */
@@ -598,6 +634,12 @@ int ndo_rhs(ndo_net* nw, double* du, const double* u, const double* p, double t,
         const batch_t* B = &nw->eb[b];
         if (hasff(&nw->kinds[B->kind])) continue;
         for (int64_t i = 1; i <= B->n; ++i) apply_edge(nw, B, i, 0, 1, buffered, du, u, o, gbuf, p, t);
+    }
+    apply_loopback(nw, aggbuf, o);                                               /* coreloop.jl:47 */
+    for (int b = 0; b < nw->nvb; ++b) {                                          /* coreloop.jl:55 */
+        const batch_t* B = &nw->vb[b];
+        if (!hasff(&nw->kinds[B->kind])) continue;
+        for (int64_t i = 1; i <= B->n; ++i) apply_vertex(nw, B, i, 1, du, u, o, aggbuf, p, t);
     }
     if (nw->lastidx_extbuf > 0) ndo_collect_externals(nw, nw->extbuf, u, o);     /* coreloop.jl:56-57 */
     if (buffered) ndo_gather(nw, gbuf, o);                                       /* coreloop.jl:67 */
@@ -658,6 +700,12 @@ int ndo_rhs_threaded(ndo_net* nw, double* du, const double* u, const double* p, 
 #endif
         for (int64_t i = 1; i <= B->n; ++i) apply_edge(nw, B, i, 0, 1, 1, du, u, o, gbuf, p, t);
     }
+    apply_loopback(nw, aggbuf, o);
+    for (int b = 0; b < nw->nvb; ++b) {
+        const batch_t* B = &nw->vb[b];
+        if (!hasff(&nw->kinds[B->kind])) continue;
+        for (int64_t i = 1; i <= B->n; ++i) apply_vertex(nw, B, i, 1, du, u, o, aggbuf, p, t);
+    }
     if (nw->lastidx_extbuf > 0) ndo_collect_externals(nw, nw->extbuf, u, o);
     ndo_gather(nw, gbuf, o);
     for (int b = 0; b < nw->neb; ++b) {
@@ -699,6 +747,7 @@ void ndo_rhs_scale(ndo_net* nw, double* S, const double* u, const double* p, dou
     ndo_rhs(nw, NULL, u, p, t, 1, NULL, NULL);    /* fills o */
     for (int64_t k = 0; k < nout; ++k) o[k] = fabs(o[k]);
     fill_d(agg, nw->lastidx_aggr, 0.0);
+    apply_loopback(nw, agg, o);                   /* |hub output| as the injectors' input */
     ndo_aggregate_seq(nw, agg, o);
     for (int b = 0; b < nw->nvb; ++b) {
         const batch_t* B = &nw->vb[b];
@@ -716,6 +765,8 @@ void ndo_rhs_scale(ndo_net* nw, double* S, const double* u, const double* p, dou
             case ND_VM_SWING_UNIT_DAMPING: s[0] = fabs(x[1]); s[1] = fabs(q[0]) + fabs(x[1]) + a; break;
             case ND_VM_LOAD: s[0] = fabs(1.0 / q[0]) * (fabs(q[1]) + a); break;
             case ND_VM_CAPACITOR_SINK: s[0] = fabs(1.0 / q[0]) * (a + 1.0 + fabs(0.1 * sin(t))); break;
+            case ND_VM_CAPACITOR: case ND_VM_INDUCTOR_INJECTOR: s[0] = a / fabs(q[0]); break;
+            case ND_VM_CRL: s[0] = (a + fabs(x[0] / q[1]) + fabs(x[1])) / fabs(q[0]); s[1] = fabs(x[0] / q[2]); break;
             case ND_VM_PI_SOURCE: {
                 const double ex = fabs(nw->extbuf[nw->v_ext[B->idx[i - 1] - 1].first - 1]);
                 s[1] = fabs(q[1]) + ex;
