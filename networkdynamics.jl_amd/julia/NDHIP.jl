@@ -47,6 +47,7 @@ end
 # e.g. register_model!(Lib.kuramoto_inertia!, 3) for ND_VM_SWING)
 const MODEL_IDS = IdDict{Any,Int32}()
 register_model!(f, id::Integer) = (MODEL_IDS[f] = Int32(id); nothing)
+register_model!(NetworkDynamics.LOOPBACK_G, 10)      # ND_EM_LOOPBACK (post_utils.jl:105-108)
 
 _check(rc) = rc == 0 || throw(ArgumentError(unsafe_string(ccall((:nd_last_error, libnd), Cstring, ()))))
 
@@ -55,7 +56,13 @@ _nomask() = ntuple(_ -> Int32(0), 4)
 
 function _kind(c::VertexModel)
     g = compg(c)
-    g isa StateMask || throw(ArgumentError("HIPExecution needs StateMask vertex outputs"))
+    if !(g isa StateMask)
+        # a computed output (a source's NoFeedForward g, an injector's PureFeedForward g):
+        # the registered g names the model, the trait goes along as ND_FF_*
+        ffid = fftype(c) isa PureFeedForward ? 1 : fftype(c) isa NetworkDynamics.FeedForward ? 2 : 3
+        return NDKind(0, get(MODEL_IDS, g, -1), 0, ffid, dim(c), pdim(c), 0, outdim(c), _nomask(), _nomask(),
+                      extdim(c))
+    end
     NDKind(0, get(MODEL_IDS, compf(c), -1), 0, 0, dim(c), pdim(c), 0, outdim(c), _nomask(), _mask(g),
            extdim(c))
 end
