@@ -4,7 +4,7 @@ The reference solves `find_fixpoint(nw, x0, p)` (src/initialization.jl:73-144) w
 NonlinearSolve's Newton and ForwardDiff Jacobians, one network at a time.  Here
 `FixpointSolver(graphs)` runs damped Newton + conjugate-gradient linear solves for
 many grids in one launch (nd_fixpoint_* in include/nd_hip.h), and `isfixpoint`
-judges the result with the network right-hand side itself
+checks the result with the network right-hand side itself
 (src/linear_analysis.jl:8-15: all |du| < tol).
 """
 import ctypes as C

@@ -95,7 +95,7 @@ def find_fixpoint(nw, x0, p, t=float("nan"), tol=1e-10, maxiters=50, linsolve="a
     float64 device tensors.  linsolve: "dense", "sparse" (colored Jacobian, sparse LU),
     "gmres" (matrix-free), or "auto" (dense up to dense_max states, else sparse).
     Returns (x, info): x the fixpoint as a float64 device tensor (max |du| <= tol, the
-    isfixpoint judge of linear_analysis.jl:8-15), info the Newton iterations, line-search
+    isfixpoint test of linear_analysis.jl:8-15), info the Newton iterations, line-search
     halvings and final residual."""
     if linsolve == "auto":
         linsolve = "dense" if nw.dim <= dense_max else "sparse"

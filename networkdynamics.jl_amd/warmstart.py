@@ -13,7 +13,7 @@ Pipeline for a batch of swing/load grids:
   3. a 4-layer message-passing network (hidden 64, FP32, nd_mp_* kernels)
      predicts the remaining source correction t = s* - s0;
   4. warm start: L_K theta_w = P + s0 + t_hat (a second linear solve);
-  5. Newton (`FixpointSolver`) from theta_w; its residual <= 1e-10 is the judge.
+  5. Newton (`FixpointSolver`) from theta_w; its residual <= 1e-10 is the acceptance test.
 Training targets come from fixpoints the batched Newton solver produced from flat
 starts.  The method without the network is Newton from theta = 0; steps 1, 2, 4
 with t_hat = 0 are the physics-only ablation.
@@ -294,7 +294,7 @@ def warm_start_device(gb, net=None, tol=1e-6):
     """Warm start of every grid in a GridBatch, entirely on the GPU: DC solve, physics
     source, network correction (net None: physics only), second linear solve from the DC
     solution.  Both linear solves stop at residual tol: they only provide a starting
-    point, Newton reaches the 1e-10 judge from it.  Returns (theta_w, dict of seconds)."""
+    point, Newton reaches the 1e-10 residual from it.  Returns (theta_w, dict of seconds)."""
     import torch
     times = {}
     th_dc, _, times["dc_solve"] = gb.solve(linearized=True, tol=tol)

@@ -1,4 +1,4 @@
-"""Batched Newton fixpoints, judged by the network RHS (isfixpoint, linear_analysis.jl:8-15)."""
+"""Batched Newton fixpoints, checked by the network RHS (isfixpoint, linear_analysis.jl:8-15)."""
 import numpy as np
 import pytest
 

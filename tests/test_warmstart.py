@@ -138,7 +138,7 @@ def test_trained_network_reduces_newton_iterations(ND, dev):
     for _ in range(300):
         last = net.train_step(batch, Xd, yd, lr=2e-3)
     assert last.item() < 0.8 * first
-    # held-out grids: flat start vs the learned warm start; every result must pass the judge
+    # held-out grids: flat start vs the learned warm start; every result must pass the acceptance test
     flat_th, flat_st, _ = W.solve_grids(test, dev)
     keep = flat_st["retcode"] == 0
     test = [g for g, o in zip(test, keep) if o]
