@@ -69,8 +69,9 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
-                    help="c2 (default): the N=1e6 swing RHS line; c1/c3/c4/c5: the other workloads")
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "extras"],
+                    help="c2 (default): the N=1e6 swing RHS line; c1/c3/c4/c5: the other workloads; "
+                         "extras: timings of the paths around the RHS kernels")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="keyword arguments of the c1/c3/c4/c5 workload runs (numbers)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
@@ -90,7 +91,7 @@ def main():
     if args.config != "c2":
         from importlib import import_module
         ex = import_module(ND.__name__ + ".experiments")
-        fn = dict(c1=ex.c1_rk4, c3=ex.c3_rhs, c4=ex.c4_ensemble, c5=ex.c5_warmstart)[args.config]
+        fn = dict(c1=ex.c1_rk4, c3=ex.c3_rhs, c4=ex.c4_ensemble, c5=ex.c5_warmstart, extras=ex.extras)[args.config]
         kw = {k: float(v) if "." in v or "e" in v else int(v) for k, v in (x.split("=", 1) for x in args.set)}
         print(json.dumps(dict(config=args.config, **fn(dev, **kw))))
         return

@@ -696,29 +696,38 @@ __global__ void __launch_bounds__(256) k_aggregate(const int32_t* __restrict__ r
 // Vertex batches outside the fused kernels' model set -- models that read external
 // inputs (external_inputs.jl: collect_externals! fills them after the :g passes, from
 // states and StateMask outputs, so here they are read from u directly) or the time,
-// and every batch of a network whose edge model the fused kernels do not carry: one
-// thread per vertex, its destination list summed in the reference order.
+// computed outputs, and every batch of a network whose edge model the fused kernels do
+// not carry.  Same block layout as k_vertex_rhs: a block per big hub, a wave per mid
+// hub (fixed-order partial sums), then one thread per remaining vertex summing its
+// destination list in the reference order.
+__device__ __forceinline__ double gen_segment(int32_t beg, int32_t end, int32_t t, int32_t G, double own,
+                                              const int2* __restrict__ slots, const uint8_t* __restrict__ slot_eb,
+                                              const EDesc* __restrict__ ed, const double* __restrict__ u,
+                                              const double* __restrict__ vo, const double* __restrict__ p) {
+    double acc = 0.0;
+    for (int32_t k = beg + t; k < end; k += G) acc += slot_value_rt(ed[slot_eb ? slot_eb[k] : 0], slots[k], own, u, vo, p);
+    return acc;
+}
+
+template <int VM>
+__device__ __forceinline__ double gen_own(const VDesc& vd, int32_t q, const double* __restrict__ u,
+                                          const double* __restrict__ vo) {
+    constexpr int DIMC = VModel<VM>::dim;
+    return vd.outkind == 0 ? u[static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * DIMC + vd.mask[0]]
+                           : vo[static_cast<int64_t>(vd.out0) + q];
+}
+
+// the injector input, the external inputs, f and the stage epilogue of vertex q
 template <int VM, bool STAGE>
-__global__ void __launch_bounds__(256) k_vertex_gen(VDesc vd, const int32_t* __restrict__ rowptr,
-                                                    const int2* __restrict__ slots, const uint8_t* __restrict__ slot_eb,
-                                                    const EDesc* __restrict__ ed, const double* __restrict__ u,
-                                                    const double* __restrict__ vo, const double* __restrict__ p,
-                                                    double* __restrict__ du, StageArgs sa) {
+__device__ __forceinline__ void gen_finish(const VDesc& vd, int32_t q, double acc, const double* __restrict__ u,
+                                           const double* __restrict__ vo, const double* __restrict__ p,
+                                           double* __restrict__ du, const StageArgs& sa) {
     constexpr int DIMC = VModel<VM>::dim;
     constexpr int NX = VExt<VM>::n > 0 ? VExt<VM>::n : 1;
-    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
-    if constexpr (DIMC <= 0) {
-        return;                                  // no states: only an output (k_vertex_out*)
-    } else {
-    if (q >= vd.count) return;
     const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * DIMC;
     double x[DIMC];
 #pragma unroll
     for (int j = 0; j < DIMC; ++j) x[j] = u[x0 + j];
-    const double own = vd.outkind == 0 ? x[vd.mask[0]] : vo[static_cast<int64_t>(vd.out0) + q];
-    double acc = 0.0;
-    const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
-    for (int32_t k = beg; k < end; ++k) acc = acc + slot_value_rt(ed[slot_eb ? slot_eb[k] : 0], slots[k], own, u, vo, p);
     if (vd.lbin) {                               // an injector: its input is the hub's output
         const int32_t h = vd.lbin[vd.lb0 + q];
         if (h >= 0) acc = acc + vo[h];
@@ -734,6 +743,58 @@ __global__ void __launch_bounds__(256) k_vertex_gen(VDesc vd, const int32_t* __r
         const double y = emit1<STAGE>(du, sa, x0 + j, dx[j]);
         if (STAGE && sa.vnext && j == vd.mask[0]) sa.vnext[static_cast<int64_t>(vd.out0) + q] = y;
     }
+}
+
+template <int VM, bool STAGE>
+__global__ void __launch_bounds__(256) k_vertex_gen(VDesc vd, const int32_t* __restrict__ rowptr,
+                                                    const int2* __restrict__ slots, const uint8_t* __restrict__ slot_eb,
+                                                    const EDesc* __restrict__ ed, const double* __restrict__ u,
+                                                    const double* __restrict__ vo, const double* __restrict__ p,
+                                                    double* __restrict__ du, StageArgs sa) {
+    constexpr int DIMC = VModel<VM>::dim;
+    if constexpr (DIMC <= 0) {
+        return;                                  // no states: only an output (k_vertex_out*)
+    } else {
+    const int32_t wpb = static_cast<int32_t>(blockDim.x / kWave);
+    const int32_t nmidb = (vd.nmid + wpb - 1) / wpb;
+    int32_t b = static_cast<int32_t>(blockIdx.x);
+    const int32_t t = static_cast<int32_t>(threadIdx.x);
+    if (b < vd.nbig) {
+        __shared__ double wpart[256 / kWave];
+        const int32_t q = vd.hubs[b];
+        const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
+        const double part = wave_sum(gen_segment(beg, end, t, static_cast<int32_t>(blockDim.x), gen_own<VM>(vd, q, u, vo),
+                                                 slots, slot_eb, ed, u, vo, p));
+        if ((t & (kWave - 1)) == 0) wpart[t / kWave] = part;
+        __syncthreads();
+        if (t == 0) {
+            double acc = wpart[0];
+            for (int32_t w = 1; w < wpb; ++w) acc += wpart[w];
+            gen_finish<VM, STAGE>(vd, q, acc, u, vo, p, du, sa);
+        }
+        return;
+    }
+    b -= vd.nbig;
+    if (b < nmidb) {
+        const int32_t w = b * wpb + t / kWave;
+        const int32_t lane = t & (kWave - 1);
+        if (w >= vd.nmid) return;
+        const int32_t q = vd.hubs[vd.nbig + w];
+        const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
+        const double acc = wave_sum(gen_segment(beg, end, lane, kWave, gen_own<VM>(vd, q, u, vo), slots, slot_eb, ed, u,
+                                                vo, p));
+        if (lane == 0) gen_finish<VM, STAGE>(vd, q, acc, u, vo, p, du, sa);
+        return;
+    }
+    b -= nmidb;
+    const int32_t q = b * static_cast<int32_t>(blockDim.x) + t;
+    if (q >= vd.count) return;
+    if (vd.lrp && (static_cast<uint32_t>(vd.lrp[q]) & 0x80000000u)) return;   // a hub, summed above
+    const double own = gen_own<VM>(vd, q, u, vo);
+    double acc = 0.0;
+    const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
+    for (int32_t k = beg; k < end; ++k) acc = acc + slot_value_rt(ed[slot_eb ? slot_eb[k] : 0], slots[k], own, u, vo, p);
+    gen_finish<VM, STAGE>(vd, q, acc, u, vo, p, du, sa);
     }
 }
 
@@ -1116,7 +1177,7 @@ int rebuild_hubs(nd_plan* pl) {
         pl->hub_offset[b] = all.size();
         lrp_offset[b] = lrp.size();
         std::vector<std::pair<int32_t, int32_t>> big, mid;   // (-length, q)
-        const bool functional = vd.model != ND_VM_NOOP && vd.depth == 1 && !vd.generic;
+        const bool functional = vd.model != ND_VM_NOOP && vd.depth == 1;
         for (int32_t q = 0; q < vd.count; ++q) {
             lrp.push_back(static_cast<int32_t>(nlow));
             if (!functional) continue;

@@ -7,6 +7,8 @@ c4_ensemble   4096 uneven Barabasi-Albert swing networks (N ~ U{100..2000}) from
               fixpoints plus a one-node perturbation, Tsit5 to t = 50 (reltol 1e-8):
               the fused ensemble kernel against the per-instance GPU path
 c5_warmstart  swing/load grids: Newton from flat starts vs from the learned warm start
+extras        find_fixpoint, colored Jacobians, the DAE integrator, and the per-vertex
+              kernel against the fused ones
 """
 import sys
 import time
@@ -312,4 +314,53 @@ def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=50, lr=2e
                newton_ratio_learned_exact_cg=out["flat_newton_mean_exact_cg"] / max(out["learned_newton_mean_exact_cg"], 1e-9),
                time_ratio_learned=out["flat_s"] / out["learned_total_s"],
                time_ratio_learned_exact_cg=out["flat_exact_cg_s"] / out["learned_total_exact_cg_s"])
+    return out
+
+
+def extras(dev, n_grid=4000, n_ba=1_000_000):
+    """Timings of the paths around the RHS kernels: find_fixpoint (sparse LU of the colored
+    Jacobian) on a DAE grid, one colored Jacobian, the DAE integrator per step, and the
+    one-thread-per-vertex kernel against the fused kernels on the same BA graph (resistor
+    edges y = (v_s - v_d)/R against weighted diffusion edges y = w (v_s - v_d))."""
+    import torch
+    from .initialization import find_fixpoint
+    from .sparsity import color_columns, jac_prototype, sparse_jacobian
+    from . import dae
+    out = {}
+    grid = swing_load_grid(n_grid, seed=53)
+    nw = grid.network(algebraic_loads=True)
+    p = grid.flat_p()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, info = find_fixpoint(nw, np.zeros(nw.dim), p)
+    torch.cuda.synchronize()
+    out["find_fixpoint"] = dict(states=nw.dim, seconds=time.perf_counter() - t0, **info)
+    P = jac_prototype(nw)
+    colors, ncolor = color_columns(P)
+    pd = torch.from_numpy(p).to(dev)
+    sparse_jacobian(nw, x, pd, 0.0, P, colors)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        sparse_jacobian(nw, x, pd, 0.0, P, colors)
+    torch.cuda.synchronize()
+    out["sparse_jacobian"] = dict(states=nw.dim, nnz=int(P.nnz), colors=int(ncolor),
+                                  ms=(time.perf_counter() - t0) / 10 * 1e3)
+    rng = np.random.default_rng(3)
+    u0 = grid.flat_u(rng.normal(scale=0.05, size=grid.g.nv()), rng.normal(scale=0.1, size=grid.g.nv()))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, dinfo = dae.rk4(nw, u0, p, 0.0, 0.01, 100)
+    torch.cuda.synchronize()
+    out["dae_rk4"] = dict(states=nw.dim, steps=100, ms_per_step=(time.perf_counter() - t0) / 100 * 1e3, **dinfo)
+    # generic vs fused kernel on one graph
+    g = G.barabasi_albert(n_ba, 2, seed=1)
+    res = {}
+    for name, em in (("fused (weighted diffusion)", Lib.diffusion_edge()), ("per-vertex (resistor)", Lib.resistor())):
+        net = Network(g, Lib.diffusion_vertex(), em)
+        u = torch.from_numpy(rng.normal(size=net.dim)).to(dev)
+        pp = torch.from_numpy(rng.uniform(0.5, 2.0, net.pdim)).to(dev)
+        du = torch.empty_like(u)
+        res[name] = _timeit(lambda: net(du, u, pp, 0.0), 200) * 1e6
+    out["rhs_us_ba_1e6_diffusion"] = res
     return out

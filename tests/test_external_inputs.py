@@ -194,3 +194,31 @@ def test_oracle_pi_source_network_invariants(ND):
                 nload += 1
         rhs -= nload * (1 + 0.1 * np.sin(t))
         assert abs(lhs - rhs) <= 1e-12 * (abs(rhs) + nload)
+
+
+@pytest.mark.gpu
+def test_per_vertex_kernel_hub_classes_match_oracle(ND, dev):
+    # the per-vertex kernel (here because of the resistor edges) on a Barabasi-Albert graph:
+    # big hubs (a block each), mid hubs (a wave each) and the rest, in both gather modes,
+    # for the RHS and for fused RK4 stages
+    import torch
+    g = ND.barabasi_albert(20_000, 2, seed=9)
+    nw = ND.Network(g, ND.Lib.diffusion_vertex(), ND.Lib.resistor())
+    rng = np.random.default_rng(9)
+    u = rng.normal(size=nw.dim)
+    p = rng.uniform(0.5, 2.0, nw.pdim)
+    on = oracle_for(nw)
+    ref, S = on.rhs(u, p), on.scale(u, p)
+    pd = torch.from_numpy(p).to(dev)
+    for hub, big in ((48, 128), (8, 32)):
+        nw.tune(hub_degree=hub, big_hub_degree=big)
+        for gm in (2, 3):
+            nw.tune(gather_mode=gm)
+            du = torch.empty(nw.dim, dtype=torch.float64, device=dev)
+            nw(du, torch.from_numpy(u).to(dev), pd, 0.0)
+            err = scaled_error(du.cpu().numpy(), ref, S)
+            assert err.max() <= 1e-12, (hub, gm, err.max())
+            # a step inside RK4's stability region for the hubs' eigenvalues (~2 * 564 / 0.5)
+            ud = torch.from_numpy(u).to(dev)
+            nw.rk4(ud, pd, 0.0, 2e-4, 5)
+            np.testing.assert_allclose(ud.cpu().numpy(), on.rk4(u.copy(), p, 0.0, 2e-4, 5), rtol=0, atol=1e-12)
