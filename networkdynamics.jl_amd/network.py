@@ -412,11 +412,20 @@ class Network:
         stream = torch.cuda.current_stream(du.device).cuda_stream
         A.check(A.lib().nd_rhs_jvp(self.plan, dptr, jptr, uptr, vptr, pptr, float(t), ndu, nu, np_, stream))
 
-    def jacobian(self, u, p, t=0.0):
-        """Dense Jacobian d(du)/du at u (dim x dim, float64 device tensor), one JVP per
-        unit vector; the matrix ForwardDiff.jacobian builds in linearize_network
-        (linear_analysis.jl:84)."""
+    def jacobian(self, u, p, t=0.0, colored=True):
+        """Dense Jacobian d(du)/du at u (dim x dim, float64 device tensor): the matrix
+        ForwardDiff.jacobian builds in linearize_network (linear_analysis.jl:84).  colored:
+        one JVP per color of the structural pattern (sparsity.py), scattered into the dense
+        matrix; otherwise one JVP per unit vector."""
         import torch
+        if colored:
+            from .sparsity import color_columns, jac_prototype, sparse_jacobian
+            if getattr(self, "_jac_colors", None) is None:
+                P = jac_prototype(self)
+                self._jac_colors = (P, color_columns(P)[0])
+            P, colors = self._jac_colors
+            J = sparse_jacobian(self, u, p, t, P, colors).toarray()
+            return torch.from_numpy(J).to(u.device)
         n = self.dim
         J = torch.empty((n, n), dtype=torch.float64, device=u.device)
         du = torch.empty(n, dtype=torch.float64, device=u.device)
