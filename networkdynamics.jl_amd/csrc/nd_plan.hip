@@ -247,6 +247,15 @@ __device__ __forceinline__ double segment_sum(int32_t beg, int32_t end, int32_t 
     return acc;
 }
 
+// LDS of the vertex kernels: hub-block wave partials, and the low path's entry window
+// (values, owners) and own outputs
+struct RhsShared {
+    double wpart[256 / kWave];
+    double val[4 * 256];
+    double own[256];
+    uint8_t owner[4 * 256];
+};
+
 // Fused RHS for one vertex batch.  Block layout (hubs first, so the longest
 // segments start earliest):
 //   [0, nbig)                 one block per big hub (segment > big_hub_degree),
@@ -255,21 +264,18 @@ __device__ __forceinline__ double segment_sum(int32_t beg, int32_t end, int32_t 
 // Hub segments: per-thread partial sums in a fixed order, xor-butterfly per wave,
 // wave partials combined in wave order -- deterministic run to run.
 template <int VM, int EM, int WRAP, bool MULTI, bool STAGE, bool KC>
-__global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __restrict__ rowptr,
-                                                    const int2* __restrict__ slots,
-                                                    const uint8_t* __restrict__ slot_eb,
-                                                    const EDesc* __restrict__ ed,
-                                                    const double* __restrict__ u, const double* __restrict__ vo,
-                                                    const double* __restrict__ p,
-                                                    double* __restrict__ du, StageArgs sa) {
+__device__ __forceinline__ void vertex_rhs_body(const VDesc& vd, int32_t b, RhsShared& sh,
+                                                const int32_t* __restrict__ rowptr, const int2* __restrict__ slots,
+                                                const uint8_t* __restrict__ slot_eb, const EDesc* __restrict__ ed,
+                                                const double* __restrict__ u, const double* __restrict__ vo,
+                                                const double* __restrict__ p, double* __restrict__ du,
+                                                const StageArgs& sa) {
     constexpr int DIMC = VModel<VM>::dim;
     const int dim = DIMC > 0 ? DIMC : vd.dim;
     const int32_t wpb = static_cast<int32_t>(blockDim.x / kWave);
     const int32_t nmidb = (vd.nmid + wpb - 1) / wpb;
-    int32_t b = static_cast<int32_t>(blockIdx.x);
     if constexpr (VM != ND_VM_NOOP) {
         if (b < vd.nbig) {
-            __shared__ double wpart[256 / kWave];
             const int32_t q = vd.hubs[b];
             const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * dim;
             const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
@@ -277,11 +283,11 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
             const int32_t t = static_cast<int32_t>(threadIdx.x);
             const double part = wave_sum(segment_sum<EM, WRAP, MULTI, KC>(beg, end, t, static_cast<int32_t>(blockDim.x),
                                                                           own, slots, slot_eb, ed, u, vo, p, vd.kfull));
-            if ((t & (kWave - 1)) == 0) wpart[t / kWave] = part;
+            if ((t & (kWave - 1)) == 0) sh.wpart[t / kWave] = part;
             __syncthreads();
             if (t == 0) {
-                double acc = wpart[0];
-                for (int32_t w = 1; w < wpb; ++w) acc += wpart[w];
+                double acc = sh.wpart[0];
+                for (int32_t w = 1; w < wpb; ++w) acc += sh.wpart[w];
                 vertex_emit<VM, STAGE>(du, sa, x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim,
                                        vd.mask[0], static_cast<int64_t>(vd.out0) + q);
             }
@@ -320,9 +326,6 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
     if constexpr (VM == ND_VM_NOOP) {
         if (valid) vertex_emit<VM, STAGE>(du, sa, x0, u + x0, 0.0, p, dim, vd.mask[0], static_cast<int64_t>(vd.out0) + q);
     } else {
-        __shared__ double s_val[4 * 256];
-        __shared__ double s_own[256];
-        __shared__ uint8_t s_owner[4 * 256];
         const int32_t* __restrict__ lrp = vd.lrp;
         // lrp[q] = offset | (hub ? 1 << 31 : 0): hub vertices keep an empty segment here and
         // their du is written by the hub paths
@@ -344,12 +347,12 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
         for (int j = 0; j < NPV; ++j)
             pvs[j] = (valid && VPar<VM>::np > 0) ? p[vd.param0 + static_cast<int64_t>(q) * vd.pdim + j] : 0.0;
         const double own = (DC == 1 || vd.mask[0] == 0) ? xs[0] : xs[DC - 1];
-        s_own[t] = own;
+        sh.own[t] = own;
         double acc = 0.0;
         const int32_t W = 4 * T;
         for (int32_t base = S0; base < S1; base += W) {
             const int32_t wend = min(base + W, S1);
-            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) s_owner[s2 - base] = static_cast<uint8_t>(t);
+            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) sh.owner[s2 - base] = static_cast<uint8_t>(t);
             __syncthreads();
             constexpr int NP = EModel<EM>::np;
             int2 sl[4];
@@ -384,18 +387,85 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
             for (int j = 0; j < 4; ++j) {
                 const int32_t s2 = base + j * T + t;
                 if (s2 < wend) {
-                    const double ow = s_own[s_owner[s2 - base]];
+                    const double ow = sh.own[sh.owner[s2 - base]];
                     double v;
                     if constexpr (MULTI) v = slot_value_rt(ed[vd.lseb[s2]], sl[j], ow, u, vo, p);
                     else v = slot_eval<EM, WRAP>(sl[j], nb[j], kk[j], ow);
-                    s_val[s2 - base] = v;
+                    sh.val[s2 - base] = v;
                 }
             }
             __syncthreads();
-            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) acc += s_val[s2 - base];
+            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) acc += sh.val[s2 - base];
             __syncthreads();
         }
         if (valid && !hub) vertex_emit<VM, STAGE>(du, sa, x0, xs, acc, pvs, dim, vd.mask[0], static_cast<int64_t>(vd.out0) + q);
+    }
+}
+
+template <int VM, int EM, int WRAP, bool MULTI, bool STAGE, bool KC>
+__global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __restrict__ rowptr,
+                                                    const int2* __restrict__ slots,
+                                                    const uint8_t* __restrict__ slot_eb,
+                                                    const EDesc* __restrict__ ed,
+                                                    const double* __restrict__ u, const double* __restrict__ vo,
+                                                    const double* __restrict__ p,
+                                                    double* __restrict__ du, StageArgs sa) {
+    __shared__ RhsShared sh;
+    vertex_rhs_body<VM, EM, WRAP, MULTI, STAGE, KC>(vd, static_cast<int32_t>(blockIdx.x), sh, rowptr, slots, slot_eb,
+                                                    ed, u, vo, p, du, sa);
+}
+
+// Several vertex batches of the fused set in one launch: batch i owns the blocks
+// [first[i], first[i+1]) and runs them exactly as its own k_vertex_rhs launch would.
+// Small batches then overlap their load-latency chains instead of running one after
+// the other.
+constexpr int kGroupMax = 4;
+struct VGroup {
+    VDesc vd[kGroupMax];
+    int32_t first[kGroupMax + 1];
+    int32_t n;
+};
+
+__device__ __forceinline__ int group_of_block(const VGroup& g, int32_t bx) {
+    int i = 0;
+    while (i + 1 < g.n && bx >= g.first[i + 1]) ++i;
+    return i;
+}
+
+template <int EM, int WRAP, bool MULTI, bool STAGE, bool KC>
+__global__ void __launch_bounds__(256) k_vertex_rhs_group(VGroup g, const int32_t* __restrict__ rowptr,
+                                                          const int2* __restrict__ slots,
+                                                          const uint8_t* __restrict__ slot_eb,
+                                                          const EDesc* __restrict__ ed,
+                                                          const double* __restrict__ u, const double* __restrict__ vo,
+                                                          const double* __restrict__ p,
+                                                          double* __restrict__ du, StageArgs sa) {
+    __shared__ RhsShared sh;
+    const int32_t bx = static_cast<int32_t>(blockIdx.x);
+    const int i = group_of_block(g, bx);
+    const VDesc& vd = g.vd[i];
+    const int32_t b = bx - g.first[i];
+    switch (vd.model) {
+    case ND_VM_DIFFUSION:
+        vertex_rhs_body<ND_VM_DIFFUSION, EM, WRAP, MULTI, STAGE, KC>(vd, b, sh, rowptr, slots, slot_eb, ed, u, vo, p, du, sa);
+        break;
+    case ND_VM_KURAMOTO:
+        vertex_rhs_body<ND_VM_KURAMOTO, EM, WRAP, MULTI, STAGE, KC>(vd, b, sh, rowptr, slots, slot_eb, ed, u, vo, p, du, sa);
+        break;
+    case ND_VM_SWING:
+        vertex_rhs_body<ND_VM_SWING, EM, WRAP, MULTI, STAGE, KC>(vd, b, sh, rowptr, slots, slot_eb, ed, u, vo, p, du, sa);
+        break;
+    case ND_VM_SWING_UNIT_DAMPING:
+        vertex_rhs_body<ND_VM_SWING_UNIT_DAMPING, EM, WRAP, MULTI, STAGE, KC>(vd, b, sh, rowptr, slots, slot_eb, ed, u, vo,
+                                                                             p, du, sa);
+        break;
+    case ND_VM_LOAD:
+        vertex_rhs_body<ND_VM_LOAD, EM, WRAP, MULTI, STAGE, KC>(vd, b, sh, rowptr, slots, slot_eb, ed, u, vo, p, du, sa);
+        break;
+    default:
+        vertex_rhs_body<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false, STAGE, false>(vd, b, sh, rowptr, slots, slot_eb, ed,
+                                                                                  u, vo, p, du, sa);
+        break;
     }
 }
 
@@ -553,31 +623,27 @@ __device__ __forceinline__ void ksync_edge(const KSync& ks, int64_t e, const dou
     }
 }
 
-__global__ void __launch_bounds__(256) k_vertex_out(VDesc vd, const double* __restrict__ u, double* __restrict__ o,
-                                                    KSync ks, const double* __restrict__ p, int tangent) {
-    if (static_cast<int32_t>(blockIdx.x) >= ks.vblocks) {
-        const int64_t nthr = static_cast<int64_t>(gridDim.x - ks.vblocks) * blockDim.x;
-        const int64_t tid = static_cast<int64_t>(blockIdx.x - ks.vblocks) * blockDim.x + threadIdx.x;
-        const bool vec = ks.np == 1 && (reinterpret_cast<uintptr_t>(ks.pe) & 15) == 0;
-        for (int64_t e = 2 * tid; e < ks.count; e += 2 * nthr) {
-            if (vec && e + 1 < ks.count) {
-                const double2 k2 = *reinterpret_cast<const double2*>(ks.pe + e);
-                const double a = k2.x, b = k2.y;
-                ksync_edge(ks, e, &a);
-                ksync_edge(ks, e + 1, &b);
-            } else {
-                for (int64_t f = e; f < e + 2 && f < ks.count; ++f) {
-                    double nv[2];
-                    for (int c = 0; c < ks.np; ++c) nv[c] = ks.pe[f * ks.np + c];
-                    ksync_edge(ks, f, nv);
-                }
+__device__ __forceinline__ void ksync_part(const KSync& ks, int64_t tid, int64_t nthr) {
+    const bool vec = ks.np == 1 && (reinterpret_cast<uintptr_t>(ks.pe) & 15) == 0;
+    for (int64_t e = 2 * tid; e < ks.count; e += 2 * nthr) {
+        if (vec && e + 1 < ks.count) {
+            const double2 k2 = *reinterpret_cast<const double2*>(ks.pe + e);
+            const double a = k2.x, b = k2.y;
+            ksync_edge(ks, e, &a);
+            ksync_edge(ks, e + 1, &b);
+        } else {
+            for (int64_t f = e; f < e + 2 && f < ks.count; ++f) {
+                double nv[2];
+                for (int c = 0; c < ks.np; ++c) nv[c] = ks.pe[f * ks.np + c];
+                ksync_edge(ks, f, nv);
             }
         }
-        return;
     }
-    const int64_t nthr = static_cast<int64_t>(ks.vblocks) * blockDim.x;
-    const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    // two-state vertices with output state 1 (swing): two vertices = one 32-byte read
+}
+
+// the outputs of one vertex batch, grid-stride over nthr threads
+__device__ __forceinline__ void vertex_out_part(const VDesc& vd, int64_t tid, int64_t nthr, const double* __restrict__ u,
+                                                double* __restrict__ o, const double* __restrict__ p, int tangent) {
     if (vd.outkind == 2) return;                 // injector outputs: k_vertex_out_ff, after the hubs'
     if (vd.outkind == 1) {                       // NoFeedForward outputs from the parameters
         for (int64_t q = tid; q < vd.count; q += nthr)
@@ -585,6 +651,7 @@ __global__ void __launch_bounds__(256) k_vertex_out(VDesc vd, const double* __re
                 tangent ? 0.0 : vertex_out_nff<double>(vd.model, p + vd.param0 + q * vd.pdim);
         return;
     }
+    // two-state vertices with output state 1 (swing): two vertices = one 32-byte read
     const bool vec2 = vd.dim == 2 && vd.depth == 1 && vd.mask[0] == 0 && (vd.state0 & 1) == 0 &&
                       (reinterpret_cast<uintptr_t>(u) & 15) == 0 && ((vd.out0 & 1) == 0) &&
                       (reinterpret_cast<uintptr_t>(o) & 15) == 0;
@@ -601,6 +668,33 @@ __global__ void __launch_bounds__(256) k_vertex_out(VDesc vd, const double* __re
             }
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_vertex_out(VDesc vd, const double* __restrict__ u, double* __restrict__ o,
+                                                    KSync ks, const double* __restrict__ p, int tangent) {
+    if (static_cast<int32_t>(blockIdx.x) >= ks.vblocks) {
+        ksync_part(ks, static_cast<int64_t>(blockIdx.x - ks.vblocks) * blockDim.x + threadIdx.x,
+                   static_cast<int64_t>(gridDim.x - ks.vblocks) * blockDim.x);
+        return;
+    }
+    vertex_out_part(vd, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x,
+                    static_cast<int64_t>(ks.vblocks) * blockDim.x, u, o, p, tangent);
+}
+
+// The same for up to kGroupMax vertex batches in one launch: batch i owns the blocks
+// [g.first[i], g.first[i+1]) and ks.vblocks = g.first[g.n].
+__global__ void __launch_bounds__(256) k_vertex_out_group(VGroup g, const double* __restrict__ u,
+                                                          double* __restrict__ o, KSync ks,
+                                                          const double* __restrict__ p, int tangent) {
+    const int32_t bx = static_cast<int32_t>(blockIdx.x);
+    if (bx >= ks.vblocks) {
+        ksync_part(ks, static_cast<int64_t>(bx - ks.vblocks) * blockDim.x + threadIdx.x,
+                   static_cast<int64_t>(gridDim.x - ks.vblocks) * blockDim.x);
+        return;
+    }
+    const int i = group_of_block(g, bx);
+    vertex_out_part(g.vd[i], static_cast<int64_t>(bx - g.first[i]) * blockDim.x + threadIdx.x,
+                    static_cast<int64_t>(g.first[i + 1] - g.first[i]) * blockDim.x, u, o, p, tangent);
 }
 
 // Injector outputs (coreloop.jl:44-55): the loopback copy of the hub's output is the
@@ -907,6 +1001,45 @@ VertexKernel pick_vertex_kernel(int vm, int em, int wrap, bool multi, bool kc) {
     }
 }
 
+using GroupKernel = void (*)(VGroup, const int32_t*, const int2*, const uint8_t*, const EDesc*, const double*,
+                             const double*, const double*, double*, StageArgs);
+
+template <int EM, int WRAP, bool STAGE>
+GroupKernel gpick3(bool kc) {
+    if constexpr (EModel<EM>::np > 0 && !is_ode_model(EM))
+        if (kc) return k_vertex_rhs_group<EM, WRAP, false, STAGE, true>;
+    return k_vertex_rhs_group<EM, WRAP, false, STAGE, false>;
+}
+
+// the group kernel with the template arguments pick_edge gives each member batch
+template <bool STAGE>
+GroupKernel pick_group_kernel(int em, int wrap, bool multi, bool kc) {
+    if (multi) return k_vertex_rhs_group<ND_EM_NOOP, ND_WRAP_NONE, true, STAGE, false>;
+    const bool sym = wrap == ND_WRAP_SYMMETRIC;
+    switch (em) {
+    case ND_EM_KURAMOTO:
+        return sym ? gpick3<ND_EM_KURAMOTO, ND_WRAP_SYMMETRIC, STAGE>(kc)
+                   : gpick3<ND_EM_KURAMOTO, ND_WRAP_ANTISYMMETRIC, STAGE>(kc);
+    case ND_EM_STATIC_LINE:
+        return sym ? gpick3<ND_EM_STATIC_LINE, ND_WRAP_SYMMETRIC, STAGE>(kc)
+                   : gpick3<ND_EM_STATIC_LINE, ND_WRAP_ANTISYMMETRIC, STAGE>(kc);
+    case ND_EM_DIFFUSION:
+        return sym ? gpick3<ND_EM_DIFFUSION, ND_WRAP_SYMMETRIC, STAGE>(false)
+                   : gpick3<ND_EM_DIFFUSION, ND_WRAP_ANTISYMMETRIC, STAGE>(false);
+    case ND_EM_DIFFUSION_WEIGHTED:
+        return sym ? gpick3<ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_SYMMETRIC, STAGE>(kc)
+                   : gpick3<ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_ANTISYMMETRIC, STAGE>(kc);
+    case ND_EM_DIFFUSION_FIDUCIAL:
+        return gpick3<ND_EM_DIFFUSION_FIDUCIAL, ND_WRAP_ANTISYMMETRIC, STAGE>(kc);
+    case ND_EM_ODE_SIN_DIFFUSION:
+    case ND_EM_ODE_SIN_DIFFUSION_FAST:
+    case ND_EM_ODE_LINEAR:
+        return gpick3<ND_EM_ODE_LINEAR, ND_WRAP_FIDUCIAL, STAGE>(false);
+    default:
+        return gpick3<ND_EM_NOOP, ND_WRAP_NONE, STAGE>(false);
+    }
+}
+
 template <bool STAGE>
 VertexKernel pick_gen_kernel(int vm) {
     switch (vm) {
@@ -1064,6 +1197,7 @@ struct nd_plan {
     std::vector<VDesc> vdesc;
     std::vector<EDesc> edesc;
     std::vector<VertexKernel> vkern, vkern_stage;
+    GroupKernel gkern = nullptr, gkern_stage = nullptr;   // several fused-set batches per launch
     std::vector<JvpKernel> vkern_jvp;
     double* d_dvout = nullptr;     // compact tangent outputs (Jacobian-vector products)
     double* rk_work = nullptr;
@@ -1284,6 +1418,8 @@ int apply_gather_mode(nd_plan* pl) {
     pl->vkern.clear();
     pl->vkern_stage.clear();
     const int em = pl->edesc.empty() ? ND_EM_NOOP : pl->em0;
+    pl->gkern = pick_group_kernel<false>(em, pl->wrap0, pl->multi, pl->kcache);
+    pl->gkern_stage = pick_group_kernel<true>(em, pl->wrap0, pl->multi, pl->kcache);
     for (const VDesc& vd : pl->vdesc) {
         pl->vkern.push_back(vd.generic ? pick_gen_kernel<false>(vd.model)
                                        : pick_vertex_kernel<false>(vd.model, em, pl->wrap0, pl->multi, pl->kcache));
@@ -1333,12 +1469,14 @@ void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const doub
     // vertex outputs o_v = u[StateMask] into a compact array (coreloop.jl:39): the edge
     // evaluations gather neighbour outputs from it instead of from the interleaved state
     if (pl->use_vout && build) {
+        // every batch's outputs, up to kGroupMax batches per launch; the first launch also
+        // brings the slot-ordered parameter copy up to date
         bool pending = sync && pl->kcache;
-        for (const VDesc& vd : pl->vdesc) {
-            if (!vd.count) continue;
-            const unsigned vb = vout_blocks(vd.count);
-            KSync ks = no_sync(vb);
-            unsigned nb = vb;
+        VGroup g{};
+        auto flush = [&]() {
+            if (!g.n) return;
+            KSync ks = no_sync(static_cast<unsigned>(g.first[g.n]));
+            unsigned nb = static_cast<unsigned>(g.first[g.n]);
             if (pending) {
                 ks.pe = p + pl->kparam0;
                 ks.kc = pl->d_kc;
@@ -1350,22 +1488,56 @@ void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const doub
                 nb += vout_blocks(pl->kcount);
                 pending = false;
             }
-            hipLaunchKernelGGL(k_vertex_out, dim3(nb), dim3(256), 0, s, vd, u, vin, ks, p, 0);
+            if (g.n == 1) hipLaunchKernelGGL(k_vertex_out, dim3(nb), dim3(256), 0, s, g.vd[0], u, vin, ks, p, 0);
+            else hipLaunchKernelGGL(k_vertex_out_group, dim3(nb), dim3(256), 0, s, g, u, vin, ks, p, 0);
+            g.n = 0;
+        };
+        for (const VDesc& vd : pl->vdesc) {
+            if (!vd.count) continue;
+            if (g.n == 0) g.first[0] = 0;
+            g.vd[g.n] = vd;
+            g.first[g.n + 1] = g.first[g.n] + static_cast<int32_t>(vout_blocks(vd.count));
+            if (++g.n == kGroupMax) flush();
         }
+        flush();
         if (pl->ff_out)
             for (const VDesc& vd : pl->vdesc)
                 if (vd.outkind == 2 && vd.count)
                     hipLaunchKernelGGL(k_vertex_out_ff, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, vin, p);
     }
+    // fused-set batches go out together, up to kGroupMax per launch; a lone one (and every
+    // per-vertex-kernel batch) takes its own kernel
+    VGroup g{};
+    size_t gb[kGroupMax] = {};
+    auto single = [&](size_t b, const VDesc& vd, int64_t nblocks) {
+        hipLaunchKernelGGL(stage ? pl->vkern_stage[b] : pl->vkern[b], dim3(static_cast<unsigned>(nblocks)),
+                           dim3(static_cast<unsigned>(B)), 0, s, vd, pl->d_rowptr, pl->d_slots, pl->d_slot_eb,
+                           pl->d_edesc, u, vin, p, du, sa);
+    };
+    auto flush = [&]() {
+        if (g.n == 1) single(gb[0], g.vd[0], g.first[1]);
+        else if (g.n > 1)
+            hipLaunchKernelGGL(stage ? pl->gkern_stage : pl->gkern, dim3(static_cast<unsigned>(g.first[g.n])),
+                               dim3(static_cast<unsigned>(B)), 0, s, g, pl->d_rowptr, pl->d_slots, pl->d_slot_eb,
+                               pl->d_edesc, u, vin, p, du, sa);
+        g.n = 0;
+    };
     for (size_t b = 0; b < pl->vdesc.size(); ++b) {
         VDesc vd = pl->vdesc[b];
         if (vd.count == 0) continue;
         vd.t = t;
         const int64_t nblocks = vd.nbig + blocks_for(static_cast<int64_t>(vd.nmid) * kWave, B) + blocks_for(vd.count, B);
-        hipLaunchKernelGGL(stage ? pl->vkern_stage[b] : pl->vkern[b], dim3(static_cast<unsigned>(nblocks)),
-                           dim3(static_cast<unsigned>(B)), 0, s, vd, pl->d_rowptr, pl->d_slots, pl->d_slot_eb,
-                           pl->d_edesc, u, vin, p, du, sa);
+        if (vd.generic || !pl->gkern) {
+            single(b, vd, nblocks);
+            continue;
+        }
+        if (g.n == 0) g.first[0] = 0;
+        g.vd[g.n] = vd;
+        gb[g.n] = b;
+        g.first[g.n + 1] = g.first[g.n] + static_cast<int32_t>(nblocks);
+        if (++g.n == kGroupMax) flush();
     }
+    flush();
     for (const EDesc& e : pl->edesc) {
         if (!e.ode || e.count == 0) continue;
         hipLaunchKernelGGL(stage ? pick_edge_ode<true>(e.model) : pick_edge_ode<false>(e.model),

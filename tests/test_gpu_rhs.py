@@ -88,6 +88,36 @@ def test_random_mixed_batches(ND, dev, seed, directed):
     _check(nw, u, p, dev)
 
 
+def _six_vertex_batches(ND, n, seed, edges):
+    """Six vertex batches of the fused kernel set (swing twice, through two distinct
+    device functions, unit-damping swing, first-order Kuramoto, diffusion, load): more
+    batches than one grouped launch takes."""
+    import copy
+    L, c = ND.Lib, ND.components
+    swing2 = c.VertexModel(f=copy.copy(c.kuramoto_inertia), sym=["delta", "omega"], psym=["M", "D", "Pm"],
+                           g=c.StateMask(1), name="swing2")
+    vt = [L.kuramoto_second(), swing2, L.bench.kuramoto_vertex_2d(), L.kuramoto_first(), L.diffusion_vertex(),
+          L.load_vertex()]
+    rng = np.random.default_rng(seed)
+    g = ND.watts_strogatz(n, 4, 0.6, seed=seed)
+    vm = [vt[i] for i in rng.integers(0, len(vt), n)]
+    em = [edges[i] for i in rng.integers(0, len(edges), g.ne())] if len(edges) > 1 else edges[0]
+    nw = ND.Network(g, vm, em)
+    assert len(list(nw.batches(False))) == 6
+    return nw, rng.normal(size=nw.dim), rng.uniform(0.5, 2.0, nw.pdim)
+
+
+def test_many_vertex_batches_grouped_launches(ND, dev):
+    L = ND.Lib
+    for edges in ([L.static_line_edge()], [L.kuramoto_edge(), L.diffusion_edge(), L.diffusion_odeedge()]):
+        nw, u, p = _six_vertex_batches(ND, 4000, 7, edges)
+        _check(nw, u, p, dev)
+        # fused RK stages go through the grouped launches as well (several batches: the
+        # four-launches-per-step path)
+        ref = oracle_for(nw).rk4(u, p, 0.0, 0.001, 50)
+        np.testing.assert_allclose(_dev_rk4(nw, u, p, dev, 0.001, 50), ref, rtol=0, atol=1e-11)
+
+
 def test_diffusion_closed_form_gpu(ND, dev):
     import scipy.sparse as sp
     g = ND.barabasi_albert(2000, 5, seed=2)
