@@ -104,7 +104,15 @@ enum {
  *   LOOPBACK             src/post_utils.jl:105-108             Directed: y_dst = -1 * v_s, the
  *                        LoopbackConnection from an injector (src) to its hub (dst); the hub's
  *                        output is copied to the injector's input (post_utils.jl:221-228)
- *   NOOP                 g does nothing (test/aggregators_test.jl:20-30)                    */
+ *   ODE_LAG              one-state form of ComponentLibrary.jl:30-34: de = 1/tau*(sin(v_s - v_d) - e),
+ *                        p = [tau]; output a wrapped StateMask (AntiSymmetric(1): y_dst = e,
+ *                        y_src = -e; Symmetric(1); Directed(1)), component_functions.jl:117-175
+ *   EXT_DRIVEN           test/external_inputs_test.jl:45-47: de = ext[1] (one external input),
+ *                        any StateMask output
+ *   NOOP                 g does nothing (test/aggregators_test.jl:20-30)
+ * ODE edge models (state dimension 2 for ODE_SIN_DIFFUSION, ODE_SIN_DIFFUSION_FAST and
+ * ODE_LINEAR, 1 for ODE_LAG and EXT_DRIVEN) take Fiducial StateMask outputs or a StateMask
+ * inside AntiSymmetric / Symmetric / Directed.                                              */
 enum {
     ND_EM_NOOP = 0,
     ND_EM_DIFFUSION = 1,
@@ -116,7 +124,9 @@ enum {
     ND_EM_ODE_SIN_DIFFUSION_FAST = 7,
     ND_EM_ODE_LINEAR = 8,
     ND_EM_RESISTOR = 9,
-    ND_EM_LOOPBACK = 10
+    ND_EM_LOOPBACK = 10,
+    ND_EM_ODE_LAG = 11,
+    ND_EM_EXT_DRIVEN = 12
 };
 
 #define ND_MAX_OUT 4
@@ -246,11 +256,13 @@ int nd_network_mass_matrix(const nd_network* nw, const nd_mass* vertex_mass, con
 typedef struct nd_plan nd_plan;
 
 /* Uploads the network's index structures to the current HIP device and selects
- * the fused kernels.  ND_ERR_UNSUPPORTED if a model has no HIP kernel (external
- * inputs of edges, ODE edges next to computed vertex outputs, output depth > 1 with a
- * functional model); ND_ERR_ARGUMENT if the network has external inputs and no ExtMap
- * (nd_network_set_extmap).  Vertex external inputs resolve to state indices: a
- * vertex output and an ODE edge's output are StateMasks of states. */
+ * the fused kernels.  ND_ERR_UNSUPPORTED if a model has no HIP kernel (ODE edges
+ * next to computed vertex outputs, output depth > 1 with a functional model);
+ * ND_ERR_ARGUMENT if the network has external inputs and no ExtMap
+ * (nd_network_set_extmap) or a model's external-input count differs from its kind's.
+ * External inputs (of vertices and of ODE edges) resolve to state indices: a vertex
+ * output and an ODE edge's output are StateMasks of states, the src side of an
+ * AntiSymmetric one the negated state. */
 int nd_plan_create(const nd_network* nw, nd_plan** out);
 void nd_plan_destroy(nd_plan* plan);
 

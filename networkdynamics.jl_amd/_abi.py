@@ -18,7 +18,7 @@ ND_FF_PURESTATEMAP, ND_FF_PUREFEEDFORWARD, ND_FF_FEEDFORWARD, ND_FF_NOFEEDFORWAR
  ND_VM_INDUCTOR_INJECTOR, ND_VM_CRL) = range(13)
 (ND_EM_NOOP, ND_EM_DIFFUSION, ND_EM_DIFFUSION_WEIGHTED, ND_EM_KURAMOTO, ND_EM_STATIC_LINE,
  ND_EM_DIFFUSION_FIDUCIAL, ND_EM_ODE_SIN_DIFFUSION, ND_EM_ODE_SIN_DIFFUSION_FAST, ND_EM_ODE_LINEAR,
- ND_EM_RESISTOR, ND_EM_LOOPBACK) = range(11)
+ ND_EM_RESISTOR, ND_EM_LOOPBACK, ND_EM_ODE_LAG, ND_EM_EXT_DRIVEN) = range(13)
 ND_MAX_OUT = 4
 ND_MAX_EXT = 8
 ND_MASS_IDENTITY, ND_MASS_UNIFORM, ND_MASS_DIAGONAL, ND_MASS_DENSE = range(4)

@@ -492,7 +492,7 @@ class EdgeModel(ComponentModel):
         self._gfn = fn
         self.outsym = [str(x) for x in outsym] if outsym is not None else None
         self.mass_matrix = _normalize_mass(mass_matrix, self.dim)
-        self.extin = _extin(extin, None)
+        self.extin = _extin(extin, f)
 
     def __repr__(self):
         return f"EdgeModel :{self.name} (dim={self.dim}, pdim={self.pdim}, outdim={self.outdim})"
@@ -531,6 +531,11 @@ diffusion_dedge_fast = DeviceFunction("diffusion_dedge_fast!", "edge_f", A.ND_EM
                                       "benchmark/benchmark_models.jl:11-15")
 real_ode_edge = DeviceFunction("real_ode_edge!", "edge_f", A.ND_EM_ODE_LINEAR, 2, 0,
                                "test/diffusion_test.jl:102-106")
+# one-state line lag de = 1/tau (sin(v_s - v_d) - e) (the dst half of ComponentLibrary.jl:30-34),
+# output through a wrapped StateMask (AntiSymmetric(1) gives the src side -e)
+ode_lag_f = DeviceFunction("ode_lag!", "edge_f", A.ND_EM_ODE_LAG, 1, 1, "test/ComponentLibrary.jl:30-32")
+# de[1] = ext[1]: the edge of test/external_inputs_test.jl:45-47
+feext = DeviceFunction("feext", "edge_f", A.ND_EM_EXT_DRIVEN, 1, 0, "test/external_inputs_test.jl:45-47", ext=1)
 resistor_g = DeviceFunction("resistor_g", "edge_g", A.ND_EM_RESISTOR, 0, 1, "docs/src/external_inputs.md:66-72")
 f_load = DeviceFunction("f_load", "vertex_f", A.ND_VM_CAPACITOR_SINK, 1, 1, "docs/src/external_inputs.md:76-82")
 f_source = DeviceFunction("f_source", "vertex_f", A.ND_VM_PI_SOURCE, 2, 4, "docs/src/external_inputs.md:86-99",

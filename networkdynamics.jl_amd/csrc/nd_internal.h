@@ -83,8 +83,10 @@ struct Network {
 };
 
 // u-index (0-based) of output-buffer index k (1-based) when that output is a StateMask
-// of a state (vertex outputs; ODE-edge Fiducial outputs), else -1.
-int64_t output_state_index(const Network& n, int64_t k);
+// of a state (vertex outputs; ODE-edge StateMask outputs, plain or wrapped), else -1.
+// *negated (if given): the output is the negated state (the src side of an
+// AntiSymmetric StateMask edge).
+int64_t output_state_index(const Network& n, int64_t k, bool* negated = nullptr);
 
 // Vertex output u-index (0-based) of vertex v, output element j: StateMask outputs only.
 inline int64_t vertex_out_u_index(const Network& n, int64_t v, int j) {

@@ -37,7 +37,33 @@ template <> __device__ __forceinline__ double zero_of<double>() { return 0.0; }
 template <> __device__ __forceinline__ Dual zero_of<Dual>() { return {0.0, 0.0}; }
 
 __host__ __device__ constexpr bool is_ode_model(int em) {
-    return em == ND_EM_ODE_SIN_DIFFUSION || em == ND_EM_ODE_SIN_DIFFUSION_FAST || em == ND_EM_ODE_LINEAR;
+    return em == ND_EM_ODE_SIN_DIFFUSION || em == ND_EM_ODE_SIN_DIFFUSION_FAST || em == ND_EM_ODE_LINEAR ||
+           em == ND_EM_ODE_LAG || em == ND_EM_EXT_DRIVEN;
+}
+// state dimension of each ODE edge model, and the external inputs its f reads
+__host__ __device__ constexpr int ode_dim(int em) {
+    return (em == ND_EM_ODE_LAG || em == ND_EM_EXT_DRIVEN) ? 1 : 2;
+}
+__host__ __device__ constexpr int edge_ext_inputs(int em) { return em == ND_EM_EXT_DRIVEN ? 1 : 0; }
+
+// f of an ODE edge: dx from its state x, the endpoint outputs and its external inputs;
+// S = double or Dual
+template <int EM, typename S>
+__device__ __forceinline__ void edge_ode_f(S* dx, const S* x, S vs, S vd, const S* ext, const double* __restrict__ pe) {
+    if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION) {            // ComponentLibrary.jl:30-34
+        dx[0] = 1.0 / pe[0] * (sin(vs - vd) - x[0]);
+        dx[1] = 1.0 / pe[0] * (sin(vd - vs) - x[1]);
+    } else if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION_FAST) { // benchmark_models.jl:11-15
+        dx[0] = 100.0 * (sin(vs - vd) - x[0]);
+        dx[1] = 100.0 * (sin(vd - vs) - x[1]);
+    } else if constexpr (EM == ND_EM_ODE_LAG) {               // ComponentLibrary.jl:30-32, one state
+        dx[0] = 1.0 / pe[0] * (sin(vs - vd) - x[0]);
+    } else if constexpr (EM == ND_EM_EXT_DRIVEN) {            // external_inputs_test.jl:45-47
+        dx[0] = ext[0];
+    } else {                                                   // diffusion_test.jl:102-106
+        dx[0] = vs - vd - x[0];
+        dx[1] = vd - vs - x[1];
+    }
 }
 
 // dst-side output of a pure feed-forward edge model g(v_s, v_d, p); S = double or Dual

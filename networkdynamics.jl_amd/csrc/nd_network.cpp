@@ -174,7 +174,8 @@ static int build(Network& n) {
     return ND_OK;
 }
 
-int64_t output_state_index(const Network& n, int64_t k) {
+int64_t output_state_index(const Network& n, int64_t k, bool* negated) {
+    if (negated) *negated = false;
     // output blocks: vertex batches, then edge batches, ascending (registration order)
     for (const Batch& b : n.vb) {
         const nd_kind& kd = n.kinds[b.kind];
@@ -193,6 +194,7 @@ int64_t output_state_index(const Network& n, int64_t k) {
             if (kd.ff != ND_FF_PURESTATEMAP) return -1;
             const int64_t q = (k - b.out_first) / stride, j = (k - b.out_first) % stride;
             const int32_t m = j < kd.outdim_src ? kd.mask_src[j] : kd.mask_dst[j - kd.outdim_src];
+            if (negated) *negated = j < kd.outdim_src && kd.wrapper == ND_WRAP_ANTISYMMETRIC;
             return b.state_first - 1 + q * kd.dim + m - 1;
         }
     }

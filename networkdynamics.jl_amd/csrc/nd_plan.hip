@@ -89,19 +89,48 @@ struct EDesc {            // one edge batch
     int32_t mask_src[ND_MAX_OUT];
     int32_t mask_dst[ND_MAX_OUT];
     const int2* ends;     // per edge (batch order): u-index of the src / dst vertex output
+    int32_t ext0, extdim; // ODE edges with external inputs: first external-buffer entry of
+    const int32_t* xmap;  // the batch (0-based), per edge; entries resolved as ext_index does
 };
+
+// An ODE edge's output as one side of it sees it: a StateMask of its state, negated on
+// the src side of an AntiSymmetric wrapper (component_functions.jl:117-127; Symmetric and
+// Fiducial pass the state, Directed has no src side).  is_src: the entry's role bit.
+template <int WRAP>
+__device__ __forceinline__ double ode_side(double x, bool is_src) {
+    return (WRAP == ND_WRAP_ANTISYMMETRIC && is_src) ? -x : x;
+}
+__device__ __forceinline__ double ode_side_rt(int wrap, double x, bool is_src) {
+    return (wrap == ND_WRAP_ANTISYMMETRIC && is_src) ? -x : x;
+}
+__device__ __forceinline__ Dual ode_side_rt(int wrap, Dual x, bool is_src) {
+    return (wrap == ND_WRAP_ANTISYMMETRIC && is_src) ? -x : x;
+}
+
+// External-input entry k from the resolved map: a u-index, or ~u-index where the source
+// is the src-side output of an AntiSymmetric StateMask edge (the negated state).
+__device__ __forceinline__ double ext_read(const int32_t* __restrict__ xmap, int64_t k, const double* __restrict__ u) {
+    const int32_t xi = xmap[k];
+    return xi >= 0 ? u[xi] : -u[~xi];
+}
+__device__ __forceinline__ Dual ext_read_dual(const int32_t* __restrict__ xmap, int64_t k, const double* __restrict__ u,
+                                              const double* __restrict__ v) {
+    const int32_t xi = xmap[k];
+    return xi >= 0 ? Dual{u[xi], v[xi]} : Dual{-u[~xi], -v[~xi]};
+}
 
 // Value of one destination-list entry for the vertex whose own output is `own`.
 // s.x: u-index of the other endpoint's output (feed-forward edges) or of the edge
 //      state element this side outputs (ODE edges);
-// s.y: (edge parameter offset << 1) | (1 if this vertex is the edge's src).
+// s.y: (edge parameter offset << 1) | (1 if this vertex is the edge's src); ODE edges
+//      carry the role bit only.
 template <int EM, int WRAP>
 __device__ __forceinline__ double slot_value(int2 s, double own, const double* __restrict__ u,
                                              const double* __restrict__ vo, const double* __restrict__ p) {
     if constexpr (EM == ND_EM_NOOP) {
         return __builtin_nan("");          // g writes nothing: o keeps its NaN fill
     } else if constexpr (is_ode_model(EM)) {
-        return u[s.x];                     // Fiducial(StateMask, StateMask)
+        return ode_side<WRAP>(u[s.x], s.y & 1);
     } else {
         const double nbr = vo[s.x];
         const double* pe = p + (s.y >> 1);
@@ -116,7 +145,7 @@ __device__ __forceinline__ double slot_value(int2 s, double own, const double* _
 __device__ __forceinline__ double slot_value_rt(const EDesc& e, int2 s, double own, const double* __restrict__ u,
                                                 const double* __restrict__ vo, const double* __restrict__ p) {
     if (e.model == ND_EM_NOOP) return __builtin_nan("");
-    if (e.ode) return u[s.x];
+    if (e.ode) return ode_side_rt(e.wrap, u[s.x], s.y & 1);
     const double nbr = vo[s.x];
     const double* pe = p + (s.y >> 1);
     if (s.y & 1) {
@@ -197,7 +226,7 @@ __device__ __forceinline__ double slot_eval(int2 s, double nbr, const double* k,
     if constexpr (EM == ND_EM_NOOP) {
         return __builtin_nan("");
     } else if constexpr (is_ode_model(EM)) {
-        return nbr;
+        return ode_side<WRAP>(nbr, s.y & 1);
     } else {
         if (s.y & 1) {
             const double y = g_dst<EM>(own, nbr, k);
@@ -469,31 +498,29 @@ __global__ void __launch_bounds__(256) k_vertex_rhs_group(VGroup g, const int32_
     }
 }
 
-// du of ODE edges (coreloop.jl:76, edge f without feed-forward)
+// du of ODE edges (coreloop.jl:76, edge f without feed-forward); external inputs are
+// read from the states they resolve to (collect_externals! fills them from u and from
+// StateMask outputs, external_inputs.jl:49-77)
 template <int EM, bool STAGE>
 __global__ void __launch_bounds__(256) k_edge_ode(EDesc e, const double* __restrict__ u,
                                                   const double* __restrict__ p, double* __restrict__ du,
                                                   StageArgs sa) {
+    constexpr int D = ode_dim(EM);
+    constexpr int NX = edge_ext_inputs(EM) > 0 ? edge_ext_inputs(EM) : 1;
     const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= e.count) return;
     const int2 en = e.ends[q];
     const double vs = u[en.x], vd = u[en.y];
-    const int64_t x0 = static_cast<int64_t>(e.state0) + static_cast<int64_t>(q) * 2;
-    const double* x = u + x0;
-    double dx[2];
-    if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION) {            // ComponentLibrary.jl:30-34
-        const double* pe = p + e.param0 + static_cast<int64_t>(q) * e.pdim;
-        dx[0] = 1.0 / pe[0] * (sin(vs - vd) - x[0]);
-        dx[1] = 1.0 / pe[0] * (sin(vd - vs) - x[1]);
-    } else if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION_FAST) { // benchmark_models.jl:11-15
-        dx[0] = 100.0 * (sin(vs - vd) - x[0]);
-        dx[1] = 100.0 * (sin(vd - vs) - x[1]);
-    } else {                                                   // diffusion_test.jl:102-106
-        dx[0] = vs - vd - x[0];
-        dx[1] = vd - vs - x[1];
-    }
-    emit1<STAGE>(du, sa, x0, dx[0]);
-    emit1<STAGE>(du, sa, x0 + 1, dx[1]);
+    const int64_t x0 = static_cast<int64_t>(e.state0) + static_cast<int64_t>(q) * D;
+    double x[D], ext[NX], dx[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) x[j] = u[x0 + j];
+#pragma unroll
+    for (int j = 0; j < NX; ++j)
+        ext[j] = j < edge_ext_inputs(EM) ? ext_read(e.xmap, e.ext0 + static_cast<int64_t>(q) * e.extdim + j, u) : 0.0;
+    edge_ode_f<EM, double>(dx, x, vs, vd, ext, p + e.param0 + static_cast<int64_t>(q) * e.pdim);
+#pragma unroll
+    for (int j = 0; j < D; ++j) emit1<STAGE>(du, sa, x0 + j, dx[j]);
 }
 
 // ---- Jacobian-vector product: the RHS evaluated on dual numbers u + eps*v -------------
@@ -507,7 +534,7 @@ __device__ __forceinline__ Dual slot_value_dual(int2 s, Dual own, const double* 
     if constexpr (EM == ND_EM_NOOP) {
         return dual_nan();
     } else if constexpr (is_ode_model(EM)) {
-        return Dual{u[s.x], v[s.x]};
+        return ode_side_rt(WRAP, Dual{u[s.x], v[s.x]}, s.y & 1);
     } else {
         const Dual nbr{vo[s.x], dvo[s.x]};
         const double* pe = p + (s.y >> 1);
@@ -523,7 +550,7 @@ __device__ __forceinline__ Dual slot_value_dual_rt(const EDesc& e, int2 s, Dual 
                                                    const double* __restrict__ v, const double* __restrict__ vo,
                                                    const double* __restrict__ dvo, const double* __restrict__ p) {
     if (e.model == ND_EM_NOOP) return dual_nan();
-    if (e.ode) return Dual{u[s.x], v[s.x]};
+    if (e.ode) return ode_side_rt(e.wrap, Dual{u[s.x], v[s.x]}, s.y & 1);
     const Dual nbr{vo[s.x], dvo[s.x]};
     const double* pe = p + (s.y >> 1);
     if (s.y & 1) {
@@ -570,28 +597,26 @@ template <int EM>
 __global__ void __launch_bounds__(256) k_edge_ode_jvp(EDesc e, const double* __restrict__ u, const double* __restrict__ v,
                                                       const double* __restrict__ p, double* __restrict__ du,
                                                       double* __restrict__ jv) {
+    constexpr int D = ode_dim(EM);
+    constexpr int NX = edge_ext_inputs(EM) > 0 ? edge_ext_inputs(EM) : 1;
     const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= e.count) return;
     const int2 en = e.ends[q];
     const Dual vs{u[en.x], v[en.x]}, vdn{u[en.y], v[en.y]};
-    const int64_t x0 = static_cast<int64_t>(e.state0) + static_cast<int64_t>(q) * 2;
-    const Dual x[2] = {Dual{u[x0], v[x0]}, Dual{u[x0 + 1], v[x0 + 1]}};
-    Dual dx[2];
-    if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION) {            // ComponentLibrary.jl:30-34
-        const double* pe = p + e.param0 + static_cast<int64_t>(q) * e.pdim;
-        dx[0] = 1.0 / pe[0] * (sin(vs - vdn) - x[0]);
-        dx[1] = 1.0 / pe[0] * (sin(vdn - vs) - x[1]);
-    } else if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION_FAST) { // benchmark_models.jl:11-15
-        dx[0] = 100.0 * (sin(vs - vdn) - x[0]);
-        dx[1] = 100.0 * (sin(vdn - vs) - x[1]);
-    } else {                                                   // diffusion_test.jl:102-106
-        dx[0] = vs - vdn - x[0];
-        dx[1] = vdn - vs - x[1];
+    const int64_t x0 = static_cast<int64_t>(e.state0) + static_cast<int64_t>(q) * D;
+    Dual x[D], ext[NX], dx[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) x[j] = Dual{u[x0 + j], v[x0 + j]};
+#pragma unroll
+    for (int j = 0; j < NX; ++j)
+        ext[j] = j < edge_ext_inputs(EM) ? ext_read_dual(e.xmap, e.ext0 + static_cast<int64_t>(q) * e.extdim + j, u, v)
+                                         : Dual{0.0, 0.0};
+    edge_ode_f<EM, Dual>(dx, x, vs, vdn, ext, p + e.param0 + static_cast<int64_t>(q) * e.pdim);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        du[x0 + j] = dx[j].v;
+        jv[x0 + j] = dx[j].d;
     }
-    du[x0] = dx[0].v;
-    du[x0 + 1] = dx[1].v;
-    jv[x0] = dx[0].d;
-    jv[x0 + 1] = dx[1].d;
 }
 
 __global__ void k_fill(double* __restrict__ x, int64_t n, double v) {
@@ -738,7 +763,7 @@ __global__ void k_edge_out(EDesc e, const double* __restrict__ u, const double* 
     const int64_t o0 = static_cast<int64_t>(e.out0) + static_cast<int64_t>(q) * (e.outdim_src + e.outdim_dst);
     if (e.ode) {
         const int64_t x0 = static_cast<int64_t>(e.state0) + static_cast<int64_t>(q) * e.dim;
-        for (int j = 0; j < e.outdim_src; ++j) o[o0 + j] = u[x0 + e.mask_src[j]];
+        for (int j = 0; j < e.outdim_src; ++j) o[o0 + j] = ode_side_rt(e.wrap, u[x0 + e.mask_src[j]], true);
         for (int j = 0; j < e.outdim_dst; ++j) o[o0 + e.outdim_src + j] = u[x0 + e.mask_dst[j]];
         return;
     }
@@ -829,7 +854,7 @@ __device__ __forceinline__ void gen_finish(const VDesc& vd, int32_t q, double ac
     double ext[NX];
 #pragma unroll
     for (int j = 0; j < NX; ++j)
-        ext[j] = j < VExt<VM>::n ? u[vd.xmap[vd.ext0 + static_cast<int64_t>(q) * vd.extdim + j]] : 0.0;
+        ext[j] = j < VExt<VM>::n ? ext_read(vd.xmap, vd.ext0 + static_cast<int64_t>(q) * vd.extdim + j, u) : 0.0;
     double dx[DIMC];
     vertex_f_ext<VM>(dx, x, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, ext, vd.t, DIMC);
 #pragma unroll
@@ -926,10 +951,7 @@ __global__ void __launch_bounds__(256) k_vertex_gen_jvp(VDesc vd, const int32_t*
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
         ext[j] = Dual{0.0, 0.0};
-        if (j < VExt<VM>::n) {
-            const int32_t xi = vd.xmap[vd.ext0 + static_cast<int64_t>(q) * vd.extdim + j];
-            ext[j] = Dual{u[xi], v[xi]};
-        }
+        if (j < VExt<VM>::n) ext[j] = ext_read_dual(vd.xmap, vd.ext0 + static_cast<int64_t>(q) * vd.extdim + j, u, v);
     }
     Dual dx[DIMC];
     vertex_f_ext<VM, Dual>(dx, x, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, ext, vd.t, DIMC);
@@ -983,7 +1005,10 @@ VertexKernel pick_edge(int em, int wrap, bool multi, bool kc) {
     case ND_EM_ODE_SIN_DIFFUSION:
     case ND_EM_ODE_SIN_DIFFUSION_FAST:
     case ND_EM_ODE_LINEAR:
-        return pick3<VM, ND_EM_ODE_LINEAR, ND_WRAP_FIDUCIAL, STAGE>(false);
+    case ND_EM_ODE_LAG:
+    case ND_EM_EXT_DRIVEN:   // ODE edges: entries read the edge state, negated on an AntiSymmetric src side
+        return wrap == ND_WRAP_ANTISYMMETRIC ? pick3<VM, ND_EM_ODE_LINEAR, ND_WRAP_ANTISYMMETRIC, STAGE>(false)
+                                             : pick3<VM, ND_EM_ODE_LINEAR, ND_WRAP_FIDUCIAL, STAGE>(false);
     default:
         return pick3<VM, ND_EM_NOOP, ND_WRAP_NONE, STAGE>(false);
     }
@@ -1034,7 +1059,10 @@ GroupKernel pick_group_kernel(int em, int wrap, bool multi, bool kc) {
     case ND_EM_ODE_SIN_DIFFUSION:
     case ND_EM_ODE_SIN_DIFFUSION_FAST:
     case ND_EM_ODE_LINEAR:
-        return gpick3<ND_EM_ODE_LINEAR, ND_WRAP_FIDUCIAL, STAGE>(false);
+    case ND_EM_ODE_LAG:
+    case ND_EM_EXT_DRIVEN:
+        return wrap == ND_WRAP_ANTISYMMETRIC ? gpick3<ND_EM_ODE_LINEAR, ND_WRAP_ANTISYMMETRIC, STAGE>(false)
+                                             : gpick3<ND_EM_ODE_LINEAR, ND_WRAP_FIDUCIAL, STAGE>(false);
     default:
         return gpick3<ND_EM_NOOP, ND_WRAP_NONE, STAGE>(false);
     }
@@ -1084,7 +1112,10 @@ JvpKernel pick_jvp_edge(int em, int wrap, bool multi) {
     case ND_EM_ODE_SIN_DIFFUSION:
     case ND_EM_ODE_SIN_DIFFUSION_FAST:
     case ND_EM_ODE_LINEAR:
-        return k_vertex_jvp<VM, ND_EM_ODE_LINEAR, ND_WRAP_FIDUCIAL, false>;
+    case ND_EM_ODE_LAG:
+    case ND_EM_EXT_DRIVEN:
+        return wrap == ND_WRAP_ANTISYMMETRIC ? k_vertex_jvp<VM, ND_EM_ODE_LINEAR, ND_WRAP_ANTISYMMETRIC, false>
+                                             : k_vertex_jvp<VM, ND_EM_ODE_LINEAR, ND_WRAP_FIDUCIAL, false>;
     default:
         return k_vertex_jvp<VM, ND_EM_NOOP, ND_WRAP_NONE, false>;
     }
@@ -1124,6 +1155,8 @@ EdgeOdeJvpKernel pick_edge_ode_jvp(int em) {
     switch (em) {
     case ND_EM_ODE_SIN_DIFFUSION: return k_edge_ode_jvp<ND_EM_ODE_SIN_DIFFUSION>;
     case ND_EM_ODE_SIN_DIFFUSION_FAST: return k_edge_ode_jvp<ND_EM_ODE_SIN_DIFFUSION_FAST>;
+    case ND_EM_ODE_LAG: return k_edge_ode_jvp<ND_EM_ODE_LAG>;
+    case ND_EM_EXT_DRIVEN: return k_edge_ode_jvp<ND_EM_EXT_DRIVEN>;
     default: return k_edge_ode_jvp<ND_EM_ODE_LINEAR>;
     }
 }
@@ -1134,6 +1167,8 @@ EdgeOdeKernel pick_edge_ode(int em) {
     switch (em) {
     case ND_EM_ODE_SIN_DIFFUSION: return k_edge_ode<ND_EM_ODE_SIN_DIFFUSION, STAGE>;
     case ND_EM_ODE_SIN_DIFFUSION_FAST: return k_edge_ode<ND_EM_ODE_SIN_DIFFUSION_FAST, STAGE>;
+    case ND_EM_ODE_LAG: return k_edge_ode<ND_EM_ODE_LAG, STAGE>;
+    case ND_EM_EXT_DRIVEN: return k_edge_ode<ND_EM_EXT_DRIVEN, STAGE>;
     default: return k_edge_ode<ND_EM_ODE_LINEAR, STAGE>;
     }
 }
@@ -1165,7 +1200,7 @@ int vertex_model_ff(int vm) {
 int vertex_ext_inputs(int vm) { return vm == ND_VM_PI_SOURCE ? VExt<ND_VM_PI_SOURCE>::n : 0; }
 // vertex models only the one-thread-per-vertex kernels carry
 bool vertex_model_generic(int vm) { return vm >= ND_VM_CAPACITOR_SINK && vm <= ND_VM_CRL; }
-bool edge_model_known(int em) { return em >= ND_EM_NOOP && em <= ND_EM_LOOPBACK; }
+bool edge_model_known(int em) { return em >= ND_EM_NOOP && em <= ND_EM_EXT_DRIVEN; }
 bool edge_model_generic(int em) { return em == ND_EM_RESISTOR || em == ND_EM_LOOPBACK; }
 int edge_params(int em) {
     switch (em) {
@@ -1579,16 +1614,26 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         if (!edge_model_known(k.model)) return nd::set_error(ND_ERR_UNSUPPORTED, "edge model %d has no HIP kernel", k.model);
         if (k.model != ND_EM_NOOP) functional_edge = true;
         if (edge_model_generic(k.model)) generic_edges = true;
-        if (k.extdim > 0) return nd::set_error(ND_ERR_UNSUPPORTED, "external inputs of edge models have no HIP kernel");
+        if (k.extdim != edge_ext_inputs(k.model))
+            return nd::set_error(ND_ERR_ARGUMENT, "edge model %d reads %d external inputs, its kind declares %d", k.model,
+                                 edge_ext_inputs(k.model), k.extdim);
         if (k.model == ND_EM_LOOPBACK && (k.wrapper != ND_WRAP_DIRECTED || k.ff != ND_FF_PUREFEEDFORWARD))
             return nd::set_error(ND_ERR_ARGUMENT, "a LoopbackConnection is a Directed feed-forward edge");
-        if (k.ff == ND_FF_PURESTATEMAP && k.wrapper != ND_WRAP_FIDUCIAL)
-            return nd::set_error(ND_ERR_UNSUPPORTED, "StateMask edge outputs inside AntiSymmetric/Symmetric/Directed have no HIP kernel");
+        if (k.ff == ND_FF_PURESTATEMAP && k.wrapper != ND_WRAP_FIDUCIAL) {
+            // a wrapped StateMask: one mask, both sides (AntiSymmetric, Symmetric) or dst only
+            const int ns = k.wrapper == ND_WRAP_DIRECTED ? 0 : k.outdim_dst;
+            bool same = k.outdim_src == ns;
+            for (int j = 0; same && j < ns; ++j) same = k.mask_src[j] == k.mask_dst[j];
+            if (!same || (k.wrapper != ND_WRAP_ANTISYMMETRIC && k.wrapper != ND_WRAP_SYMMETRIC &&
+                          k.wrapper != ND_WRAP_DIRECTED))
+                return nd::set_error(ND_ERR_ARGUMENT, "a wrapped StateMask edge output has one mask for both sides");
+        }
         if (k.model == ND_EM_NOOP && k.dim > 0)
             return nd::set_error(ND_ERR_UNSUPPORTED, "edge kinds with states but no state function have no HIP kernel");
         if (is_ode_model(k.model)) {
-            if (k.ff != ND_FF_PURESTATEMAP || k.dim != 2 || k.wrapper != ND_WRAP_FIDUCIAL)
-                return nd::set_error(ND_ERR_UNSUPPORTED, "ODE edge model %d needs dim 2 and a Fiducial StateMask output", k.model);
+            if (k.ff != ND_FF_PURESTATEMAP || k.dim != ode_dim(k.model))
+                return nd::set_error(ND_ERR_UNSUPPORTED, "ODE edge model %d needs dim %d and a StateMask output", k.model,
+                                     ode_dim(k.model));
         } else if (k.model != ND_EM_NOOP) {
             if (k.ff != ND_FF_PUREFEEDFORWARD)
                 return nd::set_error(ND_ERR_UNSUPPORTED, "edge model %d needs a pure feed-forward output", k.model);
@@ -1632,6 +1677,8 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         e.out0 = static_cast<int32_t>(B.out_first - 1);
         e.outdim_src = k.outdim_src; e.outdim_dst = k.outdim_dst;
         for (int j = 0; j < ND_MAX_OUT; ++j) { e.mask_src[j] = k.mask_src[j] - 1; e.mask_dst[j] = k.mask_dst[j] - 1; }
+        e.ext0 = static_cast<int32_t>(B.ext_first - 1);
+        e.extdim = k.extdim;
         if (n.vdepth == 1) {
             // endpoint outputs as u-indices (StateMask outputs), or as o-indices where
             // outputs are computed (get_buffers then reads them from o)
@@ -1685,6 +1732,7 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
                 int2 sl = make_int2(0, 0);
                 if (is_ode_model(kk.model)) {
                     sl.x = static_cast<int32_t>(n.e_state_first(e) - 1 + (is_src ? kk.mask_src[j] : kk.mask_dst[j]) - 1);
+                    sl.y = is_src ? 1 : 0;
                 } else if (kk.model != ND_EM_NOOP) {
                     const int64_t other = is_src ? n.dst[e] - 1 : n.src[e] - 1;
                     sl.x = static_cast<int32_t>(n.v_out_first(other) - 1);
@@ -1805,13 +1853,26 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         std::vector<int32_t> xmap(static_cast<size_t>(n.lastidx_extbuf)), emap(static_cast<size_t>(n.lastidx_extbuf));
         for (int64_t k = 0; k < n.lastidx_extbuf; ++k) {
             const int64_t m = n.extmap[k];
-            xmap[k] = static_cast<int32_t>(m > 0 ? m - 1 : nd::output_state_index(n, -m));
+            bool neg = false;
+            const int64_t xi = m > 0 ? m - 1 : nd::output_state_index(n, -m, &neg);
+            xmap[k] = neg ? ~static_cast<int32_t>(xi) : static_cast<int32_t>(xi);
             emap[k] = static_cast<int32_t>(m);
         }
         rc = upload(pl, xmap, &pl->d_xmap);
         rc = rc ? rc : upload(pl, emap, &pl->d_extmap);
         if (rc) { delete pl; return rc; }
         for (VDesc& vd : pl->vdesc) vd.xmap = pl->d_xmap;
+        bool edge_ext = false;
+        for (EDesc& e : pl->edesc) {
+            e.xmap = pl->d_xmap;
+            edge_ext = edge_ext || e.extdim > 0;
+        }
+        if (edge_ext) {                          // the device copy of the edge descriptors too
+            release(pl, pl->d_edesc);
+            pl->d_edesc = nullptr;
+            rc = upload(pl, pl->edesc, &pl->d_edesc);
+            if (rc) { delete pl; return rc; }
+        }
     }
     if (n.vdepth == 1 && n.ne > 0) {
         void* w = nullptr;

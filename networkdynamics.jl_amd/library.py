@@ -24,6 +24,20 @@ def diffusion_odeedge():
                        g=c.Fiducial(dst=1, src=2), name="diff_edge_ode")
 
 
+# One-state dynamic line: de = 1/tau * (sin(v_s - v_d) - e), the dst half of
+# test/ComponentLibrary.jl:30-34, its output a wrapped StateMask (component_functions.jl:117-175):
+# wrapper "anti" (y_src = -e, the default), "sym" (y_src = e) or "directed" (no src output).
+def ode_lag_edge(wrapper="anti", tau=1.0, name="lag_line"):
+    w = dict(anti=c.AntiSymmetric, sym=c.Symmetric, directed=c.Directed)[wrapper]
+    return c.EdgeModel(f=c.ode_lag_f, sym=["e"], psym=[("tau", tau)], g=w(1), name=name)
+
+
+# test/external_inputs_test.jl:45-53: de[1] = ext[1], output AntiSymmetric(1)
+def ext_driven_edge(extin, wrapper="anti", name="e_noff"):
+    w = dict(anti=c.AntiSymmetric, sym=c.Symmetric, directed=c.Directed)[wrapper]
+    return c.EdgeModel(f=c.feext, sym=["a"], g=w(1), extin=extin, name=name)
+
+
 # test/ComponentLibrary.jl:46
 def diffusion_vertex():
     return c.VertexModel(f=c.diffusionvertex, dim=1, g=1, name="diffusion_vertex")

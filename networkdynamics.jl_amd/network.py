@@ -58,6 +58,21 @@ def _as_list(models, n, name):
     return models, False
 
 
+def _edge_outsym_flat(m):
+    """An edge's output symbols, src side then dst side (outsym_flat with the naming of
+    _symvec_to_sym_tup, component_functions.jl:1040-1075): a single-sided wrapper over
+    names s gives AntiSymmetric src "₋s" / dst s, Symmetric s / s and Directed dst s
+    only, s being the outsym given or, for a StateMask, the masked state names.  For
+    other edges the outsym list names the dst-side outputs."""
+    w = getattr(m, "wrapper", None)
+    if w in (A.ND_WRAP_ANTISYMMETRIC, A.ND_WRAP_SYMMETRIC, A.ND_WRAP_DIRECTED):
+        base = list(m.outsym) if m.outsym else [m.sym[j - 1] for j in m.mask_dst]
+        if w == A.ND_WRAP_DIRECTED:
+            return base
+        return (["₋" + s for s in base] if w == A.ND_WRAP_ANTISYMMETRIC else list(base)) + base
+    return [None] * m.outdim_src + list(m.outsym or [])
+
+
 def _tensor_ptr(x, n, what, torch):
     if not isinstance(x, torch.Tensor):
         raise A.ArgumentError(f"{what} must be a torch tensor on the HIP device")
@@ -247,8 +262,9 @@ class Network:
                 return int(first + int(ref.sym) - 1)
         elif ref.sym in m.sym:
             return int(first + m.sym.index(ref.sym))
-        elif ref.kind == "E" and m.outsym and ref.sym in m.outsym:
-            return -int(e[i, 2, 0] + m.outsym.index(ref.sym))
+        elif ref.kind == "E" and ref.sym in _edge_outsym_flat(m):
+            # the edge's outputs: src side, then dst side from e[i, 2, 0]
+            return -int(e[i, 2, 0] - m.outdim_src + _edge_outsym_flat(m).index(ref.sym))
         raise A.ArgumentError(f"Cannot resolve external input {ref!r}! External inputs musst be states or "
                               "outputs of non-feed-forward components.")
 

@@ -469,9 +469,15 @@ static void edge_g(const nd_kind* k, double* osrc, double* odst, const double* u
                    const double* vd, const double* p) {
     int32_t ns = k->outdim_src, nd = k->outdim_dst;
     if (k->model == ND_EM_NOOP) return;
-    if (k->ff == ND_FF_PURESTATEMAP) {   /* Fiducial(StateMask src, StateMask dst) */
-        for (int j = 0; j < ns; ++j) osrc[j] = ue[k->mask_src[j] - 1];
+    if (k->ff == ND_FF_PURESTATEMAP) {
+        /* Fiducial(StateMask src, StateMask dst): each side its own mask; a StateMask
+           inside AntiSymmetric / Symmetric / Directed: y_dst = u[mask], then
+           y_src = -y_dst / y_dst / nothing (component_functions.jl:117-175) */
         for (int j = 0; j < nd; ++j) odst[j] = ue[k->mask_dst[j] - 1];
+        if (k->wrapper == ND_WRAP_ANTISYMMETRIC)
+            for (int j = 0; j < ns; ++j) osrc[j] = -odst[j];
+        else
+            for (int j = 0; j < ns; ++j) osrc[j] = ue[k->mask_src[j] - 1];
         return;
     }
     if (k->model == ND_EM_DIFFUSION_FIDUCIAL) {   /* ComponentLibrary.jl:22-25 */
@@ -488,8 +494,9 @@ static void edge_g(const nd_kind* k, double* osrc, double* odst, const double* u
     }
 }
 
-/* edge f for ODE edges */
-static void edge_f(const nd_kind* k, double* de, const double* e, const double* vs, const double* vd, const double* p) {
+/* edge f for ODE edges; ext: the edge's external inputs (external_inputs.jl) */
+static void edge_f(const nd_kind* k, double* de, const double* e, const double* vs, const double* vd, const double* ext,
+                   const double* p) {
     switch (k->model) {
     case ND_EM_ODE_SIN_DIFFUSION:      /* ComponentLibrary.jl:30-34 */
         de[0] = 1.0 / p[0] * (sin(vs[0] - vd[0]) - e[0]);
@@ -503,13 +510,20 @@ static void edge_f(const nd_kind* k, double* de, const double* e, const double* 
         de[0] = vs[0] - vd[0] - e[0];
         de[1] = vd[0] - vs[0] - e[1];
         break;
+    case ND_EM_ODE_LAG:                /* ComponentLibrary.jl:30-32 with the state of the dst side only */
+        de[0] = 1.0 / p[0] * (sin(vs[0] - vd[0]) - e[0]);
+        break;
+    case ND_EM_EXT_DRIVEN:             /* external_inputs_test.jl:45-47: de[1] = ext[1] */
+        de[0] = ext[0];
+        break;
     default:
         break;
     }
 }
 
 static int edge_has_f(const nd_kind* k) {
-    return k->model == ND_EM_ODE_SIN_DIFFUSION || k->model == ND_EM_ODE_SIN_DIFFUSION_FAST || k->model == ND_EM_ODE_LINEAR;
+    return k->model == ND_EM_ODE_SIN_DIFFUSION || k->model == ND_EM_ODE_SIN_DIFFUSION_FAST ||
+           k->model == ND_EM_ODE_LINEAR || k->model == ND_EM_ODE_LAG || k->model == ND_EM_EXT_DRIVEN;
 }
 static int vertex_has_f(const nd_kind* k) { (void)k; return 1; }
 
@@ -553,7 +567,9 @@ static void apply_edge(const ndo_net* nw, const batch_t* B, int64_t i, int evalf
         vs = o + nw->v_out[nw->src[e] - 1].first - 1;
         vd = o + nw->v_out[nw->dst[e] - 1].first - 1;
     }
-    if (evalf && edge_has_f(k)) edge_f(k, du + sr.first - 1, u + sr.first - 1, vs, vd, p + pr.first - 1);
+    if (evalf && edge_has_f(k))
+        edge_f(k, du + sr.first - 1, u + sr.first - 1, vs, vd, nw->extbuf + nw->e_ext[B->idx[i - 1] - 1].first - 1,
+               p + pr.first - 1);
     if (evalg) {
         rng_t os = bs_subrange(&B->out, i, 1), od = bs_subrange(&B->out, i, 2);
         edge_g(k, o + os.first - 1, o + od.first - 1, u + sr.first - 1, vs, vd, p + pr.first - 1);
@@ -788,10 +804,15 @@ void ndo_rhs_scale(ndo_net* nw, double* S, const double* u, const double* p, dou
             double vd = u[nw->v_data[nw->dst[e] - 1].first - 1 + nw->kinds[nw->vkind[nw->dst[e] - 1]].mask_dst[0] - 1];
             double* s = S + sr.first - 1;
             const double* x = u + sr.first - 1;
-            double c = k->model == ND_EM_ODE_SIN_DIFFUSION ? fabs(1.0 / p[pr.first - 1]) : (k->model == ND_EM_ODE_SIN_DIFFUSION_FAST ? 100.0 : 1.0);
+            if (k->model == ND_EM_EXT_DRIVEN) {
+                s[0] = fabs(nw->extbuf[nw->e_ext[e].first - 1]);
+                continue;
+            }
+            double c = (k->model == ND_EM_ODE_SIN_DIFFUSION || k->model == ND_EM_ODE_LAG)
+                           ? fabs(1.0 / p[pr.first - 1])
+                           : (k->model == ND_EM_ODE_SIN_DIFFUSION_FAST ? 100.0 : 1.0);
             double dv = k->model == ND_EM_ODE_LINEAR ? fabs(vs) + fabs(vd) : fabs(sin(vs - vd));
-            s[0] = c * (dv + fabs(x[0]));
-            s[1] = c * (dv + fabs(x[1]));
+            for (int j = 0; j < k->dim; ++j) s[j] = c * (dv + fabs(x[j]));
         }
     }
 }
