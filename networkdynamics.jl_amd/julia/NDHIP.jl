@@ -75,6 +75,13 @@ function _kind(c::EdgeModel)
                       _mask(g.src), _mask(g.dst), extdim(c))
     end
     wrap = g isa AntiSymmetric ? 1 : g isa Symmetric ? 2 : g isa Directed ? 3 : 0
+    if wrap != 0 && g.g isa StateMask
+        # an ODE edge whose output is a StateMask inside the wrapper (component_functions.jl:
+        # 117-175): the model is its f; the mask serves both sides, Directed has no src side
+        m = _mask(g.g)
+        return NDKind(1, get(MODEL_IDS, compf(c), -1), wrap, 0, dim(c), pdim(c), od.src, od.dst,
+                      wrap == 3 ? _nomask() : m, m, extdim(c))
+    end
     inner = wrap == 0 ? g : g.g
     NDKind(1, get(MODEL_IDS, inner, -1), wrap, 1, dim(c), pdim(c), od.src, od.dst, _nomask(), _nomask(),
            extdim(c))
