@@ -16,6 +16,8 @@
  *   nd_plan_create           Adapt.adapt_structure(to, nw)            ext/NetworkDynamicsCUDAExt.jl:16-45
  *   nd_rhs                   (nw::Network)(du, u, p, t)               src/coreloop.jl:1-102
  *   nd_get_buffers           get_buffers(nw, u, p, t)                 src/coreloop.jl:103-109
+ *   nd_rhs_perturbed_jvp     nw(du, u, p, t; perturb, perturb_maps)  src/coreloop.jl:1,49-87 (and its
+ *   nd_plan_set_perturbation   ForwardDiff pass in linearize_network, src/linear_analysis.jl:86-122)
  *   nd_gather                gather!(::EagerGBufProvider, gbuf, o)    src/gbufs.jl:25
  *   nd_aggregate             aggregate!(::Aggregator, aggbuf, data)   src/aggregators.jl:140-151 (additive)
  */
@@ -300,6 +302,31 @@ int nd_rk4(nd_plan* plan, double* u, const double* p, double t0, double dt, int6
  * output buffer o[lastidx_out] and the aggregation buffer aggbuf[lastidx_aggr]. */
 int nd_get_buffers(nd_plan* plan, double* o, double* aggbuf, const double* u,
                    const double* p, double t, void* stream);
+
+/* The perturbation maps of nw(du, u, p, t; perturb, perturb_maps), coreloop.jl:1,49-87
+ * (built by _classify_perturbation_channels, linear_analysis.jl:629-680): each map is
+ * n pairs (buffer index, channel index), both 1-based, flattened.  vo: output buffer
+ * (vertex outputs), vi: aggregation buffer (vertex inputs), eo: output buffer (edge
+ * outputs), ei: gather buffer (edge inputs).  ndelta: number of channels.  Replaces the
+ * plan's previous maps (zero lengths clear them).  Parameter channels are the caller's
+ * (added to p before the call, as linearize_network does). */
+int nd_plan_set_perturbation(nd_plan* plan, int64_t ndelta, int64_t nvo, const int64_t* vo, int64_t nvi,
+                             const int64_t* vi, int64_t neo, const int64_t* eo, int64_t nei, const int64_t* ei);
+
+/* nw(du, u, p, t; perturb=delta, perturb_maps) on first-order dual numbers
+ * u + eps*v, delta + eps*ddelta: the reference's buffered loop (coreloop.jl:1-102) with
+ * delta added at its four points -- after the loopback copy (vi), after the external
+ * inputs are collected (vo), after the gather (ei) and before the aggregation (eo).
+ * du, jv: f and its directional derivative, also along the parameter tangent dp (the
+ * models evaluate their parameters as dual numbers here).  v, dp, ddelta: NULL for
+ * zero tangents; delta:
+ * ndelta values (NULL allowed only without channels).  o, dout, aggbuf, dagg: NULL, or
+ * the caller's buffers [lastidx_out] / [lastidx_aggr] receiving the perturbed buffers
+ * and their tangents (the observables of linearize_network's output channels); jv may
+ * be NULL.  Stream-ordered; the plan's work buffers are shared by its calls. */
+int nd_rhs_perturbed_jvp(nd_plan* plan, double* du, double* jv, const double* u, const double* v, const double* p,
+                         const double* dp, double t, const double* delta, const double* ddelta, double* o,
+                         double* dout, double* aggbuf, double* dagg, void* stream);
 
 /* nw(du, u + eps*v, p, t) on first-order dual numbers: du = f(u) and jv = J(u) v, the
  * Jacobian-vector product ForwardDiff forms when linearize_network

@@ -99,6 +99,8 @@ PROTOTYPES = {
     "nd_get_buffers": (C.c_int, [P, P, P, P, P, D, P]),
     "nd_rk4": (C.c_int, [P, P, P, D, D, I64, P, I64, I64, P]),
     "nd_rhs_jvp": (C.c_int, [P, P, P, P, P, P, D, I64, I64, I64, P]),
+    "nd_plan_set_perturbation": (C.c_int, [P, I64, I64, P, I64, P, I64, P, I64, P]),
+    "nd_rhs_perturbed_jvp": (C.c_int, [P, P, P, P, P, P, P, D, P, P, P, P, P, P, P]),
     "nd_malloc": (C.c_int, [C.c_size_t, C.POINTER(P)]),
     "nd_free": (C.c_int, [P]),
     "nd_memcpy_h2d": (C.c_int, [P, P, C.c_size_t]),

@@ -148,6 +148,21 @@ class EIndex(VIndex):
     kind = "E"
 
 
+def _insym(insym, indim, is_edge):
+    """Input symbols (component_functions.jl:864-911): insym as given, or from indim the
+    names i / i1, i2, ...; an edge's plain list becomes src/dst names with the prefixes
+    "src₊" / "dst₊".  None: no input symbols (inputs cannot be named)."""
+    if insym is None and indim is not None:
+        insym = ["i"] if indim == 1 else [f"i{k + 1}" for k in range(indim)]
+    if insym is None:
+        return None
+    if is_edge:
+        if isinstance(insym, dict):
+            return dict(src=[str(x) for x in insym.get("src", [])], dst=[str(x) for x in insym.get("dst", [])])
+        return dict(src=["src₊" + str(x) for x in insym], dst=["dst₊" + str(x) for x in insym])
+    return [str(x) for x in insym]
+
+
 def _extin(extin, f):
     if extin is None:
         refs = []
@@ -314,7 +329,8 @@ class VertexModel(ComponentModel):
     is_edge = False
 
     def __init__(self, f=None, g=None, dim=None, sym=None, pdim=None, psym=None, outdim=None, name=None,
-                 mass_matrix=None, extin=None):
+                 mass_matrix=None, extin=None, insym=None, indim=None):
+        self.insym = _insym(insym, indim, False)
         if isinstance(g, DeviceFunction) and g.role == "vertex_g":
             self._init_output_model(f, g, dim, sym, pdim, psym, outdim, name, mass_matrix, extin)
             return
@@ -407,7 +423,8 @@ class EdgeModel(ComponentModel):
     is_edge = True
 
     def __init__(self, f=None, g=None, dim=None, sym=None, pdim=None, psym=None, outdim=None, outsym=None,
-                 name=None, mass_matrix=None, extin=None):
+                 name=None, mass_matrix=None, extin=None, insym=None, indim=None):
+        self.insym = _insym(insym, indim, True)
         if g is None:
             raise A.ArgumentError("EdgeModel needs g")
         if f is not None and (not isinstance(f, DeviceFunction) or f.role != "edge_f"):

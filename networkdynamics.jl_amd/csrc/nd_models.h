@@ -11,7 +11,9 @@ namespace nd {
 using ::sin;   // the double overloads stay visible next to sin(Dual)
 
 // First-order dual number v + d*eps (one tangent direction), the forward-mode scalar
-// the Jacobian-vector product evaluates the models with.  Parameters stay plain doubles.
+// the Jacobian-vector product evaluates the models with.  The model functions take the
+// parameters as plain doubles, or as Duals where a derivative in a parameter is wanted
+// (the parameter channels of linearize_network).
 struct Dual {
     double v, d;
 };
@@ -26,6 +28,8 @@ __device__ __forceinline__ Dual operator*(double c, Dual a) { return {c * a.v, c
 __device__ __forceinline__ Dual operator*(Dual a, double c) { return {a.v * c, a.d * c}; }
 __device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
 __device__ __forceinline__ Dual operator/(Dual a, double c) { return {a.v / c, a.d / c}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+__device__ __forceinline__ Dual operator/(double c, Dual b) { return {c / b.v, -c * b.d / (b.v * b.v)}; }
 __device__ __forceinline__ Dual sin(Dual a) { return {::sin(a.v), ::cos(a.v) * a.d}; }
 __device__ __forceinline__ Dual dual_nan() { return {__builtin_nan(""), __builtin_nan("")}; }
 
@@ -48,8 +52,8 @@ __host__ __device__ constexpr int edge_ext_inputs(int em) { return em == ND_EM_E
 
 // f of an ODE edge: dx from its state x, the endpoint outputs and its external inputs;
 // S = double or Dual
-template <int EM, typename S>
-__device__ __forceinline__ void edge_ode_f(S* dx, const S* x, S vs, S vd, const S* ext, const double* __restrict__ pe) {
+template <int EM, typename S, typename P = double>
+__device__ __forceinline__ void edge_ode_f(S* dx, const S* x, S vs, S vd, const S* ext, const P* __restrict__ pe) {
     if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION) {            // ComponentLibrary.jl:30-34
         dx[0] = 1.0 / pe[0] * (sin(vs - vd) - x[0]);
         dx[1] = 1.0 / pe[0] * (sin(vd - vs) - x[1]);
@@ -65,10 +69,21 @@ __device__ __forceinline__ void edge_ode_f(S* dx, const S* x, S vs, S vd, const 
         dx[1] = vd - vs - x[1];
     }
 }
+template <typename S, typename P = double>
+__device__ __forceinline__ void edge_ode_f_rt(int em, S* dx, const S* x, S vs, S vd, const S* ext,
+                                              const P* __restrict__ pe) {
+    switch (em) {
+    case ND_EM_ODE_SIN_DIFFUSION: edge_ode_f<ND_EM_ODE_SIN_DIFFUSION, S, P>(dx, x, vs, vd, ext, pe); break;
+    case ND_EM_ODE_SIN_DIFFUSION_FAST: edge_ode_f<ND_EM_ODE_SIN_DIFFUSION_FAST, S, P>(dx, x, vs, vd, ext, pe); break;
+    case ND_EM_ODE_LAG: edge_ode_f<ND_EM_ODE_LAG, S, P>(dx, x, vs, vd, ext, pe); break;
+    case ND_EM_EXT_DRIVEN: edge_ode_f<ND_EM_EXT_DRIVEN, S, P>(dx, x, vs, vd, ext, pe); break;
+    default: edge_ode_f<ND_EM_ODE_LINEAR, S, P>(dx, x, vs, vd, ext, pe); break;
+    }
+}
 
 // dst-side output of a pure feed-forward edge model g(v_s, v_d, p); S = double or Dual
-template <int EM, typename S = double>
-__device__ __forceinline__ S g_dst(S vs, S vd, const double* __restrict__ pe) {
+template <int EM, typename S = double, typename P = double>
+__device__ __forceinline__ S g_dst(S vs, S vd, const P* __restrict__ pe) {
     if constexpr (EM == ND_EM_KURAMOTO) return pe[0] * sin(vs - vd);             // ComponentLibrary.jl:51-53
     else if constexpr (EM == ND_EM_STATIC_LINE) return pe[0] * sin(vs - vd - pe[1]);
     else if constexpr (EM == ND_EM_DIFFUSION) return vs - vd;                     // diffusion_test.jl:57-60
@@ -79,16 +94,16 @@ __device__ __forceinline__ S g_dst(S vs, S vd, const double* __restrict__ pe) {
     else return nan_of<S>();
 }
 
-template <typename S = double>
-__device__ __forceinline__ S g_dst_rt(int em, S vs, S vd, const double* __restrict__ pe) {
+template <typename S = double, typename P = double>
+__device__ __forceinline__ S g_dst_rt(int em, S vs, S vd, const P* __restrict__ pe) {
     switch (em) {
-    case ND_EM_KURAMOTO: return g_dst<ND_EM_KURAMOTO, S>(vs, vd, pe);
-    case ND_EM_STATIC_LINE: return g_dst<ND_EM_STATIC_LINE, S>(vs, vd, pe);
-    case ND_EM_DIFFUSION: return g_dst<ND_EM_DIFFUSION, S>(vs, vd, pe);
-    case ND_EM_DIFFUSION_WEIGHTED: return g_dst<ND_EM_DIFFUSION_WEIGHTED, S>(vs, vd, pe);
-    case ND_EM_DIFFUSION_FIDUCIAL: return g_dst<ND_EM_DIFFUSION_FIDUCIAL, S>(vs, vd, pe);
-    case ND_EM_RESISTOR: return g_dst<ND_EM_RESISTOR, S>(vs, vd, pe);
-    case ND_EM_LOOPBACK: return g_dst<ND_EM_LOOPBACK, S>(vs, vd, pe);
+    case ND_EM_KURAMOTO: return g_dst<ND_EM_KURAMOTO, S, P>(vs, vd, pe);
+    case ND_EM_STATIC_LINE: return g_dst<ND_EM_STATIC_LINE, S, P>(vs, vd, pe);
+    case ND_EM_DIFFUSION: return g_dst<ND_EM_DIFFUSION, S, P>(vs, vd, pe);
+    case ND_EM_DIFFUSION_WEIGHTED: return g_dst<ND_EM_DIFFUSION_WEIGHTED, S, P>(vs, vd, pe);
+    case ND_EM_DIFFUSION_FIDUCIAL: return g_dst<ND_EM_DIFFUSION_FIDUCIAL, S, P>(vs, vd, pe);
+    case ND_EM_RESISTOR: return g_dst<ND_EM_RESISTOR, S, P>(vs, vd, pe);
+    case ND_EM_LOOPBACK: return g_dst<ND_EM_LOOPBACK, S, P>(vs, vd, pe);
     default: return nan_of<S>();
     }
 }
@@ -109,9 +124,9 @@ template <> struct VModel<ND_VM_INDUCTOR_INJECTOR> { static constexpr int dim = 
 template <> struct VModel<ND_VM_CRL> { static constexpr int dim = 2; };
 
 // vertex f (apply_compf, coreloop.jl:213-216); operation order as the reference writes it
-template <int VM, typename S = double>
+template <int VM, typename S = double, typename P = double>
 __device__ __forceinline__ void vertex_f(S* __restrict__ dx, const S* __restrict__ x, S acc,
-                                         const double* __restrict__ pv, int dim) {
+                                         const P* __restrict__ pv, int dim) {
     if constexpr (VM == ND_VM_DIFFUSION) {                 // ComponentLibrary.jl:42-45
         dx[0] = acc;
     } else if constexpr (VM == ND_VM_KURAMOTO) {           // ComponentLibrary.jl:69-71
@@ -136,9 +151,9 @@ __device__ __forceinline__ void vertex_f(S* __restrict__ dx, const S* __restrict
 // Vertex f with the arguments the fused kernels do not carry: external inputs ext
 // (external_inputs.jl; apply_compf passes them after the aggregated input) and the time
 // t.  The models without either reduce to vertex_f.
-template <int VM, typename S = double>
+template <int VM, typename S = double, typename P = double>
 __device__ __forceinline__ void vertex_f_ext(S* __restrict__ dx, const S* __restrict__ x, S acc,
-                                             const double* __restrict__ pv, const S* __restrict__ ext, double t,
+                                             const P* __restrict__ pv, const S* __restrict__ ext, double t,
                                              int dim) {
     if constexpr (VM == ND_VM_CAPACITOR_SINK) {            // external_inputs.md:76-79
         dx[0] = 1.0 / pv[0] * (acc - (1.0 + 0.1 * sin(t)));
@@ -156,19 +171,19 @@ __device__ __forceinline__ void vertex_f_ext(S* __restrict__ dx, const S* __rest
         dx[0] = (acc - v / pv[1] - x[1]) / pv[0];          // cap.p.i = i - v/R - iL
         dx[1] = v / pv[2];
     } else {
-        vertex_f<VM, S>(dx, x, acc, pv, dim);
+        vertex_f<VM, S, P>(dx, x, acc, pv, dim);
     }
 }
 
 // Computed vertex outputs: NoFeedForward g(out, x, p, t) and the PureFeedForward g of an
 // injector, g(out, in, p, t), whose input is its hub's output (the loopback copy).
-template <typename S>
-__device__ __forceinline__ S vertex_out_nff(int vm, const double* __restrict__ pv) {
+template <typename S, typename P = double>
+__device__ __forceinline__ S vertex_out_nff(int vm, const P* __restrict__ pv) {
     if (vm == ND_VM_VOLTAGE_SOURCE) return S{} + pv[0];    // loopback_test.jl:31: p.v ~ V
     return nan_of<S>();
 }
-template <typename S>
-__device__ __forceinline__ S vertex_out_ff(int vm, S in, const double* __restrict__ pv) {
+template <typename S, typename P = double>
+__device__ __forceinline__ S vertex_out_ff(int vm, S in, const P* __restrict__ pv) {
     if (vm == ND_VM_RESISTOR_INJECTOR) return in / pv[0];   // loopback_test.jl:75: p.i ~ p.v / R
     return nan_of<S>();
 }

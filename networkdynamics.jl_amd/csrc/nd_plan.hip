@@ -971,6 +971,145 @@ __global__ void k_collect_externals(const int32_t* __restrict__ map, int64_t n, 
     }
 }
 
+// ---- perturbed right-hand side (coreloop.jl:1-102 with perturb / perturb_maps) ----------
+// The buffered form of the RHS on dual numbers: every phase writes the reference's
+// buffers (o, aggbuf, gbuf, extbuf) and their tangents, so the four additive
+// perturbations of coreloop.jl:49-87 enter where the reference adds them.  Linear phases
+// (gather, aggregation, loopback copy, external inputs) run the plain kernels on the
+// value and on the tangent buffers; the kernels below evaluate the models on Duals.
+
+// apply_perturb! (coreloop.jl:258-262): buf[m.x] += delta[m.y] in map order.  One thread:
+// a map holds a handful of channels, and the order stays the reference's.
+__global__ void k_perturb(const int2* __restrict__ map, int32_t n, const double* __restrict__ delta,
+                          const double* __restrict__ ddelta, double* __restrict__ buf, double* __restrict__ dbuf) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int32_t k = 0; k < n; ++k) {
+        const int2 m = map[k];
+        buf[m.x] += delta[m.y];
+        dbuf[m.x] += ddelta[m.y];
+    }
+}
+
+// One component's parameters p[first .. first+n) with their tangents dp as Duals (the
+// models read at most kMaxPar of them)
+constexpr int kMaxPar = 8;
+__device__ __forceinline__ void dual_params(Dual* pd, const double* __restrict__ p, const double* __restrict__ dp,
+                                            int64_t first, int n) {
+#pragma unroll
+    for (int c = 0; c < kMaxPar; ++c) pd[c] = c < n ? Dual{p[first + c], dp[first + c]} : Dual{0.0, 0.0};
+}
+
+// feed-forward vertex outputs g(in, p) from the (perturbed) input buffer (coreloop.jl:55)
+__global__ void __launch_bounds__(256) k_vertex_out_ff_buf(VDesc vd, const double* __restrict__ agg,
+                                                           const double* __restrict__ dagg,
+                                                           const double* __restrict__ p, const double* __restrict__ dp,
+                                                           double* __restrict__ o, double* __restrict__ dout) {
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= vd.count) return;
+    Dual pd[kMaxPar];
+    dual_params(pd, p, dp, vd.param0 + static_cast<int64_t>(q) * vd.pdim, vd.pdim);
+    const Dual y = vertex_out_ff<Dual, Dual>(vd.model, Dual{agg[vd.row0 + q], dagg[vd.row0 + q]}, pd);
+    o[static_cast<int64_t>(vd.out0) + q] = y.v;
+    dout[static_cast<int64_t>(vd.out0) + q] = y.d;
+}
+
+// NoFeedForward vertex outputs g(p) (voltage sources): their tangents in the parameters
+__global__ void __launch_bounds__(256) k_vertex_out_nff_tangent(VDesc vd, const double* __restrict__ p,
+                                                                const double* __restrict__ dp,
+                                                                double* __restrict__ dout) {
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= vd.count) return;
+    Dual pd[kMaxPar];
+    dual_params(pd, p, dp, vd.param0 + static_cast<int64_t>(q) * vd.pdim, vd.pdim);
+    dout[static_cast<int64_t>(vd.out0) + q] = vertex_out_nff<Dual, Dual>(vd.model, pd).d;
+}
+
+// edges from the gathered buffer (coreloop.jl:76-78): ODE edges' f into du, feed-forward
+// edges' g into o.  egb: first gbuf index of each edge of the batch (src output, then dst).
+__global__ void __launch_bounds__(256) k_edge_buf(EDesc e, const int32_t* __restrict__ egb,
+                                                  const double* __restrict__ gbuf, const double* __restrict__ dgbuf,
+                                                  const double* __restrict__ u, const double* __restrict__ v,
+                                                  const double* __restrict__ ext, const double* __restrict__ dext,
+                                                  const double* __restrict__ p, const double* __restrict__ dp,
+                                                  double* __restrict__ o, double* __restrict__ dout,
+                                                  double* __restrict__ du, double* __restrict__ jv) {
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= e.count || e.model == ND_EM_NOOP) return;
+    const int32_t g0 = egb[q];
+    const Dual vs{gbuf[g0], dgbuf[g0]}, vdn{gbuf[g0 + 1], dgbuf[g0 + 1]};
+    Dual pe[kMaxPar];
+    dual_params(pe, p, dp, e.param0 + static_cast<int64_t>(q) * e.pdim, e.pdim);
+    if (e.ode) {                                 // outputs: StateMasks, written before the gather
+        const int64_t x0 = static_cast<int64_t>(e.state0) + static_cast<int64_t>(q) * e.dim;
+        Dual x[2] = {Dual{0.0, 0.0}, Dual{0.0, 0.0}}, dx[2];
+        for (int j = 0; j < e.dim && j < 2; ++j) x[j] = Dual{u[x0 + j], v[x0 + j]};
+        Dual xe[1] = {Dual{0.0, 0.0}};
+        if (e.extdim > 0) {
+            const int64_t k = static_cast<int64_t>(e.ext0) + static_cast<int64_t>(q) * e.extdim;
+            xe[0] = Dual{ext[k], dext[k]};
+        }
+        edge_ode_f_rt<Dual, Dual>(e.model, dx, x, vs, vdn, xe, pe);
+        for (int j = 0; j < e.dim && j < 2; ++j) {
+            du[x0 + j] = dx[j].v;
+            jv[x0 + j] = dx[j].d;
+        }
+        return;
+    }
+    const int64_t o0 = static_cast<int64_t>(e.out0) + static_cast<int64_t>(q) * (e.outdim_src + e.outdim_dst);
+    const Dual y = g_dst_rt<Dual, Dual>(e.model, vs, vdn, pe);
+    o[o0 + e.outdim_src] = y.v;
+    dout[o0 + e.outdim_src] = y.d;
+    if (e.outdim_src > 0) {
+        const Dual ys = e.wrap == ND_WRAP_SYMMETRIC ? y : (e.wrap == ND_WRAP_DIRECTED ? Dual{0.0, 0.0} : -y);
+        o[o0] = ys.v;
+        dout[o0] = ys.d;
+    }
+}
+
+// vertex f from the aggregated (perturbed) inputs and the collected external inputs
+// (coreloop.jl:97)
+template <int VM>
+__global__ void __launch_bounds__(256) k_vertex_f_buf(VDesc vd, const double* __restrict__ agg,
+                                                      const double* __restrict__ dagg, const double* __restrict__ u,
+                                                      const double* __restrict__ v, const double* __restrict__ ext,
+                                                      const double* __restrict__ dext, const double* __restrict__ p,
+                                                      const double* __restrict__ dp, double* __restrict__ du,
+                                                      double* __restrict__ jv) {
+    constexpr int DIMC = VModel<VM>::dim;
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= vd.count) return;
+    if constexpr (DIMC <= 0) {                   // no f (NoOp): du stays zero
+        const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * vd.dim;
+        for (int j = 0; j < vd.dim; ++j) {
+            du[x0 + j] = 0.0;
+            jv[x0 + j] = 0.0;
+        }
+    } else {
+        constexpr int NX = VExt<VM>::n > 0 ? VExt<VM>::n : 1;
+        const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * DIMC;
+        Dual x[DIMC], xe[NX], dx[DIMC];
+#pragma unroll
+        for (int j = 0; j < DIMC; ++j) x[j] = Dual{u[x0 + j], v[x0 + j]};
+        const Dual acc = agg ? Dual{agg[vd.row0 + q], dagg[vd.row0 + q]} : Dual{0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < NX; ++j) {
+            xe[j] = Dual{0.0, 0.0};
+            if (j < VExt<VM>::n) {
+                const int64_t k = static_cast<int64_t>(vd.ext0) + static_cast<int64_t>(q) * vd.extdim + j;
+                xe[j] = Dual{ext[k], dext[k]};
+            }
+        }
+        Dual pd[kMaxPar];
+        dual_params(pd, p, dp, vd.param0 + static_cast<int64_t>(q) * vd.pdim, vd.pdim);
+        vertex_f_ext<VM, Dual, Dual>(dx, x, acc, pd, xe, vd.t, DIMC);
+#pragma unroll
+        for (int j = 0; j < DIMC; ++j) {
+            du[x0 + j] = dx[j].v;
+            jv[x0 + j] = dx[j].d;
+        }
+    }
+}
+
 // ---- kernel selection ----------------------------------------------------------
 
 using VertexKernel = void (*)(VDesc, const int32_t*, const int2*, const uint8_t*, const EDesc*, const double*,
@@ -1084,6 +1223,26 @@ VertexKernel pick_gen_kernel(int vm) {
     case ND_VM_INDUCTOR_INJECTOR: return k_vertex_gen<ND_VM_INDUCTOR_INJECTOR, STAGE>;
     case ND_VM_CRL: return k_vertex_gen<ND_VM_CRL, STAGE>;
     default: return k_vertex_rhs<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false, STAGE, false>;
+    }
+}
+
+using VertexBufKernel = void (*)(VDesc, const double*, const double*, const double*, const double*, const double*,
+                                 const double*, const double*, const double*, double*, double*);
+VertexBufKernel pick_vertex_f_buf(int vm) {
+    switch (vm) {
+    case ND_VM_DIFFUSION: return k_vertex_f_buf<ND_VM_DIFFUSION>;
+    case ND_VM_KURAMOTO: return k_vertex_f_buf<ND_VM_KURAMOTO>;
+    case ND_VM_SWING: return k_vertex_f_buf<ND_VM_SWING>;
+    case ND_VM_SWING_UNIT_DAMPING: return k_vertex_f_buf<ND_VM_SWING_UNIT_DAMPING>;
+    case ND_VM_LOAD: return k_vertex_f_buf<ND_VM_LOAD>;
+    case ND_VM_CAPACITOR_SINK: return k_vertex_f_buf<ND_VM_CAPACITOR_SINK>;
+    case ND_VM_PI_SOURCE: return k_vertex_f_buf<ND_VM_PI_SOURCE>;
+    case ND_VM_CAPACITOR: return k_vertex_f_buf<ND_VM_CAPACITOR>;
+    case ND_VM_VOLTAGE_SOURCE: return k_vertex_f_buf<ND_VM_VOLTAGE_SOURCE>;
+    case ND_VM_RESISTOR_INJECTOR: return k_vertex_f_buf<ND_VM_RESISTOR_INJECTOR>;
+    case ND_VM_INDUCTOR_INJECTOR: return k_vertex_f_buf<ND_VM_INDUCTOR_INJECTOR>;
+    case ND_VM_CRL: return k_vertex_f_buf<ND_VM_CRL>;
+    default: return k_vertex_f_buf<ND_VM_NOOP>;
     }
 }
 
@@ -1289,6 +1448,16 @@ struct nd_plan {
     // compact outputs, the injectors' after their hubs'
     bool computed_out = false, ff_out = false;
     int32_t* d_lbin = nullptr;
+    // perturbed RHS: first gbuf index of every edge (edge batches in order, batch order
+    // within), the four perturbation maps (buffer index, channel; 0-based) and work buffers
+    std::vector<int32_t> egb_host;
+    std::vector<int64_t> egb_off;
+    int32_t* d_egb = nullptr;
+    int2* d_pmap[4] = {nullptr, nullptr, nullptr, nullptr};   // vo, vi, eo, ei
+    int32_t n_pmap[4] = {0, 0, 0, 0};
+    int64_t ndelta = 0;
+    double* d_pwork = nullptr;
+    size_t pwork_len = 0;
 
     ~nd_plan() {
         if (small_en) nd_ensemble_destroy(small_en);
@@ -1746,6 +1915,10 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
             }
         }
     }
+    for (const nd::Batch& B : n.eb) {
+        pl->egb_off.push_back(static_cast<int64_t>(pl->egb_host.size()));
+        for (int64_t ei : B.indices) pl->egb_host.push_back(static_cast<int32_t>(n.e_gbuf_first(ei - 1) - 1));
+    }
     std::vector<int32_t> gmap(static_cast<size_t>(n.lastidx_gbuf));
     for (int64_t e = 0; e < n.ne; ++e) {
         const int64_t g = n.e_gbuf_first(e) - 1;
@@ -2158,6 +2331,175 @@ int nd_get_buffers(nd_plan* pl, double* o, double* aggbuf, const double* u, cons
     rc = hip_check(hipGetLastError(), "nd_get_buffers launch");
     if (rc) return rc;
     return nd_aggregate(pl, aggbuf, o, stream);
+}
+
+int nd_plan_set_perturbation(nd_plan* pl, int64_t ndelta, int64_t nvo, const int64_t* vo, int64_t nvi,
+                              const int64_t* vi, int64_t neo, const int64_t* eo, int64_t nei, const int64_t* ei) {
+    nd::clear_error();
+    if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
+    if (ndelta < 0 || nvo < 0 || nvi < 0 || neo < 0 || nei < 0) return nd::set_error(ND_ERR_ARGUMENT, "negative length");
+    int rc = check_device(pl);
+    if (rc) return rc;
+    const int64_t* maps[4] = {vo, vi, eo, ei};
+    const int64_t lens[4] = {nvo, nvi, neo, nei};
+    const int64_t bufs[4] = {pl->lastidx_out, pl->lastidx_aggr, pl->lastidx_out, pl->lastidx_gbuf};
+    const char* names[4] = {"vertex output", "vertex input", "edge output", "edge input"};
+    std::vector<int2> host[4];
+    for (int m = 0; m < 4; ++m) {
+        if (lens[m] > 0 && !maps[m]) return nd::set_error(ND_ERR_ARGUMENT, "%s map is NULL", names[m]);
+        for (int64_t k = 0; k < lens[m]; ++k) {
+            const int64_t b = maps[m][2 * k], d = maps[m][2 * k + 1];
+            if (b < 1 || b > bufs[m]) return nd::set_error(ND_ERR_ARGUMENT, "%s perturbation: buffer index %lld out of range", names[m], (long long)b);
+            if (d < 1 || d > ndelta) return nd::set_error(ND_ERR_ARGUMENT, "%s perturbation: channel %lld out of range", names[m], (long long)d);
+            host[m].push_back(make_int2(static_cast<int32_t>(b - 1), static_cast<int32_t>(d - 1)));
+        }
+    }
+    rc = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");   // earlier calls may still read the old maps
+    if (rc) return rc;
+    for (int m = 0; m < 4; ++m) {
+        release(pl, pl->d_pmap[m]);
+        pl->d_pmap[m] = nullptr;
+        rc = upload(pl, host[m], &pl->d_pmap[m]);
+        if (rc) return rc;
+        pl->n_pmap[m] = static_cast<int32_t>(host[m].size());
+    }
+    pl->ndelta = ndelta;
+    if (!pl->d_egb && !pl->egb_host.empty()) {
+        rc = upload(pl, pl->egb_host, &pl->d_egb);
+        if (rc) return rc;
+    }
+    return ND_OK;
+}
+
+int nd_rhs_perturbed_jvp(nd_plan* pl, double* du, double* jv, const double* u, const double* v, const double* p,
+                         const double* dp, double t, const double* delta, const double* ddelta, double* o, double* dout,
+                         double* aggbuf, double* dagg, void* stream) {
+    if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
+    if ((pl->dim > 0 && (!du || !u)) || (pl->ndelta > 0 && !delta))
+        return nd::set_error(ND_ERR_ARGUMENT, "NULL buffer");
+    int rc = check_device(pl);
+    if (rc) return rc;
+    if (!pl->d_egb && !pl->egb_host.empty()) {
+        rc = upload(pl, pl->egb_host, &pl->d_egb);
+        if (rc) return rc;
+    }
+    // work buffers: o, do, agg, dagg, gbuf, dgbuf, ext, dext, zero tangents, jv
+    const size_t no = static_cast<size_t>(pl->lastidx_out), na = static_cast<size_t>(pl->lastidx_aggr);
+    const size_t ng = static_cast<size_t>(pl->lastidx_gbuf), nx = static_cast<size_t>(pl->lastidx_extbuf);
+    const size_t nz = std::max<size_t>({static_cast<size_t>(pl->dim), static_cast<size_t>(pl->ndelta),
+                                        static_cast<size_t>(pl->pdim), size_t{1}});
+    const size_t need = 2 * (no + na + ng + nx) + nz + static_cast<size_t>(pl->dim) + 8;
+    if (pl->pwork_len < need) {
+        rc = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        if (rc) return rc;
+        release(pl, pl->d_pwork);
+        pl->d_pwork = nullptr;
+        void* w = nullptr;
+        hipError_t e = hipMalloc(&w, need * sizeof(double));
+        if (e != hipSuccess) return nd::set_error(ND_ERR_ALLOC, "hipMalloc(perturbed RHS buffers): %s", hipGetErrorString(e));
+        pl->allocations.push_back(w);
+        pl->d_pwork = static_cast<double*>(w);
+        pl->pwork_len = need;
+    }
+    double* w = pl->d_pwork;
+    double* wo = w;            w += no;
+    double* wdo = w;           w += no;
+    double* wa = w;            w += na;
+    double* wda = w;           w += na;
+    double* gb = w;            w += ng;
+    double* dgb = w;           w += ng;
+    double* ex = w;            w += nx;
+    double* dex = w;           w += nx;
+    double* zero = w;          w += nz;
+    double* wjv = w;
+    if (!o) o = wo;
+    if (!dout) dout = wdo;
+    if (!aggbuf) aggbuf = wa;
+    if (!dagg) dagg = wda;
+    if (!jv) jv = wjv;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto fill = [&](double* x, size_t n, double val) {
+        if (n) hipLaunchKernelGGL(k_fill, dim3(std::max(1u, std::min<unsigned>(blocks_for(static_cast<int64_t>(n), 256), 65535u))),
+                                  dim3(256), 0, s, x, static_cast<int64_t>(n), val);
+    };
+    fill(zero, nz, 0.0);
+    if (!v) v = zero;
+    if (!dp) dp = zero;
+    if (!p) p = zero;
+    if (!ddelta) ddelta = zero;
+    if (!delta) delta = zero;
+    fill(du, static_cast<size_t>(pl->dim), 0.0);                              // coreloop.jl:24
+    fill(jv, static_cast<size_t>(pl->dim), 0.0);
+    fill(o, no, std::numeric_limits<double>::quiet_NaN());                     // network_structure.jl:153
+    fill(dout, no, std::numeric_limits<double>::quiet_NaN());
+    fill(aggbuf, na, 0.0);                                                     // coreloop.jl:30
+    fill(dagg, na, 0.0);
+    auto perturb = [&](int m, double* buf, double* dbuf) {
+        if (pl->n_pmap[m]) hipLaunchKernelGGL(k_perturb, dim3(1), dim3(64), 0, s, pl->d_pmap[m], pl->n_pmap[m], delta, ddelta, buf, dbuf);
+    };
+    // :g of vertices and of ODE edges without feed-forward (coreloop.jl:39-41)
+    for (const VDesc& vd : pl->vdesc)
+        if (vd.count) {
+            const unsigned nb = blocks_for(vd.count, 256);
+            hipLaunchKernelGGL(k_vertex_out, dim3(nb), dim3(256), 0, s, vd, u, o, no_sync(nb), p, 0);
+            hipLaunchKernelGGL(k_vertex_out, dim3(nb), dim3(256), 0, s, vd, v, dout, no_sync(nb), p, 1);
+            if (vd.outkind == 1) hipLaunchKernelGGL(k_vertex_out_nff_tangent, dim3(nb), dim3(256), 0, s, vd, p, dp, dout);
+        }
+    for (const EDesc& e : pl->edesc)
+        if (e.count && e.ode) {
+            hipLaunchKernelGGL(k_edge_out, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, u, p, o);
+            hipLaunchKernelGGL(k_edge_out, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, v, p, dout);
+        }
+    // loopback copies, vertex-input perturbation, feed-forward vertex g (coreloop.jl:47-55)
+    if (pl->d_lbin && na > 0)
+        for (const VDesc& vd : pl->vdesc)
+            if (vd.count) {
+                hipLaunchKernelGGL(k_loopback_copy, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, o, aggbuf);
+                hipLaunchKernelGGL(k_loopback_copy, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, dout, dagg);
+            }
+    perturb(1, aggbuf, dagg);
+    if (pl->ff_out && na > 0)
+        for (const VDesc& vd : pl->vdesc)
+            if (vd.outkind == 2 && vd.count)
+                hipLaunchKernelGGL(k_vertex_out_ff_buf, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, aggbuf, dagg,
+                                   p, dp, o, dout);
+    // external inputs, vertex-output perturbation, gather, edge-input perturbation (:56-72)
+    if (nx > 0) {
+        const unsigned nb = std::max(1u, std::min<unsigned>(blocks_for(pl->lastidx_extbuf, 256), 65535u));
+        hipLaunchKernelGGL(k_collect_externals, dim3(nb), dim3(256), 0, s, pl->d_extmap, pl->lastidx_extbuf, ex, u, o);
+        hipLaunchKernelGGL(k_collect_externals, dim3(nb), dim3(256), 0, s, pl->d_extmap, pl->lastidx_extbuf, dex, v, dout);
+    }
+    perturb(0, o, dout);
+    if (ng > 0) {
+        const unsigned nb = std::min<unsigned>(blocks_for(pl->lastidx_gbuf, 256), 65535u);
+        hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, s, pl->d_gmap, o, gb, pl->lastidx_gbuf);
+        hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, s, pl->d_gmap, dout, dgb, pl->lastidx_gbuf);
+    }
+    perturb(3, gb, dgb);
+    // edges: f of ODE edges, f and g of feed-forward edges (:74-78); edge-output perturbation
+    for (size_t b = 0; b < pl->edesc.size(); ++b) {
+        const EDesc& e = pl->edesc[b];
+        if (e.count)
+            hipLaunchKernelGGL(k_edge_buf, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, pl->d_egb + pl->egb_off[b],
+                               gb, dgb, u, v, ex, dex, p, dp, o, dout, du, jv);
+    }
+    perturb(2, o, dout);
+    // aggregation (:90) and vertex f (:97)
+    rc = hip_check(hipGetLastError(), "perturbed RHS launch");
+    if (rc) return rc;
+    if (na > 0) {
+        rc = nd_aggregate(pl, aggbuf, o, stream);
+        rc = rc ? rc : nd_aggregate(pl, dagg, dout, stream);
+        if (rc) return rc;
+    }
+    for (const VDesc& vd0 : pl->vdesc) {
+        if (!vd0.count) continue;
+        VDesc vd = vd0;
+        vd.t = t;
+        hipLaunchKernelGGL(pick_vertex_f_buf(vd.model), dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd,
+                           na > 0 ? aggbuf : nullptr, na > 0 ? dagg : nullptr, u, v, ex, dex, p, dp, du, jv);
+    }
+    return hip_check(hipGetLastError(), "perturbed RHS launch");
 }
 
 int nd_device_synchronize(void) { return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }

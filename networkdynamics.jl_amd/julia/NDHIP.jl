@@ -151,9 +151,31 @@ function NetworkDynamics.aggregate!(a::SegmentedAggregator, aggbuf::ROCVector{Fl
     nothing
 end
 
-# nw(du, u, p, t) for HIPExecution (replaces src/coreloop.jl:1-102)
-function (nw::Network{<:HIPExecution})(du::ROCVector{Float64}, u::ROCVector{Float64}, p::ROCVector{Float64}, t)
+_flatmap(m) = isempty(m) ? Int64[] : collect(Int64, Iterators.flatten(m))
+
+# nw(du, u, p, t; perturb, perturb_maps) for HIPExecution (replaces src/coreloop.jl:1-102).
+# With perturb: the buffered loop with the perturbations of coreloop.jl:49-87; the maps are
+# the (buffer index, channel) tuples of _classify_perturbation_channels.
+function (nw::Network{<:HIPExecution})(du::ROCVector{Float64}, u::ROCVector{Float64}, p::ROCVector{Float64}, t;
+                                       perturb=nothing, perturb_maps=nothing)
     a = nw.layer.aggregator::SegmentedAggregator
+    if !isnothing(perturb)
+        vo, vi = _flatmap(perturb_maps.vo_map), _flatmap(perturb_maps.vi_map)
+        eo, ei = _flatmap(perturb_maps.eo_map), _flatmap(perturb_maps.ei_map)
+        _check(ccall((:nd_plan_set_perturbation, libnd), Cint,
+                     (Ptr{Cvoid}, Int64, Int64, Ptr{Int64}, Int64, Ptr{Int64}, Int64, Ptr{Int64}, Int64, Ptr{Int64}),
+                     a.plan, length(perturb), length(vo) ÷ 2, vo, length(vi) ÷ 2, vi, length(eo) ÷ 2, eo,
+                     length(ei) ÷ 2, ei))
+        δ = ROCVector{Float64}(perturb)
+        _check(ccall((:nd_rhs_perturbed_jvp, libnd), Cint,
+                     (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                      Float64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                      Ptr{Cvoid}),
+                     a.plan, pointer(du), C_NULL, pointer(u), C_NULL, pointer(p), C_NULL, Float64(t), pointer(δ),
+                     C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, AMDGPU.stream().stream))
+        AMDGPU.synchronize()
+        return nothing
+    end
     _check(ccall((:nd_rhs, libnd), Cint,
                  (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Int64, Int64, Int64, Ptr{Cvoid}),
                  a.plan, pointer(du), pointer(u), pointer(p), Float64(t), length(du), length(u), length(p),

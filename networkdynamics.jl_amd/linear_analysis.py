@@ -71,6 +71,8 @@ class NetworkDescriptorSystem:
         if self.B is None:
             raise _abi.ArgumentError("the system has no inputs and outputs (linearize with in/out channels)")
         val = self.C @ np.linalg.solve(self.dense_M() * s - self.A, self.B) + self.D
+        if isinstance(self.insym, comp.VIndex) and isinstance(self.outsym, comp.VIndex):
+            return val[0, 0]                     # SISO: a scalar (linear_analysis_base.jl:79-89)
         return val
 
     def __repr__(self):
@@ -86,11 +88,165 @@ def _dense_mass(M, n):
     return np.asarray(M, dtype=np.float64)
 
 
-def linearize_network(nw, u, p, t=0.0):
-    """NetworkDescriptorSystem(M = nw.mass_matrix, A = d(du)/du at (u, p, t), sym):
-    linearize_network(s0) without in/out channels (linear_analysis.jl:74-85)."""
-    Jd = nw.jacobian(u, p, t).cpu().numpy()
-    return NetworkDescriptorSystem(Jd, M=nw.mass_matrix, sym=nw.variable_symbols())
+def _vertex_outsym(m):
+    out = getattr(m, "outsym", None)
+    return list(out) if out else [m.sym[j - 1] for j in m.mask_dst]
+
+
+def _edge_out_names(m):
+    from .network import _edge_outsym_flat
+    flat = _edge_outsym_flat(m)
+    return flat[:m.outdim_src], flat[m.outdim_src:]
+
+
+def perturbation_channels(nw, channels):
+    """_classify_perturbation_channels (linear_analysis.jl:629-680): (delta0, maps) for
+    a list of VIndex / EIndex channels.  A parameter name gives a p_map entry (pflat
+    index); a vertex output / input name a vo_map / vi_map entry (output buffer /
+    aggregation buffer index); an edge output name (src side first) an eo_map entry, an
+    edge input name (src₊... / dst₊..., from insym) an ei_map entry (gather buffer
+    index).  Entries are 1-based (buffer index, channel) pairs."""
+    v, e = nw.ranges()
+    maps = dict(vo_map=[], vi_map=[], eo_map=[], ei_map=[], p_map=[])
+    for k, ref in enumerate(channels, start=1):
+        i = nw._component(ref)
+        s = ref.sym
+        if ref.kind == "V":
+            m = nw.vertexm[i]
+            if s in m.psym:
+                maps["p_map"].append((int(v[i, 2, 0] + m.psym.index(s)), k))
+            elif s in _vertex_outsym(m):
+                maps["vo_map"].append((int(v[i, 1, 0] + _vertex_outsym(m).index(s)), k))
+            elif m.insym and s in m.insym:
+                maps["vi_map"].append((int(v[i, 3, 0] + m.insym.index(s)), k))
+            else:
+                raise _abi.ArgumentError(f"Perturbation channel {ref!r} is not an input or output of its component.")
+        else:
+            m = nw.edgem[i]
+            osrc, odst = _edge_out_names(m)
+            if s in m.psym:
+                maps["p_map"].append((int(e[i, 3, 0] + m.psym.index(s)), k))
+            elif s in osrc:
+                maps["eo_map"].append((int(e[i, 1, 0] + osrc.index(s)), k))
+            elif s in odst:
+                maps["eo_map"].append((int(e[i, 2, 0] + odst.index(s)), k))
+            elif m.insym and s in m.insym["src"]:
+                maps["ei_map"].append((int(e[i, 4, 0] + m.insym["src"].index(s)), k))
+            elif m.insym and s in m.insym["dst"]:
+                maps["ei_map"].append((int(e[i, 5, 0] + m.insym["dst"].index(s)), k))
+            else:
+                raise _abi.ArgumentError(f"Perturbation channel {ref!r} is not an input or output of its component.")
+    return np.zeros(len(channels)), maps
+
+
+def _observable(nw, ref):
+    """Where an output channel of linearize_network reads its value: ("u", i) a state,
+    ("o", i) the output buffer, ("agg", i) the aggregation buffer, ("gbuf", g, i) the
+    gather-buffer entry g, which holds output i (0-based indices, g 1-based)."""
+    v, e = nw.ranges()
+    i = nw._component(ref)
+    s = ref.sym
+    if ref.kind == "V":
+        m = nw.vertexm[i]
+        if s in m.sym:
+            return ("u", int(v[i, 0, 0] - 1 + m.sym.index(s)))
+        if s in _vertex_outsym(m):
+            return ("o", int(v[i, 1, 0] - 1 + _vertex_outsym(m).index(s)))
+        if m.insym and s in m.insym:
+            return ("agg", int(v[i, 3, 0] - 1 + m.insym.index(s)))
+    else:
+        m = nw.edgem[i]
+        osrc, odst = _edge_out_names(m)
+        if s in m.sym:
+            return ("u", int(e[i, 0, 0] - 1 + m.sym.index(s)))
+        if s in osrc:
+            return ("o", int(e[i, 1, 0] - 1 + osrc.index(s)))
+        if s in odst:
+            return ("o", int(e[i, 2, 0] - 1 + odst.index(s)))
+        if m.insym and (s in m.insym["src"] or s in m.insym["dst"]):
+            src = s in m.insym["src"]
+            j = (m.insym["src"] if src else m.insym["dst"]).index(s)
+            end = (nw.g.src if src else nw.g.dst)[i] - 1
+            return ("gbuf", int(e[i, 4 if src else 5, 0] + j), int(v[end, 1, 0] - 1 + j))
+    raise _abi.ArgumentError(f"Cannot observe {ref!r}: not a state, output or input of its component.")
+
+
+def linearize_network(nw, u, p, t=0.0, in_=None, out=None):
+    """linearize_network(s0; in, out), linear_analysis.jl:74-122.
+
+    Without channels: NetworkDescriptorSystem(M = nw.mass_matrix, A = d(du)/du, sym).
+    With perturbation channels in_ (VIndex / EIndex or a list of them: parameters,
+    vertex outputs / inputs, edge outputs / inputs) and observations out (states,
+    outputs, inputs): the full M x' = A x + B u, y = C x + D u.  B and D are the
+    derivatives of f and of the observations in the channels at delta = 0, C those of
+    the observations in the states: one dual-number pass of the perturbed loop per
+    channel and per state column (parameters enter as dual numbers as well).  A single
+    channel / observation (not a list) gives a SISO system whose transfer function is a
+    scalar."""
+    import torch
+    dev = u.device if isinstance(u, torch.Tensor) else torch.device("cuda", torch.cuda.current_device())
+    u = torch.as_tensor(u, dtype=torch.float64, device=dev).contiguous()
+    pd = torch.as_tensor(p if len(p) else np.zeros(1), dtype=torch.float64, device=dev).contiguous()
+    if in_ is None and out is None:
+        Jd = nw.jacobian(u, pd, t).cpu().numpy()
+        return NetworkDescriptorSystem(Jd, M=nw.mass_matrix, sym=nw.variable_symbols())
+    if in_ is None:
+        raise _abi.ArgumentError("Must specify `in` when `out` is given")
+    if out is None:
+        raise _abi.ArgumentError("Must specify `out` when `in` is given")
+    single_in, single_out = isinstance(in_, comp.VIndex), isinstance(out, comp.VIndex)
+    insym = [in_] if single_in else list(in_)
+    outsym = [out] if single_out else list(out)
+    delta0, maps = perturbation_channels(nw, insym)
+    obs = [_observable(nw, r) for r in outsym]
+    n, nin, nout = nw.dim, len(insym), len(outsym)
+    A_ = nw.jacobian(u, pd, t).cpu().numpy()
+    B = np.zeros((n, nin))
+    C = np.zeros((nout, n))
+    D = np.zeros((nout, nin))
+    du = torch.empty(n, dtype=torch.float64, device=u.device)
+    jv = torch.empty_like(du)
+    ei = maps["ei_map"]
+
+    def observe(bufs, v_host, dd_host):
+        o, do, agg, dagg = (b.cpu().numpy() for b in bufs)
+        y = np.empty(nout)
+        for r, ob in enumerate(obs):
+            if ob[0] == "u":
+                y[r] = v_host[ob[1]] if v_host is not None else 0.0
+            elif ob[0] == "o":
+                y[r] = do[ob[1]]
+            elif ob[0] == "agg":
+                y[r] = dagg[ob[1]]
+            else:                           # gbuf = gathered output + its edge-input channels
+                y[r] = do[ob[2]] + sum(dd_host[k - 1] for g, k in ei if g == ob[1])
+        return y
+
+    p_chan = {k: b for b, k in maps["p_map"]}
+    for c in range(nin):
+        dd = np.zeros(nin)
+        dp = None
+        if c + 1 in p_chan:                 # parameter channel: p + delta at pflat index b
+            dp = torch.zeros(max(nw.pdim, 1), dtype=torch.float64, device=u.device)
+            dp[p_chan[c + 1] - 1] = 1.0
+            dp = dp[:nw.pdim] if nw.pdim else dp
+        else:
+            dd[c] = 1.0
+        bufs = nw.perturbed_jvp(du, jv, u, None, pd, t, delta0, dd, maps, buffers=True, dp=dp)
+        B[:, c] = jv.cpu().numpy()
+        D[:, c] = observe(bufs, None, dd)
+    if any(ob[0] != "u" for ob in obs):
+        for j in range(n):
+            vj = np.zeros(n)
+            vj[j] = 1.0
+            bufs = nw.perturbed_jvp(du, jv, u, torch.from_numpy(vj).to(u.device), pd, t, delta0, None, maps,
+                                    buffers=True)
+            C[:, j] = observe(bufs, vj, np.zeros(nin))
+    else:
+        for r, ob in enumerate(obs):
+            C[r, ob[1]] = 1.0
+    return NetworkDescriptorSystem(A_, M=nw.mass_matrix, B=B, C=C, D=D, sym=nw.variable_symbols(),
+                                   insym=in_ if single_in else insym, outsym=out if single_out else outsym)
 
 
 def reduce_dae(sys):

@@ -396,10 +396,16 @@ class Network:
         return dict(hub_degree=t.hub_degree, block_size=t.block_size, big_hub_degree=t.big_hub_degree,
                     rk4_path=t.rk4_path, gather_mode=t.gather_mode)
 
-    def __call__(self, du, u, p, t=0.0):
-        """nw(du, u, p, t): coreloop.jl:1-102.  du is overwritten; stream-ordered on
-        torch's current stream."""
+    def __call__(self, du, u, p, t=0.0, perturb=None, perturb_maps=None):
+        """nw(du, u, p, t; perturb, perturb_maps): coreloop.jl:1-102.  du is overwritten;
+        stream-ordered on torch's current stream.  perturb: channel values added at the
+        buffer positions of perturb_maps (vo_map / vi_map / eo_map / ei_map, 1-based
+        (buffer index, channel) pairs, as linear_analysis.perturbation_channels builds
+        them); a parameter map is the caller's to apply to p."""
         import torch
+        if perturb is not None:
+            self.perturbed_jvp(du, None, u, None, p, t, perturb, None, perturb_maps)
+            return None
         dptr, ndu = _tensor_ptr(du, self.dim, "du", torch)
         uptr, nu = _tensor_ptr(u, self.dim, "u", torch)
         if self.pdim > 0:
@@ -484,6 +490,55 @@ class Network:
         A.check(A.lib().nd_get_buffers(self.plan, o.data_ptr(), agg.data_ptr() if agg.numel() else None,
                                        uptr, pptr, float(t), stream))
         return o, agg
+
+    def _set_perturbation(self, ndelta, maps):
+        key = (int(ndelta),) + tuple(tuple(map(tuple, maps.get(k, []) or [])) for k in
+                                     ("vo_map", "vi_map", "eo_map", "ei_map"))
+        if getattr(self, "_pert_key", None) == key:
+            return
+        arrs = [np.ascontiguousarray(np.asarray(maps.get(k, []) or [], np.int64).reshape(-1, 2))
+                for k in ("vo_map", "vi_map", "eo_map", "ei_map")]
+        args = []
+        for a in arrs:
+            args += [len(a), a.ctypes.data if len(a) else None]
+        A.check(A.lib().nd_plan_set_perturbation(self.plan, int(ndelta), *args))
+        self._pert_key = key
+
+    def perturbed_jvp(self, du, jv, u, v, p, t, delta, ddelta, perturb_maps, buffers=False, dp=None):
+        """nw(du, u + eps v, p + eps dp, t; perturb = delta + eps ddelta, perturb_maps) on
+        dual numbers (the buffered loop of coreloop.jl:1-102 with the four perturbation
+        points): du = f, jv = its directional derivative (None: not wanted).  v, dp,
+        ddelta: None for zero tangents.  With buffers=True returns (o, do, aggbuf, dagg),
+        the perturbed buffers and their tangents."""
+        import torch
+        dptr, _ = _tensor_ptr(du, self.dim, "du", torch)
+        uptr, _ = _tensor_ptr(u, self.dim, "u", torch)
+        jptr = _tensor_ptr(jv, self.dim, "jv", torch)[0] if jv is not None else None
+        vptr = _tensor_ptr(v, self.dim, "v", torch)[0] if v is not None else None
+        delta = torch.as_tensor(np.asarray(delta, np.float64) if not isinstance(delta, torch.Tensor) else delta,
+                                dtype=torch.float64, device=u.device).contiguous()
+        self._set_perturbation(delta.numel(), perturb_maps or {})
+        if ddelta is not None:
+            ddelta = torch.as_tensor(np.asarray(ddelta, np.float64) if not isinstance(ddelta, torch.Tensor)
+                                     else ddelta, dtype=torch.float64, device=u.device).contiguous()
+            if ddelta.numel() != delta.numel():
+                raise A.ArgumentError("ddelta must have the length of delta")
+        if self.pdim > 0:
+            pptr, _ = _tensor_ptr(p, self.pdim, "p", torch)
+        else:
+            pptr = p.data_ptr() if isinstance(p, torch.Tensor) and p.numel() else None
+        dpptr = _tensor_ptr(dp, self.pdim, "dp", torch)[0] if dp is not None else None
+        bufs = None
+        if buffers:
+            bufs = [torch.empty(n, dtype=torch.float64, device=u.device) for n in
+                    (self.info.lastidx_out, self.info.lastidx_out, self.info.lastidx_aggr, self.info.lastidx_aggr)]
+        bp = [b.data_ptr() if b is not None and b.numel() else None for b in (bufs or [None] * 4)]
+        stream = torch.cuda.current_stream(u.device).cuda_stream
+        A.check(A.lib().nd_rhs_perturbed_jvp(self.plan, dptr, jptr, uptr, vptr, pptr, dpptr, float(t),
+                                             delta.data_ptr() if delta.numel() else None,
+                                             ddelta.data_ptr() if ddelta is not None and ddelta.numel() else None,
+                                             *bp, stream))
+        return tuple(bufs) if buffers else None
 
     def gather(self, gbuf, o):
         """gather!(EagerGBufProvider, gbuf, o): gbuf[k] = o[map[k]] (gbufs.jl:25)."""

@@ -633,8 +633,37 @@ static void apply_loopback(const ndo_net* nw, double* aggbuf, const double* o) {
 /* Full RHS, SequentialExecution{buffered} + SequentialAggregator(+). Returns 0, or 1 on size mismatch
    (the ArgumentError of coreloop.jl:2-7).  If o_out/agg_out are non-NULL they receive the buffers
    (get_buffers with initbufs=true, coreloop.jl:103-109; then du may be NULL). */
+/* apply_perturb!(buf, perturb, map), coreloop.jl:258-262: map holds n pairs
+   (buffer index, channel), both 1-based */
+static void apply_perturb(double* buf, const double* delta, const int64_t* map, int64_t n) {
+    for (int64_t k = 0; k < n; ++k) buf[map[2 * k] - 1] += delta[map[2 * k + 1] - 1];
+}
+
+/* The perturbation maps of nw(du, u, p, t; perturb, perturb_maps): vo (output buffer,
+   vertex outputs), vi (aggregation buffer), eo (output buffer, edge outputs), ei (gather
+   buffer); each n pairs (buffer index, channel), 1-based, flattened. */
+typedef struct { const int64_t* map[4]; int64_t n[4]; const double* delta; } perturb_t;
+
+static int rhs_impl(ndo_net* nw, double* du, const double* u, const double* p, double t, int32_t buffered,
+                    double* o_out, double* agg_out, const perturb_t* pt);
+
 int ndo_rhs(ndo_net* nw, double* du, const double* u, const double* p, double t, int32_t buffered,
             double* o_out, double* agg_out) {
+    return rhs_impl(nw, du, u, p, t, buffered, o_out, agg_out, NULL);
+}
+
+/* nw(du, u, p, t; perturb=delta, perturb_maps) in the buffered loop (coreloop.jl:1-102):
+   delta enters after the loopback copy (vi), after collect_externals! (vo), after the
+   gather (ei) and before the aggregation (eo).  o_out/agg_out: the perturbed buffers. */
+int ndo_rhs_perturbed(ndo_net* nw, double* du, const double* u, const double* p, double t, const double* delta,
+                      const int64_t* vo, int64_t nvo, const int64_t* vi, int64_t nvi, const int64_t* eo, int64_t neo,
+                      const int64_t* ei, int64_t nei, double* o_out, double* agg_out) {
+    perturb_t pt = {{vo, vi, eo, ei}, {nvo, nvi, neo, nei}, delta};
+    return rhs_impl(nw, du, u, p, t, 1, o_out, agg_out, &pt);
+}
+
+static int rhs_impl(ndo_net* nw, double* du, const double* u, const double* p, double t, int32_t buffered,
+                    double* o_out, double* agg_out, const perturb_t* pt) {
     double* o = nw->o;
     double* aggbuf = nw->aggbuf;
     double* gbuf = nw->gbuf;
@@ -652,13 +681,16 @@ int ndo_rhs(ndo_net* nw, double* du, const double* u, const double* p, double t,
         for (int64_t i = 1; i <= B->n; ++i) apply_edge(nw, B, i, 0, 1, buffered, du, u, o, gbuf, p, t);
     }
     apply_loopback(nw, aggbuf, o);                                               /* coreloop.jl:47 */
+    if (pt) apply_perturb(aggbuf, pt->delta, pt->map[1], pt->n[1]);               /* coreloop.jl:49-52 */
     for (int b = 0; b < nw->nvb; ++b) {                                          /* coreloop.jl:55 */
         const batch_t* B = &nw->vb[b];
         if (!hasff(&nw->kinds[B->kind])) continue;
         for (int64_t i = 1; i <= B->n; ++i) apply_vertex(nw, B, i, 1, du, u, o, aggbuf, p, t);
     }
     if (nw->lastidx_extbuf > 0) ndo_collect_externals(nw, nw->extbuf, u, o);     /* coreloop.jl:56-57 */
+    if (pt) apply_perturb(o, pt->delta, pt->map[0], pt->n[0]);                    /* coreloop.jl:63-65 */
     if (buffered) ndo_gather(nw, gbuf, o);                                       /* coreloop.jl:67 */
+    if (pt) apply_perturb(gbuf, pt->delta, pt->map[3], pt->n[3]);                 /* coreloop.jl:69-72 */
     for (int b = 0; b < nw->neb; ++b) {                                          /* coreloop.jl:76 */
         const batch_t* B = &nw->eb[b];
         if (hasff(&nw->kinds[B->kind])) continue;
@@ -670,6 +702,7 @@ int ndo_rhs(ndo_net* nw, double* du, const double* u, const double* p, double t,
         if (!hasff(&nw->kinds[B->kind])) continue;
         for (int64_t i = 1; i <= B->n; ++i) apply_edge(nw, B, i, du != NULL, 1, buffered, du, u, o, gbuf, p, t);
     }
+    if (pt) apply_perturb(o, pt->delta, pt->map[2], pt->n[2]);                    /* coreloop.jl:86-88 */
     ndo_aggregate_seq(nw, aggbuf, o);                                            /* coreloop.jl:90 */
     if (o_out) memcpy(o_out, o, (size_t)nw->lastidx_out * 8);
     if (agg_out) memcpy(agg_out, aggbuf, (size_t)nw->lastidx_aggr * 8);

@@ -66,6 +66,8 @@ def lib():
         L.ndo_collect_externals.argtypes = [P, f64p, f64p, f64p]
         L.ndo_rhs.argtypes = [P, P, f64p, f64p, C.c_double, C.c_int32, P, P]
         L.ndo_rhs.restype = C.c_int
+        L.ndo_rhs_perturbed.argtypes = [P, P, f64p, f64p, C.c_double, f64p] + [P, C.c_int64] * 4 + [P, P]
+        L.ndo_rhs_perturbed.restype = C.c_int
         L.ndo_rhs_threaded.argtypes = [P, f64p, f64p, f64p, C.c_double, C.c_int32]
         L.ndo_rhs_scale.argtypes = [P, f64p, f64p, f64p, C.c_double]
         L.ndo_rk4.argtypes = [P, f64p, f64p, C.c_double, C.c_double, C.c_int64]
@@ -200,6 +202,25 @@ class OracleNet:
             raise ValueError("du or u does not have expected size")
         lib().ndo_rhs(self._h, du.ctypes.data_as(C.c_void_p), u, p, float(t), int(buffered), None, None)
         return du
+
+    def rhs_perturbed(self, u, p, delta, maps, t=0.0):
+        """nw(du, u, p, t; perturb=delta, perturb_maps): maps = dict of vo_map, vi_map,
+        eo_map, ei_map, each a list of 1-based (buffer index, channel) pairs.  Returns
+        (du, o, aggbuf) of the perturbed loop."""
+        u = np.ascontiguousarray(u, np.float64)
+        p = np.ascontiguousarray(p if len(p) else np.zeros(1), np.float64)
+        delta = np.ascontiguousarray(delta if len(delta) else np.zeros(1), np.float64)
+        du = np.empty(self.dim, np.float64)
+        o = np.empty(self.lastidx_out, np.float64)
+        a = np.empty(max(self.lastidx_aggr, 1), np.float64)
+        arrs = [np.ascontiguousarray(np.asarray(maps.get(k, []), np.int64).reshape(-1, 2))
+                for k in ("vo_map", "vi_map", "eo_map", "ei_map")]
+        args = []
+        for m in arrs:
+            args += [m.ctypes.data_as(C.c_void_p) if len(m) else None, len(m)]
+        lib().ndo_rhs_perturbed(self._h, du.ctypes.data_as(C.c_void_p), u, p, float(t), delta, *args,
+                                o.ctypes.data_as(C.c_void_p), a.ctypes.data_as(C.c_void_p))
+        return du, o, a[: self.lastidx_aggr]
 
     def buffers(self, u, p, t=0.0):
         u = np.ascontiguousarray(u, np.float64)
