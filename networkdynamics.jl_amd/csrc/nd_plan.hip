@@ -276,14 +276,12 @@ __device__ __forceinline__ double segment_sum(int32_t beg, int32_t end, int32_t 
     return acc;
 }
 
-// LDS of the vertex kernels: hub-block wave partials, and the low path's entry window
-// (values, owners) and own outputs
-struct RhsShared {
-    double wpart[256 / kWave];
-    double val[4 * 256];
-    double own[256];
-    uint8_t owner[4 * 256];
-};
+// LDS of the fused vertex kernels (each kernel that uses them gets its own copy): the
+// low path's entry window (values, owners) and own outputs, the hub blocks' wave partials
+__shared__ double rhs_val[4 * 256];
+__shared__ double rhs_own[256];
+__shared__ uint8_t rhs_owner[4 * 256];
+__shared__ double rhs_wpart[256 / kWave];
 
 // Fused RHS for one vertex batch.  Block layout (hubs first, so the longest
 // segments start earliest):
@@ -293,7 +291,7 @@ struct RhsShared {
 // Hub segments: per-thread partial sums in a fixed order, xor-butterfly per wave,
 // wave partials combined in wave order -- deterministic run to run.
 template <int VM, int EM, int WRAP, bool MULTI, bool STAGE, bool KC>
-__device__ __forceinline__ void vertex_rhs_body(const VDesc& vd, int32_t b, RhsShared& sh,
+__device__ __forceinline__ void vertex_rhs_body(const VDesc& vd, int32_t b,
                                                 const int32_t* __restrict__ rowptr, const int2* __restrict__ slots,
                                                 const uint8_t* __restrict__ slot_eb, const EDesc* __restrict__ ed,
                                                 const double* __restrict__ u, const double* __restrict__ vo,
@@ -312,11 +310,11 @@ __device__ __forceinline__ void vertex_rhs_body(const VDesc& vd, int32_t b, RhsS
             const int32_t t = static_cast<int32_t>(threadIdx.x);
             const double part = wave_sum(segment_sum<EM, WRAP, MULTI, KC>(beg, end, t, static_cast<int32_t>(blockDim.x),
                                                                           own, slots, slot_eb, ed, u, vo, p, vd.kfull));
-            if ((t & (kWave - 1)) == 0) sh.wpart[t / kWave] = part;
+            if ((t & (kWave - 1)) == 0) rhs_wpart[t / kWave] = part;
             __syncthreads();
             if (t == 0) {
-                double acc = sh.wpart[0];
-                for (int32_t w = 1; w < wpb; ++w) acc += sh.wpart[w];
+                double acc = rhs_wpart[0];
+                for (int32_t w = 1; w < wpb; ++w) acc += rhs_wpart[w];
                 vertex_emit<VM, STAGE>(du, sa, x0, u + x0, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, dim,
                                        vd.mask[0], static_cast<int64_t>(vd.out0) + q);
             }
@@ -376,12 +374,12 @@ __device__ __forceinline__ void vertex_rhs_body(const VDesc& vd, int32_t b, RhsS
         for (int j = 0; j < NPV; ++j)
             pvs[j] = (valid && VPar<VM>::np > 0) ? p[vd.param0 + static_cast<int64_t>(q) * vd.pdim + j] : 0.0;
         const double own = (DC == 1 || vd.mask[0] == 0) ? xs[0] : xs[DC - 1];
-        sh.own[t] = own;
+        rhs_own[t] = own;
         double acc = 0.0;
         const int32_t W = 4 * T;
         for (int32_t base = S0; base < S1; base += W) {
             const int32_t wend = min(base + W, S1);
-            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) sh.owner[s2 - base] = static_cast<uint8_t>(t);
+            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) rhs_owner[s2 - base] = static_cast<uint8_t>(t);
             __syncthreads();
             constexpr int NP = EModel<EM>::np;
             int2 sl[4];
@@ -416,15 +414,15 @@ __device__ __forceinline__ void vertex_rhs_body(const VDesc& vd, int32_t b, RhsS
             for (int j = 0; j < 4; ++j) {
                 const int32_t s2 = base + j * T + t;
                 if (s2 < wend) {
-                    const double ow = sh.own[sh.owner[s2 - base]];
+                    const double ow = rhs_own[rhs_owner[s2 - base]];
                     double v;
                     if constexpr (MULTI) v = slot_value_rt(ed[vd.lseb[s2]], sl[j], ow, u, vo, p);
                     else v = slot_eval<EM, WRAP>(sl[j], nb[j], kk[j], ow);
-                    sh.val[s2 - base] = v;
+                    rhs_val[s2 - base] = v;
                 }
             }
             __syncthreads();
-            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) acc += sh.val[s2 - base];
+            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) acc += rhs_val[s2 - base];
             __syncthreads();
         }
         if (valid && !hub) vertex_emit<VM, STAGE>(du, sa, x0, xs, acc, pvs, dim, vd.mask[0], static_cast<int64_t>(vd.out0) + q);
@@ -439,8 +437,7 @@ __global__ void __launch_bounds__(256) k_vertex_rhs(VDesc vd, const int32_t* __r
                                                     const double* __restrict__ u, const double* __restrict__ vo,
                                                     const double* __restrict__ p,
                                                     double* __restrict__ du, StageArgs sa) {
-    __shared__ RhsShared sh;
-    vertex_rhs_body<VM, EM, WRAP, MULTI, STAGE, KC>(vd, static_cast<int32_t>(blockIdx.x), sh, rowptr, slots, slot_eb,
+    vertex_rhs_body<VM, EM, WRAP, MULTI, STAGE, KC>(vd, static_cast<int32_t>(blockIdx.x), rowptr, slots, slot_eb,
                                                     ed, u, vo, p, du, sa);
 }
 
@@ -469,30 +466,29 @@ __global__ void __launch_bounds__(256) k_vertex_rhs_group(VGroup g, const int32_
                                                           const double* __restrict__ u, const double* __restrict__ vo,
                                                           const double* __restrict__ p,
                                                           double* __restrict__ du, StageArgs sa) {
-    __shared__ RhsShared sh;
     const int32_t bx = static_cast<int32_t>(blockIdx.x);
     const int i = group_of_block(g, bx);
     const VDesc& vd = g.vd[i];
     const int32_t b = bx - g.first[i];
     switch (vd.model) {
     case ND_VM_DIFFUSION:
-        vertex_rhs_body<ND_VM_DIFFUSION, EM, WRAP, MULTI, STAGE, KC>(vd, b, sh, rowptr, slots, slot_eb, ed, u, vo, p, du, sa);
+        vertex_rhs_body<ND_VM_DIFFUSION, EM, WRAP, MULTI, STAGE, KC>(vd, b, rowptr, slots, slot_eb, ed, u, vo, p, du, sa);
         break;
     case ND_VM_KURAMOTO:
-        vertex_rhs_body<ND_VM_KURAMOTO, EM, WRAP, MULTI, STAGE, KC>(vd, b, sh, rowptr, slots, slot_eb, ed, u, vo, p, du, sa);
+        vertex_rhs_body<ND_VM_KURAMOTO, EM, WRAP, MULTI, STAGE, KC>(vd, b, rowptr, slots, slot_eb, ed, u, vo, p, du, sa);
         break;
     case ND_VM_SWING:
-        vertex_rhs_body<ND_VM_SWING, EM, WRAP, MULTI, STAGE, KC>(vd, b, sh, rowptr, slots, slot_eb, ed, u, vo, p, du, sa);
+        vertex_rhs_body<ND_VM_SWING, EM, WRAP, MULTI, STAGE, KC>(vd, b, rowptr, slots, slot_eb, ed, u, vo, p, du, sa);
         break;
     case ND_VM_SWING_UNIT_DAMPING:
-        vertex_rhs_body<ND_VM_SWING_UNIT_DAMPING, EM, WRAP, MULTI, STAGE, KC>(vd, b, sh, rowptr, slots, slot_eb, ed, u, vo,
+        vertex_rhs_body<ND_VM_SWING_UNIT_DAMPING, EM, WRAP, MULTI, STAGE, KC>(vd, b, rowptr, slots, slot_eb, ed, u, vo,
                                                                              p, du, sa);
         break;
     case ND_VM_LOAD:
-        vertex_rhs_body<ND_VM_LOAD, EM, WRAP, MULTI, STAGE, KC>(vd, b, sh, rowptr, slots, slot_eb, ed, u, vo, p, du, sa);
+        vertex_rhs_body<ND_VM_LOAD, EM, WRAP, MULTI, STAGE, KC>(vd, b, rowptr, slots, slot_eb, ed, u, vo, p, du, sa);
         break;
     default:
-        vertex_rhs_body<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false, STAGE, false>(vd, b, sh, rowptr, slots, slot_eb, ed,
+        vertex_rhs_body<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false, STAGE, false>(vd, b, rowptr, slots, slot_eb, ed,
                                                                                   u, vo, p, du, sa);
         break;
     }
