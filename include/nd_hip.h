@@ -383,6 +383,15 @@ typedef struct {
                              <= ss_reltol*|u_i| stops there with retcode 4 (the test of the
                              TerminateSteadyState callback with which basin-stability runs
                              stop at sync).  RK4 pays one extra RHS per step for du.       */
+    /* Per instance, or both NULL: the PresetTimeComponentCallback of
+       docs/examples/cascading_failure.jl:125-131.  At t = event_t[i] (only
+       t0 < event_t <= t1 fires) edge event_e[i] (0-based within the instance, -1 = none)
+       gets its first parameter (K) set to 0, in p as well, and counts in ntrip if it was
+       not 0.  Tsit5 steps to event_t exactly (a tstop) and restarts from the new
+       derivative; RK4 fires at the end of the step that ends at event_t (the time must
+       lie on the step grid t0 + k*dt0). */
+    const double* event_t;
+    const int32_t* event_e;
 } nd_ensemble_opts;
 
 typedef struct {
@@ -390,7 +399,7 @@ typedef struct {
     int64_t naccept, nreject, nf;
     int32_t retcode;      /* 0 success, 1 maxiters, 2 dt below dtmin, 3 NaN step,
                              4 terminated at a steady state (ss_abstol / ss_reltol)          */
-    int32_t ntrip;        /* lines tripped (line_limit set)                                     */
+    int32_t ntrip;        /* lines tripped (line_limit set, or the preset event)               */
 } nd_ensemble_stat;
 
 typedef struct nd_ensemble nd_ensemble;

@@ -50,6 +50,8 @@ struct SolveArgs {
     double* u;
     double* p;                    // written only where a line trips (line_limit set)
     const double* limit;          // per edge (instance order), or null: no line tripping
+    const double* event_t;        // per instance: preset trip time, or null
+    const int32_t* event_e;       // per instance: the edge it trips (local index, -1 none)
     nd_ensemble_stat* stats;
     double t0, t1, dt0, reltol, abstol, dtmax, dtmin;
     double ss_abstol, ss_reltol;  // steady-state termination (ss != 0)
@@ -165,6 +167,22 @@ __device__ __forceinline__ int trip_lines(const SolveArgs& a, const InstDesc& I,
     return static_cast<int>(block_sum(cnt, L.red));
 }
 
+// The preset-time trip of one edge (PresetTimeComponentCallback with the same affect):
+// its first parameter goes to 0 in LDS and in p.  Returns 1 if it was not 0 before
+// (block-uniform).
+template <int VM, int EM>
+__device__ __forceinline__ int trip_edge(const SolveArgs& a, const InstDesc& I, const Lds& L, int e) {
+    constexpr int NPE = EModel<EM>::np > 0 ? EModel<EM>::np : 1;
+    const bool was = L.epar[e * NPE] != 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        L.epar[e * NPE] = 0.0;
+        a.p[I.poff + static_cast<int64_t>(I.n) * VPar<VM>::np + static_cast<int64_t>(e) * NPE] = 0.0;
+    }
+    __syncthreads();
+    return was ? 1 : 0;
+}
+
 // Steady-state test after an accepted step (the allDerivPass test of DiffEqCallbacks'
 // TerminateSteadyState): every |du_i| <= ss_abstol or <= ss_reltol*|u_i|.  Block-uniform.
 template <int V, int D>
@@ -242,6 +260,17 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
     double tcur = a.t0;
     int64_t naccept = 0, nreject = 0, nf = 0;
     int retcode = 0, ntrip = 0;
+    // preset trip: fires once at te (t0 < te <= t1), else never
+    double te = __builtin_inf();
+    int32_t ee = -1;
+    if (a.event_t) {
+        te = a.event_t[inst];
+        ee = a.event_e[inst];
+        if (!(te > a.t0 && te <= a.t1) || ee < 0 || ee >= I.ne) {
+            te = __builtin_inf();
+            ee = -1;
+        }
+    }
     double dt = a.dt0;
     const double inv_len = 1.0 / (static_cast<double>(I.n) * D);
 
@@ -278,6 +307,7 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
                 edge_flows<VM, EM, V>(I, L, y);
                 ntrip += trip_lines<VM, EM>(a, I, L);
             }
+            if (ee >= 0 && step + 1 == static_cast<int64_t>(llround((te - a.t0) / dt))) ntrip += trip_edge<VM, EM>(a, I, L, ee);
             if (a.ss) {
                 rhs<VM, EM, V>(I, L, y, pv, k);
                 ++nf;
@@ -329,11 +359,12 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
         dt = fmin(dt, a.dtmax);
         double qold = 1e-4;
         int64_t iters = 0;
+        double tstop = ee >= 0 ? te : a.t1;       // the next time a step must end on
         while (tcur < a.t1) {
             if (iters++ >= a.maxiters) { retcode = 1; break; }
             if (dt < a.dtmin) { retcode = 2; break; }
-            const bool last = tcur + dt >= a.t1;
-            const double h = last ? a.t1 - tcur : dt;
+            const bool last = tcur + dt >= tstop;
+            const double h = last ? tstop - tcur : dt;
 #define ND_STAGE(expr)                                                        \
     _Pragma("unroll") for (int q = 0; q < V; ++q)                             \
         _Pragma("unroll") for (int j = 0; j < D; ++j) ys[q][j] = y[q][j] + h * (expr);
@@ -375,7 +406,7 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
             const double q11 = pow(eest, 0.14);
             if (eest <= 1.0) {
                 double qq = eest == 0.0 ? 0.1 : fmax(0.1, fmin(5.0, q11 / pow(qold, 0.08) / 0.9));
-                tcur = last ? a.t1 : tcur + h;
+                tcur = last ? tstop : tcur + h;
 #pragma unroll
                 for (int q = 0; q < V; ++q)
 #pragma unroll
@@ -385,13 +416,22 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
                 ++naccept;
                 // fval holds the flows at the accepted state (the last evaluation was at ys);
                 // after a trip the first-same-as-last derivative is stale: re-evaluate it
+                int changed = 0;
                 if (a.limit) {
                     const int tr = trip_lines<VM, EM>(a, I, L);
-                    if (tr > 0) {
-                        ntrip += tr;
-                        rhs<VM, EM, V>(I, L, y, pv, k1);
-                        ++nf;
-                    }
+                    ntrip += tr;
+                    changed += tr;
+                }
+                if (last && ee >= 0 && tstop == te) {   // the preset trip's time is reached
+                    const int tr = trip_edge<VM, EM>(a, I, L, ee);
+                    ntrip += tr;
+                    changed += tr;
+                    ee = -1;
+                    tstop = a.t1;
+                }
+                if (changed > 0) {
+                    rhs<VM, EM, V>(I, L, y, pv, k1);
+                    ++nf;
                 }
                 if (a.ss && at_steady_state<V, D>(a, I, L, y, k1)) {
                     retcode = 4;
@@ -579,6 +619,10 @@ int make_args(nd_ensemble* en, double* u, double* p, double t0, double t1, const
     a->u = u;
     a->p = p;
     a->limit = o->line_limit;
+    if ((o->event_t == nullptr) != (o->event_e == nullptr))
+        return nd::set_error(ND_ERR_ARGUMENT, "event_t and event_e go together");
+    a->event_t = o->event_t;
+    a->event_e = o->event_e;
     a->stats = stats;
     a->t0 = t0;
     a->t1 = t1;

@@ -35,7 +35,8 @@ TSIT5_BTILDE = [-0.00178001105222577714, -0.0008164344596567469, 0.0078808780102
 class _Opts(C.Structure):
     _fields_ = [("method", C.c_int32), ("pad_", C.c_int32), ("reltol", C.c_double), ("abstol", C.c_double),
                 ("dt0", C.c_double), ("dtmax", C.c_double), ("dtmin", C.c_double), ("maxiters", C.c_int64),
-                ("line_limit", C.c_void_p), ("ss_abstol", C.c_double), ("ss_reltol", C.c_double)]
+                ("line_limit", C.c_void_p), ("ss_abstol", C.c_double), ("ss_reltol", C.c_double),
+                ("event_t", C.c_void_p), ("event_e", C.c_void_p)]
 
 # retcode of an instance stopped at a steady state
 TERMINATED = 4
@@ -116,23 +117,29 @@ class Ensemble:
         return slice(int(self.param_offsets[i]), int(self.param_offsets[i] + n))
 
     def solve(self, u, p, tspan, method=TSIT5, reltol=1e-8, abstol=1e-8, dt=0.0, dtmax=0.0, maxiters=0,
-              line_limit=None, steady_state=None):
+              line_limit=None, steady_state=None, preset_trip=None):
         """Integrates all instances in place on the device vector u; returns per-instance stats.
         line_limit: per-edge limits (device, instance order) to trip lines whose |flow|
         exceeds them at the end of an accepted step; tripped lines get K = 0 in p.
         steady_state: True or (abstol, reltol) to stop each instance after the first
         accepted step where every |du_i| <= abstol or <= reltol*|u_i| (retcode TERMINATED,
-        stats t = the stopping time), as a TerminateSteadyState callback does."""
+        stats t = the stopping time), as a TerminateSteadyState callback does.
+        preset_trip: (times, edges), one entry per instance (edge 0-based within the
+        instance, -1 = none): a PresetTimeComponentCallback that sets the edge's K to 0
+        at that time (docs/examples/cascading_failure.jl:125-131); Tsit5 steps to it
+        exactly, RK4 needs it on the step grid."""
         import torch
         self._check(u, p)
         stats = torch.zeros(self.ninst * STAT_DTYPE.itemsize // 8, dtype=torch.float64, device=u.device)
-        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters, line_limit, steady_state)
+        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters, line_limit, steady_state, preset_trip,
+                       tspan, u.device)
         stream = torch.cuda.current_stream(u.device).cuda_stream
         A.check(A.lib().nd_ensemble_solve(self._h, u.data_ptr(), p.data_ptr(), float(tspan[0]), float(tspan[1]),
                                           C.byref(o), stats.data_ptr(), stream))
         return stats
 
-    def _opts(self, method, reltol, abstol, dt, dtmax, maxiters, line_limit=None, steady_state=None):
+    def _opts(self, method, reltol, abstol, dt, dtmax, maxiters, line_limit=None, steady_state=None,
+              preset_trip=None, tspan=None, device=None):
         ss = (0.0, 0.0)
         if steady_state is True:
             ss = STEADY_STATE_TOLS
@@ -147,8 +154,24 @@ class Ensemble:
                     line_limit.numel() != int(self.ne.sum()):
                 raise A.ArgumentError("line_limit must be a float64 device vector with one entry per edge")
             lim = line_limit.data_ptr()
+        ev_t = ev_e = None
+        if preset_trip is not None:
+            import torch
+            times = np.asarray(preset_trip[0], np.float64).reshape(-1)
+            edges = np.asarray(preset_trip[1], np.int32).reshape(-1)
+            if len(times) != self.ninst or len(edges) != self.ninst:
+                raise A.ArgumentError("preset_trip needs one time and one edge per instance")
+            if ((edges >= self.ne) | (edges < -1)).any():
+                raise A.ArgumentError("preset_trip edge outside its instance")
+            if method == RK4 and tspan is not None:
+                k = np.rint((times - tspan[0]) / dt)
+                on = np.abs(tspan[0] + k * dt - times) <= 1e-9 * np.maximum(1.0, np.abs(times))
+                if not on[(edges >= 0) & (times > tspan[0]) & (times <= tspan[1])].all():
+                    raise A.ArgumentError("RK4: preset trip times must lie on the step grid t0 + k*dt")
+            self._events = (torch.from_numpy(times).to(device), torch.from_numpy(edges).to(device))
+            ev_t, ev_e = self._events[0].data_ptr(), self._events[1].data_ptr()
         return _Opts(int(method), 0, float(reltol), float(abstol), float(dt), float(dtmax), 0.0, int(maxiters), lim,
-                     ss[0], ss[1])
+                     ss[0], ss[1], ev_t, ev_e)
 
     def _check(self, u, p):
         import torch

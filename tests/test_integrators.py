@@ -397,3 +397,76 @@ def test_ensemble_tsit5_steady_state_termination(ND, dev):
         np.testing.assert_allclose(th - th.mean(), thf - thf.mean(), atol=1e-4)
     with pytest.raises(ND.ArgumentError):
         ens.solve(u_d, p_d, (0.0, 1.0), steady_state=(-1.0, 0.0))
+
+
+@pytest.mark.gpu
+def test_ensemble_rk4_preset_trip_matches_oracle_stepping(ND, dev):
+    # PresetTimeComponentCallback (docs/examples/cascading_failure.jl:125-131): the line
+    # given per instance loses its K at t = 0.1, the end of step 20 of dt = 0.005.
+    # Reference: the oracle's RK4 one step at a time, K set to 0 after step 20.
+    import torch
+    gs, rng = _instances(ND, 6, 30, 300, seed=12)
+    ps, us = _swing_params(ND, gs, rng)
+    edges = np.array([int(rng.integers(0, g.ne())) for g in gs], np.int32)
+    edges[3] = -1                                            # no event for this instance
+    times = np.full(len(gs), 0.1)
+    ens = ND.Ensemble(gs, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    u_d = torch.from_numpy(np.concatenate(us)).to(dev)
+    p_d = torch.from_numpy(np.concatenate(ps)).to(dev)
+    dt, nsteps = 0.005, 60
+    st = ND.Ensemble.stats_to_numpy(ens.solve(u_d, p_d, (0.0, dt * nsteps), method=ND.ensemble.RK4, dt=dt,
+                                              preset_trip=(times, edges)))
+    torch.cuda.synchronize()
+    got_u, got_p = u_d.cpu().numpy(), p_d.cpu().numpy()
+    for i, g in enumerate(gs):
+        nw = ND.Network(g, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+        on = oracle_for(nw)
+        u, p = us[i].copy(), ps[i].copy()
+        for k in range(nsteps):
+            u = on.rk4(u, p, 0.0, dt, 1)
+            if k + 1 == 20 and edges[i] >= 0:
+                p[3 * g.nv() + edges[i]] = 0.0
+        assert st["ntrip"][i] == (1 if edges[i] >= 0 else 0)
+        np.testing.assert_array_equal(got_p[ens.param_slice(i)], p)
+        np.testing.assert_allclose(got_u[ens.state_slice(i)], u, rtol=0, atol=1e-10, err_msg=f"instance {i}")
+    with pytest.raises(ND.ArgumentError, match="step grid"):
+        ens.solve(u_d, p_d, (0.0, dt * nsteps), method=ND.ensemble.RK4, dt=dt,
+                  preset_trip=(np.full(len(gs), 0.1013), edges))
+    with pytest.raises(ND.ArgumentError, match="one time and one edge"):
+        ens.solve(u_d, p_d, (0.0, 1.0), preset_trip=(times[:2], edges[:2]))
+
+
+@pytest.mark.gpu
+def test_ensemble_tsit5_preset_trip_is_a_tstop(ND, dev):
+    # Tsit5 lands on the trip time and restarts there: the same as two solves split at
+    # t* with K set to 0 in between (to the integration tolerance), and clearly unlike a
+    # trip at a slightly later time
+    import torch
+    gs, rng = _instances(ND, 8, 50, 400, seed=13)
+    ps, us = _swing_params(ND, gs, rng)
+    edges = np.array([int(rng.integers(0, g.ne())) for g in gs], np.int32)
+    ens = ND.Ensemble(gs, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    tol = dict(reltol=1e-11, abstol=1e-11)
+
+    def run(t_event):
+        u_d = torch.from_numpy(np.concatenate(us)).to(dev)
+        p_d = torch.from_numpy(np.concatenate(ps)).to(dev)
+        st = ens.solve(u_d, p_d, (0.0, 2.0), preset_trip=(np.full(len(gs), t_event), edges), **tol)
+        torch.cuda.synchronize()
+        return u_d.cpu().numpy(), p_d.cpu().numpy(), ND.Ensemble.stats_to_numpy(st)
+
+    u1, p1, st = run(0.7)
+    assert (st["retcode"] == 0).all() and (st["ntrip"] == 1).all() and np.all(st["t"] == 2.0)
+    u_d = torch.from_numpy(np.concatenate(us)).to(dev)
+    p_d = torch.from_numpy(np.concatenate(ps)).to(dev)
+    ens.solve(u_d, p_d, (0.0, 0.7), **tol)
+    p_h = p_d.cpu().numpy()
+    for i, g in enumerate(gs):
+        p_h[ens.param_slice(i).start + 3 * g.nv() + edges[i]] = 0.0
+    np.testing.assert_array_equal(p_h, p1)
+    p_d = torch.from_numpy(p_h).to(dev)
+    ens.solve(u_d, p_d, (0.7, 2.0), **tol)
+    u2 = u_d.cpu().numpy()
+    np.testing.assert_allclose(u1, u2, rtol=0, atol=1e-7)
+    u3, _, _ = run(0.71)
+    assert np.abs(u3 - u2).max() > 100 * np.abs(u1 - u2).max()
