@@ -521,6 +521,9 @@ noop_vertex = DeviceFunction("noop_vertex", "vertex_f", A.ND_VM_NOOP, None, 0,
                              "test/aggregators_test.jl:15")
 diffusionvertex = DeviceFunction("diffusionvertex!", "vertex_f", A.ND_VM_DIFFUSION, 1, 0,
                                  "test/ComponentLibrary.jl:42-45")
+# dv .= esum with any number of states (one per output component)
+diffusionvertex_vec = DeviceFunction("diffusionvertex_f!", "vertex_f", A.ND_VM_DIFFUSION, None, 0,
+                                     "docs/examples/getting_started_with_network_dynamics.jl:129-132")
 kuramoto_vertex = DeviceFunction("kuramoto_vertex!", "vertex_f", A.ND_VM_KURAMOTO, 1, 1,
                                  "test/ComponentLibrary.jl:69-71")
 kuramoto_inertia = DeviceFunction("kuramoto_inertia!", "vertex_f", A.ND_VM_SWING, 2, 3,

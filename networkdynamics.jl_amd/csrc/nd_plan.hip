@@ -765,12 +765,16 @@ __global__ void k_edge_out(EDesc e, const double* __restrict__ u, const double* 
     }
     const int2 en = e.ends[q];
     const double* pe = p + e.param0 + static_cast<int64_t>(q) * e.pdim;
-    const double y = g_dst_rt(e.model, u[en.x], u[en.y], pe);
-    o[o0 + e.outdim_src] = y;
-    if (e.outdim_src > 0) {
-        if (e.wrap == ND_WRAP_SYMMETRIC) o[o0] = y;
-        else if (e.wrap == ND_WRAP_DIRECTED) o[o0] = 0.0;
-        else o[o0] = -y;
+    // component j of each side (componentwise models at output depth > 1: the endpoint
+    // outputs are consecutive states)
+    for (int j = 0; j < e.outdim_dst; ++j) {
+        const double y = g_dst_rt(e.model, u[en.x + j], u[en.y + j], pe);
+        o[o0 + e.outdim_src + j] = y;
+        if (j < e.outdim_src) {
+            if (e.wrap == ND_WRAP_SYMMETRIC) o[o0 + j] = y;
+            else if (e.wrap == ND_WRAP_DIRECTED) o[o0 + j] = 0.0;
+            else o[o0 + j] = -y;
+        }
     }
 }
 
@@ -956,6 +960,63 @@ __global__ void __launch_bounds__(256) k_vertex_gen_jvp(VDesc vd, const int32_t*
     }
 }
 
+// Vertex batches with d > 1 output components (vdepth = edepth = d): the componentwise
+// models only (diffusion vertices dv .= esum, getting_started_with_network_dynamics.jl:
+// 129-132, with diffusion edges e .= v_s .- v_d).  One thread per vertex; component j's
+// destination row (row0 + q*d + j) is summed in the reference order, every entry
+// evaluated on component j of both endpoints.
+template <bool STAGE>
+__global__ void __launch_bounds__(256) k_vertex_vec(VDesc vd, const int32_t* __restrict__ rowptr,
+                                                    const int2* __restrict__ slots, const uint8_t* __restrict__ slot_eb,
+                                                    const EDesc* __restrict__ ed, const double* __restrict__ u,
+                                                    const double* __restrict__ vo, const double* __restrict__ p,
+                                                    double* __restrict__ du, StageArgs sa) {
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= vd.count) return;
+    const int d = vd.depth;
+    const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * vd.dim;
+    if (vd.model == ND_VM_NOOP) {                // f does nothing: du keeps its zero fill
+        for (int j = 0; j < vd.dim; ++j) emit1<STAGE>(du, sa, x0 + j, 0.0);
+        return;
+    }
+    for (int j = 0; j < d; ++j) {
+        const double own = u[x0 + vd.mask[j]];
+        const int32_t row = vd.row0 + q * d + j;
+        double acc = 0.0;
+        for (int32_t k = rowptr[row]; k < rowptr[row + 1]; ++k)
+            acc = acc + slot_value_rt(ed[slot_eb ? slot_eb[k] : 0], slots[k], own, u, vo, p);
+        const double y = emit1<STAGE>(du, sa, x0 + j, acc);    // dv_j = esum_j
+        if (STAGE && sa.vnext) sa.vnext[static_cast<int64_t>(vd.out0) + static_cast<int64_t>(q) * d + j] = y;
+    }
+}
+
+// its dual-number form (du and J(u) v)
+__global__ void __launch_bounds__(256) k_vertex_vec_jvp(VDesc vd, const int32_t* __restrict__ rowptr,
+                                                        const int2* __restrict__ slots,
+                                                        const uint8_t* __restrict__ slot_eb,
+                                                        const EDesc* __restrict__ ed, const double* __restrict__ u,
+                                                        const double* __restrict__ v, const double* __restrict__ vo,
+                                                        const double* __restrict__ dvo, const double* __restrict__ p,
+                                                        double* __restrict__ du, double* __restrict__ jv) {
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= vd.count) return;
+    const int d = vd.depth;
+    const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * vd.dim;
+    if (vd.model == ND_VM_NOOP) {
+        for (int j = 0; j < vd.dim; ++j) { du[x0 + j] = 0.0; jv[x0 + j] = 0.0; }
+        return;
+    }
+    for (int j = 0; j < d; ++j) {
+        const Dual own{u[x0 + vd.mask[j]], v[x0 + vd.mask[j]]};
+        const int32_t row = vd.row0 + q * d + j;
+        Dual acc{0.0, 0.0};
+        for (int32_t k = rowptr[row]; k < rowptr[row + 1]; ++k)
+            acc = acc + slot_value_dual_rt(ed[slot_eb ? slot_eb[k] : 0], slots[k], own, u, v, vo, dvo, p);
+        du[x0 + j] = acc.v;
+        jv[x0 + j] = acc.d;
+    }
+}
+
 // collect_externals!(map, extbuf, u, o), external_inputs.jl:49-77: map > 0 reads u,
 // map < 0 reads o (1-based magnitudes)
 __global__ void k_collect_externals(const int32_t* __restrict__ map, int64_t n, double* __restrict__ extbuf,
@@ -1032,7 +1093,8 @@ __global__ void __launch_bounds__(256) k_edge_buf(EDesc e, const int32_t* __rest
     const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= e.count || e.model == ND_EM_NOOP) return;
     const int32_t g0 = egb[q];
-    const Dual vs{gbuf[g0], dgbuf[g0]}, vdn{gbuf[g0 + 1], dgbuf[g0 + 1]};
+    const int32_t gd = e.ode ? 1 : e.outdim_dst;  // gbuf: the src output's components, then the dst's
+    const Dual vs{gbuf[g0], dgbuf[g0]}, vdn{gbuf[g0 + gd], dgbuf[g0 + gd]};
     Dual pe[kMaxPar];
     dual_params(pe, p, dp, e.param0 + static_cast<int64_t>(q) * e.pdim, e.pdim);
     if (e.ode) {                                 // outputs: StateMasks, written before the gather
@@ -1052,13 +1114,16 @@ __global__ void __launch_bounds__(256) k_edge_buf(EDesc e, const int32_t* __rest
         return;
     }
     const int64_t o0 = static_cast<int64_t>(e.out0) + static_cast<int64_t>(q) * (e.outdim_src + e.outdim_dst);
-    const Dual y = g_dst_rt<Dual, Dual>(e.model, vs, vdn, pe);
-    o[o0 + e.outdim_src] = y.v;
-    dout[o0 + e.outdim_src] = y.d;
-    if (e.outdim_src > 0) {
-        const Dual ys = e.wrap == ND_WRAP_SYMMETRIC ? y : (e.wrap == ND_WRAP_DIRECTED ? Dual{0.0, 0.0} : -y);
-        o[o0] = ys.v;
-        dout[o0] = ys.d;
+    for (int j = 0; j < e.outdim_dst; ++j) {
+        const Dual y = g_dst_rt<Dual, Dual>(e.model, Dual{gbuf[g0 + j], dgbuf[g0 + j]},
+                                            Dual{gbuf[g0 + gd + j], dgbuf[g0 + gd + j]}, pe);
+        o[o0 + e.outdim_src + j] = y.v;
+        dout[o0 + e.outdim_src + j] = y.d;
+        if (j < e.outdim_src) {
+            const Dual ys = e.wrap == ND_WRAP_SYMMETRIC ? y : (e.wrap == ND_WRAP_DIRECTED ? Dual{0.0, 0.0} : -y);
+            o[o0 + j] = ys.v;
+            dout[o0 + j] = ys.d;
+        }
     }
 }
 
@@ -1074,6 +1139,15 @@ __global__ void __launch_bounds__(256) k_vertex_f_buf(VDesc vd, const double* __
     constexpr int DIMC = VModel<VM>::dim;
     const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= vd.count) return;
+    if (VM == ND_VM_DIFFUSION && vd.depth > 1) { // d components: dv .= esum
+        const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * vd.dim;
+        for (int j = 0; j < vd.depth; ++j) {
+            const int64_t r = static_cast<int64_t>(vd.row0) + static_cast<int64_t>(q) * vd.depth + j;
+            du[x0 + j] = agg ? agg[r] : 0.0;
+            jv[x0 + j] = agg ? dagg[r] : 0.0;
+        }
+        return;
+    }
     if constexpr (DIMC <= 0) {                   // no f (NoOp): du stays zero
         const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * vd.dim;
         for (int j = 0; j < vd.dim; ++j) {
@@ -1621,10 +1695,12 @@ int apply_gather_mode(nd_plan* pl) {
     pl->gkern = pick_group_kernel<false>(em, pl->wrap0, pl->multi, pl->kcache);
     pl->gkern_stage = pick_group_kernel<true>(em, pl->wrap0, pl->multi, pl->kcache);
     for (const VDesc& vd : pl->vdesc) {
-        pl->vkern.push_back(vd.generic ? pick_gen_kernel<false>(vd.model)
-                                       : pick_vertex_kernel<false>(vd.model, em, pl->wrap0, pl->multi, pl->kcache));
-        pl->vkern_stage.push_back(vd.generic ? pick_gen_kernel<true>(vd.model)
-                                             : pick_vertex_kernel<true>(vd.model, em, pl->wrap0, pl->multi, pl->kcache));
+        pl->vkern.push_back(vd.depth > 1 ? k_vertex_vec<false>
+                            : vd.generic ? pick_gen_kernel<false>(vd.model)
+                                         : pick_vertex_kernel<false>(vd.model, em, pl->wrap0, pl->multi, pl->kcache));
+        pl->vkern_stage.push_back(vd.depth > 1 ? k_vertex_vec<true>
+                                  : vd.generic ? pick_gen_kernel<true>(vd.model)
+                                               : pick_vertex_kernel<true>(vd.model, em, pl->wrap0, pl->multi, pl->kcache));
     }
     return rebuild_hubs(pl);
 }
@@ -1767,7 +1843,9 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
             return nd::set_error(ND_ERR_UNSUPPORTED, "vertex model %d: this output form has no HIP kernel", k.model);
         if (k.model != ND_VM_NOOP) {
             functional_vertex = true;
-            if (k.dim != VModelDim(k.model)) return nd::set_error(ND_ERR_ARGUMENT, "vertex model %d needs dim %d", k.model, VModelDim(k.model));
+            // with d > 1 output components, diffusion vertices carry d states (dv .= esum)
+            if (k.dim != VModelDim(k.model) && !(n.vdepth > 1 && k.model == ND_VM_DIFFUSION && k.dim == n.vdepth))
+                return nd::set_error(ND_ERR_ARGUMENT, "vertex model %d needs dim %d", k.model, VModelDim(k.model));
             if (k.extdim != vertex_ext_inputs(k.model))
                 return nd::set_error(ND_ERR_ARGUMENT, "vertex model %d reads %d external inputs, its kind declares %d",
                                      k.model, vertex_ext_inputs(k.model), k.extdim);
@@ -1805,9 +1883,27 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
             if (k.dim != 0) return nd::set_error(ND_ERR_UNSUPPORTED, "edge model %d has no state", k.model);
         }
     }
-    if ((functional_vertex || functional_edge) && (n.vdepth != 1 || (n.ne > 0 && n.edepth != 1)))
-        return nd::set_error(ND_ERR_UNSUPPORTED, "models with output depth other than 1 have no HIP kernel (vdepth=%d, edepth=%d)",
-                             n.vdepth, n.edepth);
+    if ((functional_vertex || functional_edge) && (n.vdepth != 1 || (n.ne > 0 && n.edepth != 1))) {
+        // d output components per vertex and per edge side: the componentwise models
+        // (diffusion vertices dv .= esum and diffusion edges e .= v_s .- v_d), every vertex
+        // output a StateMask of d consecutive states
+        const int d = n.vdepth;
+        bool ok = d >= 1 && d <= ND_MAX_OUT && (n.ne == 0 || n.edepth == d);
+        for (const nd::Batch& b : n.vb) {
+            const nd_kind& k = n.kinds[b.kind];
+            ok = ok && (k.model == ND_VM_NOOP || k.model == ND_VM_DIFFUSION) && k.outdim_dst == d &&
+                 k.ff == ND_FF_PURESTATEMAP && k.extdim == 0;
+            for (int j = 1; ok && j < d; ++j) ok = k.mask_dst[j] == k.mask_dst[0] + j;
+        }
+        for (const nd::Batch& b : n.eb) {
+            const nd_kind& k = n.kinds[b.kind];
+            ok = ok && (k.model == ND_EM_NOOP || k.model == ND_EM_DIFFUSION) && k.outdim_dst == d && k.dim == 0;
+        }
+        if (!ok)
+            return nd::set_error(ND_ERR_UNSUPPORTED, "output depth %d (vdepth=%d, edepth=%d): only componentwise models "
+                                 "(diffusion vertices and edges over consecutive states) have a HIP kernel",
+                                 d, n.vdepth, n.edepth);
+    }
     if (n.lastidx_extbuf > 0 && static_cast<int64_t>(n.extmap.size()) != n.lastidx_extbuf)
         return nd::set_error(ND_ERR_ARGUMENT, "the network has external inputs: set its ExtMap (nd_network_set_extmap) first");
     bool computed_out = false;
@@ -1844,9 +1940,10 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         for (int j = 0; j < ND_MAX_OUT; ++j) { e.mask_src[j] = k.mask_src[j] - 1; e.mask_dst[j] = k.mask_dst[j] - 1; }
         e.ext0 = static_cast<int32_t>(B.ext_first - 1);
         e.extdim = k.extdim;
-        if (n.vdepth == 1) {
-            // endpoint outputs as u-indices (StateMask outputs), or as o-indices where
-            // outputs are computed (get_buffers then reads them from o)
+        if (n.vdepth >= 1) {
+            // endpoint outputs (component 0; further components follow it) as u-indices
+            // (StateMask outputs), or as o-indices where outputs are computed (get_buffers
+            // then reads them from o)
             ends[b].resize(B.indices.size());
             for (size_t q = 0; q < B.indices.size(); ++q) {
                 const int64_t ei = B.indices[q] - 1;
@@ -1900,9 +1997,10 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
                     sl.y = is_src ? 1 : 0;
                 } else if (kk.model != ND_EM_NOOP) {
                     const int64_t other = is_src ? n.dst[e] - 1 : n.src[e] - 1;
-                    sl.x = static_cast<int32_t>(n.v_out_first(other) - 1);
+                    // component j of the other endpoint's output (j = 0 at output depth 1)
+                    sl.x = static_cast<int32_t>(n.v_out_first(other) - 1 + j);
                     sl.y = static_cast<int32_t>(((n.e_param_first(e) - 1) << 1) | (is_src ? 1 : 0));
-                    uidx[s] = static_cast<int32_t>(nd::vertex_out_u_index(n, other, 0));
+                    uidx[s] = static_cast<int32_t>(nd::vertex_out_u_index(n, other, j));
                 }
                 if (uidx[s] < 0) uidx[s] = sl.x;
                 slots[s] = sl;
@@ -1984,12 +2082,12 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         vd.count = static_cast<int32_t>(B.indices.size());
         vd.out0 = static_cast<int32_t>(B.out_first - 1);
         for (int j = 0; j < ND_MAX_OUT; ++j) vd.mask[j] = k.mask_dst[j] - 1;
-        vd.generic = k.model != ND_VM_NOOP && (vertex_model_generic(k.model) || generic_edges) ? 1 : 0;
+        vd.generic = (k.model != ND_VM_NOOP && (vertex_model_generic(k.model) || generic_edges)) || n.vdepth > 1 ? 1 : 0;
         vd.ext0 = static_cast<int32_t>(B.ext_first - 1);
         vd.extdim = k.extdim;
         vd.outkind = k.ff == ND_FF_NOFEEDFORWARD ? 1 : k.ff == ND_FF_PUREFEEDFORWARD ? 2 : 0;
         pl->vdesc.push_back(vd);
-        pl->vkern_jvp.push_back(vd.generic ? pick_gen_jvp(k.model)
+        pl->vkern_jvp.push_back(vd.depth > 1 ? k_vertex_vec_jvp : vd.generic ? pick_gen_jvp(k.model)
                                            : pick_jvp_kernel(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi));
     }
     // loopback: each injector's input is its hub's output (gen_loopback_map, post_utils.jl:205-217)

@@ -38,6 +38,17 @@ def ext_driven_edge(extin, wrapper="anti", name="e_noff"):
     return c.EdgeModel(f=c.feext, sym=["a"], g=w(1), extin=extin, name=name)
 
 
+# docs/examples/getting_started_with_network_dynamics.jl:218-221: d independent diffusions
+# on one graph -- vertices with d states and outputs 1:d (dv .= esum), edges whose d
+# outputs are v_s .- v_d (AntiSymmetric)
+def diffusion_vertex_multi(d, sym=None, name="diffusion_vertex_d"):
+    return c.VertexModel(f=c.diffusionvertex_vec, dim=d, sym=sym, g=c.StateMask(range(1, d + 1)), name=name)
+
+
+def diffusion_edge_multi(d, outsym=None, name="diffusion_edge_d"):
+    return c.EdgeModel(g=c.AntiSymmetric(c.diffusionedge), outdim=d, outsym=outsym, name=name)
+
+
 # test/ComponentLibrary.jl:46
 def diffusion_vertex():
     return c.VertexModel(f=c.diffusionvertex, dim=1, g=1, name="diffusion_vertex")

@@ -58,9 +58,13 @@ def jac_prototype(nw):
     v, e = nw.ranges()
     ost = _output_states(nw)
     oedge = _output_edges(nw)
-    vout_state = np.full(nw.nv(), -1, np.int64)
-    has_out = v[:, 1, 1] >= v[:, 1, 0]
-    vout_state[has_out] = ost[v[has_out, 1, 0] - 1]
+    # states behind each vertex's output components (-1: none or not a StateMask)
+    depth = max(int(nw.info.vdepth), 1)
+    vout_states = np.full((nw.nv(), depth), -1, np.int64)
+    for j in range(depth):
+        has = v[:, 1, 1] >= v[:, 1, 0] + j
+        vout_states[has, j] = ost[v[has, 1, 0] - 1 + j]
+    vout_state = vout_states[:, 0]
     rows, cols = [], []
 
     def block(r0, n, c):
@@ -94,9 +98,11 @@ def jac_prototype(nw):
             st = ost[k_out]
             ff = st < 0
             nb = np.where(ff, -1, st)
-            src_s = np.where(ff, vout_state[nw.g.src[eid] - 1], -1)
-            dst_s = np.where(ff, vout_state[nw.g.dst[eid] - 1], -1)
-            for c in (nb, src_s, dst_s):
+            cols_ = [nb]
+            for j in range(depth):              # a feed-forward entry reads every output component
+                cols_.append(np.where(ff, vout_states[nw.g.src[eid] - 1, j], -1))
+                cols_.append(np.where(ff, vout_states[nw.g.dst[eid] - 1, j], -1))
+            for c in cols_:
                 rr = r0[owner][:, None] + np.arange(d)[None, :]
                 cc = np.repeat(c[:, None], d, axis=1)
                 ok = cc.reshape(-1) >= 0

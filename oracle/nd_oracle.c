@@ -387,8 +387,9 @@ static int hasff(const nd_kind* k) { return k->ff == ND_FF_PUREFEEDFORWARD || k-
 static void vertex_f(const nd_kind* k, double* du, const double* u, const double* acc, const double* ext,
                      const double* p, double t) {
     switch (k->model) {
-    case ND_VM_DIFFUSION:          /* ComponentLibrary.jl:42-45 */
-        du[0] = acc[0];
+    case ND_VM_DIFFUSION:          /* ComponentLibrary.jl:42-45; with d outputs dv .= esum
+                                      (getting_started_with_network_dynamics.jl:129-133) */
+        for (int j = 0; j < k->dim; ++j) du[j] = acc[j];
         break;
     case ND_VM_KURAMOTO:           /* ComponentLibrary.jl:69-71 */
         du[0] = p[0] + acc[0];
@@ -808,7 +809,7 @@ void ndo_rhs_scale(ndo_net* nw, double* S, const double* u, const double* p, dou
             const double* q = p + pr.first - 1;
             double a = agg[ir.first - 1];
             switch (k->model) {
-            case ND_VM_DIFFUSION: s[0] = a; break;
+            case ND_VM_DIFFUSION: for (int j = 0; j < k->dim; ++j) s[j] = agg[ir.first - 1 + j]; break;
             case ND_VM_KURAMOTO: s[0] = fabs(q[0]) + a; break;
             case ND_VM_SWING: s[0] = fabs(x[1]); s[1] = fabs(1.0 / q[0]) * (fabs(q[2]) + fabs(q[1] * x[1]) + a); break;
             case ND_VM_SWING_UNIT_DAMPING: s[0] = fabs(x[1]); s[1] = fabs(q[0]) + fabs(x[1]) + a; break;
