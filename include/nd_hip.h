@@ -76,6 +76,8 @@ enum {
  *   INDUCTOR_INJECTOR  test/loopback_test.jl:80-92           di = in / L, output i (StateMask), p = [L]
  *   CRL                test/loopback_test.jl:113-131         dv = (acc - v/R - iL) / C; diL = v / L,
  *                                                            output v, p = [C, R, L]
+ *   FHN                docs/examples/directed_and_weighted_graphs.jl:89-95  FitzHugh-Nagumo:
+ *                      du = u - u^3/3 - v + acc; dv = (u - a)*eps, output u, p = [a, eps]
  *   NOOP               f does nothing (aggregator tests, test/aggregators_test.jl:15-18)  */
 enum {
     ND_VM_NOOP = 0,
@@ -90,7 +92,8 @@ enum {
     ND_VM_VOLTAGE_SOURCE = 9,
     ND_VM_RESISTOR_INJECTOR = 10,
     ND_VM_INDUCTOR_INJECTOR = 11,
-    ND_VM_CRL = 12
+    ND_VM_CRL = 12,
+    ND_VM_FHN = 13
 };
 
 /* Edge models.  Reference twins:
@@ -111,6 +114,8 @@ enum {
  *                        y_src = -e; Symmetric(1); Directed(1)), component_functions.jl:117-175
  *   EXT_DRIVEN           test/external_inputs_test.jl:45-47: de = ext[1] (one external input),
  *                        any StateMask output
+ *   ELECTRICAL           docs/examples/directed_and_weighted_graphs.jl:97-101: Directed gap
+ *                        junction y = w*(v_s - v_d)*sigma, p = [w, sigma]
  *   NOOP                 g does nothing (test/aggregators_test.jl:20-30)
  * ODE edge models (state dimension 2 for ODE_SIN_DIFFUSION, ODE_SIN_DIFFUSION_FAST and
  * ODE_LINEAR, 1 for ODE_LAG and EXT_DRIVEN) take Fiducial StateMask outputs or a StateMask
@@ -128,7 +133,8 @@ enum {
     ND_EM_RESISTOR = 9,
     ND_EM_LOOPBACK = 10,
     ND_EM_ODE_LAG = 11,
-    ND_EM_EXT_DRIVEN = 12
+    ND_EM_EXT_DRIVEN = 12,
+    ND_EM_ELECTRICAL = 13
 };
 
 #define ND_MAX_OUT 4

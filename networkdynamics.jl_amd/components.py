@@ -560,6 +560,12 @@ resistor_g = DeviceFunction("resistor_g", "edge_g", A.ND_EM_RESISTOR, 0, 1, "doc
 f_load = DeviceFunction("f_load", "vertex_f", A.ND_VM_CAPACITOR_SINK, 1, 1, "docs/src/external_inputs.md:76-82")
 f_source = DeviceFunction("f_source", "vertex_f", A.ND_VM_PI_SOURCE, 2, 4, "docs/src/external_inputs.md:86-99",
                           ext=1)
+# docs/examples/directed_and_weighted_graphs.jl:89-101: FitzHugh-Nagumo neurons coupled by
+# directed, weighted electrical gap junctions
+fhn_electrical_vertex = DeviceFunction("fhn_electrical_vertex!", "vertex_f", A.ND_VM_FHN, 2, 2,
+                                       "docs/examples/directed_and_weighted_graphs.jl:89-93")
+electrical_edge_g = DeviceFunction("electrical_edge!", "edge_g", A.ND_EM_ELECTRICAL, 0, 2,
+                                   "docs/examples/directed_and_weighted_graphs.jl:97-100")
 # test/loopback_test.jl:19-131: the components of the injector network and of its merged twin
 capacitor_f = DeviceFunction("capacitor", "vertex_f", A.ND_VM_CAPACITOR, 1, 1, "test/loopback_test.jl:53-63")
 voltage_source_g = DeviceFunction("voltage_source", "vertex_g", A.ND_VM_VOLTAGE_SOURCE, 0, 1,

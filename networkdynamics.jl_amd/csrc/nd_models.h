@@ -91,6 +91,7 @@ __device__ __forceinline__ S g_dst(S vs, S vd, const P* __restrict__ pe) {
     else if constexpr (EM == ND_EM_DIFFUSION_FIDUCIAL) return pe[0] * (vs - vd);  // ComponentLibrary.jl:22-25
     else if constexpr (EM == ND_EM_RESISTOR) return (vs - vd) / pe[0];            // external_inputs.md:66-68
     else if constexpr (EM == ND_EM_LOOPBACK) return -1.0 * vs;                     // post_utils.jl:105-108
+    else if constexpr (EM == ND_EM_ELECTRICAL) return pe[0] * (vs - vd) * pe[1];  // directed_and_weighted_graphs.jl:97-100
     else return nan_of<S>();
 }
 
@@ -104,6 +105,7 @@ __device__ __forceinline__ S g_dst_rt(int em, S vs, S vd, const P* __restrict__ 
     case ND_EM_DIFFUSION_FIDUCIAL: return g_dst<ND_EM_DIFFUSION_FIDUCIAL, S, P>(vs, vd, pe);
     case ND_EM_RESISTOR: return g_dst<ND_EM_RESISTOR, S, P>(vs, vd, pe);
     case ND_EM_LOOPBACK: return g_dst<ND_EM_LOOPBACK, S, P>(vs, vd, pe);
+    case ND_EM_ELECTRICAL: return g_dst<ND_EM_ELECTRICAL, S, P>(vs, vd, pe);
     default: return nan_of<S>();
     }
 }
@@ -111,6 +113,7 @@ __device__ __forceinline__ S g_dst_rt(int em, S vs, S vd, const P* __restrict__ 
 template <int VM>
 struct VModel { static constexpr int dim = -1; };
 template <> struct VModel<ND_VM_DIFFUSION> { static constexpr int dim = 1; };
+template <> struct VModel<ND_VM_FHN> { static constexpr int dim = 2; };
 template <> struct VModel<ND_VM_KURAMOTO> { static constexpr int dim = 1; };
 template <> struct VModel<ND_VM_SWING> { static constexpr int dim = 2; };
 template <> struct VModel<ND_VM_SWING_UNIT_DAMPING> { static constexpr int dim = 2; };
@@ -143,6 +146,10 @@ __device__ __forceinline__ void vertex_f(S* __restrict__ dx, const S* __restrict
         dx[1] = d;
     } else if constexpr (VM == ND_VM_LOAD) {
         dx[0] = 1.0 / pv[0] * (pv[1] + acc);
+    } else if constexpr (VM == ND_VM_FHN) {                // directed_and_weighted_graphs.jl:89-93
+        const S a = x[0], b = x[1];
+        dx[0] = a - a * a * a / 3.0 - b + acc;              // v[1]^3 is v[1]*v[1]*v[1] (literal power)
+        dx[1] = (a - pv[0]) * pv[1];
     } else {                                               // f does nothing: du keeps its zero fill
         for (int j = 0; j < dim; ++j) dx[j] = zero_of<S>();
     }
@@ -209,6 +216,7 @@ template <> struct VPar<ND_VM_VOLTAGE_SOURCE> { static constexpr int np = 1; };
 template <> struct VPar<ND_VM_RESISTOR_INJECTOR> { static constexpr int np = 1; };
 template <> struct VPar<ND_VM_INDUCTOR_INJECTOR> { static constexpr int np = 1; };
 template <> struct VPar<ND_VM_CRL> { static constexpr int np = 3; };
+template <> struct VPar<ND_VM_FHN> { static constexpr int np = 2; };
 
 // external inputs each vertex model reads
 template <int VM> struct VExt { static constexpr int n = 0; };

@@ -1292,6 +1292,7 @@ VertexKernel pick_gen_kernel(int vm) {
     case ND_VM_RESISTOR_INJECTOR: return k_vertex_gen<ND_VM_RESISTOR_INJECTOR, STAGE>;
     case ND_VM_INDUCTOR_INJECTOR: return k_vertex_gen<ND_VM_INDUCTOR_INJECTOR, STAGE>;
     case ND_VM_CRL: return k_vertex_gen<ND_VM_CRL, STAGE>;
+    case ND_VM_FHN: return k_vertex_gen<ND_VM_FHN, STAGE>;
     default: return k_vertex_rhs<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false, STAGE, false>;
     }
 }
@@ -1312,6 +1313,7 @@ VertexBufKernel pick_vertex_f_buf(int vm) {
     case ND_VM_RESISTOR_INJECTOR: return k_vertex_f_buf<ND_VM_RESISTOR_INJECTOR>;
     case ND_VM_INDUCTOR_INJECTOR: return k_vertex_f_buf<ND_VM_INDUCTOR_INJECTOR>;
     case ND_VM_CRL: return k_vertex_f_buf<ND_VM_CRL>;
+    case ND_VM_FHN: return k_vertex_f_buf<ND_VM_FHN>;
     default: return k_vertex_f_buf<ND_VM_NOOP>;
     }
 }
@@ -1375,6 +1377,7 @@ JvpKernel pick_gen_jvp(int vm) {
     case ND_VM_RESISTOR_INJECTOR: return k_vertex_gen_jvp<ND_VM_RESISTOR_INJECTOR>;
     case ND_VM_INDUCTOR_INJECTOR: return k_vertex_gen_jvp<ND_VM_INDUCTOR_INJECTOR>;
     case ND_VM_CRL: return k_vertex_gen_jvp<ND_VM_CRL>;
+    case ND_VM_FHN: return k_vertex_gen_jvp<ND_VM_FHN>;
     default: return k_vertex_jvp<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false>;
     }
 }
@@ -1402,13 +1405,14 @@ EdgeOdeKernel pick_edge_ode(int em) {
     }
 }
 
-bool vertex_model_known(int vm) { return vm >= ND_VM_NOOP && vm <= ND_VM_CRL; }
+bool vertex_model_known(int vm) { return vm >= ND_VM_NOOP && vm <= ND_VM_FHN; }
 int VModelDim(int vm) {
     switch (vm) {
     case ND_VM_SWING:
     case ND_VM_SWING_UNIT_DAMPING:
     case ND_VM_PI_SOURCE:
-    case ND_VM_CRL: return 2;
+    case ND_VM_CRL:
+    case ND_VM_FHN: return 2;
     case ND_VM_DIFFUSION:
     case ND_VM_KURAMOTO:
     case ND_VM_LOAD:
@@ -1428,9 +1432,9 @@ int vertex_model_ff(int vm) {
 }
 int vertex_ext_inputs(int vm) { return vm == ND_VM_PI_SOURCE ? VExt<ND_VM_PI_SOURCE>::n : 0; }
 // vertex models only the one-thread-per-vertex kernels carry
-bool vertex_model_generic(int vm) { return vm >= ND_VM_CAPACITOR_SINK && vm <= ND_VM_CRL; }
-bool edge_model_known(int em) { return em >= ND_EM_NOOP && em <= ND_EM_EXT_DRIVEN; }
-bool edge_model_generic(int em) { return em == ND_EM_RESISTOR || em == ND_EM_LOOPBACK; }
+bool vertex_model_generic(int vm) { return vm >= ND_VM_CAPACITOR_SINK && vm <= ND_VM_FHN; }
+bool edge_model_known(int em) { return em >= ND_EM_NOOP && em <= ND_EM_ELECTRICAL; }
+bool edge_model_generic(int em) { return em == ND_EM_RESISTOR || em == ND_EM_LOOPBACK || em == ND_EM_ELECTRICAL; }
 int edge_params(int em) {
     switch (em) {
     case ND_EM_KURAMOTO: return EModel<ND_EM_KURAMOTO>::np;

@@ -129,6 +129,27 @@ def loopback_connection(name="loopback"):
     return c.EdgeModel(g=c.Directed(c.loopback_g), outdim=1, name=name)
 
 
+# docs/examples/directed_and_weighted_graphs.jl:89-101
+def fhn_vertex(a=0.5, eps=0.05, name="fhn"):
+    return c.VertexModel(f=c.fhn_electrical_vertex, g=1, sym=["u", "v"], psym=[("a", a), ("eps", eps)], name=name)
+
+
+def electrical_edge(sigma=0.5, name="electrical_edge"):
+    return c.EdgeModel(g=c.Directed(c.electrical_edge_g), outdim=1, psym=["weight", ("sigma", sigma)], name=name)
+
+
+# docs/examples/heterogeneous_system.jl:75-105: a vertex whose output is its parameter
+# (NoFeedForward g(out, p) = p, the voltage source's equation) and the Kuramoto vertex with
+# inertia and unit damping (benchmark/benchmark_models.jl:37-41 with psym omega0)
+def static_vertex(theta_fix=0.0, name="static"):
+    return c.VertexModel(g=c.voltage_source_g, psym=[("theta_fix", theta_fix)], name=name)
+
+
+def kuramoto_inertia(name="inertia"):
+    return c.VertexModel(f=c.kuramoto_inertia_unit_damping, sym=["theta", "omega"], psym=["omega0"], g=1,
+                         name=name)
+
+
 class bench:
     """benchmark/benchmark_models.jl:5-42."""
 

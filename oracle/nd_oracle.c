@@ -391,6 +391,10 @@ static void vertex_f(const nd_kind* k, double* du, const double* u, const double
                                       (getting_started_with_network_dynamics.jl:129-133) */
         for (int j = 0; j < k->dim; ++j) du[j] = acc[j];
         break;
+    case ND_VM_FHN:                /* directed_and_weighted_graphs.jl:89-93; v^3 = v*v*v */
+        du[0] = u[0] - u[0] * u[0] * u[0] / 3.0 - u[1] + acc[0];
+        du[1] = (u[0] - p[0]) * p[1];
+        break;
     case ND_VM_KURAMOTO:           /* ComponentLibrary.jl:69-71 */
         du[0] = p[0] + acc[0];
         break;
@@ -447,6 +451,9 @@ static void edge_gdst(const nd_kind* k, double* y, const double* vs, const doubl
         break;
     case ND_EM_DIFFUSION_WEIGHTED: /* ComponentLibrary.jl:8-10 */
         y[0] = p[0] * (vs[0] - vd[0]);
+        break;
+    case ND_EM_ELECTRICAL:         /* directed_and_weighted_graphs.jl:97-100 */
+        y[0] = p[0] * (vs[0] - vd[0]) * p[1];
         break;
     case ND_EM_KURAMOTO:           /* ComponentLibrary.jl:51-53 */
         y[0] = p[0] * sin(vs[0] - vd[0]);
@@ -811,6 +818,10 @@ void ndo_rhs_scale(ndo_net* nw, double* S, const double* u, const double* p, dou
             switch (k->model) {
             case ND_VM_DIFFUSION: for (int j = 0; j < k->dim; ++j) s[j] = agg[ir.first - 1 + j]; break;
             case ND_VM_KURAMOTO: s[0] = fabs(q[0]) + a; break;
+            case ND_VM_FHN:
+                s[0] = fabs(x[0]) + fabs(x[0] * x[0] * x[0] / 3.0) + fabs(x[1]) + a;
+                s[1] = fabs(x[0] * q[1]) + fabs(q[0] * q[1]);
+                break;
             case ND_VM_SWING: s[0] = fabs(x[1]); s[1] = fabs(1.0 / q[0]) * (fabs(q[2]) + fabs(q[1] * x[1]) + a); break;
             case ND_VM_SWING_UNIT_DAMPING: s[0] = fabs(x[1]); s[1] = fabs(q[0]) + fabs(x[1]) + a; break;
             case ND_VM_LOAD: s[0] = fabs(1.0 / q[0]) * (fabs(q[1]) + a); break;
