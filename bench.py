@@ -33,7 +33,7 @@ def _traffic_from_profile():
     return None
 
 
-def cpu_baseline(nw, p, u0, seconds=10.0):
+def cpu_baseline(nw, p, u0, seconds=10.0, what="full N=1e6 swing network"):
     """ThreadedExecution{true} + ThreadedAggregator(+) restated in C/OpenMP, all host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -57,8 +57,33 @@ def cpu_baseline(nw, p, u0, seconds=10.0):
     el = time.perf_counter() - t0
     return dict(value=n / el, unit="RHS evals/s", cores=cores, kind="port",
                 best_ms=best * 1e3,
-                sample=f"full N=1e6 swing network, {n} calls in {el:.1f} s, {cores} OpenMP threads "
+                sample=f"{what}, {n} calls in {el:.1f} s, {cores} OpenMP threads "
                        "(C restatement of ThreadedExecution{true}+ThreadedAggregator(+))")
+
+
+def cpu_baseline_c1(ND, seconds=10.0):
+    """The C1 workload on the host, SequentialExecution restated in C (one thread): RHS
+    calls for about half the budget, then whole RK4 runs (dt = 0.01 to t = 10)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    nw, p, u0 = ND.workloads.kuramoto_er(1000, 4000, 3.0, seed=1)
+    on = oracle.OracleNet(nw.nv(), nw.g.src, nw.g.dst, nw.kinds, nw.vertex_kind, nw.edge_kind)
+    on.rhs(u0, p)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds / 2 or n < 3:
+        on.rhs(u0, p)
+        n += 1
+    t_rhs = (time.perf_counter() - t0) / n
+    m, t1 = 0, time.perf_counter()
+    while time.perf_counter() - t1 < seconds / 2 or m < 1:
+        on.rk4(u0, p, 0.0, 0.01, 1000)
+        m += 1
+    t_rk4 = (time.perf_counter() - t1) / m
+    return dict(rhs_us=t_rhs * 1e6, rhs_us_inside_rk4=t_rk4 / 4000 * 1e6, rk4_ms=t_rk4 * 1e3, cores=1,
+                kind="port",
+                sample=f"the C1 network: {n} RHS calls through the ctypes binding (its call overhead "
+                       f"included), then {m} RK4 runs of 1000 steps in C, one thread (C restatement of "
+                       "SequentialExecution)")
 
 
 def main():
@@ -93,7 +118,13 @@ def main():
         ex = import_module(ND.__name__ + ".experiments")
         fn = dict(c1=ex.c1_rk4, c3=ex.c3_rhs, c4=ex.c4_ensemble, c5=ex.c5_warmstart, extras=ex.extras)[args.config]
         kw = {k: float(v) if "." in v or "e" in v else int(v) for k, v in (x.split("=", 1) for x in args.set)}
-        print(json.dumps(dict(config=args.config, **fn(dev, **kw))))
+        res = fn(dev, **kw)
+        if not args.no_cpu and args.config == "c1":
+            res["cpu_baseline"] = cpu_baseline_c1(ND, args.cpu_seconds)
+        elif not args.no_cpu and args.config == "c3":
+            nw3, p3, u3 = ND.workloads.heterogeneous_rgg(200_000, seed=1)
+            res["cpu_baseline"] = cpu_baseline(nw3, p3, u3, args.cpu_seconds, "the C3 network (N=2e5 RGG)")
+        print(json.dumps(dict(config=args.config, **res)))
         return
     nw, p, u0 = ND.workloads.swing_ba(args.n, 2, seed=1)
     N, E = nw.nv(), nw.ne()
