@@ -78,6 +78,11 @@ enum {
  *                                                            output v, p = [C, R, L]
  *   FHN                docs/examples/directed_and_weighted_graphs.jl:89-95  FitzHugh-Nagumo:
  *                      du = u - u^3/3 - v + acc; dv = (u - a)*eps, output u, p = [a, eps]
+ *   TRUSS_FREE         docs/examples/stress_on_truss.jl:55-62  a free truss node, states (vx, vy, x, y),
+ *                      outputs (x, y), inputs (Fx, Fy): dv = (F - gamma v)/M, dvy -= g, dpos = v,
+ *                      p = [M, gamma, g]
+ *   TRUSS_FIXED        docs/examples/stress_on_truss.jl:45-48  a fixed node: no state, outputs (x, y) = p
+ *                      (NoFeedForward), p = [xfix, yfix]
  *   NOOP               f does nothing (aggregator tests, test/aggregators_test.jl:15-18)  */
 enum {
     ND_VM_NOOP = 0,
@@ -93,7 +98,9 @@ enum {
     ND_VM_RESISTOR_INJECTOR = 10,
     ND_VM_INDUCTOR_INJECTOR = 11,
     ND_VM_CRL = 12,
-    ND_VM_FHN = 13
+    ND_VM_FHN = 13,
+    ND_VM_TRUSS_FREE = 14,
+    ND_VM_TRUSS_FIXED = 15
 };
 
 /* Edge models.  Reference twins:
@@ -116,6 +123,9 @@ enum {
  *                        any StateMask output
  *   ELECTRICAL           docs/examples/directed_and_weighted_graphs.jl:97-101: Directed gap
  *                        junction y = w*(v_s - v_d)*sigma, p = [w, sigma]
+ *   BEAM                 docs/examples/stress_on_truss.jl:77-86: AntiSymmetric spring between two
+ *                        positions, F = K*(L - d)*(dst - src)/d, d = |dst - src|, two outputs
+ *                        (Fx, Fy), p = [K, L]
  *   NOOP                 g does nothing (test/aggregators_test.jl:20-30)
  * ODE edge models (state dimension 2 for ODE_SIN_DIFFUSION, ODE_SIN_DIFFUSION_FAST and
  * ODE_LINEAR, 1 for ODE_LAG and EXT_DRIVEN) take Fiducial StateMask outputs or a StateMask
@@ -134,7 +144,8 @@ enum {
     ND_EM_LOOPBACK = 10,
     ND_EM_ODE_LAG = 11,
     ND_EM_EXT_DRIVEN = 12,
-    ND_EM_ELECTRICAL = 13
+    ND_EM_ELECTRICAL = 13,
+    ND_EM_BEAM = 14
 };
 
 #define ND_MAX_OUT 4
@@ -265,8 +276,8 @@ typedef struct nd_plan nd_plan;
 
 /* Uploads the network's index structures to the current HIP device and selects
  * the fused kernels.  ND_ERR_UNSUPPORTED if a model has no HIP kernel (ODE edges
- * next to computed vertex outputs; output depth > 1 except for the componentwise
- * diffusion vertices and edges over consecutive states);
+ * next to computed vertex outputs; output depth > 1 except for diffusion vertices and
+ * edges over consecutive states and, at depth 2, the truss nodes and beams);
  * ND_ERR_ARGUMENT if the network has external inputs and no ExtMap
  * (nd_network_set_extmap) or a model's external-input count differs from its kind's.
  * External inputs (of vertices and of ODE edges) resolve to state indices: a vertex

@@ -15,10 +15,10 @@ ND_WRAP_NONE, ND_WRAP_ANTISYMMETRIC, ND_WRAP_SYMMETRIC, ND_WRAP_DIRECTED, ND_WRA
 ND_FF_PURESTATEMAP, ND_FF_PUREFEEDFORWARD, ND_FF_FEEDFORWARD, ND_FF_NOFEEDFORWARD = range(4)
 (ND_VM_NOOP, ND_VM_DIFFUSION, ND_VM_KURAMOTO, ND_VM_SWING, ND_VM_SWING_UNIT_DAMPING, ND_VM_LOAD,
  ND_VM_CAPACITOR_SINK, ND_VM_PI_SOURCE, ND_VM_CAPACITOR, ND_VM_VOLTAGE_SOURCE, ND_VM_RESISTOR_INJECTOR,
- ND_VM_INDUCTOR_INJECTOR, ND_VM_CRL, ND_VM_FHN) = range(14)
+ ND_VM_INDUCTOR_INJECTOR, ND_VM_CRL, ND_VM_FHN, ND_VM_TRUSS_FREE, ND_VM_TRUSS_FIXED) = range(16)
 (ND_EM_NOOP, ND_EM_DIFFUSION, ND_EM_DIFFUSION_WEIGHTED, ND_EM_KURAMOTO, ND_EM_STATIC_LINE,
  ND_EM_DIFFUSION_FIDUCIAL, ND_EM_ODE_SIN_DIFFUSION, ND_EM_ODE_SIN_DIFFUSION_FAST, ND_EM_ODE_LINEAR,
- ND_EM_RESISTOR, ND_EM_LOOPBACK, ND_EM_ODE_LAG, ND_EM_EXT_DRIVEN, ND_EM_ELECTRICAL) = range(14)
+ ND_EM_RESISTOR, ND_EM_LOOPBACK, ND_EM_ODE_LAG, ND_EM_EXT_DRIVEN, ND_EM_ELECTRICAL, ND_EM_BEAM) = range(15)
 ND_MAX_OUT = 4
 ND_MAX_EXT = 8
 ND_MASS_IDENTITY, ND_MASS_UNIFORM, ND_MASS_DIAGONAL, ND_MASS_DENSE = range(4)

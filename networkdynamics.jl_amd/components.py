@@ -390,7 +390,8 @@ class VertexModel(ComponentModel):
             raise A.UnsupportedError(f"{g.name} has no state function in the library")
         if (dim or 0) != 0 or sym:
             raise A.ArgumentError(f"{g.name} has no states")
-        if outdim is not None and outdim != 1:
+        od = getattr(g, "outdim", 1)
+        if outdim is not None and outdim != od:
             raise A.ArgumentError("outdim and g disagree")
         self.f, self._gfn, self.model = None, g, g.model
         self.sym, self.sym_default, self.dim = [], [], 0
@@ -406,7 +407,7 @@ class VertexModel(ComponentModel):
         self.g = g
         self.ff = g.ff
         self.wrapper = A.ND_WRAP_NONE
-        self.outdim_src, self.outdim_dst = 0, 1
+        self.outdim_src, self.outdim_dst = 0, od
         self.mask_src, self.mask_dst = (), ()
         self.name = name or g.name
         self.mass_matrix = _normalize_mass(mass_matrix, 0)
@@ -560,6 +561,14 @@ resistor_g = DeviceFunction("resistor_g", "edge_g", A.ND_EM_RESISTOR, 0, 1, "doc
 f_load = DeviceFunction("f_load", "vertex_f", A.ND_VM_CAPACITOR_SINK, 1, 1, "docs/src/external_inputs.md:76-82")
 f_source = DeviceFunction("f_source", "vertex_f", A.ND_VM_PI_SOURCE, 2, 4, "docs/src/external_inputs.md:86-99",
                           ext=1)
+# docs/examples/stress_on_truss.jl:45-84: the truss's free and fixed nodes (two outputs: the
+# position) and its beams (two outputs: the force)
+truss_free_f = DeviceFunction("free_f", "vertex_f", A.ND_VM_TRUSS_FREE, 4, 3, "docs/examples/stress_on_truss.jl:55-61")
+truss_fixed_g = DeviceFunction("fixed_g", "vertex_g", A.ND_VM_TRUSS_FIXED, 0, 2, "docs/examples/stress_on_truss.jl:45-48",
+                               ff=NoFeedForward())
+truss_fixed_g.outdim = 2
+beam_g = DeviceFunction("edge_g!", "edge_g", A.ND_EM_BEAM, 0, 2, "docs/examples/stress_on_truss.jl:77-84")
+
 # docs/examples/directed_and_weighted_graphs.jl:89-101: FitzHugh-Nagumo neurons coupled by
 # directed, weighted electrical gap junctions
 fhn_electrical_vertex = DeviceFunction("fhn_electrical_vertex!", "vertex_f", A.ND_VM_FHN, 2, 2,

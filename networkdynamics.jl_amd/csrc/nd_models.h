@@ -8,7 +8,8 @@
 
 namespace nd {
 
-using ::sin;   // the double overloads stay visible next to sin(Dual)
+using ::sin;   // the double overloads stay visible next to sin(Dual) and sqrt(Dual)
+using ::sqrt;
 
 // First-order dual number v + d*eps (one tangent direction), the forward-mode scalar
 // the Jacobian-vector product evaluates the models with.  The model functions take the
@@ -31,6 +32,10 @@ __device__ __forceinline__ Dual operator/(Dual a, double c) { return {a.v / c, a
 __device__ __forceinline__ Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
 __device__ __forceinline__ Dual operator/(double c, Dual b) { return {c / b.v, -c * b.d / (b.v * b.v)}; }
 __device__ __forceinline__ Dual sin(Dual a) { return {::sin(a.v), ::cos(a.v) * a.d}; }
+__device__ __forceinline__ Dual sqrt(Dual a) {
+    const double r = ::sqrt(a.v);
+    return {r, a.d / (2.0 * r)};
+}
 __device__ __forceinline__ Dual dual_nan() { return {__builtin_nan(""), __builtin_nan("")}; }
 
 template <typename S> __device__ __forceinline__ S nan_of();
@@ -114,6 +119,8 @@ template <int VM>
 struct VModel { static constexpr int dim = -1; };
 template <> struct VModel<ND_VM_DIFFUSION> { static constexpr int dim = 1; };
 template <> struct VModel<ND_VM_FHN> { static constexpr int dim = 2; };
+template <> struct VModel<ND_VM_TRUSS_FREE> { static constexpr int dim = 4; };
+template <> struct VModel<ND_VM_TRUSS_FIXED> { static constexpr int dim = 0; };
 template <> struct VModel<ND_VM_KURAMOTO> { static constexpr int dim = 1; };
 template <> struct VModel<ND_VM_SWING> { static constexpr int dim = 2; };
 template <> struct VModel<ND_VM_SWING_UNIT_DAMPING> { static constexpr int dim = 2; };
@@ -185,14 +192,36 @@ __device__ __forceinline__ void vertex_f_ext(S* __restrict__ dx, const S* __rest
 // Computed vertex outputs: NoFeedForward g(out, x, p, t) and the PureFeedForward g of an
 // injector, g(out, in, p, t), whose input is its hub's output (the loopback copy).
 template <typename S, typename P = double>
-__device__ __forceinline__ S vertex_out_nff(int vm, const P* __restrict__ pv) {
+__device__ __forceinline__ S vertex_out_nff(int vm, const P* __restrict__ pv, int j = 0) {
     if (vm == ND_VM_VOLTAGE_SOURCE) return S{} + pv[0];    // loopback_test.jl:31: p.v ~ V
+    if (vm == ND_VM_TRUSS_FIXED) return S{} + pv[j];       // stress_on_truss.jl:45-47: pos .= p
     return nan_of<S>();
 }
 template <typename S, typename P = double>
 __device__ __forceinline__ S vertex_out_ff(int vm, S in, const P* __restrict__ pv) {
     if (vm == ND_VM_RESISTOR_INJECTOR) return in / pv[0];   // loopback_test.jl:75: p.i ~ p.v / R
     return nan_of<S>();
+}
+
+// Component j of a two-output vector edge model's dst output, from the source and
+// destination outputs ps, pd (stress_on_truss.jl:77-84: the beam force
+// F = K*(L - d)*(pd - ps)/d, d = |pd - ps|; x^2 is x*x)
+template <typename S, typename P = double>
+__device__ __forceinline__ S beam_dst(const S* ps, const S* pd, const P* __restrict__ pe, int j) {
+    const S dx = pd[0] - ps[0], dy = pd[1] - ps[1];
+    const S d = sqrt(dx * dx + dy * dy);
+    const S f = pe[0] * (pe[1] - d);
+    return j == 0 ? f * dx / d : f * dy / d;
+}
+
+// f of a free truss node (stress_on_truss.jl:55-61): states (vx, vy, x, y), inputs (Fx, Fy)
+template <typename S, typename P = double>
+__device__ __forceinline__ void truss_free_f(S* dx, const S* x, const S* acc, const P* __restrict__ pv) {
+    dx[0] = (acc[0] - pv[1] * x[0]) / pv[0];
+    dx[1] = (acc[1] - pv[1] * x[1]) / pv[0];
+    dx[1] = dx[1] - pv[2];
+    dx[2] = x[0];
+    dx[3] = x[1];
 }
 
 // Parameters each edge model reads (p entries per edge).
@@ -217,6 +246,8 @@ template <> struct VPar<ND_VM_RESISTOR_INJECTOR> { static constexpr int np = 1; 
 template <> struct VPar<ND_VM_INDUCTOR_INJECTOR> { static constexpr int np = 1; };
 template <> struct VPar<ND_VM_CRL> { static constexpr int np = 3; };
 template <> struct VPar<ND_VM_FHN> { static constexpr int np = 2; };
+template <> struct VPar<ND_VM_TRUSS_FREE> { static constexpr int np = 3; };
+template <> struct VPar<ND_VM_TRUSS_FIXED> { static constexpr int np = 2; };
 
 // external inputs each vertex model reads
 template <int VM> struct VExt { static constexpr int n = 0; };

@@ -668,8 +668,9 @@ __device__ __forceinline__ void vertex_out_part(const VDesc& vd, int64_t tid, in
     if (vd.outkind == 2) return;                 // injector outputs: k_vertex_out_ff, after the hubs'
     if (vd.outkind == 1) {                       // NoFeedForward outputs from the parameters
         for (int64_t q = tid; q < vd.count; q += nthr)
-            o[static_cast<int64_t>(vd.out0) + q] =
-                tangent ? 0.0 : vertex_out_nff<double>(vd.model, p + vd.param0 + q * vd.pdim);
+            for (int j = 0; j < vd.depth; ++j)
+                o[static_cast<int64_t>(vd.out0) + q * vd.depth + j] =
+                    tangent ? 0.0 : vertex_out_nff<double>(vd.model, p + vd.param0 + q * vd.pdim, j);
         return;
     }
     // two-state vertices with output state 1 (swing): two vertices = one 32-byte read
@@ -765,10 +766,11 @@ __global__ void k_edge_out(EDesc e, const double* __restrict__ u, const double* 
     }
     const int2 en = e.ends[q];
     const double* pe = p + e.param0 + static_cast<int64_t>(q) * e.pdim;
-    // component j of each side (componentwise models at output depth > 1: the endpoint
-    // outputs are consecutive states)
+    // component j of each side (models at output depth > 1: the endpoint outputs are
+    // consecutive)
     for (int j = 0; j < e.outdim_dst; ++j) {
-        const double y = g_dst_rt(e.model, u[en.x + j], u[en.y + j], pe);
+        const double y = e.model == ND_EM_BEAM ? beam_dst<double>(u + en.x, u + en.y, pe, j)
+                                               : g_dst_rt(e.model, u[en.x + j], u[en.y + j], pe);
         o[o0 + e.outdim_src + j] = y;
         if (j < e.outdim_src) {
             if (e.wrap == ND_WRAP_SYMMETRIC) o[o0 + j] = y;
@@ -960,11 +962,35 @@ __global__ void __launch_bounds__(256) k_vertex_gen_jvp(VDesc vd, const int32_t*
     }
 }
 
-// Vertex batches with d > 1 output components (vdepth = edepth = d): the componentwise
-// models only (diffusion vertices dv .= esum, getting_started_with_network_dynamics.jl:
-// 129-132, with diffusion edges e .= v_s .- v_d).  One thread per vertex; component j's
-// destination row (row0 + q*d + j) is summed in the reference order, every entry
-// evaluated on component j of both endpoints.
+// One entry of a destination row at output depth d: component j of the edge output that
+// this vertex receives.  own: this vertex's output vector; vo + (s.x - j): the other
+// endpoint's (its d components are consecutive).  Componentwise models read component j
+// of both; the beam reads both vectors.
+template <typename S>
+__device__ __forceinline__ S vec_slot_value(const EDesc& e, int2 s, int j, const S* own, const S* nb,
+                                            const double* __restrict__ p) {
+    if (e.model == ND_EM_NOOP) return nan_of<S>();
+    const double* pe = p + (s.y >> 1);
+    const bool is_src = s.y & 1;
+    S y;
+    if (e.model == ND_EM_BEAM) y = is_src ? beam_dst<S>(own, nb, pe, j) : beam_dst<S>(nb, own, pe, j);
+    else y = is_src ? g_dst_rt<S>(e.model, own[j], nb[j], pe) : g_dst_rt<S>(e.model, nb[j], own[j], pe);
+    if (is_src) return e.wrap == ND_WRAP_SYMMETRIC ? y : -y;
+    return y;
+}
+
+// f at output depth d: diffusion dv .= esum, the free truss node
+template <typename S>
+__device__ __forceinline__ void vec_vertex_f(const VDesc& vd, S* dx, const S* x, const S* acc,
+                                             const double* __restrict__ pv) {
+    if (vd.model == ND_VM_TRUSS_FREE) truss_free_f<S>(dx, x, acc, pv);
+    else for (int j = 0; j < vd.depth; ++j) dx[j] = acc[j];
+}
+
+// Vertex batches with d > 1 output components (vdepth = edepth = d): componentwise
+// diffusion (getting_started_with_network_dynamics.jl:218-221) and the truss of
+// stress_on_truss.jl.  One thread per vertex; component j's destination row
+// (row0 + q*d + j) is summed in the reference order.
 template <bool STAGE>
 __global__ void __launch_bounds__(256) k_vertex_vec(VDesc vd, const int32_t* __restrict__ rowptr,
                                                     const int2* __restrict__ slots, const uint8_t* __restrict__ slot_eb,
@@ -975,18 +1001,29 @@ __global__ void __launch_bounds__(256) k_vertex_vec(VDesc vd, const int32_t* __r
     if (q >= vd.count) return;
     const int d = vd.depth;
     const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * vd.dim;
-    if (vd.model == ND_VM_NOOP) {                // f does nothing: du keeps its zero fill
+    if (vd.model == ND_VM_NOOP || vd.dim == 0) { // f does nothing (or no states): du keeps zero
         for (int j = 0; j < vd.dim; ++j) emit1<STAGE>(du, sa, x0 + j, 0.0);
         return;
     }
+    double own[ND_MAX_OUT], acc[ND_MAX_OUT], x[4], dx[4];
+    for (int k = 0; k < d; ++k)
+        own[k] = vd.outkind == 0 ? u[x0 + vd.mask[k]] : vo[static_cast<int64_t>(vd.out0) + static_cast<int64_t>(q) * d + k];
     for (int j = 0; j < d; ++j) {
-        const double own = u[x0 + vd.mask[j]];
         const int32_t row = vd.row0 + q * d + j;
-        double acc = 0.0;
-        for (int32_t k = rowptr[row]; k < rowptr[row + 1]; ++k)
-            acc = acc + slot_value_rt(ed[slot_eb ? slot_eb[k] : 0], slots[k], own, u, vo, p);
-        const double y = emit1<STAGE>(du, sa, x0 + j, acc);    // dv_j = esum_j
-        if (STAGE && sa.vnext) sa.vnext[static_cast<int64_t>(vd.out0) + static_cast<int64_t>(q) * d + j] = y;
+        double a = 0.0;
+        for (int32_t k = rowptr[row]; k < rowptr[row + 1]; ++k) {
+            const int2 sl = slots[k];
+            a = a + vec_slot_value<double>(ed[slot_eb ? slot_eb[k] : 0], sl, j, own, vo + (sl.x - j), p);
+        }
+        acc[j] = a;
+    }
+    for (int j = 0; j < vd.dim && j < 4; ++j) x[j] = u[x0 + j];
+    vec_vertex_f<double>(vd, dx, x, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim);
+    for (int j = 0; j < vd.dim && j < 4; ++j) {
+        const double y = emit1<STAGE>(du, sa, x0 + j, dx[j]);
+        if (STAGE && sa.vnext)
+            for (int k = 0; k < d; ++k)
+                if (vd.mask[k] == j) sa.vnext[static_cast<int64_t>(vd.out0) + static_cast<int64_t>(q) * d + k] = y;
     }
 }
 
@@ -1002,19 +1039,29 @@ __global__ void __launch_bounds__(256) k_vertex_vec_jvp(VDesc vd, const int32_t*
     if (q >= vd.count) return;
     const int d = vd.depth;
     const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * vd.dim;
-    if (vd.model == ND_VM_NOOP) {
+    if (vd.model == ND_VM_NOOP || vd.dim == 0) {
         for (int j = 0; j < vd.dim; ++j) { du[x0 + j] = 0.0; jv[x0 + j] = 0.0; }
         return;
     }
-    for (int j = 0; j < d; ++j) {
-        const Dual own{u[x0 + vd.mask[j]], v[x0 + vd.mask[j]]};
-        const int32_t row = vd.row0 + q * d + j;
-        Dual acc{0.0, 0.0};
-        for (int32_t k = rowptr[row]; k < rowptr[row + 1]; ++k)
-            acc = acc + slot_value_dual_rt(ed[slot_eb ? slot_eb[k] : 0], slots[k], own, u, v, vo, dvo, p);
-        du[x0 + j] = acc.v;
-        jv[x0 + j] = acc.d;
+    Dual own[ND_MAX_OUT], acc[ND_MAX_OUT], x[4], dx[4];
+    for (int k = 0; k < d; ++k) {
+        const int64_t oi = static_cast<int64_t>(vd.out0) + static_cast<int64_t>(q) * d + k;
+        own[k] = vd.outkind == 0 ? Dual{u[x0 + vd.mask[k]], v[x0 + vd.mask[k]]} : Dual{vo[oi], dvo[oi]};
     }
+    for (int j = 0; j < d; ++j) {
+        const int32_t row = vd.row0 + q * d + j;
+        Dual a{0.0, 0.0};
+        for (int32_t k = rowptr[row]; k < rowptr[row + 1]; ++k) {
+            const int2 sl = slots[k];
+            Dual nb[ND_MAX_OUT];
+            for (int c = 0; c < d; ++c) nb[c] = Dual{vo[sl.x - j + c], dvo[sl.x - j + c]};
+            a = a + vec_slot_value<Dual>(ed[slot_eb ? slot_eb[k] : 0], sl, j, own, nb, p);
+        }
+        acc[j] = a;
+    }
+    for (int j = 0; j < vd.dim && j < 4; ++j) x[j] = Dual{u[x0 + j], v[x0 + j]};
+    vec_vertex_f<Dual>(vd, dx, x, acc, p + vd.param0 + static_cast<int64_t>(q) * vd.pdim);
+    for (int j = 0; j < vd.dim && j < 4; ++j) { du[x0 + j] = dx[j].v; jv[x0 + j] = dx[j].d; }
 }
 
 // collect_externals!(map, extbuf, u, o), external_inputs.jl:49-77: map > 0 reads u,
@@ -1078,7 +1125,9 @@ __global__ void __launch_bounds__(256) k_vertex_out_nff_tangent(VDesc vd, const 
     if (q >= vd.count) return;
     Dual pd[kMaxPar];
     dual_params(pd, p, dp, vd.param0 + static_cast<int64_t>(q) * vd.pdim, vd.pdim);
-    dout[static_cast<int64_t>(vd.out0) + q] = vertex_out_nff<Dual, Dual>(vd.model, pd).d;
+    for (int j = 0; j < vd.depth; ++j)
+        dout[static_cast<int64_t>(vd.out0) + static_cast<int64_t>(q) * vd.depth + j] =
+            vertex_out_nff<Dual, Dual>(vd.model, pd, j).d;
 }
 
 // edges from the gathered buffer (coreloop.jl:76-78): ODE edges' f into du, feed-forward
@@ -1114,9 +1163,14 @@ __global__ void __launch_bounds__(256) k_edge_buf(EDesc e, const int32_t* __rest
         return;
     }
     const int64_t o0 = static_cast<int64_t>(e.out0) + static_cast<int64_t>(q) * (e.outdim_src + e.outdim_dst);
+    Dual vsv[ND_MAX_OUT], vdv[ND_MAX_OUT];
+    for (int j = 0; j < e.outdim_dst && j < ND_MAX_OUT; ++j) {
+        vsv[j] = Dual{gbuf[g0 + j], dgbuf[g0 + j]};
+        vdv[j] = Dual{gbuf[g0 + gd + j], dgbuf[g0 + gd + j]};
+    }
     for (int j = 0; j < e.outdim_dst; ++j) {
-        const Dual y = g_dst_rt<Dual, Dual>(e.model, Dual{gbuf[g0 + j], dgbuf[g0 + j]},
-                                            Dual{gbuf[g0 + gd + j], dgbuf[g0 + gd + j]}, pe);
+        const Dual y = e.model == ND_EM_BEAM ? beam_dst<Dual, Dual>(vsv, vdv, pe, j)
+                                             : g_dst_rt<Dual, Dual>(e.model, vsv[j], vdv[j], pe);
         o[o0 + e.outdim_src + j] = y.v;
         dout[o0 + e.outdim_src + j] = y.d;
         if (j < e.outdim_src) {
@@ -1139,6 +1193,17 @@ __global__ void __launch_bounds__(256) k_vertex_f_buf(VDesc vd, const double* __
     constexpr int DIMC = VModel<VM>::dim;
     const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= vd.count) return;
+    if (VM == ND_VM_TRUSS_FREE) {                // the free truss node, inputs (Fx, Fy)
+        const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * 4;
+        const int64_t r = static_cast<int64_t>(vd.row0) + static_cast<int64_t>(q) * 2;
+        Dual x[4], acc[2], dx[4], pd[kMaxPar];
+        for (int j = 0; j < 4; ++j) x[j] = Dual{u[x0 + j], v[x0 + j]};
+        for (int j = 0; j < 2; ++j) acc[j] = agg ? Dual{agg[r + j], dagg[r + j]} : Dual{0.0, 0.0};
+        dual_params(pd, p, dp, vd.param0 + static_cast<int64_t>(q) * vd.pdim, vd.pdim);
+        truss_free_f<Dual, Dual>(dx, x, acc, pd);
+        for (int j = 0; j < 4; ++j) { du[x0 + j] = dx[j].v; jv[x0 + j] = dx[j].d; }
+        return;
+    }
     if (VM == ND_VM_DIFFUSION && vd.depth > 1) { // d components: dv .= esum
         const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * vd.dim;
         for (int j = 0; j < vd.depth; ++j) {
@@ -1314,6 +1379,7 @@ VertexBufKernel pick_vertex_f_buf(int vm) {
     case ND_VM_INDUCTOR_INJECTOR: return k_vertex_f_buf<ND_VM_INDUCTOR_INJECTOR>;
     case ND_VM_CRL: return k_vertex_f_buf<ND_VM_CRL>;
     case ND_VM_FHN: return k_vertex_f_buf<ND_VM_FHN>;
+    case ND_VM_TRUSS_FREE: return k_vertex_f_buf<ND_VM_TRUSS_FREE>;
     default: return k_vertex_f_buf<ND_VM_NOOP>;
     }
 }
@@ -1405,7 +1471,7 @@ EdgeOdeKernel pick_edge_ode(int em) {
     }
 }
 
-bool vertex_model_known(int vm) { return vm >= ND_VM_NOOP && vm <= ND_VM_FHN; }
+bool vertex_model_known(int vm) { return vm >= ND_VM_NOOP && vm <= ND_VM_TRUSS_FIXED; }
 int VModelDim(int vm) {
     switch (vm) {
     case ND_VM_SWING:
@@ -1419,22 +1485,26 @@ int VModelDim(int vm) {
     case ND_VM_CAPACITOR_SINK:
     case ND_VM_CAPACITOR:
     case ND_VM_INDUCTOR_INJECTOR: return 1;
+    case ND_VM_TRUSS_FREE: return 4;
     case ND_VM_VOLTAGE_SOURCE:
-    case ND_VM_RESISTOR_INJECTOR: return 0;
+    case ND_VM_RESISTOR_INJECTOR:
+    case ND_VM_TRUSS_FIXED: return 0;
     default: return -1;
     }
 }
 // the output kind each vertex model has: StateMask, NoFeedForward or PureFeedForward
 int vertex_model_ff(int vm) {
-    if (vm == ND_VM_VOLTAGE_SOURCE) return ND_FF_NOFEEDFORWARD;
+    if (vm == ND_VM_VOLTAGE_SOURCE || vm == ND_VM_TRUSS_FIXED) return ND_FF_NOFEEDFORWARD;
     if (vm == ND_VM_RESISTOR_INJECTOR) return ND_FF_PUREFEEDFORWARD;
     return ND_FF_PURESTATEMAP;
 }
 int vertex_ext_inputs(int vm) { return vm == ND_VM_PI_SOURCE ? VExt<ND_VM_PI_SOURCE>::n : 0; }
 // vertex models only the one-thread-per-vertex kernels carry
-bool vertex_model_generic(int vm) { return vm >= ND_VM_CAPACITOR_SINK && vm <= ND_VM_FHN; }
-bool edge_model_known(int em) { return em >= ND_EM_NOOP && em <= ND_EM_ELECTRICAL; }
-bool edge_model_generic(int em) { return em == ND_EM_RESISTOR || em == ND_EM_LOOPBACK || em == ND_EM_ELECTRICAL; }
+bool vertex_model_generic(int vm) { return vm >= ND_VM_CAPACITOR_SINK && vm <= ND_VM_TRUSS_FIXED; }
+bool edge_model_known(int em) { return em >= ND_EM_NOOP && em <= ND_EM_BEAM; }
+bool edge_model_generic(int em) {
+    return em == ND_EM_RESISTOR || em == ND_EM_LOOPBACK || em == ND_EM_ELECTRICAL || em == ND_EM_BEAM;
+}
 int edge_params(int em) {
     switch (em) {
     case ND_EM_KURAMOTO: return EModel<ND_EM_KURAMOTO>::np;
@@ -1888,20 +1958,23 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         }
     }
     if ((functional_vertex || functional_edge) && (n.vdepth != 1 || (n.ne > 0 && n.edepth != 1))) {
-        // d output components per vertex and per edge side: the componentwise models
-        // (diffusion vertices dv .= esum and diffusion edges e .= v_s .- v_d), every vertex
-        // output a StateMask of d consecutive states
+        // d output components per vertex and per edge side: componentwise diffusion
+        // (vertices dv .= esum, edges e .= v_s .- v_d) and, at d = 2, the truss's free and
+        // fixed nodes and beams; every vertex output a StateMask of d consecutive states or,
+        // for fixed nodes, the parameters
         const int d = n.vdepth;
         bool ok = d >= 1 && d <= ND_MAX_OUT && (n.ne == 0 || n.edepth == d);
         for (const nd::Batch& b : n.vb) {
             const nd_kind& k = n.kinds[b.kind];
-            ok = ok && (k.model == ND_VM_NOOP || k.model == ND_VM_DIFFUSION) && k.outdim_dst == d &&
-                 k.ff == ND_FF_PURESTATEMAP && k.extdim == 0;
-            for (int j = 1; ok && j < d; ++j) ok = k.mask_dst[j] == k.mask_dst[0] + j;
+            const bool fixed = k.model == ND_VM_TRUSS_FIXED && d == 2;
+            ok = ok && (k.model == ND_VM_NOOP || k.model == ND_VM_DIFFUSION || (k.model == ND_VM_TRUSS_FREE && d == 2) ||
+                        fixed) && k.outdim_dst == d && k.extdim == 0 && (fixed || k.ff == ND_FF_PURESTATEMAP);
+            for (int j = 1; ok && !fixed && j < d; ++j) ok = k.mask_dst[j] == k.mask_dst[0] + j;
         }
         for (const nd::Batch& b : n.eb) {
             const nd_kind& k = n.kinds[b.kind];
-            ok = ok && (k.model == ND_EM_NOOP || k.model == ND_EM_DIFFUSION) && k.outdim_dst == d && k.dim == 0;
+            ok = ok && (k.model == ND_EM_NOOP || k.model == ND_EM_DIFFUSION || (k.model == ND_EM_BEAM && d == 2)) &&
+                 k.outdim_dst == d && k.dim == 0;
         }
         if (!ok)
             return nd::set_error(ND_ERR_UNSUPPORTED, "output depth %d (vdepth=%d, edepth=%d): only componentwise models "
@@ -2145,13 +2218,13 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
             if (rc) { delete pl; return rc; }
         }
     }
-    if (n.vdepth == 1 && n.ne > 0) {
+    if (n.ne > 0) {                              // compact vertex outputs: nv * vdepth
         void* w = nullptr;
-        hipError_t e = hipMalloc(&w, static_cast<size_t>(n.nv) * sizeof(double));
+        hipError_t e = hipMalloc(&w, static_cast<size_t>(n.nv) * n.vdepth * sizeof(double));
         if (e != hipSuccess) { delete pl; return nd::set_error(ND_ERR_ALLOC, "hipMalloc(vertex outputs): %s", hipGetErrorString(e)); }
         pl->allocations.push_back(w);
         pl->d_vout = static_cast<double*>(w);
-        pl->vout_len = static_cast<size_t>(n.nv);
+        pl->vout_len = static_cast<size_t>(n.nv) * n.vdepth;
     }
     rc = apply_gather_mode(pl);
     if (rc) { delete pl; return rc; }

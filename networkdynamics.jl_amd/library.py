@@ -138,6 +138,21 @@ def electrical_edge(sigma=0.5, name="electrical_edge"):
     return c.EdgeModel(g=c.Directed(c.electrical_edge_g), outdim=1, psym=["weight", ("sigma", sigma)], name=name)
 
 
+# docs/examples/stress_on_truss.jl:45-88: free nodes (velocity and position; the beam forces
+# are their inputs), fixed nodes (the position is the parameter) and beams
+def truss_free_vertex(M=10.0, gamma=200.0, g=9.81, name="free"):
+    return c.VertexModel(f=c.truss_free_f, g=c.StateMask([3, 4]), sym=[("vx", 0.0), ("vy", 0.0), "x", "y"],
+                         psym=[("M", M), ("gamma", gamma), ("g", g)], insym=["Fx", "Fy"], name=name)
+
+
+def truss_fixed_vertex(name="fixed"):
+    return c.VertexModel(g=c.truss_fixed_g, psym=["xfix", "yfix"], name=name)
+
+
+def beam(K=0.5e6, name="beam"):
+    return c.EdgeModel(g=c.AntiSymmetric(c.beam_g), psym=[("K", K), "L"], outsym=["Fx", "Fy"], name=name)
+
+
 # docs/examples/heterogeneous_system.jl:75-105: a vertex whose output is its parameter
 # (NoFeedForward g(out, p) = p, the voltage source's equation) and the Kuramoto vertex with
 # inertia and unit damping (benchmark/benchmark_models.jl:37-41 with psym omega0)

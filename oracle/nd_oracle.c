@@ -391,6 +391,14 @@ static void vertex_f(const nd_kind* k, double* du, const double* u, const double
                                       (getting_started_with_network_dynamics.jl:129-133) */
         for (int j = 0; j < k->dim; ++j) du[j] = acc[j];
         break;
+    case ND_VM_TRUSS_FREE: {       /* stress_on_truss.jl:55-61: free_f */
+        du[0] = (acc[0] - p[1] * u[0]) / p[0];
+        du[1] = (acc[1] - p[1] * u[1]) / p[0];
+        du[1] = du[1] - p[2];
+        du[2] = u[0];
+        du[3] = u[1];
+        break;
+    }
     case ND_VM_FHN:                /* directed_and_weighted_graphs.jl:89-93; v^3 = v*v*v */
         du[0] = u[0] - u[0] * u[0] * u[0] / 3.0 - u[1] + acc[0];
         du[1] = (u[0] - p[0]) * p[1];
@@ -452,6 +460,14 @@ static void edge_gdst(const nd_kind* k, double* y, const double* vs, const doubl
     case ND_EM_DIFFUSION_WEIGHTED: /* ComponentLibrary.jl:8-10 */
         y[0] = p[0] * (vs[0] - vd[0]);
         break;
+    case ND_EM_BEAM: {             /* stress_on_truss.jl:77-84: edge_g!; x^2 = x*x */
+        const double dx = vd[0] - vs[0], dy = vd[1] - vs[1];
+        const double d = sqrt(dx * dx + dy * dy);
+        const double fabs_ = p[0] * (p[1] - d);
+        y[0] = fabs_ * dx / d;
+        y[1] = fabs_ * dy / d;
+        break;
+    }
     case ND_EM_ELECTRICAL:         /* directed_and_weighted_graphs.jl:97-100 */
         y[0] = p[0] * (vs[0] - vd[0]) * p[1];
         break;
@@ -547,6 +563,8 @@ static void apply_vertex(const ndo_net* nw, const batch_t* B, int64_t i, int pas
         double* out = o + orr.first - 1;
         if (k->ff == ND_FF_PURESTATEMAP) {          /* StateMask: g(o, u) */
             for (int j = 0; j < k->outdim_dst; ++j) out[j] = u[sr.first - 1 + k->mask_dst[j] - 1];
+        } else if (k->model == ND_VM_TRUSS_FIXED) {    /* stress_on_truss.jl:45-47: pos .= p */
+            for (int j = 0; j < k->outdim_dst; ++j) out[j] = p[pr.first - 1 + j];
         } else if (k->model == ND_VM_VOLTAGE_SOURCE) { /* loopback_test.jl:31: p.v ~ V (NoFeedForward) */
             out[0] = p[pr.first - 1];
         } else if (k->model == ND_VM_RESISTOR_INJECTOR) { /* loopback_test.jl:75: p.i ~ p.v / R, input = hub */
@@ -818,6 +836,14 @@ void ndo_rhs_scale(ndo_net* nw, double* S, const double* u, const double* p, dou
             switch (k->model) {
             case ND_VM_DIFFUSION: for (int j = 0; j < k->dim; ++j) s[j] = agg[ir.first - 1 + j]; break;
             case ND_VM_KURAMOTO: s[0] = fabs(q[0]) + a; break;
+            case ND_VM_TRUSS_FREE: {
+                const double a0 = agg[ir.first - 1], a1 = agg[ir.first];
+                s[0] = (a0 + fabs(q[1] * x[0])) / fabs(q[0]);
+                s[1] = (a1 + fabs(q[1] * x[1])) / fabs(q[0]) + fabs(q[2]);
+                s[2] = fabs(x[0]);
+                s[3] = fabs(x[1]);
+                break;
+            }
             case ND_VM_FHN:
                 s[0] = fabs(x[0]) + fabs(x[0] * x[0] * x[0] / 3.0) + fabs(x[1]) + a;
                 s[1] = fabs(x[0] * q[1]) + fabs(q[0] * q[1]);
