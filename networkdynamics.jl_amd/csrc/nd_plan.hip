@@ -1013,7 +1013,9 @@ __global__ void __launch_bounds__(256) k_vertex_vec(VDesc vd, const int32_t* __r
         double a = 0.0;
         for (int32_t k = rowptr[row]; k < rowptr[row + 1]; ++k) {
             const int2 sl = slots[k];
-            a = a + vec_slot_value<double>(ed[slot_eb ? slot_eb[k] : 0], sl, j, own, vo + (sl.x - j), p);
+            const EDesc& e = ed[slot_eb ? slot_eb[k] : 0];
+            // NoOp entries read no endpoint (their slot holds none)
+            a = a + (e.model == ND_EM_NOOP ? __builtin_nan("") : vec_slot_value<double>(e, sl, j, own, vo + (sl.x - j), p));
         }
         acc[j] = a;
     }
@@ -1053,9 +1055,14 @@ __global__ void __launch_bounds__(256) k_vertex_vec_jvp(VDesc vd, const int32_t*
         Dual a{0.0, 0.0};
         for (int32_t k = rowptr[row]; k < rowptr[row + 1]; ++k) {
             const int2 sl = slots[k];
+            const EDesc& e = ed[slot_eb ? slot_eb[k] : 0];
+            if (e.model == ND_EM_NOOP) {         // g writes nothing (its slot holds no endpoint)
+                a = a + dual_nan();
+                continue;
+            }
             Dual nb[ND_MAX_OUT];
             for (int c = 0; c < d; ++c) nb[c] = Dual{vo[sl.x - j + c], dvo[sl.x - j + c]};
-            a = a + vec_slot_value<Dual>(ed[slot_eb ? slot_eb[k] : 0], sl, j, own, nb, p);
+            a = a + vec_slot_value<Dual>(e, sl, j, own, nb, p);
         }
         acc[j] = a;
     }

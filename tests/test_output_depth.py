@@ -93,3 +93,51 @@ def test_output_depth_refuses_other_models(ND, dev):
     with pytest.raises(ND.UnsupportedError, match="componentwise"):
         nw(torch.empty(6, dtype=torch.float64, device=dev), torch.zeros(6, dtype=torch.float64, device=dev),
            torch.ones(2, dtype=torch.float64, device=dev), 0.0)
+
+
+def _with_noop_edges(ND, n=400, seed=4):
+    """Depth-2 diffusion with every fifth edge a NoOp edge (test/aggregators_test.jl:20): a
+    NoOp edge writes no output, so the rows it reaches are NaN on both sides."""
+    L = ND.Lib
+    g = ND.watts_strogatz(n, 4, 0.4, seed=seed)
+    noop = ND.EdgeModel(g=ND.AntiSymmetric(ND.components.noop_edge), outdim=2)
+    em = [noop if k % 5 == 0 else L.diffusion_edge_multi(2) for k in range(g.ne())]
+    return g, ND.Network(g, L.diffusion_vertex_multi(2), em)
+
+
+def test_noop_edges_at_depth_two_oracle(ND):
+    g, nw = _with_noop_edges(ND)
+    u = np.random.default_rng(1).normal(size=nw.dim)
+    du = oracle_for(nw).rhs(u, np.zeros(0))
+    hit = np.zeros(g.nv(), bool)
+    for k, (s, t) in enumerate(zip(g.src - 1, g.dst - 1)):
+        if k % 5 == 0:
+            hit[s] = hit[t] = True
+    assert hit.any() and not hit.all()
+    for j in range(2):
+        assert np.array_equal(np.isnan(du[j::2]), hit)
+
+
+@pytest.mark.gpu
+def test_noop_edges_at_depth_two_on_the_device(ND, dev):
+    import torch
+    g, nw = _with_noop_edges(ND)
+    rng = np.random.default_rng(2)
+    u = rng.normal(size=nw.dim)
+    p0 = np.zeros(0)
+    on = oracle_for(nw)
+    ud, pd = torch.from_numpy(u).to(dev), torch.zeros(1, dtype=torch.float64, device=dev)
+    du = torch.full((nw.dim,), 7.0, dtype=torch.float64, device=dev)
+    nw(du, ud, pd, 0.0)
+    ref = on.rhs(u, p0)
+    got = du.cpu().numpy()
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    ok = ~np.isnan(ref)
+    err = scaled_error(got[ok], ref[ok], on.scale(u, p0)[ok])
+    assert err.max() <= 1e-12, err.max()
+    v = rng.normal(size=nw.dim)
+    jv = torch.full_like(du, 7.0)
+    nw.jvp(du, jv, ud, torch.from_numpy(v).to(dev), pd, 0.0)
+    jr = fd_jvp(on, u, v, p0)
+    assert np.array_equal(np.isnan(jv.cpu().numpy()), np.isnan(jr))
+    np.testing.assert_allclose(jv.cpu().numpy()[ok], jr[ok], rtol=1e-8, atol=1e-8)
