@@ -275,9 +275,9 @@ int nd_network_mass_matrix(const nd_network* nw, const nd_mass* vertex_mass, con
 typedef struct nd_plan nd_plan;
 
 /* Uploads the network's index structures to the current HIP device and selects
- * the fused kernels.  ND_ERR_UNSUPPORTED if a model has no HIP kernel (ODE edges
- * next to computed vertex outputs; output depth > 1 except for diffusion vertices and
- * edges over consecutive states and, at depth 2, the truss nodes and beams);
+ * the fused kernels.  ND_ERR_UNSUPPORTED if a model has no HIP kernel (output
+ * depth > 1 except for diffusion vertices and edges over consecutive states and, at
+ * depth 2, the truss nodes and beams);
  * ND_ERR_ARGUMENT if the network has external inputs and no ExtMap
  * (nd_network_set_extmap) or a model's external-input count differs from its kind's.
  * External inputs (of vertices and of ODE edges) resolve to state indices: a vertex

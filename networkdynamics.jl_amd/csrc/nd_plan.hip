@@ -496,9 +496,11 @@ __global__ void __launch_bounds__(256) k_vertex_rhs_group(VGroup g, const int32_
 
 // du of ODE edges (coreloop.jl:76, edge f without feed-forward); external inputs are
 // read from the states they resolve to (collect_externals! fills them from u and from
-// StateMask outputs, external_inputs.jl:49-77)
+// StateMask outputs, external_inputs.jl:49-77).  eo: where e.ends point -- u, or the
+// compact vertex outputs when some vertex outputs are computed by a g.
 template <int EM, bool STAGE>
 __global__ void __launch_bounds__(256) k_edge_ode(EDesc e, const double* __restrict__ u,
+                                                  const double* __restrict__ eo,
                                                   const double* __restrict__ p, double* __restrict__ du,
                                                   StageArgs sa) {
     constexpr int D = ode_dim(EM);
@@ -506,7 +508,7 @@ __global__ void __launch_bounds__(256) k_edge_ode(EDesc e, const double* __restr
     const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= e.count) return;
     const int2 en = e.ends[q];
-    const double vs = u[en.x], vd = u[en.y];
+    const double vs = eo[en.x], vd = eo[en.y];
     const int64_t x0 = static_cast<int64_t>(e.state0) + static_cast<int64_t>(q) * D;
     double x[D], ext[NX], dx[D];
 #pragma unroll
@@ -588,9 +590,11 @@ __global__ void __launch_bounds__(256) k_vertex_jvp(VDesc vd, const int32_t* __r
     }
 }
 
-// du, jv of ODE edges (the dual form of k_edge_ode)
+// du, jv of ODE edges (the dual form of k_edge_ode); eo, deo: the endpoint outputs and
+// their tangents (u and v, or the compact vertex outputs)
 template <int EM>
 __global__ void __launch_bounds__(256) k_edge_ode_jvp(EDesc e, const double* __restrict__ u, const double* __restrict__ v,
+                                                      const double* __restrict__ eo, const double* __restrict__ deo,
                                                       const double* __restrict__ p, double* __restrict__ du,
                                                       double* __restrict__ jv) {
     constexpr int D = ode_dim(EM);
@@ -598,7 +602,7 @@ __global__ void __launch_bounds__(256) k_edge_ode_jvp(EDesc e, const double* __r
     const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= e.count) return;
     const int2 en = e.ends[q];
-    const Dual vs{u[en.x], v[en.x]}, vdn{u[en.y], v[en.y]};
+    const Dual vs{eo[en.x], deo[en.x]}, vdn{eo[en.y], deo[en.y]};
     const int64_t x0 = static_cast<int64_t>(e.state0) + static_cast<int64_t>(q) * D;
     Dual x[D], ext[NX], dx[D];
 #pragma unroll
@@ -751,9 +755,10 @@ __global__ void __launch_bounds__(256) k_loopback_copy(VDesc vd, const double* _
 }
 
 // both outputs of one edge batch (coreloop.jl:41,78 with the wrappers of
-// component_functions.jl:81-203)
-__global__ void k_edge_out(EDesc e, const double* __restrict__ u, const double* __restrict__ p,
-                           double* __restrict__ o) {
+// component_functions.jl:81-203).  ODE edges copy their states from u; feed-forward
+// edges read the endpoint outputs from eo (u, or o where vertex outputs are computed).
+__global__ void k_edge_out(EDesc e, const double* __restrict__ u, const double* __restrict__ eo,
+                           const double* __restrict__ p, double* __restrict__ o) {
     const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= e.count) return;
     if (e.model == ND_EM_NOOP) return;
@@ -769,8 +774,8 @@ __global__ void k_edge_out(EDesc e, const double* __restrict__ u, const double* 
     // component j of each side (models at output depth > 1: the endpoint outputs are
     // consecutive)
     for (int j = 0; j < e.outdim_dst; ++j) {
-        const double y = e.model == ND_EM_BEAM ? beam_dst<double>(u + en.x, u + en.y, pe, j)
-                                               : g_dst_rt(e.model, u[en.x + j], u[en.y + j], pe);
+        const double y = e.model == ND_EM_BEAM ? beam_dst<double>(eo + en.x, eo + en.y, pe, j)
+                                               : g_dst_rt(e.model, eo[en.x + j], eo[en.y + j], pe);
         o[o0 + e.outdim_src + j] = y;
         if (j < e.outdim_src) {
             if (e.wrap == ND_WRAP_SYMMETRIC) o[o0 + j] = y;
@@ -1455,7 +1460,8 @@ JvpKernel pick_gen_jvp(int vm) {
     }
 }
 
-using EdgeOdeJvpKernel = void (*)(EDesc, const double*, const double*, const double*, double*, double*);
+using EdgeOdeJvpKernel = void (*)(EDesc, const double*, const double*, const double*, const double*, const double*,
+                                  double*, double*);
 EdgeOdeJvpKernel pick_edge_ode_jvp(int em) {
     switch (em) {
     case ND_EM_ODE_SIN_DIFFUSION: return k_edge_ode_jvp<ND_EM_ODE_SIN_DIFFUSION>;
@@ -1466,7 +1472,7 @@ EdgeOdeJvpKernel pick_edge_ode_jvp(int em) {
     }
 }
 
-using EdgeOdeKernel = void (*)(EDesc, const double*, const double*, double*, StageArgs);
+using EdgeOdeKernel = void (*)(EDesc, const double*, const double*, const double*, double*, StageArgs);
 template <bool STAGE>
 EdgeOdeKernel pick_edge_ode(int em) {
     switch (em) {
@@ -1898,7 +1904,8 @@ void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const doub
     for (const EDesc& e : pl->edesc) {
         if (!e.ode || e.count == 0) continue;
         hipLaunchKernelGGL(stage ? pick_edge_ode<true>(e.model) : pick_edge_ode<false>(e.model),
-                           dim3(blocks_for(e.count, B)), dim3(static_cast<unsigned>(B)), 0, s, e, u, p, du, sa);
+                           dim3(blocks_for(e.count, B)), dim3(static_cast<unsigned>(B)), 0, s, e, u,
+                           pl->computed_out ? vin : u, p, du, sa);
     }
 }
 
@@ -1992,10 +1999,6 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         return nd::set_error(ND_ERR_ARGUMENT, "the network has external inputs: set its ExtMap (nd_network_set_extmap) first");
     bool computed_out = false;
     for (const nd::Batch& b : n.vb) computed_out = computed_out || n.kinds[b.kind].ff != ND_FF_PURESTATEMAP;
-    if (computed_out)
-        for (const nd::Batch& b : n.eb)
-            if (is_ode_model(n.kinds[b.kind].model))
-                return nd::set_error(ND_ERR_UNSUPPORTED, "ODE edges next to computed vertex outputs have no HIP kernel");
 
     nd_plan* pl = new (std::nothrow) nd_plan;
     if (!pl) return nd::set_error(ND_ERR_ALLOC, "host allocation failed");
@@ -2343,8 +2346,8 @@ int nd_rhs_jvp(nd_plan* pl, double* du, double* jv, const double* u, const doubl
     }
     for (const EDesc& e : pl->edesc) {
         if (!e.ode || e.count == 0) continue;
-        hipLaunchKernelGGL(pick_edge_ode_jvp(e.model), dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, u, v, p,
-                           du, jv);
+        hipLaunchKernelGGL(pick_edge_ode_jvp(e.model), dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, u, v,
+                           pl->computed_out ? vo : u, pl->computed_out ? dvo : v, p, du, jv);
     }
     return hip_check(hipGetLastError(), "nd_rhs_jvp launch");
 }
@@ -2497,10 +2500,10 @@ int nd_get_buffers(nd_plan* pl, double* o, double* aggbuf, const double* u, cons
         for (const VDesc& vd : pl->vdesc)
             if (vd.outkind == 2 && vd.count)
                 hipLaunchKernelGGL(k_vertex_out_ff, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, o, p);
-    // with computed outputs the edges' endpoint indices address o (no ODE edges then)
+    // with computed outputs the edges' endpoint indices address o
     for (const EDesc& e : pl->edesc)
         if (e.count)
-            hipLaunchKernelGGL(k_edge_out, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e,
+            hipLaunchKernelGGL(k_edge_out, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, u,
                                pl->computed_out ? static_cast<const double*>(o) : u, p, o);
     if (pl->d_lbin && pl->lastidx_aggr > 0)
         for (const VDesc& vd : pl->vdesc)
@@ -2625,8 +2628,8 @@ int nd_rhs_perturbed_jvp(nd_plan* pl, double* du, double* jv, const double* u, c
         }
     for (const EDesc& e : pl->edesc)
         if (e.count && e.ode) {
-            hipLaunchKernelGGL(k_edge_out, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, u, p, o);
-            hipLaunchKernelGGL(k_edge_out, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, v, p, dout);
+            hipLaunchKernelGGL(k_edge_out, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, u, u, p, o);
+            hipLaunchKernelGGL(k_edge_out, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, v, v, p, dout);
         }
     // loopback copies, vertex-input perturbation, feed-forward vertex g (coreloop.jl:47-55)
     if (pl->d_lbin && na > 0)

@@ -8,7 +8,9 @@ The oracle is pinned three ways:
   so the pair stays antisymmetric bit for bit);
 - closed forms of the three wrappers on a single directed edge;
 - the reference test's e_noff edge (de = ext, AntiSymmetric(1), extin = EIndex(2, :a)):
-  hand-derived layout and ExtMap, and de equal to the referenced state.
+  hand-derived layout and ExtMap, and de equal to the referenced state;
+- lag lines next to vertices whose output is computed (a parameter, NoFeedForward, as the
+  static vertex of docs/examples/heterogeneous_system.jl): the equations written out.
 """
 import numpy as np
 import pytest
@@ -127,6 +129,54 @@ def test_ext_network_oracle_invariants(ND):
         # de of a driven edge is its input: omega of a vertex, a lag state, or minus it
         want = u[v[j, 0, 0]] if kind == 0 else (u[e[j, 0, 0] - 1] if kind == 1 else -u[e[j, 0, 0] - 1])
         assert du[e[i, 0, 0] - 1] == want, (i, kind, j)
+
+
+def _lines_next_to_static_vertices(ND, n=60, seed=6):
+    """Swing vertices and every seventh vertex a static one (output = its parameter),
+    joined by one-state AntiSymmetric lag lines: the lines read computed outputs."""
+    L = ND.Lib
+    g = ND.watts_strogatz(n, 4, 0.5, seed=seed)
+    static = np.arange(n) % 7 == 0
+    vm = [L.static_vertex() if s else L.kuramoto_second() for s in static]
+    nw = ND.Network(g, vm, L.ode_lag_edge())
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=nw.dim)
+    p = rng.uniform(0.5, 2.0, nw.pdim)
+    return nw, g, u, p, static
+
+
+def test_lag_lines_next_to_static_vertices_equations(ND):
+    nw, g, u, p, static = _lines_next_to_static_vertices(ND)
+    v, e = nw.ranges()
+    n = g.nv()
+    theta = np.array([p[v[i, 2, 0] - 1] if static[i] else u[v[i, 0, 0] - 1] for i in range(n)])
+    acc = np.zeros(n)
+    de = np.zeros(g.ne())
+    for k, (a, b) in enumerate(zip(g.src - 1, g.dst - 1)):
+        x, tau = u[e[k, 0, 0] - 1], p[e[k, 3, 0] - 1]
+        de[k] = 1.0 / tau * (np.sin(theta[a] - theta[b]) - x)       # ode_lag_f
+        acc[b] += x
+        acc[a] -= x                                                 # AntiSymmetric src side
+    du = oracle_for(nw).rhs(u, p)
+    for k in range(g.ne()):
+        np.testing.assert_allclose(du[e[k, 0, 0] - 1], de[k], rtol=1e-14, atol=1e-14)
+    for i in np.flatnonzero(~static):
+        x0, p0 = v[i, 0, 0] - 1, v[i, 2, 0] - 1
+        M, D, Pm = p[p0: p0 + 3]
+        np.testing.assert_allclose(du[x0: x0 + 2], [u[x0 + 1], (Pm - D * u[x0 + 1] + acc[i]) / M],
+                                   rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.gpu
+def test_lag_lines_next_to_static_vertices_on_the_device(ND, dev):
+    import torch
+    nw, g, u, p, _ = _lines_next_to_static_vertices(ND, n=3000)
+    _check_device(ND, dev, nw, u, p)
+    on = oracle_for(nw)
+    du = torch.empty(nw.dim, dtype=torch.float64, device=dev)
+    nw(du, torch.from_numpy(u).to(dev), torch.from_numpy(p).to(dev), 0.0, perturb=np.zeros(0), perturb_maps={})
+    err = scaled_error(du.cpu().numpy(), on.rhs(u, p), on.scale(u, p))
+    assert err.max() <= 1e-12, err.max()
 
 
 def _check_device(ND, dev, nw, u, p, rtol=1e-12):
