@@ -426,7 +426,8 @@ int nd_ensemble_create(int32_t ninst, const int64_t* nv, const int64_t* ne, cons
 void nd_ensemble_destroy(nd_ensemble* en);
 int nd_ensemble_sizes(const nd_ensemble* en, int64_t* total_states, int64_t* total_params,
                       int64_t* state_offsets, int64_t* param_offsets);
-/* size class c holds instances with nv <= threads*vper (first class that fits) */
+/* size class c holds instances with nv <= threads*vper (first class that fits);
+ * threads: a multiple of 64 up to 1024, vper 1-4 (1-2 above 512 threads) */
 int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t vper);
 /* integrates every instance from t0 to t1 in place on u (device); stats: device array
  * of ninst entries.  p is read only, except that with opts->line_limit the first

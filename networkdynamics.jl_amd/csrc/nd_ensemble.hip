@@ -215,8 +215,8 @@ constexpr double c_bt1 = -0.00178001105222577714, c_bt2 = -0.0008164344596567469
                  c_bt4 = -0.1447110071732629, c_bt5 = 0.5823571654525552, c_bt6 = -0.45808210592918697,
                  c_bt7 = 0.015151515151515152;
 
-template <int VM, int EM, int METHOD, int V>
-__global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
+template <int VM, int EM, int METHOD, int V, int LB>
+__global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
     constexpr int D = VModel<VM>::dim;
     constexpr int NPV = VPar<VM>::np > 0 ? VPar<VM>::np : 1;
     constexpr int NPE = EModel<EM>::np;
@@ -466,26 +466,30 @@ __global__ void __launch_bounds__(512) k_ensemble(SolveArgs a) {
 
 using EnsKernel = void (*)(SolveArgs);
 
+// Workgroups of up to 512 threads take any vper; 1024-thread workgroups (16 waves, at
+// most 128 VGPRs each) take one or two vertices per thread.
 template <int VM, int EM, int METHOD>
-EnsKernel pick_v(int vper) {
+EnsKernel pick_v(int vper, int threads) {
+    if (threads > 512) return vper == 1 ? k_ensemble<VM, EM, METHOD, 1, 1024> : k_ensemble<VM, EM, METHOD, 2, 1024>;
     switch (vper) {
-    case 1: return k_ensemble<VM, EM, METHOD, 1>;
-    case 3: return k_ensemble<VM, EM, METHOD, 3>;
-    case 4: return k_ensemble<VM, EM, METHOD, 4>;
-    default: return k_ensemble<VM, EM, METHOD, 2>;
+    case 1: return k_ensemble<VM, EM, METHOD, 1, 512>;
+    case 3: return k_ensemble<VM, EM, METHOD, 3, 512>;
+    case 4: return k_ensemble<VM, EM, METHOD, 4, 512>;
+    default: return k_ensemble<VM, EM, METHOD, 2, 512>;
     }
 }
 
 template <int VM, int EM>
-EnsKernel pick_m(int method, int vper) {
-    return method == ND_METHOD_RK4 ? pick_v<VM, EM, 1>(vper) : pick_v<VM, EM, 0>(vper);
+EnsKernel pick_m(int method, int vper, int threads) {
+    return method == ND_METHOD_RK4 ? pick_v<VM, EM, 1>(vper, threads) : pick_v<VM, EM, 0>(vper, threads);
 }
 
-EnsKernel pick_ensemble(int vm, int em, int method, int vper) {
-    if (vm == ND_VM_SWING && em == ND_EM_KURAMOTO) return pick_m<ND_VM_SWING, ND_EM_KURAMOTO>(method, vper);
-    if (vm == ND_VM_SWING && em == ND_EM_STATIC_LINE) return pick_m<ND_VM_SWING, ND_EM_STATIC_LINE>(method, vper);
-    if (vm == ND_VM_KURAMOTO && em == ND_EM_KURAMOTO) return pick_m<ND_VM_KURAMOTO, ND_EM_KURAMOTO>(method, vper);
-    if (vm == ND_VM_KURAMOTO && em == ND_EM_STATIC_LINE) return pick_m<ND_VM_KURAMOTO, ND_EM_STATIC_LINE>(method, vper);
+EnsKernel pick_ensemble(int vm, int em, int method, int vper, int threads) {
+    if (vm == ND_VM_SWING && em == ND_EM_KURAMOTO) return pick_m<ND_VM_SWING, ND_EM_KURAMOTO>(method, vper, threads);
+    if (vm == ND_VM_SWING && em == ND_EM_STATIC_LINE) return pick_m<ND_VM_SWING, ND_EM_STATIC_LINE>(method, vper, threads);
+    if (vm == ND_VM_KURAMOTO && em == ND_EM_KURAMOTO) return pick_m<ND_VM_KURAMOTO, ND_EM_KURAMOTO>(method, vper, threads);
+    if (vm == ND_VM_KURAMOTO && em == ND_EM_STATIC_LINE)
+        return pick_m<ND_VM_KURAMOTO, ND_EM_STATIC_LINE>(method, vper, threads);
     return nullptr;
 }
 
@@ -662,7 +666,7 @@ int prepare(nd_ensemble* en, int method) {
     int32_t lds[ND_ENSEMBLE_CLASSES] = {};
     for (int c = 0; c < ND_ENSEMBLE_CLASSES; ++c) {
         if (en->cls_first[c + 1] == en->cls_first[c]) continue;
-        ks[c] = pick_ensemble(en->vk.model, en->ek.model, method, en->cls_vper[c]);
+        ks[c] = pick_ensemble(en->vk.model, en->ek.model, method, en->cls_vper[c], en->cls_threads[c]);
         if (!ks[c]) return nd::set_error(ND_ERR_UNSUPPORTED, "no ensemble kernel for these models");
         lds[c] = class_lds(en, c);
     }
@@ -687,7 +691,7 @@ int launch_classes(nd_ensemble* en, SolveArgs a, int method, hipStream_t s) {
     for (int c = ND_ENSEMBLE_CLASSES - 1; c >= 0 && !rc; --c) {
         const int32_t cnt = en->cls_first[c + 1] - en->cls_first[c];
         if (cnt == 0) continue;
-        EnsKernel k = pick_ensemble(en->vk.model, en->ek.model, method, en->cls_vper[c]);
+        EnsKernel k = pick_ensemble(en->vk.model, en->ek.model, method, en->cls_vper[c], en->cls_threads[c]);
         hipStream_t cs = en->cls_stream[c];
         rc = hip_err(hipStreamWaitEvent(cs, en->ev_fork, 0), "hipStreamWaitEvent");
         if (rc) break;
@@ -830,8 +834,9 @@ int nd_ensemble_sizes(const nd_ensemble* en, int64_t* total_states, int64_t* tot
 
 int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t vper) {
     if (!en || cls < 0 || cls >= ND_ENSEMBLE_CLASSES) return nd::set_error(ND_ERR_ARGUMENT, "bad class");
-    if (threads < kWave || threads > 512 || threads % kWave || vper < 1 || vper > 4)
-        return nd::set_error(ND_ERR_ARGUMENT, "threads must be a multiple of 64 in [64,512], vper in {1,2,3,4}");
+    if (threads < kWave || threads > 1024 || threads % kWave || vper < 1 || vper > 4 || (threads > 512 && vper > 2))
+        return nd::set_error(ND_ERR_ARGUMENT, "threads must be a multiple of 64 in [64,1024], vper in {1,2,3,4} "
+                             "(in {1,2} above 512 threads)");
     // regrouping replaces the instance-order array: queued solves finish with the old one
     int rc = hip_err(hipDeviceSynchronize(), "hipDeviceSynchronize");
     if (rc) return rc;

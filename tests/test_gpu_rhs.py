@@ -473,12 +473,13 @@ def test_dense_jacobian_kuramoto(ND, dev):
     assert ev.max() <= 1e-10 * np.abs(ev).max()
 
 
-@pytest.mark.parametrize("em", ["kuramoto", "static_line"])
-def test_rk4_single_workgroup_path(ND, dev, em):
-    # small single-batch networks: nd_rk4 runs all steps in one workgroup; same answer as
-    # the oracle and as the four-launches-per-step path
+@pytest.mark.parametrize("em,n", [("kuramoto", 1500), ("static_line", 1500), ("kuramoto", 1000)])
+def test_rk4_single_workgroup_path(ND, dev, em, n):
+    # small single-batch networks: nd_rk4 runs all steps in one workgroup (1024 threads up
+    # to 1024 vertices, 512 x 4 above); same answer as the oracle and as the
+    # four-launches-per-step path
     rng = np.random.default_rng(51)
-    g = ND.barabasi_albert(1500, 2, seed=51)
+    g = ND.barabasi_albert(n, 2, seed=51)
     n, m = g.nv(), g.ne()
     edge = ND.Lib.kuramoto_edge() if em == "kuramoto" else ND.Lib.static_line_edge()
     nw = ND.Network(g, ND.Lib.kuramoto_second(), edge)

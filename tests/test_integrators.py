@@ -89,6 +89,36 @@ def test_ensemble_rk4_matches_oracle(ND, dev):
 
 
 @pytest.mark.gpu
+def test_ensemble_launch_shapes_agree(ND, dev):
+    # 64..1024 threads per workgroup, 1-4 vertices per thread: RK4 sums every destination
+    # list in the same chunks whatever the shape, so the results agree bit for bit; Tsit5's
+    # error norm adds per-wave partials, so there the shapes agree to tolerance
+    import torch
+    gs, rng = _instances(ND, 10, 100, 1000, seed=12)
+    ps, us = _swing_params(ND, gs, rng)
+    ens = ND.Ensemble(gs, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    u0 = torch.from_numpy(np.concatenate(us)).to(dev)
+    p = torch.from_numpy(np.concatenate(ps)).to(dev)
+    shapes = [[(512, 2)], [(1024, 1)], [(512, 4)], [(1024, 2)], [(256, 4)]]
+    rk, ts = [], []
+    for sh in shapes:
+        ens.set_classes(sh)
+        u = u0.clone()
+        st = ND.Ensemble.stats_to_numpy(ens.solve(u, p, (0.0, 0.5), method=ND.ensemble.RK4, dt=0.005))
+        assert (st["retcode"] == 0).all(), sh
+        rk.append(u.cpu().numpy())
+        u = u0.clone()
+        st = ND.Ensemble.stats_to_numpy(ens.solve(u, p, (0.0, 1.0), reltol=1e-8, abstol=1e-8))
+        assert (st["retcode"] == 0).all(), sh
+        ts.append(u.cpu().numpy())
+    for sh, r, t in zip(shapes[1:], rk[1:], ts[1:]):
+        assert np.array_equal(r, rk[0]), sh
+        np.testing.assert_allclose(t, ts[0], rtol=0, atol=1e-6, err_msg=str(sh))
+    with pytest.raises(ND.ArgumentError):
+        ens.set_class(0, 1024, 3)
+
+
+@pytest.mark.gpu
 def test_ensemble_tsit5_matches_per_instance(ND, dev):
     import torch
     gs, rng = _instances(ND, 12, 50, 600, seed=4)
