@@ -2370,9 +2370,9 @@ int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_
             const int64_t nv = static_cast<int64_t>(pl->vdesc[0].count), ne = static_cast<int64_t>(pl->small_src.size());
             rc = nd_ensemble_create(1, &nv, &ne, pl->small_src.data(), pl->small_dst.data(), &pl->small_vk,
                                     &pl->small_ek, &pl->small_en);
-            // up to 1024 vertices: one vertex per thread of a 1024-thread workgroup (16
-            // waves hide the latency of the per-stage barriers better than 512 x 2)
-            if (!rc && nv <= 1024) rc = nd_ensemble_set_class(pl->small_en, 0, 1024, 1);
+            // a 1024-thread workgroup, one or two vertices per thread (16 waves hide the
+            // latency of the per-stage barriers better than 512 x 2 or 512 x 4)
+            if (!rc) rc = nd_ensemble_set_class(pl->small_en, 0, 1024, nv <= 1024 ? 1 : 2);
             if (rc) return rc;
             void* w = nullptr;
             hipError_t e = hipMalloc(&w, sizeof(nd_ensemble_stat));

@@ -475,9 +475,9 @@ def test_dense_jacobian_kuramoto(ND, dev):
 
 @pytest.mark.parametrize("em,n", [("kuramoto", 1500), ("static_line", 1500), ("kuramoto", 1000)])
 def test_rk4_single_workgroup_path(ND, dev, em, n):
-    # small single-batch networks: nd_rk4 runs all steps in one workgroup (1024 threads up
-    # to 1024 vertices, 512 x 4 above); same answer as the oracle and as the
-    # four-launches-per-step path
+    # small single-batch networks: nd_rk4 runs all steps in one 1024-thread workgroup (one
+    # vertex per thread up to 1024 vertices, two above); same answer as the oracle and as
+    # the four-launches-per-step path
     rng = np.random.default_rng(51)
     g = ND.barabasi_albert(n, 2, seed=51)
     n, m = g.nv(), g.ne()
