@@ -119,6 +119,36 @@ def test_ensemble_launch_shapes_agree(ND, dev):
 
 
 @pytest.mark.gpu
+def test_network_tsit5_paths(ND, dev):
+    # Network.tsit5: a single swing network runs in one workgroup and agrees with the
+    # host-stepped path to tolerance; a network the ensemble kernel does not implement
+    # (load vertices next to swing ones, static lines) takes the host-stepped path itself
+    import torch
+    gs, rng = _instances(ND, 1, 600, 600, seed=21)
+    ps, us = _swing_params(ND, gs, rng)
+    nw = ND.Network(gs[0], ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    p = torch.from_numpy(ps[0]).to(dev)
+    u = torch.from_numpy(us[0]).to(dev)
+    info = nw.tsit5(u, p, (0.0, 2.0))
+    assert info["path"] == "one workgroup" and info["retcode"] == 0 and info["t"] == 2.0, info
+    uh = torch.from_numpy(us[0]).to(dev)
+    ref = ND.tsit5_per_instance(nw, uh, p, (0.0, 2.0))
+    assert abs(ref["naccept"] - info["naccept"]) <= 2 + 0.02 * ref["naccept"]
+    np.testing.assert_allclose(u.cpu().numpy(), uh.cpu().numpy(), rtol=0, atol=1e-6)
+    g = gs[0]
+    vm = [ND.Lib.kuramoto_second() if i % 3 else ND.Lib.load_vertex() for i in range(g.nv())]
+    nw2 = ND.Network(g, vm, ND.Lib.static_line_edge())
+    p2 = rng.uniform(0.5, 2.0, nw2.pdim)
+    u2 = rng.normal(size=nw2.dim) * 0.1
+    a = torch.from_numpy(u2).to(dev)
+    info2 = nw2.tsit5(a, torch.from_numpy(p2).to(dev), (0.0, 0.5))
+    assert info2["path"] == "host steps" and info2["retcode"] == 0
+    b = torch.from_numpy(u2).to(dev)
+    ND.tsit5_per_instance(nw2, b, torch.from_numpy(p2).to(dev), (0.0, 0.5))
+    assert torch.equal(a, b)
+
+
+@pytest.mark.gpu
 def test_ensemble_tsit5_matches_per_instance(ND, dev):
     import torch
     gs, rng = _instances(ND, 12, 50, 600, seed=4)
