@@ -515,15 +515,16 @@ class Network:
         if getattr(self, "_ens1", False) is False:
             self._ens1 = None
             from .ensemble import Ensemble
-            vk, ek = self.kinds[self.vertex_kind[0]] if self.nv() else None, \
-                self.kinds[self.edge_kind[0]] if self.ne() else None
             one = (self.nv() > 0 and self.ne() > 0 and len(self.batches(False)) == 1 and
                    len(self.batches(True)) == 1 and self.info.lastidx_extbuf == 0)
-            ok = one and vk["model"] in (A.ND_VM_KURAMOTO, A.ND_VM_SWING) and vk["ff"] == A.ND_FF_PURESTATEMAP and \
-                vk["outdim_dst"] == 1 and vk["mask_dst"][0] == 1 and \
-                ek["model"] in (A.ND_EM_KURAMOTO, A.ND_EM_STATIC_LINE) and \
-                ek["wrapper"] == A.ND_WRAP_ANTISYMMETRIC and ek["dim"] == 0 and ek["outdim_dst"] == 1
-            if ok:
+            if not one:
+                return None
+            vk, ek = self.kinds[self.vertex_kind[0]], self.kinds[self.edge_kind[0]]
+            vok = (vk["model"] in (A.ND_VM_KURAMOTO, A.ND_VM_SWING) and vk["ff"] == A.ND_FF_PURESTATEMAP and
+                   vk["outdim_dst"] == 1 and vk["mask_dst"][0] == 1)
+            eok = (ek["model"] in (A.ND_EM_KURAMOTO, A.ND_EM_STATIC_LINE) and
+                   ek["wrapper"] == A.ND_WRAP_ANTISYMMETRIC and ek["dim"] == 0 and ek["outdim_dst"] == 1)
+            if vok and eok:
                 try:
                     self._ens1 = Ensemble([self.g], self.vertexm[0], self.edgem[0])
                 except (A.UnsupportedError, A.ArgumentError):

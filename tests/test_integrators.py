@@ -118,6 +118,19 @@ def test_ensemble_launch_shapes_agree(ND, dev):
         ens.set_class(0, 1024, 3)
 
 
+def test_network_tsit5_one_workgroup_eligibility(ND):
+    # networks outside the ensemble kernel's layout or models step on the host: two vertex
+    # batches, a diffusion edge, ODE edges; decided before any device work
+    L = ND.Lib
+    g = ND.watts_strogatz(30, 4, 0.3, seed=2)
+    two = [L.kuramoto_second() if i % 2 else L.load_vertex() for i in range(g.nv())]
+    for nw in (ND.Network(g, two, L.static_line_edge()),
+               ND.Network(g, L.diffusion_vertex(), L.diffusion_edge()),
+               ND.Network(g, L.kuramoto_second(), L.ode_lag_edge()),
+               ND.Network(ND.SimpleGraph(3, [], []), L.kuramoto_second(), L.kuramoto_edge())):
+        assert nw._one_instance_ensemble() is None
+
+
 @pytest.mark.gpu
 def test_network_tsit5_paths(ND, dev):
     # Network.tsit5: a single swing network runs in one workgroup and agrees with the
