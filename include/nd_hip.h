@@ -20,6 +20,8 @@
  *   nd_plan_set_perturbation   ForwardDiff pass in linearize_network, src/linear_analysis.jl:86-122)
  *   nd_gather                gather!(::EagerGBufProvider, gbuf, o)    src/gbufs.jl:25
  *   nd_aggregate             aggregate!(::Aggregator, aggbuf, data)   src/aggregators.jl:140-151 (additive)
+ *   nd_rk4, nd_tsit5         solve(ODEProblem(nw, u0, tspan, p), RK4() / Tsit5())  src/post_utils.jl:43-82
+ *                            (the OrdinaryDiffEq solves of the network's ODEFunction)
  */
 #ifndef ND_HIP_H
 #define ND_HIP_H
@@ -434,6 +436,16 @@ int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t
  * parameter of each tripped edge is set to 0 (the callback's affect on the parameters) */
 int nd_ensemble_solve(nd_ensemble* en, double* u, double* p, double t0, double t1,
                       const nd_ensemble_opts* opts, nd_ensemble_stat* stats, void* stream);
+/* Adaptive Tsit5 for one network (the Tsitouras 5(4) pair, OrdinaryDiffEq's PI
+ * controller and initial-dt heuristic), in place on u from t0 to t1, every step in one
+ * workgroup: the single-instance ensemble kernel, whose flat layout is the plan's.  For
+ * the networks nd_rk4 runs in one workgroup (one vertex batch of Kuramoto or swing
+ * vertices, one batch of AntiSymmetric Kuramoto or static-line edges, up to 2048
+ * vertices); any other network -> ND_ERR_UNSUPPORTED, and the host steps it on nd_rhs.
+ * dt0 = 0: the initial-dt heuristic; maxiters <= 0: 1e6.  stats: device pointer to one
+ * nd_ensemble_stat (final t, accepted and rejected steps, retcode), or NULL. */
+int nd_tsit5(nd_plan* plan, double* u, const double* p, double t0, double t1, double reltol, double abstol,
+             double dt0, int64_t maxiters, nd_ensemble_stat* stats, int64_t len_u, int64_t len_p, void* stream);
 /* The same solve captured once into a hipGraph (class launches on parallel branches)
  * and replayed with nd_ensemble_graph_launch on any stream; u, p, stats, tspan and
  * opts are fixed at capture.  A graph refuses to launch after nd_ensemble_set_class

@@ -1561,7 +1561,8 @@ struct nd_plan {
     int32_t rk4_path = 1;          // 1 automatic, 2 per-stage launches
     std::vector<int64_t> small_src, small_dst;
     nd_kind small_vk{}, small_ek{};
-    nd_ensemble* small_en = nullptr;
+    nd_ensemble* small_en = nullptr;      // RK4: 1024-thread workgroup
+    nd_ensemble* small_en_ts = nullptr;   // Tsit5: the default size classes
     nd_ensemble_stat* d_small_stat = nullptr;
     std::vector<int32_t> rowptr_host;
     std::vector<int2> slots_host;
@@ -1618,6 +1619,7 @@ struct nd_plan {
 
     ~nd_plan() {
         if (small_en) nd_ensemble_destroy(small_en);
+        if (small_en_ts) nd_ensemble_destroy(small_en_ts);
         for (void* p : allocations) (void)hipFree(p);
     }
 };
@@ -2352,6 +2354,65 @@ int nd_rhs_jvp(nd_plan* pl, double* du, double* jv, const double* u, const doubl
     return hip_check(hipGetLastError(), "nd_rhs_jvp launch");
 }
 
+// The single-instance ensemble of a network that fits one workgroup (created on first
+// use): one vertex batch in vertex order and one edge batch in edge order, so the plan's
+// flat layout is the ensemble's.  RK4 runs a 1024-thread workgroup with one or two
+// vertices per thread (16 waves hide the per-stage barriers better than 512 x 2 or
+// 512 x 4); Tsit5 keeps the default classes, since its 1024-thread kernels spill.
+static int one_instance_ensemble(nd_plan* pl, bool rk4) {
+    nd_ensemble*& en = rk4 ? pl->small_en : pl->small_en_ts;
+    if (en) return ND_OK;
+    const int64_t nv = static_cast<int64_t>(pl->vdesc[0].count), ne = static_cast<int64_t>(pl->small_src.size());
+    nd_ensemble* made = nullptr;
+    int rc = nd_ensemble_create(1, &nv, &ne, pl->small_src.data(), pl->small_dst.data(), &pl->small_vk,
+                                &pl->small_ek, &made);
+    if (!rc && rk4) rc = nd_ensemble_set_class(made, 0, 1024, nv <= 1024 ? 1 : 2);
+    if (!rc && !pl->d_small_stat) {
+        void* w = nullptr;
+        hipError_t e = hipMalloc(&w, sizeof(nd_ensemble_stat));
+        if (e != hipSuccess) rc = nd::set_error(ND_ERR_ALLOC, "hipMalloc(stats): %s", hipGetErrorString(e));
+        else {
+            pl->allocations.push_back(w);
+            pl->d_small_stat = static_cast<nd_ensemble_stat*>(w);
+        }
+    }
+    if (rc) {
+        if (made) nd_ensemble_destroy(made);
+        return rc;
+    }
+    en = made;
+    return ND_OK;
+}
+
+int nd_tsit5(nd_plan* pl, double* u, const double* p, double t0, double t1, double reltol, double abstol,
+             double dt0, int64_t maxiters, nd_ensemble_stat* stats, int64_t len_u, int64_t len_p, void* stream) {
+    if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
+    if (len_u != pl->dim) return nd::set_error(ND_ERR_ARGUMENT, "u does not have expected size %lld", (long long)pl->dim);
+    if (pl->pdim > 0 && len_p != pl->pdim)
+        return nd::set_error(ND_ERR_ARGUMENT, "p does not has expecte size %lld", (long long)pl->pdim);
+    if (!(t1 >= t0) || !(reltol > 0.0) || !(abstol > 0.0) || !(dt0 >= 0.0))
+        return nd::set_error(ND_ERR_ARGUMENT, "need t1 >= t0, positive tolerances and dt0 >= 0");
+    if (pl->dim > 0 && !u) return nd::set_error(ND_ERR_ARGUMENT, "NULL state pointer");
+    if (pl->pdim > 0 && !p) return nd::set_error(ND_ERR_ARGUMENT, "NULL parameter pointer");
+    if (!pl->rk4_single)
+        return nd::set_error(ND_ERR_UNSUPPORTED, "nd_tsit5 runs networks of one vertex batch and one edge batch "
+                             "that fit one workgroup (Kuramoto or swing vertices, AntiSymmetric Kuramoto or static "
+                             "line edges, up to 2048 vertices); step others with nd_rhs");
+    int rc = check_device(pl);
+    if (rc) return rc;
+    rc = one_instance_ensemble(pl, false);
+    if (rc) return rc;
+    nd_ensemble_opts o{};
+    o.method = ND_METHOD_TSIT5;
+    o.reltol = reltol;
+    o.abstol = abstol;
+    o.dt0 = dt0;
+    o.maxiters = maxiters;
+    // without line limits the ensemble kernel never writes p
+    return nd_ensemble_solve(pl->small_en_ts, u, const_cast<double*>(p), t0, t1, &o,
+                             stats ? stats : pl->d_small_stat, stream);
+}
+
 int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_t nsteps, double* work,
            int64_t len_u, int64_t len_p, void* stream) {
     if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
@@ -2364,22 +2425,8 @@ int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_
     int rc = check_device(pl);
     if (rc) return rc;
     if (pl->rk4_single && pl->rk4_path == 1 && nsteps > 0 && dt > 0.0) {
-        // one vertex batch (vertex order) and one edge batch (edge order): the plan's flat
-        // layout is the ensemble's layout for a single instance
-        if (!pl->small_en) {
-            const int64_t nv = static_cast<int64_t>(pl->vdesc[0].count), ne = static_cast<int64_t>(pl->small_src.size());
-            rc = nd_ensemble_create(1, &nv, &ne, pl->small_src.data(), pl->small_dst.data(), &pl->small_vk,
-                                    &pl->small_ek, &pl->small_en);
-            // a 1024-thread workgroup, one or two vertices per thread (16 waves hide the
-            // latency of the per-stage barriers better than 512 x 2 or 512 x 4)
-            if (!rc) rc = nd_ensemble_set_class(pl->small_en, 0, 1024, nv <= 1024 ? 1 : 2);
-            if (rc) return rc;
-            void* w = nullptr;
-            hipError_t e = hipMalloc(&w, sizeof(nd_ensemble_stat));
-            if (e != hipSuccess) return nd::set_error(ND_ERR_ALLOC, "hipMalloc(stats): %s", hipGetErrorString(e));
-            pl->allocations.push_back(w);
-            pl->d_small_stat = static_cast<nd_ensemble_stat*>(w);
-        }
+        rc = one_instance_ensemble(pl, true);
+        if (rc) return rc;
         nd_ensemble_opts o{};
         o.method = ND_METHOD_RK4;
         o.dt0 = dt;

@@ -484,52 +484,30 @@ class Network:
         the device vector u from tspan[0] to tspan[1]; returns t, naccept, nreject, retcode
         (0 done, 1 maxiters) and the path taken.
 
-        A network of one vertex batch and one edge batch that the ensemble kernel
-        implements (Kuramoto or swing vertices with output x[1], AntiSymmetric Kuramoto or
-        static-line edges) and that fits one workgroup runs every step in one launch: the
-        ensemble kernel with a single instance, whose flat layout is this network's.  Any
-        other network steps on the host, one device RHS per stage, with the same tableau
-        and controller."""
+        Networks that nd_tsit5 runs in one workgroup (one vertex batch of Kuramoto or swing
+        vertices, one batch of AntiSymmetric Kuramoto or static-line edges, up to 2048
+        vertices) take every step in one launch; any other network steps on the host, one
+        device RHS per stage, with the same tableau and controller."""
         import torch
+        from .ensemble import STAT_DTYPE, tsit5_per_instance
         self._require_identity_mass("tsit5")
-        _tensor_ptr(u, self.dim, "u", torch)
+        uptr, nu = _tensor_ptr(u, self.dim, "u", torch)
         if self.pdim > 0:
-            _tensor_ptr(p, self.pdim, "p", torch)
-        ens = self._one_instance_ensemble()
-        if ens is not None:
-            try:
-                st = ens.stats_to_numpy(ens.solve(u, p, tspan, reltol=reltol, abstol=abstol, dt=dt0,
-                                                  maxiters=maxiters))[0]
-                return dict(t=float(st["t"]), naccept=int(st["naccept"]), nreject=int(st["nreject"]),
-                            retcode=int(st["retcode"]), path="one workgroup")
-            except A.UnsupportedError:   # a size the kernel's limits refuse: host steps below
-                self._ens1 = None
-        from .ensemble import tsit5_per_instance
-        info = tsit5_per_instance(self, u, p, tspan, reltol, abstol, dt0, maxiters)
-        info.update(retcode=0 if info["t"] >= float(tspan[1]) else 1, path="host steps")
-        return info
-
-    def _one_instance_ensemble(self):
-        """The single-instance ensemble of this network, or None where its layout or models
-        are not the ensemble kernel's (cached)."""
-        if getattr(self, "_ens1", False) is False:
-            self._ens1 = None
-            from .ensemble import Ensemble
-            one = (self.nv() > 0 and self.ne() > 0 and len(self.batches(False)) == 1 and
-                   len(self.batches(True)) == 1 and self.info.lastidx_extbuf == 0)
-            if not one:
-                return None
-            vk, ek = self.kinds[self.vertex_kind[0]], self.kinds[self.edge_kind[0]]
-            vok = (vk["model"] in (A.ND_VM_KURAMOTO, A.ND_VM_SWING) and vk["ff"] == A.ND_FF_PURESTATEMAP and
-                   vk["outdim_dst"] == 1 and vk["mask_dst"][0] == 1)
-            eok = (ek["model"] in (A.ND_EM_KURAMOTO, A.ND_EM_STATIC_LINE) and
-                   ek["wrapper"] == A.ND_WRAP_ANTISYMMETRIC and ek["dim"] == 0 and ek["outdim_dst"] == 1)
-            if vok and eok:
-                try:
-                    self._ens1 = Ensemble([self.g], self.vertexm[0], self.edgem[0])
-                except (A.UnsupportedError, A.ArgumentError):
-                    self._ens1 = None
-        return self._ens1
+            pptr, np_ = _tensor_ptr(p, self.pdim, "p", torch)
+        else:
+            pptr, np_ = None, 0
+        stats = torch.zeros(STAT_DTYPE.itemsize // 8, dtype=torch.float64, device=u.device)
+        stream = torch.cuda.current_stream(u.device).cuda_stream
+        rc = A.lib().nd_tsit5(self.plan, uptr, pptr, float(tspan[0]), float(tspan[1]), float(reltol), float(abstol),
+                              float(dt0), int(maxiters), stats.data_ptr(), nu, np_, stream)
+        if rc == A.ND_ERR_UNSUPPORTED:
+            info = tsit5_per_instance(self, u, p, tspan, reltol, abstol, dt0, maxiters)
+            info.update(retcode=0 if info["t"] >= float(tspan[1]) else 1, path="host steps")
+            return info
+        A.check(rc)
+        st = stats.cpu().numpy().view(STAT_DTYPE)[0]
+        return dict(t=float(st["t"]), naccept=int(st["naccept"]), nreject=int(st["nreject"]),
+                    retcode=int(st["retcode"]), path="one workgroup")
 
     def get_buffers(self, u, p, t=0.0):
         """get_buffers(nw, u, p, t; initbufs=true): (o, aggbuf) device tensors, coreloop.jl:103-109."""

@@ -118,24 +118,12 @@ def test_ensemble_launch_shapes_agree(ND, dev):
         ens.set_class(0, 1024, 3)
 
 
-def test_network_tsit5_one_workgroup_eligibility(ND):
-    # networks outside the ensemble kernel's layout or models step on the host: two vertex
-    # batches, a diffusion edge, ODE edges; decided before any device work
-    L = ND.Lib
-    g = ND.watts_strogatz(30, 4, 0.3, seed=2)
-    two = [L.kuramoto_second() if i % 2 else L.load_vertex() for i in range(g.nv())]
-    for nw in (ND.Network(g, two, L.static_line_edge()),
-               ND.Network(g, L.diffusion_vertex(), L.diffusion_edge()),
-               ND.Network(g, L.kuramoto_second(), L.ode_lag_edge()),
-               ND.Network(ND.SimpleGraph(3, [], []), L.kuramoto_second(), L.kuramoto_edge())):
-        assert nw._one_instance_ensemble() is None
-
-
 @pytest.mark.gpu
 def test_network_tsit5_paths(ND, dev):
-    # Network.tsit5: a single swing network runs in one workgroup and agrees with the
-    # host-stepped path to tolerance; a network the ensemble kernel does not implement
-    # (load vertices next to swing ones, static lines) takes the host-stepped path itself
+    # Network.tsit5 (nd_tsit5 through the C ABI): a single swing network runs in one
+    # workgroup and agrees with the host-stepped path to tolerance, and bit for bit with
+    # the one-instance Ensemble; a network outside the kernel's models (load vertices next
+    # to swing ones, static lines) gets ND_ERR_UNSUPPORTED and takes the host-stepped path
     import torch
     gs, rng = _instances(ND, 1, 600, 600, seed=21)
     ps, us = _swing_params(ND, gs, rng)
@@ -148,6 +136,10 @@ def test_network_tsit5_paths(ND, dev):
     ref = ND.tsit5_per_instance(nw, uh, p, (0.0, 2.0))
     assert abs(ref["naccept"] - info["naccept"]) <= 2 + 0.02 * ref["naccept"]
     np.testing.assert_allclose(u.cpu().numpy(), uh.cpu().numpy(), rtol=0, atol=1e-6)
+    ue = torch.from_numpy(us[0]).to(dev)
+    ens = ND.Ensemble(gs, ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    st = ND.Ensemble.stats_to_numpy(ens.solve(ue, p, (0.0, 2.0), reltol=1e-8, abstol=1e-8))
+    assert torch.equal(u, ue) and int(st["naccept"][0]) == info["naccept"]
     g = gs[0]
     vm = [ND.Lib.kuramoto_second() if i % 3 else ND.Lib.load_vertex() for i in range(g.nv())]
     nw2 = ND.Network(g, vm, ND.Lib.static_line_edge())
