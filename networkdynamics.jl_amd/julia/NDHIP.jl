@@ -21,7 +21,7 @@ using NetworkDynamics: ExecutionStyle, Aggregator, IndexManager, ComponentBatch,
                        Symmetric, Directed, Fiducial, ExtMap, StateBufIdx, has_external_input
 using AMDGPU: ROCVector, AMDGPU
 
-export HIPExecution, SegmentedAggregator, register_model!
+export HIPExecution, SegmentedAggregator, register_model!, rk4!, tsit5!
 
 const libnd = normpath(joinpath(@__DIR__, "..", "libnd_hip.so"))
 
@@ -182,6 +182,45 @@ function (nw::Network{<:HIPExecution})(du::ROCVector{Float64}, u::ROCVector{Floa
                  AMDGPU.stream().stream))
     AMDGPU.synchronize()   # the reference call returns after its work is done (coreloop.jl:100)
     nothing
+end
+
+# Solves of ODEProblem(nw, u0, tspan, p) with RK4 / Tsit5 (src/post_utils.jl:43-82) in the
+# library's steppers, in place on u.  tsit5! returns `nothing` when the network does not fit
+# one workgroup (ND_ERR_UNSUPPORTED): solve it with OrdinaryDiffEq on `nw` instead.
+function rk4!(nw::Network{<:HIPExecution}, u::ROCVector{Float64}, p::ROCVector{Float64}, t0, dt, nsteps::Integer)
+    a = nw.layer.aggregator::SegmentedAggregator
+    _check(ccall((:nd_rk4, libnd), Cint,
+                 (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Int64, Ptr{Float64}, Int64, Int64, Ptr{Cvoid}),
+                 a.plan, pointer(u), pointer(p), Float64(t0), Float64(dt), Int64(nsteps), C_NULL, length(u), length(p),
+                 AMDGPU.stream().stream))
+    AMDGPU.synchronize()
+    u
+end
+
+# mirrors nd_ensemble_stat in include/nd_hip.h
+struct NDStat
+    t::Float64
+    dt::Float64
+    naccept::Int64
+    nreject::Int64
+    nf::Int64
+    retcode::Int32
+    ntrip::Int32
+end
+
+function tsit5!(nw::Network{<:HIPExecution}, u::ROCVector{Float64}, p::ROCVector{Float64}, tspan;
+                reltol=1e-8, abstol=1e-8, dt0=0.0, maxiters=10^6)
+    a = nw.layer.aggregator::SegmentedAggregator
+    st = ROCVector{NDStat}(undef, 1)
+    rc = ccall((:nd_tsit5, libnd), Cint,
+               (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64, Float64, Float64, Int64, Ptr{NDStat},
+                Int64, Int64, Ptr{Cvoid}),
+               a.plan, pointer(u), pointer(p), Float64(tspan[1]), Float64(tspan[2]), Float64(reltol), Float64(abstol),
+               Float64(dt0), Int64(maxiters), pointer(st), length(u), length(p), AMDGPU.stream().stream)
+    rc == 2 && return nothing                              # ND_ERR_UNSUPPORTED
+    _check(rc)
+    AMDGPU.synchronize()
+    Array(st)[1]
 end
 
 end # module
