@@ -89,7 +89,7 @@ def c4_instances(ninst, seed=1, nmin=100, nmax=2000):
 
 
 def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_seconds=30.0, seed=1, tune=0,
-                steady_state_tol=1e-5):
+                steady_state_tol=1e-5, one_workgroup_seconds=10.0):
     import torch
     t0 = time.perf_counter()
     inst, rng = c4_instances(ninst, seed)
@@ -184,6 +184,31 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
         if t_base > baseline_seconds:
             break
     _log(f"per-instance path: {done} trajectories in {t_base:.2f} s")
+    # one network at a time through nd_tsit5 (all steps of a network in one workgroup), on
+    # a time-bounded sample; each network's one-instance setup is inside the timing
+    done1, t_one, steps1, paths = 0, 0.0, [], set()
+    for i in order:
+        if one_workgroup_seconds <= 0:
+            break
+        nw = Network(graphs[i], Lib.kuramoto_second(), Lib.kuramoto_edge())
+        nw.plan                                                                  # plan construction not timed
+        ui = torch.from_numpy(us[i]).to(dev)
+        pi = torch.from_numpy(ps[i]).to(dev)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        info = nw.tsit5(ui, pi, (0.0, t1), reltol=reltol, abstol=abstol)
+        torch.cuda.synchronize()
+        t_one += time.perf_counter() - tb
+        steps1.append(info["naccept"])
+        paths.add(info["path"])
+        done1 += 1
+        if t_one > one_workgroup_seconds:
+            break
+    if done1:
+        _log(f"one-workgroup per-instance path: {done1} trajectories in {t_one:.2f} s")
+    one_wg = None if not done1 else dict(
+        instances=done1, seconds=t_one, traj_per_s=done1 / t_one, steps_mean=float(np.mean(steps1)),
+        paths=sorted(paths), ensemble_speedup=(len(graphs) / t_ens) / (done1 / t_one))
     traj_ens = len(graphs) / t_ens
     traj_base = done / t_base
     return dict(workload=f"{len(graphs)} BA(m=2) swing networks N~U{{100..2000}}, Tsit5 t=0..{t1}, "
@@ -196,7 +221,8 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
                 ensemble_retcodes=np.bincount(est["retcode"], minlength=5).tolist(),
                 returned_to_sync=int(sync.sum()), baseline_instances=done, baseline_s=t_base,
                 baseline_traj_per_s=traj_base, baseline_steps_mean=float(np.mean(steps)),
-                speedup=traj_ens / traj_base, classes=[list(c) for c in ens.classes], steady_state=ss,
+                speedup=traj_ens / traj_base, one_workgroup_per_instance=one_wg,
+                classes=[list(c) for c in ens.classes], steady_state=ss,
                 tuning=None if tuning is None else dict(predicted_s=tuning["predicted_s"], bins=tuning["bins"]))
 
 
