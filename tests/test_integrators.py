@@ -151,6 +151,11 @@ def test_network_tsit5_paths(ND, dev):
     b = torch.from_numpy(u2).to(dev)
     ND.tsit5_per_instance(nw2, b, torch.from_numpy(p2).to(dev), (0.0, 0.5))
     assert torch.equal(a, b)
+    # arguments are checked before either path is chosen
+    for bad in (dict(tspan=(1.0, 0.0)), dict(tspan=(0.0, 1.0), reltol=0.0)):
+        for net, uu, pp in ((nw, u, p), (nw2, a, torch.from_numpy(p2).to(dev))):
+            with pytest.raises(ND.ArgumentError):
+                net.tsit5(uu, pp, **bad)
 
 
 @pytest.mark.gpu
