@@ -446,6 +446,12 @@ int nd_ensemble_solve(nd_ensemble* en, double* u, double* p, double t0, double t
  * nd_ensemble_stat (final t, accepted and rejected steps, retcode), or NULL. */
 int nd_tsit5(nd_plan* plan, double* u, const double* p, double t0, double t1, double reltol, double abstol,
              double dt0, int64_t maxiters, nd_ensemble_stat* stats, int64_t len_u, int64_t len_p, void* stream);
+/* The Tsit5 coefficients the ensemble kernel and nd_tsit5 step with (Tsitouras 2011;
+ * the method of OrdinaryDiffEqTsit5's Tsit5(), docs/examples/cascading_failure.jl:145):
+ * a[7*6] the stage matrix row-major (row s: the coefficients of stage s+1), btilde[7] the
+ * error weights b - bhat, c[7] the nodes (stage s+1 is evaluated at t + c[s]*h).  Host
+ * only; no device work. */
+int nd_tsit5_tableau(double* a, double* btilde, double* c);
 /* The same solve captured once into a hipGraph (class launches on parallel branches)
  * and replayed with nd_ensemble_graph_launch on any stream; u, p, stats, tspan and
  * opts are fixed at capture.  A graph refuses to launch after nd_ensemble_set_class

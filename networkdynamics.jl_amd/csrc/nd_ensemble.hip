@@ -201,19 +201,36 @@ __device__ __forceinline__ bool at_steady_state(const SolveArgs& a, const InstDe
     return block_sum(fail, L.red) == 0.0;
 }
 
-// Tsitouras (2011) 5(4) coefficients
-constexpr double c_a21 = 0.161;
-constexpr double c_a31 = -0.008480655492356989, c_a32 = 0.335480655492357;
-constexpr double c_a41 = 2.897153057105493, c_a42 = -6.359448489975075, c_a43 = 4.3622954328695815;
-constexpr double c_a51 = 5.325864828439257, c_a52 = -11.748883564062828, c_a53 = 7.4955393428898365,
-                 c_a54 = -0.09249506636175525;
-constexpr double c_a61 = 5.86145544294642, c_a62 = -12.92096931784711, c_a63 = 8.159367898576159,
-                 c_a64 = -0.071584973281401, c_a65 = -0.028269050394068383;
-constexpr double c_a71 = 0.09646076681806523, c_a72 = 0.01, c_a73 = 0.4798896504144996,
-                 c_a74 = 1.379008574103742, c_a75 = -3.290069515436081, c_a76 = 2.324710524099774;
-constexpr double c_bt1 = -0.00178001105222577714, c_bt2 = -0.0008164344596567469, c_bt3 = 0.007880878010261995,
-                 c_bt4 = -0.1447110071732629, c_bt5 = 0.5823571654525552, c_bt6 = -0.45808210592918697,
-                 c_bt7 = 0.015151515151515152;
+// Tsitouras (2011) 5(4) coefficients: stage matrix a (row s = stage s+1), error weights
+// btilde = b - bhat (b = the last row, first same as last) and nodes c = a * 1.  The
+// kernel reads these names and nd_tsit5_tableau exports the same table, so the order
+// conditions checked on the exported copy hold for the kernel's.
+struct Tsit5Tableau {
+    double a[7][6];
+    double bt[7];
+    double c[7];
+};
+constexpr Tsit5Tableau kTsit5 = {
+    {{0, 0, 0, 0, 0, 0},
+     {0.161, 0, 0, 0, 0, 0},
+     {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},
+     {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},
+     {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},
+     {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383, 0},
+     {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774}},
+    {-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995, -0.1447110071732629,
+     0.5823571654525552, -0.45808210592918697, 0.015151515151515152},
+    {0.0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0}};
+constexpr double c_a21 = kTsit5.a[1][0];
+constexpr double c_a31 = kTsit5.a[2][0], c_a32 = kTsit5.a[2][1];
+constexpr double c_a41 = kTsit5.a[3][0], c_a42 = kTsit5.a[3][1], c_a43 = kTsit5.a[3][2];
+constexpr double c_a51 = kTsit5.a[4][0], c_a52 = kTsit5.a[4][1], c_a53 = kTsit5.a[4][2], c_a54 = kTsit5.a[4][3];
+constexpr double c_a61 = kTsit5.a[5][0], c_a62 = kTsit5.a[5][1], c_a63 = kTsit5.a[5][2], c_a64 = kTsit5.a[5][3],
+                 c_a65 = kTsit5.a[5][4];
+constexpr double c_a71 = kTsit5.a[6][0], c_a72 = kTsit5.a[6][1], c_a73 = kTsit5.a[6][2], c_a74 = kTsit5.a[6][3],
+                 c_a75 = kTsit5.a[6][4], c_a76 = kTsit5.a[6][5];
+constexpr double c_bt1 = kTsit5.bt[0], c_bt2 = kTsit5.bt[1], c_bt3 = kTsit5.bt[2], c_bt4 = kTsit5.bt[3],
+                 c_bt5 = kTsit5.bt[4], c_bt6 = kTsit5.bt[5], c_bt7 = kTsit5.bt[6];
 
 template <int VM, int EM, int METHOD, int V, int LB>
 __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
@@ -406,7 +423,11 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
             const double q11 = pow(eest, 0.14);
             if (eest <= 1.0) {
                 double qq = eest == 0.0 ? 0.1 : fmax(0.1, fmin(5.0, q11 / pow(qold, 0.08) / 0.9));
-                tcur = last ? tstop : tcur + h;
+                // a step that ends within 100 ulp of the stop time ends on it (as
+                // OrdinaryDiffEq's fixed_t_for_floatingpoint_error! does)
+                const double tn = tcur + h;
+                const bool at_stop = last || fabs(tn - tstop) < 100.0 * 2.220446049250313e-16 * fmax(fabs(tcur), fabs(tstop));
+                tcur = at_stop ? tstop : tn;
 #pragma unroll
                 for (int q = 0; q < V; ++q)
 #pragma unroll
@@ -422,7 +443,7 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
                     ntrip += tr;
                     changed += tr;
                 }
-                if (last && ee >= 0 && tstop == te) {   // the preset trip's time is reached
+                if (at_stop && ee >= 0 && tstop == te) {   // the preset trip's time is reached
                     const int tr = trip_edge<VM, EM>(a, I, L, ee);
                     ntrip += tr;
                     changed += tr;
@@ -843,6 +864,16 @@ int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t
     en->cls_threads[cls] = threads;
     en->cls_vper[cls] = vper;
     return regroup(en);
+}
+
+int nd_tsit5_tableau(double* a, double* btilde, double* c) {
+    if (!a || !btilde || !c) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
+    for (int i = 0; i < 7; ++i) {
+        for (int j = 0; j < 6; ++j) a[i * 6 + j] = kTsit5.a[i][j];
+        btilde[i] = kTsit5.bt[i];
+        c[i] = kTsit5.c[i];
+    }
+    return ND_OK;
 }
 
 int nd_ensemble_solve(nd_ensemble* en, double* u, double* p, double t0, double t1, const nd_ensemble_opts* o,

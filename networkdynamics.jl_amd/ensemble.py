@@ -17,19 +17,18 @@ TSIT5, RK4 = 0, 1
 NCLASS = 6
 DEFAULT_CLASSES = [(128, 2), (256, 2), (512, 2), (512, 3), (512, 4), (512, 4)]   # (threads, vertices per thread)
 
-# Tsitouras (2011) 5(4) pair: stage matrix, solution weights b (= last stage row,
-# first-same-as-last) and error weights btilde = b - bhat.
-TSIT5_A = [
-    [],
-    [0.161],
-    [-0.008480655492356989, 0.335480655492357],
-    [2.897153057105493, -6.359448489975075, 4.3622954328695815],
-    [5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525],
-    [5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383],
-    [0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774],
-]
-TSIT5_BTILDE = [-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995, -0.1447110071732629,
-                0.5823571654525552, -0.45808210592918697, 0.015151515151515152]
+
+def tsit5_tableau():
+    """The Tsitouras (2011) 5(4) pair the device steppers use, read from the library
+    (nd_tsit5_tableau): A (7x7, strictly lower triangular; row s = stage s+1), btilde =
+    b - bhat (7; b is A's last row, first same as last) and the nodes c (7)."""
+    a = np.zeros(42)
+    bt = np.zeros(7)
+    c = np.zeros(7)
+    A.check(A.lib().nd_tsit5_tableau(a.ctypes.data, bt.ctypes.data, c.ctypes.data))
+    tab = np.zeros((7, 7))
+    tab[:, :6] = a.reshape(7, 6)
+    return tab, bt, c
 
 
 class _Opts(C.Structure):
@@ -216,11 +215,21 @@ class EnsembleGraph:
             self._h = None
 
 
-def tsit5_per_instance(nw, u, p, tspan, reltol=1e-8, abstol=1e-8, dt0=0.0, maxiters=10 ** 7):
-    """Adaptive Tsit5 for one network: device RHS per stage, host-side step control
-    (same tableau and PI controller as the ensemble kernel).  u: device vector, in place."""
+def tsit5_per_instance(nw, u, p, tspan, reltol=1e-8, abstol=1e-8, dt0=0.0, maxiters=10 ** 7, dtmin=0.0):
+    """Adaptive Tsit5 for one network: device RHS per stage at t + c_s*h, host-side step
+    control (the tableau, PI controller, initial-dt heuristic and retcodes of the ensemble
+    kernel).  u: device vector, in place.  Returns t, naccept, nreject and retcode: 0
+    reached tspan[1], 1 maxiters, 2 dt below dtmin (default 1e-14*max(1, |t1|)), 3 NaN
+    in the error estimate or the step size."""
     import torch
     t0, t1 = float(tspan[0]), float(tspan[1])
+    if not t1 >= t0:
+        raise A.ArgumentError("tspan[1] < tspan[0]")
+    if t1 == t0:
+        return dict(t=t0, naccept=0, nreject=0, retcode=0)
+    tab, btilde, c = tsit5_tableau()
+    if not dtmin > 0:
+        dtmin = 1e-14 * max(1.0, abs(t1))
     ks = [torch.empty_like(u) for _ in range(7)]
     ys = torch.empty_like(u)
     nw(ks[0], u, p, t0)
@@ -241,23 +250,34 @@ def tsit5_per_instance(nw, u, p, tspan, reltol=1e-8, abstol=1e-8, dt0=0.0, maxit
         h1 = max(1e-6, h0 * 1e-3) if m <= 1e-15 else (0.01 / m) ** (1.0 / 6.0)
         dt = min(100 * h0, h1)
     dt = min(dt, t1 - t0)
-    t, qold, naccept, nreject, it = t0, 1e-4, 0, 0, 0
-    while t < t1 and it < maxiters:
+    t, qold, naccept, nreject, it, retcode = t0, 1e-4, 0, 0, 0, 0
+    while t < t1:
+        if it >= maxiters:
+            retcode = 1
+            break
+        if dt < dtmin:
+            retcode = 2
+            break
         it += 1
         last = t + dt >= t1
         h = t1 - t if last else dt
         for s in range(1, 7):
             ys.copy_(u)
-            for j, a in enumerate(TSIT5_A[s]):
-                ys.add_(ks[j], alpha=h * a)
-            nw(ks[s], ys, p, t + h)
-        err = sum(bt * k for bt, k in zip(TSIT5_BTILDE, ks)) * h
+            for j in range(s):
+                ys.add_(ks[j], alpha=h * tab[s, j])
+            nw(ks[s], ys, p, t + c[s] * h)
+        err = sum(float(bt) * k for bt, k in zip(btilde, ks)) * h
         sc = abstol + torch.maximum(u.abs(), ys.abs()) * reltol
         eest = torch.sqrt(torch.sum((err / sc) ** 2) / n).item()
+        if eest != eest:
+            retcode = 3
+            break
         q11 = eest ** 0.14
         if eest <= 1.0:
             qq = 0.1 if eest == 0 else max(0.1, min(5.0, q11 / qold ** 0.08 / 0.9))
-            t = t1 if last else t + h
+            tn = t + h
+            # a step ending within 100 ulp of t1 ends on it (fixed_t_for_floatingpoint_error!)
+            t = t1 if last or abs(tn - t1) < 100.0 * 2.220446049250313e-16 * max(abs(t), abs(t1)) else tn
             u.copy_(ys)
             ks[0].copy_(ks[6])
             qold = max(eest, 1e-4)
@@ -266,4 +286,7 @@ def tsit5_per_instance(nw, u, p, tspan, reltol=1e-8, abstol=1e-8, dt0=0.0, maxit
         else:
             dt = h / min(5.0, q11 / 0.9)
             nreject += 1
-    return dict(t=t, naccept=naccept, nreject=nreject)
+        if dt != dt:
+            retcode = 3
+            break
+    return dict(t=t, naccept=naccept, nreject=nreject, retcode=retcode)

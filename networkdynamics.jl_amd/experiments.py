@@ -88,12 +88,14 @@ def c4_instances(ninst, seed=1, nmin=100, nmax=2000):
     return out, rng
 
 
-def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_seconds=30.0, seed=1, tune=0,
-                steady_state_tol=1e-5, one_workgroup_seconds=10.0):
+def c4_start(dev, ninst=4096, seed=1, nmin=100, nmax=2000):
+    """The C4 starting states: uneven BA swing networks at their synchronous fixpoints
+    (batched Newton on the device) plus one perturbed node each (dtheta ~ U(-pi, pi),
+    domega ~ U(-10, 10)).  Returns graphs, per-instance states u and parameters p
+    (flat layout [M, D, P] per vertex then K per edge), the number of instances dropped
+    for lack of a fixpoint, and the Newton time."""
     import torch
-    t0 = time.perf_counter()
-    inst, rng = c4_instances(ninst, seed)
-    # synchronous fixpoints (batched Newton); drop instances without one
+    inst, rng = c4_instances(ninst, seed, nmin, nmax)
     fs = FixpointSolver([x[0] for x in inst])
     th = torch.zeros(int(fs.voff[-1]), dtype=torch.float64, device=dev)
     P = torch.from_numpy(np.concatenate([x[3] for x in inst])).to(dev)
@@ -107,9 +109,10 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
     ok = st["retcode"] == 0
     _log(f"fixpoints: {int(ok.sum())} of {len(inst)} in {t_newton:.2f} s")
     thh = th.cpu().numpy()
-    keep = [i for i in range(len(inst)) if ok[i]]
     graphs, us, ps = [], [], []
-    for i in keep:
+    for i in range(len(inst)):
+        if not ok[i]:
+            continue
         g, M, D, Pi, Ki = inst[i]
         n = g.nv()
         u = np.zeros(2 * n)
@@ -122,6 +125,19 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
         graphs.append(g)
         us.append(u)
         ps.append(p)
+    return graphs, us, ps, int((~ok).sum()), t_newton, rng
+
+
+def returned_to_sync(u):
+    """Sync classification of a swing state (theta, omega interleaved): every |omega| < 0.5."""
+    return bool(np.abs(np.asarray(u)[1::2]).max() < 0.5)
+
+
+def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_seconds=30.0, seed=1, tune=0,
+                steady_state_tol=1e-5, one_workgroup_seconds=10.0):
+    import torch
+    t0 = time.perf_counter()
+    graphs, us, ps, dropped, t_newton, rng = c4_start(dev, ninst, seed)
     t_setup = time.perf_counter() - t0
     ens = Ensemble(graphs, Lib.kuramoto_second(), Lib.kuramoto_edge())
     u_d = torch.from_numpy(np.concatenate(us)).to(dev)
@@ -138,7 +154,7 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
     torch.cuda.synchronize()
     t_ens = time.perf_counter() - ts
     uf = u_d.cpu().numpy()
-    sync = np.array([np.abs(uf[ens.state_slice(i)][1::2]).max() < 0.5 for i in range(len(graphs))])
+    sync = np.array([returned_to_sync(uf[ens.state_slice(i)]) for i in range(len(graphs))])
     # the same solve captured once as a hipGraph and replayed
     u_g = torch.from_numpy(np.concatenate(us)).to(dev)
     graph = ens.capture(u_g, p_d, (0.0, t1), method=TSIT5, reltol=reltol, abstol=abstol)
@@ -213,7 +229,7 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
     traj_base = done / t_base
     return dict(workload=f"{len(graphs)} BA(m=2) swing networks N~U{{100..2000}}, Tsit5 t=0..{t1}, "
                          f"reltol={reltol}, abstol={abstol}",
-                instances=len(graphs), dropped_without_fixpoint=int((~ok).sum()), newton_s=t_newton,
+                instances=len(graphs), dropped_without_fixpoint=dropped, newton_s=t_newton,
                 setup_s=t_setup, ensemble_s=t_ens, ensemble_traj_per_s=traj_ens,
                 graph_replay_s=t_graph, graph_replay_traj_per_s=len(graphs) / t_graph,
                 graph_replay_equals_eager=graph_equal,

@@ -911,3 +911,174 @@ int ndo_rk4(ndo_net* nw, double* u, const double* p, double t0, double dt, int64
     free(k1); free(k2); free(k3); free(k4); free(y);
     return 0;
 }
+
+/* ---- Tsit5 -------------------------------------------------------------------------
+ * Adaptive Tsit5: the 5(4) pair of Ch. Tsitouras, "Runge-Kutta pairs of order 5(4)
+ * satisfying only the first column simplifying assumption", Comput. Math. Appl. 62
+ * (2011) 770-775, with the step control of OrdinaryDiffEqTsit5 (compat "2",
+ * test/Project.toml:36,96; the solver of docs/examples/cascading_failure.jl:24,145):
+ *   error norm   sqrt(mean((h*sum_i btilde_i k_i / (abstol + reltol*max(|y|,|ynew|)))^2))
+ *   PI control   q11 = E^beta1, q = clamp(q11 / qold^beta2 / gamma, 1/qmax, 1/qmin),
+ *                beta1 = 7/50, beta2 = 2/25, gamma = 9/10, qmin = 1/5, qmax = 10,
+ *                qold starts at 1e-4 and becomes max(E, 1e-4) on accept; E == 0 gives
+ *                q = 1/qmax; a reject takes h / min(1/qmin, q11/gamma)
+ *   initial dt   Hairer, Norsett & Wanner, Solving ODEs I, II.4 (d0, d1, d2; order+1 = 6)
+ * Stages are evaluated at t + c_i h.  The last step is shortened to end on t1 (or on a
+ * preset trip time, a tstop); a step that ends within 100 ulp of the stop time ends on it
+ * (fixed_t_for_floatingpoint_error! in OrdinaryDiffEq).  Callbacks after an accepted step, in this order: line
+ * trips (|edge dst output| > limit sets the edge's first parameter to 0), the preset
+ * trip, then a re-evaluated first-same-as-last derivative if a parameter changed, then
+ * the steady-state test (every |du_i| <= ss_abstol or <= ss_reltol*|u_i|).
+ * retcodes: 0 reached t1, 1 maxiters, 2 dt < dtmin, 3 NaN step size, 4 steady state.
+ * stats[6] = t, dt, naccept, nreject, nf, ntrip.
+ * -------------------------------------------------------------------------------------*/
+static const double T5_C[7] = {0.0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0};
+static const double T5_A[7][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {0.161, 0, 0, 0, 0, 0},
+    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383, 0},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774},
+};
+/* btilde = b - bhat (b = last row of A, first same as last) */
+static const double T5_BT[7] = {-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995,
+                                -0.1447110071732629, 0.5823571654525552, -0.45808210592918697,
+                                0.015151515151515152};
+
+/* the coefficients above, for the order-condition test: A[7*6] row-major, btilde[7], c[7] */
+void ndo_tsit5_tableau(double* A, double* bt, double* c) {
+    for (int i = 0; i < 7; ++i) {
+        for (int j = 0; j < 6; ++j) A[i * 6 + j] = T5_A[i][j];
+        bt[i] = T5_BT[i];
+        c[i] = T5_C[i];
+    }
+}
+
+static double wnorm2(const double* x, const double* r1, const double* r2, int64_t n, double abstol, double reltol) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double m = r2 ? fmax(fabs(r1[i]), fabs(r2[i])) : fabs(r1[i]);
+        const double q = x[i] / (abstol + m * reltol);
+        s += q * q;
+    }
+    return s;
+}
+
+/* edge dst outputs at (u, p, t) into flow[ne] (the trip condition's |P_line|) */
+static void edge_flows(ndo_net* nw, double* flow, const double* u, const double* p, double t, double* scratch_du) {
+    rhs_impl(nw, scratch_du, u, p, t, 1, NULL, NULL, NULL);
+    for (int64_t e = 0; e < nw->ne; ++e) flow[e] = nw->o[nw->e_out_dst[e].first - 1];
+}
+
+int ndo_tsit5(ndo_net* nw, double* u, double* p, double t0, double t1, double reltol, double abstol, double dt0,
+              double dtmax, double dtmin, int64_t maxiters, const double* line_limit, double ss_abstol,
+              double ss_reltol, double event_t, int64_t event_e, double* stats) {
+    const int64_t n = nw->lastidx_dynamic;
+    double* k[7];
+    for (int s = 0; s < 7; ++s) k[s] = (double*)xcalloc((size_t)n, 8);
+    double* ys = (double*)xcalloc((size_t)n, 8);
+    double* err = (double*)xcalloc((size_t)n, 8);
+    double* flow = (double*)xcalloc((size_t)(nw->ne ? nw->ne : 1), 8);
+    if (!(dtmax > 0)) dtmax = t1 - t0;
+    if (!(dtmin > 0)) dtmin = 1e-14 * fmax(1.0, fabs(t1));
+    if (maxiters <= 0) maxiters = 1000000;
+    const int ss = ss_abstol > 0 || ss_reltol > 0;
+    double te = INFINITY;
+    int64_t ee = -1;
+    if (event_e >= 0 && event_e < nw->ne && event_t > t0 && event_t <= t1) { te = event_t; ee = event_e; }
+    int64_t naccept = 0, nreject = 0, nf = 0, ntrip = 0, iters = 0;
+    int retcode = 0;
+    double t = t0;
+    rhs_impl(nw, k[0], u, p, t, 1, NULL, NULL, NULL);
+    ++nf;
+    double dt = dt0;
+    const double inv_len = 1.0 / (double)n;
+    if (!(dt > 0) && t1 > t0) {
+        const double d0 = sqrt(wnorm2(u, u, NULL, n, abstol, reltol) * inv_len);
+        const double d1 = sqrt(wnorm2(k[0], u, NULL, n, abstol, reltol) * inv_len);
+        double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+        h0 = fmin(h0, t1 - t0);
+        for (int64_t i = 0; i < n; ++i) ys[i] = u[i] + h0 * k[0][i];
+        rhs_impl(nw, k[1], ys, p, t + h0, 1, NULL, NULL, NULL);
+        ++nf;
+        for (int64_t i = 0; i < n; ++i) err[i] = k[1][i] - k[0][i];
+        const double d2 = sqrt(wnorm2(err, u, NULL, n, abstol, reltol) * inv_len) / h0;
+        const double m = fmax(d1, d2);
+        const double h1 = m <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / m, 1.0 / 6.0);
+        dt = fmin(100.0 * h0, h1);
+    }
+    dt = fmin(dt, dtmax);
+    double qold = 1e-4;
+    double tstop = ee >= 0 ? te : t1;
+    while (t < t1) {
+        if (iters++ >= maxiters) { retcode = 1; break; }
+        if (dt < dtmin) { retcode = 2; break; }
+        const int last = t + dt >= tstop;
+        const double h = last ? tstop - t : dt;
+        for (int s = 1; s < 7; ++s) {
+            for (int64_t i = 0; i < n; ++i) {
+                double acc = T5_A[s][0] * k[0][i];
+                for (int j = 1; j < s; ++j) acc = acc + T5_A[s][j] * k[j][i];
+                ys[i] = u[i] + h * acc;
+            }
+            rhs_impl(nw, k[s], ys, p, t + T5_C[s] * h, 1, NULL, NULL, NULL);
+        }
+        nf += 6;
+        for (int64_t i = 0; i < n; ++i) {
+            double e = T5_BT[0] * k[0][i];
+            for (int j = 1; j < 7; ++j) e = e + T5_BT[j] * k[j][i];
+            err[i] = h * e;
+        }
+        const double eest = sqrt(wnorm2(err, u, ys, n, abstol, reltol) * inv_len);
+        const double q11 = pow(eest, 0.14);
+        if (eest <= 1.0) {
+            const double qq = eest == 0.0 ? 0.1 : fmax(0.1, fmin(5.0, q11 / pow(qold, 0.08) / 0.9));
+            const double tn = t + h;
+            const int at_stop = last || fabs(tn - tstop) < 100.0 * 2.220446049250313e-16 * fmax(fabs(t), fabs(tstop));
+            t = at_stop ? tstop : tn;
+            memcpy(u, ys, (size_t)n * 8);
+            memcpy(k[0], k[6], (size_t)n * 8);
+            qold = fmax(eest, 1e-4);
+            dt = fmin(h / qq, dtmax);
+            ++naccept;
+            int changed = 0;
+            if (line_limit) {
+                edge_flows(nw, flow, u, p, t, err);
+                for (int64_t e = 0; e < nw->ne; ++e) {
+                    double* K = &p[nw->e_para[e].first - 1];
+                    if (*K != 0.0 && fabs(flow[e]) > line_limit[e]) { *K = 0.0; ++ntrip; ++changed; }
+                }
+            }
+            if (at_stop && ee >= 0 && tstop == te) {
+                double* K = &p[nw->e_para[ee].first - 1];
+                if (*K != 0.0) { ++ntrip; ++changed; }
+                *K = 0.0;
+                ee = -1;
+                tstop = t1;
+            }
+            if (changed) {
+                rhs_impl(nw, k[0], u, p, t, 1, NULL, NULL, NULL);
+                ++nf;
+            }
+            if (ss) {
+                int pass = 1;
+                for (int64_t i = 0; i < n && pass; ++i) {
+                    const double d = fabs(k[0][i]);
+                    if (!(d <= ss_abstol || d <= ss_reltol * fabs(u[i]))) pass = 0;
+                }
+                if (pass) { retcode = 4; break; }
+            }
+        } else {
+            dt = h / fmin(5.0, q11 / 0.9);
+            ++nreject;
+        }
+        if (!(dt == dt)) { retcode = 3; break; }
+    }
+    stats[0] = t; stats[1] = dt; stats[2] = (double)naccept; stats[3] = (double)nreject;
+    stats[4] = (double)nf; stats[5] = (double)ntrip;
+    for (int s = 0; s < 7; ++s) free(k[s]);
+    free(ys); free(err); free(flow);
+    return retcode;
+}

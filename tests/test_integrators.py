@@ -9,14 +9,8 @@ from conftest import oracle_for
 
 
 def _tableau(ND):
-    A_ = ND.ensemble.TSIT5_A
-    s = 7
-    A = np.zeros((s, s))
-    for i, row in enumerate(A_):
-        A[i, :len(row)] = row
+    A, bt, c = ND.ensemble.tsit5_tableau()
     b = A[6].copy()
-    bt = np.array(ND.ensemble.TSIT5_BTILDE)
-    c = A.sum(axis=1)
     return A, b, b - bt, c
 
 
