@@ -181,6 +181,73 @@ def test_swing_dense_bruteforce_and_invariant(ND):
     assert abs(lhs - (P.sum() - np.sum(D * w))) < 1e-11 * np.sum(np.abs(M * du[1::2]) + 1)
 
 
+def _swing_load_bruteforce(th, w, is_gen, M, D, P, tau, s, d, B, phi):
+    """The heterogeneous network's equations written out pair by pair: the static line
+    from s to d carries y = B sin(th_s - th_d - phi) into d and -y into s (AntiSymmetric,
+    test/ComponentLibrary.jl:51-57 with a phase shift); swing vertices
+    dth = w, dw = (P - D w + acc) / M (test/ComponentLibrary.jl:59-67); load vertices
+    tau dth = P + acc."""
+    n = len(th)
+    acc = np.zeros(n)
+    for i in range(n):
+        for e in range(len(s)):
+            if d[e] == i:
+                acc[i] += B[e] * np.sin(th[s[e]] - th[i] - phi[e])
+            if s[e] == i:
+                acc[i] -= B[e] * np.sin(th[i] - th[d[e]] - phi[e])
+    dth = np.where(is_gen, w, (P + acc) / tau)
+    dw = (P - D * w + acc) / M
+    return dth, dw
+
+
+def test_load_vertex_and_phase_shifted_line_bruteforce(ND):
+    """The heterogeneous swing/load network's first-order load vertex and phase-shifted
+    static line (build-defined kinds) against their equations written out pair by pair,
+    with phase shifts and time constants far from 0 and 1 so that a sign error on phi or
+    a product instead of a quotient by tau shows; plus the invariant
+    sum_gen M dw + sum_load tau dth = sum P - sum_gen D w: the src end of each line takes
+    exactly the negated dst output (AntiSymmetric), so the coupling cancels, phase shift
+    or not."""
+    rng = np.random.default_rng(31)
+    n = 40
+    g = ND.random_geometric(n, 0.3, seed=31)
+    is_gen = rng.random(n) < 0.4
+    swing, load = ND.Lib.kuramoto_second(), ND.Lib.load_vertex()
+    nw = ND.Network(g, [swing if x else load for x in is_gen], ND.Lib.static_line_edge())
+    on = oracle_for(nw)
+    v, e = nw.ranges()
+    M, D, P, tau = rng.uniform(1, 5, n), rng.uniform(0.1, 1, n), rng.normal(size=n), rng.uniform(0.3, 3, n)
+    B, phi = rng.uniform(1, 20, g.ne()), rng.uniform(0.2, 0.6, g.ne())
+    p = np.full(nw.pdim, np.nan)
+    u = np.full(nw.dim, np.nan)
+    th, w = rng.uniform(-np.pi, np.pi, n), np.where(is_gen, rng.normal(size=n), 0.0)
+    for i in range(n):
+        pf, sf = v[i, 2, 0] - 1, v[i, 0, 0] - 1
+        u[sf] = th[i]
+        if is_gen[i]:
+            p[pf:pf + 3] = M[i], D[i], P[i]      # psym [M, D, Pm] (ComponentLibrary.jl:64-66)
+            u[sf + 1] = w[i]
+        else:
+            p[pf:pf + 2] = tau[i], P[i]
+    for k in range(g.ne()):
+        p[e[k, 3, 0] - 1:e[k, 3, 0] + 1] = B[k], phi[k]
+    assert not np.isnan(p).any() and not np.isnan(u).any()
+    du = on.rhs(u, p)
+    dth_ref, dw_ref = _swing_load_bruteforce(th, w, is_gen, M, D, P, tau, g.src - 1, g.dst - 1, B, phi)
+    dth = du[v[:, 0, 0] - 1]
+    np.testing.assert_allclose(dth, dth_ref, rtol=1e-13, atol=1e-13)
+    gen = np.flatnonzero(is_gen)
+    np.testing.assert_allclose(du[v[gen, 0, 0]], dw_ref[gen], rtol=1e-13, atol=1e-13)
+    # invariant: the coupling cancels line by line
+    lhs = np.sum(M[gen] * du[v[gen, 0, 0]]) + np.sum(np.where(is_gen, 0.0, tau * dth))
+    rhs = P.sum() - np.sum(D[gen] * w[gen])
+    assert abs(lhs - rhs) < 1e-12 * (np.abs(P).sum() + B.sum())
+    # the mistakes the check is there for
+    wrong_phi = _swing_load_bruteforce(th, w, is_gen, M, D, P, tau, g.src - 1, g.dst - 1, B, -phi)[0]
+    wrong_tau = _swing_load_bruteforce(th, w, is_gen, M, D, P, 1.0 / tau, g.src - 1, g.dst - 1, B, phi)[0]
+    assert np.abs(dth - wrong_phi).max() > 1e-2 and np.abs(dth - wrong_tau).max() > 1e-2
+
+
 def test_swing_ring_fixpoint(ND):
     # test/initialization_test.jl:15-38: ring of 4, P = [1,-1,1,-1], K = 6: each line carries 0.5
     g = ND.cycle_graph(4)
