@@ -2401,6 +2401,7 @@ int nd_tsit5(nd_plan* pl, double* u, const double* p, double t0, double t1, doub
     int rc = check_device(pl);
     if (rc) return rc;
     rc = one_instance_ensemble(pl, false);
+    if (rc == ND_ERR_UNSUPPORTED) pl->rk4_single = false;   // too dense for one workgroup's LDS
     if (rc) return rc;
     nd_ensemble_opts o{};
     o.method = ND_METHOD_TSIT5;
@@ -2426,7 +2427,15 @@ int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_
     if (rc) return rc;
     if (pl->rk4_single && pl->rk4_path == 1 && nsteps > 0 && dt > 0.0) {
         rc = one_instance_ensemble(pl, true);
+        if (rc == ND_ERR_UNSUPPORTED) {
+            // the network's topology needs more LDS than one workgroup has (dense
+            // graphs): it steps through the stage launches below, and nd_tsit5 refuses it
+            pl->rk4_single = false;
+            rc = ND_OK;
+        }
         if (rc) return rc;
+    }
+    if (pl->rk4_single && pl->rk4_path == 1 && nsteps > 0 && dt > 0.0) {
         nd_ensemble_opts o{};
         o.method = ND_METHOD_RK4;
         o.dt0 = dt;

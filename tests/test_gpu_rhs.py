@@ -497,6 +497,27 @@ def test_rk4_single_workgroup_path(ND, dev, em, n):
     np.testing.assert_allclose(four, ref, rtol=0, atol=1e-10)
 
 
+@pytest.mark.gpu
+def test_rk4_dense_network_beyond_one_workgroup(ND, dev):
+    # a complete graph K_120 (7140 edges) fits the one-workgroup kernel by its model kinds
+    # and size, but its topology needs more LDS than a workgroup has: nd_rk4 steps it
+    # through the stage launches instead of refusing, and Network.tsit5 steps it on the host
+    rng = np.random.default_rng(57)
+    g = ND.complete_graph(120)
+    n, m = g.nv(), g.ne()
+    nw = ND.Network(g, ND.Lib.kuramoto_first(), ND.Lib.kuramoto_edge())
+    p = np.r_[rng.normal(size=n), rng.uniform(0.5, 1.0, m) / n]
+    u0 = rng.uniform(-np.pi, np.pi, n)
+    ref = oracle_for(nw).rk4(u0, p, 0.0, 0.01, 200)
+    assert nw.tuning()["rk4_path"] == 1
+    got = _dev_rk4(nw, u0, p, dev, 0.01, 200)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-10)
+    import torch
+    u = torch.from_numpy(u0).to(dev)
+    info = nw.tsit5(u, torch.from_numpy(p).to(dev), (0.0, 1.0))
+    assert info["path"] == "host steps" and info["retcode"] == 0
+
+
 def test_plain_pointer_round_trip_through_the_c_abi(ND, dev):
     # what a host shim without a GPU array package does: nd_malloc, nd_memcpy_h2d,
     # nd_rhs on the default stream, nd_device_synchronize, nd_memcpy_d2h, nd_free
