@@ -5,11 +5,11 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/nd_hip.h"
+#include "nd_math.h"
 
 namespace nd {
 
-using ::sin;   // the double overloads stay visible next to sin(Dual) and sqrt(Dual)
-using ::sqrt;
+using ::sqrt;   // the double overload stays visible next to sqrt(Dual)
 
 // First-order dual number v + d*eps (one tangent direction), the forward-mode scalar
 // the Jacobian-vector product evaluates the models with.  The model functions take the
@@ -31,7 +31,9 @@ __device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a
 __device__ __forceinline__ Dual operator/(Dual a, double c) { return {a.v / c, a.d / c}; }
 __device__ __forceinline__ Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
 __device__ __forceinline__ Dual operator/(double c, Dual b) { return {c / b.v, -c * b.d / (b.v * b.v)}; }
-__device__ __forceinline__ Dual sin(Dual a) { return {::sin(a.v), ::cos(a.v) * a.d}; }
+// the models' sine: nd_sin (nd_math.h) on doubles, and its derivative nd_cos on duals
+__device__ __forceinline__ double msin(double x) { return nd_sin(x); }
+__device__ __forceinline__ Dual msin(Dual a) { return {nd_sin(a.v), nd_cos(a.v) * a.d}; }
 __device__ __forceinline__ Dual sqrt(Dual a) {
     const double r = ::sqrt(a.v);
     return {r, a.d / (2.0 * r)};
@@ -60,13 +62,13 @@ __host__ __device__ constexpr int edge_ext_inputs(int em) { return em == ND_EM_E
 template <int EM, typename S, typename P = double>
 __device__ __forceinline__ void edge_ode_f(S* dx, const S* x, S vs, S vd, const S* ext, const P* __restrict__ pe) {
     if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION) {            // ComponentLibrary.jl:30-34
-        dx[0] = 1.0 / pe[0] * (sin(vs - vd) - x[0]);
-        dx[1] = 1.0 / pe[0] * (sin(vd - vs) - x[1]);
+        dx[0] = 1.0 / pe[0] * (msin(vs - vd) - x[0]);
+        dx[1] = 1.0 / pe[0] * (msin(vd - vs) - x[1]);
     } else if constexpr (EM == ND_EM_ODE_SIN_DIFFUSION_FAST) { // benchmark_models.jl:11-15
-        dx[0] = 100.0 * (sin(vs - vd) - x[0]);
-        dx[1] = 100.0 * (sin(vd - vs) - x[1]);
+        dx[0] = 100.0 * (msin(vs - vd) - x[0]);
+        dx[1] = 100.0 * (msin(vd - vs) - x[1]);
     } else if constexpr (EM == ND_EM_ODE_LAG) {               // ComponentLibrary.jl:30-32, one state
-        dx[0] = 1.0 / pe[0] * (sin(vs - vd) - x[0]);
+        dx[0] = 1.0 / pe[0] * (msin(vs - vd) - x[0]);
     } else if constexpr (EM == ND_EM_EXT_DRIVEN) {            // external_inputs_test.jl:45-47
         dx[0] = ext[0];
     } else {                                                   // diffusion_test.jl:102-106
@@ -89,8 +91,8 @@ __device__ __forceinline__ void edge_ode_f_rt(int em, S* dx, const S* x, S vs, S
 // dst-side output of a pure feed-forward edge model g(v_s, v_d, p); S = double or Dual
 template <int EM, typename S = double, typename P = double>
 __device__ __forceinline__ S g_dst(S vs, S vd, const P* __restrict__ pe) {
-    if constexpr (EM == ND_EM_KURAMOTO) return pe[0] * sin(vs - vd);             // ComponentLibrary.jl:51-53
-    else if constexpr (EM == ND_EM_STATIC_LINE) return pe[0] * sin(vs - vd - pe[1]);
+    if constexpr (EM == ND_EM_KURAMOTO) return pe[0] * msin(vs - vd);             // ComponentLibrary.jl:51-53
+    else if constexpr (EM == ND_EM_STATIC_LINE) return pe[0] * msin(vs - vd - pe[1]);
     else if constexpr (EM == ND_EM_DIFFUSION) return vs - vd;                     // diffusion_test.jl:57-60
     else if constexpr (EM == ND_EM_DIFFUSION_WEIGHTED) return pe[0] * (vs - vd);  // ComponentLibrary.jl:8-10
     else if constexpr (EM == ND_EM_DIFFUSION_FIDUCIAL) return pe[0] * (vs - vd);  // ComponentLibrary.jl:22-25
@@ -170,7 +172,7 @@ __device__ __forceinline__ void vertex_f_ext(S* __restrict__ dx, const S* __rest
                                              const P* __restrict__ pv, const S* __restrict__ ext, double t,
                                              int dim) {
     if constexpr (VM == ND_VM_CAPACITOR_SINK) {            // external_inputs.md:76-79
-        dx[0] = 1.0 / pv[0] * (acc - (1.0 + 0.1 * sin(t)));
+        dx[0] = 1.0 / pv[0] * (acc - (1.0 + 0.1 * msin(t)));
     } else if constexpr (VM == ND_VM_PI_SOURCE) {          // external_inputs.md:87-92
         const S d = pv[1] - ext[0];                        // tracking error Vref - V_remote
         dx[1] = d;                                         // integrator state

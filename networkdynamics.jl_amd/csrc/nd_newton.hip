@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "nd_internal.h"
+#include "nd_math.h"
 
 namespace {
 
@@ -112,7 +113,7 @@ __global__ void __launch_bounds__(kThreads) k_newton(NewtonArgs a) {
                 if (a.linearized)
                     for (int s = rp[v]; s < rp[v + 1]; ++s) acc += a.K[se[s]] * (th[nb[s]] - own);
                 else
-                    for (int s = rp[v]; s < rp[v + 1]; ++s) acc += a.K[se[s]] * sin(th[nb[s]] - own);
+                    for (int s = rp[v]; s < rp[v + 1]; ++s) acc += a.K[se[s]] * nd::nd_sin(th[nb[s]] - own);
                 out[k] = Pg[v] + acc;
                 m = fmax(m, fabs(out[k] / scg[v]));
             }
@@ -134,7 +135,7 @@ __global__ void __launch_bounds__(kThreads) k_newton(NewtonArgs a) {
                 const double own = th[v];
                 double dsum = 0.0;
                 for (int s = rp[v]; s < rp[v + 1]; ++s) {
-                    const double ws = a.linearized ? a.K[se[s]] : a.K[se[s]] * cos(th[nb[s]] - own);
+                    const double ws = a.linearized ? a.K[se[s]] : a.K[se[s]] * nd::nd_cos(th[nb[s]] - own);
                     w[s] = ws;
                     dsum += ws;
                 }
