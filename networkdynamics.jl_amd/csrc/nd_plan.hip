@@ -214,6 +214,14 @@ __device__ __forceinline__ void vertex_emit(double* __restrict__ du, const Stage
     }
 }
 
+// Orders LDS accesses across the lanes of one wave (no block-wide barrier): earlier LDS
+// writes of any lane are visible to later reads of every lane of the wave.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
@@ -252,22 +260,27 @@ __device__ __forceinline__ double segment_sum(int32_t beg, int32_t end, int32_t 
         for (int32_t s = beg + t; s < end; s += G) acc += slot_value_rt(ed[slot_eb[s]], slots[s], own, u, vo, p);
     } else {
         constexpr int NP = EModel<EM>::np;
+        // Entries past the segment's end load its last entry instead (a valid index): with
+        // no branch around them the loads all go out before the first wait, where loads
+        // guarded by a condition are each followed by a wait for their result.
         for (int32_t s0 = beg + t; s0 < end; s0 += kUnroll * G) {
             int2 sl[kUnroll];
             double nb[kUnroll];
             double kk[kUnroll][NP > 0 ? NP : 1];
 #pragma unroll
-            for (int j = 0; j < kUnroll; ++j)
-                if (s0 + j * G < end) sl[j] = slots[s0 + j * G];
+            for (int j = 0; j < kUnroll; ++j) sl[j] = slots[min(s0 + j * G, end - 1)];
 #pragma unroll
-            for (int j = 0; j < kUnroll; ++j)
-                if (s0 + j * G < end) {
-                    if constexpr (is_ode_model(EM)) nb[j] = u[sl[j].x];
-                        else if constexpr (EM != ND_EM_NOOP) nb[j] = vo[sl[j].x];
+            for (int j = 0; j < kUnroll; ++j) {
+                const int32_t sc = min(s0 + j * G, end - 1);
 #pragma unroll
-                    for (int c = 0; c < NP; ++c)
-                        kk[j][c] = KC ? kslot[static_cast<int64_t>(s0 + j * G) * NP + c] : p[(sl[j].y >> 1) + c];
-                }
+                for (int c = 0; c < NP; ++c)
+                    kk[j][c] = KC ? kslot[static_cast<int64_t>(sc) * NP + c] : p[(sl[j].y >> 1) + c];
+            }
+#pragma unroll
+            for (int j = 0; j < kUnroll; ++j) {
+                if constexpr (is_ode_model(EM)) nb[j] = u[sl[j].x];
+                else if constexpr (EM != ND_EM_NOOP) nb[j] = vo[sl[j].x];
+            }
 #pragma unroll
             for (int j = 0; j < kUnroll; ++j)
                 if (s0 + j * G < end) acc += slot_eval<EM, WRAP>(sl[j], nb[j], kk[j], own);
@@ -277,7 +290,8 @@ __device__ __forceinline__ double segment_sum(int32_t beg, int32_t end, int32_t 
 }
 
 // LDS of the fused vertex kernels (each kernel that uses them gets its own copy): the
-// low path's entry window (values, owners) and own outputs, the hub blocks' wave partials
+// low path's per-wave entry windows (values, owners) and own outputs, the hub blocks'
+// wave partials (blocks of at most 256 threads)
 __shared__ double rhs_val[4 * 256];
 __shared__ double rhs_own[256];
 __shared__ uint8_t rhs_owner[4 * 256];
@@ -338,92 +352,100 @@ __device__ __forceinline__ void vertex_rhs_body(const VDesc& vd, int32_t b,
         }
         b -= nmidb;
     }
-    // Low-degree vertices: block b takes the T consecutive vertices q0..q0+T-1 and their
-    // hub-free destination lists lrp/lslots.  Entries are evaluated lane-densely in LDS
-    // windows of 4*T (loads issued together, no lane idles on a neighbour's long
-    // segment); then each thread sums its own segment from LDS in order, so the
-    // summation order is the reference's.
-    const int32_t T = static_cast<int32_t>(blockDim.x);
-    const int32_t t = static_cast<int32_t>(threadIdx.x);
-    const int32_t q0 = b * T;
-    const int32_t q = q0 + t;
+    // Low-degree vertices: wave w of block b takes the 64 consecutive vertices
+    // q0..q0+63 and their hub-free destination lists lrp/lslots.  Entries are evaluated
+    // lane-densely in per-wave LDS windows of 4*64 (loads issued together, no lane idles
+    // on a neighbour's long segment); then each lane sums its own segment from LDS in
+    // order, so the summation order is the reference's.  The waves of a block share no
+    // data and synchronise only within themselves.
+    const int32_t lane = static_cast<int32_t>(threadIdx.x & (kWave - 1));
+    const int32_t wib = static_cast<int32_t>(threadIdx.x / kWave);
+    const int32_t q0 = (b * static_cast<int32_t>(blockDim.x / kWave) + wib) * kWave;
+    const int32_t q = q0 + lane;
     if (q0 >= vd.count) return;
     const bool valid = q < vd.count;
-    const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(valid ? q : q0) * dim;
+    const int32_t qv = valid ? q : q0;             // a valid vertex for the loads of idle lanes
+    const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(qv) * dim;
     if constexpr (VM == ND_VM_NOOP) {
         if (valid) vertex_emit<VM, STAGE>(du, sa, x0, u + x0, 0.0, p, dim, vd.mask[0], static_cast<int64_t>(vd.out0) + q);
     } else {
         const int32_t* __restrict__ lrp = vd.lrp;
         // lrp[q] = offset | (hub ? 1 << 31 : 0): hub vertices keep an empty segment here and
-        // their du is written by the hub paths
+        // their du is written by the hub paths.  One offset load per lane: a segment ends
+        // where the next lane's begins, the wave's last one at lrp[q0 + 64].
         constexpr int32_t kOff = 0x7fffffff;
-        const int32_t qc = valid ? q : vd.count;
-        const int32_t raw = lrp[qc];
+        const int32_t raw = lrp[valid ? q : vd.count];
+        const int32_t tail = lrp[min(q0 + kWave, vd.count)];
         const bool hub = raw < 0;
         const int32_t sb = raw & kOff;
-        const int32_t se = valid ? (lrp[qc + 1] & kOff) : sb;
-        const int32_t S0 = lrp[q0] & kOff;
-        const int32_t S1 = lrp[min(q0 + T, vd.count)] & kOff;
+        const int32_t nxt = __shfl_down(raw, 1, kWave);
+        const int32_t se = valid ? ((lane == kWave - 1 ? tail : nxt) & kOff) : sb;
+        const int32_t S0 = __shfl(sb, 0, kWave);
+        const int32_t S1 = tail & kOff;
         // own state and parameters are loaded up front, in parallel with the offsets
         constexpr int DC = VModel<VM>::dim > 0 ? VModel<VM>::dim : 1;
         constexpr int NPV = VPar<VM>::np > 0 ? VPar<VM>::np : 1;
         double xs[DC], pvs[NPV];
 #pragma unroll
-        for (int j = 0; j < DC; ++j) xs[j] = valid ? u[x0 + j] : 0.0;
+        for (int j = 0; j < DC; ++j) xs[j] = u[x0 + j];
 #pragma unroll
         for (int j = 0; j < NPV; ++j)
-            pvs[j] = (valid && VPar<VM>::np > 0) ? p[vd.param0 + static_cast<int64_t>(q) * vd.pdim + j] : 0.0;
+            pvs[j] = VPar<VM>::np > 0 ? p[vd.param0 + static_cast<int64_t>(qv) * vd.pdim + j] : 0.0;
         const double own = (DC == 1 || vd.mask[0] == 0) ? xs[0] : xs[DC - 1];
-        rhs_own[t] = own;
+        double* __restrict__ wval = rhs_val + wib * 4 * kWave;
+        uint8_t* __restrict__ wowner = rhs_owner + wib * 4 * kWave;
+        const double* __restrict__ wown = rhs_own + wib * kWave;
+        rhs_own[threadIdx.x] = own;
         double acc = 0.0;
-        const int32_t W = 4 * T;
+        constexpr int32_t W = 4 * kWave;
         for (int32_t base = S0; base < S1; base += W) {
             const int32_t wend = min(base + W, S1);
-            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) rhs_owner[s2 - base] = static_cast<uint8_t>(t);
-            __syncthreads();
+            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) wowner[s2 - base] = static_cast<uint8_t>(lane);
+            wave_sync();
             constexpr int NP = EModel<EM>::np;
             int2 sl[4];
             double nb[4];
             double kk[4][NP > 0 ? NP : 1];
+            // entries past the window's end load its last entry instead (a valid index), so
+            // the loads carry no branch and all go out before the first wait
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int32_t s2 = base + j * T + t;
-                if (s2 < wend) {
-                    if constexpr (KC) {
-                        const uint32_t w = static_cast<uint32_t>(vd.lnbr[s2]);
-                        sl[j] = make_int2(static_cast<int32_t>(w & 0x7fffffffu), static_cast<int32_t>(w >> 31));
-                    } else {
-                        sl[j] = vd.lslots[s2];
-                    }
+                const int32_t s2c = min(base + j * kWave + lane, wend - 1);
+                if constexpr (KC) {
+                    const uint32_t w = static_cast<uint32_t>(vd.lnbr[s2c]);
+                    sl[j] = make_int2(static_cast<int32_t>(w & 0x7fffffffu), static_cast<int32_t>(w >> 31));
+                } else {
+                    sl[j] = vd.lslots[s2c];
                 }
             }
             if constexpr (!MULTI) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const int32_t s2 = base + j * T + t;
-                    if (s2 < wend) {
-                        if constexpr (is_ode_model(EM)) nb[j] = u[sl[j].x];
-                        else if constexpr (EM != ND_EM_NOOP) nb[j] = vo[sl[j].x];
+                    const int32_t s2c = min(base + j * kWave + lane, wend - 1);
 #pragma unroll
-                        for (int c = 0; c < NP; ++c)
-                            kk[j][c] = KC ? vd.klow[static_cast<int64_t>(s2) * NP + c] : p[(sl[j].y >> 1) + c];
-                    }
+                    for (int c = 0; c < NP; ++c)
+                        kk[j][c] = KC ? vd.klow[static_cast<int64_t>(s2c) * NP + c] : p[(sl[j].y >> 1) + c];
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if constexpr (is_ode_model(EM)) nb[j] = u[sl[j].x];
+                    else if constexpr (EM != ND_EM_NOOP) nb[j] = vo[sl[j].x];
                 }
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int32_t s2 = base + j * T + t;
+                const int32_t s2 = base + j * kWave + lane;
                 if (s2 < wend) {
-                    const double ow = rhs_own[rhs_owner[s2 - base]];
+                    const double ow = wown[wowner[s2 - base]];
                     double v;
                     if constexpr (MULTI) v = slot_value_rt(ed[vd.lseb[s2]], sl[j], ow, u, vo, p);
                     else v = slot_eval<EM, WRAP>(sl[j], nb[j], kk[j], ow);
-                    rhs_val[s2 - base] = v;
+                    wval[s2 - base] = v;
                 }
             }
-            __syncthreads();
-            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) acc += rhs_val[s2 - base];
-            __syncthreads();
+            wave_sync();
+            for (int32_t s2 = max(sb, base); s2 < min(se, wend); ++s2) acc += wval[s2 - base];
+            wave_sync();
         }
         if (valid && !hub) vertex_emit<VM, STAGE>(du, sa, x0, xs, acc, pvs, dim, vd.mask[0], static_cast<int64_t>(vd.out0) + q);
     }
