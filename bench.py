@@ -4,10 +4,13 @@ Workload: second-order Kuramoto (swing) vertices on a Barabasi-Albert graph
 N = 1,000,000, m = 2 (E = 1,999,996), Kuramoto edges K ~ U(5,10); FP64.
 A step is one RHS evaluation with u, p, du resident in HBM.
 
-Prints one JSON line: RHS evals/s, the roofline of the fused vertex kernel
-(algorithmic bytes 56 N + 16 E per launch over its HIP-event-timed average
-duration, against 8 TB/s), and the CPU baseline (OpenMP restatement of the
-reference's ThreadedExecution + ThreadedAggregator path, timed on this host).
+Prints one JSON line: RHS evals/s, the roofline of one RHS (algorithmic bytes
+56 N + 16 E over the HIP-event-timed average duration of the RHS's launches,
+against 8 TB/s), the CPU baseline (OpenMP restatement of the reference's
+ThreadedExecution + ThreadedAggregator path, timed on this host), the achievable
+bandwidth of a STREAM-like device copy, and `extended.c4`: the ensemble's
+trajectories/s on the basin-stability workload (4096 uneven swing networks, Tsit5
+to t = 50) against the per-instance paths.
 """
 import argparse
 import json
@@ -20,6 +23,26 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "swing_ba_1e6_traffic.txt")
+
+
+def stream_copy(dev, mib, reps=20):
+    """Achievable bandwidth of a device copy of `mib` MiB (read + write counted), GB/s."""
+    import torch
+    n = mib * 2 ** 20 // 8
+    a = torch.ones(n, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e.record()
+    torch.cuda.synchronize()
+    t = s.elapsed_time(e) * 1e-3 / reps
+    del a, b
+    return 2 * n * 8 / t / 1e9
 
 
 def _traffic_from_profile():
@@ -55,7 +78,7 @@ def cpu_baseline(nw, p, u0, seconds=10.0, what="full N=1e6 swing network"):
         if time.perf_counter() - t0 > seconds and n >= 3:
             break
     el = time.perf_counter() - t0
-    return dict(value=n / el, unit="RHS evals/s", cores=cores, kind="port",
+    return dict(value=n / el, unit="RHS evals/s", cores=cores, kind="port", label="restatement",
                 best_ms=best * 1e3,
                 sample=f"{what}, {n} calls in {el:.1f} s, {cores} OpenMP threads "
                        "(C restatement of ThreadedExecution{true}+ThreadedAggregator(+))")
@@ -80,7 +103,7 @@ def cpu_baseline_c1(ND, seconds=10.0):
         m += 1
     t_rk4 = (time.perf_counter() - t1) / m
     return dict(rhs_us=t_rhs * 1e6, rhs_us_inside_rk4=t_rk4 / 4000 * 1e6, rk4_ms=t_rk4 * 1e3, cores=1,
-                kind="port",
+                kind="port", label="restatement",
                 sample=f"the C1 network: {n} RHS calls through the ctypes binding (its call overhead "
                        f"included), then {m} RK4 runs of 1000 steps in C, one thread (C restatement of "
                        "SequentialExecution)")
@@ -94,6 +117,9 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="leave out the extended.c4 ensemble measurement")
+    ap.add_argument("--c4-baseline-seconds", type=float, default=10.0,
+                    help="time budget of each per-instance C4 baseline sample")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "extras"],
                     help="c2 (default): the N=1e6 swing RHS line; c1/c3/c4/c5: the other workloads; "
                          "extras: timings of the paths around the RHS kernels")
@@ -155,14 +181,37 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ev_ms = start.elapsed_time(stop)
-    per_launch_s = ev_ms * 1e-3 / args.steps        # one kernel launch per RHS on this network
+    per_rhs_s = ev_ms * 1e-3 / args.steps           # all launches of one RHS, HIP events on its stream
     value = args.steps / wall
 
     algo_bytes = ND.workloads.algorithmic_bytes_swing(N, E)
-    achieved = algo_bytes / per_launch_s / 1e9
+    achieved = algo_bytes / per_rhs_s / 1e9
     traffic = _traffic_from_profile()
+    copy_hbm = stream_copy(dev, 2048)               # 4 GiB moved: beyond the 256 MB Infinity Cache
+    copy_small = stream_copy(dev, 64)               # 128 MiB moved: cache-resident
 
     cpu = None if args.no_cpu else cpu_baseline(nw, p, u0, args.cpu_seconds)
+    c4 = None
+    if not args.no_c4:
+        del u, pd, du
+        torch.cuda.empty_cache()
+        from importlib import import_module
+        ex = import_module(ND.__name__ + ".experiments")
+        r = ex.c4_ensemble(dev, baseline_seconds=args.c4_baseline_seconds,
+                           one_workgroup_seconds=args.c4_baseline_seconds / 2, steady_state_tol=0.0)
+        ow = r["one_workgroup_per_instance"] or {}
+        c4 = {"workload": r["workload"], "unit": "trajectories/s", "instances": r["instances"],
+              "ensemble_traj_per_s": r["ensemble_traj_per_s"],
+              "graph_replay_traj_per_s": r["graph_replay_traj_per_s"],
+              "graph_replay_equals_eager": r["graph_replay_equals_eager"],
+              "one_workgroup_per_instance_traj_per_s": ow.get("traj_per_s"),
+              "one_workgroup_sample": ow.get("instances"),
+              "host_stepped_per_instance_traj_per_s": r["baseline_traj_per_s"],
+              "host_stepped_sample": r["baseline_instances"],
+              "ratio_vs_one_workgroup": ow.get("ensemble_speedup"),
+              "ratio_vs_host_stepped": r["speedup"],
+              "steps_mean": r["ensemble_steps_mean"], "retcodes": r["ensemble_retcodes"],
+              "returned_to_sync": r["returned_to_sync"], "target_ratio": 20.0}
     out = {
         "metric": "RHS evals/s + % HBM peak at N=1e6 swing",
         "value": value,
@@ -180,8 +229,11 @@ def main():
                    "n": N, "e": E, "hub_degree": nw.tuning()["hub_degree"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": algo_bytes, "kernel_us": per_launch_s * 1e6},
+                     "traffic_source": "profiles/swing_ba_1e6_traffic.txt (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
+                     "algorithmic_bytes": algo_bytes, "kernel_us": per_rhs_s * 1e6,
+                     "stream_copy_GBs": {"4GiB_moved": copy_hbm, "128MiB_moved": copy_small}},
         "cpu_baseline": cpu,
+        "extended": {"c4": c4},
     }
     print(json.dumps(out))
 
