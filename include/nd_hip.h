@@ -317,6 +317,17 @@ int nd_rhs(nd_plan* plan, double* du, const double* u, const double* p, double t
  * documents the time axis. */
 int nd_rk4(nd_plan* plan, double* u, const double* p, double t0, double dt, int64_t nsteps, double* work,
            int64_t len_u, int64_t len_p, void* stream);
+/* nd_rk4 without allocating: the same fixed-step RK4 by the stage launches (never the
+ * one-workgroup path), with its three stage vectors and second compact-output buffer
+ * taken from a library-owned device array of 2^23 doubles (64 MiB) per GPU.  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The plan must exist already; a network needing more than the array (3 x dim plus the
+ * compact outputs) -> ND_ERR_ARGUMENT.  Calls on one GPU share the array: calls that
+ * overlap on a GPU (other streams, other threads) race.  Results equal nd_rk4's
+ * stage-launch path (rk4_path 2) bit for bit. */
+int nd_rk4_noalloc(nd_plan* plan, double* u, const double* p, double t0, double dt, int64_t nsteps, int64_t len_u,
+                   int64_t len_p, void* stream);
 
 /* get_buffers(nw, u, p, t; initbufs=true), coreloop.jl:103-109: fills the
  * output buffer o[lastidx_out] and the aggregation buffer aggbuf[lastidx_aggr]. */

@@ -98,6 +98,7 @@ PROTOTYPES = {
     "nd_rhs": (C.c_int, [P, P, P, P, D, I64, I64, I64, P]),
     "nd_get_buffers": (C.c_int, [P, P, P, P, P, D, P]),
     "nd_rk4": (C.c_int, [P, P, P, D, D, I64, P, I64, I64, P]),
+    "nd_rk4_noalloc": (C.c_int, [P, P, P, D, D, I64, I64, I64, P]),
     "nd_tsit5": (C.c_int, [P, P, P, D, D, D, D, D, I64, P, I64, I64, P]),
     "nd_rhs_jvp": (C.c_int, [P, P, P, P, P, P, D, I64, I64, I64, P]),
     "nd_plan_set_perturbation": (C.c_int, [P, I64, I64, P, I64, P, I64, P, I64, P]),

@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1935,6 +1936,34 @@ void launch_rhs(nd_plan* pl, bool stage, double* du, const double* u, const doub
 
 }  // namespace
 
+// Scratch of nd_rk4_noalloc, the allocation-free RK4: the three stage vectors and the
+// second compact-output buffer of one call, in one file-scope device array.  Every GPU the
+// library runs on holds its own copy until the process ends; calls on one GPU share it
+// (and race if they overlap).  Host code never passes the array's name to a kernel: its
+// device address comes from hipGetSymbolAddress, looked up once per device with that
+// device current, and kept in noalloc_ptr.
+constexpr int64_t kNoAllocScratch = int64_t(1) << 23;   // doubles (64 MiB)
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+static std::mutex noalloc_mu;
+static double* noalloc_ptr[kNoAllocDevices] = {};
+
+static int noalloc_scratch(int dev, double** out) {
+    if (dev < 0 || dev >= kNoAllocDevices)
+        return nd::set_error(ND_ERR_UNSUPPORTED, "nd_rk4_noalloc: device %d beyond the %d it keeps scratch for", dev,
+                             kNoAllocDevices);
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return nd::set_error(ND_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        noalloc_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = noalloc_ptr[dev];
+    return ND_OK;
+}
+
 extern "C" {
 
 int nd_plan_create(const nd_network* h, nd_plan** out) {
@@ -2436,6 +2465,43 @@ int nd_tsit5(nd_plan* pl, double* u, const double* p, double t0, double t1, doub
                              stats ? stats : pl->d_small_stat, stream);
 }
 
+// Classic RK4 as four fused stage launches per step.  work: 3*dim doubles (the stage
+// sum, two stage states); vout2: a second compact-output buffer of vout_len doubles when
+// the plan gathers from compact outputs (unused otherwise).  Allocates nothing.
+static int rk4_stage_launches(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_t nsteps,
+                              double* work, double* vout2, hipStream_t s) {
+    double* acc = work;
+    double* ya = work + pl->dim;
+    double* yb = work + 2 * pl->dim;
+    const double half = 0.5 * dt, sixth = dt / 6.0;
+    // Compact vertex outputs ping-pong between two buffers: every stage epilogue writes the
+    // outputs of the state it produces into the buffer the next stage reads, so only the
+    // first stage of the call builds them.  p is const for the whole call: its parameters
+    // are compared with the cached copy once.
+    if (pl->computed_out) {
+        // computed outputs (voltage sources, injectors) are rebuilt for every stage
+        for (int64_t k = 0; k < nsteps; ++k) {
+            const double tk = t0 + static_cast<double>(k) * dt;
+            launch_rhs(pl, true, nullptr, u, p, StageArgs{u, acc, ya, nullptr, half, 0.0, 1}, s, tk, k == 0);
+            launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, yb, nullptr, half, 2.0, 2}, s, tk + half, false);
+            launch_rhs(pl, true, nullptr, yb, p, StageArgs{u, acc, ya, nullptr, dt, 2.0, 2}, s, tk + half, false);
+            launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, u, nullptr, sixth, 0.0, 3}, s, tk + dt, false);
+        }
+        return hip_check(hipGetLastError(), "nd_rk4 launch");
+    }
+    double* va = pl->use_vout ? pl->d_vout : nullptr;
+    double* vb = pl->use_vout ? vout2 : nullptr;
+    for (int64_t k = 0; k < nsteps; ++k) {
+        // stage times t_k, t_k + dt/2, t_k + dt/2, t_k + dt (time-dependent models)
+        const double tk = t0 + static_cast<double>(k) * dt;
+        launch_rhs(pl, true, nullptr, u, p, StageArgs{u, acc, ya, vb, half, 0.0, 1}, s, tk, k == 0, va, k == 0);
+        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, yb, va, half, 2.0, 2}, s, tk + half, false, vb, false);
+        launch_rhs(pl, true, nullptr, yb, p, StageArgs{u, acc, ya, vb, dt, 2.0, 2}, s, tk + half, false, va, false);
+        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, u, va, sixth, 0.0, 3}, s, tk + dt, false, vb, false);
+    }
+    return hip_check(hipGetLastError(), "nd_rk4 launch");
+}
+
 int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_t nsteps, double* work,
            int64_t len_u, int64_t len_p, void* stream) {
     if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
@@ -2476,47 +2542,40 @@ int nd_rk4(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_
         }
         work = pl->rk_work;
     }
-    double* acc = work;
-    double* ya = work + pl->dim;
-    double* yb = work + 2 * pl->dim;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const double half = 0.5 * dt, sixth = dt / 6.0;
-    // Compact vertex outputs ping-pong between two buffers: every stage epilogue writes the
-    // outputs of the state it produces into the buffer the next stage reads, so only the
-    // first stage of the call builds them.  p is const for the whole call: its parameters
-    // are compared with the cached copy once.
-    double* va = pl->use_vout && !pl->computed_out ? pl->d_vout : nullptr;
-    double* vb = nullptr;
-    if (pl->computed_out) {
-        // computed outputs (voltage sources, injectors) are rebuilt for every stage
-        for (int64_t k = 0; k < nsteps; ++k) {
-            const double tk = t0 + static_cast<double>(k) * dt;
-            launch_rhs(pl, true, nullptr, u, p, StageArgs{u, acc, ya, nullptr, half, 0.0, 1}, s, tk, k == 0);
-            launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, yb, nullptr, half, 2.0, 2}, s, tk + half, false);
-            launch_rhs(pl, true, nullptr, yb, p, StageArgs{u, acc, ya, nullptr, dt, 2.0, 2}, s, tk + half, false);
-            launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, u, nullptr, sixth, 0.0, 3}, s, tk + dt, false);
-        }
-        return hip_check(hipGetLastError(), "nd_rk4 launch");
+    if (pl->use_vout && !pl->computed_out && !pl->d_vout2) {
+        void* w = nullptr;
+        hipError_t e = hipMalloc(&w, pl->vout_len * sizeof(double));
+        if (e != hipSuccess) return nd::set_error(ND_ERR_ALLOC, "hipMalloc(vertex outputs): %s", hipGetErrorString(e));
+        pl->allocations.push_back(w);
+        pl->d_vout2 = static_cast<double*>(w);
     }
-    if (va) {
-        if (!pl->d_vout2) {
-            void* w = nullptr;
-            hipError_t e = hipMalloc(&w, pl->vout_len * sizeof(double));
-            if (e != hipSuccess) return nd::set_error(ND_ERR_ALLOC, "hipMalloc(vertex outputs): %s", hipGetErrorString(e));
-            pl->allocations.push_back(w);
-            pl->d_vout2 = static_cast<double*>(w);
-        }
-        vb = pl->d_vout2;
-    }
-    for (int64_t k = 0; k < nsteps; ++k) {
-        // stage times t_k, t_k + dt/2, t_k + dt/2, t_k + dt (time-dependent models)
-        const double tk = t0 + static_cast<double>(k) * dt;
-        launch_rhs(pl, true, nullptr, u, p, StageArgs{u, acc, ya, vb, half, 0.0, 1}, s, tk, k == 0, va, k == 0);
-        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, yb, va, half, 2.0, 2}, s, tk + half, false, vb, false);
-        launch_rhs(pl, true, nullptr, yb, p, StageArgs{u, acc, ya, vb, dt, 2.0, 2}, s, tk + half, false, va, false);
-        launch_rhs(pl, true, nullptr, ya, p, StageArgs{u, acc, u, va, sixth, 0.0, 3}, s, tk + dt, false, vb, false);
-    }
-    return hip_check(hipGetLastError(), "nd_rk4 launch");
+    return rk4_stage_launches(pl, u, p, t0, dt, nsteps, work, pl->d_vout2, static_cast<hipStream_t>(stream));
+}
+
+int nd_rk4_noalloc(nd_plan* pl, double* u, const double* p, double t0, double dt, int64_t nsteps, int64_t len_u,
+                   int64_t len_p, void* stream) {
+    if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
+    if (len_u != pl->dim) return nd::set_error(ND_ERR_ARGUMENT, "u does not have expected size %lld", (long long)pl->dim);
+    if (pl->pdim > 0 && len_p != pl->pdim)
+        return nd::set_error(ND_ERR_ARGUMENT, "p does not has expecte size %lld", (long long)pl->pdim);
+    if (nsteps < 0 || !(dt == dt)) return nd::set_error(ND_ERR_ARGUMENT, "bad step count or dt");
+    if (pl->dim > 0 && !u) return nd::set_error(ND_ERR_ARGUMENT, "NULL state pointer");
+    if (pl->pdim > 0 && !p) return nd::set_error(ND_ERR_ARGUMENT, "NULL parameter pointer");
+    const bool two_outputs = pl->use_vout && !pl->computed_out;
+    const int64_t work = 3 * std::max<int64_t>(pl->dim, 1);
+    const int64_t need = work + (two_outputs ? static_cast<int64_t>(pl->vout_len) : 0);
+    if (need > kNoAllocScratch)
+        return nd::set_error(ND_ERR_ARGUMENT,
+                             "nd_rk4_noalloc: the network needs %lld doubles of scratch, above its maximum of %lld "
+                             "(3 x the state dimension plus the compact outputs); use nd_rk4",
+                             (long long)need, (long long)kNoAllocScratch);
+    int rc = check_device(pl);
+    if (rc) return rc;
+    double* sp = nullptr;
+    rc = noalloc_scratch(pl->device, &sp);
+    if (rc) return rc;
+    return rk4_stage_launches(pl, u, p, t0, dt, nsteps, sp, two_outputs ? sp + work : nullptr,
+                              static_cast<hipStream_t>(stream));
 }
 
 int nd_collect_externals(nd_plan* pl, double* extbuf, const double* u, const double* o, void* stream) {
