@@ -54,18 +54,22 @@ struct NewtonArgs {
     int32_t first;
 };
 
+// max that propagates NaN (fmax drops it): a residual with a NaN entry must read as NaN,
+// never as converged
+__device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
+
 __device__ __forceinline__ double block_reduce(double v, double* red, bool is_max) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
         const double o = __shfl_xor(v, off, kWave);
-        v = is_max ? fmax(v, o) : v + o;
+        v = is_max ? max_nan(v, o) : v + o;
     }
     const int t = static_cast<int>(threadIdx.x), nw = static_cast<int>(blockDim.x) / kWave;
     if ((t & (kWave - 1)) == 0) red[t / kWave] = v;
     __syncthreads();
     if (t == 0) {
         double s = red[0];
-        for (int w = 1; w < nw; ++w) s = is_max ? fmax(s, red[w]) : s + red[w];
+        for (int w = 1; w < nw; ++w) s = is_max ? max_nan(s, red[w]) : s + red[w];
         red[16] = s;
     }
     __syncthreads();
@@ -115,7 +119,7 @@ __global__ void __launch_bounds__(kThreads) k_newton(NewtonArgs a) {
                 else
                     for (int s = rp[v]; s < rp[v + 1]; ++s) acc += a.K[se[s]] * nd::nd_sin(th[nb[s]] - own);
                 out[k] = Pg[v] + acc;
-                m = fmax(m, fabs(out[k] / scg[v]));
+                m = max_nan(m, fabs(out[k] / scg[v]));
             }
         }
         return block_reduce(m, red, true);

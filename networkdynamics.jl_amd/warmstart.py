@@ -290,14 +290,15 @@ class GridBatch:
         return s0, X
 
 
-def warm_start_device(gb, net=None, tol=1e-6):
+def warm_start_device(gb, net=None, tol=1e-6, dc_tol=None):
     """Warm start of every grid in a GridBatch, entirely on the GPU: DC solve, physics
     source, network correction (net None: physics only), second linear solve from the DC
-    solution.  Both linear solves stop at residual tol: they only provide a starting
-    point, Newton reaches the 1e-10 residual from it.  Returns (theta_w, dict of seconds)."""
+    solution.  The linear solves stop at residual dc_tol (default tol) and tol: they only
+    provide a starting point, Newton reaches the 1e-10 residual from it.  Returns
+    (theta_w, dict of seconds)."""
     import torch
     times = {}
-    th_dc, _, times["dc_solve"] = gb.solve(linearized=True, tol=tol)
+    th_dc, _, times["dc_solve"] = gb.solve(linearized=True, tol=tol if dc_tol is None else dc_tol)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     th_dc = gb.center(th_dc)
@@ -313,3 +314,60 @@ def warm_start_device(gb, net=None, tol=1e-6):
     times["features_and_network"] = time.perf_counter() - t0
     th_w, _, times["warm_solve"] = gb.solve(theta0=th_dc, P=src, linearized=True, tol=tol)
     return gb.center(th_w), times
+
+
+def solve_with_fallback(gb, theta0, tol=1e-10, first_max_newton=None, times=None, **kw):
+    """Newton on every grid of a GridBatch from the starts theta0 (a device vector, e.g.
+    the learned warm starts), then Newton from the flat start theta = 0 -- the method
+    without the network -- on every grid whose first solve did not end with a residual
+    <= tol (no convergence, a stalled line search, a NaN).  Grids the flat start does not
+    bring to tol either raise NetworkInitError, as find_fixpoint does on a failed solve
+    (src/initialization.jl:127-141).
+
+    first_max_newton caps the Newton steps of the first solve (a good start converges in a
+    few quadratic steps; a bad one then falls back early instead of holding the batch for
+    the full budget); the flat solve keeps max_newton.  The second solve runs the whole
+    batch with theta = 0 on the failed grids only: the other grids start at their
+    solutions, already at tol, and return unchanged without a Newton step.  Returns
+    (theta, stats per grid -- the accepted solve's --, fell_back per grid); times, a dict,
+    receives the seconds of both solves."""
+    import torch
+    from .initialization import NetworkInitError
+    first = dict(kw)
+    if first_max_newton is not None:
+        first["max_newton"] = int(first_max_newton)
+    th, st, t1 = gb.solve(theta0=theta0, tol=tol, **first)
+
+    def bad(s):
+        return (s["retcode"] != 0) | ~(s["residual"] <= tol)
+
+    failed = bad(st)
+    t2 = 0.0
+    if failed.any():
+        mask = torch.from_numpy(np.repeat(failed, np.diff(gb.voff))).to(gb.dev)
+        th2, st2, t2 = gb.solve(theta0=torch.where(mask, torch.zeros_like(th), th), tol=tol, **kw)
+        still = failed & bad(st2)
+        if still.any():
+            idx = np.flatnonzero(still)
+            raise NetworkInitError(
+                f"Initialization failed: {len(idx)} grid(s) reach no fixpoint with residual <= {tol:g} from the "
+                f"learned start or from the flat start (grid indices {idx[:10].tolist()}"
+                f"{' ...' if len(idx) > 10 else ''}; flat-start retcodes {st2['retcode'][idx[:10]].tolist()})")
+        th = torch.where(mask, th2, th)
+        st = st.copy()
+        st[failed] = st2[failed]
+    if times is not None:
+        times.update(newton=t1, fallback_newton=t2)
+    return th, st, failed
+
+
+def learned_solve(gb, net, tol=1e-10, warm_tol=1e-2, dc_tol=1e-2, first_max_newton=8, **kw):
+    """Fixpoints of every grid of a GridBatch from the learned warm start, with the flat
+    start as the fallback (solve_with_fallback).  net None: the physics-only start.
+    The two linear solves of the warm start stop at residual 1e-2 by default: the
+    features need a rough DC solution only, and Newton takes the start to tol anyway
+    (tighter pre-solves cost more than the Newton steps they save).
+    Returns (theta, stats, fell_back, dict of seconds)."""
+    theta_w, times = warm_start_device(gb, net, tol=warm_tol, dc_tol=dc_tol)
+    th, st, fell = solve_with_fallback(gb, theta_w, tol=tol, first_max_newton=first_max_newton, times=times, **kw)
+    return th, st, fell, times

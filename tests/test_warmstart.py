@@ -168,3 +168,40 @@ def test_device_features_and_warm_start_match_host(ND, dev):
     th_w, _ = W.warm_start_device(gb, None, tol=1e-10)
     th_w_host, _ = W.warm_start(grids, s0_host, None, dev)
     np.testing.assert_allclose(th_w.cpu().numpy(), np.concatenate(th_w_host), rtol=0, atol=1e-9)
+
+
+@pytest.mark.gpu
+def test_learned_start_falls_back_to_the_flat_start(ND, dev):
+    """solve_with_fallback: grids whose start fails (here a NaN prediction, as a diverged
+    network would give) are solved again from the flat start and then equal the flat
+    solve bit for bit; the others keep their warm-started solution; every result passes
+    the network-RHS judge; a grid without a fixpoint (injections far beyond its lines'
+    capacity) raises NetworkInitError."""
+    import copy
+    import torch
+    from networkdynamics_jl_amd import warmstart as W
+    from networkdynamics_jl_amd.fixpoint import isfixpoint_residual, swing_load_grid
+    grids = [swing_load_grid(int(n), seed=900 + i) for i, n in enumerate([300, 450, 600, 350, 500, 400])]
+    gb = W.GridBatch(grids, dev)
+    start, _ = W.warm_start_device(gb, None)
+    bad = start.clone()
+    broken = np.array([False, True, False, False, True, False])
+    for i in np.flatnonzero(broken):
+        bad[int(gb.voff[i]):int(gb.voff[i + 1])] = float("nan")
+    th, st, fell = W.solve_with_fallback(gb, bad)
+    assert np.array_equal(fell, broken)
+    assert (st["retcode"] == 0).all() and (st["residual"] <= 1e-10).all()
+    flat, fst, _ = gb.solve()
+    warm, wst, _ = gb.solve(theta0=start)
+    thh, flh, wmh = th.cpu().numpy(), flat.cpu().numpy(), warm.cpu().numpy()
+    for i, gr in enumerate(grids):
+        sl = slice(int(gb.voff[i]), int(gb.voff[i + 1]))
+        assert np.array_equal(thh[sl], flh[sl] if broken[i] else wmh[sl]), i
+        assert st["newton_iters"][i] == (fst if broken[i] else wst)["newton_iters"][i]
+        assert isfixpoint_residual(gr, thh[sl], dev) <= 1e-10
+    # an overloaded grid: the flat start fails too
+    over = copy.copy(grids[2])
+    over.P = grids[2].P * 200.0
+    gb2 = W.GridBatch([grids[0], over], dev)
+    with pytest.raises(ND.NetworkInitError, match="grid indices \\[1\\]"):
+        W.solve_with_fallback(gb2, torch.zeros(gb2.n, dtype=torch.float64, device=dev) + float("nan"))
