@@ -22,7 +22,6 @@
 #include <vector>
 
 #include "nd_internal.h"
-#include "nd_math.h"
 
 namespace {
 
@@ -104,7 +103,9 @@ __global__ void __launch_bounds__(kThreads) k_newton(NewtonArgs a) {
     }
     __syncthreads();
 
-    // F(theta) for the thread's vertices from th[]; returns max |F/scale| over the grid
+    // F(theta) for the thread's vertices from th[]; returns max |F/scale| over the grid.
+    // The library sin/cos stay here: with V residual slots unrolled, the inlined
+    // reduction of nd_math.h and its out-of-line call made the V = 10 instance spill.
     auto residual = [&](double (&out)[V]) {
         double m = 0.0;
 #pragma unroll
@@ -117,7 +118,7 @@ __global__ void __launch_bounds__(kThreads) k_newton(NewtonArgs a) {
                 if (a.linearized)
                     for (int s = rp[v]; s < rp[v + 1]; ++s) acc += a.K[se[s]] * (th[nb[s]] - own);
                 else
-                    for (int s = rp[v]; s < rp[v + 1]; ++s) acc += a.K[se[s]] * nd::nd_sin(th[nb[s]] - own);
+                    for (int s = rp[v]; s < rp[v + 1]; ++s) acc += a.K[se[s]] * sin(th[nb[s]] - own);
                 out[k] = Pg[v] + acc;
                 m = max_nan(m, fabs(out[k] / scg[v]));
             }
@@ -139,7 +140,7 @@ __global__ void __launch_bounds__(kThreads) k_newton(NewtonArgs a) {
                 const double own = th[v];
                 double dsum = 0.0;
                 for (int s = rp[v]; s < rp[v + 1]; ++s) {
-                    const double ws = a.linearized ? a.K[se[s]] : a.K[se[s]] * nd::nd_cos(th[nb[s]] - own);
+                    const double ws = a.linearized ? a.K[se[s]] : a.K[se[s]] * cos(th[nb[s]] - own);
                     w[s] = ws;
                     dsum += ws;
                 }
