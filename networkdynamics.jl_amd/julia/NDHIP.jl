@@ -20,6 +20,7 @@ using NetworkDynamics: ExecutionStyle, Aggregator, IndexManager, ComponentBatch,
                        dim, pdim, outdim, extdim, PureFeedForward, PureStateMap, StateMask, AntiSymmetric,
                        Symmetric, Directed, Fiducial, ExtMap, StateBufIdx, has_external_input
 using AMDGPU: ROCVector, AMDGPU
+using ForwardDiff: Dual
 
 export HIPExecution, SegmentedAggregator, register_model!, rk4!, tsit5!
 
@@ -93,6 +94,11 @@ mutable struct SegmentedAggregator{F} <: Aggregator
     plan::Ptr{Cvoid}
 end
 SegmentedAggregator(f) = (im, batches) -> SegmentedAggregator(im, batches, f)
+
+# the copy constructor Network(nw; ...) / copy(nw) (src/construction.jl:475-536) rebuilds
+# the aggregator through this (src/aggregators.jl:287-292): each copy gets its own device
+# plan and buffers, so copies can run concurrently (one per thread or stream)
+NetworkDynamics.get_aggr_constructor(a::SegmentedAggregator) = SegmentedAggregator(a.f)
 
 function SegmentedAggregator(im::IndexManager, batches, f)
     f === (+) || throw(ArgumentError("SegmentedAggregator only supports +"))
@@ -182,6 +188,40 @@ function (nw::Network{<:HIPExecution})(du::ROCVector{Float64}, u::ROCVector{Floa
                  AMDGPU.stream().stream))
     AMDGPU.synchronize()   # the reference call returns after its work is done (coreloop.jl:100)
     nothing
+end
+
+# nw(du, u, p, t) on first-order dual numbers (one partial, Float64), as ForwardDiff calls
+# it through the DiffCache buffers (src/network_structure.jl:152-155) for Jacobian-vector
+# products (linearize_network, implicit solvers): du = f(u) and its partial = J(u) v.
+# The Duals are split into value and partial vectors for nd_rhs_jvp and joined back.
+function (nw::Network{<:HIPExecution})(du::ROCVector{Dual{T,Float64,1}}, u::ROCVector{Dual{T,Float64,1}},
+                                       p::ROCVector{Float64}, t) where {T}
+    a = nw.layer.aggregator::SegmentedAggregator
+    n = length(u)
+    length(du) == n || throw(ArgumentError("du or u does not have expected size $(n)"))
+    uf = reinterpret(Float64, u)           # (value, partial) pairs, contiguous
+    uval, upart = uf[1:2:end], uf[2:2:end]
+    dval, dpart = similar(uval), similar(upart)
+    _check(ccall((:nd_rhs_jvp, libnd), Cint,
+                 (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Int64, Int64,
+                  Int64, Ptr{Cvoid}),
+                 a.plan, pointer(dval), pointer(dpart), pointer(uval), pointer(upart), pointer(p), Float64(t), n, n,
+                 length(p), AMDGPU.stream().stream))
+    df = reinterpret(Float64, du)
+    df[1:2:end] .= dval
+    df[2:2:end] .= dpart
+    AMDGPU.synchronize()
+    nothing
+end
+
+# Everything else the HIP path cannot run: host arrays, other element types, chunked
+# Duals.  The reference's own argument checks throw ArgumentError (src/coreloop.jl:2-7,
+# src/network_structure.jl:146-151); so does this, before any device work, instead of
+# falling through to the generic coreloop.
+function (nw::Network{<:HIPExecution})(du, u, p, t; kwargs...)
+    throw(ArgumentError("HIPExecution runs nw(du, u, p, t) with du, u, p as ROCVector{Float64} on the HIP " *
+                        "device (du and u may be ROCVectors of ForwardDiff.Dual{T,Float64,1} for Jacobian-vector " *
+                        "products); got $(typeof(du)), $(typeof(u)), $(typeof(p))"))
 end
 
 # Solves of ODEProblem(nw, u0, tspan, p) with RK4 / Tsit5 (src/post_utils.jl:43-82) in the

@@ -179,6 +179,21 @@ def test_size_and_type_errors(ND, dev):
         nw(du, u, pd[:-1], 0.0)
     with pytest.raises(ND.ArgumentError):                      # network_structure.jl:146-151
         nw(du.float(), u.float(), pd.float(), 0.0)
+    # host arrays: the HIP path takes device vectors only (no CPU fallback)
+    with pytest.raises(ND.ArgumentError, match="HIP device"):
+        nw(du.cpu(), u.cpu(), pd.cpu(), 0.0)
+    with pytest.raises(ND.ArgumentError, match="torch tensor"):
+        nw(np.empty(nw.dim), u0, p, 0.0)
+    # the same checks in the C ABI itself, for a binding that passes raw pointers
+    import ctypes as C
+    L = ND._abi.lib()
+    args = (C.c_void_p(du.data_ptr()), C.c_void_p(u.data_ptr()), C.c_void_p(pd.data_ptr()), 0.0)
+    assert L.nd_rhs(nw.plan, *args, nw.dim - 1, nw.dim, nw.pdim, None) == ND._abi.ND_ERR_ARGUMENT
+    assert "du" in L.nd_last_error().decode()
+    assert L.nd_rhs(nw.plan, *args, nw.dim, nw.dim, nw.pdim + 1, None) == ND._abi.ND_ERR_ARGUMENT
+    assert L.nd_rhs(nw.plan, None, args[1], args[2], 0.0, nw.dim, nw.dim, nw.pdim, None) == ND._abi.ND_ERR_ARGUMENT
+    assert L.nd_rhs_jvp(nw.plan, args[0], None, args[1], None, args[2], 0.0, nw.dim, nw.dim, nw.pdim,
+                        None) == ND._abi.ND_ERR_ARGUMENT
 
 
 def test_get_buffers_and_primitives(ND, dev):
@@ -495,6 +510,43 @@ def test_rk4_single_workgroup_path(ND, dev, em, n):
     four = _dev_rk4(nw, u0, p, dev, 0.002, 250)
     np.testing.assert_allclose(one, ref, rtol=0, atol=1e-10)
     np.testing.assert_allclose(four, ref, rtol=0, atol=1e-10)
+
+
+@pytest.mark.gpu
+def test_device_sine_against_numpy(ND, dev):
+    # The models' sine (a Cody-Waite reduction with Taylor kernels below |x| = 2^20 pi/2,
+    # the library sine beyond): disjoint pairs of first-order Kuramoto vertices with
+    # omega = 0 and K = 1 put exactly sin(theta_s - theta_d) into the dst side's du and
+    # its negation into the src side's.  Against numpy's sine: within 2 ulp everywhere,
+    # within 1 ulp for |x| < 40, and NaN for non-finite arguments.
+    import torch
+    rng = np.random.default_rng(81)
+    near = np.arange(-3000, 3001) * (np.pi / 2)
+    xs = np.concatenate([rng.uniform(-np.pi, np.pi, 20000), rng.uniform(-40, 40, 20000),
+                         rng.uniform(-1e5, 1e5, 10000), rng.uniform(1.5e6, 1.8e6, 10000),
+                         10.0 ** rng.uniform(7, 300, 4000), np.nextafter(near, np.inf), np.nextafter(near, -np.inf),
+                         [0.0, 1e-300, 5e-324, 1647099.3291652855, np.nextafter(1647099.3291652855, 0), -1e308]])
+    n = len(xs)
+    g = ND.SimpleGraph(2 * n, np.arange(1, 2 * n, 2), np.arange(2, 2 * n + 1, 2))
+    nw = ND.Network(g, ND.Lib.kuramoto_first(), ND.Lib.kuramoto_edge())
+    u = np.zeros(2 * n)
+    u[0::2] = xs
+    p = np.r_[np.zeros(2 * n), np.ones(n)]
+    du = torch.empty(2 * n, dtype=torch.float64, device=dev)
+    nw(du, torch.from_numpy(u).to(dev), torch.from_numpy(p).to(dev), 0.0)
+    got = du.cpu().numpy()
+    ref = np.sin(xs)
+    np.testing.assert_array_equal(got[0::2], -got[1::2])
+    ulp = np.spacing(np.maximum(np.abs(ref), np.finfo(float).tiny))
+    err = np.abs(got[1::2] - ref) / ulp
+    assert err.max() <= 2.0, (err.max(), xs[np.argmax(err)])
+    small = np.abs(xs) < 40
+    assert err[small].max() <= 1.0
+    inf = torch.tensor([np.inf, 0.0, -np.inf, 0.0, np.nan, 0.0], dtype=torch.float64, device=dev)
+    nw3 = ND.Network(ND.SimpleGraph(6, [1, 3, 5], [2, 4, 6]), ND.Lib.kuramoto_first(), ND.Lib.kuramoto_edge())
+    d3 = torch.empty(6, dtype=torch.float64, device=dev)
+    nw3(d3, inf, torch.from_numpy(np.r_[np.zeros(6), np.ones(3)]).to(dev), 0.0)
+    assert torch.isnan(d3).all()
 
 
 @pytest.mark.gpu
