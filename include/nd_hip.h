@@ -368,6 +368,7 @@ int nd_rhs_perturbed_jvp(nd_plan* plan, double* du, double* jv, const double* u,
 int nd_rhs_jvp(nd_plan* plan, double* du, double* jv, const double* u, const double* v, const double* p, double t,
                int64_t len_du, int64_t len_u, int64_t len_p, void* stream);
 
+
 /* collect_externals!(map, extbuf, u, o), external_inputs.jl:49-77:
  * extbuf[k] = u[i] for StateBufIdx(i), o[i] for OutBufIdx(i).  extbuf: lastidx_extbuf
  * doubles on the device; o: the output buffer (nd_get_buffers). */
@@ -396,19 +397,35 @@ int nd_aggregate(nd_plan* plan, double* aggbuf, const double* o, void* stream);
  * 128x2, 256x2, 512x2, 512x3, 512x4, 512x4). */
 #define ND_ENSEMBLE_CLASSES 6
 enum { ND_METHOD_TSIT5 = 0, ND_METHOD_RK4 = 1 };
+/* When a line-limit condition is checked (nd_ensemble_opts.line_events) */
+enum { ND_LINES_DEFAULT = 0, ND_LINES_STEP_END = 1, ND_LINES_CROSSING = 2 };
 
 typedef struct {
     int32_t method;       /* ND_METHOD_TSIT5 (adaptive) or ND_METHOD_RK4 (fixed dt0)          */
-    int32_t pad_;
+    int32_t line_events;  /* with line_limit: ND_LINES_CROSSING locates each crossing of
+                             |edge output| - limit inside the step, as the reference's
+                             ContinuousComponentCallback does (src/callbacks.jl:113-140,
+                             batched into a VectorContinuousCallback, callbacks.jl:450-476):
+                             after an accepted Tsit5 step the condition is compared at the
+                             nine dense-output points t + h*m/9 (m = 1..9) with its sign at
+                             the step start; an up-crossing (< 0 to >= 0) or down-crossing
+                             (> 0 to <= 0) at the first such point is bisected on the dense
+                             output to adjacent doubles, the solution moves to the left end
+                             of the bracket, the crossing edges get K = 0 and the step
+                             restarts there with the step size the controller proposed.
+                             ND_LINES_STEP_END: |edge output| > limit is checked at the end
+                             of each accepted step.  ND_LINES_DEFAULT: crossing for Tsit5,
+                             step end for RK4 (which has no dense output here).            */
     double reltol, abstol;
     double dt0;           /* RK4: the step; Tsit5: initial step, 0 = automatic                  */
     double dtmax, dtmin;  /* 0 = t1-t0 / 1e-14*max(1,|t1|)                                       */
     int64_t maxiters;     /* steps per instance, 0 = 1e6                                         */
     const double* line_limit; /* per edge, concatenated in instance order, or NULL.  Line
                              tripping (the cascading-failure callback of
-                             docs/examples/cascading_failure.jl, checked at the end of each
-                             accepted step): when |edge output| > limit the edge's first
-                             parameter (K) is set to 0, in p as well                        */
+                             docs/examples/cascading_failure.jl:104-111, condition
+                             |P| - limit, affect K = 0 on either crossing direction; when
+                             it is checked: line_events): a tripped edge's first parameter
+                             (K) is set to 0, in p as well                                  */
     double ss_abstol, ss_reltol; /* steady-state termination, both 0 = off: after an accepted
                              step, an instance whose every |du_i| <= ss_abstol or
                              <= ss_reltol*|u_i| stops there with retcode 4 (the test of the
@@ -463,6 +480,12 @@ int nd_tsit5(nd_plan* plan, double* u, const double* p, double t0, double t1, do
  * error weights b - bhat, c[7] the nodes (stage s+1 is evaluated at t + c[s]*h).  Host
  * only; no device work. */
 int nd_tsit5_tableau(double* a, double* btilde, double* c);
+/* The dense output the ensemble kernel locates line crossings on (Tsitouras' continuous
+ * extension of the same pair, the interpolant OrdinaryDiffEqTsit5 gives Tsit5
+ * solutions and callbacks): u(t + s*h) = u + h * sum_i b_i(s) k_i with
+ * b_i(s) = sum_{j=1..4} r[4*i + j-1] s^j, i = 0..6 over the seven stage derivatives.
+ * Host only; no device work. */
+int nd_tsit5_dense(double* r);
 /* The same solve captured once into a hipGraph (class launches on parallel branches)
  * and replayed with nd_ensemble_graph_launch on any stream; u, p, stats, tspan and
  * opts are fixed at capture.  A graph refuses to launch after nd_ensemble_set_class

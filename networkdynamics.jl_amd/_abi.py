@@ -115,6 +115,7 @@ PROTOTYPES = {
     "nd_ensemble_set_class": (C.c_int, [P, I32, I32, I32]),
     "nd_ensemble_solve": (C.c_int, [P, P, P, D, D, P, P, P]),
     "nd_tsit5_tableau": (C.c_int, [P, P, P]),
+    "nd_tsit5_dense": (C.c_int, [P]),
     "nd_ensemble_graph_create": (C.c_int, [P, P, P, D, D, P, P, P]),
     "nd_ensemble_graph_launch": (C.c_int, [P, P]),
     "nd_ensemble_graph_destroy": (None, [P]),

@@ -201,6 +201,57 @@ __device__ __forceinline__ bool at_steady_state(const SolveArgs& a, const InstDe
     return block_sum(fail, L.red) == 0.0;
 }
 
+// Line crossings (ND_LINES_CROSSING).  sign8 codes the sign of a condition value (2 for
+// NaN, which never crosses); crossed(p, n) is the event test of DiffEqBase's
+// VectorContinuousCallback when both directions have an affect: the sign p at the step
+// start is nonzero and p*n <= 0.
+__device__ __forceinline__ int8_t sign8(double g) {
+    return static_cast<int8_t>(g > 0.0 ? 1 : g < 0.0 ? -1 : g == 0.0 ? 0 : 2);
+}
+__device__ __forceinline__ bool crossed(int8_t p, int8_t n) {
+    return (p == -1 && (n == 1 || n == 0)) || (p == 1 && (n == -1 || n == 0));
+}
+
+// the condition |P_line| - limit of edge e (docs/examples/cascading_failure.jl:104-106)
+// at the phases th
+template <int EM>
+__device__ __forceinline__ double line_cond(const Lds& L, const double* th, const double* lim, int e) {
+    constexpr int NPE = EModel<EM>::np > 0 ? EModel<EM>::np : 1;
+    return fabs(g_dst<EM>(th[L.ends[2 * e]], th[L.ends[2 * e + 1]], L.epar + e * NPE)) - lim[e];
+}
+
+// sg[e] = sign8 of every edge's condition at the state y (two block barriers)
+template <int VM, int EM, int V>
+__device__ __forceinline__ void line_signs(const InstDesc& I, const Lds& L, const double* lim,
+                                           const double (&y)[V][VModel<VM>::dim], int8_t* sg) {
+    const int T = static_cast<int>(blockDim.x), t = static_cast<int>(threadIdx.x);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const int v = t + k * T;
+        if (v < I.n) L.theta[v] = y[k][0];
+    }
+    __syncthreads();
+    for (int e = t; e < I.ne; e += T) sg[e] = sign8(line_cond<EM>(L, L.theta, lim, e));
+    __syncthreads();
+}
+
+__device__ __forceinline__ double block_min(double v, double* red) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, kWave));
+    const int t = static_cast<int>(threadIdx.x), nw = static_cast<int>(blockDim.x) / kWave;
+    if ((t & (kWave - 1)) == 0) red[t / kWave] = v;
+    __syncthreads();
+    if (t == 0) {
+        double s = red[0];
+        for (int w = 1; w < nw; ++w) s = fmin(s, red[w]);
+        red[16] = s;
+    }
+    __syncthreads();
+    const double r = red[16];
+    __syncthreads();
+    return r;
+}
+
 // Tsitouras (2011) 5(4) coefficients: stage matrix a (row s = stage s+1), error weights
 // btilde = b - bhat (b = the last row, first same as last) and nodes c = a * 1.  The
 // kernel reads these names and nd_tsit5_tableau exports the same table, so the order
@@ -231,6 +282,29 @@ constexpr double c_a71 = kTsit5.a[6][0], c_a72 = kTsit5.a[6][1], c_a73 = kTsit5.
                  c_a75 = kTsit5.a[6][4], c_a76 = kTsit5.a[6][5];
 constexpr double c_bt1 = kTsit5.bt[0], c_bt2 = kTsit5.bt[1], c_bt3 = kTsit5.bt[2], c_bt4 = kTsit5.bt[3],
                  c_bt5 = kTsit5.bt[4], c_bt6 = kTsit5.bt[5], c_bt7 = kTsit5.bt[6];
+
+// Tsitouras' continuous extension of the pair, the dense output OrdinaryDiffEqTsit5
+// interpolates solutions and locates callback events on: u(t + s h) = u + h sum_i b_i(s) k_i,
+// b_i(s) = sum_j r[i][j-1] s^j (only stage 1 has a linear term; b_i(1) is the last row of
+// a).  nd_tsit5_dense exports this table, so its order conditions are checked on the
+// kernel's copy.
+struct Tsit5Dense {
+    double r[7][4];
+};
+constexpr Tsit5Dense kTsit5Dense = {{{1.0, -2.763706197274826, 2.9132554618219126, -1.0530884977290216},
+                                     {0.0, 0.13169999999999998, -0.2234, 0.1017},
+                                     {0.0, 3.9302962368947516, -5.941033872131505, 2.490627285651253},
+                                     {0.0, -12.411077166933676, 30.33818863028232, -16.548102889244902},
+                                     {0.0, 37.50931341651104, -88.1789048947664, 47.37952196281928},
+                                     {0.0, -27.896526289197286, 65.09189467479366, -34.8706578614966},
+                                     {0.0, 1.5, -4.0, 2.5}}};
+#define ND_R(i, j) kTsit5Dense.r[i - 1][j - 1]
+
+// the dense output at s from the step's start value y0, first derivative k1 and the
+// coefficients d_j = sum_i r_ij k_i of s^2, s^3, s^4
+__device__ __forceinline__ double dense_at(double y0, double k1, double d2, double d3, double d4, double h, double s) {
+    return y0 + h * (s * (k1 + s * (d2 + s * (d3 + s * d4))));
+}
 
 template <int VM, int EM, int METHOD, int V, int LB>
 __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
@@ -338,6 +412,7 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
         tcur = a.t0 + static_cast<double>(done) * dt;
     } else {
         double k1[V][D], k2[V][D], k3[V][D], k4[V][D], k5[V][D], k6[V][D], k7[V][D], ys[V][D];
+        double d2[V][D], d3[V][D], d4[V][D];      // dense output (crossing location only)
         rhs<VM, EM, V>(I, L, y, pv, k1);
         ++nf;
         auto wnorm2 = [&](const double (&x)[V][D], const double (&ref)[V][D]) {
@@ -377,6 +452,22 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
         double qold = 1e-4;
         int64_t iters = 0;
         double tstop = ee >= 0 ? te : a.t1;       // the next time a step must end on
+        // crossing location: a second phase buffer and, per edge, the condition's sign at
+        // the step start (gsign) and end (gnext) and the first point of the step where it
+        // crossed (gev), in LDS after the u16 tables
+        constexpr bool kLocate = METHOD == 2;
+        const double* lim = a.limit ? a.limit + I.soff / 2 : nullptr;
+        double* theta2 = nullptr;
+        int8_t *gsign = nullptr, *gnext = nullptr, *gev = nullptr;
+        if constexpr (kLocate) {
+            char* base = reinterpret_cast<char*>(smem);
+            const size_t off = (static_cast<size_t>(reinterpret_cast<char*>(L.cptr + I.n + 1) - base) + 7) & ~size_t(7);
+            theta2 = reinterpret_cast<double*>(base + off);
+            gsign = reinterpret_cast<int8_t*>(theta2 + I.n);
+            gnext = gsign + I.ne;
+            gev = gnext + I.ne;
+            if (lim) line_signs<VM, EM, V>(I, L, lim, y, gsign);
+        }
         while (tcur < a.t1) {
             if (iters++ >= a.maxiters) { retcode = 1; break; }
             if (dt < a.dtmin) { retcode = 2; break; }
@@ -398,6 +489,21 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
             ND_STAGE(c_a71 * k1[q][j] + c_a72 * k2[q][j] + c_a73 * k3[q][j] + c_a74 * k4[q][j] + c_a75 * k5[q][j] +
                      c_a76 * k6[q][j]);
 #undef ND_STAGE
+            if constexpr (kLocate) {
+                // dense-output coefficients from k1..k6 (k7 follows); they take the place
+                // of k3..k6 in registers
+#pragma unroll
+                for (int q = 0; q < V; ++q)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) {
+                        d2[q][j] = ND_R(1, 2) * k1[q][j] + ND_R(2, 2) * k2[q][j] + ND_R(3, 2) * k3[q][j] +
+                                   ND_R(4, 2) * k4[q][j] + ND_R(5, 2) * k5[q][j] + ND_R(6, 2) * k6[q][j];
+                        d3[q][j] = ND_R(1, 3) * k1[q][j] + ND_R(2, 3) * k2[q][j] + ND_R(3, 3) * k3[q][j] +
+                                   ND_R(4, 3) * k4[q][j] + ND_R(5, 3) * k5[q][j] + ND_R(6, 3) * k6[q][j];
+                        d4[q][j] = ND_R(1, 4) * k1[q][j] + ND_R(2, 4) * k2[q][j] + ND_R(3, 4) * k3[q][j] +
+                                   ND_R(4, 4) * k4[q][j] + ND_R(5, 4) * k5[q][j] + ND_R(6, 4) * k6[q][j];
+                    }
+            }
             // error-estimate combination of k1..k6 before the last evaluation, so only
             // y, ynew and this sum stay live across it
 #pragma unroll
@@ -408,6 +514,16 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
                                c_bt5 * k5[q][j] + c_bt6 * k6[q][j];
             rhs<VM, EM, V>(I, L, ys, pv, k7);
             nf += 6;
+            if constexpr (kLocate) {
+#pragma unroll
+                for (int q = 0; q < V; ++q)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) {
+                        d2[q][j] = d2[q][j] + ND_R(7, 2) * k7[q][j];
+                        d3[q][j] = d3[q][j] + ND_R(7, 3) * k7[q][j];
+                        d4[q][j] = d4[q][j] + ND_R(7, 4) * k7[q][j];
+                    }
+            }
             double s = 0.0;
 #pragma unroll
             for (int q = 0; q < V; ++q)
@@ -427,32 +543,130 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
                 // OrdinaryDiffEq's fixed_t_for_floatingpoint_error! does)
                 const double tn = tcur + h;
                 const bool at_stop = last || fabs(tn - tstop) < 100.0 * 2.220446049250313e-16 * fmax(fabs(tcur), fabs(tstop));
-                tcur = at_stop ? tstop : tn;
-#pragma unroll
-                for (int q = 0; q < V; ++q)
-#pragma unroll
-                    for (int j = 0; j < D; ++j) { y[q][j] = ys[q][j]; k1[q][j] = k7[q][j]; }
                 qold = fmax(eest, 1e-4);
                 dt = fmin(h / qq, a.dtmax);
                 ++naccept;
-                // fval holds the flows at the accepted state (the last evaluation was at ys);
-                // after a trip the first-same-as-last derivative is stale: re-evaluate it
                 int changed = 0;
-                if (a.limit) {
-                    const int tr = trip_lines<VM, EM>(a, I, L);
-                    ntrip += tr;
-                    changed += tr;
+                bool located = false;
+                if constexpr (kLocate) {
+                    if (lim) {
+                        // the condition at the points s = m/9 of the step (m = 9: the accepted
+                        // state), against its sign at the step start
+                        double first = 10.0;
+#pragma unroll 1
+                        for (int m = 1; m <= 9; ++m) {
+                            double* th = (m & 1) ? theta2 : L.theta;   // alternate: one barrier a point
+                            const double s = static_cast<double>(m) / 9.0;
+#pragma unroll
+                            for (int q = 0; q < V; ++q) {
+                                const int v = t + q * T;
+                                if (v < I.n)
+                                    th[v] = m == 9 ? ys[q][0]
+                                                   : dense_at(y[q][0], k1[q][0], d2[q][0], d3[q][0], d4[q][0], h, s);
+                            }
+                            __syncthreads();
+                            for (int e = t; e < I.ne; e += T) {
+                                const int8_t ns = sign8(line_cond<EM>(L, th, lim, e));
+                                if (m == 1) gev[e] = 0;
+                                if (gev[e] == 0 && crossed(gsign[e], ns)) {
+                                    gev[e] = static_cast<int8_t>(m);
+                                    first = fmin(first, static_cast<double>(m));
+                                }
+                                if (m == 9) gnext[e] = ns;
+                            }
+                        }
+                        first = block_min(first, L.red);
+                        if (first < 10.0) {
+                            // bisect [point mf-1, point mf] on the dense output, over the edges
+                            // that crossed at point mf, down to adjacent doubles
+                            const int mf = static_cast<int>(first);
+                            double lo = tcur + h * (static_cast<double>(mf - 1) / 9.0);
+                            double hi = tcur + h * (static_cast<double>(mf) / 9.0);
+                            bool hi_end = mf == 9;                  // hi: still the accepted state
+                            for (int it = 0; it < 256; ++it) {
+                                const double mid = lo + 0.5 * (hi - lo);
+                                if (!(mid > lo && mid < hi)) break;
+                                const double s = (mid - tcur) / h;
+#pragma unroll
+                                for (int q = 0; q < V; ++q) {
+                                    const int v = t + q * T;
+                                    if (v < I.n) L.theta[v] = dense_at(y[q][0], k1[q][0], d2[q][0], d3[q][0], d4[q][0], h, s);
+                                }
+                                __syncthreads();
+                                double hit = 0.0;
+                                for (int e = t; e < I.ne; e += T)
+                                    if (gev[e] == mf && crossed(gsign[e], sign8(line_cond<EM>(L, L.theta, lim, e)))) hit = 1.0;
+                                if (block_sum(hit, L.red) > 0.0) {
+                                    hi = mid;
+                                    hi_end = false;
+                                } else {
+                                    lo = mid;
+                                }
+                            }
+                            // the edges that crossed by hi trip; the solution moves to lo, the
+                            // left end of the bracket (DiffEqBase's LeftRootFind)
+                            const double shi = (hi - tcur) / h;
+#pragma unroll
+                            for (int q = 0; q < V; ++q) {
+                                const int v = t + q * T;
+                                if (v < I.n)
+                                    L.theta[v] = hi_end ? ys[q][0]
+                                                        : dense_at(y[q][0], k1[q][0], d2[q][0], d3[q][0], d4[q][0], h, shi);
+                            }
+                            __syncthreads();
+                            constexpr int NPE1 = NPE > 0 ? NPE : 1;
+                            double* pw = a.p + I.poff + static_cast<int64_t>(I.n) * VPar<VM>::np;
+                            double cnt = 0.0;
+                            for (int e = t; e < I.ne; e += T)
+                                if (gev[e] == mf && crossed(gsign[e], sign8(line_cond<EM>(L, L.theta, lim, e)))) {
+                                    L.epar[e * NPE1] = 0.0;
+                                    pw[static_cast<int64_t>(e) * NPE1] = 0.0;
+                                    cnt += 1.0;
+                                }
+                            ntrip += static_cast<int>(block_sum(cnt, L.red));
+                            const double slo = (lo - tcur) / h;
+#pragma unroll
+                            for (int q = 0; q < V; ++q)
+#pragma unroll
+                                for (int j = 0; j < D; ++j)
+                                    y[q][j] = dense_at(y[q][j], k1[q][j], d2[q][j], d3[q][j], d4[q][j], h, slo);
+                            tcur = lo;
+                            located = true;
+                            changed = 1;                            // the state moved: new derivative
+                        }
+                    }
                 }
-                if (at_stop && ee >= 0 && tstop == te) {   // the preset trip's time is reached
-                    const int tr = trip_edge<VM, EM>(a, I, L, ee);
-                    ntrip += tr;
-                    changed += tr;
-                    ee = -1;
-                    tstop = a.t1;
+                if (!located) {
+                    tcur = at_stop ? tstop : tn;
+#pragma unroll
+                    for (int q = 0; q < V; ++q)
+#pragma unroll
+                        for (int j = 0; j < D; ++j) { y[q][j] = ys[q][j]; k1[q][j] = k7[q][j]; }
+                    if constexpr (kLocate) {
+                        int8_t* sw = gsign;                         // the end signs start the next step
+                        gsign = gnext;
+                        gnext = sw;
+                    } else if (a.limit) {
+                        // fval holds the flows at the accepted state (the last evaluation was at ys)
+                        const int tr = trip_lines<VM, EM>(a, I, L);
+                        ntrip += tr;
+                        changed += tr;
+                    }
+                    if (at_stop && ee >= 0 && tstop == te) {   // the preset trip's time is reached
+                        const int tr = trip_edge<VM, EM>(a, I, L, ee);
+                        ntrip += tr;
+                        changed += tr;
+                        ee = -1;
+                        tstop = a.t1;
+                    }
                 }
+                // after a trip or a move to a crossing the first-same-as-last derivative
+                // (and the condition signs) are stale: re-evaluate them
                 if (changed > 0) {
                     rhs<VM, EM, V>(I, L, y, pv, k1);
                     ++nf;
+                    if constexpr (kLocate)
+                        if (lim) line_signs<VM, EM, V>(I, L, lim, y, gsign);
                 }
                 if (a.ss && at_steady_state<V, D>(a, I, L, y, k1)) {
                     retcode = 4;
@@ -500,18 +714,30 @@ EnsKernel pick_v(int vper, int threads) {
     }
 }
 
+// kernel method: 0 Tsit5, 1 RK4, 2 Tsit5 with line crossings located on the dense output
 template <int VM, int EM>
-EnsKernel pick_m(int method, int vper, int threads) {
-    return method == ND_METHOD_RK4 ? pick_v<VM, EM, 1>(vper, threads) : pick_v<VM, EM, 0>(vper, threads);
+EnsKernel pick_m(int kmethod, int vper, int threads) {
+    switch (kmethod) {
+    case 1: return pick_v<VM, EM, 1>(vper, threads);
+    case 2: return pick_v<VM, EM, 2>(vper, threads);
+    default: return pick_v<VM, EM, 0>(vper, threads);
+    }
 }
 
-EnsKernel pick_ensemble(int vm, int em, int method, int vper, int threads) {
-    if (vm == ND_VM_SWING && em == ND_EM_KURAMOTO) return pick_m<ND_VM_SWING, ND_EM_KURAMOTO>(method, vper, threads);
-    if (vm == ND_VM_SWING && em == ND_EM_STATIC_LINE) return pick_m<ND_VM_SWING, ND_EM_STATIC_LINE>(method, vper, threads);
-    if (vm == ND_VM_KURAMOTO && em == ND_EM_KURAMOTO) return pick_m<ND_VM_KURAMOTO, ND_EM_KURAMOTO>(method, vper, threads);
+EnsKernel pick_ensemble(int vm, int em, int kmethod, int vper, int threads) {
+    if (vm == ND_VM_SWING && em == ND_EM_KURAMOTO) return pick_m<ND_VM_SWING, ND_EM_KURAMOTO>(kmethod, vper, threads);
+    if (vm == ND_VM_SWING && em == ND_EM_STATIC_LINE) return pick_m<ND_VM_SWING, ND_EM_STATIC_LINE>(kmethod, vper, threads);
+    if (vm == ND_VM_KURAMOTO && em == ND_EM_KURAMOTO) return pick_m<ND_VM_KURAMOTO, ND_EM_KURAMOTO>(kmethod, vper, threads);
     if (vm == ND_VM_KURAMOTO && em == ND_EM_STATIC_LINE)
-        return pick_m<ND_VM_KURAMOTO, ND_EM_STATIC_LINE>(method, vper, threads);
+        return pick_m<ND_VM_KURAMOTO, ND_EM_STATIC_LINE>(kmethod, vper, threads);
     return nullptr;
+}
+
+// the kernel method of a solve: crossings are located for Tsit5 with line limits unless
+// step-end checks were asked for
+int kernel_method(const nd_ensemble_opts* o) {
+    if (o->method == ND_METHOD_RK4) return 1;
+    return (o->line_limit && o->line_events != ND_LINES_STEP_END) ? 2 : 0;
 }
 
 int vpar(int vm) {
@@ -532,6 +758,7 @@ int hip_err(hipError_t e, const char* what) {
 
 struct nd_ensemble {
     int device = 0;
+    int lds_max = 0;                                // dynamic LDS a workgroup may take
     int32_t ninst = 0;
     nd_kind vk{}, ek{};
     int32_t dim = 0;
@@ -630,6 +857,11 @@ int make_args(nd_ensemble* en, double* u, double* p, double t0, double t1, const
     if (!(t1 >= t0)) return nd::set_error(ND_ERR_ARGUMENT, "t1 < t0");
     if (o->method != ND_METHOD_TSIT5 && o->method != ND_METHOD_RK4) return nd::set_error(ND_ERR_ARGUMENT, "unknown method");
     if (o->method == ND_METHOD_RK4 && !(o->dt0 > 0.0)) return nd::set_error(ND_ERR_ARGUMENT, "RK4 needs dt0 > 0");
+    if (o->line_events != ND_LINES_DEFAULT && o->line_events != ND_LINES_STEP_END && o->line_events != ND_LINES_CROSSING)
+        return nd::set_error(ND_ERR_ARGUMENT, "line_events must be ND_LINES_DEFAULT, ND_LINES_STEP_END or ND_LINES_CROSSING");
+    if (o->method == ND_METHOD_RK4 && o->line_events == ND_LINES_CROSSING)
+        return nd::set_error(ND_ERR_UNSUPPORTED, "line crossings are located on Tsit5's dense output; RK4 checks limits "
+                             "at step ends (ND_LINES_STEP_END)");
     int dev = -1;
     (void)hipGetDevice(&dev);
     if (dev != en->device) return nd::set_error(ND_ERR_ARGUMENT, "ensemble lives on device %d", en->device);
@@ -665,15 +897,20 @@ int make_args(nd_ensemble* en, double* u, double* p, double t0, double t1, const
     return ND_OK;
 }
 
-int32_t class_lds(const nd_ensemble* en, int c) {
+// dynamic LDS of class c: the largest instance's tables, plus for crossing location a
+// second phase buffer and three bytes per edge (after 8-byte alignment)
+int32_t class_lds(const nd_ensemble* en, int c, int kmethod) {
     int32_t lds = 0;
-    for (int32_t q = en->cls_first[c]; q < en->cls_first[c + 1]; ++q) lds = std::max(lds, en->inst[en->order[q]].lds_bytes);
+    for (int32_t q = en->cls_first[c]; q < en->cls_first[c + 1]; ++q) {
+        const InstDesc& I = en->inst[en->order[q]];
+        lds = std::max(lds, I.lds_bytes + (kmethod == 2 ? 8 + 8 * I.n + 3 * I.ne : 0));
+    }
     return lds;
 }
 
 // Streams, events and the dynamic-LDS limit of every class kernel; host-side only,
 // so a capture that follows records nothing but the launches and the event edges.
-int prepare(nd_ensemble* en, int method) {
+int prepare(nd_ensemble* en, int kmethod) {
     int rc = ND_OK;
     if (!en->ev_fork) rc = hip_err(hipEventCreateWithFlags(&en->ev_fork, hipEventDisableTiming), "hipEventCreate");
     for (int c = 0; c < ND_ENSEMBLE_CLASSES && !rc; ++c) {
@@ -687,9 +924,12 @@ int prepare(nd_ensemble* en, int method) {
     int32_t lds[ND_ENSEMBLE_CLASSES] = {};
     for (int c = 0; c < ND_ENSEMBLE_CLASSES; ++c) {
         if (en->cls_first[c + 1] == en->cls_first[c]) continue;
-        ks[c] = pick_ensemble(en->vk.model, en->ek.model, method, en->cls_vper[c], en->cls_threads[c]);
+        ks[c] = pick_ensemble(en->vk.model, en->ek.model, kmethod, en->cls_vper[c], en->cls_threads[c]);
         if (!ks[c]) return nd::set_error(ND_ERR_UNSUPPORTED, "no ensemble kernel for these models");
-        lds[c] = class_lds(en, c);
+        lds[c] = class_lds(en, c, kmethod);
+        if (lds[c] > en->lds_max)
+            return nd::set_error(ND_ERR_UNSUPPORTED, "class %d needs %d bytes of LDS with line crossings located "
+                                 "(device limit %d)", c, lds[c], en->lds_max);
     }
     // one template instance may serve several classes: give it the largest need
     for (int c = 0; c < ND_ENSEMBLE_CLASSES; ++c) {
@@ -707,19 +947,19 @@ int prepare(nd_ensemble* en, int method) {
 
 // Fork: every non-empty class runs on its own stream after the caller's prior work;
 // join: the caller's stream waits for all of them.  Largest class first.
-int launch_classes(nd_ensemble* en, SolveArgs a, int method, hipStream_t s) {
+int launch_classes(nd_ensemble* en, SolveArgs a, int kmethod, hipStream_t s) {
     int rc = hip_err(hipEventRecord(en->ev_fork, s), "hipEventRecord");
     for (int c = ND_ENSEMBLE_CLASSES - 1; c >= 0 && !rc; --c) {
         const int32_t cnt = en->cls_first[c + 1] - en->cls_first[c];
         if (cnt == 0) continue;
-        EnsKernel k = pick_ensemble(en->vk.model, en->ek.model, method, en->cls_vper[c], en->cls_threads[c]);
+        EnsKernel k = pick_ensemble(en->vk.model, en->ek.model, kmethod, en->cls_vper[c], en->cls_threads[c]);
         hipStream_t cs = en->cls_stream[c];
         rc = hip_err(hipStreamWaitEvent(cs, en->ev_fork, 0), "hipStreamWaitEvent");
         if (rc) break;
         a.first = en->cls_first[c];
         a.count = cnt;
         hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(cnt)), dim3(static_cast<unsigned>(en->cls_threads[c])),
-                           static_cast<unsigned>(class_lds(en, c)), cs, a);
+                           static_cast<unsigned>(class_lds(en, c, kmethod)), cs, a);
         rc = hip_err(hipGetLastError(), "ensemble launch");
         rc = rc ? rc : hip_err(hipEventRecord(en->ev_join[c], cs), "hipEventRecord");
         rc = rc ? rc : hip_err(hipStreamWaitEvent(s, en->ev_join[c], 0), "hipStreamWaitEvent");
@@ -751,6 +991,7 @@ int nd_ensemble_create(int32_t ninst, const int64_t* nv, const int64_t* ne, cons
     int lds_max = 0;
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, en->device) != hipSuccess || lds_max <= 0)
         lds_max = 64 * 1024;
+    en->lds_max = lds_max;
     en->ninst = ninst;
     en->vk = *vkind;
     en->ek = *ekind;
@@ -866,6 +1107,13 @@ int nd_ensemble_set_class(nd_ensemble* en, int32_t cls, int32_t threads, int32_t
     return regroup(en);
 }
 
+int nd_tsit5_dense(double* r) {
+    if (!r) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
+    for (int i = 0; i < 7; ++i)
+        for (int j = 0; j < 4; ++j) r[i * 4 + j] = kTsit5Dense.r[i][j];
+    return ND_OK;
+}
+
 int nd_tsit5_tableau(double* a, double* btilde, double* c) {
     if (!a || !btilde || !c) return nd::set_error(ND_ERR_ARGUMENT, "NULL argument");
     for (int i = 0; i < 7; ++i) {
@@ -880,8 +1128,8 @@ int nd_ensemble_solve(nd_ensemble* en, double* u, double* p, double t0, double t
                       nd_ensemble_stat* stats, void* stream) {
     SolveArgs a{};
     int rc = make_args(en, u, p, t0, t1, o, stats, &a);
-    rc = rc ? rc : prepare(en, o->method);
-    return rc ? rc : launch_classes(en, a, o->method, static_cast<hipStream_t>(stream));
+    rc = rc ? rc : prepare(en, kernel_method(o));
+    return rc ? rc : launch_classes(en, a, kernel_method(o), static_cast<hipStream_t>(stream));
 }
 
 int nd_ensemble_graph_create(nd_ensemble* en, double* u, double* p, double t0, double t1,
@@ -890,7 +1138,7 @@ int nd_ensemble_graph_create(nd_ensemble* en, double* u, double* p, double t0, d
     *out = nullptr;
     SolveArgs a{};
     int rc = make_args(en, u, p, t0, t1, o, stats, &a);
-    rc = rc ? rc : prepare(en, o->method);
+    rc = rc ? rc : prepare(en, kernel_method(o));
     if (rc) return rc;
     if (!en->cap_stream) {
         rc = hip_err(hipStreamCreateWithFlags(&en->cap_stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
@@ -902,7 +1150,7 @@ int nd_ensemble_graph_create(nd_ensemble* en, double* u, double* p, double t0, d
     gr->generation = en->generation;
     rc = hip_err(hipStreamBeginCapture(en->cap_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     if (rc) { delete gr; return rc; }
-    const int lrc = launch_classes(en, a, o->method, en->cap_stream);
+    const int lrc = launch_classes(en, a, kernel_method(o), en->cap_stream);
     hipGraph_t g = nullptr;
     rc = hip_err(hipStreamEndCapture(en->cap_stream, &g), "hipStreamEndCapture");
     gr->graph = g;

@@ -31,8 +31,20 @@ def tsit5_tableau():
     return tab, bt, c
 
 
+def tsit5_dense():
+    """The dense output the ensemble kernel locates line crossings on (nd_tsit5_dense):
+    r (7x4) with b_i(s) = sum_j r[i, j-1] s^j."""
+    r = np.zeros(28)
+    A.check(A.lib().nd_tsit5_dense(r.ctypes.data))
+    return r.reshape(7, 4)
+
+
+# when line limits are checked (nd_ensemble_opts.line_events)
+LINE_EVENTS = {None: 0, "step_end": 1, "crossing": 2}
+
+
 class _Opts(C.Structure):
-    _fields_ = [("method", C.c_int32), ("pad_", C.c_int32), ("reltol", C.c_double), ("abstol", C.c_double),
+    _fields_ = [("method", C.c_int32), ("line_events", C.c_int32), ("reltol", C.c_double), ("abstol", C.c_double),
                 ("dt0", C.c_double), ("dtmax", C.c_double), ("dtmin", C.c_double), ("maxiters", C.c_int64),
                 ("line_limit", C.c_void_p), ("ss_abstol", C.c_double), ("ss_reltol", C.c_double),
                 ("event_t", C.c_void_p), ("event_e", C.c_void_p)]
@@ -116,10 +128,14 @@ class Ensemble:
         return slice(int(self.param_offsets[i]), int(self.param_offsets[i] + n))
 
     def solve(self, u, p, tspan, method=TSIT5, reltol=1e-8, abstol=1e-8, dt=0.0, dtmax=0.0, maxiters=0,
-              line_limit=None, steady_state=None, preset_trip=None):
+              line_limit=None, steady_state=None, preset_trip=None, line_events=None):
         """Integrates all instances in place on the device vector u; returns per-instance stats.
-        line_limit: per-edge limits (device, instance order) to trip lines whose |flow|
-        exceeds them at the end of an accepted step; tripped lines get K = 0 in p.
+        line_limit: per-edge limits (device, instance order) of the cascading-failure
+        callback (condition |P| - limit, affect K = 0, in p as well).  line_events:
+        "crossing" locates every up- or down-crossing of the condition inside the step on
+        Tsit5's dense output, as the reference's ContinuousComponentCallback does;
+        "step_end" trips lines whose |P| exceeds the limit at the end of an accepted step;
+        None: "crossing" for Tsit5, "step_end" for RK4.
         steady_state: True or (abstol, reltol) to stop each instance after the first
         accepted step where every |du_i| <= abstol or <= reltol*|u_i| (retcode TERMINATED,
         stats t = the stopping time), as a TerminateSteadyState callback does.
@@ -131,14 +147,16 @@ class Ensemble:
         self._check(u, p)
         stats = torch.zeros(self.ninst * STAT_DTYPE.itemsize // 8, dtype=torch.float64, device=u.device)
         o = self._opts(method, reltol, abstol, dt, dtmax, maxiters, line_limit, steady_state, preset_trip,
-                       tspan, u.device)
+                       tspan, u.device, line_events)
         stream = torch.cuda.current_stream(u.device).cuda_stream
         A.check(A.lib().nd_ensemble_solve(self._h, u.data_ptr(), p.data_ptr(), float(tspan[0]), float(tspan[1]),
                                           C.byref(o), stats.data_ptr(), stream))
         return stats
 
     def _opts(self, method, reltol, abstol, dt, dtmax, maxiters, line_limit=None, steady_state=None,
-              preset_trip=None, tspan=None, device=None):
+              preset_trip=None, tspan=None, device=None, line_events=None):
+        if line_events not in LINE_EVENTS:
+            raise A.ArgumentError("line_events must be None, 'step_end' or 'crossing'")
         ss = (0.0, 0.0)
         if steady_state is True:
             ss = STEADY_STATE_TOLS
@@ -169,7 +187,7 @@ class Ensemble:
                     raise A.ArgumentError("RK4: preset trip times must lie on the step grid t0 + k*dt")
             self._events = (torch.from_numpy(times).to(device), torch.from_numpy(edges).to(device))
             ev_t, ev_e = self._events[0].data_ptr(), self._events[1].data_ptr()
-        return _Opts(int(method), 0, float(reltol), float(abstol), float(dt), float(dtmax), 0.0, int(maxiters), lim,
+        return _Opts(int(method), LINE_EVENTS[line_events], float(reltol), float(abstol), float(dt), float(dtmax), 0.0, int(maxiters), lim,
                      ss[0], ss[1], ev_t, ev_e)
 
     def _check(self, u, p):
@@ -180,12 +198,13 @@ class Ensemble:
             raise A.ArgumentError("u / p do not match the ensemble's sizes")
 
     def capture(self, u, p, tspan, method=TSIT5, reltol=1e-8, abstol=1e-8, dt=0.0, dtmax=0.0, maxiters=0,
-                line_limit=None, steady_state=None):
+                line_limit=None, steady_state=None, line_events=None):
         """Captures the solve of (u, p, tspan) into a hipGraph; `.launch()` replays it in place on u."""
         import torch
         self._check(u, p)
         stats = torch.zeros(self.ninst * STAT_DTYPE.itemsize // 8, dtype=torch.float64, device=u.device)
-        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters, line_limit, steady_state)
+        o = self._opts(method, reltol, abstol, dt, dtmax, maxiters, line_limit, steady_state,
+                       line_events=line_events)
         h = C.c_void_p()
         A.check(A.lib().nd_ensemble_graph_create(self._h, u.data_ptr(), p.data_ptr(), float(tspan[0]),
                                                  float(tspan[1]), C.byref(o), stats.data_ptr(), C.byref(h)))

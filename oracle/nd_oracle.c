@@ -926,9 +926,22 @@ int ndo_rk4(ndo_net* nw, double* u, const double* p, double t0, double dt, int64
  * Stages are evaluated at t + c_i h.  The last step is shortened to end on t1 (or on a
  * preset trip time, a tstop); a step that ends within 100 ulp of the stop time ends on it
  * (fixed_t_for_floatingpoint_error! in OrdinaryDiffEq).  Callbacks after an accepted step, in this order: line
- * trips (|edge dst output| > limit sets the edge's first parameter to 0), the preset
- * trip, then a re-evaluated first-same-as-last derivative if a parameter changed, then
- * the steady-state test (every |du_i| <= ss_abstol or <= ss_reltol*|u_i|).
+ * trips, the preset trip, then a re-evaluated first-same-as-last derivative if a
+ * parameter or the state changed, then the steady-state test (every |du_i| <= ss_abstol
+ * or <= ss_reltol*|u_i|).  Line trips set an edge's first parameter to 0:
+ *   line_events = 1 (step end): where |edge dst output| > limit at the accepted state;
+ *   line_events = 2 (crossing): the reference's ContinuousComponentCallback
+ *     (src/callbacks.jl:113-140; the cascading-failure condition |P| - limit with the
+ *     same affect on up- and down-crossings, docs/examples/cascading_failure.jl:104-111)
+ *     run as DiffEqBase's VectorContinuousCallback (DiffEqBase 7, Project.toml:74; not in
+ *     the reference tree): the condition at the step start (uprev) against the
+ *     condition at t + h*m/9, m = 1..9 on the dense output (m = 9: the accepted state);
+ *     an edge crosses where its start value g0 != 0 and g0*g <= 0 (sign-wise); at the
+ *     first m with a crossing each crossing edge's root is bisected on the dense output
+ *     between the points m-1 and m down to adjacent doubles, keeping the left end
+ *     (LeftRootFind); the earliest root wins, every edge whose root equals it trips, the
+ *     solution moves to the dense output there and the step size proposed by the
+ *     controller is kept.  A crossing inside the step comes before the preset trip.
  * retcodes: 0 reached t1, 1 maxiters, 2 dt < dtmin, 3 NaN step size, 4 steady state.
  * stats[6] = t, dt, naccept, nreject, nf, ntrip.
  * -------------------------------------------------------------------------------------*/
@@ -956,6 +969,37 @@ void ndo_tsit5_tableau(double* A, double* bt, double* c) {
     }
 }
 
+/* Tsitouras' continuous extension (the dense output OrdinaryDiffEqTsit5 interpolates
+ * with): b_i(s) = sum_j T5_R[i][j-1] s^j, i over the seven stage derivatives */
+static const double T5_R[7][4] = {
+    {1.0, -2.763706197274826, 2.9132554618219126, -1.0530884977290216},
+    {0.0, 0.13169999999999998, -0.2234, 0.1017},
+    {0.0, 3.9302962368947516, -5.941033872131505, 2.490627285651253},
+    {0.0, -12.411077166933676, 30.33818863028232, -16.548102889244902},
+    {0.0, 37.50931341651104, -88.1789048947664, 47.37952196281928},
+    {0.0, -27.896526289197286, 65.09189467479366, -34.8706578614966},
+    {0.0, 1.5, -4.0, 2.5},
+};
+
+void ndo_tsit5_dense(double* r) {
+    for (int i = 0; i < 7; ++i)
+        for (int j = 0; j < 4; ++j) r[i * 4 + j] = T5_R[i][j];
+}
+
+/* out = u + h * s * (k1 + s * (d2 + s * (d3 + s * d4))), d_j = sum_i r_ij k_i */
+static void t5_dense_at(double* out, const double* u, double* const* k, const double* d, int64_t n, double h,
+                        double s) {
+    const double* d2 = d;
+    const double* d3 = d + n;
+    const double* d4 = d + 2 * n;
+    for (int64_t i = 0; i < n; ++i) out[i] = u[i] + h * (s * (k[0][i] + s * (d2[i] + s * (d3[i] + s * d4[i]))));
+}
+
+/* the sign test of a crossing: start value g0 nonzero, g0*g <= 0 (NaN never crosses) */
+static int crossed(double g0, double g) {
+    return (g0 < 0.0 && g >= 0.0) || (g0 > 0.0 && g <= 0.0);
+}
+
 static double wnorm2(const double* x, const double* r1, const double* r2, int64_t n, double abstol, double reltol) {
     double s = 0.0;
     for (int64_t i = 0; i < n; ++i) {
@@ -973,14 +1017,21 @@ static void edge_flows(ndo_net* nw, double* flow, const double* u, const double*
 }
 
 int ndo_tsit5(ndo_net* nw, double* u, double* p, double t0, double t1, double reltol, double abstol, double dt0,
-              double dtmax, double dtmin, int64_t maxiters, const double* line_limit, double ss_abstol,
-              double ss_reltol, double event_t, int64_t event_e, double* stats) {
+              double dtmax, double dtmin, int64_t maxiters, const double* line_limit, int line_events,
+              double ss_abstol, double ss_reltol, double event_t, int64_t event_e, double* stats) {
     const int64_t n = nw->lastidx_dynamic;
+    const int64_t ne1 = nw->ne ? nw->ne : 1;
     double* k[7];
     for (int s = 0; s < 7; ++s) k[s] = (double*)xcalloc((size_t)n, 8);
     double* ys = (double*)xcalloc((size_t)n, 8);
     double* err = (double*)xcalloc((size_t)n, 8);
-    double* flow = (double*)xcalloc((size_t)(nw->ne ? nw->ne : 1), 8);
+    double* flow = (double*)xcalloc((size_t)ne1, 8);
+    const int locate = line_limit && line_events != 1;
+    double* dc = (double*)xcalloc((size_t)(3 * n), 8);      /* dense coefficients d2, d3, d4 */
+    double* ui = (double*)xcalloc((size_t)n, 8);            /* dense-output state */
+    double* g0 = (double*)xcalloc((size_t)ne1, 8);          /* condition at the step start */
+    double* root = (double*)xcalloc((size_t)ne1, 8);
+    int* cand = (int*)xcalloc((size_t)ne1, sizeof(int));
     if (!(dtmax > 0)) dtmax = t1 - t0;
     if (!(dtmin > 0)) dtmin = 1e-14 * fmax(1.0, fabs(t1));
     if (maxiters <= 0) maxiters = 1000000;
@@ -1026,6 +1077,13 @@ int ndo_tsit5(ndo_net* nw, double* u, double* p, double t0, double t1, double re
             rhs_impl(nw, k[s], ys, p, t + T5_C[s] * h, 1, NULL, NULL, NULL);
         }
         nf += 6;
+        if (locate)
+            for (int j = 1; j < 4; ++j)
+                for (int64_t i = 0; i < n; ++i) {
+                    double acc = T5_R[0][j] * k[0][i];
+                    for (int s = 1; s < 7; ++s) acc = acc + T5_R[s][j] * k[s][i];
+                    dc[(j - 1) * n + i] = acc;
+                }
         for (int64_t i = 0; i < n; ++i) {
             double e = T5_BT[0] * k[0][i];
             for (int j = 1; j < 7; ++j) e = e + T5_BT[j] * k[j][i];
@@ -1037,26 +1095,71 @@ int ndo_tsit5(ndo_net* nw, double* u, double* p, double t0, double t1, double re
             const double qq = eest == 0.0 ? 0.1 : fmax(0.1, fmin(5.0, q11 / pow(qold, 0.08) / 0.9));
             const double tn = t + h;
             const int at_stop = last || fabs(tn - tstop) < 100.0 * 2.220446049250313e-16 * fmax(fabs(t), fabs(tstop));
-            t = at_stop ? tstop : tn;
-            memcpy(u, ys, (size_t)n * 8);
-            memcpy(k[0], k[6], (size_t)n * 8);
             qold = fmax(eest, 1e-4);
             dt = fmin(h / qq, dtmax);
             ++naccept;
-            int changed = 0;
-            if (line_limit) {
+            int changed = 0, located = 0;
+            if (locate) {
+                /* condition at the step start (uprev, with the current parameters) */
                 edge_flows(nw, flow, u, p, t, err);
-                for (int64_t e = 0; e < nw->ne; ++e) {
-                    double* K = &p[nw->e_para[e].first - 1];
-                    if (*K != 0.0 && fabs(flow[e]) > line_limit[e]) { *K = 0.0; ++ntrip; ++changed; }
+                for (int64_t e = 0; e < nw->ne; ++e) g0[e] = fabs(flow[e]) - line_limit[e];
+                int mf = 0;
+                for (int m = 1; m <= 9 && !mf; ++m) {
+                    if (m == 9) memcpy(ui, ys, (size_t)n * 8);
+                    else t5_dense_at(ui, u, k, dc, n, h, (double)m / 9.0);
+                    edge_flows(nw, flow, ui, p, t, err);
+                    for (int64_t e = 0; e < nw->ne; ++e) {
+                        cand[e] = crossed(g0[e], fabs(flow[e]) - line_limit[e]);
+                        if (cand[e]) mf = m;
+                    }
+                }
+                if (mf) {
+                    double tmin = INFINITY;
+                    for (int64_t e = 0; e < nw->ne; ++e) {
+                        if (!cand[e]) continue;
+                        double lo = t + h * ((double)(mf - 1) / 9.0), hi = t + h * ((double)mf / 9.0);
+                        for (int it = 0; it < 256; ++it) {
+                            const double mid = lo + 0.5 * (hi - lo);
+                            if (!(mid > lo && mid < hi)) break;
+                            t5_dense_at(ui, u, k, dc, n, h, (mid - t) / h);
+                            edge_flows(nw, flow, ui, p, mid, err);
+                            if (crossed(g0[e], fabs(flow[e]) - line_limit[e])) hi = mid;
+                            else lo = mid;
+                        }
+                        root[e] = lo;
+                        if (lo < tmin) tmin = lo;
+                    }
+                    for (int64_t e = 0; e < nw->ne; ++e)
+                        if (cand[e] && root[e] == tmin) {
+                            double* K = &p[nw->e_para[e].first - 1];
+                            *K = 0.0;
+                            ++ntrip;
+                        }
+                    t5_dense_at(ui, u, k, dc, n, h, (tmin - t) / h);
+                    memcpy(u, ui, (size_t)n * 8);
+                    t = tmin;
+                    located = 1;
+                    changed = 1;
                 }
             }
-            if (at_stop && ee >= 0 && tstop == te) {
-                double* K = &p[nw->e_para[ee].first - 1];
-                if (*K != 0.0) { ++ntrip; ++changed; }
-                *K = 0.0;
-                ee = -1;
-                tstop = t1;
+            if (!located) {
+                t = at_stop ? tstop : tn;
+                memcpy(u, ys, (size_t)n * 8);
+                memcpy(k[0], k[6], (size_t)n * 8);
+                if (line_limit && !locate) {
+                    edge_flows(nw, flow, u, p, t, err);
+                    for (int64_t e = 0; e < nw->ne; ++e) {
+                        double* K = &p[nw->e_para[e].first - 1];
+                        if (*K != 0.0 && fabs(flow[e]) > line_limit[e]) { *K = 0.0; ++ntrip; ++changed; }
+                    }
+                }
+                if (at_stop && ee >= 0 && tstop == te) {
+                    double* K = &p[nw->e_para[ee].first - 1];
+                    if (*K != 0.0) { ++ntrip; ++changed; }
+                    *K = 0.0;
+                    ee = -1;
+                    tstop = t1;
+                }
             }
             if (changed) {
                 rhs_impl(nw, k[0], u, p, t, 1, NULL, NULL, NULL);
@@ -1080,5 +1183,6 @@ int ndo_tsit5(ndo_net* nw, double* u, double* p, double t0, double t1, double re
     stats[4] = (double)nf; stats[5] = (double)ntrip;
     for (int s = 0; s < 7; ++s) free(k[s]);
     free(ys); free(err); free(flow);
+    free(dc); free(ui); free(g0); free(root); free(cand);
     return retcode;
 }

@@ -71,10 +71,11 @@ def lib():
         L.ndo_rhs_threaded.argtypes = [P, f64p, f64p, f64p, C.c_double, C.c_int32]
         L.ndo_rhs_scale.argtypes = [P, f64p, f64p, f64p, C.c_double]
         L.ndo_rk4.argtypes = [P, f64p, f64p, C.c_double, C.c_double, C.c_int64]
-        L.ndo_tsit5.argtypes = [P, f64p, f64p] + [C.c_double] * 7 + [C.c_int64, P, C.c_double, C.c_double,
-                                                                     C.c_double, C.c_int64, f64p]
+        L.ndo_tsit5.argtypes = [P, f64p, f64p] + [C.c_double] * 7 + [C.c_int64, P, C.c_int, C.c_double,
+                                                                     C.c_double, C.c_double, C.c_int64, f64p]
         L.ndo_tsit5.restype = C.c_int
         L.ndo_tsit5_tableau.argtypes = [f64p, f64p, f64p]
+        L.ndo_tsit5_dense.argtypes = [f64p]
         L.ndo_aggregate_seq.argtypes = [P, f64p, f64p]
         L.ndo_aggregate_inv.argtypes = [P, f64p, f64p, C.c_int32]
         L.ndo_aggregate_naive.argtypes = [P, f64p, f64p]
@@ -98,6 +99,13 @@ def tsit5_tableau():
     A = np.zeros((7, 7))
     A[:, :6] = A6.reshape(7, 6)
     return A, bt, c
+
+
+def tsit5_dense():
+    """The oracle's dense-output table r (7x4): b_i(s) = sum_j r[i, j-1] s^j."""
+    r = np.zeros(28)
+    lib().ndo_tsit5_dense(r)
+    return r.reshape(7, 4)
 
 
 def make_kind_array(kinds):
@@ -267,11 +275,15 @@ class OracleNet:
         return u
 
     def tsit5(self, u0, p, tspan, reltol=1e-8, abstol=1e-8, dt0=0.0, dtmax=0.0, dtmin=0.0, maxiters=0,
-              line_limit=None, steady_state=(0.0, 0.0), preset_trip=(0.0, -1)):
+              line_limit=None, steady_state=(0.0, 0.0), preset_trip=(0.0, -1), line_events="crossing"):
         """Adaptive Tsit5 from tspan[0] to tspan[1] (ndo_tsit5).  Returns (u, p, stats):
         p with tripped lines' first parameter at 0; stats: t, dt, naccept, nreject, nf,
         ntrip and retcode (0 reached t1, 1 maxiters, 2 dt < dtmin, 3 NaN, 4 steady state).
-        preset_trip: (time, 0-based edge), edge -1 for none."""
+        preset_trip: (time, 0-based edge), edge -1 for none.  line_events: "crossing"
+        (each crossing of |P| - limit located on the dense output) or "step_end"
+        (|P| > limit checked at the end of each accepted step)."""
+        if line_events not in ("crossing", "step_end"):
+            raise ValueError("line_events must be 'crossing' or 'step_end'")
         u = np.array(u0, dtype=np.float64, copy=True)
         if self.check_sizes(len(u), len(u), self.pdim if self.pdim == 0 else len(p)):
             raise ValueError("u or p does not have expected size")
@@ -285,7 +297,7 @@ class OracleNet:
         rc = lib().ndo_tsit5(self._h, u, p, float(tspan[0]), float(tspan[1]), float(reltol), float(abstol),
                              float(dt0), float(dtmax), float(dtmin), int(maxiters),
                              None if lim is None else lim.ctypes.data_as(C.c_void_p),
-                             float(steady_state[0]), float(steady_state[1]), float(preset_trip[0]),
+                             2 if line_events == "crossing" else 1, float(steady_state[0]), float(steady_state[1]), float(preset_trip[0]),
                              int(preset_trip[1]), st)
         stats = dict(t=st[0], dt=st[1], naccept=int(st[2]), nreject=int(st[3]), nf=int(st[4]), ntrip=int(st[5]),
                      retcode=int(rc))

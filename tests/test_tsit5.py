@@ -209,12 +209,14 @@ def test_ensemble_tsit5_matches_oracle_at_c4_starts(ND, dev):
 
 
 @pytest.mark.gpu
-def test_ensemble_tsit5_trips_match_oracle(ND, dev):
-    """Line tripping and a preset trip in the ensemble kernel against the oracle's Tsit5
-    with the same callbacks: the same lines trip (parameters bitwise equal), the retcodes
-    agree.  States agree within 1e-5 scaled: a trip near its threshold may fire one step
-    earlier or later in one of the two, and step sequences that differ by a step already
-    differ by O(reltol)."""
+@pytest.mark.parametrize("mode", ["crossing", "step_end"])
+def test_ensemble_tsit5_trips_match_oracle(ND, dev, mode):
+    """Line tripping (crossings located in the step, or limits checked at step ends) and
+    a preset trip in the ensemble kernel against the oracle's Tsit5 with the same
+    callbacks: the same lines trip (parameters bitwise equal), the retcodes agree.  States
+    agree within 1e-5 scaled: a step-end trip near its threshold may fire one step earlier
+    or later in one of the two, and step sequences that differ by a step already differ
+    by O(reltol)."""
     import torch
     from networkdynamics_jl_amd import experiments as X
     graphs, us, ps, _, _, rng = X.c4_start(dev, ninst=8, seed=23, nmin=60, nmax=400)
@@ -233,13 +235,14 @@ def test_ensemble_tsit5_trips_match_oracle(ND, dev):
     u_d = torch.from_numpy(np.concatenate(us)).to(dev)
     p_d = torch.from_numpy(np.concatenate(ps)).to(dev)
     st = ens.stats_to_numpy(ens.solve(u_d, p_d, (0.0, 20.0), reltol=1e-8, abstol=1e-8,
-                                      line_limit=torch.from_numpy(lim_h).to(dev), preset_trip=(times, edges)))
+                                      line_limit=torch.from_numpy(lim_h).to(dev), preset_trip=(times, edges),
+                                      line_events=mode))
     uf, pf = u_d.cpu().numpy(), p_d.cpu().numpy()
 
     def run(i):
         nw = ND.Network(graphs[i], ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
         return oracle_for(nw).tsit5(us[i], ps[i], (0.0, 20.0), reltol=1e-8, abstol=1e-8, line_limit=lims[i],
-                                    preset_trip=(0.7, int(edges[i])))
+                                    preset_trip=(0.7, int(edges[i])), line_events=mode)
 
     with ThreadPoolExecutor(8) as ex:
         refs = list(ex.map(run, range(n_inst)))
