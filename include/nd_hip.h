@@ -368,6 +368,16 @@ int nd_rhs_perturbed_jvp(nd_plan* plan, double* du, double* jv, const double* u,
 int nd_rhs_jvp(nd_plan* plan, double* du, double* jv, const double* u, const double* v, const double* p, double t,
                int64_t len_du, int64_t len_u, int64_t len_p, void* stream);
 
+/* nw(du, u + sum_j eps_j v_j, p, t) on dual numbers with k partials (1 <= k <= 8): du =
+ * f(u) and jv[:, j] = J(u) v[:, j] in one pass, the chunk of ForwardDiff duals the
+ * reference sizes its DiffCache buffers for (src/construction.jl:205-208, pickchunksize;
+ * src/network_structure.jl:152-155), so a colored Jacobian takes ceil(colors / k)
+ * launches instead of one per color.  v, jv: k device columns of length dim, column j at
+ * v + j*dim.  Column j equals nd_rhs_jvp along v[:, j] bit for bit.  k = 1 is nd_rhs_jvp.
+ * Networks whose vertex batches take the generic or vector-output kernels, or with
+ * feed-forward vertex outputs -> ND_ERR_UNSUPPORTED (nd_rhs_jvp per tangent serves them). */
+int nd_rhs_jvp_multi(nd_plan* plan, double* du, double* jv, const double* u, const double* v, int32_t k,
+                     const double* p, double t, int64_t len_du, int64_t len_u, int64_t len_p, void* stream);
 
 /* collect_externals!(map, extbuf, u, o), external_inputs.jl:49-77:
  * extbuf[k] = u[i] for StateBufIdx(i), o[i] for OutBufIdx(i).  extbuf: lastidx_extbuf

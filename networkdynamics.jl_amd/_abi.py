@@ -101,6 +101,7 @@ PROTOTYPES = {
     "nd_rk4_noalloc": (C.c_int, [P, P, P, D, D, I64, I64, I64, P]),
     "nd_tsit5": (C.c_int, [P, P, P, D, D, D, D, D, I64, P, I64, I64, P]),
     "nd_rhs_jvp": (C.c_int, [P, P, P, P, P, P, D, I64, I64, I64, P]),
+    "nd_rhs_jvp_multi": (C.c_int, [P, P, P, P, P, C.c_int32, P, D, I64, I64, I64, P]),
     "nd_plan_set_perturbation": (C.c_int, [P, I64, I64, P, I64, P, I64, P, I64, P]),
     "nd_rhs_perturbed_jvp": (C.c_int, [P, P, P, P, P, P, P, D, P, P, P, P, P, P, P]),
     "nd_malloc": (C.c_int, [C.c_size_t, C.POINTER(P)]),

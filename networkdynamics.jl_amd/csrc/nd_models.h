@@ -47,6 +47,62 @@ template <typename S> __device__ __forceinline__ S zero_of();
 template <> __device__ __forceinline__ double zero_of<double>() { return 0.0; }
 template <> __device__ __forceinline__ Dual zero_of<Dual>() { return {0.0, 0.0}; }
 
+// K tangent directions at once, the chunk of a ForwardDiff Dual{T,Float64,K}: value v
+// and partials d[0..K).  Every operation applies the single-tangent rule above to each
+// partial in the same operation order, so partial j equals the Dual along direction j
+// bit for bit.
+template <int K>
+struct DualK {
+    double v;
+    double d[K];
+};
+#define ND_DUALK_OP(ret_v, part)                                    \
+    DualK<K> r;                                                     \
+    r.v = (ret_v);                                                  \
+    _Pragma("unroll") for (int j = 0; j < K; ++j) r.d[j] = (part);  \
+    return r;
+template <int K> __device__ __forceinline__ DualK<K> operator+(DualK<K> a, DualK<K> b) { ND_DUALK_OP(a.v + b.v, a.d[j] + b.d[j]) }
+template <int K> __device__ __forceinline__ DualK<K> operator+(double c, DualK<K> a) { ND_DUALK_OP(c + a.v, a.d[j]) }
+template <int K> __device__ __forceinline__ DualK<K> operator+(DualK<K> a, double c) { ND_DUALK_OP(a.v + c, a.d[j]) }
+template <int K> __device__ __forceinline__ DualK<K> operator-(DualK<K> a, DualK<K> b) { ND_DUALK_OP(a.v - b.v, a.d[j] - b.d[j]) }
+template <int K> __device__ __forceinline__ DualK<K> operator-(DualK<K> a) { ND_DUALK_OP(-a.v, -a.d[j]) }
+template <int K> __device__ __forceinline__ DualK<K> operator-(DualK<K> a, double c) { ND_DUALK_OP(a.v - c, a.d[j]) }
+template <int K> __device__ __forceinline__ DualK<K> operator-(double c, DualK<K> a) { ND_DUALK_OP(c - a.v, -a.d[j]) }
+template <int K> __device__ __forceinline__ DualK<K> operator*(double c, DualK<K> a) { ND_DUALK_OP(c * a.v, c * a.d[j]) }
+template <int K> __device__ __forceinline__ DualK<K> operator*(DualK<K> a, double c) { ND_DUALK_OP(a.v * c, a.d[j] * c) }
+template <int K> __device__ __forceinline__ DualK<K> operator*(DualK<K> a, DualK<K> b) {
+    ND_DUALK_OP(a.v * b.v, a.d[j] * b.v + a.v * b.d[j])
+}
+template <int K> __device__ __forceinline__ DualK<K> operator/(DualK<K> a, double c) { ND_DUALK_OP(a.v / c, a.d[j] / c) }
+template <int K> __device__ __forceinline__ DualK<K> operator/(DualK<K> a, DualK<K> b) {
+    const double bb = b.v * b.v;
+    ND_DUALK_OP(a.v / b.v, (a.d[j] * b.v - a.v * b.d[j]) / bb)
+}
+template <int K> __device__ __forceinline__ DualK<K> operator/(double c, DualK<K> b) {
+    const double bb = b.v * b.v;
+    ND_DUALK_OP(c / b.v, -c * b.d[j] / bb)
+}
+template <int K> __device__ __forceinline__ DualK<K> msin(DualK<K> a) {
+    const double cs = nd_cos(a.v);
+    ND_DUALK_OP(nd_sin(a.v), cs * a.d[j])
+}
+template <int K> __device__ __forceinline__ DualK<K> sqrt(DualK<K> a) {
+    const double rt = ::sqrt(a.v);
+    ND_DUALK_OP(rt, a.d[j] / (2.0 * rt))
+}
+#undef ND_DUALK_OP
+template <int K> __device__ __forceinline__ DualK<K> dualk_const(double x) {
+    DualK<K> r;
+    r.v = x;
+#pragma unroll
+    for (int j = 0; j < K; ++j) r.d[j] = x == x ? 0.0 : x;
+    return r;
+}
+template <> __device__ __forceinline__ DualK<4> nan_of<DualK<4>>() { return dualk_const<4>(__builtin_nan("")); }
+template <> __device__ __forceinline__ DualK<8> nan_of<DualK<8>>() { return dualk_const<8>(__builtin_nan("")); }
+template <> __device__ __forceinline__ DualK<4> zero_of<DualK<4>>() { return dualk_const<4>(0.0); }
+template <> __device__ __forceinline__ DualK<8> zero_of<DualK<8>>() { return dualk_const<8>(0.0); }
+
 __host__ __device__ constexpr bool is_ode_model(int em) {
     return em == ND_EM_ODE_SIN_DIFFUSION || em == ND_EM_ODE_SIN_DIFFUSION_FAST || em == ND_EM_ODE_LINEAR ||
            em == ND_EM_ODE_LAG || em == ND_EM_EXT_DRIVEN;

@@ -642,6 +642,140 @@ __global__ void __launch_bounds__(256) k_edge_ode_jvp(EDesc e, const double* __r
     }
 }
 
+// ---- K tangents per launch (the chunked duals of ForwardDiff) -----------------------------
+// du = f(u) and, for k <= K tangent columns, jv[:, j] = J(u) v[:, j]: column j of v and
+// jv at + j*ldu (ldu = the state length), of the tangent outputs dvo at + j*ldo.  Columns
+// k..K-1 are carried as zeros and not written.  Each column equals the single-tangent
+// kernels' result along v[:, j] bit for bit (the same per-partial operations).
+template <int K>
+__device__ __forceinline__ DualK<K> dk_load(const double* __restrict__ x, const double* __restrict__ dx, int64_t ld,
+                                            int k, int64_t i) {
+    DualK<K> r;
+    r.v = x[i];
+#pragma unroll
+    for (int j = 0; j < K; ++j) r.d[j] = j < k ? dx[j * ld + i] : 0.0;
+    return r;
+}
+template <int K>
+__device__ __forceinline__ void dk_store(double* __restrict__ y, double* __restrict__ dy, int64_t ld, int k, int64_t i,
+                                         const DualK<K>& r) {
+    y[i] = r.v;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if (j < k) dy[j * ld + i] = r.d[j];
+}
+template <int K>
+__device__ __forceinline__ DualK<K> ode_side_k(int wrap, DualK<K> x, bool is_src) {
+    return (wrap == ND_WRAP_ANTISYMMETRIC && is_src) ? -x : x;
+}
+template <int K>
+__device__ __forceinline__ DualK<K> ext_read_k(const int32_t* __restrict__ xmap, int64_t q, const double* __restrict__ u,
+                                               const double* __restrict__ v, int64_t ldu, int k) {
+    const int32_t xi = xmap[q];
+    return xi >= 0 ? dk_load<K>(u, v, ldu, k, xi) : -dk_load<K>(u, v, ldu, k, ~xi);
+}
+
+struct JvpKArgs {
+    const double* u;
+    const double* v;              // k tangent columns of the state, stride ldu
+    const double* vo;
+    const double* dvo;            // k tangent columns of the vertex outputs, stride ldo
+    const double* p;
+    double* du;
+    double* jv;                   // k columns, stride ldu
+    int64_t ldu, ldo;
+    int32_t k;
+};
+
+template <int EM, int WRAP, int K>
+__device__ __forceinline__ DualK<K> slot_value_k(int2 s, DualK<K> own, const JvpKArgs& a) {
+    if constexpr (EM == ND_EM_NOOP) {
+        return nan_of<DualK<K>>();
+    } else if constexpr (is_ode_model(EM)) {
+        return ode_side_k<K>(WRAP, dk_load<K>(a.u, a.v, a.ldu, a.k, s.x), s.y & 1);
+    } else {
+        const DualK<K> nbr = dk_load<K>(a.vo, a.dvo, a.ldo, a.k, s.x);
+        const double* pe = a.p + (s.y >> 1);
+        if (s.y & 1) {
+            const DualK<K> y = g_dst<EM, DualK<K>>(own, nbr, pe);
+            return WRAP == ND_WRAP_SYMMETRIC ? y : -y;
+        }
+        return g_dst<EM, DualK<K>>(nbr, own, pe);
+    }
+}
+
+template <int K>
+__device__ __forceinline__ DualK<K> slot_value_k_rt(const EDesc& e, int2 s, DualK<K> own, const JvpKArgs& a) {
+    if (e.model == ND_EM_NOOP) return nan_of<DualK<K>>();
+    if (e.ode) return ode_side_k<K>(e.wrap, dk_load<K>(a.u, a.v, a.ldu, a.k, s.x), s.y & 1);
+    const DualK<K> nbr = dk_load<K>(a.vo, a.dvo, a.ldo, a.k, s.x);
+    const double* pe = a.p + (s.y >> 1);
+    if (s.y & 1) {
+        const DualK<K> y = g_dst_rt<DualK<K>>(e.model, own, nbr, pe);
+        return e.wrap == ND_WRAP_SYMMETRIC ? y : -y;
+    }
+    return g_dst_rt<DualK<K>>(e.model, nbr, own, pe);
+}
+
+// the K-tangent form of k_vertex_jvp: one thread per vertex
+template <int VM, int EM, int WRAP, bool MULTI, int K>
+__global__ void __launch_bounds__(256) k_vertex_jvpk(VDesc vd, const int32_t* __restrict__ rowptr,
+                                                     const int2* __restrict__ slots, const uint8_t* __restrict__ slot_eb,
+                                                     const EDesc* __restrict__ ed, JvpKArgs a) {
+    constexpr int DIMC = VModel<VM>::dim;
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= vd.count) return;
+    const int dim = DIMC > 0 ? DIMC : vd.dim;
+    const int64_t x0 = static_cast<int64_t>(vd.state0) + static_cast<int64_t>(q) * dim;
+    if constexpr (VM == ND_VM_NOOP) {
+        for (int j = 0; j < dim; ++j) dk_store<K>(a.du, a.jv, a.ldu, a.k, x0 + j, dualk_const<K>(0.0));
+    } else {
+        DualK<K> x[DIMC];
+#pragma unroll
+        for (int j = 0; j < DIMC; ++j) x[j] = dk_load<K>(a.u, a.v, a.ldu, a.k, x0 + j);
+        DualK<K> own = x[0];          // x[mask] by selects: a runtime index would put x in scratch
+#pragma unroll
+        for (int j = 1; j < DIMC; ++j)
+            if (vd.mask[0] == j) own = x[j];
+        DualK<K> acc = dualk_const<K>(0.0);
+        const int32_t beg = rowptr[vd.row0 + q], end = rowptr[vd.row0 + q + 1];
+        for (int32_t s = beg; s < end; ++s) {
+            if constexpr (MULTI) acc = acc + slot_value_k_rt<K>(ed[slot_eb[s]], slots[s], own, a);
+            else acc = acc + slot_value_k<EM, WRAP, K>(slots[s], own, a);
+        }
+        DualK<K> dx[DIMC];
+        vertex_f<VM, DualK<K>>(dx, x, acc, a.p + vd.param0 + static_cast<int64_t>(q) * vd.pdim, DIMC);
+#pragma unroll
+        for (int j = 0; j < DIMC; ++j) dk_store<K>(a.du, a.jv, a.ldu, a.k, x0 + j, dx[j]);
+    }
+}
+
+// the K-tangent form of k_edge_ode_jvp; eo/deo: a.vo/a.dvo (compact outputs) or u/v
+template <int EM, int K>
+__global__ void __launch_bounds__(256) k_edge_ode_jvpk(EDesc e, JvpKArgs a, int compact) {
+    constexpr int D = ode_dim(EM);
+    constexpr int NX = edge_ext_inputs(EM) > 0 ? edge_ext_inputs(EM) : 1;
+    const int32_t q = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= e.count) return;
+    const int2 en = e.ends[q];
+    const double* eo = compact ? a.vo : a.u;
+    const double* deo = compact ? a.dvo : a.v;
+    const int64_t ldo = compact ? a.ldo : a.ldu;
+    const DualK<K> vs = dk_load<K>(eo, deo, ldo, a.k, en.x), vdn = dk_load<K>(eo, deo, ldo, a.k, en.y);
+    const int64_t x0 = static_cast<int64_t>(e.state0) + static_cast<int64_t>(q) * D;
+    DualK<K> x[D], ext[NX], dx[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) x[j] = dk_load<K>(a.u, a.v, a.ldu, a.k, x0 + j);
+#pragma unroll
+    for (int j = 0; j < NX; ++j)
+        ext[j] = j < edge_ext_inputs(EM) ? ext_read_k<K>(e.xmap, e.ext0 + static_cast<int64_t>(q) * e.extdim + j, a.u, a.v,
+                                                         a.ldu, a.k)
+                                         : dualk_const<K>(0.0);
+    edge_ode_f<EM, DualK<K>>(dx, x, vs, vdn, ext, a.p + e.param0 + static_cast<int64_t>(q) * e.pdim);
+#pragma unroll
+    for (int j = 0; j < D; ++j) dk_store<K>(a.du, a.jv, a.ldu, a.k, x0 + j, dx[j]);
+}
+
 __global__ void k_fill(double* __restrict__ x, int64_t n, double v) {
     for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x)
@@ -1495,6 +1629,66 @@ EdgeOdeJvpKernel pick_edge_ode_jvp(int em) {
     }
 }
 
+using JvpKKernel = void (*)(VDesc, const int32_t*, const int2*, const uint8_t*, const EDesc*, JvpKArgs);
+
+template <int VM, int K>
+JvpKKernel pick_jvpk_edge(int em, int wrap, bool multi) {
+    if (multi) return k_vertex_jvpk<VM, ND_EM_NOOP, ND_WRAP_NONE, true, K>;
+    const bool sym = wrap == ND_WRAP_SYMMETRIC;
+    switch (em) {
+    case ND_EM_KURAMOTO:
+        return sym ? k_vertex_jvpk<VM, ND_EM_KURAMOTO, ND_WRAP_SYMMETRIC, false, K>
+                   : k_vertex_jvpk<VM, ND_EM_KURAMOTO, ND_WRAP_ANTISYMMETRIC, false, K>;
+    case ND_EM_STATIC_LINE:
+        return sym ? k_vertex_jvpk<VM, ND_EM_STATIC_LINE, ND_WRAP_SYMMETRIC, false, K>
+                   : k_vertex_jvpk<VM, ND_EM_STATIC_LINE, ND_WRAP_ANTISYMMETRIC, false, K>;
+    case ND_EM_DIFFUSION:
+        return sym ? k_vertex_jvpk<VM, ND_EM_DIFFUSION, ND_WRAP_SYMMETRIC, false, K>
+                   : k_vertex_jvpk<VM, ND_EM_DIFFUSION, ND_WRAP_ANTISYMMETRIC, false, K>;
+    case ND_EM_DIFFUSION_WEIGHTED:
+        return sym ? k_vertex_jvpk<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_SYMMETRIC, false, K>
+                   : k_vertex_jvpk<VM, ND_EM_DIFFUSION_WEIGHTED, ND_WRAP_ANTISYMMETRIC, false, K>;
+    case ND_EM_DIFFUSION_FIDUCIAL:
+        return k_vertex_jvpk<VM, ND_EM_DIFFUSION_FIDUCIAL, ND_WRAP_ANTISYMMETRIC, false, K>;
+    case ND_EM_ODE_SIN_DIFFUSION:
+    case ND_EM_ODE_SIN_DIFFUSION_FAST:
+    case ND_EM_ODE_LINEAR:
+    case ND_EM_ODE_LAG:
+    case ND_EM_EXT_DRIVEN:
+        return wrap == ND_WRAP_ANTISYMMETRIC ? k_vertex_jvpk<VM, ND_EM_ODE_LINEAR, ND_WRAP_ANTISYMMETRIC, false, K>
+                                             : k_vertex_jvpk<VM, ND_EM_ODE_LINEAR, ND_WRAP_FIDUCIAL, false, K>;
+    default:
+        return k_vertex_jvpk<VM, ND_EM_NOOP, ND_WRAP_NONE, false, K>;
+    }
+}
+
+// the K-tangent vertex kernel of a batch, or null where the batch takes the generic or
+// vector-output kernels (those have single-tangent forms only)
+template <int K>
+JvpKKernel pick_jvpk_kernel(int vm, int em, int wrap, bool multi) {
+    switch (vm) {
+    case ND_VM_DIFFUSION: return pick_jvpk_edge<ND_VM_DIFFUSION, K>(em, wrap, multi);
+    case ND_VM_KURAMOTO: return pick_jvpk_edge<ND_VM_KURAMOTO, K>(em, wrap, multi);
+    case ND_VM_SWING: return pick_jvpk_edge<ND_VM_SWING, K>(em, wrap, multi);
+    case ND_VM_SWING_UNIT_DAMPING: return pick_jvpk_edge<ND_VM_SWING_UNIT_DAMPING, K>(em, wrap, multi);
+    case ND_VM_LOAD: return pick_jvpk_edge<ND_VM_LOAD, K>(em, wrap, multi);
+    case ND_VM_NOOP: return k_vertex_jvpk<ND_VM_NOOP, ND_EM_NOOP, ND_WRAP_NONE, false, K>;
+    default: return nullptr;
+    }
+}
+
+using EdgeOdeJvpKKernel = void (*)(EDesc, JvpKArgs, int);
+template <int K>
+EdgeOdeJvpKKernel pick_edge_ode_jvpk(int em) {
+    switch (em) {
+    case ND_EM_ODE_SIN_DIFFUSION: return k_edge_ode_jvpk<ND_EM_ODE_SIN_DIFFUSION, K>;
+    case ND_EM_ODE_SIN_DIFFUSION_FAST: return k_edge_ode_jvpk<ND_EM_ODE_SIN_DIFFUSION_FAST, K>;
+    case ND_EM_ODE_LAG: return k_edge_ode_jvpk<ND_EM_ODE_LAG, K>;
+    case ND_EM_EXT_DRIVEN: return k_edge_ode_jvpk<ND_EM_EXT_DRIVEN, K>;
+    default: return k_edge_ode_jvpk<ND_EM_ODE_LINEAR, K>;
+    }
+}
+
 using EdgeOdeKernel = void (*)(EDesc, const double*, const double*, const double*, double*, StageArgs);
 template <bool STAGE>
 EdgeOdeKernel pick_edge_ode(int em) {
@@ -1573,6 +1767,8 @@ struct nd_plan {
     std::vector<VertexKernel> vkern, vkern_stage;
     GroupKernel gkern = nullptr, gkern_stage = nullptr;   // several fused-set batches per launch
     std::vector<JvpKernel> vkern_jvp;
+    std::vector<JvpKKernel> vkern_jvpk4, vkern_jvpk8;   // null: no K-tangent form for the batch
+    double* d_dvoutk = nullptr;                          // 8 tangent columns of the vertex outputs
     double* d_dvout = nullptr;     // compact tangent outputs (Jacobian-vector products)
     double* rk_work = nullptr;
     double* d_vout = nullptr;      // compact vertex outputs (vdepth 1 networks with functional edges)
@@ -2229,6 +2425,11 @@ int nd_plan_create(const nd_network* h, nd_plan** out) {
         pl->vdesc.push_back(vd);
         pl->vkern_jvp.push_back(vd.depth > 1 ? k_vertex_vec_jvp : vd.generic ? pick_gen_jvp(k.model)
                                            : pick_jvp_kernel(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi));
+        const bool kform = vd.depth <= 1 && !vd.generic;
+        pl->vkern_jvpk4.push_back(kform ? pick_jvpk_kernel<4>(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi)
+                                        : nullptr);
+        pl->vkern_jvpk8.push_back(kform ? pick_jvpk_kernel<8>(k.model, n.eb.empty() ? ND_EM_NOOP : em0, wrap0, pl->multi)
+                                        : nullptr);
     }
     // loopback: each injector's input is its hub's output (gen_loopback_map, post_utils.jl:205-217)
     pl->computed_out = computed_out;
@@ -2403,6 +2604,72 @@ int nd_rhs_jvp(nd_plan* pl, double* du, double* jv, const double* u, const doubl
                            pl->computed_out ? vo : u, pl->computed_out ? dvo : v, p, du, jv);
     }
     return hip_check(hipGetLastError(), "nd_rhs_jvp launch");
+}
+
+int nd_rhs_jvp_multi(nd_plan* pl, double* du, double* jv, const double* u, const double* v, int32_t k,
+                     const double* p, double t, int64_t len_du, int64_t len_u, int64_t len_p, void* stream) {
+    if (!pl) return nd::set_error(ND_ERR_ARGUMENT, "plan is NULL");
+    if (k < 1 || k > 8) return nd::set_error(ND_ERR_ARGUMENT, "k = %d tangents: need 1 <= k <= 8", k);
+    if (k == 1) return nd_rhs_jvp(pl, du, jv, u, v, p, t, len_du, len_u, len_p, stream);
+    if (!(len_du == len_u && len_u == pl->dim))   // coreloop.jl:2-4
+        return nd::set_error(ND_ERR_ARGUMENT, "du or u does not have expected size %lld", (long long)pl->dim);
+    if (pl->pdim > 0 && len_p != pl->pdim)        // coreloop.jl:5-7
+        return nd::set_error(ND_ERR_ARGUMENT, "p does not has expecte size %lld", (long long)pl->pdim);
+    if (pl->dim > 0 && (!du || !u || !jv || !v)) return nd::set_error(ND_ERR_ARGUMENT, "NULL state pointer");
+    if (pl->pdim > 0 && !p) return nd::set_error(ND_ERR_ARGUMENT, "NULL parameter pointer");
+    if (pl->ff_out) return nd::set_error(ND_ERR_UNSUPPORTED, "feed-forward vertex outputs: one tangent per nd_rhs_jvp");
+    for (size_t b = 0; b < pl->vdesc.size(); ++b)
+        if (pl->vdesc[b].count && !pl->vkern_jvpk8[b])
+            return nd::set_error(ND_ERR_UNSUPPORTED, "vertex batch %zu takes the generic kernels: one tangent per "
+                                 "nd_rhs_jvp", b);
+    int rc = check_device(pl);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    JvpKArgs a{};
+    a.u = u;
+    a.v = v;
+    a.vo = u;
+    a.dvo = v;
+    a.p = p;
+    a.du = du;
+    a.jv = jv;
+    a.ldu = pl->dim;
+    a.ldo = pl->dim;
+    a.k = k;
+    if (pl->use_vout) {
+        if (!pl->d_dvoutk) {
+            void* w = nullptr;
+            hipError_t e = hipMalloc(&w, 8 * pl->vout_len * sizeof(double));
+            if (e != hipSuccess) return nd::set_error(ND_ERR_ALLOC, "hipMalloc(tangent outputs): %s", hipGetErrorString(e));
+            pl->allocations.push_back(w);
+            pl->d_dvoutk = static_cast<double*>(w);
+        }
+        for (const VDesc& vd : pl->vdesc) {
+            if (!vd.count) continue;
+            const unsigned vb = blocks_for(vd.count, 256);
+            hipLaunchKernelGGL(k_vertex_out, dim3(vb), dim3(256), 0, s, vd, u, pl->d_vout, no_sync(vb), p, 0);
+            for (int j = 0; j < k; ++j)
+                hipLaunchKernelGGL(k_vertex_out, dim3(vb), dim3(256), 0, s, vd, v + static_cast<int64_t>(j) * pl->dim,
+                                   pl->d_dvoutk + static_cast<int64_t>(j) * pl->vout_len, no_sync(vb), p, 1);
+        }
+        a.vo = pl->d_vout;
+        a.dvo = pl->d_dvoutk;
+        a.ldo = pl->vout_len;
+    }
+    for (size_t b = 0; b < pl->vdesc.size(); ++b) {
+        VDesc vd = pl->vdesc[b];
+        if (vd.count == 0) continue;
+        vd.t = t;
+        JvpKKernel kern = k <= 4 ? pl->vkern_jvpk4[b] : pl->vkern_jvpk8[b];
+        hipLaunchKernelGGL(kern, dim3(blocks_for(vd.count, 256)), dim3(256), 0, s, vd, pl->d_rowptr, pl->d_slots,
+                           pl->d_slot_eb, pl->d_edesc, a);
+    }
+    for (const EDesc& e : pl->edesc) {
+        if (!e.ode || e.count == 0) continue;
+        EdgeOdeJvpKKernel kern = k <= 4 ? pick_edge_ode_jvpk<4>(e.model) : pick_edge_ode_jvpk<8>(e.model);
+        hipLaunchKernelGGL(kern, dim3(blocks_for(e.count, 256)), dim3(256), 0, s, e, a, pl->computed_out ? 1 : 0);
+    }
+    return hip_check(hipGetLastError(), "nd_rhs_jvp_multi launch");
 }
 
 // The single-instance ensemble of a network that fits one workgroup (created on first

@@ -112,7 +112,7 @@ class Ensemble:
         self.classes[int(cls)] = (int(threads), int(vper))
 
     def set_classes(self, classes):
-        """classes: up to four (threads, vertices per thread) pairs; missing ones repeat the last."""
+        """classes: up to six (threads, vertices per thread) pairs; missing ones repeat the last."""
         cl = [tuple(map(int, c)) for c in classes]
         if not 1 <= len(cl) <= NCLASS:
             raise A.ArgumentError(f"need 1..{NCLASS} classes")

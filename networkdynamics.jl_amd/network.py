@@ -434,6 +434,30 @@ class Network:
         stream = torch.cuda.current_stream(du.device).cuda_stream
         A.check(A.lib().nd_rhs_jvp(self.plan, dptr, jptr, uptr, vptr, pptr, float(t), ndu, nu, np_, stream))
 
+    def jvp_multi(self, du, jv, u, v, p, t=0.0):
+        """du = f(u) and jv[j] = J(u) v[j] for the k = v.shape[0] <= 8 rows of v in one
+        launch (nd_rhs_jvp_multi): the chunked duals ForwardDiff carries through the
+        network's DiffCache buffers (construction.jl:205-208).  v, jv: (k, dim) float64
+        device tensors, C-contiguous.  Raises UnsupportedError for networks whose vertex
+        batches take the generic kernels (jvp serves them one tangent at a time)."""
+        import torch
+        dptr, ndu = _tensor_ptr(du, self.dim, "du", torch)
+        uptr, nu = _tensor_ptr(u, self.dim, "u", torch)
+        for name, x in (("v", v), ("jv", jv)):
+            if not isinstance(x, torch.Tensor) or x.dtype != torch.float64 or x.device.type != "cuda" or \
+                    x.dim() != 2 or x.shape[1] != self.dim or not x.is_contiguous():
+                raise A.ArgumentError(f"{name} must be a contiguous float64 device tensor of shape (k, {self.dim})")
+        k = v.shape[0]
+        if jv.shape[0] != k or not 1 <= k <= 8:
+            raise A.ArgumentError(f"v and jv need the same number of rows k, 1 <= k <= 8 (got {k}, {jv.shape[0]})")
+        if self.pdim > 0:
+            pptr, np_ = _tensor_ptr(p, self.pdim, "p", torch)
+        else:
+            pptr, np_ = (p.data_ptr(), p.numel()) if isinstance(p, torch.Tensor) else (None, 0)
+        stream = torch.cuda.current_stream(du.device).cuda_stream
+        A.check(A.lib().nd_rhs_jvp_multi(self.plan, dptr, jv.data_ptr(), uptr, v.data_ptr(), int(k), pptr, float(t),
+                                         ndu, nu, np_, stream))
+
     def jacobian(self, u, p, t=0.0, colored=True):
         """Dense Jacobian d(du)/du at u (dim x dim, float64 device tensor): the matrix
         ForwardDiff.jacobian builds in linearize_network (linear_analysis.jl:84).  colored:

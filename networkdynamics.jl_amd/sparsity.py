@@ -154,9 +154,11 @@ def color_columns(P):
     return color, ncolor
 
 
-def sparse_jacobian(nw, u, p, t=0.0, pattern=None, colors=None):
-    """d(du)/du at (u, p, t) as a scipy CSR matrix (host), from one device JVP per color.
-    pattern / colors: from jac_prototype / color_columns (computed when None)."""
+def sparse_jacobian(nw, u, p, t=0.0, pattern=None, colors=None, chunk=8):
+    """d(du)/du at (u, p, t) as a scipy CSR matrix (host), from device JVPs of the color
+    seeds: `chunk` (<= 8) colors per launch through nw.jvp_multi where the network's
+    kernels have the K-tangent form, one color per nw.jvp otherwise.  pattern / colors:
+    from jac_prototype / color_columns (computed when None)."""
     import torch
     from scipy.sparse import csr_matrix
     P = jac_prototype(nw) if pattern is None else pattern
@@ -166,16 +168,25 @@ def sparse_jacobian(nw, u, p, t=0.0, pattern=None, colors=None):
         ncolor = int(colors.max()) + 1 if len(colors) else 0
     Pc = P.tocoo()
     rows, cols = Pc.row, Pc.col
-    vals = np.zeros(len(rows))
     n = nw.dim
-    seed = torch.zeros(n, dtype=torch.float64, device=u.device)
-    du = torch.empty(n, dtype=torch.float64, device=u.device)
-    jv = torch.empty(n, dtype=torch.float64, device=u.device)
-    col_color = colors[cols]
-    colors_d = torch.from_numpy(colors).to(u.device)
-    for c in range(ncolor):
-        seed.copy_((colors_d == c).to(torch.float64))
-        nw.jvp(du, jv, u, seed, p, t)
-        sel = col_color == c
-        vals[sel] = jv.cpu().numpy()[rows[sel]]
+    dev = u.device
+    du = torch.empty(n, dtype=torch.float64, device=dev)
+    jall = torch.empty((max(ncolor, 1), n), dtype=torch.float64, device=dev)
+    colors_d = torch.from_numpy(np.asarray(colors, np.int64)).to(dev)
+    multi = chunk > 1 and hasattr(nw, "jvp_multi")
+    c0 = 0
+    while c0 < ncolor:
+        k = min(chunk, ncolor - c0) if multi else 1
+        ids = torch.arange(c0, c0 + k, device=dev)
+        seeds = (colors_d[None, :] == ids[:, None]).to(torch.float64)
+        if multi:
+            try:
+                nw.jvp_multi(du, jall[c0:c0 + k], u, seeds, p, t)
+            except A.UnsupportedError:
+                multi = False
+                continue
+        else:
+            nw.jvp(du, jall[c0], u, seeds[0], p, t)
+        c0 += k
+    vals = jall.cpu().numpy()[colors[cols], rows] if len(rows) else np.zeros(0)
     return csr_matrix((vals, (rows, cols)), shape=(n, n))

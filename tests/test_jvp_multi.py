@@ -1,0 +1,106 @@
+"""K tangents per launch (nd_rhs_jvp_multi): the chunked dual numbers the reference sizes
+its DiffCache buffers for (src/construction.jl:205-208).  Each tangent column must equal
+the single-tangent nd_rhs_jvp bit for bit (the same per-partial operations), which the
+closed-form and finite-difference tests of nd_rhs_jvp pin; the colored Jacobian built
+eight colors per launch must equal the one built a color at a time."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _networks(ND):
+    from test_oracle import _random_mixed
+    out = []
+    nw, p, u = ND.workloads.kuramoto_er(1000, 4000, 3.0, seed=1)
+    out.append(("kuramoto_er", nw, u, p))
+    nw, p, u = ND.workloads.swing_ba(20_000, 2, seed=22)
+    out.append(("swing_ba", nw, u, p))
+    for directed in (False, True):
+        nw, u, p = _random_mixed(ND, n=400, directed=directed, seed=31)
+        out.append((f"mixed directed={directed}", nw, u, p))
+    grid = ND.fixpoint.swing_load_grid(800, seed=5)
+    nw = grid.network(algebraic_loads=True)
+    rng = np.random.default_rng(5)
+    out.append(("swing_load_grid", nw, rng.normal(scale=0.3, size=nw.dim), grid.flat_p()))
+    return out
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 8])
+def test_columns_equal_single_tangent_bitwise(ND, dev, k):
+    import torch
+    d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    for name, nw, u, p in _networks(ND):
+        rng = np.random.default_rng(k)
+        V = rng.normal(size=(k, nw.dim))
+        mode = nw.tuning()["gather_mode"]
+        for gm in (2, 3):                                    # gathered and compact vertex outputs
+            nw.tune(gather_mode=gm)
+            ud, pd = d(u), d(p if len(p) else np.zeros(1))
+            du = torch.full((nw.dim,), float("nan"), dtype=torch.float64, device=dev)
+            jv = torch.full((k, nw.dim), float("nan"), dtype=torch.float64, device=dev)
+            nw.jvp_multi(du, jv, ud, d(V), pd, 0.0)
+            du1 = torch.empty_like(du)
+            col = torch.empty_like(du)
+            for j in range(k):
+                nw.jvp(du1, col, ud, d(V[j]), pd, 0.0)
+                assert torch.equal(col, jv[j]), (name, gm, j)
+            assert torch.equal(du1, du), (name, gm)
+        nw.tune(gather_mode=mode)
+
+
+def test_chunked_colored_jacobian_equals_one_color_per_launch(ND, dev):
+    import torch
+    from networkdynamics_jl_amd.sparsity import color_columns, jac_prototype, sparse_jacobian
+    grid = ND.fixpoint.swing_load_grid(2000, seed=53)
+    nw = grid.network(algebraic_loads=True)
+    u = torch.from_numpy(np.random.default_rng(53).normal(scale=0.2, size=nw.dim)).to(dev)
+    p = torch.from_numpy(grid.flat_p()).to(dev)
+    P = jac_prototype(nw)
+    colors, ncolor = color_columns(P)
+    assert ncolor > 8
+    J8 = sparse_jacobian(nw, u, p, 0.0, P, colors)
+    J1 = sparse_jacobian(nw, u, p, 0.0, P, colors, chunk=1)
+    assert (J8 != J1).nnz == 0
+    # against the dense Jacobian one unit vector at a time
+    Jd = nw.jacobian(u, p, colored=False).cpu().numpy()
+    assert np.array_equal(J8.toarray(), Jd)
+
+
+def test_generic_kernels_fall_back_to_one_tangent(ND, dev):
+    # a network of generic-kernel vertices has no K-tangent form: jvp_multi raises and
+    # the colored Jacobian falls back to one color per launch
+    import torch
+    L = ND.Lib
+    g = ND.watts_strogatz(200, 4, 0.5, seed=3)
+    nw = ND.Network(g, L.capacitor(), L.resistor())
+    rng = np.random.default_rng(3)
+    u = torch.from_numpy(rng.normal(size=nw.dim)).to(dev)
+    p = torch.from_numpy(rng.uniform(0.5, 2.0, nw.pdim)).to(dev)
+    du = torch.empty(nw.dim, dtype=torch.float64, device=dev)
+    jv = torch.empty((2, nw.dim), dtype=torch.float64, device=dev)
+    with pytest.raises(ND.UnsupportedError):
+        nw.jvp_multi(du, jv, u, torch.ones((2, nw.dim), dtype=torch.float64, device=dev), p)
+    from networkdynamics_jl_amd.sparsity import sparse_jacobian
+    J = sparse_jacobian(nw, u, p).toarray()
+    Jd = nw.jacobian(u, p, colored=False).cpu().numpy()
+    assert np.array_equal(J, Jd)
+
+
+def test_argument_errors(ND, dev):
+    import torch
+    nw, p, u0 = ND.workloads.kuramoto_er(100, 300, 3.0, seed=2)
+    u = torch.from_numpy(u0).to(dev)
+    pd = torch.from_numpy(p).to(dev)
+    du = torch.empty(nw.dim, dtype=torch.float64, device=dev)
+    for k in (0, 9):
+        V = torch.zeros((k, nw.dim), dtype=torch.float64, device=dev)
+        with pytest.raises(ND.ArgumentError):
+            nw.jvp_multi(du, V.clone(), u, V, pd)
+    V = torch.zeros((3, nw.dim), dtype=torch.float64, device=dev)
+    with pytest.raises(ND.ArgumentError):
+        nw.jvp_multi(du, torch.zeros((2, nw.dim), dtype=torch.float64, device=dev), u, V, pd)
+    with pytest.raises(ND.ArgumentError):
+        nw.jvp_multi(du, V.clone(), u, torch.zeros((3, nw.dim + 1), dtype=torch.float64, device=dev), pd)
+    with pytest.raises(ND.ArgumentError):
+        nw.jvp_multi(du, V.clone(), u, V.t().contiguous().t(), pd)       # not row-contiguous
