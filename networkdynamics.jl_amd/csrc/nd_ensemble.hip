@@ -87,13 +87,20 @@ struct Lds {
     uint16_t *ends, *eidx, *cs, *cv, *cptr;
 };
 
+// Vertex parameters stay in registers for the whole solve up to three vertices per
+// thread; at four they are read from memory (L1/L2-resident) in each vertex update, which
+// frees the registers whose spill would otherwise cost more
+template <int V>
+constexpr bool kParamsInMemory = V >= 4;
+
 // out = f(ys) for the thread's vertices (three block barriers): each edge's output is
 // evaluated once (the src side takes it negated, AntiSymmetric), then every chunk sums
-// its entries in list order
+// its entries in list order.  pv: the vertex parameters in registers, or pg: the
+// instance's vertex parameters in memory (kParamsInMemory)
 template <int VM, int EM, int V>
 __device__ __forceinline__ void rhs(const InstDesc& I, const Lds& L, const double (&ys)[V][VModel<VM>::dim],
                                     const double (&pv)[V][VPar<VM>::np > 0 ? VPar<VM>::np : 1],
-                                    double (&out)[V][VModel<VM>::dim]) {
+                                    const double* __restrict__ pg, double (&out)[V][VModel<VM>::dim]) {
     constexpr int D = VModel<VM>::dim;
     constexpr int NPE = EModel<EM>::np;
     const int T = static_cast<int>(blockDim.x), t = static_cast<int>(threadIdx.x);
@@ -124,7 +131,10 @@ __device__ __forceinline__ void rhs(const InstDesc& I, const Lds& L, const doubl
             double acc = 0.0;
             const int c1 = L.cptr[v + 1];
             for (int c = L.cptr[v]; c < c1; ++c) acc += L.partial[c];
-            vertex_f<VM>(out[k], ys[k], acc, pv[k], D);
+            if constexpr (kParamsInMemory<V>)
+                vertex_f<VM>(out[k], ys[k], acc, pg + static_cast<int64_t>(v) * VPar<VM>::np, D);
+            else
+                vertex_f<VM>(out[k], ys[k], acc, pv[k], D);
         }
     }
 }
@@ -337,6 +347,7 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
     for (int i = t; i < I.ne * NPE; i += T) L.epar[i] = pe_g[i];
 
     double y[V][D], pv[V][NPV];
+    const double* pg = a.p + I.poff;                 // the instance's vertex parameters
 #pragma unroll
     for (int k = 0; k < V; ++k) {
         const int v = t + k * T;
@@ -344,7 +355,8 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
         for (int j = 0; j < D; ++j) y[k][j] = v < I.n ? a.u[I.uoff + static_cast<int64_t>(v) * D + j] : 0.0;
 #pragma unroll
         for (int j = 0; j < NPV; ++j)
-            pv[k][j] = (v < I.n && VPar<VM>::np > 0) ? a.p[I.poff + static_cast<int64_t>(v) * VPar<VM>::np + j] : 0.0;
+            pv[k][j] = (!kParamsInMemory<V> && v < I.n && VPar<VM>::np > 0)
+                           ? a.p[I.poff + static_cast<int64_t>(v) * VPar<VM>::np + j] : 0.0;
     }
     __syncthreads();
 
@@ -372,22 +384,22 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
         const double half = 0.5 * dt, sixth = dt / 6.0;
         int64_t done = nsteps;
         for (int64_t step = 0; step < nsteps; ++step) {
-            rhs<VM, EM, V>(I, L, y, pv, k);
+            rhs<VM, EM, V>(I, L, y, pv, pg, k);
 #pragma unroll
             for (int q = 0; q < V; ++q)
 #pragma unroll
                 for (int j = 0; j < D; ++j) { acc[q][j] = k[q][j]; ys[q][j] = y[q][j] + half * k[q][j]; }
-            rhs<VM, EM, V>(I, L, ys, pv, k);
+            rhs<VM, EM, V>(I, L, ys, pv, pg, k);
 #pragma unroll
             for (int q = 0; q < V; ++q)
 #pragma unroll
                 for (int j = 0; j < D; ++j) { acc[q][j] = acc[q][j] + 2.0 * k[q][j]; ys[q][j] = y[q][j] + half * k[q][j]; }
-            rhs<VM, EM, V>(I, L, ys, pv, k);
+            rhs<VM, EM, V>(I, L, ys, pv, pg, k);
 #pragma unroll
             for (int q = 0; q < V; ++q)
 #pragma unroll
                 for (int j = 0; j < D; ++j) { acc[q][j] = acc[q][j] + 2.0 * k[q][j]; ys[q][j] = y[q][j] + dt * k[q][j]; }
-            rhs<VM, EM, V>(I, L, ys, pv, k);
+            rhs<VM, EM, V>(I, L, ys, pv, pg, k);
 #pragma unroll
             for (int q = 0; q < V; ++q)
 #pragma unroll
@@ -400,7 +412,7 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
             }
             if (ee >= 0 && step + 1 == static_cast<int64_t>(llround((te - a.t0) / dt))) ntrip += trip_edge<VM, EM>(a, I, L, ee);
             if (a.ss) {
-                rhs<VM, EM, V>(I, L, y, pv, k);
+                rhs<VM, EM, V>(I, L, y, pv, pg, k);
                 ++nf;
                 if (at_steady_state<V, D>(a, I, L, y, k)) {
                     done = step + 1;
@@ -413,7 +425,7 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
     } else {
         double k1[V][D], k2[V][D], k3[V][D], k4[V][D], k5[V][D], k6[V][D], k7[V][D], ys[V][D];
         double d2[V][D], d3[V][D], d4[V][D];      // dense output (crossing location only)
-        rhs<VM, EM, V>(I, L, y, pv, k1);
+        rhs<VM, EM, V>(I, L, y, pv, pg, k1);
         ++nf;
         auto wnorm2 = [&](const double (&x)[V][D], const double (&ref)[V][D]) {
             double s = 0.0;
@@ -437,7 +449,7 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
             for (int q = 0; q < V; ++q)
 #pragma unroll
                 for (int j = 0; j < D; ++j) ys[q][j] = y[q][j] + h0 * k1[q][j];
-            rhs<VM, EM, V>(I, L, ys, pv, k2);
+            rhs<VM, EM, V>(I, L, ys, pv, pg, k2);
             ++nf;
 #pragma unroll
             for (int q = 0; q < V; ++q)
@@ -477,15 +489,15 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
     _Pragma("unroll") for (int q = 0; q < V; ++q)                             \
         _Pragma("unroll") for (int j = 0; j < D; ++j) ys[q][j] = y[q][j] + h * (expr);
             ND_STAGE(c_a21 * k1[q][j]);
-            rhs<VM, EM, V>(I, L, ys, pv, k2);
+            rhs<VM, EM, V>(I, L, ys, pv, pg, k2);
             ND_STAGE(c_a31 * k1[q][j] + c_a32 * k2[q][j]);
-            rhs<VM, EM, V>(I, L, ys, pv, k3);
+            rhs<VM, EM, V>(I, L, ys, pv, pg, k3);
             ND_STAGE(c_a41 * k1[q][j] + c_a42 * k2[q][j] + c_a43 * k3[q][j]);
-            rhs<VM, EM, V>(I, L, ys, pv, k4);
+            rhs<VM, EM, V>(I, L, ys, pv, pg, k4);
             ND_STAGE(c_a51 * k1[q][j] + c_a52 * k2[q][j] + c_a53 * k3[q][j] + c_a54 * k4[q][j]);
-            rhs<VM, EM, V>(I, L, ys, pv, k5);
+            rhs<VM, EM, V>(I, L, ys, pv, pg, k5);
             ND_STAGE(c_a61 * k1[q][j] + c_a62 * k2[q][j] + c_a63 * k3[q][j] + c_a64 * k4[q][j] + c_a65 * k5[q][j]);
-            rhs<VM, EM, V>(I, L, ys, pv, k6);
+            rhs<VM, EM, V>(I, L, ys, pv, pg, k6);
             ND_STAGE(c_a71 * k1[q][j] + c_a72 * k2[q][j] + c_a73 * k3[q][j] + c_a74 * k4[q][j] + c_a75 * k5[q][j] +
                      c_a76 * k6[q][j]);
 #undef ND_STAGE
@@ -512,7 +524,7 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
                 for (int j = 0; j < D; ++j)
                     k2[q][j] = c_bt1 * k1[q][j] + c_bt2 * k2[q][j] + c_bt3 * k3[q][j] + c_bt4 * k4[q][j] +
                                c_bt5 * k5[q][j] + c_bt6 * k6[q][j];
-            rhs<VM, EM, V>(I, L, ys, pv, k7);
+            rhs<VM, EM, V>(I, L, ys, pv, pg, k7);
             nf += 6;
             if constexpr (kLocate) {
 #pragma unroll
@@ -663,7 +675,7 @@ __global__ void __launch_bounds__(LB) k_ensemble(SolveArgs a) {
                 // after a trip or a move to a crossing the first-same-as-last derivative
                 // (and the condition signs) are stale: re-evaluate them
                 if (changed > 0) {
-                    rhs<VM, EM, V>(I, L, y, pv, k1);
+                    rhs<VM, EM, V>(I, L, y, pv, pg, k1);
                     ++nf;
                     if constexpr (kLocate)
                         if (lim) line_signs<VM, EM, V>(I, L, lim, y, gsign);

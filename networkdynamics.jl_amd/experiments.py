@@ -263,15 +263,18 @@ def c5_grids(count, seed, nmin=500, nmax=5000, workers=16):
         return pool.map(_grid_job, jobs, chunksize=32)
 
 
-def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=50, lr=2e-3, seed=7, warm_tol=1e-6):
+def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=50, lr=2e-3, seed=7, warm_tol=1e-6,
+                 workers=16):
     """Train on fixpoints of ntrain grids, evaluate Newton from flat / DC / physics / learned
-    starts on neval held-out grids (grids without a flat-start fixpoint are dropped)."""
+    starts on neval held-out grids (grids without a flat-start fixpoint are dropped).
+    workers: processes generating the grids (1: in this process, e.g. under a profiler
+    that would initialise the GPU in each forked worker)."""
     import torch
     from . import warmstart as W
     out = {}
     t0 = time.perf_counter()
-    train = c5_grids(ntrain, seed)
-    evalg = c5_grids(neval, seed + 1)
+    train = c5_grids(ntrain, seed, workers=workers)
+    evalg = c5_grids(neval, seed + 1, workers=workers)
     out["generate_s"] = time.perf_counter() - t0
     _log(f"generated {len(train)} + {len(evalg)} grids in {out['generate_s']:.1f} s")
     th, st, tsol = W.solve_grids(train, dev)
