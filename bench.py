@@ -8,9 +8,11 @@ Prints one JSON line: RHS evals/s, the roofline of one RHS (algorithmic bytes
 56 N + 16 E over the HIP-event-timed average duration of the RHS's launches,
 against 8 TB/s), the CPU baseline (OpenMP restatement of the reference's
 ThreadedExecution + ThreadedAggregator path, timed on this host), the achievable
-bandwidth of a STREAM-like device copy, and `extended.c4`: the ensemble's
+bandwidth of a STREAM-like device copy, `extended.c4`: the ensemble's
 trajectories/s on the basin-stability workload (4096 uneven swing networks, Tsit5
-to t = 50) against the per-instance paths.
+to t = 50) against the per-instance paths, and `extended.c5`: Newton iterations and
+device seconds of the learned warm start against the flat start on held-out grids,
+with the default inexact and with exact CG linear solves.
 """
 import argparse
 import json
@@ -118,6 +120,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="leave out the extended.c4 ensemble measurement")
+    ap.add_argument("--no-c5", action="store_true", help="leave out the extended.c5 warm-start measurement")
     ap.add_argument("--c4-baseline-seconds", type=float, default=10.0,
                     help="time budget of each per-instance C4 baseline sample")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "extras"],
@@ -212,6 +215,29 @@ def main():
               "ratio_vs_host_stepped": r["speedup"],
               "steps_mean": r["ensemble_steps_mean"], "retcodes": r["ensemble_retcodes"],
               "returned_to_sync": r["returned_to_sync"], "target_ratio": 20.0}
+    c5 = None
+    if not args.no_c5:
+        # in a child process: its grid generation forks a process pool, which needs a
+        # process that has not initialised the GPU yet; the child's progress lines pass through
+        import subprocess
+        torch.cuda.empty_cache()
+        proc = subprocess.run([sys.executable, os.path.abspath(__file__), "--config", "c5", "--no-cpu"],
+                              stdout=subprocess.PIPE, text=True, timeout=1500)
+        if proc.returncode != 0:
+            raise RuntimeError(f"extended.c5 child exited with status {proc.returncode}")
+        r = json.loads(proc.stdout.strip().splitlines()[-1])
+        c5 = {"workload": f"{r['eval_grids']} held-out RGG swing/load grids N~U{{500..5000}}, Newton to residual 1e-10; "
+                          f"network trained on {r['train_grids']} grids, {r['epochs']} epochs",
+              "unit": "Newton iterations (mean) and device seconds",
+              "flat_newton_mean": r["flat_newton_mean"], "learned_newton_mean": r["learned_newton_mean"],
+              "newton_ratio_inexact_cg": r["newton_ratio_learned"],
+              "newton_ratio_exact_cg": r["newton_ratio_learned_exact_cg"],
+              "flat_s": r["flat_s"], "learned_pipeline_s": r["pipeline_total_s"],
+              "time_ratio_inexact_cg": r["time_ratio_pipeline"],
+              "flat_exact_cg_s": r["flat_exact_cg_s"], "learned_exact_cg_s": r["learned_total_exact_cg_s"],
+              "time_ratio_exact_cg": r["time_ratio_learned_exact_cg"],
+              "pipeline_fell_back": r["pipeline_fell_back"], "pipeline_accepted": r["pipeline_accepted"],
+              "target_r2": r["target_r2"], "target_newton_ratio": 3.0}
     out = {
         "metric": "RHS evals/s + % HBM peak at N=1e6 swing",
         "value": value,
@@ -233,7 +259,7 @@ def main():
                      "algorithmic_bytes": algo_bytes, "kernel_us": per_rhs_s * 1e6,
                      "stream_copy_GBs": {"4GiB_moved": copy_hbm, "128MiB_moved": copy_small}},
         "cpu_baseline": cpu,
-        "extended": {"c4": c4},
+        "extended": {"c4": c4, "c5": c5},
     }
     print(json.dumps(out))
 

@@ -214,14 +214,52 @@ function (nw::Network{<:HIPExecution})(du::ROCVector{Dual{T,Float64,1}}, u::ROCV
     nothing
 end
 
-# Everything else the HIP path cannot run: host arrays, other element types, chunked
-# Duals.  The reference's own argument checks throw ArgumentError (src/coreloop.jl:2-7,
-# src/network_structure.jl:146-151); so does this, before any device work, instead of
-# falling through to the generic coreloop.
+# Chunked duals, N = 2..8 partials (ForwardDiff's chunk, the size the reference gives its
+# DiffCache buffers, src/construction.jl:205-208): the value and the N partial columns go
+# to nd_rhs_jvp_multi in one launch.  Networks on the library's generic kernels have no
+# K-partial form there (ND_ERR_UNSUPPORTED): they take one nd_rhs_jvp per partial.
+function (nw::Network{<:HIPExecution})(du::ROCVector{Dual{T,Float64,N}}, u::ROCVector{Dual{T,Float64,N}},
+                                       p::ROCVector{Float64}, t) where {T,N}
+    2 <= N <= 8 || throw(ArgumentError("HIPExecution carries at most 8 partials per dual (got $(N))"))
+    a = nw.layer.aggregator::SegmentedAggregator
+    n = length(u)
+    length(du) == n || throw(ArgumentError("du or u does not have expected size $(n)"))
+    um = reshape(reinterpret(Float64, u), N + 1, n)     # per state: value, then the partials
+    uval = um[1, :]
+    upart = permutedims(um[2:end, :])                  # n x N: partial j contiguous, column j
+    dval, dpart = similar(uval), similar(upart)
+    rc = ccall((:nd_rhs_jvp_multi, libnd), Cint,
+               (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int32, Ptr{Float64}, Float64,
+                Int64, Int64, Int64, Ptr{Cvoid}),
+               a.plan, pointer(dval), pointer(dpart), pointer(uval), pointer(upart), Int32(N), pointer(p), Float64(t),
+               n, n, length(p), AMDGPU.stream().stream)
+    if rc == 2                                          # ND_ERR_UNSUPPORTED
+        for j in 1:N
+            off = (j - 1) * n + 1
+            _check(ccall((:nd_rhs_jvp, libnd), Cint,
+                         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64,
+                          Int64, Int64, Int64, Ptr{Cvoid}),
+                         a.plan, pointer(dval), pointer(dpart, off), pointer(uval), pointer(upart, off), pointer(p),
+                         Float64(t), n, n, length(p), AMDGPU.stream().stream))
+        end
+    else
+        _check(rc)
+    end
+    dm = reshape(reinterpret(Float64, du), N + 1, n)
+    dm[1, :] .= dval
+    dm[2:end, :] .= permutedims(dpart)
+    AMDGPU.synchronize()
+    nothing
+end
+
+# Everything else the HIP path cannot run: host arrays, other element types, duals with
+# more than 8 partials.  The reference's own argument checks throw ArgumentError
+# (src/coreloop.jl:2-7, src/network_structure.jl:146-151); so does this, before any
+# device work, instead of falling through to the generic coreloop.
 function (nw::Network{<:HIPExecution})(du, u, p, t; kwargs...)
     throw(ArgumentError("HIPExecution runs nw(du, u, p, t) with du, u, p as ROCVector{Float64} on the HIP " *
-                        "device (du and u may be ROCVectors of ForwardDiff.Dual{T,Float64,1} for Jacobian-vector " *
-                        "products); got $(typeof(du)), $(typeof(u)), $(typeof(p))"))
+                        "device (du and u may be ROCVectors of ForwardDiff.Dual{T,Float64,N}, N <= 8, for " *
+                        "Jacobian-vector products); got $(typeof(du)), $(typeof(u)), $(typeof(p))"))
 end
 
 # Solves of ODEProblem(nw, u0, tspan, p) with RK4 / Tsit5 (src/post_utils.jl:43-82) in the
