@@ -104,3 +104,48 @@ def test_argument_errors(ND, dev):
         nw.jvp_multi(du, V.clone(), u, torch.zeros((3, nw.dim + 1), dtype=torch.float64, device=dev), pd)
     with pytest.raises(ND.ArgumentError):
         nw.jvp_multi(du, V.clone(), u, V.t().contiguous().t(), pd)       # not row-contiguous
+
+
+def test_ad_test_network_jacobians(ND, dev):
+    """test/AD_test.jl: the complete_graph(4) network of mixed vertex and edge batches
+    (ODE, Fiducial and AntiSymmetric edges) at NWParameter's defaults with
+    p0.e[2:3, :K] .= 1.0 and a random state; its Jacobian in x (identity seeds, eight
+    columns per nd_rhs_jvp_multi launch) and in p (one parameter tangent per launch of
+    the perturbed loop) against Richardson finite differences of the oracle, as the
+    reference compares ForwardDiff with FiniteDiff."""
+    import torch
+    from conftest import fd_jvp, oracle_for
+    from test_oracle import _complete4
+    nw = _complete4(ND)
+    on = oracle_for(nw)
+    p0 = nw.default_p()
+    assert np.isnan(p0).sum() == 2          # the two Kuramoto edges' K have no default
+    p0[np.isnan(p0)] = 1.0
+    x0 = np.random.default_rng(4).random(nw.dim)
+    d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    n, m = nw.dim, nw.pdim
+    du = torch.empty(n, dtype=torch.float64, device=dev)
+    Jx = np.empty((n, n))
+    for c0 in range(0, n, 8):
+        k = min(8, n - c0)
+        V = np.zeros((k, n))
+        V[np.arange(k), c0 + np.arange(k)] = 1.0
+        jv = torch.empty((k, n), dtype=torch.float64, device=dev)
+        nw.jvp_multi(du, jv, d(x0), d(V), d(p0), 0.0)
+        Jx[:, c0:c0 + k] = jv.cpu().numpy().T
+    Jp = np.empty((n, m))
+    col = torch.empty(n, dtype=torch.float64, device=dev)
+    for j in range(m):
+        dp = np.zeros(m)
+        dp[j] = 1.0
+        nw.perturbed_jvp(du, col, d(x0), None, d(p0), 0.0, np.zeros(0), None, {}, dp=d(dp))
+        Jp[:, j] = col.cpu().numpy()
+    Jx_fd = np.stack([fd_jvp(on, x0, e, p0) for e in np.eye(n)], axis=1)
+
+    def fd_p(j, h=1e-3):
+        dd = lambda hh: (on.rhs(x0, p0 + hh * np.eye(m)[j]) - on.rhs(x0, p0 - hh * np.eye(m)[j])) / (2 * hh)
+        return (4 * dd(h / 2) - dd(h)) / 3
+    Jp_fd = np.stack([fd_p(j) for j in range(m)], axis=1)
+    np.testing.assert_allclose(Jx, Jx_fd, rtol=0, atol=1e-9 * max(1.0, np.abs(Jx_fd).max()))
+    np.testing.assert_allclose(Jp, Jp_fd, rtol=0, atol=1e-9 * max(1.0, np.abs(Jp_fd).max()))
+    assert np.count_nonzero(Jx) > n and np.count_nonzero(Jp) > n
