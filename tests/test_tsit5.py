@@ -288,9 +288,10 @@ def test_ensemble_tsit5_steady_state_matches_oracle(ND, dev):
 def test_ensemble_c4_full_size(ND, dev):
     """The basin-stability workload at its full size: 4096 uneven BA swing networks from
     their fixpoints plus a one-node kick, Tsit5 to t = 50 at reltol 1e-8.  Every instance
-    reaches t1, the captured hipGraph's replay equals the eager solve bit for bit, and on 16
-    sampled instances the oracle's Tsit5 gives the same return-to-sync class, accepted
-    steps within 2 and states within 1e-6 scaled."""
+    reaches t1, the captured hipGraph's replay equals the eager solve bit for bit, instances
+    solved in a smaller ensemble of their own end in the same bits, and on 16 sampled
+    instances the oracle's Tsit5 gives the same return-to-sync class, accepted steps
+    within 2 and states within 1e-6 scaled."""
     import torch
     from networkdynamics_jl_amd import experiments as X
     graphs, us, ps, dropped, _, rng = X.c4_start(dev, ninst=4096, seed=1)
@@ -308,6 +309,17 @@ def test_ensemble_c4_full_size(ND, dev):
     assert np.array_equal(sg["naccept"], st["naccept"])
     del graph
     uf = u_d.cpu().numpy()
+    # instances are independent: 64 of them from the smallest and the largest size
+    # class, solved in an ensemble of their own, end in the same bits
+    nv = np.array([g.nv() for g in graphs])
+    sub = list(np.flatnonzero(nv <= 256)[:32]) + list(np.flatnonzero(nv > 1536)[:32])
+    ens_s = ND.ensemble.Ensemble([graphs[i] for i in sub], ND.Lib.kuramoto_second(), ND.Lib.kuramoto_edge())
+    u_s = torch.from_numpy(np.concatenate([us[i] for i in sub])).to(dev)
+    p_s = torch.from_numpy(np.concatenate([ps[i] for i in sub])).to(dev)
+    ens_s.solve(u_s, p_s, (0.0, 50.0), reltol=1e-8, abstol=1e-8)
+    us_f = u_s.cpu().numpy()
+    for k, i in enumerate(sub):
+        assert np.array_equal(us_f[ens_s.state_slice(k)], uf[ens.state_slice(i)]), i
     sync = np.array([X.returned_to_sync(uf[ens.state_slice(i)]) for i in range(len(graphs))])
     assert 0.5 < sync.mean() < 1.0              # both classes occur
     # sample both classes, small and large instances
