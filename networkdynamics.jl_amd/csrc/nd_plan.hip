@@ -290,6 +290,15 @@ __device__ __forceinline__ double segment_sum(int32_t beg, int32_t end, int32_t 
     return acc;
 }
 
+// Streamed reads of the low path (entry lists, parameter copy, own states and
+// parameters): nontemporal, so that they displace fewer neighbour outputs from L2
+constexpr bool kStreamNT = true;
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+    if constexpr (kStreamNT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
 // LDS of the fused vertex kernels (each kernel that uses them gets its own copy): the
 // low path's per-wave entry windows (values, owners) and own outputs, the hub blocks'
 // wave partials (blocks of at most 256 threads)
@@ -388,10 +397,10 @@ __device__ __forceinline__ void vertex_rhs_body(const VDesc& vd, int32_t b,
         constexpr int NPV = VPar<VM>::np > 0 ? VPar<VM>::np : 1;
         double xs[DC], pvs[NPV];
 #pragma unroll
-        for (int j = 0; j < DC; ++j) xs[j] = u[x0 + j];
+        for (int j = 0; j < DC; ++j) xs[j] = ld_stream(u + x0 + j);
 #pragma unroll
         for (int j = 0; j < NPV; ++j)
-            pvs[j] = VPar<VM>::np > 0 ? p[vd.param0 + static_cast<int64_t>(qv) * vd.pdim + j] : 0.0;
+            pvs[j] = VPar<VM>::np > 0 ? ld_stream(p + vd.param0 + static_cast<int64_t>(qv) * vd.pdim + j) : 0.0;
         const double own = (DC == 1 || vd.mask[0] == 0) ? xs[0] : xs[DC - 1];
         double* __restrict__ wval = rhs_val + wib * 4 * kWave;
         uint8_t* __restrict__ wowner = rhs_owner + wib * 4 * kWave;
@@ -413,7 +422,7 @@ __device__ __forceinline__ void vertex_rhs_body(const VDesc& vd, int32_t b,
             for (int j = 0; j < 4; ++j) {
                 const int32_t s2c = min(base + j * kWave + lane, wend - 1);
                 if constexpr (KC) {
-                    const uint32_t w = static_cast<uint32_t>(vd.lnbr[s2c]);
+                    const uint32_t w = static_cast<uint32_t>(ld_stream(vd.lnbr + s2c));
                     sl[j] = make_int2(static_cast<int32_t>(w & 0x7fffffffu), static_cast<int32_t>(w >> 31));
                 } else {
                     sl[j] = vd.lslots[s2c];
@@ -425,7 +434,7 @@ __device__ __forceinline__ void vertex_rhs_body(const VDesc& vd, int32_t b,
                     const int32_t s2c = min(base + j * kWave + lane, wend - 1);
 #pragma unroll
                     for (int c = 0; c < NP; ++c)
-                        kk[j][c] = KC ? vd.klow[static_cast<int64_t>(s2c) * NP + c] : p[(sl[j].y >> 1) + c];
+                        kk[j][c] = KC ? ld_stream(vd.klow + static_cast<int64_t>(s2c) * NP + c) : p[(sl[j].y >> 1) + c];
                 }
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
