@@ -356,7 +356,7 @@ def c5_warmstart(dev, ntrain=20000, neval=1000, epochs=30, batch_grids=50, lr=2e
         out[f"{name}_total_s"] = prep + stt
         out[f"{name}_total_exact_cg_s"] = prep + xstt
     # the pipeline as a user runs it: learned start (loose pre-solves, capped Newton),
-    # flat-start fallback for the grids it fails on, every result judged by the RHS
+    # flat-start fallback for the grids it fails on, every result checked by its RHS residual
     for rep in range(2):           # the first call also compiles nothing; keep the second
         t0 = time.perf_counter()
         pth, pst, fell, ptimes = W.learned_solve(gb, net)

@@ -175,7 +175,7 @@ def test_learned_start_falls_back_to_the_flat_start(ND, dev):
     """solve_with_fallback: grids whose start fails (here a NaN prediction, as a diverged
     network would give) are solved again from the flat start and then equal the flat
     solve bit for bit; the others keep their warm-started solution; every result passes
-    the network-RHS judge; a grid without a fixpoint (injections far beyond its lines'
+    the network-RHS residual check; a grid without a fixpoint (injections far beyond its lines'
     capacity) raises NetworkInitError."""
     import copy
     import torch
