@@ -205,3 +205,29 @@ def test_learned_start_falls_back_to_the_flat_start(ND, dev):
     gb2 = W.GridBatch([grids[0], over], dev)
     with pytest.raises(ND.NetworkInitError, match="grid indices \\[1\\]"):
         W.solve_with_fallback(gb2, torch.zeros(gb2.n, dtype=torch.float64, device=dev) + float("nan"))
+
+
+@pytest.mark.gpu
+def test_pipeline_at_c5_grid_sizes(ND, dev):
+    """The whole C5 pipeline (experiments.c5_warmstart, which bench.py runs at 20,000 /
+    1,000 grids) at its grid-size distribution N ~ U{500..5000}, on 120 training and 24
+    held-out grids (generated at 680-4943 vertices, 20 of them above 1500 and 5 above
+    4000; those without a flat-start fixpoint are dropped) and a 3-epoch network:
+    every start (flat, DC, physics, learned, and learned_solve's pipeline with its
+    flat fallback) must end at a fixpoint that the network RHS accepts (max |du| <=
+    1e-10), the physics-informed starts must need fewer Newton iterations than the flat
+    start, and the reported ratios must be the quotients of the reported means."""
+    from networkdynamics_jl_amd.experiments import c5_warmstart
+    out = c5_warmstart(dev, ntrain=120, neval=24, epochs=3, batch_grids=40, seed=7, workers=1)
+    n = out["eval_grids"]
+    assert n + out["eval_dropped"] == 24 and n >= 16
+    assert out["accepted_flat"] == n
+    for name in ("dc", "physics", "learned"):
+        assert out[f"{name}_accepted"] == n, name
+        assert out[f"{name}_retcodes"][0] == n, name
+    assert out["pipeline_accepted"] == n
+    assert out["loss_last"] < out["loss_first"]
+    assert out["dc_newton_mean"] < out["flat_newton_mean"]
+    assert out["physics_newton_mean"] < out["flat_newton_mean"]
+    assert out["learned_newton_mean"] < out["flat_newton_mean"]
+    assert out["newton_ratio_learned"] == pytest.approx(out["flat_newton_mean"] / out["learned_newton_mean"])
