@@ -561,6 +561,14 @@ int nd_mp_backward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const f
                    float* loss_part, void* stream);
 int nd_mp_adam(int64_t np, float* params, const float* grads, float* m, float* v, float lr, float b1, float b2,
                float eps, int64_t step, void* stream);
+/* The network's per-node inputs for the stacked nodes of a batch of grids (CSR rowptr /
+ * nbr of the symmetric adjacency, w = K of each entry, theta = a phase estimate such as
+ * the DC solution): with D_s = theta[nbr_s] - theta[i] and kdeg_i = sum_s w_s,
+ * s0_i = sum_s w_s (sin D_s - D_s), the nonlinear part of the line flows, and
+ * X[i] = {P_i, is_gen_i, log kdeg_i, deg_i / 6, s0_i, sum_s w_s s0[nbr_s] / kdeg_i,
+ * sum_s w_s D_s^2, max_s |D_s|} (n x 8, row-major).  Sums in CSR order. */
+int nd_ws_features(int64_t n, const int32_t* rowptr, const int32_t* nbr, const double* w, const double* theta,
+                   const double* P, const double* is_gen, double* s0, double* X, void* stream);
 
 /* ---- synthetic inputs ----------------------------------------------------------
  * Seeded graph generators for the workloads.  No counterpart in the reference's

@@ -152,8 +152,78 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int64_t M, int K, const float
             }
 }
 
+// C[M x N] = A[M x K] * op(B) (+ bias, relu) for the network's full-width products:
+// K = 64, N = 128 (a hidden layer's [W | U]) and K = 128, N = 64 (the backward's
+// [dYa | g] [W | U]^T), A dense with K columns.  op(B) is staged once per workgroup in
+// LDS; A goes straight from memory into the MFMA operand registers: lane (li, lk) of a
+// wave loads row li, columns K/4*lk .. K/4*lk + K/4 - 1 as 16-byte loads, and step s of
+// the 16x16x4 products pairs column K/4*lk + s with row K/4*lk + s of op(B), so every
+// k enters once.  Each wave takes 16-row tiles, grid-stride (the grid is sized to the
+// chip, not to M).  B's odd stride N + 1 spreads the four k rows of one step over the
+// LDS banks.
+template <int K, int N, bool TRANSB>
+__global__ void __launch_bounds__(256) k_gemm_reg(int64_t M, const float* __restrict__ A, const float* __restrict__ B,
+                                                  int ldb, float* __restrict__ C, int ldc, const float* __restrict__ bias,
+                                                  int relu) {
+    constexpr int SB = N + 1, CT = N / 16, KS = K / 4;
+    __shared__ float Bs[K * SB];
+    const int t = threadIdx.x, lane = t % kWave, w = t / kWave;
+    for (int i = t; i < K * N; i += 256) {
+        const int k = i / N, j = i % N;
+        Bs[k * SB + j] = TRANSB ? B[static_cast<int64_t>(j) * ldb + k] : B[static_cast<int64_t>(k) * ldb + j];
+    }
+    __syncthreads();
+    const int li = lane % 16, lk = lane / 16;
+    const int64_t ntiles = (M + 15) / 16;
+    for (int64_t tile = static_cast<int64_t>(blockIdx.x) * 4 + w; tile < ntiles; tile += static_cast<int64_t>(gridDim.x) * 4) {
+        const int64_t row = tile * 16 + li;
+        float a[KS];
+        if (row < M) {
+            const float4* ap = reinterpret_cast<const float4*>(A + row * K + KS * lk);
+#pragma unroll
+            for (int q = 0; q < KS / 4; ++q) {
+                const float4 v = ap[q];
+                a[4 * q] = v.x;
+                a[4 * q + 1] = v.y;
+                a[4 * q + 2] = v.z;
+                a[4 * q + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < KS; ++q) a[q] = 0.0f;
+        }
+        floatx4 acc[CT];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) acc[ct] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        // four steps at a time: the fence keeps the compiler from hoisting every B read of
+        // the tile into registers at once (which halves the waves a SIMD can hold)
+#pragma unroll
+        for (int s0 = 0; s0 < KS; s0 += 4) {
+#pragma unroll
+            for (int s = s0; s < s0 + 4; ++s)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct)
+                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], Bs[(KS * lk + s) * SB + ct * 16 + li], acc[ct], 0,
+                                                                   0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t ro = tile * 16 + 4 * lk + r;
+                const int col = ct * 16 + li;
+                if (ro < M) {
+                    float v = acc[ct][r] + (bias ? bias[col] : 0.0f);
+                    if (relu) v = fmaxf(v, 0.0f);
+                    C[ro * ldc + col] = v;
+                }
+            }
+    }
+}
+
 // partial[b] = A[rows_b]^T * B[rows_b] on MFMA: KA <= 64 output rows (one 16-row tile per
-// wave), NB <= 128 output columns, kRowsPerBlock rows of the product's inner dimension
+// wave), NB <= 128 output columns, part_rows(M) rows of the product's inner dimension
 // per workgroup, staged through LDS 64 rows at a time.
 constexpr int kTnChunk = 64;      // rows staged per LDS round
 template <int NBT>
@@ -200,17 +270,26 @@ __global__ void __launch_bounds__(256) k_gemm_tn_mfma(int64_t M, int KA, int NB,
 }
 
 // ---- dense: partial[b] = A[rows_b]^T * B[rows_b]  (KA x NB, KA*NB <= 8192) ---------------
-constexpr int kRowsPerBlock = 1024;
+// Rows of the inner dimension per workgroup of these two-stage sums (A^T B, column and
+// vector sums): enough workgroups to fill the chip, at most kMaxPartBlocks of them, at
+// least 64 rows each.  The partials are then added in a fixed order (k_reduce_partials),
+// so every sum is deterministic for a given M.
+constexpr int kMaxPartBlocks = 1024;
+inline int part_rows(int64_t M) {
+    const int64_t r = (M + kMaxPartBlocks - 1) / kMaxPartBlocks;
+    return static_cast<int>(std::max<int64_t>(64, (r + 63) / 64 * 64));
+}
 __global__ void __launch_bounds__(256) k_sgemm_tn_partial(int64_t M, int KA, int NB, const float* __restrict__ A, int lda,
-                                                          const float* __restrict__ B, int ldb, float* __restrict__ part) {
+                                                          const float* __restrict__ B, int ldb, float* __restrict__ part,
+                                                          int rows_per_block) {
     __shared__ float As[16][128];
     __shared__ float Bs[16][128];
     const int nout = KA * NB;
     float acc[32];
 #pragma unroll
     for (int q = 0; q < 32; ++q) acc[q] = 0.0f;
-    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kRowsPerBlock;
-    const int64_t r1 = std::min<int64_t>(r0 + kRowsPerBlock, M);
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+    const int64_t r1 = std::min<int64_t>(r0 + rows_per_block, M);
     for (int64_t rb = r0; rb < r1; rb += 16) {
         for (int i = threadIdx.x; i < 16 * 128; i += 256) {
             const int rr = i / 128, c = i % 128;
@@ -240,20 +319,36 @@ __global__ void __launch_bounds__(256) k_sgemm_tn_partial(int64_t M, int KA, int
 }
 
 // out[i] (+)= sum_b part[b][i], in block order
-__global__ void k_reduce_partials(int nb, int nout, const float* __restrict__ part, float* __restrict__ out, int add) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nout) return;
+// (64 outputs per workgroup of kRedWaves waves: wave w adds the partials of blocks
+// [nb*w/kRedWaves, nb*(w+1)/kRedWaves) in order, then the wave sums are added in wave order)
+constexpr int kRedWaves = 16;
+__global__ void __launch_bounds__(kRedWaves * 64) k_reduce_partials(int nb, int nout, const float* __restrict__ part,
+                                                                    float* __restrict__ out, int add) {
+    __shared__ float sh[kRedWaves][kWave];
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    const int i = blockIdx.x * kWave + lane;
+    const int b0 = nb * w / kRedWaves, b1 = nb * (w + 1) / kRedWaves;
     float s = 0.0f;
-    for (int b = 0; b < nb; ++b) s += part[static_cast<int64_t>(b) * nout + i];
-    out[i] = add ? out[i] + s : s;
+    if (i < nout) {
+#pragma unroll 4
+        for (int b = b0; b < b1; ++b) s += part[static_cast<int64_t>(b) * nout + i];
+    }
+    sh[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && i < nout) {
+        float t = sh[0][lane];
+        for (int q = 1; q < kRedWaves; ++q) t += sh[q][lane];
+        out[i] = add ? out[i] + t : t;
+    }
 }
 
 // column sums of G[M x 64] into per-block partials (bias gradients)
-__global__ void __launch_bounds__(256) k_colsum_partial(int64_t M, const float* __restrict__ G, int ldg, float* __restrict__ part) {
+__global__ void __launch_bounds__(256) k_colsum_partial(int64_t M, const float* __restrict__ G, int ldg, float* __restrict__ part,
+                                                        int rows_per_block) {
     __shared__ float sh[4][kH];
     const int c = threadIdx.x % kH, w = threadIdx.x / kH;
-    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kRowsPerBlock;
-    const int64_t r1 = std::min<int64_t>(r0 + kRowsPerBlock, M);
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+    const int64_t r1 = std::min<int64_t>(r0 + rows_per_block, M);
     float s = 0.0f;
     for (int64_t r = r0 + w; r < r1; r += 4) s += G[r * ldg + c];
     sh[w][c] = s;
@@ -262,10 +357,11 @@ __global__ void __launch_bounds__(256) k_colsum_partial(int64_t M, const float* 
 }
 
 // per-block sums of a vector (fixed tree inside the block)
-__global__ void __launch_bounds__(256) k_vsum_partial(int64_t n, const float* __restrict__ x, float* __restrict__ part) {
+__global__ void __launch_bounds__(256) k_vsum_partial(int64_t n, const float* __restrict__ x, float* __restrict__ part,
+                                                      int rows_per_block) {
     __shared__ float sh[256];
-    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kRowsPerBlock;
-    const int64_t r1 = std::min<int64_t>(r0 + kRowsPerBlock, n);
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+    const int64_t r1 = std::min<int64_t>(r0 + rows_per_block, n);
     float s = 0.0f;
     for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) s += x[r];
     sh[threadIdx.x] = s;
@@ -277,16 +373,40 @@ __global__ void __launch_bounds__(256) k_vsum_partial(int64_t n, const float* __
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// Workgroups are dispatched to the chip's 8 XCDs round robin (workgroup b to XCD b % 8).
+// xcd_block(b, nb) renumbers them so that each XCD works through one contiguous eighth
+// of the grid: rows of one grid, which read each other's rows, then share one L2.
+// A bijection of [0, nb).
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+    const int64_t x = b % 8, q = nb / 8, r = nb % 8;
+    return x * q + (x < r ? x : r) + b / 8;
+}
+
 // ---- sparse: out[i, :] = sum_s a[s] * Y[nbr[s], 0:64]  (+ Y[i, 64:128] + bias, relu) -------
 // one 64-lane wave per node, one lane per feature
 __global__ void __launch_bounds__(256) k_spmm(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ nbr,
                                               const float* __restrict__ a, const float* __restrict__ Y, int ldy,
                                               float* __restrict__ out, int ldo, const float* __restrict__ bias, int fused) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * 4 + threadIdx.x / kWave;
+    // the node is the same for the whole wave: its list offsets, neighbours and weights
+    // come through scalar loads, four entries' rows in flight at once, summed in order
+    const int64_t i = __builtin_amdgcn_readfirstlane(
+        static_cast<int32_t>(xcd_block(blockIdx.x, gridDim.x) * 4 + threadIdx.x / kWave));
     const int c = threadIdx.x % kWave;
     if (i >= n) return;
     float s = 0.0f;
-    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s += a[k] * Y[static_cast<int64_t>(nbr[k]) * ldy + c];
+    int32_t k = rp[i];
+    const int32_t k1 = rp[i + 1];
+    for (; k + 4 <= k1; k += 4) {
+        const int32_t j0 = nbr[k], j1 = nbr[k + 1], j2 = nbr[k + 2], j3 = nbr[k + 3];
+        const float a0 = a[k], a1 = a[k + 1], a2 = a[k + 2], a3 = a[k + 3];
+        const float y0 = Y[static_cast<int64_t>(j0) * ldy + c], y1 = Y[static_cast<int64_t>(j1) * ldy + c];
+        const float y2 = Y[static_cast<int64_t>(j2) * ldy + c], y3 = Y[static_cast<int64_t>(j3) * ldy + c];
+        s += a0 * y0;
+        s += a1 * y1;
+        s += a2 * y2;
+        s += a3 * y3;
+    }
+    for (; k < k1; ++k) s += a[k] * Y[static_cast<int64_t>(nbr[k]) * ldy + c];
     if (fused) {
         s += Y[i * ldy + kH + c] + bias[c];
         s = fmaxf(s, 0.0f);
@@ -349,6 +469,54 @@ __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict
 
 unsigned nblk(int64_t n, int b) { return static_cast<unsigned>((n + b - 1) / b); }
 
+// ---- per-node inputs of the network (nd_ws_features) ------------------------------------
+// One thread per node, entries in CSR order: D = theta[j] - theta[i]; s0 = sum w (sin D - D),
+// sum w D^2, max |D|, kdeg = sum w; the columns of X that need only the node's own entries.
+constexpr int kFeat = 8;
+__global__ void __launch_bounds__(256) k_ws_features(int64_t n, const int32_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ nbr, const double* __restrict__ w,
+                                                     const double* __restrict__ th, const double* __restrict__ P,
+                                                     const double* __restrict__ gen, double* __restrict__ s0,
+                                                     double* __restrict__ X) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double ti = th[i];
+    double s = 0.0, sq = 0.0, mx = 0.0, kd = 0.0;
+    const int32_t k1 = rp[i + 1];
+    for (int32_t k = rp[i]; k < k1; ++k) {
+        const double D = th[nbr[k]] - ti, wk = w[k];
+        s += wk * (sin(D) - D);
+        sq += wk * D * D;
+        mx = fmax(mx, fabs(D));
+        kd += wk;
+    }
+    s0[i] = s;
+    double* x = X + i * kFeat;
+    x[0] = P[i];
+    x[1] = gen[i];
+    x[2] = log(kd);
+    x[3] = static_cast<double>(k1 - rp[i]) / 6.0;
+    x[4] = s;
+    x[6] = sq;
+    x[7] = mx;
+}
+
+// the K-weighted neighbour mean of s0 (X column 5), after k_ws_features
+__global__ void __launch_bounds__(256) k_ws_features_nbr(int64_t n, const int32_t* __restrict__ rp,
+                                                         const int32_t* __restrict__ nbr, const double* __restrict__ w,
+                                                         const double* __restrict__ s0, double* __restrict__ X) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    double ns = 0.0, kd = 0.0;
+    const int32_t k1 = rp[i + 1];
+    for (int32_t k = rp[i]; k < k1; ++k) {
+        const double wk = w[k];
+        ns += wk * s0[nbr[k]];
+        kd += wk;
+    }
+    X[i * kFeat + 5] = ns / kd;
+}
+
 struct Offsets {
     int64_t win, bin, wl[kLayers], bl[kLayers], wout, bout, total;
 };
@@ -382,6 +550,16 @@ void gemm_mfma(hipStream_t s, int64_t M, int K, const float* A, int lda, const f
 // LDS-tiled FP32 kernel otherwise
 void gemm(hipStream_t s, bool transb, int64_t M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C,
           int ldc, const float* bias, int relu) {
+    const bool dense16 = lda == K && (reinterpret_cast<uintptr_t>(A) & 15) == 0;
+    if (dense16 && ((K == 64 && N == 128 && !transb) || (K == 128 && N == 64 && transb))) {
+        // 16-row tiles, four per workgroup; four workgroups per CU fit their LDS
+        const unsigned nb = static_cast<unsigned>(std::min<int64_t>((M + 63) / 64, 1024));
+        if (K == 64)
+            hipLaunchKernelGGL((k_gemm_reg<64, 128, false>), dim3(nb), dim3(256), 0, s, M, A, B, ldb, C, ldc, bias, relu);
+        else
+            hipLaunchKernelGGL((k_gemm_reg<128, 64, true>), dim3(nb), dim3(256), 0, s, M, A, B, ldb, C, ldc, bias, relu);
+        return;
+    }
     if (K <= 128 && (N == 64 || N == 128)) {
         if (N == 64) {
             if (transb) gemm_mfma<true, 64>(s, M, K, A, lda, B, ldb, C, ldc, bias, relu);
@@ -401,23 +579,26 @@ void gemm(hipStream_t s, bool transb, int64_t M, int N, int K, const float* A, i
 // MFMA for KA <= 64 and 16 <= NB <= 128; the FP32 VALU kernel for the thin cases.
 void gemm_tn(hipStream_t s, int64_t M, int KA, int NB, const float* A, int lda, const float* B, int ldb, float* part,
              float* out, int add) {
-    const unsigned nb = std::max(1u, nblk(M, kRowsPerBlock));
+    const int rows = part_rows(M);
+    const unsigned nb = std::max(1u, nblk(M, rows));
     if (KA <= 64 && NB >= 16 && NB <= 128) {
         const int nbt = (NB + 15) / 16;
         if (nbt <= 4)
-            hipLaunchKernelGGL(k_gemm_tn_mfma<4>, dim3(nb), dim3(256), 0, s, M, KA, NB, A, lda, B, ldb, part, kRowsPerBlock);
+            hipLaunchKernelGGL(k_gemm_tn_mfma<4>, dim3(nb), dim3(256), 0, s, M, KA, NB, A, lda, B, ldb, part, rows);
         else
-            hipLaunchKernelGGL(k_gemm_tn_mfma<8>, dim3(nb), dim3(256), 0, s, M, KA, NB, A, lda, B, ldb, part, kRowsPerBlock);
+            hipLaunchKernelGGL(k_gemm_tn_mfma<8>, dim3(nb), dim3(256), 0, s, M, KA, NB, A, lda, B, ldb, part, rows);
     } else {
-        hipLaunchKernelGGL(k_sgemm_tn_partial, dim3(nb), dim3(256), 0, s, M, KA, NB, A, lda, B, ldb, part);
+        hipLaunchKernelGGL(k_sgemm_tn_partial, dim3(nb), dim3(256), 0, s, M, KA, NB, A, lda, B, ldb, part, rows);
     }
-    hipLaunchKernelGGL(k_reduce_partials, dim3(nblk(KA * NB, 256)), dim3(256), 0, s, static_cast<int>(nb), KA * NB, part, out, add);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nblk(KA * NB, kWave)), dim3(kRedWaves * 64), 0, s, static_cast<int>(nb),
+                       KA * NB, part, out, add);
 }
 
 void colsum(hipStream_t s, int64_t M, const float* G, int ldg, float* part, float* out) {
-    const unsigned nb = std::max(1u, nblk(M, kRowsPerBlock));
-    hipLaunchKernelGGL(k_colsum_partial, dim3(nb), dim3(256), 0, s, M, G, ldg, part);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kH), 0, s, static_cast<int>(nb), kH, part, out, 0);
+    const int rows = part_rows(M);
+    const unsigned nb = std::max(1u, nblk(M, rows));
+    hipLaunchKernelGGL(k_colsum_partial, dim3(nb), dim3(256), 0, s, M, G, ldg, part, rows);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedWaves * 64), 0, s, static_cast<int>(nb), kH, part, out, 0);
 }
 
 }  // namespace
@@ -429,12 +610,13 @@ int64_t nd_mp_param_count(int32_t nfeat) { return offsets(nfeat).total; }
 int64_t nd_mp_workspace_floats(int64_t nnodes) {
     // activations H0..H4 (5*64) + Y (128) per node, backward: g (64), [dYa|g] (128), dH (64),
     // head gradient (1) + partial buffers
-    return nnodes * (5 * kH + 2 * kH + kH + 2 * kH + kH + 1) + (nnodes / kRowsPerBlock + 2) * (kH * 2 * kH) + nnodes / 4 + 8;
+    const int64_t nb = std::min<int64_t>(kMaxPartBlocks, (nnodes + 63) / 64);
+    return nnodes * (5 * kH + 2 * kH + kH + 2 * kH + kH + 1) + (nb + 2) * (kH * 2 * kH) + nnodes / 4 + 8;
 }
 
 int nd_mp_forward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const float* a_fwd, const float* X, int32_t nfeat,
                   const float* params, float* work, float* out, void* stream) {
-    if (n < 1 || !rowptr || !X || !params || !work || !out || nfeat < 1 || nfeat > 128)
+    if (n < 1 || n > INT32_MAX - 8 || !rowptr || !X || !params || !work || !out || nfeat < 1 || nfeat > 128)
         return nd::set_error(ND_ERR_ARGUMENT, "bad arguments");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const Offsets o = offsets(nfeat);
@@ -456,7 +638,8 @@ int nd_mp_forward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const fl
 int nd_mp_backward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const float* a_bwd, const float* X,
                    int32_t nfeat, const float* params, const float* target, float* work, float* grads,
                    float* loss_part, void* stream) {
-    if (n < 1 || !rowptr || !X || !params || !work || !grads || !target || !loss_part || nfeat < 1 || nfeat > 128)
+    if (n < 1 || n > INT32_MAX - 8 || !rowptr || !X || !params || !work || !grads || !target || !loss_part || nfeat < 1 ||
+        nfeat > 128)
         return nd::set_error(ND_ERR_ARGUMENT, "bad arguments");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const Offsets o = offsets(nfeat);
@@ -470,9 +653,11 @@ int nd_mp_backward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const f
                        nullptr, target, dout, DH, loss_part);
     gemm_tn(s, n, kH, 1, H + 4 * n * kH, kH, dout, 1, part, grads + o.wout, 0);
     {
-        const unsigned nb = std::max(1u, nblk(n, kRowsPerBlock));
-        hipLaunchKernelGGL(k_vsum_partial, dim3(nb), dim3(256), 0, s, n, dout, part);
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, static_cast<int>(nb), 1, part, grads + o.bout, 0);
+        const int rows = part_rows(n);
+        const unsigned nb = std::max(1u, nblk(n, rows));
+        hipLaunchKernelGGL(k_vsum_partial, dim3(nb), dim3(256), 0, s, n, dout, part, rows);
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedWaves * 64), 0, s, static_cast<int>(nb), 1, part,
+                           grads + o.bout, 0);
     }
     for (int l = kLayers - 1; l >= 0; --l) {
         // g = dH_{l+1} * relu'(H_{l+1}), into G and the right half of [dYa | g]
@@ -489,6 +674,16 @@ int nd_mp_backward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const f
     colsum(s, n, G, kH, part, grads + o.bin);
     gemm_tn(s, n, nfeat, kH, X, nfeat, G, kH, part, grads + o.win, 0);
     return herr(hipGetLastError(), "nd_mp_backward");
+}
+
+int nd_ws_features(int64_t n, const int32_t* rowptr, const int32_t* nbr, const double* w, const double* theta,
+                   const double* P, const double* is_gen, double* s0, double* X, void* stream) {
+    if (n < 1 || !rowptr || !nbr || !w || !theta || !P || !is_gen || !s0 || !X)
+        return nd::set_error(ND_ERR_ARGUMENT, "bad arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_ws_features, dim3(nblk(n, 256)), dim3(256), 0, s, n, rowptr, nbr, w, theta, P, is_gen, s0, X);
+    hipLaunchKernelGGL(k_ws_features_nbr, dim3(nblk(n, 256)), dim3(256), 0, s, n, rowptr, nbr, w, s0, X);
+    return herr(hipGetLastError(), "nd_ws_features");
 }
 
 int nd_mp_adam(int64_t np, float* params, const float* grads, float* m, float* v, float lr, float b1, float b2,

@@ -279,14 +279,16 @@ class GridBatch:
         return torch.segment_reduce(self.w * (torch.sin(D) - D), "sum", lengths=self.lengths)
 
     def features(self, theta_dc):
-        """(s0, X): the physics source and the per-vertex inputs of `features`, on the GPU."""
+        """(s0, X): the physics source and the per-vertex inputs of `features`, on the GPU
+        (nd_ws_features: two kernels, sums in CSR order)."""
         import torch
-        D = theta_dc[self.cols] - theta_dc[self.rows]
-        s0 = torch.segment_reduce(self.w * (torch.sin(D) - D), "sum", lengths=self.lengths)
-        ns0 = torch.segment_reduce(self.w * s0[self.cols], "sum", lengths=self.lengths) / self.kdeg
-        sq = torch.segment_reduce(self.w * D * D, "sum", lengths=self.lengths)
-        mx = torch.segment_reduce(D.abs(), "max", lengths=self.lengths)
-        X = torch.stack([self.P, self.is_gen, torch.log(self.kdeg), self.deg / 6.0, s0, ns0, sq, mx], dim=1)
+        th = theta_dc.contiguous()
+        s0 = torch.empty(self.n, dtype=torch.float64, device=self.dev)
+        X = torch.empty((self.n, 8), dtype=torch.float64, device=self.dev)
+        stream = torch.cuda.current_stream(self.dev).cuda_stream
+        A.check(A.lib().nd_ws_features(self.n, self.graph.rowptr.data_ptr(), self.graph.nbr.data_ptr(),
+                                       self.w.data_ptr(), th.data_ptr(), self.P.data_ptr(), self.is_gen.data_ptr(),
+                                       s0.data_ptr(), X.data_ptr(), stream))
         return s0, X
 
 
