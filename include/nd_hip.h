@@ -569,6 +569,11 @@ int nd_mp_adam(int64_t np, float* params, const float* grads, float* m, float* v
  * sum_s w_s D_s^2, max_s |D_s|} (n x 8, row-major).  Sums in CSR order. */
 int nd_ws_features(int64_t n, const int32_t* rowptr, const int32_t* nbr, const double* w, const double* theta,
                    const double* P, const double* is_gen, double* s0, double* X, void* stream);
+/* Per-node fixpoint residual of a phase vector on the same stacked CSR: r_i = |F_i| / scale_i
+ * with F_i = P_i + sum_s w_s sin(theta[nbr_s] - theta_i), the quantity nd_fixpoint_solve
+ * drives to its tolerance; its per-grid maximum is the residual of a start. */
+int nd_ws_residual(int64_t n, const int32_t* rowptr, const int32_t* nbr, const double* w, const double* theta,
+                   const double* P, const double* scale, double* r, void* stream);
 
 /* ---- synthetic inputs ----------------------------------------------------------
  * Seeded graph generators for the workloads.  No counterpart in the reference's

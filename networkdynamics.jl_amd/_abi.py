@@ -129,6 +129,7 @@ PROTOTYPES = {
     "nd_mp_backward": (C.c_int, [I64, P, P, P, P, I32, P, P, P, P, P, P]),
     "nd_mp_adam": (C.c_int, [I64, P, P, P, P, C.c_float, C.c_float, C.c_float, C.c_float, I64, P]),
     "nd_ws_features": (C.c_int, [I64, P, P, P, P, P, P, P, P, P]),
+    "nd_ws_residual": (C.c_int, [I64, P, P, P, P, P, P, P, P]),
     "nd_graph_erdos_renyi": (C.c_int, [I64, I64, C.c_uint64, C.POINTER(P)]),
     "nd_graph_barabasi_albert": (C.c_int, [I64, I64, C.c_uint64, C.POINTER(P)]),
     "nd_graph_random_geometric": (C.c_int, [I64, D, C.c_uint64, P, C.POINTER(P)]),

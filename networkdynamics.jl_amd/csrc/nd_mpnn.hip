@@ -501,6 +501,21 @@ __global__ void __launch_bounds__(256) k_ws_features(int64_t n, const int32_t* _
     x[7] = mx;
 }
 
+// |F_i| / scale_i of the fixpoint residual F_i = P_i + sum_s w_s sin(theta[nbr_s] - theta_i),
+// the quantity the batched Newton solver drives to its tolerance (entries in CSR order)
+__global__ void __launch_bounds__(256) k_ws_residual(int64_t n, const int32_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ nbr, const double* __restrict__ w,
+                                                     const double* __restrict__ th, const double* __restrict__ P,
+                                                     const double* __restrict__ scale, double* __restrict__ r) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double ti = th[i];
+    double F = P[i];
+    const int32_t k1 = rp[i + 1];
+    for (int32_t k = rp[i]; k < k1; ++k) F += w[k] * sin(th[nbr[k]] - ti);
+    r[i] = fabs(F) / scale[i];
+}
+
 // the K-weighted neighbour mean of s0 (X column 5), after k_ws_features
 __global__ void __launch_bounds__(256) k_ws_features_nbr(int64_t n, const int32_t* __restrict__ rp,
                                                          const int32_t* __restrict__ nbr, const double* __restrict__ w,
@@ -684,6 +699,14 @@ int nd_ws_features(int64_t n, const int32_t* rowptr, const int32_t* nbr, const d
     hipLaunchKernelGGL(k_ws_features, dim3(nblk(n, 256)), dim3(256), 0, s, n, rowptr, nbr, w, theta, P, is_gen, s0, X);
     hipLaunchKernelGGL(k_ws_features_nbr, dim3(nblk(n, 256)), dim3(256), 0, s, n, rowptr, nbr, w, s0, X);
     return herr(hipGetLastError(), "nd_ws_features");
+}
+
+int nd_ws_residual(int64_t n, const int32_t* rowptr, const int32_t* nbr, const double* w, const double* theta,
+                   const double* P, const double* scale, double* r, void* stream) {
+    if (n < 1 || !rowptr || !nbr || !w || !theta || !P || !scale || !r) return nd::set_error(ND_ERR_ARGUMENT, "bad arguments");
+    hipLaunchKernelGGL(k_ws_residual, dim3(nblk(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), n, rowptr, nbr, w,
+                       theta, P, scale, r);
+    return herr(hipGetLastError(), "nd_ws_residual");
 }
 
 int nd_mp_adam(int64_t np, float* params, const float* grads, float* m, float* v, float lr, float b1, float b2,

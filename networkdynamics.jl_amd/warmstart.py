@@ -254,6 +254,20 @@ class GridBatch:
         self.is_gen = cat([gr.is_gen.astype(float) for gr in self.grids])
         self.kdeg = torch.segment_reduce(self.w, "sum", lengths=self.lengths)
         self.deg = self.lengths.to(torch.float64)
+        # the residual of the flat start theta = 0 (F = P) per grid
+        self.flat_residual = np.array([np.max(np.abs(gr.P / gr.scale)) for gr in self.grids])
+
+    def start_residual(self, theta):
+        """Per grid max_i |F_i / scale_i| at theta (nd_ws_residual and a per-grid max): the
+        residual the Newton solver starts from.  Host array; NaN where theta gives a NaN."""
+        import torch
+        th = theta.contiguous()
+        r = torch.empty(self.n, dtype=torch.float64, device=self.dev)
+        stream = torch.cuda.current_stream(self.dev).cuda_stream
+        A.check(A.lib().nd_ws_residual(self.n, self.graph.rowptr.data_ptr(), self.graph.nbr.data_ptr(),
+                                       self.w.data_ptr(), th.data_ptr(), self.P.data_ptr(), self.S.data_ptr(),
+                                       r.data_ptr(), stream))
+        return torch.segment_reduce(r, "max", lengths=self.glen).cpu().numpy()
 
     def solve(self, theta0=None, P=None, linearized=False, **kw):
         """Batched Newton / linear solve; theta0, P: device tensors (defaults 0, the grids' P).
@@ -363,7 +377,7 @@ def solve_with_fallback(gb, theta0, tol=1e-10, first_max_newton=None, times=None
     return th, st, failed
 
 
-def learned_solve(gb, net, tol=1e-10, warm_tol=1e-1, dc_tol=3e-1, first_max_newton=12, **kw):
+def learned_solve(gb, net, tol=1e-10, warm_tol=1e-1, dc_tol=3e-1, first_max_newton=20, **kw):
     """Fixpoints of every grid of a GridBatch from the learned warm start, with the flat
     start as the fallback (solve_with_fallback).  net None: the physics-only start.
     The two linear solves of the warm start stop at residuals 0.3 (DC) and 0.1 (warm
@@ -371,8 +385,11 @@ def learned_solve(gb, net, tol=1e-10, warm_tol=1e-1, dc_tol=3e-1, first_max_newt
     solution only, and Newton takes the start to tol anyway.  Tighter pre-solves leave
     the Newton iterations from the start nearly unchanged (2.97-3.09 on 884 held-out
     grids from (1e-2, 1e-2) to (0.3, 0.1)) and cost 20 ms more device time.  The first
-    solve takes up to 12 Newton steps: at 8 an occasional grid that needs 9-10 from its
-    start falls back to a whole second solve from flat.
+    solve takes up to 20 Newton steps, as many as the flat start needs on its hardest
+    held-out grids: a start that lands far from the fixpoint can need 14, and a grid the
+    cap stops falls back to a whole second solve from flat.  The start's residual
+    (GridBatch.start_residual) does not tell such starts apart: on the held-out grids a
+    learned start above the flat start's residual needs 4 steps where flat needs 19.
     Returns (theta, stats, fell_back, dict of seconds)."""
     theta_w, times = warm_start_device(gb, net, tol=warm_tol, dc_tol=dc_tol)
     th, st, fell = solve_with_fallback(gb, theta_w, tol=tol, first_max_newton=first_max_newton, times=times, **kw)

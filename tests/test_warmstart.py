@@ -231,3 +231,31 @@ def test_pipeline_at_c5_grid_sizes(ND, dev):
     assert out["physics_newton_mean"] < out["flat_newton_mean"]
     assert out["learned_newton_mean"] < out["flat_newton_mean"]
     assert out["newton_ratio_learned"] == pytest.approx(out["flat_newton_mean"] / out["learned_newton_mean"])
+
+
+@pytest.mark.gpu
+def test_start_residual(ND, dev):
+    """GridBatch.start_residual (nd_ws_residual, then a per-grid max) is the fixpoint
+    residual max_i |F_i / scale_i|, F_i = P_i + sum_j K_ij sin(theta_j - theta_i): exactly
+    max |P / scale| at the flat start, below the tolerance at the solver's fixpoints, and
+    numpy's at a random state."""
+    import torch
+    from networkdynamics_jl_amd import warmstart as W
+    from networkdynamics_jl_amd.fixpoint import swing_load_grid
+    grids = [swing_load_grid(int(n), seed=1300 + i) for i, n in enumerate([400, 700, 550])]
+    gb = W.GridBatch(grids, dev)
+    z = torch.zeros(gb.n, dtype=torch.float64, device=dev)
+    assert np.array_equal(gb.start_residual(z), gb.flat_residual)
+    flat, fst, _ = gb.solve()
+    assert (fst["retcode"] == 0).all()
+    assert (gb.start_residual(flat) <= 1e-9).all()
+    rng = np.random.default_rng(13)
+    th = rng.uniform(-np.pi, np.pi, gb.n)
+    r = gb.start_residual(torch.from_numpy(th).to(dev))
+    for i, gr in enumerate(grids):
+        t = th[gb.voff[i]:gb.voff[i + 1]]
+        s, d = gr.g.src - 1, gr.g.dst - 1
+        F = gr.P.copy()
+        np.add.at(F, s, gr.K * np.sin(t[d] - t[s]))
+        np.add.at(F, d, gr.K * np.sin(t[s] - t[d]))
+        np.testing.assert_allclose(r[i], np.max(np.abs(F / gr.scale)), rtol=1e-12)
