@@ -526,8 +526,9 @@ typedef struct {
     int32_t cg_maxiter;    /* 0 = 20000                                                    */
     int32_t linearized;    /* 1: solve the DC power flow P + sum K (theta_j - theta_i) = 0 */
     int32_t inexact;       /* 1: each Newton step's CG stops at the relative residual
-                              max(cg_rtol, min(1e-2, current residual)) (inexact Newton
+                              max(cg_rtol, min(eta_max, current residual)) (inexact Newton
                               with a residual-based forcing term); 0: always cg_rtol       */
+    double eta_max;        /* cap of that forcing term, 0 = 1e-2                           */
 } nd_newton_opts;
 typedef struct {
     double residual;       /* final max |F_i/scale_i|                                      */

@@ -47,7 +47,7 @@ struct NewtonArgs {
     const double* K;         // per edge
     nd_newton_stat* stats;   // original grid order
     const int32_t* order;    // launch position -> grid index
-    double tol, cg_rtol;
+    double tol, cg_rtol, eta_max;
     int32_t max_newton, cg_maxiter;
     int32_t linearized, inexact;
     int32_t first;
@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(kThreads) k_newton(NewtonArgs a) {
         }
         rz = block_reduce(rz, red, false);
         rr0 = block_reduce(rr0, red, false);
-        const double eta = a.inexact ? fmax(a.cg_rtol, fmin(1e-2, res)) : a.cg_rtol;
+        const double eta = a.inexact ? fmax(a.cg_rtol, fmin(a.eta_max, res)) : a.cg_rtol;
         const double stop2 = eta * eta * rr0;
         double rr = rr0;
 #pragma unroll
@@ -394,6 +394,7 @@ int nd_fixpoint_solve(nd_fixpoint* f, double* theta, const double* P, const doub
     a.order = f->d_order;
     a.tol = o->tol > 0 ? o->tol : 1e-10;
     a.cg_rtol = o->cg_rtol > 0 ? o->cg_rtol : 1e-12;
+    a.eta_max = o->eta_max > 0 ? o->eta_max : 1e-2;
     a.max_newton = o->max_newton > 0 ? o->max_newton : 50;
     a.cg_maxiter = o->cg_maxiter > 0 ? o->cg_maxiter : 20000;
     a.linearized = o->linearized ? 1 : 0;

@@ -22,7 +22,7 @@ STAT_DTYPE = np.dtype([("residual", "f8"), ("newton_iters", "i4"), ("cg_iters", 
 
 class _Opts(C.Structure):
     _fields_ = [("tol", C.c_double), ("cg_rtol", C.c_double), ("max_newton", C.c_int32),
-                ("cg_maxiter", C.c_int32), ("linearized", C.c_int32), ("inexact", C.c_int32)]
+                ("cg_maxiter", C.c_int32), ("linearized", C.c_int32), ("inexact", C.c_int32), ("eta_max", C.c_double)]
 
 
 class FixpointSolver:
@@ -50,17 +50,17 @@ class FixpointSolver:
             self._h = None
 
     def solve(self, theta, P, scale, K, tol=1e-10, cg_rtol=1e-12, max_newton=50, cg_maxiter=20000, linearized=False,
-              inexact=True):
+              inexact=True, eta_max=1e-2):
         """theta (in/out), P, scale: concatenated per vertex; K per edge; float64 device tensors.
         inexact: loose CG tolerances while the Newton residual is large (forcing term
-        min(1e-2, residual)), cg_rtol once it is small."""
+        min(eta_max, residual)), cg_rtol once it is small."""
         import torch
         for x, n in ((theta, self.voff[-1]), (P, self.voff[-1]), (scale, self.voff[-1]), (K, self.eoff[-1])):
             if x.dtype != torch.float64 or x.device.type != "cuda" or x.numel() != n:
                 raise A.ArgumentError("theta/P/scale/K must be float64 device vectors of the right size")
         stats = torch.zeros(self.ngrid * STAT_DTYPE.itemsize // 8, dtype=torch.float64, device=theta.device)
         o = _Opts(float(tol), float(cg_rtol), int(max_newton), int(cg_maxiter), int(bool(linearized)),
-                  int(bool(inexact)))
+                  int(bool(inexact)), float(eta_max))
         stream = torch.cuda.current_stream(theta.device).cuda_stream
         A.check(A.lib().nd_fixpoint_solve(self._h, theta.data_ptr(), P.data_ptr(), scale.data_ptr(), K.data_ptr(),
                                           C.byref(o), stats.data_ptr(), stream))
