@@ -200,8 +200,10 @@ def main():
         torch.cuda.empty_cache()
         from importlib import import_module
         ex = import_module(ND.__name__ + ".experiments")
+        # size classes picked by the autotuner (its short probing solves run before the
+        # timed solve, like plan construction)
         r = ex.c4_ensemble(dev, baseline_seconds=args.c4_baseline_seconds,
-                           one_workgroup_seconds=args.c4_baseline_seconds / 2, steady_state_tol=0.0)
+                           one_workgroup_seconds=args.c4_baseline_seconds / 2, steady_state_tol=0.0, tune=1)
         ow = r["one_workgroup_per_instance"] or {}
         c4 = {"workload": r["workload"], "unit": "trajectories/s", "instances": r["instances"],
               "ensemble_traj_per_s": r["ensemble_traj_per_s"],
@@ -214,7 +216,8 @@ def main():
               "ratio_vs_one_workgroup": ow.get("ensemble_speedup"),
               "ratio_vs_host_stepped": r["speedup"],
               "steps_mean": r["ensemble_steps_mean"], "retcodes": r["ensemble_retcodes"],
-              "returned_to_sync": r["returned_to_sync"], "target_ratio": 20.0}
+              "returned_to_sync": r["returned_to_sync"], "classes": r["classes"],
+              "autotune_s": r["tuning"]["seconds"], "target_ratio": 20.0}
     c5 = None
     if not args.no_c5:
         # in a child process: its grid generation forks a process pool, which needs a

@@ -142,12 +142,13 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
     ens = Ensemble(graphs, Lib.kuramoto_second(), Lib.kuramoto_edge())
     u_d = torch.from_numpy(np.concatenate(us)).to(dev)
     p_d = torch.from_numpy(np.concatenate(ps)).to(dev)
-    tuning = None
+    tuning, t_tune = None, 0.0
     if tune:
         from .ensemble_tune import autotune
         tt = time.perf_counter()
         tuning = autotune(ens, u_d, p_d, probe_t=0.5, reltol=reltol, abstol=abstol)
-        _log(f"autotune: {tuning['classes']} in {time.perf_counter() - tt:.1f} s")
+        t_tune = time.perf_counter() - tt
+        _log(f"autotune: {tuning['classes']} in {t_tune:.1f} s")
     torch.cuda.synchronize()
     ts = time.perf_counter()
     est = Ensemble.stats_to_numpy(ens.solve(u_d, p_d, (0.0, t1), method=TSIT5, reltol=reltol, abstol=abstol))
@@ -239,7 +240,8 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
                 baseline_traj_per_s=traj_base, baseline_steps_mean=float(np.mean(steps)),
                 speedup=traj_ens / traj_base, one_workgroup_per_instance=one_wg,
                 classes=[list(c) for c in ens.classes], steady_state=ss,
-                tuning=None if tuning is None else dict(predicted_s=tuning["predicted_s"], bins=tuning["bins"]))
+                tuning=None if tuning is None else dict(predicted_s=tuning["predicted_s"], bins=tuning["bins"],
+                                                        seconds=t_tune))
 
 
 def _grid_job(args):
