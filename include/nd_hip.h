@@ -551,7 +551,8 @@ int nd_fixpoint_solve(nd_fixpoint* f, double* theta, const double* P, const doub
  * out = H_4 w_out + b_out, with A the row-normalised K-weighted adjacency (CSR rowptr
  * / nbr / a_fwd; a_bwd = weights of its transpose on the same pattern).  Parameter
  * layout: W_in[nfeat*64], b_in[64], {[W_l|U_l][64*128], b_l[64]} x 4, w_out[64], b_out.
- * work: nd_mp_workspace_floats(n) floats, shared by a forward and its backward. */
+ * work: nd_mp_workspace_floats(n) floats, shared by a forward and its backward; params and
+ * work 16-byte aligned. */
 int64_t nd_mp_param_count(int32_t nfeat);
 int64_t nd_mp_workspace_floats(int64_t nnodes);
 int nd_mp_forward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const float* a_fwd, const float* X,

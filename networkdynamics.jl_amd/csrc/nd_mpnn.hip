@@ -383,35 +383,44 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
 }
 
 // ---- sparse: out[i, :] = sum_s a[s] * Y[nbr[s], 0:64]  (+ Y[i, 64:128] + bias, relu) -------
-// one 64-lane wave per node, one lane per feature
+// 16 lanes per node, four features per lane (16-byte loads), so a wave works on four
+// nodes at once; each lane sums its features over the node's entries in order, with four
+// entries' rows in flight.  Y, out and bias must be 16-byte aligned (ldy, ldo multiples of 4).
 __global__ void __launch_bounds__(256) k_spmm(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ nbr,
                                               const float* __restrict__ a, const float* __restrict__ Y, int ldy,
                                               float* __restrict__ out, int ldo, const float* __restrict__ bias, int fused) {
-    // the node is the same for the whole wave: its list offsets, neighbours and weights
-    // come through scalar loads, four entries' rows in flight at once, summed in order
-    const int64_t i = __builtin_amdgcn_readfirstlane(
-        static_cast<int32_t>(xcd_block(blockIdx.x, gridDim.x) * 4 + threadIdx.x / kWave));
-    const int c = threadIdx.x % kWave;
+    const int64_t i = xcd_block(blockIdx.x, gridDim.x) * 16 + threadIdx.x / 16;
+    const int c = static_cast<int>(threadIdx.x % 16) * 4;
     if (i >= n) return;
-    float s = 0.0f;
+    auto row = [&](int32_t j) { return *reinterpret_cast<const float4*>(Y + static_cast<int64_t>(j) * ldy + c); };
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    auto add = [&](float w, const float4& y) {
+        s.x += w * y.x;
+        s.y += w * y.y;
+        s.z += w * y.z;
+        s.w += w * y.w;
+    };
     int32_t k = rp[i];
     const int32_t k1 = rp[i + 1];
     for (; k + 4 <= k1; k += 4) {
         const int32_t j0 = nbr[k], j1 = nbr[k + 1], j2 = nbr[k + 2], j3 = nbr[k + 3];
         const float a0 = a[k], a1 = a[k + 1], a2 = a[k + 2], a3 = a[k + 3];
-        const float y0 = Y[static_cast<int64_t>(j0) * ldy + c], y1 = Y[static_cast<int64_t>(j1) * ldy + c];
-        const float y2 = Y[static_cast<int64_t>(j2) * ldy + c], y3 = Y[static_cast<int64_t>(j3) * ldy + c];
-        s += a0 * y0;
-        s += a1 * y1;
-        s += a2 * y2;
-        s += a3 * y3;
+        const float4 y0 = row(j0), y1 = row(j1), y2 = row(j2), y3 = row(j3);
+        add(a0, y0);
+        add(a1, y1);
+        add(a2, y2);
+        add(a3, y3);
     }
-    for (; k < k1; ++k) s += a[k] * Y[static_cast<int64_t>(nbr[k]) * ldy + c];
+    for (; k < k1; ++k) add(a[k], row(nbr[k]));
     if (fused) {
-        s += Y[i * ldy + kH + c] + bias[c];
-        s = fmaxf(s, 0.0f);
+        const float4 y = *reinterpret_cast<const float4*>(Y + i * ldy + kH + c);
+        const float4 b = *reinterpret_cast<const float4*>(bias + c);
+        s.x = fmaxf(s.x + (y.x + b.x), 0.0f);
+        s.y = fmaxf(s.y + (y.y + b.y), 0.0f);
+        s.z = fmaxf(s.z + (y.z + b.z), 0.0f);
+        s.w = fmaxf(s.w + (y.w + b.w), 0.0f);
     }
-    out[i * ldo + c] = s;
+    *reinterpret_cast<float4*>(out + i * ldo + c) = s;
 }
 
 // g = dH * relu'(H): G[i, c] = H[i, c] > 0 ? DH[i, c] : 0, also copied to DY[i, 64 + c] when DY is given
@@ -633,6 +642,8 @@ int nd_mp_forward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const fl
                   const float* params, float* work, float* out, void* stream) {
     if (n < 1 || n > INT32_MAX - 8 || !rowptr || !X || !params || !work || !out || nfeat < 1 || nfeat > 128)
         return nd::set_error(ND_ERR_ARGUMENT, "bad arguments");
+    if ((reinterpret_cast<uintptr_t>(params) | reinterpret_cast<uintptr_t>(work)) & 15)
+        return nd::set_error(ND_ERR_ARGUMENT, "params and work must be 16-byte aligned");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const Offsets o = offsets(nfeat);
     float* H = work;                        // H_l at H + l*n*64
@@ -640,7 +651,7 @@ int nd_mp_forward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const fl
     gemm(s, false, n, kH, nfeat, X, nfeat, params + o.win, kH, H, kH, params + o.bin, 1);
     for (int l = 0; l < kLayers; ++l) {
         gemm(s, false, n, 2 * kH, kH, H + l * n * kH, kH, params + o.wl[l], 2 * kH, Y, 2 * kH, nullptr, 0);
-        hipLaunchKernelGGL(k_spmm, dim3(nblk(n, 4)), dim3(256), 0, s, n, rowptr, nbr, a_fwd, Y, 2 * kH,
+        hipLaunchKernelGGL(k_spmm, dim3(nblk(n, 16)), dim3(256), 0, s, n, rowptr, nbr, a_fwd, Y, 2 * kH,
                            H + (l + 1) * n * kH, kH, params + o.bl[l], 1);
     }
     hipLaunchKernelGGL(k_head, dim3(nblk(n, 4)), dim3(256), 0, s, n, H + 4 * n * kH, params + o.wout, params + o.bout,
@@ -656,6 +667,8 @@ int nd_mp_backward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const f
     if (n < 1 || n > INT32_MAX - 8 || !rowptr || !X || !params || !work || !grads || !target || !loss_part || nfeat < 1 ||
         nfeat > 128)
         return nd::set_error(ND_ERR_ARGUMENT, "bad arguments");
+    if ((reinterpret_cast<uintptr_t>(params) | reinterpret_cast<uintptr_t>(work)) & 15)
+        return nd::set_error(ND_ERR_ARGUMENT, "params and work must be 16-byte aligned");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const Offsets o = offsets(nfeat);
     float* H = work;
@@ -679,7 +692,7 @@ int nd_mp_backward(int64_t n, const int32_t* rowptr, const int32_t* nbr, const f
         hipLaunchKernelGGL(k_relu_grad, dim3(nblk(n * kH, 256)), dim3(256), 0, s, n, DH, H + (l + 1) * n * kH, G, DY);
         colsum(s, n, G, kH, part, grads + o.bl[l]);
         // dYa = A^T g into the left half
-        hipLaunchKernelGGL(k_spmm, dim3(nblk(n, 4)), dim3(256), 0, s, n, rowptr, nbr, a_bwd, G, kH, DY, 2 * kH,
+        hipLaunchKernelGGL(k_spmm, dim3(nblk(n, 16)), dim3(256), 0, s, n, rowptr, nbr, a_bwd, G, kH, DY, 2 * kH,
                            nullptr, 0);
         // d[W|U] = H_l^T [dYa | g];  dH_l = [dYa | g] [W|U]^T
         gemm_tn(s, n, kH, 2 * kH, H + l * n * kH, kH, DY, 2 * kH, part, grads + o.wl[l], 0);
