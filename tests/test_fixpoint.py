@@ -53,6 +53,28 @@ def test_batched_grids_are_fixpoints(ND, dev):
 
 
 @pytest.mark.gpu
+def test_forcing_cap(ND, dev):
+    """eta_max caps the inexact-Newton forcing term min(eta_max, residual): a looser cap
+    takes fewer CG iterations per Newton step and more steps, and every grid still ends
+    at the requested residual, in the nonlinear and in the linearized (DC) problem."""
+    import torch
+    grids = [ND.swing_load_grid(int(n), seed=400 + i) for i, n in enumerate([600, 1500, 3000])]
+    fs = ND.FixpointSolver([gr.g for gr in grids])
+    d = lambda xs: torch.from_numpy(np.concatenate(xs)).to(dev)
+    P, S, K = d([gr.P for gr in grids]), d([gr.scale for gr in grids]), d([gr.K for gr in grids])
+    for lin in (False, True):
+        out = {}
+        for eta in (1e-2, 0.5):
+            th = torch.zeros(int(fs.voff[-1]), dtype=torch.float64, device=dev)
+            st = ND.FixpointSolver.stats_to_numpy(fs.solve(th, P, S, K, linearized=lin, eta_max=eta))
+            assert (st["retcode"] == 0).all() and (st["residual"] <= 1e-10).all(), (lin, eta, st)
+            out[eta] = st
+        per_step = lambda st: st["cg_iters"].sum() / st["newton_iters"].sum()
+        assert per_step(out[0.5]) < per_step(out[1e-2]), (lin, out)
+        assert out[0.5]["newton_iters"].sum() > out[1e-2]["newton_iters"].sum(), (lin, out)
+
+
+@pytest.mark.gpu
 def test_linearization_at_a_fixpoint(ND, dev):
     # linearize_network / jacobian_eigenvals at a batched-Newton fixpoint of a swing/load
     # grid: the state is a fixpoint, one zero eigenvalue (global phase), the rest stable
