@@ -79,3 +79,42 @@ def test_struct_layouts_match_the_header(ND, tmp_path):
                                                                 text=True).stdout.splitlines())
     for k, v in expect.items():
         assert int(got[k]) == v, (k, int(got[k]), v)
+
+
+def _ctypes_for(ct):
+    """the ctypes types a C parameter type may be bound with"""
+    import ctypes as C
+    t = ct.replace("const ", "").strip()
+    if t.count("*") == 0:
+        return {"int": {C.c_int}, "int32_t": {C.c_int32}, "int64_t": {C.c_int64}, "double": {C.c_double},
+                "float": {C.c_float}, "size_t": {C.c_size_t}, "uint64_t": {C.c_uint64}, "void": {None}}[t]
+    if t == "char *":
+        return {C.c_char_p}
+    return None                                  # any pointer type
+
+
+def test_ctypes_prototypes_match_the_header(ND):
+    """_abi.PROTOTYPES binds every function the header declares and no other; each has
+    the header's argument count, its scalar argument and return types exactly, and a
+    pointer type wherever the header has a pointer."""
+    import ctypes as C
+    from networkdynamics_jl_amd import _abi
+    from test_julia_binding import _header_prototypes
+    protos = _header_prototypes()
+    assert set(protos) == set(_abi.PROTOTYPES), set(protos) ^ set(_abi.PROTOTYPES)
+    ptr_types = (C.c_void_p, C.c_char_p, C._Pointer)
+    for name, (res, args) in _abi.PROTOTYPES.items():
+        assert name in protos, name
+        cret, cargs = protos[name]
+        assert len(args) == len(cargs), (name, args, cargs)
+        want = _ctypes_for(cret)
+        if want is None:
+            assert res in (C.c_void_p, C.c_char_p) or issubclass(res, C._Pointer), (name, res)
+        else:
+            assert res in want, (name, cret, res)
+        for pos, (a, ct) in enumerate(zip(args, cargs)):
+            want = _ctypes_for(ct)
+            if want is None:
+                assert a is C.c_void_p or issubclass(a, ptr_types), (name, pos, ct, a)
+            else:
+                assert a in want, (name, pos, ct, a)
