@@ -238,13 +238,8 @@ class GridBatch:
         self.dev = dev
         self.fs = FixpointSolver([gr.g for gr in self.grids])
         self.graph = GraphBatch(self.grids, dev)
-        h = self.graph.host
         self.n = self.graph.n
         self.voff = self.graph.voff
-        self.lengths = torch.from_numpy(np.diff(h["rowptr"]).astype(np.int64)).to(dev)
-        rows = np.repeat(np.arange(self.n), np.diff(h["rowptr"]))
-        self.rows = torch.from_numpy(rows).to(dev)
-        self.cols = torch.from_numpy(np.asarray(h["nbr"], np.int64)).to(dev)
         self.w = torch.from_numpy(self.graph.w).to(dev)                     # K of each CSR entry
         self.glen = torch.from_numpy(np.diff(self.voff).astype(np.int64)).to(dev)
         cat = lambda a: torch.from_numpy(np.ascontiguousarray(np.concatenate(a))).to(dev)
@@ -252,8 +247,6 @@ class GridBatch:
         self.S = cat([gr.scale for gr in self.grids])
         self.K = cat([gr.K for gr in self.grids])
         self.is_gen = cat([gr.is_gen.astype(float) for gr in self.grids])
-        self.kdeg = torch.segment_reduce(self.w, "sum", lengths=self.lengths)
-        self.deg = self.lengths.to(torch.float64)
         # the residual of the flat start theta = 0 (F = P) per grid
         self.flat_residual = np.array([np.max(np.abs(gr.P / gr.scale)) for gr in self.grids])
 
@@ -287,10 +280,8 @@ class GridBatch:
         return x - torch.repeat_interleave(mean, self.glen)
 
     def source(self, theta):
-        """s_i = sum_j K_ij (sin(D_ij) - D_ij) per vertex (CSR segment sums, deterministic)."""
-        import torch
-        D = theta[self.cols] - theta[self.rows]
-        return torch.segment_reduce(self.w * (torch.sin(D) - D), "sum", lengths=self.lengths)
+        """s_i = sum_j K_ij (sin(D_ij) - D_ij) per vertex (the s0 output of nd_ws_features)."""
+        return self.features(theta)[0]
 
     def features(self, theta_dc):
         """(s0, X): the physics source and the per-vertex inputs of `features`, on the GPU
