@@ -146,7 +146,14 @@ def main():
         from importlib import import_module
         ex = import_module(ND.__name__ + ".experiments")
         fn = dict(c1=ex.c1_rk4, c3=ex.c3_rhs, c4=ex.c4_ensemble, c5=ex.c5_warmstart, extras=ex.extras)[args.config]
-        kw = {k: float(v) if "." in v or "e" in v else int(v) for k, v in (x.split("=", 1) for x in args.set)}
+        def value(v):
+            for conv in (int, float):
+                try:
+                    return conv(v)
+                except ValueError:
+                    pass
+            return v                            # a string, such as a file path
+        kw = {k: value(v) for k, v in (x.split("=", 1) for x in args.set)}
         res = fn(dev, **kw)
         if not args.no_cpu and args.config == "c1":
             res["cpu_baseline"] = cpu_baseline_c1(ND, args.cpu_seconds)

@@ -134,7 +134,10 @@ def returned_to_sync(u):
 
 
 def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_seconds=30.0, seed=1, tune=0,
-                steady_state_tol=1e-5, one_workgroup_seconds=10.0):
+                steady_state_tol=1e-5, one_workgroup_seconds=10.0, tune_cache=None):
+    """tune: pick the size classes with ensemble_tune.autotune first (tune_cache: its JSON
+    cache file, so that a later run, e.g. under a profiler, reuses the choice without
+    probing)."""
     import torch
     t0 = time.perf_counter()
     graphs, us, ps, dropped, t_newton, rng = c4_start(dev, ninst, seed)
@@ -146,7 +149,7 @@ def c4_ensemble(dev, ninst=4096, t1=50.0, reltol=1e-8, abstol=1e-8, baseline_sec
     if tune:
         from .ensemble_tune import autotune
         tt = time.perf_counter()
-        tuning = autotune(ens, u_d, p_d, probe_t=0.5, reltol=reltol, abstol=abstol)
+        tuning = autotune(ens, u_d, p_d, probe_t=0.5, reltol=reltol, abstol=abstol, cache=tune_cache)
         t_tune = time.perf_counter() - tt
         _log(f"autotune: {tuning['classes']} in {t_tune:.1f} s")
     torch.cuda.synchronize()
