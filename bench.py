@@ -27,6 +27,37 @@ HBM_PEAK_GBS = 8000.0
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "swing_ba_1e6_traffic.txt")
 
 
+def neighbour_gather(dev, g, reps=20):
+    """The RHS's neighbour-read pattern on its own: torch.index_select of an N-vector of
+    doubles (the compact vertex outputs) at every list entry's neighbour, entries in list
+    order (each vertex's neighbours sorted, as the destination lists are), int32 indices;
+    and the same number of reads at sequential indices.  Microseconds per gather."""
+    import numpy as np
+    import torch
+    n = g.nv()
+    s, d = g.src - 1, g.dst - 1
+    owner = np.concatenate([d, s])
+    nbr = np.concatenate([s, d])
+    order = np.lexsort((nbr, owner))
+    idx = torch.from_numpy(nbr[order].astype(np.int32)).to(dev)
+    seq = torch.arange(idx.numel(), dtype=torch.int32, device=dev) % n
+    vo = torch.rand(n, dtype=torch.float64, device=dev)
+    out = {}
+    for name, ix in (("random", idx), ("sequential", seq)):
+        for _ in range(3):
+            torch.index_select(vo, 0, ix)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            torch.index_select(vo, 0, ix)
+        b.record()
+        torch.cuda.synchronize()
+        out[name + "_us"] = a.elapsed_time(b) * 1e3 / reps
+    out["reads"] = int(idx.numel())
+    return out
+
+
 def stream_copy(dev, mib, reps=20):
     """Achievable bandwidth of a device copy of `mib` MiB (read + write counted), GB/s."""
     import torch
@@ -199,6 +230,7 @@ def main():
     traffic = _traffic_from_profile()
     copy_hbm = stream_copy(dev, 2048)               # 4 GiB moved: beyond the 256 MB Infinity Cache
     copy_small = stream_copy(dev, 64)               # 128 MiB moved: cache-resident
+    gather = neighbour_gather(dev, nw.g)            # the neighbour reads alone, through torch
 
     cpu = None if args.no_cpu else cpu_baseline(nw, p, u0, args.cpu_seconds)
     c4 = None
@@ -267,7 +299,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": "profiles/swing_ba_1e6_traffic.txt (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
                      "algorithmic_bytes": algo_bytes, "kernel_us": per_rhs_s * 1e6,
-                     "stream_copy_GBs": {"4GiB_moved": copy_hbm, "128MiB_moved": copy_small}},
+                     "stream_copy_GBs": {"4GiB_moved": copy_hbm, "128MiB_moved": copy_small},
+                     "neighbour_gather_torch": gather},
         "cpu_baseline": cpu,
         "extended": {"c4": c4, "c5": c5},
     }
