@@ -259,3 +259,38 @@ def test_start_residual(ND, dev):
         np.add.at(F, s, gr.K * np.sin(t[d] - t[s]))
         np.add.at(F, d, gr.K * np.sin(t[s] - t[d]))
         np.testing.assert_allclose(r[i], np.max(np.abs(F / gr.scale)), rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_network_entry_points_refuse_bad_arguments(ND, dev):
+    """The warm-start entry points check their arguments on the host before any launch:
+    a params or workspace pointer that is not 16-byte aligned (the products and the
+    aggregation load 16 bytes at a time), a null pointer, an empty batch."""
+    import torch
+    L = ND._abi.lib()
+    chk = ND._abi.check
+    n = 64
+    rp = torch.arange(0, n + 1, dtype=torch.int32, device=dev)
+    nbr = torch.arange(n, dtype=torch.int32, device=dev)
+    a = torch.ones(n, dtype=torch.float32, device=dev)
+    X = torch.zeros((n, 8), dtype=torch.float32, device=dev)
+    params = torch.zeros(int(L.nd_mp_param_count(8)) + 4, dtype=torch.float32, device=dev)
+    work = torch.zeros(int(L.nd_mp_workspace_floats(n)) + 4, dtype=torch.float32, device=dev)
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    s = torch.cuda.current_stream(dev).cuda_stream
+    fwd = lambda pp, ww, nn=n: chk(L.nd_mp_forward(nn, rp.data_ptr(), nbr.data_ptr(), a.data_ptr(), X.data_ptr(), 8,
+                                                   pp, ww, out.data_ptr(), s))
+    fwd(params.data_ptr(), work.data_ptr())
+    torch.cuda.synchronize()
+    for pp, ww in ((params.data_ptr() + 4, work.data_ptr()), (params.data_ptr(), work.data_ptr() + 8)):
+        with pytest.raises(ND.ArgumentError, match="16-byte aligned"):
+            fwd(pp, ww)
+    with pytest.raises(ND.ArgumentError):
+        fwd(params.data_ptr(), work.data_ptr(), 0)
+    th = torch.zeros(n, dtype=torch.float64, device=dev)
+    with pytest.raises(ND.ArgumentError):
+        chk(L.nd_ws_features(n, rp.data_ptr(), nbr.data_ptr(), None, th.data_ptr(), th.data_ptr(), th.data_ptr(),
+                             th.data_ptr(), th.data_ptr(), s))
+    with pytest.raises(ND.ArgumentError):
+        chk(L.nd_ws_residual(0, rp.data_ptr(), nbr.data_ptr(), th.data_ptr(), th.data_ptr(), th.data_ptr(),
+                             th.data_ptr(), th.data_ptr(), s))
