@@ -568,7 +568,8 @@ int nd_mp_adam(int64_t np, float* params, const float* grads, float* m, float* v
  * the DC solution): with D_s = theta[nbr_s] - theta[i] and kdeg_i = sum_s w_s,
  * s0_i = sum_s w_s (sin D_s - D_s), the nonlinear part of the line flows, and
  * X[i] = {P_i, is_gen_i, log kdeg_i, deg_i / 6, s0_i, sum_s w_s s0[nbr_s] / kdeg_i,
- * sum_s w_s D_s^2, max_s |D_s|} (n x 8, row-major).  Sums in CSR order. */
+ * sum_s w_s D_s^2, max_s |D_s|} (n x 8, row-major).  Sums in CSR order; a node without
+ * entries gets 0 for log kdeg_i and the neighbour mean. */
 int nd_ws_features(int64_t n, const int32_t* rowptr, const int32_t* nbr, const double* w, const double* theta,
                    const double* P, const double* is_gen, double* s0, double* X, void* stream);
 /* Per-node fixpoint residual of a phase vector on the same stacked CSR: r_i = |F_i| / scale_i

@@ -503,7 +503,7 @@ __global__ void __launch_bounds__(256) k_ws_features(int64_t n, const int32_t* _
     double* x = X + i * kFeat;
     x[0] = P[i];
     x[1] = gen[i];
-    x[2] = log(kd);
+    x[2] = kd > 0.0 ? log(kd) : 0.0;        // a node without entries: 0
     x[3] = static_cast<double>(k1 - rp[i]) / 6.0;
     x[4] = s;
     x[6] = sq;
@@ -538,7 +538,7 @@ __global__ void __launch_bounds__(256) k_ws_features_nbr(int64_t n, const int32_
         ns += wk * s0[nbr[k]];
         kd += wk;
     }
-    X[i * kFeat + 5] = ns / kd;
+    X[i * kFeat + 5] = kd > 0.0 ? ns / kd : 0.0;
 }
 
 struct Offsets {

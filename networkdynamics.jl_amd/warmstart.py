@@ -74,7 +74,8 @@ def source(grid, theta):
 
 def features(grid, theta_dc, s0=None):
     """Per-vertex inputs: P, generator flag, log weighted degree, degree, the physics
-    source s0 and its K-weighted neighbour mean, sum K D^2 and max |D| of the DC flows."""
+    source s0 and its K-weighted neighbour mean, sum K D^2 and max |D| of the DC flows.
+    A vertex without lines gets 0 for the log degree and the neighbour mean."""
     g = grid.g
     s, d = g.src - 1, g.dst - 1
     n = g.nv()
@@ -84,12 +85,13 @@ def features(grid, theta_dc, s0=None):
         s0 = source(grid, theta_dc)
     kdeg = np.bincount(s, weights=K, minlength=n) + np.bincount(d, weights=K, minlength=n)
     deg = np.bincount(s, minlength=n) + np.bincount(d, minlength=n)
-    ns0 = (np.bincount(s, weights=K * s0[d], minlength=n) + np.bincount(d, weights=K * s0[s], minlength=n)) / kdeg
+    ns0 = np.bincount(s, weights=K * s0[d], minlength=n) + np.bincount(d, weights=K * s0[s], minlength=n)
+    ns0 = np.divide(ns0, kdeg, out=np.zeros(n), where=kdeg > 0)
     sq = np.bincount(s, weights=K * x * x, minlength=n) + np.bincount(d, weights=K * x * x, minlength=n)
     mx = np.zeros(n)
     np.maximum.at(mx, s, np.abs(x))
     np.maximum.at(mx, d, np.abs(x))
-    return np.stack([grid.P, grid.is_gen.astype(float), np.log(kdeg), deg / 6.0, s0, ns0, sq, mx], axis=1)
+    return np.stack([grid.P, grid.is_gen.astype(float), np.log(kdeg, out=np.zeros(n), where=kdeg > 0), deg / 6.0, s0, ns0, sq, mx], axis=1)
 
 
 class WarmStartNet:

@@ -294,3 +294,60 @@ def test_network_entry_points_refuse_bad_arguments(ND, dev):
     with pytest.raises(ND.ArgumentError):
         chk(L.nd_ws_residual(0, rp.data_ptr(), nbr.data_ptr(), th.data_ptr(), th.data_ptr(), th.data_ptr(),
                              th.data_ptr(), th.data_ptr(), s))
+
+
+def _ws_features(ND, dev, rows, theta, P, gen):
+    """nd_ws_features on a CSR given as per-node lists of (neighbour, weight)"""
+    import torch
+    L = ND._abi.lib()
+    n = len(rows)
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    nbr = np.array([j for r in rows for j, _ in r] or [0], np.int32)
+    ww = np.array([x for r in rows for _, x in r] or [0.0])
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    d_rp, d_nbr, d_w, d_th, d_P, d_gen = T(rp), T(nbr), T(ww), T(theta), T(P), T(gen)
+    s0 = torch.empty(n, dtype=torch.float64, device=dev)
+    X = torch.empty((n, 8), dtype=torch.float64, device=dev)
+    scale = torch.ones(n, dtype=torch.float64, device=dev)
+    r = torch.empty(n, dtype=torch.float64, device=dev)
+    s = torch.cuda.current_stream(dev).cuda_stream
+    ND._abi.check(L.nd_ws_features(n, d_rp.data_ptr(), d_nbr.data_ptr(), d_w.data_ptr(), d_th.data_ptr(),
+                                   d_P.data_ptr(), d_gen.data_ptr(), s0.data_ptr(), X.data_ptr(), s))
+    ND._abi.check(L.nd_ws_residual(n, d_rp.data_ptr(), d_nbr.data_ptr(), d_w.data_ptr(), d_th.data_ptr(),
+                                   d_P.data_ptr(), scale.data_ptr(), r.data_ptr(), s))
+    torch.cuda.synchronize()
+    return s0.cpu().numpy(), X.cpu().numpy(), r.cpu().numpy()
+
+
+@pytest.mark.gpu
+def test_features_of_isolated_nodes(ND, dev):
+    """Nodes without entries next to a connected part: the other nodes' features, sources
+    and residuals are those of the graph without them, bit for bit, and an isolated node's
+    row holds P and the generator flag with zeros elsewhere (log degree and neighbour mean
+    included), its residual |P| / scale."""
+    K = {(0, 1): 1.5, (1, 2): 0.7, (0, 2): 2.0, (2, 3): 1.1, (3, 4): 0.9}
+    rng = np.random.default_rng(5)
+
+    def rows_of(n, relabel):
+        rows = [[] for _ in range(n)]
+        for (a, b), k in K.items():
+            rows[relabel[a]].append((relabel[b], k))
+            rows[relabel[b]].append((relabel[a], k))
+        return [sorted(r) for r in rows]
+
+    th5, P5, g5 = rng.uniform(-0.5, 0.5, 5), rng.normal(size=5), (rng.uniform(size=5) < 0.5).astype(float)
+    sA, XA, rA = _ws_features(ND, dev, rows_of(5, list(range(5))), th5, P5, g5)
+    new = [0, 2, 3, 4, 6]                       # isolated nodes at 1 and 5
+    iso = [1, 5]
+    th7, P7, g7 = np.full(7, 0.3), np.zeros(7), np.zeros(7)
+    th7[new], P7[new], g7[new] = th5, P5, g5
+    P7[iso], g7[iso] = [0.25, -1.5], [1.0, 0.0]
+    sB, XB, rB = _ws_features(ND, dev, rows_of(7, new), th7, P7, g7)
+    assert np.all(np.isfinite(XB)) and np.all(np.isfinite(sB))
+    np.testing.assert_array_equal(XB[new], XA)
+    np.testing.assert_array_equal(sB[new], sA)
+    np.testing.assert_array_equal(rB[new], rA)
+    for i in iso:
+        np.testing.assert_array_equal(XB[i], [P7[i], g7[i], 0, 0, 0, 0, 0, 0])
+        assert sB[i] == 0.0 and rB[i] == abs(P7[i])
