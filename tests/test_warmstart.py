@@ -351,3 +351,24 @@ def test_features_of_isolated_nodes(ND, dev):
     for i in iso:
         np.testing.assert_array_equal(XB[i], [P7[i], g7[i], 0, 0, 0, 0, 0, 0])
         assert sB[i] == 0.0 and rB[i] == abs(P7[i])
+
+
+def test_host_features_of_isolated_vertex(ND):
+    """The host features (the training data's path) on a grid with a vertex without lines:
+    the other vertices' rows equal those of the grid without it, bit for bit, and its row
+    holds P and the generator flag with zeros elsewhere."""
+    from types import SimpleNamespace
+    from networkdynamics_jl_amd import warmstart as W
+
+    def grid(nv, src, dst, P, gen):
+        g = SimpleNamespace(src=np.array(src), dst=np.array(dst), nv=lambda: nv)
+        return SimpleNamespace(g=g, K=np.array([1.5, 0.7, 2.0, 1.1]), P=np.array(P), is_gen=np.array(gen))
+
+    th = np.array([0.1, -0.2, 0.05, 0.3])
+    a = grid(4, [1, 2, 1, 3], [2, 3, 3, 4], [1.0, -0.5, -0.25, -0.25], [1, 0, 0, 0])
+    b = grid(5, [1, 2, 1, 3], [2, 3, 3, 5], [1.0, -0.5, -0.25, 0.4, -0.25], [1, 0, 0, 1, 0])
+    XA = W.features(a, th)
+    XB = W.features(b, np.array([0.1, -0.2, 0.05, 0.7, 0.3]))
+    assert np.all(np.isfinite(XB))
+    np.testing.assert_array_equal(XB[[0, 1, 2, 4]], XA)
+    np.testing.assert_array_equal(XB[3], [0.4, 1.0, 0, 0, 0, 0, 0, 0])
